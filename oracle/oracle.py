@@ -1,0 +1,188 @@
+"""ctypes binding of the CPU oracle (oracle/rnj_oracle.c) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import this module, and
+only to check or time the reference algorithm; the product (``rapidnj_amd``) never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "librnj_oracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+JC, K2P = 0, 1
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u64_3 = ctypes.c_uint64 * 3
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.rnj_oracle_bitstrings_count.restype = ctypes.c_uint
+        L.rnj_oracle_bitstrings_count.argtypes = [ctypes.c_uint]
+        L.rnj_oracle_is_dna.restype = ctypes.c_int
+        L.rnj_oracle_is_dna.argtypes = [ctypes.c_char_p, ctypes.c_uint]
+        L.rnj_oracle_pack.restype = None
+        L.rnj_oracle_pack.argtypes = [ctypes.c_char_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                      _u32p, _u32p]
+        for name in ("rnj_oracle_count_chars", "rnj_oracle_count_naive"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint, _u64_3]
+        for name in ("rnj_oracle_count_words", "rnj_oracle_count_swar"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint, _u64_3]
+        for name in ("rnj_oracle_k2p", "rnj_oracle_jc"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_float
+            f.argtypes = [ctypes.c_uint64] * 3
+        L.rnj_oracle_counts_triangle.restype = ctypes.c_int
+        L.rnj_oracle_counts_triangle.argtypes = [_u32p, _u32p, ctypes.c_uint, ctypes.c_uint,
+                                                 ctypes.c_int, _u32p, _u32p, _u32p]
+        L.rnj_oracle_distance_matrix.restype = ctypes.c_int
+        L.rnj_oracle_distance_matrix.argtypes = [_u32p, _u32p, ctypes.c_uint, ctypes.c_uint,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.rnj_oracle_correct_triangle.restype = ctypes.c_int
+        L.rnj_oracle_correct_triangle.argtypes = [_u32p, _u32p, _u32p, ctypes.c_uint, ctypes.c_int,
+                                                  ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.POINTER(ctypes.c_float)]
+        _lib = L
+    return _lib
+
+
+def model_id(model: str) -> int:
+    return {"jc": JC, "k2p": K2P, "kim": K2P}[model]
+
+
+def bitstrings_count(L: int) -> int:
+    return lib().rnj_oracle_bitstrings_count(L)
+
+
+def is_dna(seq: bytes) -> bool:
+    return bool(lib().rnj_oracle_is_dna(seq, len(seq)))
+
+
+def pack(aln: np.ndarray) -> tuple[np.ndarray, np.ndarray, int]:
+    """Reference-layout packing of an (n, L) uint8 alignment -> (bits, gaps, bsc)."""
+    aln = np.ascontiguousarray(aln, dtype=np.uint8)
+    n, L = aln.shape
+    bsc = bitstrings_count(L)
+    bits = np.zeros((n, bsc * 4), dtype=np.uint32)
+    gaps = np.zeros((n, bsc * 4), dtype=np.uint32)
+    lib().rnj_oracle_pack(aln.tobytes(), n, L, bsc, bits, gaps)
+    return bits, gaps, bsc
+
+
+def count_chars(a: bytes, b: bytes) -> tuple[int, int, int]:
+    out = _u64_3()
+    lib().rnj_oracle_count_chars(a, b, len(a), out)
+    return tuple(out)
+
+
+def count_naive(a: bytes, b: bytes) -> tuple[int, int, int]:
+    out = _u64_3()
+    lib().rnj_oracle_count_naive(a, b, len(a), out)
+    return tuple(out)
+
+
+def _row_ptr(arr: np.ndarray, i: int) -> int:
+    return arr.ctypes.data + i * arr.strides[0]
+
+
+def count_words(bits, gaps, i, j) -> tuple[int, int, int]:
+    out = _u64_3()
+    lib().rnj_oracle_count_words(_row_ptr(bits, i), _row_ptr(gaps, i), _row_ptr(bits, j),
+                                 _row_ptr(gaps, j), bits.shape[1], out)
+    return tuple(out)
+
+
+def count_swar(bits, gaps, i, j) -> tuple[int, int, int]:
+    out = _u64_3()
+    lib().rnj_oracle_count_swar(_row_ptr(bits, i), _row_ptr(gaps, i), _row_ptr(bits, j),
+                                _row_ptr(gaps, j), bits.shape[1] // 4, out)
+    return tuple(out)
+
+
+def k2p(ts: int, tv: int, valid: int) -> float:
+    return lib().rnj_oracle_k2p(ts, tv, valid)
+
+
+def jc(ts: int, tv: int, valid: int) -> float:
+    return lib().rnj_oracle_jc(ts, tv, valid)
+
+
+def counts_triangle(aln: np.ndarray, threads: int = 8):
+    bits, gaps, bsc = pack(aln)
+    n = aln.shape[0]
+    m = n * (n - 1) // 2
+    ts = np.zeros(max(m, 1), dtype=np.uint32)
+    tv = np.zeros_like(ts)
+    va = np.zeros_like(ts)
+    rc = lib().rnj_oracle_counts_triangle(bits, gaps, n, bsc, threads, ts, tv, va)
+    if rc != 0:
+        raise RuntimeError("oracle counts failed")
+    return ts[:m], tv[:m], va[:m]
+
+
+def distance_matrix(aln: np.ndarray, model: str, threads: int = 8):
+    """-> (triangle before post-processing, full post-processed n x n matrix, max)."""
+    bits, gaps, bsc = pack(aln)
+    n = aln.shape[0]
+    m = n * (n - 1) // 2
+    tri = np.zeros(max(m, 1), dtype=np.float32)
+    full = np.zeros((n, n), dtype=np.float32)
+    mx = ctypes.c_float()
+    rc = lib().rnj_oracle_distance_matrix(bits, gaps, n, bsc, model_id(model), threads,
+                                          tri.ctypes.data, full.ctypes.data, ctypes.byref(mx))
+    if rc != 0:
+        raise RuntimeError("oracle distance matrix failed")
+    return tri[:m], full, mx.value
+
+
+def correct_triangle(ts, tv, valid, n: int, model: str):
+    m = n * (n - 1) // 2
+    tri = np.zeros(max(m, 1), dtype=np.float32)
+    full = np.zeros((n, n), dtype=np.float32)
+    mx = ctypes.c_float()
+    ts = np.ascontiguousarray(ts, dtype=np.uint32)
+    tv = np.ascontiguousarray(tv, dtype=np.uint32)
+    valid = np.ascontiguousarray(valid, dtype=np.uint32)
+    if m == 0:
+        ts = tv = valid = np.zeros(1, dtype=np.uint32)
+    lib().rnj_oracle_correct_triangle(ts, tv, valid, n, model_id(model), tri.ctypes.data,
+                                      full.ctypes.data, ctypes.byref(mx))
+    return tri[:m], full, mx.value
+
+
+def triangle_to_full(tri: np.ndarray, n: int, maxd: float | None = None) -> np.ndarray:
+    """Post-process a raw upper triangle the way the reference does (symmetric, diag 0, -1 -> 2max)."""
+    full = np.zeros((n, n), dtype=np.float32)
+    iu = np.triu_indices(n, 1)
+    t = tri.astype(np.float32).copy()
+    if maxd is None:
+        maxd = float(np.max(np.concatenate([[np.float32(0)], t[t > 0]])))
+    t[t == -1] = np.float32(maxd) * np.float32(2)
+    full[iu] = t
+    full[(iu[1], iu[0])] = t
+    return full
+
+
+def ref_harness_available() -> bool:
+    return os.path.exists(REF_HARNESS)

@@ -1,0 +1,212 @@
+// Command-line driver around the reference distance engine, built by `make -C oracle ref`
+// from the reference's own sources (compiled in place, never copied). It is test
+// infrastructure: it writes golden vectors for tests/golden/ and serves as the measured
+// CPU baseline ("kind": "reference") of bench.py. It is never part of the product path.
+//
+// Entry points it drives, all in the reference tree:
+//   dataloaderFasta::readData / getBitStrings / getGapFilters   dataloaderFasta.hpp:24-99,219-251
+//   bitDistanceGap::computeDistance                             bitDistanceGap.cpp:60-98
+//   simpleDistanceCalculator::computeDistance (the -f path)     simpleDistanceCalculator.cpp:8-16
+//   KimuraDistance::computeDistances / getDistanceMatrix        KimuraDistance.cpp:75-99,397-399
+//   JCdistance::computeDistanceMatrix / getDistanceMatrix       JCdistance.cpp:24-44,232-234
+//   distMatrixReader + rapidNJ::run + polytree::serialize_tree  src/main.cpp:419-437,288-296
+//
+// Modes (all binary outputs are little-endian):
+//   pack   FASTA OUT                  u32 n, L, bitStringsCount; then n*bsc*4 u32 bit strings,
+//                                     then n*bsc*4 u32 gap filters
+//   counts FASTA OUT [naive]          u32 n, L; then for i<j (row-major) u32 ts, tv, valid
+//   matrix FASTA jc|kim T OUT [naive] u32 n; then n*n f32 (post-processed matrix); prints seconds
+//   time   FASTA jc|kim T REPS        prints {"best_s":..,"mean_s":..,"n":..,"L":..,"threads":..}
+//   nj     MATRIX OUT                 MATRIX = u32 n + n*n f32; writes Newick (rapidNJ) with
+//                                     leaf names "t<i>"
+//   tree   FASTA jc|kim T OUT         distance matrix + rapidNJ; writes Newick
+
+#include "stdinclude.h"
+#include "dataloaderFasta.hpp"
+#include "bitDistanceGap.hpp"
+#include "simpleDistanceCalculator.hpp"
+#include "KimuraDistance.hpp"
+#include "JCdistance.hpp"
+#include "distMatrixReader.hpp"
+#include "rapidNJ.h"
+#include "polytree.h"
+#include "ProgressBar.hpp"
+
+#include <chrono>
+#include <cstdint>
+#include <fstream>
+#include <sstream>
+
+namespace {
+
+void die(const std::string& msg) {
+  std::cerr << "ref_harness: " << msg << std::endl;
+  std::exit(2);
+}
+
+dataloaderFasta* load(const std::string& path, bool fastdist) {
+  dataloaderFasta* dl = new dataloaderFasta();
+  dl->readData(path, fastdist, false, UNKNOWN, false);
+  if (dl->type != DNA) die("input is not detected as DNA");
+  return dl;
+}
+
+void put_u32(std::ofstream& o, uint32_t v) { o.write(reinterpret_cast<const char*>(&v), 4); }
+
+std::ofstream open_out(const std::string& path) {
+  std::ofstream o(path.c_str(), std::ios::binary);
+  if (!o) die("cannot write " + path);
+  return o;
+}
+
+distType** compute_matrix(dataloader* dl, const std::string& model, int threads, bool fastdist,
+                          double* seconds) {
+  auto t0 = std::chrono::steady_clock::now();
+  distType** m = NULL;
+  if (model == "kim") {
+    KimuraDistance* alg = new KimuraDistance(false, fastdist, dl, NULL);
+    alg->computeDistances(threads);
+    m = alg->getDistanceMatrix();
+    delete alg;
+  } else if (model == "jc") {
+    JCdistance* alg = new JCdistance(false, fastdist, dl, NULL);
+    alg->computeDistanceMatrix(threads);
+    m = alg->getDistanceMatrix();
+    delete alg;
+  } else {
+    die("model must be jc or kim");
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return m;
+}
+
+void free_matrix(distType** m, int n) {
+  for (int i = 0; i < n; i++) delete[] m[i];
+  delete[] m;
+}
+
+int mode_pack(int argc, char** argv) {
+  if (argc < 4) die("pack FASTA OUT");
+  dataloaderFasta* dl = load(argv[2], true);
+  uint32_t n = dl->getSequenceCount(), L = dl->getSequenceLength(), bsc = dl->getBitStringsCount();
+  std::ofstream o = open_out(argv[3]);
+  put_u32(o, n); put_u32(o, L); put_u32(o, bsc);
+  for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getBitStrings()[i]), bsc * 16);
+  for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getGapFilters()[i]), bsc * 16);
+  delete dl;
+  return 0;
+}
+
+int mode_counts(int argc, char** argv) {
+  if (argc < 4) die("counts FASTA OUT [naive]");
+  bool naive = argc > 4 && std::string(argv[4]) == "naive";
+  dataloaderFasta* dl = load(argv[2], !naive);
+  DistanceEstimate* est = naive ? static_cast<DistanceEstimate*>(new simpleDistanceCalculator(dl))
+                                : static_cast<DistanceEstimate*>(new bitDistanceGap(dl));
+  uint32_t n = dl->getSequenceCount(), L = dl->getSequenceLength();
+  std::ofstream o = open_out(argv[3]);
+  put_u32(o, n); put_u32(o, L);
+  unsigned long long d[3];
+  for (uint32_t i = 0; i < n; i++)
+    for (uint32_t j = i + 1; j < n; j++) {
+      est->computeDistance(i, j, d);
+      put_u32(o, (uint32_t)d[0]); put_u32(o, (uint32_t)d[1]); put_u32(o, (uint32_t)d[2]);
+    }
+  delete est;
+  delete dl;
+  return 0;
+}
+
+int mode_matrix(int argc, char** argv) {
+  if (argc < 6) die("matrix FASTA jc|kim T OUT [naive]");
+  bool naive = argc > 6 && std::string(argv[6]) == "naive";
+  dataloaderFasta* dl = load(argv[2], !naive);
+  int n = dl->getSequenceCount();
+  double secs = 0;
+  distType** m = compute_matrix(dl, argv[3], atoi(argv[4]), !naive, &secs);
+  std::ofstream o = open_out(argv[5]);
+  put_u32(o, (uint32_t)n);
+  for (int i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(m[i]), sizeof(distType) * n);
+  std::cout << "seconds=" << secs << std::endl;
+  free_matrix(m, n);
+  delete dl;
+  return 0;
+}
+
+int mode_time(int argc, char** argv) {
+  if (argc < 6) die("time FASTA jc|kim T REPS");
+  dataloaderFasta* dl = load(argv[2], true);
+  int n = dl->getSequenceCount();
+  int threads = atoi(argv[4]), reps = atoi(argv[5]);
+  double best = 1e30, total = 0;
+  for (int r = 0; r < reps; r++) {
+    double s = 0;
+    distType** m = compute_matrix(dl, argv[3], threads, true, &s);
+    free_matrix(m, n);
+    best = std::min(best, s);
+    total += s;
+  }
+  std::cout << "{\"best_s\": " << best << ", \"mean_s\": " << total / reps << ", \"n\": " << n
+            << ", \"L\": " << dl->getSequenceLength() << ", \"threads\": " << threads << "}" << std::endl;
+  delete dl;
+  return 0;
+}
+
+void run_nj(distType** m, int n, std::ostream& out) {
+  vector<string>* names = new vector<string>;
+  for (int i = 0; i < n; i++) {
+    std::ostringstream s;
+    s << "t" << i;
+    names->push_back(s.str());
+  }
+  ProgressBar* pb = new ProgressBar();
+  distMatrixReader* reader = new distMatrixReader(false, n, false, names, m);
+  reader->initializeData();
+  rapidNJ* nj = new rapidNJ(reader, n, false, pb);
+  polytree* tree = nj->run();
+  tree->serialize_tree(out);
+  out.flush();
+}
+
+int mode_nj(int argc, char** argv) {
+  if (argc < 4) die("nj MATRIX OUT");
+  std::ifstream in(argv[2], std::ios::binary);
+  if (!in) die(std::string("cannot read ") + argv[2]);
+  uint32_t n = 0;
+  in.read(reinterpret_cast<char*>(&n), 4);
+  distType** m = new distType*[n];
+  for (uint32_t i = 0; i < n; i++) {
+    m[i] = new distType[n];
+    in.read(reinterpret_cast<char*>(m[i]), sizeof(distType) * n);
+  }
+  if (!in) die("short matrix file");
+  std::ofstream o(argv[3]);
+  run_nj(m, (int)n, o);  // rapidNJ's destructor path owns the rows; the process exits next
+  return 0;
+}
+
+int mode_tree(int argc, char** argv) {
+  if (argc < 6) die("tree FASTA jc|kim T OUT");
+  dataloaderFasta* dl = load(argv[2], true);
+  int n = dl->getSequenceCount();
+  distType** m = compute_matrix(dl, argv[3], atoi(argv[4]), true, NULL);
+  std::ofstream o(argv[5]);
+  run_nj(m, n, o);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) die("usage: ref_harness pack|counts|matrix|time|nj|tree ...");
+  std::string mode = argv[1];
+  if (mode == "pack") return mode_pack(argc, argv);
+  if (mode == "counts") return mode_counts(argc, argv);
+  if (mode == "matrix") return mode_matrix(argc, argv);
+  if (mode == "time") return mode_time(argc, argv);
+  if (mode == "nj") return mode_nj(argc, argv);
+  if (mode == "tree") return mode_tree(argc, argv);
+  die("unknown mode " + mode);
+  return 2;
+}

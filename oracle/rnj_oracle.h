@@ -1,0 +1,87 @@
+/* CPU restatement of rapidNJ's DNA pairwise-distance path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+ * may load this library, and only as the checker; the product path (rapidnj_amd, include/) never
+ * links, loads or falls back to it.
+ *
+ * Every function cites the reference lines it restates. Layout of packed data follows the
+ * reference exactly: per taxon `bsc` 128-bit words (bsc = L/64 + 6), 64 sites per word, site s in
+ * 32-bit word s/16 at bits 2*(s%16); codes A=00 G=01 T=10 C=11; the gap filter holds 01 in the
+ * slot of every A/C/G/T (either case) and 00 elsewhere.
+ *
+ * Parity pins (tests/test_oracle.py):
+ *   packing        golden words from the compiled reference (tests/golden/) + char rules
+ *   counts         golden ts/tv/valid from the compiled reference; char-level brute force with
+ *                  an independent purine/pyrimidine formulation; invariants
+ *   corrections    golden float bits from the compiled reference; closed-form special cases
+ *   post-process   golden matrices (symmetry, diagonal, sentinel = 2*max)
+ */
+#ifndef RNJ_ORACLE_H
+#define RNJ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { RNJ_ORACLE_JC = 0, RNJ_ORACLE_K2P = 1 };
+
+/* bitStringsCount for a DNA alignment of length L (dataloaderFasta.hpp:222). */
+unsigned rnj_oracle_bitstrings_count(unsigned L);
+
+/* 1 if the first sequence classifies the alignment as DNA (dataloaderFasta.hpp:54-76). */
+int rnj_oracle_is_dna(const char* seq, unsigned L);
+
+/* Pack n sequences (row-major n x L chars) into bits/gaps, each n x (bsc*4) u32
+ * (dataloaderFasta.hpp:147-182 encoding, :219-251 padding with '-'). */
+void rnj_oracle_pack(const char* aln, unsigned n, unsigned L, unsigned bsc, uint32_t* bits,
+                     uint32_t* gaps);
+
+/* Char-level brute force with the packed path's semantics: a site counts when both chars are
+ * one of ACGTacgt; tv when one is a purine (A,G) and the other a pyrimidine (C,T); ts when both
+ * are in the same class and differ. out = {ts, tv, valid}. */
+void rnj_oracle_count_chars(const char* a, const char* b, unsigned L, uint64_t out[3]);
+
+/* The reference's naive (-f) counter on raw chars, including its resolveChar step
+ * (simpleDistanceCalculator.cpp:58-98, dataloaderFasta.hpp:184-199,268-271). */
+void rnj_oracle_count_naive(const char* a, const char* b, unsigned L, uint64_t out[3]);
+
+/* Word-level restatement of bitDistanceGap's per-word logic (bitDistanceGap.cpp:138-161) with
+ * scalar popcounts over `nwords` 32-bit words. */
+void rnj_oracle_count_words(const uint32_t* bi, const uint32_t* gi, const uint32_t* bj,
+                            const uint32_t* gj, unsigned nwords, uint64_t out[3]);
+
+/* SSE2 SWAR counter with the reference's lane-widening schedule (bitDistanceGap.cpp:60-98,
+ * 134-336): processes 6*floor(bsc/6) 128-bit words. */
+void rnj_oracle_count_swar(const uint32_t* bi, const uint32_t* gi, const uint32_t* bj,
+                           const uint32_t* gj, unsigned bsc, uint64_t out[3]);
+
+/* Corrections (KimuraDistance.cpp:339-348, JCdistance.cpp:172-183); -1 marks saturation. */
+float rnj_oracle_k2p(uint64_t ts, uint64_t tv, uint64_t valid);
+float rnj_oracle_jc(uint64_t ts, uint64_t tv, uint64_t valid);
+
+/* Counts of every pair i<j into packed upper-triangle arrays (row-major, row i holds j=i+1..n-1),
+ * with `threads` pthreads walking rows i = tid (mod threads) (KimuraDistance.cpp:300-320). */
+int rnj_oracle_counts_triangle(const uint32_t* bits, const uint32_t* gaps, unsigned n,
+                               unsigned bsc, int threads, uint32_t* ts, uint32_t* tv,
+                               uint32_t* valid);
+
+/* The whole driver: threaded counts + correction + global max + post-process
+ * (KimuraDistance.cpp:75-99,212-297,300-364,366-395; JCdistance.cpp:24-44,92-145,148-230).
+ * tri (n(n-1)/2, may be NULL) receives the per-pair values before post-processing (-1 for
+ * saturated pairs); full (n*n, may be NULL) the post-processed symmetric matrix; *max_out the
+ * global maximum used for the sentinel (sentinel = 2*max). Returns 0 on success. */
+int rnj_oracle_distance_matrix(const uint32_t* bits, const uint32_t* gaps, unsigned n,
+                               unsigned bsc, int model, int threads, float* tri, float* full,
+                               float* max_out);
+
+/* Correction + post-process only, from given upper-triangle counts. */
+int rnj_oracle_correct_triangle(const uint32_t* ts, const uint32_t* tv, const uint32_t* valid,
+                                unsigned n, int model, float* tri, float* full, float* max_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RNJ_ORACLE_H */
