@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X distance engine on one GPU, with roofline and CPU baseline.
+
+A step is one pass of the hot path over the resident input: the packed alignment is in HBM, the
+step computes every pair's ts/tv/valid counts, the correction, the running maximum and the
+saturation patch (2 * max) into a float32 upper triangle that stays in HBM (the reference's
+computeDistances region, KimuraDistance.cpp:75-99, minus parsing). Default workload: 20000 taxa x
+5000 sites, 5% gaps, K2P. Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--workload large|plain|small_jc|bootstrap]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "G site-pairs/s on distance matrix vs CPU; p50/p99 job latency served"
+# int32 VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (peak engine clock)
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
+OPS_PER_WORD_PAIR = {"k2p": 7, "jc": 6}  # VALU instructions per 32 sites and pair
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(aln, model: str, sample_n: int, threads: int) -> dict:
+    """Reference CPU engine (compiled from its sources into oracle/_ref) on the first sample_n
+    taxa of the same alignment; falls back to the oracle's restatement when the harness is absent."""
+    from oracle import oracle as O
+    from rapidnj_amd.workloads import write_fasta
+    sub = aln[:sample_n]
+    n, L = sub.shape
+    site_pairs = n * (n - 1) // 2 * L
+    if O.ref_harness_available():
+        with tempfile.TemporaryDirectory() as tmp:
+            fa = os.path.join(tmp, "sample.fa")
+            write_fasta(fa, sub)
+            out = subprocess.run([O.REF_HARNESS, "time", fa, "kim" if model == "k2p" else "jc",
+                                  str(threads), "1"], check=True, capture_output=True, text=True)
+            secs = json.loads(out.stdout.strip().splitlines()[-1])["best_s"]
+        kind = "reference"
+    else:
+        t0 = time.perf_counter()
+        O.distance_matrix(sub, model, threads=threads)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    return {"value": round(site_pairs / secs / 1e9, 3), "unit": "G site-pairs/s",
+            "cores": threads, "kind": kind,
+            "sample": f"first {n} taxa x {L} sites of the same alignment, {model.upper()}, "
+                      f"compute+post-process {secs:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="large",
+                    choices=["large", "plain", "small_jc", "bootstrap"])
+    ap.add_argument("--tile-config", type=int, default=-1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 2000 x 10000 side run")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per "
+              "box); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from rapidnj_amd import native as N
+    from rapidnj_amd.workloads import WORKLOADS, bootstrap_weights, simulate_alignment
+
+    wl = WORKLOADS[args.workload]
+    log(f"workload {wl.name}: generating {wl.n} x {wl.L} (seed {wl.seed})")
+    aln = simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    ctx = N.Context(0)
+    if args.tile_config >= 0:
+        ctx.set_tile_config(args.tile_config)
+    A = ctx.alignment(aln)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    n, L = wl.n, wl.L
+    pairs = n * (n - 1) // 2
+    R = max(1, wl.replicates)
+    site_pairs = pairs * L * R
+
+    if wl.replicates:
+        weights = bootstrap_weights(L, wl.replicates, seed0=0)
+        out = torch.empty((R, pairs), dtype=torch.float32, device=dev)
+
+        def step(raw=False):
+            ctx.bootstrap(A, weights, wl.model, out=out, raw=raw, stream=stream)
+    else:
+        out = torch.empty(pairs, dtype=torch.float32, device=dev)
+
+        def step(raw=False):
+            ctx.distances(A, wl.model, out=out, raw=raw, stream=stream)
+
+    log(f"warmup {args.warmup} (first launch per shape tunes the tile configuration)")
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    ms = secs / args.steps * 1e3
+    value = site_pairs / (ms / 1e3) / 1e9
+    log(f"{ms:.3f} ms/step = {value:.1f} G site-pairs/s")
+
+    # kernel duration: raw launches (statistics reset + tile kernel, no patch) between HIP
+    # events recorded on the stream the library launches on
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(3, args.steps)
+    step(raw=True)
+    e0.record(stream)
+    for _ in range(reps):
+        step(raw=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kern_ms = e0.elapsed_time(e1) / reps
+    if wl.replicates:  # the bootstrap step also gathers replicate columns; report the whole launch
+        kern_label = "bootstrap launch (gather + tile kernel)"
+    else:
+        kern_label = "tile_kernel (+ stats reset)"
+    ops = OPS_PER_WORD_PAIR[wl.model] * pairs * (L / 32.0) * R
+    achieved = ops / (kern_ms / 1e3) / 1e12
+    algo_bytes = pairs * 4 * R + n * 3 * L / 8 * R
+    cfg = ctx.tile_config(n, L, wl.model)
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                "kernel": kern_label, "kernel_ms": round(kern_ms, 4),
+                "ops_per_launch": ops, "tile_config": N.tile_configs()[cfg],
+                "hbm": {"achieved": round(algo_bytes / (kern_ms / 1e3) / 1e9, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "algo_bytes": algo_bytes}}
+
+    extra = {}
+    if not args.no_extra and args.workload == "large":
+        pw = WORKLOADS["plain"]
+        log(f"side run {pw.name}")
+        p_aln = simulate_alignment(pw.n, pw.L, pw.gap, pw.seed)
+        PA = ctx.alignment(p_aln)
+        p_pairs = pw.n * (pw.n - 1) // 2
+        p_out = torch.empty(p_pairs, dtype=torch.float32, device=dev)
+        for _ in range(3):
+            ctx.distances(PA, pw.model, out=p_out, stream=stream)
+        torch.cuda.synchronize()
+        k = 20
+        t1 = time.perf_counter()
+        for _ in range(k):
+            ctx.distances(PA, pw.model, out=p_out, stream=stream)
+        torch.cuda.synchronize()
+        p_ms = (time.perf_counter() - t1) / k * 1e3
+        extra["plain_2000x10000_k2p"] = {"ms_per_step": round(p_ms, 4),
+                                         "value": round(p_pairs * pw.L / (p_ms / 1e3) / 1e9, 2),
+                                         "unit": "G site-pairs/s"}
+        PA.close()
+
+    base = None
+    if not args.no_cpu_baseline:
+        threads = cpu_cores()
+        sample_n = min(n, 8000) if L >= 5000 else min(n, 12000)
+        log(f"CPU baseline on {sample_n} taxa with {threads} threads")
+        base = cpu_baseline(aln, wl.model, sample_n, threads)
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "G site-pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded K2P-evolved alignment on a random-joining tree)",
+        "config": {"workload": wl.name, "taxa": n, "sites": L, "gap": wl.gap, "model": wl.model,
+                   "replicates": wl.replicates, "output": "float32 upper triangle in HBM",
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": base,
+    }
+    if extra:
+        line["extra"] = extra
+    print(json.dumps(line), flush=True)
+    A.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

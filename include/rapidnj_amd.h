@@ -1,0 +1,204 @@
+/* rapidnj_amd: MI355X (gfx950) pairwise evolutionary-distance engine for DNA alignments.
+ *
+ * C ABI of librapidnj_amd.so. Plain pointers and sizes only. Two groups of entry points:
+ *
+ *  1. The reference's own device boundary, same names, arguments and meaning
+ *     (src/distanceCalculation/gpu/kimuraDNA_gpu.cu:199-276,
+ *      src/distanceCalculation/gpu/distanceCalculatorDNA_gpu.cu:190-287, declared by the caller at
+ *      src/distanceCalculation/KimuraDistance.cpp:17-28). One implicit module-wide state on device
+ *      0; errors print a message and exit(1), as the reference's cutilSafeCall does.
+ *
+ *  2. A handle-based API (rnj_*) that returns a status code (RNJ_OK = 0) and keeps the reason in
+ *     a per-thread message (rnj_last_error). Used by the C++ host classes in
+ *     include/rapidnj_amd_distance.hpp, which mirror the reference's KimuraDistance / JCdistance.
+ *
+ * Counts are exact integers: ts (transitions), tv (transversions) and valid (sites where both
+ * taxa hold one of ACGTacgt). Distances follow the reference's CPU recipes in FP64
+ * (KimuraDistance.cpp:339-348, JCdistance.cpp:170-183); saturated pairs carry -1 until the
+ * reference post-process replaces them by 2 * max (KimuraDistance.cpp:366-395).
+ */
+#ifndef RAPIDNJ_AMD_H
+#define RAPIDNJ_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define RNJ_EXPORT __attribute__((visibility("default")))
+#else
+#define RNJ_EXPORT
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RNJ_API_VERSION 1
+
+/* status codes */
+enum {
+  RNJ_OK = 0,
+  RNJ_E_INVALID = 1,    /* bad argument (null pointer, n < 2, unknown model, ...) */
+  RNJ_E_HIP = 2,        /* a HIP runtime call failed */
+  RNJ_E_NOMEM = 3,      /* device or pinned host allocation failed */
+  RNJ_E_NOT_DNA = 4,    /* first sequence is not DNA (dataloaderFasta.hpp:54-76) */
+  RNJ_E_ADMISSION = 5,  /* a served job can never fit the server's HBM budget */
+  RNJ_E_CANCELLED = 6,  /* a served job was cancelled */
+  RNJ_E_NOT_READY = 7,  /* a served job has not finished (rnj_server_poll) */
+  RNJ_E_NO_DEVICE = 8   /* no HIP device */
+};
+
+/* models: rnj_*_distances */
+enum { RNJ_JC = 0, RNJ_K2P = 1 };
+
+/* output layouts for device distance matrices */
+enum {
+  RNJ_TRIANGLE = 0, /* n(n-1)/2 floats, upper triangle row-major: row i holds j = i+1..n-1 */
+  RNJ_SQUARE = 1    /* n rows of `ld` floats: symmetric, diagonal 0 */
+};
+
+/* flags for rnj_distances_device / rnj_bootstrap_device */
+enum {
+  RNJ_FLAG_RAW = 1 /* keep -1 for saturated pairs (no 2*max post-process) */
+};
+
+typedef struct rnj_context rnj_context;
+typedef struct rnj_alignment rnj_alignment;
+typedef struct rnj_server rnj_server;
+
+/* ---- reference device boundary (group 1) ------------------------------------------------ */
+
+/* kimuraDNA_gpu.cu:241-276. bsStride: u32 words per taxon in the staging layout of
+ * dataloader::initGPUData (dataloader.hpp:38-70, 16 sites per word); the first
+ * min(bsStride, 4*bitStringCount) words of each taxon are counted. */
+RNJ_EXPORT void initialiseDNA2_gpu(unsigned int sequenceCount, unsigned int bitStringCount,
+                        unsigned int bsStride);
+/* kimuraDNA_gpu.cu:235-239: host arrays [sequenceCount][bsStride] of codes and gap filters. */
+RNJ_EXPORT void storeDataDNA2_gpu(unsigned int* bitStrings, unsigned int* gapFilters);
+/* kimuraDNA_gpu.cu:199-233: K2P distances into host results[sequenceCount*sequenceCount],
+ * row-major. Entries j > i hold the distance with -1 for saturated pairs (the caller,
+ * KimuraDistance::computeDistancesDNAGPU2 at KimuraDistance.cpp:164-193, post-processes);
+ * this implementation also fills j < i symmetrically and the diagonal with 0. */
+RNJ_EXPORT void computeDistancesDNA2_gpu(float* results);
+
+/* distanceCalculatorDNA_gpu.cu:244-287, :238-242: same arguments for the raw-count path. */
+RNJ_EXPORT void initialiseDNA_gpu(unsigned int sequenceCount, unsigned int bitStringCount,
+                       unsigned int bsStride);
+RNJ_EXPORT void storeDataDNA_gpu(unsigned int* bitStrings, unsigned int* gapFilters);
+/* distanceCalculatorDNA_gpu.cu:190-226: ts, tv and gaps (= valid sites) into host arrays of
+ * sequenceCount*sequenceCount u32, row-major, entries j > i (j < i mirrored, diagonal 0). */
+RNJ_EXPORT void computeDistancesDNA_gpu(unsigned int* ts, unsigned int* tv, unsigned int* gaps);
+/* distanceCalculatorDNA_gpu.cu:228-236: waits for the device; results were already copied. */
+RNJ_EXPORT void getResultsDNA_gpu(unsigned int* ts, unsigned int* tv, unsigned int* gaps);
+
+/* ---- handle API (group 2) --------------------------------------------------------------- */
+
+RNJ_EXPORT const char* rnj_last_error(void);           /* message of this thread's last failure */
+RNJ_EXPORT const char* rnj_status_string(int status);  /* static name of a status code */
+RNJ_EXPORT int rnj_api_version(void);
+
+/* Context on HIP device `device`: streams, kernel configuration and scratch buffers.
+ * No counterpart in the reference (its GPU state is module-static, kimuraDNA_gpu.cu:178-191). */
+RNJ_EXPORT int rnj_create(int device, rnj_context** out);
+RNJ_EXPORT int rnj_destroy(rnj_context* ctx);
+RNJ_EXPORT int rnj_synchronize(rnj_context* ctx); /* waits for all work the context enqueued */
+
+/* Tile-kernel configurations: -1 = choose per shape with the built-in autotuner (default). */
+RNJ_EXPORT int rnj_tile_config_count(void);
+RNJ_EXPORT const char* rnj_tile_config_name(int config);
+RNJ_EXPORT int rnj_set_tile_config(rnj_context* ctx, int config);
+RNJ_EXPORT int rnj_get_tile_config(rnj_context* ctx, unsigned n, unsigned L, int model, int* config);
+
+/* Alignments packed once into HBM as H/L/V bit planes.
+ * from_chars: n rows of L bytes, row t at chars + t*pitch, on the host (chars_on_device = 0) or
+ *   already in device memory (chars_on_device = 1). Character rules of the reference's FASTA
+ *   packer: ACGT of either case are counted, everything else ('-', N, U, IUPAC, '.', '?') is not
+ *   (dataloaderFasta.hpp:147-182). The first row must classify as DNA (dataloaderFasta.hpp:54-76)
+ *   unless `skip_type_check`; n >= 2 and L >= 1 (dataloaderFasta.hpp:32-35).
+ * from_rows: one pointer per taxon, like dataloader::getSequences (dataloader.hpp:26).
+ * from_bitstrings: the reference's packed strings, per taxon bitStringsCount x 128 bits of codes
+ *   and gap filters, like dataloader::getBitStrings / getGapFilters (dataloader.hpp:20-24,
+ *   dataloaderFasta.hpp:219-251); the sites bitDistanceGap counts (6*floor(bsc/6) words of 64,
+ *   bitDistanceGap.cpp:19) are taken. */
+RNJ_EXPORT int rnj_alignment_from_chars(rnj_context* ctx, const char* chars, size_t pitch, unsigned n,
+                             unsigned L, int chars_on_device, int skip_type_check,
+                             rnj_alignment** out);
+RNJ_EXPORT int rnj_alignment_from_rows(rnj_context* ctx, const char* const* rows, unsigned n, unsigned L,
+                            rnj_alignment** out);
+RNJ_EXPORT int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned int* const* bitStrings,
+                                  const unsigned int* const* gapFilters, unsigned n,
+                                  unsigned bitStringsCount, rnj_alignment** out);
+RNJ_EXPORT int rnj_alignment_free(rnj_alignment* aln);
+/* n taxa, L sites, W 32-site words counted per taxon, bytes of HBM held. */
+RNJ_EXPORT int rnj_alignment_info(const rnj_alignment* aln, unsigned* n, unsigned* L, unsigned* W,
+                       size_t* bytes);
+/* Copy the packed planes [3][W][n] (H, L, V; site s of word w at bit s%32) to the host. */
+RNJ_EXPORT int rnj_alignment_planes(const rnj_alignment* aln, uint32_t* host_planes);
+
+/* One distance matrix into device memory `out` (layout RNJ_TRIANGLE or RNJ_SQUARE with row
+ * stride `ld` floats), enqueued on `stream` (a hipStream_t; NULL = the context's stream).
+ * Saturated pairs become 2 * max unless RNJ_FLAG_RAW. If max_out (device pointer) is non-null,
+ * the matrix maximum (KimuraDistance.cpp:282-287) is written there as float.
+ * Replaces KimuraDistance::computeDistances / JCdistance::computeDistanceMatrix
+ * (KimuraDistance.cpp:75-99, JCdistance.cpp:24-44) for one job. */
+RNJ_EXPORT int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, int model, int layout,
+                         float* out, size_t ld, int flags, float* max_out, void* stream);
+
+/* Exact ts / tv / valid of every pair into device arrays of n(n-1)/2 u32, upper triangle
+ * row-major (DistanceEstimate::computeDistance, DistanceEstimate.hpp:14). */
+RNJ_EXPORT int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, uint32_t* tv,
+                      uint32_t* valid, void* stream);
+
+/* Host hand-off: rows[i] are n caller-owned arrays of n floats (e.g. new distType[n]), filled
+ * with the post-processed symmetric matrix, diagonal 0, sentinels patched: the in-memory matrix
+ * handed to distMatrixReader (src/main.cpp:419-437, distMatrixReader.cpp:14-23). Synchronous.
+ * With row0/rows a slab of rows [row0, row0+rows) is written (rows[k] is row row0+k), so a
+ * caller can stream rows to disk (diskMatrix::writeArray, diskMatrix.cpp:96-101). */
+RNJ_EXPORT int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model, float** rows,
+                       float* max_out);
+RNJ_EXPORT int rnj_distances_row_slab(rnj_context* ctx, const rnj_alignment* aln, int model,
+                           unsigned row0, unsigned rows, float** out_rows, float* max_out);
+
+/* Batched bootstrap: R replicates given as column-weight vectors weights[r*L + c] (host u16;
+ * replicate r repeats column c weights[r][c] times, the expanded alignment of
+ * dataloader::sample_fast_dist, dataloader.cpp:16-58). All replicates run in one tile launch.
+ * out holds R matrices, matrix r at out + r*matrix_stride floats (layout as above). */
+RNJ_EXPORT int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
+                         const uint16_t* weights, unsigned R, int layout, float* out,
+                         size_t matrix_stride, size_t ld, int flags, void* stream);
+
+/* ---- served jobs ----------------------------------------------------------------------------
+ * A request loop on its own thread admits submitted jobs while their HBM need fits the budget,
+ * batches the admitted jobs into one multi-job tile launch per batch, keeps up to `slots` batches
+ * on the device at once (each on its own stream, so uploads, counting and downloads of
+ * different batches overlap) and hands each job's post-processed upper triangle back. */
+typedef struct {
+  unsigned slots;           /* batches in flight (default 2) */
+  unsigned max_batch_jobs;  /* jobs per launch (default 64) */
+  size_t hbm_budget;        /* bytes the server may hold; 0 = 80% of free HBM at creation */
+} rnj_server_config;
+
+typedef struct {
+  double queued_s;   /* submit -> admitted */
+  double device_s;   /* admitted -> results on the host */
+  double total_s;    /* submit -> results on the host */
+  unsigned batch_id; /* launch the job ran in */
+  unsigned batch_jobs;
+} rnj_job_stats;
+
+RNJ_EXPORT int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server** out);
+/* chars: n rows of L bytes (host), row t at chars + t*pitch; copied before returning. */
+RNJ_EXPORT int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned n, unsigned L,
+                      int model, uint64_t* ticket);
+/* Blocks until the job ends; tri_out (host, n(n-1)/2 floats, may be null) receives the
+ * post-processed upper triangle. Releases the ticket. */
+RNJ_EXPORT int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_stats* stats);
+RNJ_EXPORT int rnj_server_poll(rnj_server* srv, uint64_t ticket); /* RNJ_OK when done, RNJ_E_NOT_READY */
+RNJ_EXPORT int rnj_server_cancel(rnj_server* srv, uint64_t ticket); /* only before admission */
+RNJ_EXPORT int rnj_server_destroy(rnj_server* srv);                  /* finishes admitted work first */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAPIDNJ_AMD_H */

@@ -1,0 +1,90 @@
+// Internal declarations shared by the HIP kernels (kernels.hip) and the runtime (runtime.cpp).
+//
+// Packed alignment in HBM ("bit planes"). For taxon t and 32-site word w:
+//   planes[0][w][t]  H bits: 1 for a pyrimidine (C/T), the high bit of the reference's 2-bit code
+//   planes[1][w][t]  L bits: 1 for G/C, the low bit of that code (A=00 G=01 T=10 C=11)
+//   planes[2][w][t]  V bits: 1 where the site holds one of ACGTacgt
+// with site s of word w at bit s % 32. Taxa are the fastest index, so a row of taxa for one word
+// is one contiguous, coalesced run. n_pad is n rounded up to kTaxaPad and the padded taxa are
+// all-zero; W_pad is the word count rounded up to kWordPad and padded words are zero, so padding
+// contributes nothing to any count. For a taxon pair (i, j) and one word:
+//   g = Vi & Vj, tv = (Hi ^ Hj) & g, ts = (Li ^ Lj) & (g & ~tv)
+// which is the reference's per-word logic (bitDistanceGap.cpp:138-161) on split planes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rnj {
+
+constexpr unsigned kTaxaPad = 128;    // n_pad multiple (largest tile edge)
+constexpr unsigned kWordPad = 16;     // W_pad multiple (largest LDS site chunk, in words)
+constexpr unsigned kBlockThreads = 256;
+constexpr uint32_t kSentinelBits = 0xBF800000u;  // -1.0f, the reference's saturation marker
+
+enum Model : int { kJC = 0, kK2P = 1, kCounts = 2 };
+enum Layout : int { kTriangle = 0, kSquare = 1 };
+
+// One distance matrix inside a launch. A launch covers jobs[0..njobs); the work items of job k
+// (tiles of TI x TJ taxon pairs over the upper triangle of tile blocks) are numbered
+// item_base .. item_base + tiles - 1.
+struct JobDesc {
+  const uint32_t* planes;     // [3][W_pad][n_pad]
+  unsigned long long plane_stride;  // W_pad * n_pad
+  unsigned n, n_pad, W;       // W: words to process (multiple of kWordPad, <= W_pad)
+  unsigned nblk;              // ceil(n / TI) for the launch's tile edge (TI == TJ)
+  unsigned tiles;             // nblk * (nblk + 1) / 2
+  unsigned item_base;
+  int model;                  // Model
+  int layout;                 // Layout (distances only)
+  unsigned long long ld;      // row stride in elements for kSquare
+  float* out;                 // distances
+  uint32_t* out_ts;           // counts (kCounts), upper triangle
+  uint32_t* out_tv;
+  uint32_t* out_valid;
+  unsigned* maxbits;          // bits of the running maximum distance (non-negative floats)
+  unsigned long long* sent;   // (i << 32 | j) of saturated pairs
+  unsigned sent_cap;
+  unsigned* sent_count;
+};
+
+// Upper-triangle position of pair (i, j), i < j: row i holds j = i+1 .. n-1 (row-major).
+__host__ __device__ inline unsigned long long tri_index(unsigned long long i, unsigned long long j,
+                                                       unsigned long long n) {
+  return i * n - i * (i + 1) / 2 + (j - i - 1);
+}
+
+// Tile-kernel configurations (pair-tile edge, per-thread micro-tile, LDS site chunk).
+struct TileConfig {
+  const char* name;
+  unsigned tile;     // TI == TJ
+  unsigned micro_i;  // pairs per thread along i
+  unsigned micro_j;  // pairs per thread along j
+  unsigned chunk;    // words of 32 sites staged in LDS per step
+};
+extern const TileConfig kTileConfigs[];
+extern const int kNumTileConfigs;
+
+// Launchers (kernels.hip). All enqueue on `stream` and return the launch status.
+hipError_t launch_pack_chars(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                             unsigned n_pad, unsigned W_pad, uint32_t* planes,
+                             hipStream_t stream);
+hipError_t launch_pack_legacy(const uint32_t* bits, const uint32_t* gaps, size_t stride,
+                              unsigned words16, unsigned n, unsigned n_pad, unsigned W_pad,
+                              uint32_t* planes, hipStream_t stream);
+hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
+hipError_t launch_tiles(int config, bool jc_only, const JobDesc* jobs, unsigned njobs,
+                        unsigned total_items, hipStream_t stream);
+hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
+hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
+                                  unsigned L, uint8_t* codes, hipStream_t stream);
+hipError_t launch_bootstrap_index(const uint16_t* weights, unsigned R, unsigned L,
+                                  unsigned* col_index, unsigned idx_stride, unsigned* lengths,
+                                  hipStream_t stream);
+hipError_t launch_bootstrap_gather(const uint8_t* codes, unsigned n_pad, const unsigned* col_index,
+                                   unsigned idx_stride, const unsigned* lengths, unsigned R,
+                                   unsigned W_pad, uint32_t* planes, hipStream_t stream);
+
+}  // namespace rnj

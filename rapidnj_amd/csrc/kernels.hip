@@ -1,0 +1,537 @@
+// HIP kernels for gfx950: alignment packing, the pair-tile count kernel with the fused JC / K2P
+// correction, the saturation patch, and bootstrap column resampling.
+//
+// The count kernel is integer VALU work: per 32 sites and taxon pair it issues
+//   g = Vi & Vj; t = bitop3((Hi ^ Hj) & g); u = g ^ t; s = bitop3((Li ^ Lj) & u);
+//   tv += bcnt(t); ts += bcnt(s); valid += bcnt(g)
+// i.e. 7 VALU instructions (v_and, 2x v_bitop3, v_xor, 3x v_bcnt with accumulate). JC needs only
+// ts + tv, which costs 6: the mismatch mask (Hi^Hj | Li^Lj) & g and g itself.
+#include "internal.hpp"
+
+#include <math.h>
+
+namespace rnj {
+
+const TileConfig kTileConfigs[] = {
+    {"t64_m4x4_c16", 64, 4, 4, 16},
+    {"t64_m4x4_c8", 64, 4, 4, 8},
+    {"t128_m8x8_c16", 128, 8, 8, 16},
+    {"t128_m8x4_c16", 128, 8, 4, 16},
+};
+const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
+
+namespace {
+
+// (a ^ b) & c in one v_bitop3_b32 (truth table over a=0xF0, b=0xCC, c=0xAA).
+__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x28u);
+}
+
+// popcount(x) + acc as one v_bcnt_u32_b32. Written out because LLVM otherwise re-associates
+// the per-word accumulations into v_bcnt + v_add3 trees (1.5 extra VALU ops per pair and word).
+__device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+  return r;
+}
+
+// Pointer in the global address space, so loads compile to global_load rather than flat_load
+// (the job table hands over generic pointers).
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+// Native 4 x u32 vector (ext_vector_type) for 128-bit global and LDS accesses.
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// ---- corrections ---------------------------------------------------------------------------
+// Kimura 2-parameter, KimuraDistance.cpp:339-348. The proportions are rounded to float first,
+// then promoted; the logs are double; -1 marks saturation and valid == 0.
+#pragma clang fp contract(off)
+__device__ __noinline__ float k2p_distance(uint32_t ts, uint32_t tv, uint32_t valid) {
+  float d = -1.0f;
+  if (valid != 0) {
+    const float P = (float)((double)ts / (double)valid);
+    const float Q = (float)((double)tv / (double)valid);
+    const float t1 = (float)(1.0 - 2.0 * (double)P - (double)Q);
+    const float t2 = (float)(1.0 - 2.0 * (double)Q);
+    if (!(t1 <= 0.0f || t2 <= 0.0f))
+      d = (float)(-0.5 * log(1.0 - 2.0 * (double)P - (double)Q) - 0.25 * log(1.0 - 2.0 * (double)Q));
+  }
+  return d;
+}
+
+// Jukes-Cantor with alpha = 4, JCdistance.cpp:170-183: float proportion, double log; valid == 0
+// gives p = 0 (and so -0.0f), not a sentinel.
+__device__ __noinline__ float jc_distance(uint32_t mismatches, uint32_t valid) {
+  const double b = 0.75;  // (alpha - 1) / alpha
+  float p = 0.0f;
+  if (valid != 0) p = (float)mismatches / (float)valid;
+  if ((double)p / b >= 1.0) return -1.0f;
+  return (float)(-b * log(1.0 - (double)p / b));
+}
+#pragma clang fp contract(on)
+
+// Row r of the block triangle holds tiles (r, r..nb-1); it starts at r*nb - r*(r-1)/2.
+__device__ __forceinline__ void tile_coords(unsigned t, unsigned nb, unsigned& bi, unsigned& bj) {
+  const double f = 2.0 * nb + 1.0;
+  long long r = (long long)((f - sqrt(f * f - 8.0 * (double)t)) * 0.5);
+  if (r < 0) r = 0;
+  if (r > (long long)nb - 1) r = nb - 1;
+  auto start = [nb](long long q) { return q * (long long)nb - q * (q - 1) / 2; };
+  while (r + 1 < (long long)nb && start(r + 1) <= (long long)t) ++r;
+  while (r > 0 && start(r) > (long long)t) --r;
+  bi = (unsigned)r;
+  bj = (unsigned)(r + ((long long)t - start(r)));
+}
+
+// Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
+// items so that tiles sharing a block row also share that XCD's L2.
+__device__ __forceinline__ unsigned xcd_item(unsigned bid, unsigned grid) {
+  const unsigned q = grid / 8, r = grid % 8, x = bid % 8, k = bid / 8;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+__device__ __forceinline__ unsigned find_job(const JobDesc* jobs, unsigned njobs, unsigned item) {
+  unsigned lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi + 1) / 2;
+    if (jobs[mid].item_base <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void push_sentinel(const JobDesc& J, unsigned i, unsigned j) {
+  const unsigned k = atomicAdd(J.sent_count, 1u);
+  if (k < J.sent_cap) J.sent[k] = ((unsigned long long)i << 32) | j;
+}
+
+// Chunk `chunk` of the i and j taxon blocks into registers: linear element lin = q + Q*(w + CW*p)
+// of a block is word row (chunk*CW + w) of plane p, uint4 column q.
+template <int EP, int NT, int Q, int CW, int E>
+__device__ __forceinline__ void fetch_chunk(v4u (&pI)[EP], v4u (&pJ)[EP], gptr<v4u> gI,
+                                            gptr<v4u> gJ, unsigned tid, unsigned chunk,
+                                            size_t row4, size_t plane4) {
+#pragma unroll
+  for (int e = 0; e < EP; ++e) {
+    const unsigned lin = tid + e * NT;
+    if (E % NT != 0 && lin >= (unsigned)E) break;
+    const unsigned q = lin % Q, w = (lin / Q) % CW, p = lin / (Q * CW);
+    const size_t off = p * plane4 + (size_t)(chunk * CW + w) * row4 + q;
+    pI[e] = gI[off];
+    pJ[e] = gJ[off];
+  }
+}
+
+// ---- the pair-tile kernel ------------------------------------------------------------------
+// One work-group computes a TILE x TILE block of taxon pairs (bi <= bj); thread (ty, tx) owns the
+// MI x MJ micro-tile i = bi*TILE + ty*MI + a, j = bj*TILE + tx*MJ + b. Sites are walked in chunks
+// of CW words: the chunk's H/L/V words of both taxon blocks are staged in LDS (layout
+// [plane][word][taxon], written linearly), the next chunk is prefetched into registers while
+// the current one is counted.
+template <int TILE, int MI, int MJ, int CW, bool JC6>
+__global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
+    const JobDesc* __restrict__ jobs, unsigned njobs) {
+  constexpr int NTX = TILE / MJ;
+  constexpr int NT = (TILE / MI) * NTX;
+  constexpr int Q = TILE / 4;                 // uint4 per plane-word row of one block
+  constexpr int E = 3 * CW * Q;               // uint4 per block and chunk
+  static_assert(MI % 4 == 0 && MJ % 4 == 0, "micro-tiles are read as uint4");
+  constexpr int EP = (E + NT - 1) / NT;       // uint4 per thread and block (last may be partial)
+
+  __shared__ v4u sI[E];
+  __shared__ v4u sJ[E];
+
+  const unsigned item = xcd_item(blockIdx.x, gridDim.x);
+  const unsigned k = find_job(jobs, njobs, item);
+  const JobDesc& J = jobs[k];
+  unsigned bi, bj;
+  tile_coords(item - J.item_base, J.nblk, bi, bj);
+
+  const unsigned tid = threadIdx.x;
+  const unsigned ty = tid / NTX, tx = tid % NTX;
+  const gptr<v4u> gI = (gptr<v4u>)(J.planes + (size_t)bi * TILE);
+  const gptr<v4u> gJ = (gptr<v4u>)(J.planes + (size_t)bj * TILE);
+  const size_t row4 = J.n_pad / 4;                       // uint4 per word row
+  const size_t plane4 = J.plane_stride / 4;
+
+  uint32_t acc_a[MI][MJ], acc_b[MI][MJ], acc_v[MI][MJ];  // a: tv (K2P) / mismatches (JC6); b: ts
+#pragma unroll
+  for (int a = 0; a < MI; ++a)
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) acc_a[a][b] = acc_b[a][b] = acc_v[a][b] = 0;
+
+  v4u pI[EP], pJ[EP];
+  const unsigned nchunks = J.W / CW;
+  if (nchunks > 0) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, 0, row4, plane4);
+  for (unsigned c = 0; c < nchunks; ++c) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EP; ++e) {
+      if (E % NT != 0 && tid + e * NT >= (unsigned)E) break;
+      sI[tid + e * NT] = pI[e];
+      sJ[tid + e * NT] = pJ[e];
+    }
+    __syncthreads();
+    if (c + 1 < nchunks) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
+
+#pragma unroll 4
+    for (int w = 0; w < CW; ++w) {
+      uint32_t hi[MI], li[MI], vi[MI], hj[MJ], lj[MJ], vj[MJ];
+#pragma unroll
+      for (int r = 0; r < MI / 4; ++r) {
+        const v4u h = sI[(0 * CW + w) * Q + ty * (MI / 4) + r];
+        const v4u l = sI[(1 * CW + w) * Q + ty * (MI / 4) + r];
+        const v4u v = sI[(2 * CW + w) * Q + ty * (MI / 4) + r];
+        hi[4 * r] = h.x; hi[4 * r + 1] = h.y; hi[4 * r + 2] = h.z; hi[4 * r + 3] = h.w;
+        li[4 * r] = l.x; li[4 * r + 1] = l.y; li[4 * r + 2] = l.z; li[4 * r + 3] = l.w;
+        vi[4 * r] = v.x; vi[4 * r + 1] = v.y; vi[4 * r + 2] = v.z; vi[4 * r + 3] = v.w;
+      }
+#pragma unroll
+      for (int r = 0; r < MJ / 4; ++r) {
+        const v4u h = sJ[(0 * CW + w) * Q + tx * (MJ / 4) + r];
+        const v4u l = sJ[(1 * CW + w) * Q + tx * (MJ / 4) + r];
+        const v4u v = sJ[(2 * CW + w) * Q + tx * (MJ / 4) + r];
+        hj[4 * r] = h.x; hj[4 * r + 1] = h.y; hj[4 * r + 2] = h.z; hj[4 * r + 3] = h.w;
+        lj[4 * r] = l.x; lj[4 * r + 1] = l.y; lj[4 * r + 2] = l.z; lj[4 * r + 3] = l.w;
+        vj[4 * r] = v.x; vj[4 * r + 1] = v.y; vj[4 * r + 2] = v.z; vj[4 * r + 3] = v.w;
+      }
+#pragma unroll
+      for (int a = 0; a < MI; ++a)
+#pragma unroll
+        for (int b = 0; b < MJ; ++b) {
+          const uint32_t g = vi[a] & vj[b];
+          const uint32_t t = xor_and(hi[a], hj[b], g);
+          acc_v[a][b] = bcnt_add(g, acc_v[a][b]);
+          if constexpr (JC6) {
+            const uint32_t m = t | xor_and(li[a], lj[b], g);
+            acc_a[a][b] = bcnt_add(m, acc_a[a][b]);
+          } else {
+            const uint32_t s = xor_and(li[a], lj[b], g ^ t);
+            acc_a[a][b] = bcnt_add(t, acc_a[a][b]);
+            acc_b[a][b] = bcnt_add(s, acc_b[a][b]);
+          }
+        }
+    }
+  }
+
+  // ---- epilogue: correction, output, running max, saturated pairs -------------------------
+  const unsigned n = J.n;
+  const int model = J.model;
+  float lmax = 0.0f;
+#pragma unroll
+  for (int a = 0; a < MI; ++a) {
+    const unsigned i = bi * TILE + ty * MI + a;
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) {
+      const unsigned j = bj * TILE + tx * MJ + b;
+      if (j >= n) continue;
+      if (i < j) {
+        if (model == kCounts) {
+          const unsigned long long p = tri_index(i, j, n);
+          J.out_tv[p] = acc_a[a][b];
+          J.out_ts[p] = acc_b[a][b];
+          J.out_valid[p] = acc_v[a][b];
+          continue;
+        }
+        float d;
+        if (model == kJC)
+          d = jc_distance(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b], acc_v[a][b]);
+        else
+          d = k2p_distance(acc_b[a][b], acc_a[a][b], acc_v[a][b]);
+        if (d == -1.0f) push_sentinel(J, i, j);
+        else if (d > lmax) lmax = d;
+        if (J.layout == kSquare) {
+          J.out[(size_t)i * J.ld + j] = d;
+          J.out[(size_t)j * J.ld + i] = d;
+        } else {
+          J.out[tri_index(i, j, n)] = d;
+        }
+      } else if (i == j && model != kCounts && J.layout == kSquare) {
+        J.out[(size_t)i * J.ld + i] = 0.0f;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
+  if ((tid & 63) == 0 && lmax > 0.0f) atomicMax(J.maxbits, __float_as_uint(lmax));
+}
+
+template <int TILE, int MI, int MJ, int CW>
+hipError_t launch_cfg(bool jc6, const JobDesc* jobs, unsigned njobs, unsigned total,
+                      hipStream_t s) {
+  constexpr int NT = (TILE / MI) * (TILE / MJ);
+  if (jc6)
+    tile_kernel<TILE, MI, MJ, CW, true><<<dim3(total), dim3(NT), 0, s>>>(jobs, njobs);
+  else
+    tile_kernel<TILE, MI, MJ, CW, false><<<dim3(total), dim3(NT), 0, s>>>(jobs, njobs);
+  return hipGetLastError();
+}
+
+// ---- packing ---------------------------------------------------------------------------------
+// Bits of one character: H (pyrimidine), L (G or C), V (counted base). Only ACGT of either case
+// are counted (dataloaderFasta.hpp:147-182 sets the gap-filter slot for exactly these); c | 0x20
+// folds case and maps no other byte onto a, c, g or t.
+__device__ __forceinline__ void char_bits(uint8_t c, uint32_t& h, uint32_t& l, uint32_t& v) {
+  const uint8_t lc = c | 0x20;
+  v = (lc == 'a') | (lc == 'c') | (lc == 'g') | (lc == 't');
+  h = (lc == 'c') | (lc == 't');
+  l = (lc == 'g') | (lc == 'c');
+}
+
+__global__ void pack_chars_kernel(const uint8_t* __restrict__ chars, size_t pitch, unsigned n,
+                                  unsigned L, unsigned n_pad, unsigned W_pad,
+                                  uint32_t* __restrict__ planes) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_pad) return;
+  for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
+  uint32_t H = 0, Lo = 0, V = 0;
+  if (t < n) {
+    const uint8_t* row = chars + (size_t)t * pitch;
+    for (unsigned k = 0; k < 32; ++k) {
+      const unsigned s = w * 32 + k;
+      if (s >= L) break;
+      uint32_t h, l, v;
+      char_bits(row[s], h, l, v);
+      H |= h << k;
+      Lo |= l << k;
+      V |= v << k;
+    }
+  }
+  const size_t ps = (size_t)W_pad * n_pad;
+  planes[0 * ps + (size_t)w * n_pad + t] = H;
+  planes[1 * ps + (size_t)w * n_pad + t] = Lo;
+  planes[2 * ps + (size_t)w * n_pad + t] = V;
+  }
+}
+
+// Bits 0, 2, 4, .., 30 of x gathered into bits 0..15.
+__device__ __forceinline__ uint32_t even_bits(uint32_t x) {
+  x &= 0x55555555u;
+  x = (x | (x >> 1)) & 0x33333333u;
+  x = (x | (x >> 2)) & 0x0f0f0f0fu;
+  x = (x | (x >> 4)) & 0x00ff00ffu;
+  x = (x | (x >> 8)) & 0x0000ffffu;
+  return x;
+}
+
+// From the reference's packed layout: per taxon `stride` u32 words of 16 sites, site s at bits
+// 2*(s%16) of word s/16 holding the code A=00 G=01 T=10 C=11, and a gap filter with 01 in the
+// slot of every counted site (dataloaderFasta.hpp:219-251, dataloader.hpp:38-70).
+__global__ void pack_legacy_kernel(const uint32_t* __restrict__ bits,
+                                   const uint32_t* __restrict__ gaps, size_t stride,
+                                   unsigned words16, unsigned n, unsigned n_pad, unsigned W_pad,
+                                   uint32_t* __restrict__ planes) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_pad) return;
+  for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
+  uint32_t b0 = 0, b1 = 0, g0 = 0, g1 = 0;
+  if (t < n) {
+    const size_t base = (size_t)t * stride;
+    if (2 * w < words16) { b0 = bits[base + 2 * w]; g0 = gaps[base + 2 * w]; }
+    if (2 * w + 1 < words16) { b1 = bits[base + 2 * w + 1]; g1 = gaps[base + 2 * w + 1]; }
+  }
+  const size_t ps = (size_t)W_pad * n_pad;
+  planes[0 * ps + (size_t)w * n_pad + t] = even_bits(b0 >> 1) | (even_bits(b1 >> 1) << 16);
+  planes[1 * ps + (size_t)w * n_pad + t] = even_bits(b0) | (even_bits(b1) << 16);
+  planes[2 * ps + (size_t)w * n_pad + t] = even_bits(g0) | (even_bits(g1) << 16);
+  }
+}
+
+__global__ void reset_stats_kernel(const JobDesc* __restrict__ jobs, unsigned njobs) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= njobs) return;
+  *jobs[k].maxbits = 0u;
+  *jobs[k].sent_count = 0u;
+}
+
+// The reference post-process (KimuraDistance.cpp:366-395, JCdistance.cpp:201-230) replaces every
+// -1 by 2 * maxDistance. Saturated pairs were listed by the tile kernel; if the list overflowed
+// its capacity, the job's whole output is scanned instead.
+__global__ void patch_kernel(const JobDesc* __restrict__ jobs) {
+  const JobDesc& J = jobs[blockIdx.y];
+  if (J.model == kCounts) return;
+  const unsigned cnt = *J.sent_count;
+  if (cnt == 0) return;
+  const float val = __uint_as_float(*J.maxbits) * 2.0f;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t start = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cnt <= J.sent_cap) {
+    for (size_t k = start; k < cnt; k += stride) {
+      const unsigned long long p = J.sent[k];
+      const unsigned i = (unsigned)(p >> 32), j = (unsigned)(p & 0xffffffffu);
+      if (J.layout == kSquare) {
+        J.out[(size_t)i * J.ld + j] = val;
+        J.out[(size_t)j * J.ld + i] = val;
+      } else {
+        J.out[tri_index(i, j, J.n)] = val;
+      }
+    }
+    return;
+  }
+  uint32_t* o = reinterpret_cast<uint32_t*>(J.out);
+  if (J.layout == kSquare) {
+    const size_t total = (size_t)J.n * J.n;
+    for (size_t k = start; k < total; k += stride) {
+      const size_t p = (k / J.n) * J.ld + (k % J.n);
+      if (o[p] == kSentinelBits) J.out[p] = val;
+    }
+  } else {
+    const size_t total = (size_t)J.n * (J.n - 1) / 2;
+    for (size_t k = start; k < total; k += stride)
+      if (o[k] == kSentinelBits) J.out[k] = val;
+  }
+}
+
+// codes[s][t] = V << 2 | H << 1 | L for site s < L of taxon t.
+__global__ void planes_to_codes_kernel(const uint32_t* __restrict__ planes, unsigned n_pad,
+                                       unsigned W_pad, unsigned L, uint8_t* __restrict__ codes) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_pad) return;
+  const size_t ps = (size_t)W_pad * n_pad;
+  for (unsigned s = blockIdx.y; s < L; s += gridDim.y) {
+    const size_t o = (size_t)(s / 32) * n_pad + t;
+    const unsigned sh = s % 32;
+    const uint32_t h = (planes[o] >> sh) & 1u;
+    const uint32_t l = (planes[ps + o] >> sh) & 1u;
+    const uint32_t v = (planes[2 * ps + o] >> sh) & 1u;
+    codes[(size_t)s * n_pad + t] = (uint8_t)(v << 2 | h << 1 | l);
+  }
+}
+
+// One work-group per replicate: exclusive prefix sum of the column weights, then column c is
+// written weights[c] times, in column order (the expanded alignment's column list).
+__global__ __launch_bounds__(1024) void bootstrap_index_kernel(
+    const uint16_t* __restrict__ weights, unsigned L, unsigned* __restrict__ col_index,
+    unsigned idx_stride, unsigned* __restrict__ lengths) {
+  __shared__ unsigned wave_sum[16];
+  __shared__ unsigned carry_s;
+  const unsigned r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint16_t* wr = weights + (size_t)r * L;
+  unsigned* out = col_index + (size_t)r * idx_stride;
+  if (tid == 0) carry_s = 0;
+  __syncthreads();
+  for (unsigned base = 0; base < L; base += 1024) {
+    const unsigned c = base + tid;
+    const unsigned w = c < L ? wr[c] : 0u;
+    unsigned incl = w;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned y = __shfl_up(incl, off);
+      if (lane >= (unsigned)off) incl += y;
+    }
+    if (lane == 63) wave_sum[wid] = incl;
+    __syncthreads();
+    unsigned before = carry_s;
+    for (unsigned q = 0; q < wid; ++q) before += wave_sum[q];
+    const unsigned excl = before + incl - w;
+    for (unsigned m = 0; m < w; ++m)
+      if (excl + m < idx_stride) out[excl + m] = c;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned tot = 0;
+      for (unsigned q = 0; q < 16; ++q) tot += wave_sum[q];
+      carry_s += tot;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) lengths[r] = carry_s;
+}
+
+// Replicate r's packed planes from the expanded column list: target site s takes source column
+// col_index[r][s]. Sites at or beyond lengths[r] are zero (not counted).
+__global__ void bootstrap_gather_kernel(const uint8_t* __restrict__ codes, unsigned n_pad,
+                                        const unsigned* __restrict__ col_index,
+                                        unsigned idx_stride, const unsigned* __restrict__ lengths,
+                                        unsigned W_pad, uint32_t* __restrict__ planes) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned r = blockIdx.z;
+  if (t >= n_pad) return;
+  const unsigned len = min(lengths[r], idx_stride);
+  const unsigned* idx = col_index + (size_t)r * idx_stride;
+  for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
+  uint32_t H = 0, Lo = 0, V = 0;
+  for (unsigned k = 0; k < 32; ++k) {
+    const unsigned s = w * 32 + k;
+    if (s >= len) break;
+    const uint32_t c = codes[(size_t)idx[s] * n_pad + t];
+    Lo |= (c & 1u) << k;
+    H |= ((c >> 1) & 1u) << k;
+    V |= ((c >> 2) & 1u) << k;
+  }
+  const size_t ps = (size_t)W_pad * n_pad;
+  uint32_t* pr = planes + (size_t)r * 3 * ps;
+  pr[0 * ps + (size_t)w * n_pad + t] = H;
+  pr[1 * ps + (size_t)w * n_pad + t] = Lo;
+  pr[2 * ps + (size_t)w * n_pad + t] = V;
+  }
+}
+
+}  // namespace
+
+// ---- launchers ---------------------------------------------------------------------------------
+// grid.y is limited to 65535; the packing kernels loop over the rest.
+static unsigned ygrid(unsigned rows) { return rows < 1 ? 1 : (rows > 65535 ? 65535 : rows); }
+
+hipError_t launch_pack_chars(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                             unsigned n_pad, unsigned W_pad, uint32_t* planes,
+                             hipStream_t stream) {
+  pack_chars_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(W_pad)), dim3(256), 0, stream>>>(
+      chars, pitch, n, L, n_pad, W_pad, planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_legacy(const uint32_t* bits, const uint32_t* gaps, size_t stride,
+                              unsigned words16, unsigned n, unsigned n_pad, unsigned W_pad,
+                              uint32_t* planes, hipStream_t stream) {
+  pack_legacy_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(W_pad)), dim3(256), 0, stream>>>(
+      bits, gaps, stride, words16, n, n_pad, W_pad, planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t stream) {
+  reset_stats_kernel<<<dim3((njobs + 255) / 256), dim3(256), 0, stream>>>(jobs, njobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_tiles(int config, bool jc_only, const JobDesc* jobs, unsigned njobs,
+                        unsigned total_items, hipStream_t stream) {
+  if (total_items == 0) return hipSuccess;
+  switch (config) {
+    case 0: return launch_cfg<64, 4, 4, 16>(jc_only, jobs, njobs, total_items, stream);
+    case 1: return launch_cfg<64, 4, 4, 8>(jc_only, jobs, njobs, total_items, stream);
+    case 2: return launch_cfg<128, 8, 8, 16>(jc_only, jobs, njobs, total_items, stream);
+    case 3: return launch_cfg<128, 8, 4, 16>(jc_only, jobs, njobs, total_items, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream) {
+  if (njobs == 0) return hipSuccess;
+  patch_kernel<<<dim3(64, njobs), dim3(256), 0, stream>>>(jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
+                                  unsigned L, uint8_t* codes, hipStream_t stream) {
+  planes_to_codes_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(L)), dim3(256), 0, stream>>>(
+      planes, n_pad, W_pad, L, codes);
+  return hipGetLastError();
+}
+
+hipError_t launch_bootstrap_index(const uint16_t* weights, unsigned R, unsigned L,
+                                  unsigned* col_index, unsigned idx_stride, unsigned* lengths,
+                                  hipStream_t stream) {
+  bootstrap_index_kernel<<<dim3(R), dim3(1024), 0, stream>>>(weights, L, col_index, idx_stride,
+                                                             lengths);
+  return hipGetLastError();
+}
+
+hipError_t launch_bootstrap_gather(const uint8_t* codes, unsigned n_pad, const unsigned* col_index,
+                                   unsigned idx_stride, const unsigned* lengths, unsigned R,
+                                   unsigned W_pad, uint32_t* planes, hipStream_t stream) {
+  bootstrap_gather_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(W_pad), R), dim3(256), 0,
+                            stream>>>(codes, n_pad, col_index, idx_stride, lengths, W_pad, planes);
+  return hipGetLastError();
+}
+
+}  // namespace rnj

@@ -1,0 +1,874 @@
+// Host runtime of librapidnj_amd.so: contexts, alignments resident in HBM, multi-job launches,
+// the handle-based C ABI and the reference's own device entry points.
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace rnj {
+
+// ---- errors ---------------------------------------------------------------------------------
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+int fail(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? RNJ_E_NOMEM : RNJ_E_HIP;
+}
+
+// ---- sizes ----------------------------------------------------------------------------------
+unsigned round_up(unsigned x, unsigned m) { return (x + m - 1) / m * m; }
+
+unsigned words_for_sites(unsigned sites) {
+  const unsigned w = (sites + 31) / 32;
+  return round_up(w == 0 ? 1 : w, kWordPad);
+}
+
+size_t sentinel_capacity(unsigned n) {
+  const size_t pairs = (size_t)n * (n - 1) / 2;
+  return std::min<size_t>(std::max<size_t>(pairs / 64, 1024), (size_t)1 << 24);
+}
+
+int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out) {
+  auto* a = new rnj_alignment();
+  a->ctx = ctx;
+  a->n = n;
+  a->L = L;
+  a->n_pad = round_up(n, kTaxaPad);
+  a->W = words_for_sites(sites);
+  hipError_t e = hipMalloc((void**)&a->planes, a->bytes());
+  if (e != hipSuccess) {
+    delete a;
+    return hip_fail(e, "hipMalloc(alignment planes)");
+  }
+  *out = a;
+  return RNJ_OK;
+}
+
+// ---- workspaces -------------------------------------------------------------------------------
+namespace {
+
+void free_workspace(Workspace* ws) {
+  if (ws->h_jobs) (void)hipHostFree(ws->h_jobs);
+  if (ws->d_jobs) (void)hipFree(ws->d_jobs);
+  if (ws->d_stats) (void)hipFree(ws->d_stats);
+  if (ws->d_sent) (void)hipFree(ws->d_sent);
+  ws->h_jobs = nullptr;
+  ws->d_jobs = nullptr;
+  ws->d_stats = nullptr;
+  ws->d_sent = nullptr;
+  ws->cap_jobs = 0;
+  ws->cap_sent = 0;
+}
+
+int grow_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
+  if (njobs > ws->cap_jobs) {
+    const unsigned cap = std::max(njobs, 2 * ws->cap_jobs);
+    if (ws->h_jobs) (void)hipHostFree(ws->h_jobs);
+    if (ws->d_jobs) (void)hipFree(ws->d_jobs);
+    if (ws->d_stats) (void)hipFree(ws->d_stats);
+    ws->h_jobs = nullptr;
+    ws->d_jobs = nullptr;
+    ws->d_stats = nullptr;
+    ws->cap_jobs = 0;
+    RNJ_HIP(hipHostMalloc((void**)&ws->h_jobs, cap * sizeof(JobDesc), hipHostMallocDefault));
+    RNJ_HIP(hipMalloc((void**)&ws->d_jobs, cap * sizeof(JobDesc)));
+    RNJ_HIP(hipMalloc((void**)&ws->d_stats, cap * 2 * sizeof(unsigned)));
+    ws->cap_jobs = cap;
+  }
+  if (nsent > ws->cap_sent) {
+    const size_t cap = std::max(nsent, 2 * ws->cap_sent);
+    if (ws->d_sent) (void)hipFree(ws->d_sent);
+    ws->d_sent = nullptr;
+    ws->cap_sent = 0;
+    RNJ_HIP(hipMalloc((void**)&ws->d_sent, cap * sizeof(unsigned long long)));
+    ws->cap_sent = cap;
+  }
+  return RNJ_OK;
+}
+
+int acquire_workspace(Context* ctx, unsigned njobs, size_t nsent, Workspace** out) {
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  Workspace* ws = nullptr;
+  for (auto& w : ctx->pool) {
+    if (w->in_use) continue;
+    if (w->recorded && hipEventQuery(w->done) != hipSuccess) continue;
+    ws = w.get();
+    break;
+  }
+  if (!ws) {
+    ctx->pool.emplace_back(new Workspace());
+    ws = ctx->pool.back().get();
+    RNJ_HIP(hipEventCreateWithFlags(&ws->done, hipEventDisableTiming));
+  }
+  int rc = grow_workspace(ws, njobs, nsent);
+  if (rc != RNJ_OK) return rc;
+  ws->in_use = true;
+  *out = ws;
+  return RNJ_OK;
+}
+
+void release_workspace(Context* ctx, Workspace* ws, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (hipEventRecord(ws->done, stream) == hipSuccess) ws->recorded = true;
+  ws->in_use = false;
+}
+
+int ilog2(unsigned long long x) {
+  int r = 0;
+  while (x > 1) { x >>= 1; ++r; }
+  return r;
+}
+
+// Fill item numbering for tile edge `tile` and bind scratch. Returns total work items.
+unsigned long long number_jobs(std::vector<JobSpec>& specs, unsigned tile, Workspace* ws,
+                               size_t* sent_offsets) {
+  unsigned long long base = 0;
+  size_t soff = 0;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    const JobSpec& s = specs[k];
+    JobDesc& d = ws->h_jobs[k];
+    std::memset(&d, 0, sizeof d);
+    d.planes = s.planes;
+    d.plane_stride = (unsigned long long)s.W_pad * s.n_pad;
+    d.n = s.n;
+    d.n_pad = s.n_pad;
+    d.W = s.W;
+    d.nblk = (s.n + tile - 1) / tile;
+    d.tiles = d.nblk * (d.nblk + 1) / 2;
+    d.item_base = (unsigned)base;
+    d.model = s.model;
+    d.layout = s.layout;
+    d.ld = s.ld;
+    d.out = s.out;
+    d.out_ts = s.out_ts;
+    d.out_tv = s.out_tv;
+    d.out_valid = s.out_valid;
+    d.maxbits = ws->d_stats + 2 * k;
+    d.sent_count = ws->d_stats + 2 * k + 1;
+    d.sent = ws->d_sent + soff;
+    d.sent_cap = (unsigned)sent_offsets[k];
+    soff += sent_offsets[k];
+    base += d.tiles;
+  }
+  return base;
+}
+
+}  // namespace
+
+void free_workspace_public(Workspace* ws) { free_workspace(ws); }
+
+// Default configuration before (or without) tuning: the 64-edge tile with 4x4 micro-tiles.
+static int default_config(unsigned n, unsigned W, bool jc_only) {
+  (void)n; (void)W; (void)jc_only;
+  return 0;
+}
+
+int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
+  if (ctx->tile_config >= 0) return ctx->tile_config;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->tuned.find(std::make_tuple(ilog2(n), ilog2(W), (int)jc_only));
+  if (it != ctx->tuned.end()) return it->second;
+  return default_config(n, W, jc_only);
+}
+
+// Time every configuration on this launch (stream-synchronous) and remember the fastest for the
+// launch's shape class. Only for launches big enough that the timing means something and only
+// outside stream capture.
+static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, size_t* sent_caps,
+                    bool jc_only, unsigned n_key, unsigned W_key, hipStream_t stream, int* best) {
+  hipEvent_t e0, e1;
+  RNJ_HIP(hipEventCreate(&e0));
+  RNJ_HIP(hipEventCreate(&e1));
+  float best_ms = 1e30f;
+  *best = default_config(n_key, W_key, jc_only);
+  for (int c = 0; c < kNumTileConfigs; ++c) {
+    const unsigned long long items = number_jobs(specs, kTileConfigs[c].tile, ws, sent_caps);
+    if (items >= (1ull << 31)) continue;
+    RNJ_HIP(hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
+                           hipMemcpyHostToDevice, stream));
+    float ms = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+      RNJ_HIP(launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream));
+      RNJ_HIP(hipEventRecord(e0, stream));
+      RNJ_HIP(launch_tiles(c, jc_only, ws->d_jobs, (unsigned)specs.size(), (unsigned)items, stream));
+      RNJ_HIP(hipEventRecord(e1, stream));
+      RNJ_HIP(hipEventSynchronize(e1));
+      RNJ_HIP(hipEventElapsedTime(&ms, e0, e1));
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->tuned[std::make_tuple(ilog2(n_key), ilog2(W_key), (int)jc_only)] = *best;
+  return RNJ_OK;
+}
+
+int launch_jobs(Context* ctx, std::vector<JobSpec>& specs, bool raw, hipStream_t stream) {
+  if (specs.empty()) return RNJ_OK;
+  bool jc_only = true;
+  unsigned n_key = 0, W_key = 0;
+  unsigned long long work = 0;
+  std::vector<size_t> caps(specs.size());
+  size_t nsent = 0;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    const JobSpec& s = specs[k];
+    if (s.model != kJC) jc_only = false;
+    n_key = std::max(n_key, s.n);
+    W_key = std::max(W_key, s.W);
+    work += (unsigned long long)s.n * s.n * s.W;
+    caps[k] = s.model == kCounts ? 0 : sentinel_capacity(s.n);
+    nsent += caps[k];
+  }
+  Workspace* ws = nullptr;
+  int rc = acquire_workspace(ctx, (unsigned)specs.size(), std::max<size_t>(nsent, 1), &ws);
+  if (rc != RNJ_OK) return rc;
+
+  int config = ctx->tile_config;
+  if (config < 0) {
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lock(ctx->mu);
+      auto it = ctx->tuned.find(std::make_tuple(ilog2(n_key), ilog2(W_key), (int)jc_only));
+      if (it != ctx->tuned.end()) { config = it->second; cached = true; }
+    }
+    if (!cached) {
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      (void)hipStreamIsCapturing(stream, &cap);
+      // tune only launches worth >= ~2^33 word-pairs (a few ms): small ones keep the default
+      if (cap == hipStreamCaptureStatusNone && work >= (1ull << 34)) {
+        rc = autotune(ctx, specs, ws, caps.data(), jc_only, n_key, W_key, stream, &config);
+        if (rc != RNJ_OK) { release_workspace(ctx, ws, stream); return rc; }
+      } else {
+        config = default_config(n_key, W_key, jc_only);
+      }
+    }
+  }
+  const unsigned long long items = number_jobs(specs, kTileConfigs[config].tile, ws, caps.data());
+  if (items >= (1ull << 31)) {
+    release_workspace(ctx, ws, stream);
+    return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
+  }
+  hipError_t e = hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
+                                hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream);
+  if (e == hipSuccess)
+    e = launch_tiles(config, jc_only, ws->d_jobs, (unsigned)specs.size(), (unsigned)items, stream);
+  if (e == hipSuccess && !raw) e = launch_patch(ws->d_jobs, (unsigned)specs.size(), stream);
+  for (size_t k = 0; e == hipSuccess && k < specs.size(); ++k)
+    if (specs[k].max_out)
+      e = hipMemcpyAsync(specs[k].max_out, ws->d_stats + 2 * k, sizeof(float),
+                         hipMemcpyDeviceToDevice, stream);
+  release_workspace(ctx, ws, stream);
+  if (e != hipSuccess) return hip_fail(e, "launch_jobs");
+  return RNJ_OK;
+}
+
+}  // namespace rnj
+
+using namespace rnj;
+
+// ---- helpers for the C ABI ---------------------------------------------------------------------
+namespace {
+
+hipStream_t stream_of(rnj_context* ctx, void* stream) {
+  return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+int set_device(rnj_context* ctx) {
+  RNJ_HIP(hipSetDevice(ctx->device));
+  return RNJ_OK;
+}
+
+// dataloaderFasta.hpp:54-76 on the first sequence: bytes below 'A' (signed chars, so also bytes
+// >= 128) and ACGTUN of either case classify as DNA, anything else as protein.
+bool first_row_is_dna(const char* row, unsigned L) {
+  for (unsigned k = 0; k < L; ++k) {
+    const signed char c = (signed char)row[k];
+    if (c < 65) continue;
+    switch (c) {
+      case 'A': case 'a': case 'C': case 'c': case 'G': case 'g': case 'T': case 't':
+      case 'U': case 'u': case 'N': case 'n':
+        continue;
+      default:
+        return false;
+    }
+  }
+  return true;
+}
+
+int check_model(int model) {
+  if (model != RNJ_JC && model != RNJ_K2P) return fail(RNJ_E_INVALID, "model must be RNJ_JC or RNJ_K2P");
+  return RNJ_OK;
+}
+
+int check_aln(rnj_context* ctx, const rnj_alignment* aln) {
+  if (!ctx) return fail(RNJ_E_INVALID, "null context");
+  if (!aln || !aln->planes) return fail(RNJ_E_INVALID, "null alignment");
+  if (aln->ctx != ctx) return fail(RNJ_E_INVALID, "alignment belongs to another context");
+  return RNJ_OK;
+}
+
+JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size_t ld) {
+  JobSpec s{};
+  s.planes = a->planes;
+  s.n = a->n;
+  s.n_pad = a->n_pad;
+  s.W_pad = a->W;
+  s.W = a->W;
+  s.model = model;
+  s.layout = layout;
+  s.ld = ld;
+  s.out = out;
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rnj_last_error(void) { return last_error_cstr(); }
+
+const char* rnj_status_string(int status) {
+  switch (status) {
+    case RNJ_OK: return "RNJ_OK";
+    case RNJ_E_INVALID: return "RNJ_E_INVALID";
+    case RNJ_E_HIP: return "RNJ_E_HIP";
+    case RNJ_E_NOMEM: return "RNJ_E_NOMEM";
+    case RNJ_E_NOT_DNA: return "RNJ_E_NOT_DNA";
+    case RNJ_E_ADMISSION: return "RNJ_E_ADMISSION";
+    case RNJ_E_CANCELLED: return "RNJ_E_CANCELLED";
+    case RNJ_E_NOT_READY: return "RNJ_E_NOT_READY";
+    case RNJ_E_NO_DEVICE: return "RNJ_E_NO_DEVICE";
+    default: return "unknown status";
+  }
+}
+
+int rnj_api_version(void) { return RNJ_API_VERSION; }
+
+int rnj_create(int device, rnj_context** out) {
+  if (!out) return fail(RNJ_E_INVALID, "null output pointer");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(RNJ_E_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= count) return fail(RNJ_E_INVALID, "device index out of range");
+  auto* ctx = new rnj_context();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&ctx->prop, device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    // keep freed stream-ordered allocations cached in the pool instead of returning them
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+      uint64_t thr = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+  }
+  if (e != hipSuccess) {
+    delete ctx;
+    return hip_fail(e, "rnj_create");
+  }
+  *out = ctx;
+  return RNJ_OK;
+}
+
+int rnj_synchronize(rnj_context* ctx) {
+  if (!ctx) return fail(RNJ_E_INVALID, "null context");
+  RNJ_HIP(hipSetDevice(ctx->device));
+  RNJ_HIP(hipDeviceSynchronize());
+  return RNJ_OK;
+}
+
+int rnj_destroy(rnj_context* ctx) {
+  if (!ctx) return RNJ_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (auto& w : ctx->pool) {
+    free_workspace_public(w.get());
+    if (w->done) (void)hipEventDestroy(w->done);
+  }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RNJ_OK;
+}
+
+int rnj_tile_config_count(void) { return kNumTileConfigs; }
+
+const char* rnj_tile_config_name(int config) {
+  if (config < 0 || config >= kNumTileConfigs) return nullptr;
+  return kTileConfigs[config].name;
+}
+
+int rnj_set_tile_config(rnj_context* ctx, int config) {
+  if (!ctx) return fail(RNJ_E_INVALID, "null context");
+  if (config < -1 || config >= kNumTileConfigs) return fail(RNJ_E_INVALID, "no such tile configuration");
+  ctx->tile_config = config;
+  return RNJ_OK;
+}
+
+int rnj_get_tile_config(rnj_context* ctx, unsigned n, unsigned L, int model, int* config) {
+  if (!ctx || !config) return fail(RNJ_E_INVALID, "null argument");
+  *config = choose_config(ctx, n, words_for_sites(L), model == RNJ_JC);
+  return RNJ_OK;
+}
+
+// ---- alignments -----------------------------------------------------------------------------
+int rnj_alignment_from_chars(rnj_context* ctx, const char* chars, size_t pitch, unsigned n,
+                             unsigned L, int chars_on_device, int skip_type_check,
+                             rnj_alignment** out) {
+  if (!ctx || !chars || !out) return fail(RNJ_E_INVALID, "null argument");
+  if (n < 2) return fail(RNJ_E_INVALID, "need at least 2 sequences");
+  if (L < 1) return fail(RNJ_E_INVALID, "empty sequences");
+  if (pitch < L) return fail(RNJ_E_INVALID, "pitch smaller than sequence length");
+  int rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  if (!skip_type_check) {
+    std::vector<char> first(L);
+    if (chars_on_device) RNJ_HIP(hipMemcpy(first.data(), chars, L, hipMemcpyDeviceToHost));
+    else std::memcpy(first.data(), chars, L);
+    if (!first_row_is_dna(first.data(), L))
+      return fail(RNJ_E_NOT_DNA, "first sequence is not DNA; only DNA distances are supported");
+  }
+  Alignment* a = nullptr;
+  rc = alloc_alignment(ctx, n, L, L, &a);
+  if (rc != RNJ_OK) return rc;
+  const uint8_t* dchars = (const uint8_t*)chars;
+  void* staging = nullptr;
+  hipError_t e = hipSuccess;
+  if (!chars_on_device) {
+    e = hipMalloc(&staging, (size_t)n * pitch);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(staging, chars, (size_t)n * pitch, hipMemcpyHostToDevice, ctx->stream);
+    dchars = (const uint8_t*)staging;
+  }
+  if (e == hipSuccess)
+    e = launch_pack_chars(dchars, pitch, n, L, a->n_pad, a->W, a->planes, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (staging) (void)hipFree(staging);
+  if (e != hipSuccess) {
+    (void)hipFree(a->planes);
+    delete a;
+    return hip_fail(e, "rnj_alignment_from_chars");
+  }
+  *out = static_cast<rnj_alignment*>(a);
+  return RNJ_OK;
+}
+
+int rnj_alignment_from_rows(rnj_context* ctx, const char* const* rows, unsigned n, unsigned L,
+                            rnj_alignment** out) {
+  if (!ctx || !rows || !out) return fail(RNJ_E_INVALID, "null argument");
+  if (n < 2) return fail(RNJ_E_INVALID, "need at least 2 sequences");
+  std::vector<char> buf((size_t)n * L);
+  for (unsigned t = 0; t < n; ++t) {
+    if (!rows[t]) return fail(RNJ_E_INVALID, "null sequence row");
+    std::memcpy(buf.data() + (size_t)t * L, rows[t], L);
+  }
+  return rnj_alignment_from_chars(ctx, buf.data(), L, n, L, 0, 0, out);
+}
+
+int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned int* const* bitStrings,
+                                  const unsigned int* const* gapFilters, unsigned n,
+                                  unsigned bitStringsCount, rnj_alignment** out) {
+  if (!ctx || !bitStrings || !gapFilters || !out) return fail(RNJ_E_INVALID, "null argument");
+  if (n < 2) return fail(RNJ_E_INVALID, "need at least 2 sequences");
+  const unsigned blocks = bitStringsCount / 6;     // bitDistanceGap.cpp:19
+  const unsigned words16 = blocks * 6 * 4;         // 16-site u32 words counted per taxon
+  if (words16 == 0) return fail(RNJ_E_INVALID, "bitStringsCount below 6 counts no site");
+  int rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  const size_t stride = words16;
+  std::vector<uint32_t> hb((size_t)n * stride), hg((size_t)n * stride);
+  for (unsigned t = 0; t < n; ++t) {
+    if (!bitStrings[t] || !gapFilters[t]) return fail(RNJ_E_INVALID, "null bit string row");
+    std::memcpy(&hb[(size_t)t * stride], bitStrings[t], stride * 4);
+    std::memcpy(&hg[(size_t)t * stride], gapFilters[t], stride * 4);
+  }
+  Alignment* a = nullptr;
+  rc = alloc_alignment(ctx, n, words16 * 16, words16 * 16, &a);
+  if (rc != RNJ_OK) return rc;
+  uint32_t *db = nullptr, *dg = nullptr;
+  hipError_t e = hipMalloc((void**)&db, hb.size() * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&dg, hg.size() * 4);
+  if (e == hipSuccess) e = hipMemcpy(db, hb.data(), hb.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dg, hg.data(), hg.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_pack_legacy(db, dg, stride, words16, n, a->n_pad, a->W, a->planes, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (db) (void)hipFree(db);
+  if (dg) (void)hipFree(dg);
+  if (e != hipSuccess) {
+    (void)hipFree(a->planes);
+    delete a;
+    return hip_fail(e, "rnj_alignment_from_bitstrings");
+  }
+  *out = static_cast<rnj_alignment*>(a);
+  return RNJ_OK;
+}
+
+int rnj_alignment_free(rnj_alignment* aln) {
+  if (!aln) return RNJ_OK;
+  if (aln->ctx) (void)hipSetDevice(aln->ctx->device);
+  if (aln->planes) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(aln->planes);
+  }
+  delete aln;
+  return RNJ_OK;
+}
+
+int rnj_alignment_info(const rnj_alignment* aln, unsigned* n, unsigned* L, unsigned* W,
+                       size_t* bytes) {
+  if (!aln) return fail(RNJ_E_INVALID, "null alignment");
+  if (n) *n = aln->n;
+  if (L) *L = aln->L;
+  if (W) *W = aln->W;
+  if (bytes) *bytes = aln->bytes();
+  return RNJ_OK;
+}
+
+int rnj_alignment_planes(const rnj_alignment* aln, uint32_t* host_planes) {
+  if (!aln || !host_planes) return fail(RNJ_E_INVALID, "null argument");
+  RNJ_HIP(hipSetDevice(aln->ctx->device));
+  RNJ_HIP(hipStreamSynchronize(aln->ctx->stream));
+  for (unsigned p = 0; p < 3; ++p)
+    RNJ_HIP(hipMemcpy2D(host_planes + (size_t)p * aln->W * aln->n, aln->n * sizeof(uint32_t),
+                        aln->planes + (size_t)p * aln->W * aln->n_pad,
+                        aln->n_pad * sizeof(uint32_t), aln->n * sizeof(uint32_t), aln->W,
+                        hipMemcpyDeviceToHost));
+  return RNJ_OK;
+}
+
+// ---- one matrix ---------------------------------------------------------------------------------
+int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, int model, int layout,
+                         float* out, size_t ld, int flags, float* max_out, void* stream) {
+  int rc = check_aln(ctx, aln);
+  if (rc == RNJ_OK) rc = check_model(model);
+  if (rc != RNJ_OK) return rc;
+  if (!out) return fail(RNJ_E_INVALID, "null output");
+  if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");
+  if (layout == RNJ_SQUARE && ld < aln->n) return fail(RNJ_E_INVALID, "ld smaller than n");
+  rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  std::vector<JobSpec> jobs{spec_for(aln, model, layout, out, ld)};
+  jobs[0].max_out = max_out;
+  return launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, stream_of(ctx, stream));
+}
+
+int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, uint32_t* tv,
+                      uint32_t* valid, void* stream) {
+  int rc = check_aln(ctx, aln);
+  if (rc != RNJ_OK) return rc;
+  if (!ts || !tv || !valid) return fail(RNJ_E_INVALID, "null output");
+  rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  std::vector<JobSpec> jobs{spec_for(aln, kCounts, kTriangle, nullptr, 0)};
+  jobs[0].out_ts = ts;
+  jobs[0].out_tv = tv;
+  jobs[0].out_valid = valid;
+  return launch_jobs(ctx, jobs, true, stream_of(ctx, stream));
+}
+
+// ---- host hand-off ------------------------------------------------------------------------------
+// Compute the square matrix in HBM, then stream row slabs through two pinned buffers into the
+// caller's rows (a D2H of one slab overlaps the host copy of the previous one).
+static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, unsigned row0,
+                        unsigned nrows, float** rows, float* max_out, bool raw) {
+  int rc = check_aln(ctx, aln);
+  if (rc == RNJ_OK) rc = check_model(model);
+  if (rc != RNJ_OK) return rc;
+  if (!rows) return fail(RNJ_E_INVALID, "null rows");
+  const unsigned n = aln->n;
+  if (row0 > n || nrows > n - row0) return fail(RNJ_E_INVALID, "row range outside the matrix");
+  rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  hipStream_t s = ctx->stream;
+  float* dmat = nullptr;
+  float* dmax = nullptr;
+  RNJ_HIP(hipMallocAsync((void**)&dmat, (size_t)n * n * sizeof(float), s));
+  RNJ_HIP(hipMallocAsync((void**)&dmax, sizeof(float), s));
+  std::vector<JobSpec> jobs{spec_for(aln, model, kSquare, dmat, n)};
+  jobs[0].max_out = dmax;
+  rc = launch_jobs(ctx, jobs, raw, s);
+  if (rc != RNJ_OK) {
+    (void)hipFreeAsync(dmat, s);
+    (void)hipFreeAsync(dmax, s);
+    return rc;
+  }
+  const size_t slab_rows = std::max<size_t>(1, ((size_t)32 << 20) / ((size_t)n * sizeof(float)));
+  float* pinned[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipError_t e = hipSuccess;
+  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+    e = hipHostMalloc((void**)&pinned[b], slab_rows * n * sizeof(float), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming);
+  }
+  size_t prev_rows = 0, prev_first = 0;
+  int prev_buf = -1, buf = 0;
+  for (size_t r = row0; e == hipSuccess && r < (size_t)row0 + nrows; r += slab_rows, buf ^= 1) {
+    const size_t cnt = std::min(slab_rows, (size_t)row0 + nrows - r);
+    e = hipMemcpyAsync(pinned[buf], dmat + r * n, cnt * n * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[buf], s);
+    if (e == hipSuccess && prev_buf >= 0) {
+      e = hipEventSynchronize(ev[prev_buf]);
+      for (size_t q = 0; e == hipSuccess && q < prev_rows; ++q)
+        std::memcpy(rows[prev_first - row0 + q], pinned[prev_buf] + q * n, n * sizeof(float));
+    }
+    prev_buf = buf;
+    prev_rows = cnt;
+    prev_first = r;
+  }
+  if (e == hipSuccess && prev_buf >= 0) {
+    e = hipEventSynchronize(ev[prev_buf]);
+    for (size_t q = 0; e == hipSuccess && q < prev_rows; ++q)
+      std::memcpy(rows[prev_first - row0 + q], pinned[prev_buf] + q * n, n * sizeof(float));
+  }
+  if (e == hipSuccess && max_out) e = hipMemcpy(max_out, dmax, sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (int b = 0; b < 2; ++b) {
+    if (pinned[b]) (void)hipHostFree(pinned[b]);
+    if (ev[b]) (void)hipEventDestroy(ev[b]);
+  }
+  (void)hipFreeAsync(dmat, s);
+  (void)hipFreeAsync(dmax, s);
+  if (e != hipSuccess) return hip_fail(e, "rnj_distances_rows");
+  return RNJ_OK;
+}
+
+int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model, float** rows,
+                       float* max_out) {
+  if (!aln) return fail(RNJ_E_INVALID, "null alignment");
+  return rows_handoff(ctx, aln, model, 0, aln->n, rows, max_out, false);
+}
+
+int rnj_distances_row_slab(rnj_context* ctx, const rnj_alignment* aln, int model, unsigned row0,
+                           unsigned rows, float** out_rows, float* max_out) {
+  return rows_handoff(ctx, aln, model, row0, rows, out_rows, max_out, false);
+}
+
+// ---- bootstrap ------------------------------------------------------------------------------------
+int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
+                         const uint16_t* weights, unsigned R, int layout, float* out,
+                         size_t matrix_stride, size_t ld, int flags, void* stream) {
+  int rc = check_aln(ctx, aln);
+  if (rc == RNJ_OK) rc = check_model(model);
+  if (rc != RNJ_OK) return rc;
+  if (!weights || !out) return fail(RNJ_E_INVALID, "null argument");
+  if (R == 0) return RNJ_OK;
+  if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");
+  const unsigned n = aln->n, L = aln->L;
+  const size_t min_stride = layout == RNJ_SQUARE ? (size_t)n * ld : (size_t)n * (n - 1) / 2;
+  if (layout == RNJ_SQUARE && ld < n) return fail(RNJ_E_INVALID, "ld smaller than n");
+  if (matrix_stride < min_stride) return fail(RNJ_E_INVALID, "matrix_stride too small");
+  rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  hipStream_t s = stream_of(ctx, stream);
+  // replicate lengths (sum of weights) size the column lists and the replicate planes
+  std::vector<unsigned> len(R);
+  unsigned maxlen = 1;
+  for (unsigned r = 0; r < R; ++r) {
+    unsigned long long sum = 0;
+    for (unsigned c = 0; c < L; ++c) sum += weights[(size_t)r * L + c];
+    if (sum >= (1ull << 31)) return fail(RNJ_E_INVALID, "replicate weights sum too large");
+    len[r] = (unsigned)sum;
+    maxlen = std::max(maxlen, len[r]);
+  }
+  const unsigned Wr = words_for_sites(maxlen);
+  const size_t plane_words = (size_t)3 * Wr * aln->n_pad;
+  uint16_t* dw = nullptr;
+  unsigned *didx = nullptr, *dlen = nullptr;
+  uint8_t* dcodes = nullptr;
+  uint32_t* dplanes = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dw, (size_t)R * L * sizeof(uint16_t), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&didx, (size_t)R * maxlen * sizeof(unsigned), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dlen, (size_t)R * sizeof(unsigned), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dcodes, (size_t)L * aln->n_pad, s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dplanes, (size_t)R * plane_words * 4, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dw, weights, (size_t)R * L * sizeof(uint16_t), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_planes_to_codes(aln->planes, aln->n_pad, aln->W, L, dcodes, s);
+  if (e == hipSuccess) e = launch_bootstrap_index(dw, R, L, didx, maxlen, dlen, s);
+  if (e == hipSuccess)
+    e = launch_bootstrap_gather(dcodes, aln->n_pad, didx, maxlen, dlen, R, Wr, dplanes, s);
+  if (e == hipSuccess) {
+    std::vector<JobSpec> jobs(R);
+    for (unsigned r = 0; r < R; ++r) {
+      JobSpec j{};
+      j.planes = dplanes + (size_t)r * plane_words;
+      j.n = n;
+      j.n_pad = aln->n_pad;
+      j.W_pad = Wr;
+      j.W = words_for_sites(len[r]);
+      j.model = model;
+      j.layout = layout;
+      j.ld = ld;
+      j.out = out + (size_t)r * matrix_stride;
+      jobs[r] = j;
+    }
+    rc = launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, s);
+  } else {
+    rc = hip_fail(e, "rnj_bootstrap_device");
+  }
+  if (dw) (void)hipFreeAsync(dw, s);
+  if (didx) (void)hipFreeAsync(didx, s);
+  if (dlen) (void)hipFreeAsync(dlen, s);
+  if (dcodes) (void)hipFreeAsync(dcodes, s);
+  if (dplanes) (void)hipFreeAsync(dplanes, s);
+  return rc;
+}
+
+}  // extern "C"
+
+// ---- the reference's device boundary ---------------------------------------------------------------
+// Module-wide state like the reference's (kimuraDNA_gpu.cu:178-191); errors print and exit(1) as
+// cutilSafeCall does (cutil_inline_runtime.h:159-166).
+namespace {
+struct LegacyState {
+  rnj_context* ctx = nullptr;
+  unsigned n = 0, bsc = 0, stride = 0, words16 = 0;
+  uint32_t* d_bits = nullptr;
+  uint32_t* d_gaps = nullptr;
+  rnj_alignment* aln = nullptr;
+};
+LegacyState g_k2p, g_counts;
+
+void legacy_die(const char* where) {
+  std::fprintf(stderr, "rapidnj_amd: %s failed: %s\n", where, last_error_cstr());
+  std::exit(1);
+}
+
+void legacy_check(int rc, const char* where) {
+  if (rc != RNJ_OK) legacy_die(where);
+}
+
+void legacy_init(LegacyState& st, unsigned n, unsigned bsc, unsigned bsStride, const char* where) {
+  if (!st.ctx) legacy_check(rnj_create(0, &st.ctx), where);
+  if (n < 2) {
+    set_error("need at least 2 sequences");
+    legacy_die(where);
+  }
+  if (st.aln) { rnj_alignment_free(st.aln); st.aln = nullptr; }
+  if (st.d_bits) (void)hipFree(st.d_bits);
+  if (st.d_gaps) (void)hipFree(st.d_gaps);
+  st.d_bits = st.d_gaps = nullptr;
+  st.n = n;
+  st.bsc = bsc;
+  st.stride = bsStride;
+  st.words16 = std::min(bsStride, bsc * 4);  // kimuraDNA_gpu.cu:245
+  const size_t bytes = (size_t)n * bsStride * sizeof(uint32_t);
+  hipError_t e = hipMalloc((void**)&st.d_bits, bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&st.d_gaps, bytes);
+  if (e != hipSuccess) legacy_check(hip_fail(e, "hipMalloc"), where);
+  Alignment* a = nullptr;
+  legacy_check(alloc_alignment(st.ctx, n, st.words16 * 16, st.words16 * 16, &a), where);
+  st.aln = static_cast<rnj_alignment*>(a);
+}
+
+void legacy_store(LegacyState& st, const unsigned* bits, const unsigned* gaps, const char* where) {
+  if (!st.ctx) {
+    set_error("called before initialisation");
+    legacy_die(where);
+  }
+  const size_t bytes = (size_t)st.n * st.stride * sizeof(uint32_t);
+  hipError_t e = hipMemcpy(st.d_bits, bits, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(st.d_gaps, gaps, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_pack_legacy(st.d_bits, st.d_gaps, st.stride, st.words16, st.n, st.aln->n_pad,
+                           st.aln->W, st.aln->planes, st.ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(st.ctx->stream);
+  if (e != hipSuccess) legacy_check(hip_fail(e, where), where);
+}
+}  // namespace
+
+extern "C" {
+
+void initialiseDNA2_gpu(unsigned int sequenceCount, unsigned int bitStringCount,
+                        unsigned int bsStride) {
+  legacy_init(g_k2p, sequenceCount, bitStringCount, bsStride, "initialiseDNA2_gpu");
+}
+
+void storeDataDNA2_gpu(unsigned int* bitStrings, unsigned int* gapFilters) {
+  legacy_store(g_k2p, bitStrings, gapFilters, "storeDataDNA2_gpu");
+}
+
+void computeDistancesDNA2_gpu(float* results) {
+  const char* where = "computeDistancesDNA2_gpu";
+  if (!g_k2p.ctx) {
+    set_error("called before initialisation");
+    legacy_die(where);
+  }
+  const unsigned n = g_k2p.n;
+  float* d = nullptr;
+  hipError_t e = hipMalloc((void**)&d, (size_t)n * n * sizeof(float));
+  if (e != hipSuccess) legacy_check(hip_fail(e, "hipMalloc"), where);
+  legacy_check(rnj_distances_device(g_k2p.ctx, g_k2p.aln, RNJ_K2P, RNJ_SQUARE, d, n, RNJ_FLAG_RAW,
+                                    nullptr, nullptr),
+               where);
+  e = hipMemcpy(results, d, (size_t)n * n * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) legacy_check(hip_fail(e, where), where);
+}
+
+void initialiseDNA_gpu(unsigned int sequenceCount, unsigned int bitStringCount,
+                       unsigned int bsStride) {
+  legacy_init(g_counts, sequenceCount, bitStringCount, bsStride, "initialiseDNA_gpu");
+}
+
+void storeDataDNA_gpu(unsigned int* bitStrings, unsigned int* gapFilters) {
+  legacy_store(g_counts, bitStrings, gapFilters, "storeDataDNA_gpu");
+}
+
+void computeDistancesDNA_gpu(unsigned int* ts, unsigned int* tv, unsigned int* gaps) {
+  const char* where = "computeDistancesDNA_gpu";
+  if (!g_counts.ctx) {
+    set_error("called before initialisation");
+    legacy_die(where);
+  }
+  const unsigned n = g_counts.n;
+  const size_t m = (size_t)n * (n - 1) / 2;
+  uint32_t* d = nullptr;
+  hipError_t e = hipMalloc((void**)&d, 3 * m * sizeof(uint32_t));
+  if (e != hipSuccess) legacy_check(hip_fail(e, "hipMalloc"), where);
+  legacy_check(rnj_counts_device(g_counts.ctx, g_counts.aln, d, d + m, d + 2 * m, nullptr), where);
+  std::vector<uint32_t> h(3 * m);
+  e = hipMemcpy(h.data(), d, 3 * m * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) legacy_check(hip_fail(e, where), where);
+  unsigned int* outs[3] = {ts, tv, gaps};
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t* src = h.data() + q * m;
+    unsigned int* dst = outs[q];
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+      dst[i * n + i] = 0;
+      for (size_t j = i + 1; j < n; ++j, ++k) {
+        dst[i * n + j] = src[k];
+        dst[j * n + i] = src[k];
+      }
+    }
+  }
+}
+
+void getResultsDNA_gpu(unsigned int* ts, unsigned int* tv, unsigned int* gaps) {
+  (void)ts; (void)tv; (void)gaps;
+  if (g_counts.ctx) legacy_check(rnj_synchronize(g_counts.ctx), "getResultsDNA_gpu");
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// Host runtime shared by runtime.cpp (context, alignments, launches, C ABI) and server.cpp
+// (served jobs). Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/rapidnj_amd.h"
+#include "internal.hpp"
+
+namespace rnj {
+
+// ---- errors ---------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* last_error_cstr();
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define RNJ_HIP(call)                                   \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return hip_fail(_e, #call);   \
+  } while (0)
+
+// ---- scratch for one launch -----------------------------------------------------------------
+// Job table (pinned host copy + device copy), per-job statistics (max bits, saturated count) and
+// the saturated-pair list. Reused once the event recorded after its last launch has completed.
+struct Workspace {
+  JobDesc* h_jobs = nullptr;
+  JobDesc* d_jobs = nullptr;
+  unsigned cap_jobs = 0;
+  unsigned* d_stats = nullptr;  // [cap_jobs][2]: maxbits, sent_count
+  unsigned long long* d_sent = nullptr;
+  size_t cap_sent = 0;
+  hipEvent_t done = nullptr;
+  bool recorded = false;
+  bool in_use = false;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int tile_config = -1;  // -1: autotune per shape class
+  std::mutex mu;
+  std::vector<std::unique_ptr<Workspace>> pool;
+  // autotune cache: (log2 n bucket, log2 W bucket, jc_only) -> config
+  std::map<std::tuple<int, int, int>, int> tuned;
+  hipDeviceProp_t prop{};
+};
+
+struct Alignment {
+  Context* ctx = nullptr;
+  unsigned n = 0, L = 0;     // taxa, sites
+  unsigned n_pad = 0;        // multiple of kTaxaPad
+  unsigned W = 0;            // 32-site words, multiple of kWordPad (= W_pad of the planes)
+  uint32_t* planes = nullptr;  // [3][W][n_pad]
+  size_t bytes() const { return (size_t)3 * W * n_pad * sizeof(uint32_t); }
+};
+
+// One matrix of a launch, before the runtime numbers tiles and binds scratch.
+struct JobSpec {
+  const uint32_t* planes;
+  unsigned n, n_pad, W_pad, W;
+  int model;   // Model
+  int layout;  // Layout
+  size_t ld;
+  float* out;
+  uint32_t *out_ts, *out_tv, *out_valid;
+  float* max_out;  // device pointer or null
+};
+
+unsigned round_up(unsigned x, unsigned m);
+unsigned words_for_sites(unsigned sites);  // ceil(sites/32) rounded up to kWordPad
+size_t sentinel_capacity(unsigned n);
+
+// Allocate planes for an alignment of n taxa and `sites` counted sites (zeroed).
+int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out);
+
+// Enqueue a multi-job launch on `stream`: reset statistics, tile kernel, saturation patch
+// (unless raw), max copies. All jobs must share one tile configuration (chosen here).
+int launch_jobs(Context* ctx, std::vector<JobSpec>& jobs, bool raw, hipStream_t stream);
+
+int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only);
+void free_workspace_public(Workspace* ws);
+
+}  // namespace rnj
+
+struct rnj_context : rnj::Context {};
+struct rnj_alignment : rnj::Alignment {};

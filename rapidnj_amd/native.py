@@ -1,0 +1,358 @@
+"""ctypes binding of librapidnj_amd.so (the C ABI declared in include/rapidnj_amd.h).
+
+The library is the product: every distance, count and packing call below runs the HIP kernels.
+There is no CPU fallback -- importing this module without the built library raises, and so does
+creating a :class:`Context` without a HIP device. Device buffers are torch tensors (allocation
+and stream plumbing only).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librapidnj_amd.so")
+
+RNJ_OK, RNJ_E_INVALID, RNJ_E_HIP, RNJ_E_NOMEM, RNJ_E_NOT_DNA = 0, 1, 2, 3, 4
+RNJ_E_ADMISSION, RNJ_E_CANCELLED, RNJ_E_NOT_READY, RNJ_E_NO_DEVICE = 5, 6, 7, 8
+JC, K2P = 0, 1
+TRIANGLE, SQUARE = 0, 1
+FLAG_RAW = 1
+
+_vp = ctypes.c_void_p
+_u = ctypes.c_uint
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); the exported C ABI, in header order
+SIGNATURES = {
+    "initialiseDNA2_gpu": (None, [_u, _u, _u]),
+    "storeDataDNA2_gpu": (None, [_vp, _vp]),
+    "computeDistancesDNA2_gpu": (None, [_vp]),
+    "initialiseDNA_gpu": (None, [_u, _u, _u]),
+    "storeDataDNA_gpu": (None, [_vp, _vp]),
+    "computeDistancesDNA_gpu": (None, [_vp, _vp, _vp]),
+    "getResultsDNA_gpu": (None, [_vp, _vp, _vp]),
+    "rnj_last_error": (ctypes.c_char_p, []),
+    "rnj_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "rnj_api_version": (ctypes.c_int, []),
+    "rnj_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "rnj_destroy": (ctypes.c_int, [_vp]),
+    "rnj_synchronize": (ctypes.c_int, [_vp]),
+    "rnj_tile_config_count": (ctypes.c_int, []),
+    "rnj_tile_config_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "rnj_set_tile_config": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "rnj_get_tile_config": (ctypes.c_int, [_vp, _u, _u, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "rnj_alignment_from_chars": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(_vp)]),
+    "rnj_alignment_from_rows": (ctypes.c_int, [_vp, _vp, _u, _u, ctypes.POINTER(_vp)]),
+    "rnj_alignment_from_bitstrings": (ctypes.c_int, [_vp, _vp, _vp, _u, _u, ctypes.POINTER(_vp)]),
+    "rnj_alignment_free": (ctypes.c_int, [_vp]),
+    "rnj_alignment_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u), ctypes.POINTER(_u),
+                                          ctypes.POINTER(_u), ctypes.POINTER(_sz)]),
+    "rnj_alignment_planes": (ctypes.c_int, [_vp, _vp]),
+    "rnj_distances_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _sz,
+                                            ctypes.c_int, _vp, _vp]),
+    "rnj_counts_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rnj_distances_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
+    "rnj_distances_row_slab": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u, _u, _vp, _vp]),
+    "rnj_bootstrap_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _u, ctypes.c_int, _vp,
+                                            _sz, _sz, ctypes.c_int, _vp]),
+    "rnj_server_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "rnj_server_submit": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "rnj_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _vp]),
+    "rnj_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "rnj_server_cancel": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "rnj_server_destroy": (ctypes.c_int, [_vp]),
+}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", _u), ("max_batch_jobs", _u), ("hbm_budget", _sz)]
+
+
+class JobStats(ctypes.Structure):
+    _fields_ = [("queued_s", ctypes.c_double), ("device_s", ctypes.c_double),
+                ("total_s", ctypes.c_double), ("batch_id", _u), ("batch_jobs", _u)]
+
+
+class RnjError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_status_name(status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load librapidnj_amd.so; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with "
+                              "`python -c 'import __graft_entry__ as g; g.build()'` "
+                              "or `make -C rapidnj_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _status_name(status: int) -> str:
+    try:
+        return lib().rnj_status_string(status).decode()
+    except Exception:  # noqa: BLE001 - only used while formatting another error
+        return f"status {status}"
+
+
+def check(status: int) -> None:
+    if status != RNJ_OK:
+        raise RnjError(status, (lib().rnj_last_error() or b"").decode())
+
+
+def _model(model) -> int:
+    if isinstance(model, str):
+        return {"jc": JC, "k2p": K2P, "kim": K2P}[model.lower()]
+    return int(model)
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return int(stream)
+
+
+@dataclass
+class AlignmentInfo:
+    n: int
+    L: int
+    W: int
+    bytes: int
+
+
+class Alignment:
+    """A DNA alignment packed into HBM as H/L/V bit planes (owned by a :class:`Context`)."""
+
+    def __init__(self, ctx: "Context", handle: int):
+        self.ctx = ctx
+        self.handle = handle
+        n, L, W, b = _u(), _u(), _u(), _sz()
+        check(lib().rnj_alignment_info(handle, ctypes.byref(n), ctypes.byref(L), ctypes.byref(W),
+                                       ctypes.byref(b)))
+        self.info = AlignmentInfo(n.value, L.value, W.value, b.value)
+
+    @property
+    def n(self) -> int:
+        return self.info.n
+
+    @property
+    def L(self) -> int:
+        return self.info.L
+
+    def planes(self) -> np.ndarray:
+        out = np.zeros((3, self.info.W, self.info.n), dtype=np.uint32)
+        check(lib().rnj_alignment_planes(self.handle, out.ctypes.data))
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rnj_alignment_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+class Context:
+    """One HIP device: streams, tile configuration, scratch. Raises without a device."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        check(lib().rnj_create(device, ctypes.byref(h)))
+        self.handle = h.value
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rnj_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self) -> None:
+        check(lib().rnj_synchronize(self.handle))
+
+    def set_tile_config(self, config: int) -> None:
+        check(lib().rnj_set_tile_config(self.handle, config))
+
+    def tile_config(self, n: int, L: int, model="k2p") -> int:
+        c = ctypes.c_int()
+        check(lib().rnj_get_tile_config(self.handle, n, L, _model(model), ctypes.byref(c)))
+        return c.value
+
+    # ---- alignments ---------------------------------------------------------------------------
+    def alignment(self, aln, skip_type_check: bool = False) -> Alignment:
+        """Pack an (n, L) uint8 alignment (numpy on the host, or a uint8 torch tensor on the device)."""
+        h = _vp()
+        if isinstance(aln, np.ndarray):
+            a = np.ascontiguousarray(aln, dtype=np.uint8)
+            n, L = a.shape
+            check(lib().rnj_alignment_from_chars(self.handle, a.ctypes.data, L, n, L, 0,
+                                                 int(skip_type_check), ctypes.byref(h)))
+        else:
+            import torch
+            assert aln.dtype == torch.uint8 and aln.dim() == 2 and aln.is_cuda
+            n, L = aln.shape
+            pitch = aln.stride(0)
+            assert aln.stride(1) == 1
+            torch.cuda.current_stream().synchronize()
+            check(lib().rnj_alignment_from_chars(self.handle, aln.data_ptr(), pitch, n, L, 1,
+                                                 int(skip_type_check), ctypes.byref(h)))
+        return Alignment(self, h.value)
+
+    def alignment_from_rows(self, rows: list[bytes]) -> Alignment:
+        n, L = len(rows), len(rows[0])
+        arr = (ctypes.c_char_p * n)(*rows)
+        h = _vp()
+        check(lib().rnj_alignment_from_rows(self.handle, ctypes.cast(arr, _vp), n, L,
+                                            ctypes.byref(h)))
+        return Alignment(self, h.value)
+
+    def alignment_from_bitstrings(self, bits: np.ndarray, gaps: np.ndarray) -> Alignment:
+        """From the reference's packed layout: (n, 4*bitStringsCount) uint32 codes and filters."""
+        bits = np.ascontiguousarray(bits, dtype=np.uint32)
+        gaps = np.ascontiguousarray(gaps, dtype=np.uint32)
+        n, words = bits.shape
+        pb = (_vp * n)(*[bits.ctypes.data + i * bits.strides[0] for i in range(n)])
+        pg = (_vp * n)(*[gaps.ctypes.data + i * gaps.strides[0] for i in range(n)])
+        h = _vp()
+        check(lib().rnj_alignment_from_bitstrings(self.handle, ctypes.cast(pb, _vp),
+                                                  ctypes.cast(pg, _vp), n, words // 4,
+                                                  ctypes.byref(h)))
+        return Alignment(self, h.value)
+
+    # ---- distances ----------------------------------------------------------------------------
+    def distances(self, aln: Alignment, model="k2p", layout: str = "triangle", out=None,
+                  raw: bool = False, max_out=None, stream=None):
+        """Distance matrix into a device tensor (float32): triangle (n(n-1)/2) or square (n, n)."""
+        import torch
+        n = aln.n
+        lay = {"triangle": TRIANGLE, "square": SQUARE}[layout]
+        if out is None:
+            shape = (n * (n - 1) // 2,) if lay == TRIANGLE else (n, n)
+            out = torch.empty(shape, dtype=torch.float32, device=f"cuda:{self.device}")
+        ld = out.stride(0) if lay == SQUARE else 0
+        check(lib().rnj_distances_device(self.handle, aln.handle, _model(model), lay,
+                                         out.data_ptr(), ld, FLAG_RAW if raw else 0,
+                                         max_out.data_ptr() if max_out is not None else None,
+                                         _stream_handle(stream)))
+        return out
+
+    def counts(self, aln: Alignment, stream=None):
+        """(ts, tv, valid) upper-triangle uint32 device tensors (as int32 storage)."""
+        import torch
+        m = aln.n * (aln.n - 1) // 2
+        buf = torch.empty((3, max(m, 1)), dtype=torch.int32, device=f"cuda:{self.device}")
+        check(lib().rnj_counts_device(self.handle, aln.handle, buf[0].data_ptr(),
+                                      buf[1].data_ptr(), buf[2].data_ptr(),
+                                      _stream_handle(stream)))
+        return buf[0, :m], buf[1, :m], buf[2, :m]
+
+    def distance_rows(self, aln: Alignment, model="k2p") -> tuple[np.ndarray, float]:
+        """Post-processed square matrix handed to host rows (the in-memory NJ hand-off)."""
+        n = aln.n
+        mat = np.empty((n, n), dtype=np.float32)
+        rows = (_vp * n)(*[mat.ctypes.data + i * mat.strides[0] for i in range(n)])
+        mx = ctypes.c_float()
+        check(lib().rnj_distances_rows(self.handle, aln.handle, _model(model),
+                                       ctypes.cast(rows, _vp), ctypes.byref(mx)))
+        return mat, mx.value
+
+    def bootstrap(self, aln: Alignment, weights: np.ndarray, model="k2p",
+                  layout: str = "triangle", out=None, raw: bool = False, stream=None):
+        """R replicate matrices from (R, L) uint16 column weights, in one launch."""
+        import torch
+        w = np.ascontiguousarray(weights, dtype=np.uint16)
+        R, L = w.shape
+        assert L == aln.L, "one weight per alignment column"
+        n = aln.n
+        lay = {"triangle": TRIANGLE, "square": SQUARE}[layout]
+        if out is None:
+            shape = (R, n * (n - 1) // 2) if lay == TRIANGLE else (R, n, n)
+            out = torch.empty(shape, dtype=torch.float32, device=f"cuda:{self.device}")
+        mstride = out.stride(0)
+        ld = out.stride(1) if lay == SQUARE else 0
+        check(lib().rnj_bootstrap_device(self.handle, aln.handle, _model(model), w.ctypes.data, R,
+                                         lay, out.data_ptr(), mstride, ld,
+                                         FLAG_RAW if raw else 0, _stream_handle(stream)))
+        return out
+
+    def server(self, slots: int = 2, max_batch_jobs: int = 64, hbm_budget: int = 0) -> "Server":
+        return Server(self, slots, max_batch_jobs, hbm_budget)
+
+
+class Server:
+    """Served jobs: admission by HBM budget, batching, overlap of batches on the device."""
+
+    def __init__(self, ctx: Context, slots: int, max_batch_jobs: int, hbm_budget: int):
+        self.ctx = ctx
+        cfg = ServerConfig(slots, max_batch_jobs, hbm_budget)
+        h = _vp()
+        check(lib().rnj_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+
+    def submit(self, aln: np.ndarray, model="k2p") -> int:
+        a = np.ascontiguousarray(aln, dtype=np.uint8)
+        n, L = a.shape
+        t = ctypes.c_uint64()
+        check(lib().rnj_server_submit(self.handle, a.ctypes.data, L, n, L, _model(model),
+                                      ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        rc = lib().rnj_server_poll(self.handle, ticket)
+        if rc == RNJ_E_NOT_READY:
+            return False
+        check(rc)
+        return True
+
+    def wait(self, ticket: int, n: int) -> tuple[np.ndarray, JobStats]:
+        out = np.empty(max(n * (n - 1) // 2, 1), dtype=np.float32)
+        st = JobStats()
+        check(lib().rnj_server_wait(self.handle, ticket, out.ctypes.data, ctypes.byref(st)))
+        return out[: n * (n - 1) // 2], st
+
+    def cancel(self, ticket: int) -> None:
+        check(lib().rnj_server_cancel(self.handle, ticket))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rnj_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def tile_configs() -> list[str]:
+    return [lib().rnj_tile_config_name(i).decode() for i in range(lib().rnj_tile_config_count())]

@@ -1,0 +1,294 @@
+"""Parity of the HIP path (through the C ABI) with the reference engine and the oracle.
+
+Bars: counts (ts, tv, valid) and packed planes bit-exact; distances within 1e-6 relative of the
+reference's float values, with identical sign of zero, identical saturated pairs and identical
+sentinel value 2*max. Expected values come from the golden vectors the reference engine wrote
+(tests/golden/) and, at sizes without fixtures, from the CPU oracle on the same seeded input.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rapidnj_amd.workloads import bootstrap_weights, expand_columns, simulate_alignment
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+EDGE = sorted(glob.glob(os.path.join(GOLDEN, "edge_L*.npz")))
+FULL_CASES = EDGE + [os.path.join(GOLDEN, f) for f in ("c1.npz", "lgt70.npz", "saturated.npz")]
+DIST_RTOL = 1e-6  # relative tolerance on corrected distances (FP64 log on device vs glibc)
+
+
+def _load(path):
+    return dict(np.load(path, allow_pickle=False))
+
+
+@pytest.fixture(scope="module")
+def N():
+    from rapidnj_amd import native
+    return native
+
+
+@pytest.fixture(scope="module")
+def ctx(N):
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _u32(t):
+    return _np(t).view(np.uint32)
+
+
+def assert_distances_match(got, ref):
+    """Reference post-processed matrices: 1e-6 relative, same zero signs, same sentinels."""
+    got = np.asarray(got, dtype=np.float32)
+    ref = np.asarray(ref, dtype=np.float32)
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, rtol=DIST_RTOL, atol=0)
+    np.testing.assert_array_equal(np.signbit(got), np.signbit(ref))
+
+
+def planes_from_reference_words(bits, gaps, L):
+    """The reference's 16-sites-per-word codes/filters -> H/L/V planes [3][ceil(L/32)][n]."""
+    n = bits.shape[0]
+    s = np.arange(L)
+    code = (bits[:, s // 16] >> (2 * (s % 16))) & 3
+    val = (gaps[:, s // 16] >> (2 * (s % 16))) & 1
+    W = (L + 31) // 32
+    out = np.zeros((3, W, n), dtype=np.uint32)
+    for p, field in enumerate((code >> 1, code & 1, val)):
+        for w in range(W):
+            blk = field[:, 32 * w: 32 * w + 32].astype(np.uint64)
+            out[p, w] = (blk << np.arange(blk.shape[1], dtype=np.uint64)).sum(axis=1).astype(np.uint32)
+    return out
+
+
+@pytest.mark.parametrize("path", EDGE, ids=os.path.basename)
+def test_planes_match_reference_packing(ctx, path):
+    d = _load(path)
+    aln = d["aln"]
+    A = ctx.alignment(aln)
+    got = A.planes()
+    ref = planes_from_reference_words(d["bits"], d["gaps"], aln.shape[1])
+    W = ref.shape[1]
+    np.testing.assert_array_equal(got[:, :W], ref)
+    assert not got[:, W:].any()  # padding words count nothing
+
+
+@pytest.mark.parametrize("path", FULL_CASES, ids=os.path.basename)
+def test_counts_bit_exact(ctx, path):
+    d = _load(path)
+    ts, tv, va = ctx.counts(ctx.alignment(d["aln"]))
+    np.testing.assert_array_equal(_u32(ts), d["ts"])
+    np.testing.assert_array_equal(_u32(tv), d["tv"])
+    np.testing.assert_array_equal(_u32(va), d["valid"])
+
+
+@pytest.mark.parametrize("path", FULL_CASES, ids=os.path.basename)
+@pytest.mark.parametrize("model", ["jc", "k2p"])
+def test_square_matrix_matches_reference(ctx, path, model):
+    d = _load(path)
+    got = _np(ctx.distances(ctx.alignment(d["aln"]), model, layout="square"))
+    assert_distances_match(got, d[model])
+    assert np.all(got.diagonal().view(np.uint32) == 0)
+
+
+@pytest.mark.parametrize("path", FULL_CASES, ids=os.path.basename)
+@pytest.mark.parametrize("model", ["jc", "k2p"])
+def test_raw_triangle_matches_oracle(ctx, N, path, model):
+    """Before post-processing: identical saturated pairs (-1) and per-pair values."""
+    import torch
+    d = _load(path)
+    tri_ref, full_ref, mx_ref = O.distance_matrix(d["aln"], model, threads=4)
+    A = ctx.alignment(d["aln"])
+    mx = torch.zeros(1, dtype=torch.float32, device="cuda:0")
+    got = _np(ctx.distances(A, model, layout="triangle", raw=True, max_out=mx))
+    np.testing.assert_array_equal(got == -1, tri_ref == -1)
+    assert_distances_match(got, tri_ref)
+    assert float(mx.item()) == pytest.approx(mx_ref, rel=DIST_RTOL)
+    post = _np(ctx.distances(A, model, layout="triangle"))
+    iu = np.triu_indices(A.n, 1)
+    assert_distances_match(post, full_ref[iu])
+
+
+def test_host_rows_handoff(ctx):
+    d = _load(os.path.join(GOLDEN, "lgt70.npz"))
+    for model in ("jc", "k2p"):
+        mat, mx = ctx.distance_rows(ctx.alignment(d["aln"]), model)
+        assert_distances_match(mat, d[model])
+        assert np.array_equal(mat, mat.T)
+
+
+@pytest.mark.parametrize("path", EDGE[:6] + [os.path.join(GOLDEN, "c1.npz")], ids=os.path.basename)
+def test_from_reference_bitstrings(ctx, path):
+    d = _load(path)
+    if "bits" not in d:
+        d["bits"], d["gaps"], _ = O.pack(d["aln"])
+    A = ctx.alignment_from_bitstrings(d["bits"], d["gaps"])
+    ts, tv, va = ctx.counts(A)
+    np.testing.assert_array_equal(_u32(ts), d["ts"])
+    np.testing.assert_array_equal(_u32(tv), d["tv"])
+    np.testing.assert_array_equal(_u32(va), d["valid"])
+
+
+def _staging(bits, gaps, L):
+    """dataloader::initGPUData layout: [n][bsStride], bsStride = roundup(ceil(L/16), 128)."""
+    n, words = bits.shape
+    bsc = words // 4
+    stride = ((L + 15) // 16 + 127) & ~127
+    sb = np.zeros((n, stride), dtype=np.uint32)
+    sg = np.zeros((n, stride), dtype=np.uint32)
+    k = min(stride, words)
+    sb[:, :k] = bits[:, :k]
+    sg[:, :k] = gaps[:, :k]
+    return sb, sg, bsc, stride
+
+
+def test_reference_gpu_abi_k2p_and_counts(N):
+    d = _load(os.path.join(GOLDEN, "lgt70.npz"))
+    bits, gaps, _ = O.pack(d["aln"])
+    sb, sg, bsc, stride = _staging(bits, gaps, d["aln"].shape[1])
+    n = bits.shape[0]
+    L = N.lib()
+    L.initialiseDNA2_gpu(n, bsc, stride)
+    L.storeDataDNA2_gpu(sb.ctypes.data, sg.ctypes.data)
+    res = np.zeros((n, n), dtype=np.float32)
+    L.computeDistancesDNA2_gpu(res.ctypes.data)
+    tri_ref, _, _ = O.distance_matrix(d["aln"], "k2p")
+    iu = np.triu_indices(n, 1)
+    np.testing.assert_array_equal(res[iu] == -1, tri_ref == -1)
+    assert_distances_match(res[iu], tri_ref)
+    L.initialiseDNA_gpu(n, bsc, stride)
+    L.storeDataDNA_gpu(sb.ctypes.data, sg.ctypes.data)
+    ts = np.zeros((n, n), dtype=np.uint32)
+    tv = np.zeros_like(ts)
+    gp = np.zeros_like(ts)
+    L.computeDistancesDNA_gpu(ts.ctypes.data, tv.ctypes.data, gp.ctypes.data)
+    L.getResultsDNA_gpu(ts.ctypes.data, tv.ctypes.data, gp.ctypes.data)
+    np.testing.assert_array_equal(ts[iu], d["ts"])
+    np.testing.assert_array_equal(tv[iu], d["tv"])
+    np.testing.assert_array_equal(gp[iu], d["valid"])
+
+
+def test_every_tile_config_agrees(ctx, N):
+    d = _load(os.path.join(GOLDEN, "c1.npz"))
+    A = ctx.alignment(d["aln"])
+    try:
+        for c in range(len(N.tile_configs())):
+            ctx.set_tile_config(c)
+            ts, tv, va = ctx.counts(A)
+            np.testing.assert_array_equal(_u32(ts), d["ts"], err_msg=f"config {c}")
+            np.testing.assert_array_equal(_u32(tv), d["tv"], err_msg=f"config {c}")
+            np.testing.assert_array_equal(_u32(va), d["valid"], err_msg=f"config {c}")
+            for model in ("jc", "k2p"):
+                got = _np(ctx.distances(A, model, layout="square"))
+                assert_distances_match(got, d[model])
+    finally:
+        ctx.set_tile_config(-1)
+
+
+def test_bootstrap_matches_reference_on_expanded_columns(ctx):
+    d = _load(os.path.join(GOLDEN, "bootstrap.npz"))
+    A = ctx.alignment(d["aln"])
+    got = _np(ctx.bootstrap(A, d["weights"], "k2p", layout="square"))
+    for r in range(d["weights"].shape[0]):
+        assert_distances_match(got[r], d[f"k2p{r}"])
+
+
+def test_bootstrap_counts_equal_plain_run_on_expanded(ctx):
+    aln = simulate_alignment(70, 900, 0.05, 31)
+    w = bootstrap_weights(900, 5, seed0=40)
+    w[2] = 0
+    w[2, :17] = 3                               # a short replicate (51 sites)
+    A = ctx.alignment(aln)
+    for model in ("jc", "k2p"):
+        got = _np(ctx.bootstrap(A, w, model))
+        for r in range(w.shape[0]):
+            ref = _np(ctx.distances(ctx.alignment(expand_columns(aln, w[r])), model))
+            np.testing.assert_array_equal(got[r].view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("n,L,gap", [(2, 1, 0.0), (3, 5, 0.5), (129, 77, 0.1), (200, 2049, 0.05),
+                                     (257, 4000, 0.3)])
+def test_shapes_against_oracle(ctx, n, L, gap):
+    aln = simulate_alignment(n, L, gap, 1000 + n + L)
+    ts_r, tv_r, va_r = O.counts_triangle(aln)
+    A = ctx.alignment(aln)
+    ts, tv, va = ctx.counts(A)
+    np.testing.assert_array_equal(_u32(ts), ts_r)
+    np.testing.assert_array_equal(_u32(tv), tv_r)
+    np.testing.assert_array_equal(_u32(va), va_r)
+    for model in ("jc", "k2p"):
+        _, full, _ = O.distance_matrix(aln, model)
+        assert_distances_match(_np(ctx.distances(A, model, layout="square")), full)
+
+
+def test_plain_gpu_config_against_oracle(ctx):
+    """2000 taxa x 10000 sites, 5% gaps, K2P (the plain single-job GPU configuration)."""
+    aln = simulate_alignment(2000, 10000, 0.05, 2)
+    A = ctx.alignment(aln)
+    ts_r, tv_r, va_r = O.counts_triangle(aln, threads=os.cpu_count() or 8)
+    ts, tv, va = ctx.counts(A)
+    np.testing.assert_array_equal(_u32(ts), ts_r)
+    np.testing.assert_array_equal(_u32(tv), tv_r)
+    np.testing.assert_array_equal(_u32(va), va_r)
+    tri_ref, _, _ = O.correct_triangle(ts_r, tv_r, va_r, 2000, "k2p")
+    got = _np(ctx.distances(A, "k2p", raw=True))
+    np.testing.assert_array_equal(got == -1, tri_ref == -1)
+    assert_distances_match(got, tri_ref)
+
+
+def test_large_properties(ctx):
+    """6000 x 3000: symmetric, zero diagonal, sentinel = 2*max, sampled pairs exact."""
+    import torch
+    n, L = 6000, 3000
+    aln = simulate_alignment(n, L, 0.05, 77)
+    A = ctx.alignment(aln)
+    mx = torch.zeros(1, dtype=torch.float32, device="cuda:0")
+    sq = ctx.distances(A, "k2p", layout="square", max_out=mx)
+    assert torch.equal(sq, sq.T)
+    assert torch.all(sq.diagonal() == 0)
+    m = float(mx.item())
+    assert m > 0 and float(sq.max().item()) <= 2 * m
+    ts, tv, va = ctx.counts(A)
+    bits, gaps, _ = O.pack(aln)
+    rng = np.random.default_rng(5)
+    ii = rng.integers(0, n, 400)
+    jj = rng.integers(0, n, 400)
+    tsn, tvn, van = _u32(ts), _u32(tv), _u32(va)
+    sqn = _np(sq)
+    for i, j in zip(ii, jj):
+        if i == j:
+            continue
+        a, b = min(i, j), max(i, j)
+        k = a * n - a * (a + 1) // 2 + (b - a - 1)
+        ref = O.count_words(bits, gaps, a, b)
+        assert (int(tsn[k]), int(tvn[k]), int(van[k])) == ref
+        dk = O.k2p(*ref)
+        if dk != -1:
+            assert sqn[a, b] == pytest.approx(dk, rel=DIST_RTOL)
+
+
+def test_error_paths(ctx, N):
+    with pytest.raises(N.RnjError) as e:
+        ctx.alignment(np.frombuffer(b"ACGT", dtype=np.uint8).reshape(1, 4))
+    assert e.value.status == N.RNJ_E_INVALID
+    prot = np.frombuffer(b"MKVLAAGG" * 2, dtype=np.uint8).reshape(2, 8)
+    with pytest.raises(N.RnjError) as e:
+        ctx.alignment(prot)
+    assert e.value.status == N.RNJ_E_NOT_DNA
+    A = ctx.alignment(simulate_alignment(10, 50, 0.0, 3))
+    import torch
+    small = torch.empty((10, 9), dtype=torch.float32, device="cuda:0")
+    with pytest.raises(N.RnjError):
+        ctx.distances(A, "k2p", layout="square", out=small)
+    with pytest.raises(KeyError):
+        ctx.distances(A, "logdet")
