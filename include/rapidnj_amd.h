@@ -136,7 +136,7 @@ RNJ_EXPORT int rnj_alignment_info(const rnj_alignment* aln, unsigned* n, unsigne
 RNJ_EXPORT int rnj_alignment_planes(const rnj_alignment* aln, uint32_t* host_planes);
 
 /* One distance matrix into device memory `out` (layout RNJ_TRIANGLE or RNJ_SQUARE with row
- * stride `ld` floats), enqueued on `stream` (a hipStream_t; NULL = the context's stream).
+ * stride `ld` floats), enqueued on `stream` (a hipStream_t; NULL = HIP's null stream).
  * Saturated pairs become 2 * max unless RNJ_FLAG_RAW. If max_out (device pointer) is non-null,
  * the matrix maximum (KimuraDistance.cpp:282-287) is written there as float.
  * Replaces KimuraDistance::computeDistances / JCdistance::computeDistanceMatrix
@@ -152,12 +152,18 @@ RNJ_EXPORT int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uin
 /* Host hand-off: rows[i] are n caller-owned arrays of n floats (e.g. new distType[n]), filled
  * with the post-processed symmetric matrix, diagonal 0, sentinels patched: the in-memory matrix
  * handed to distMatrixReader (src/main.cpp:419-437, distMatrixReader.cpp:14-23). Synchronous.
- * With row0/rows a slab of rows [row0, row0+rows) is written (rows[k] is row row0+k), so a
- * caller can stream rows to disk (diskMatrix::writeArray, diskMatrix.cpp:96-101). */
-RNJ_EXPORT int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model, float** rows,
-                       float* max_out);
-RNJ_EXPORT int rnj_distances_row_slab(rnj_context* ctx, const rnj_alignment* aln, int model,
-                           unsigned row0, unsigned rows, float** out_rows, float* max_out);
+ * max_out (host, may be null) receives the maximum used for the sentinel. */
+RNJ_EXPORT int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
+                                  float** rows, float* max_out);
+
+/* Row streaming for the disk-backed path: the matrix is computed once in HBM, then every row
+ * (post-processed, full length n) is passed to `sink` in row order, as
+ * KimuraDistance::computeDistanceMatrixMT hands rows to diskMatrix::writeArray
+ * (KimuraDistance.cpp:267-281, diskMatrix.cpp:96-101). A non-zero return from the sink stops
+ * the stream with RNJ_E_CANCELLED. `values` is valid only during the call. */
+typedef int (*rnj_row_sink)(void* user, unsigned row, const float* values, unsigned n);
+RNJ_EXPORT int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
+                                         rnj_row_sink sink, void* user, float* max_out);
 
 /* Batched bootstrap: R replicates given as column-weight vectors weights[r*L + c] (host u16;
  * replicate r repeats column c weights[r][c] times, the expanded alignment of

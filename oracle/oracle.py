@@ -103,7 +103,7 @@ def count_naive(a: bytes, b: bytes) -> tuple[int, int, int]:
 
 
 def _row_ptr(arr: np.ndarray, i: int) -> int:
-    return arr.ctypes.data + i * arr.strides[0]
+    return int(arr.ctypes.data) + int(i) * int(arr.strides[0])
 
 
 def count_words(bits, gaps, i, j) -> tuple[int, int, int]:

@@ -288,8 +288,11 @@ using namespace rnj;
 // ---- helpers for the C ABI ---------------------------------------------------------------------
 namespace {
 
+// The caller's stream; NULL is HIP's null stream, which orders with every blocking stream (the
+// convention of HIP libraries, and what torch's default stream handle is).
 hipStream_t stream_of(rnj_context* ctx, void* stream) {
-  return stream ? (hipStream_t)stream : ctx->stream;
+  (void)ctx;
+  return (hipStream_t)stream;
 }
 
 int set_device(rnj_context* ctx) {
@@ -586,19 +589,20 @@ int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, 
   return launch_jobs(ctx, jobs, true, stream_of(ctx, stream));
 }
 
+}  // extern "C"
+
 // ---- host hand-off ------------------------------------------------------------------------------
-// Compute the square matrix in HBM, then stream row slabs through two pinned buffers into the
-// caller's rows (a D2H of one slab overlaps the host copy of the previous one).
-static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, unsigned row0,
-                        unsigned nrows, float** rows, float* max_out, bool raw) {
+// Compute the square matrix in HBM once, then stream row slabs through two pinned buffers: the
+// D2H of one slab overlaps the host-side delivery of the previous one.
+template <class Deliver>
+static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, float* max_out,
+                        Deliver deliver) {
   int rc = check_aln(ctx, aln);
   if (rc == RNJ_OK) rc = check_model(model);
   if (rc != RNJ_OK) return rc;
-  if (!rows) return fail(RNJ_E_INVALID, "null rows");
-  const unsigned n = aln->n;
-  if (row0 > n || nrows > n - row0) return fail(RNJ_E_INVALID, "row range outside the matrix");
   rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
+  const unsigned n = aln->n;
   hipStream_t s = ctx->stream;
   float* dmat = nullptr;
   float* dmax = nullptr;
@@ -606,61 +610,72 @@ static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, u
   RNJ_HIP(hipMallocAsync((void**)&dmax, sizeof(float), s));
   std::vector<JobSpec> jobs{spec_for(aln, model, kSquare, dmat, n)};
   jobs[0].max_out = dmax;
-  rc = launch_jobs(ctx, jobs, raw, s);
-  if (rc != RNJ_OK) {
-    (void)hipFreeAsync(dmat, s);
-    (void)hipFreeAsync(dmax, s);
-    return rc;
-  }
+  rc = launch_jobs(ctx, jobs, false, s);
   const size_t slab_rows = std::max<size_t>(1, ((size_t)32 << 20) / ((size_t)n * sizeof(float)));
   float* pinned[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipError_t e = hipSuccess;
-  for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+  for (int b = 0; rc == RNJ_OK && b < 2 && e == hipSuccess; ++b) {
     e = hipHostMalloc((void**)&pinned[b], slab_rows * n * sizeof(float), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming);
   }
   size_t prev_rows = 0, prev_first = 0;
   int prev_buf = -1, buf = 0;
-  for (size_t r = row0; e == hipSuccess && r < (size_t)row0 + nrows; r += slab_rows, buf ^= 1) {
-    const size_t cnt = std::min(slab_rows, (size_t)row0 + nrows - r);
+  auto flush = [&](int b, size_t first, size_t cnt) -> int {
+    hipError_t w = hipEventSynchronize(ev[b]);
+    if (w != hipSuccess) return hip_fail(w, "hipEventSynchronize");
+    for (size_t q = 0; q < cnt; ++q) {
+      const int r = deliver((unsigned)(first + q), pinned[b] + q * n);
+      if (r != RNJ_OK) return r;
+    }
+    return RNJ_OK;
+  };
+  for (size_t r = 0; rc == RNJ_OK && e == hipSuccess && r < n; r += slab_rows, buf ^= 1) {
+    const size_t cnt = std::min(slab_rows, (size_t)n - r);
     e = hipMemcpyAsync(pinned[buf], dmat + r * n, cnt * n * sizeof(float), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipEventRecord(ev[buf], s);
-    if (e == hipSuccess && prev_buf >= 0) {
-      e = hipEventSynchronize(ev[prev_buf]);
-      for (size_t q = 0; e == hipSuccess && q < prev_rows; ++q)
-        std::memcpy(rows[prev_first - row0 + q], pinned[prev_buf] + q * n, n * sizeof(float));
-    }
+    if (e == hipSuccess && prev_buf >= 0) rc = flush(prev_buf, prev_first, prev_rows);
     prev_buf = buf;
     prev_rows = cnt;
     prev_first = r;
   }
-  if (e == hipSuccess && prev_buf >= 0) {
-    e = hipEventSynchronize(ev[prev_buf]);
-    for (size_t q = 0; e == hipSuccess && q < prev_rows; ++q)
-      std::memcpy(rows[prev_first - row0 + q], pinned[prev_buf] + q * n, n * sizeof(float));
-  }
-  if (e == hipSuccess && max_out) e = hipMemcpy(max_out, dmax, sizeof(float), hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (rc == RNJ_OK && e == hipSuccess && prev_buf >= 0) rc = flush(prev_buf, prev_first, prev_rows);
+  if (rc == RNJ_OK && e == hipSuccess && max_out)
+    e = hipMemcpy(max_out, dmax, sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipStreamSynchronize(s);
   for (int b = 0; b < 2; ++b) {
     if (pinned[b]) (void)hipHostFree(pinned[b]);
     if (ev[b]) (void)hipEventDestroy(ev[b]);
   }
   (void)hipFreeAsync(dmat, s);
   (void)hipFreeAsync(dmax, s);
-  if (e != hipSuccess) return hip_fail(e, "rnj_distances_rows");
+  if (rc != RNJ_OK) return rc;
+  if (e != hipSuccess) return hip_fail(e, "row hand-off");
   return RNJ_OK;
 }
 
+extern "C" {
+
 int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model, float** rows,
                        float* max_out) {
-  if (!aln) return fail(RNJ_E_INVALID, "null alignment");
-  return rows_handoff(ctx, aln, model, 0, aln->n, rows, max_out, false);
+  if (!aln || !rows) return fail(RNJ_E_INVALID, "null argument");
+  for (unsigned i = 0; i < aln->n; ++i)
+    if (!rows[i]) return fail(RNJ_E_INVALID, "null row");
+  const unsigned n = aln->n;
+  return rows_handoff(ctx, aln, model, max_out, [rows, n](unsigned row, const float* v) {
+    std::memcpy(rows[row], v, (size_t)n * sizeof(float));
+    return (int)RNJ_OK;
+  });
 }
 
-int rnj_distances_row_slab(rnj_context* ctx, const rnj_alignment* aln, int model, unsigned row0,
-                           unsigned rows, float** out_rows, float* max_out) {
-  return rows_handoff(ctx, aln, model, row0, rows, out_rows, max_out, false);
+int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
+                              rnj_row_sink sink, void* user, float* max_out) {
+  if (!aln || !sink) return fail(RNJ_E_INVALID, "null argument");
+  const unsigned n = aln->n;
+  return rows_handoff(ctx, aln, model, max_out, [sink, user, n](unsigned row, const float* v) {
+    if (sink(user, row, v, n) != 0) return fail(RNJ_E_CANCELLED, "row sink stopped the stream");
+    return (int)RNJ_OK;
+  });
 }
 
 // ---- bootstrap ------------------------------------------------------------------------------------

@@ -57,7 +57,7 @@ SIGNATURES = {
                                             ctypes.c_int, _vp, _vp]),
     "rnj_counts_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rnj_distances_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
-    "rnj_distances_row_slab": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u, _u, _vp, _vp]),
+    "rnj_distances_stream_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "rnj_bootstrap_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _u, ctypes.c_int, _vp,
                                             _sz, _sz, ctypes.c_int, _vp]),
     "rnj_server_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
@@ -92,6 +92,14 @@ def lib() -> ctypes.CDLL:
     """Load librapidnj_amd.so; raises if it has not been built (no fallback exists)."""
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime under the same sonames (libamdhip64.so.7,
+        # libhsa-runtime64.so.1). Loading torch first makes the dynamic linker bind this library
+        # to that one runtime; loaded the other way round a process would hold two HSA runtimes
+        # and torch could no longer see the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with "
                               "`python -c 'import __graft_entry__ as g; g.build()'` "

@@ -1,0 +1,68 @@
+"""The C++ host classes (include/rapidnj_amd_distance.hpp) used the way rapidNJ's main uses
+KimuraDistance / JCdistance: in-memory rows, caller-owned (bootstrap) rows and the disk-matrix
+row stream. Expected matrices are the reference engine's own (tests/golden/)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "cpp", "_build", "adapter_test")
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def _write_input(path, d):
+    aln = np.ascontiguousarray(d["aln"], dtype=np.uint8)
+    n, L = aln.shape
+    if "bits" in d:
+        bits, gaps = d["bits"], d["gaps"]
+    else:
+        from oracle import oracle as O
+        bits, gaps, _ = O.pack(aln)
+    with open(path, "wb") as f:
+        np.array([n, L, bits.shape[1] // 4], dtype=np.uint32).tofile(f)
+        aln.tofile(f)
+        np.ascontiguousarray(bits, dtype=np.uint32).tofile(f)
+        np.ascontiguousarray(gaps, dtype=np.uint32).tofile(f)
+
+
+def _run(tmp_path, d, model, mode, source):
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_input(inp, d)
+    r = subprocess.run([DRIVER, str(inp), str(out), model, mode, source], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint32)
+    n = int(raw[0])
+    mat = raw[1:1 + n * n].view(np.float32).reshape(n, n)
+    mx = raw[1 + n * n:].view(np.float32)[0]
+    return mat, mx
+
+
+@pytest.mark.parametrize("case", ["lgt70.npz", "c1.npz", "saturated.npz", "edge_L129.npz"])
+@pytest.mark.parametrize("model", ["kim", "jc"])
+@pytest.mark.parametrize("mode", ["mem", "disk", "reuse"])
+def test_host_classes_match_reference(tmp_path, case, model, mode):
+    if not os.path.exists(DRIVER):
+        pytest.fail(f"{DRIVER} missing: run build()")
+    if mode == "reuse" and model == "jc":
+        pytest.skip("the reference's JCdistance has no caller-rows constructor")
+    d = dict(np.load(os.path.join(GOLDEN, case), allow_pickle=False))
+    ref = d["k2p" if model == "kim" else "jc"]
+    mat, mx = _run(tmp_path, d, model, mode, "fast")
+    np.testing.assert_allclose(mat, ref, rtol=1e-6, atol=0)
+    np.testing.assert_array_equal(np.signbit(mat), np.signbit(ref))
+    off = ~np.eye(mat.shape[0], dtype=bool)
+    if np.any(ref[off] == np.float32(2) * np.float32(mx)):   # saturated pairs carry 2 * max
+        assert mx > 0
+
+
+def test_character_rows_path(tmp_path):
+    """fastdist off: the loader's character rows are packed on the device (uppercase ACGT and
+    '-' give the same counts as the bit strings)."""
+    d = dict(np.load(os.path.join(GOLDEN, "c1.npz"), allow_pickle=False))
+    mat, _ = _run(tmp_path, d, "kim", "mem", "chars")
+    np.testing.assert_allclose(mat, d["k2p"], rtol=1e-6, atol=0)
