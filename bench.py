@@ -68,27 +68,148 @@ def cpu_baseline(aln, model: str, sample_n: int, threads: int) -> dict:
                       f"compute+post-process {secs:.2f} s"}
 
 
+def _gen_job(job):
+    from rapidnj_amd.workloads import simulate_alignment
+    return simulate_alignment(job.n, job.L, job.gap, job.seed)
+
+
+def generate_stream(jobs, procs: int):
+    """Alignments of the served stream, generated in a process pool (before any GPU use)."""
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(procs) as pool:
+        return pool.map(_gen_job, jobs, chunksize=1)
+
+
+def served_run(ctx, jobs, alns, open_loop: bool, slots: int = 2, max_batch: int = 64) -> dict:
+    """Submit the jobs (at their Poisson arrival times, or all at once) and wait for each in order.
+    Latency = submit -> post-processed triangle on the host."""
+    import threading
+    import numpy as np
+    srv = ctx.server(slots=slots, max_batch_jobs=max_batch)
+    tickets = [None] * len(jobs)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for k, j in enumerate(jobs):
+            if open_loop:
+                delay = t0 + j.arrival_s - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            tickets[k] = srv.submit(alns[k], j.model)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat, batch_jobs = [], []
+    for k, j in enumerate(jobs):
+        while tickets[k] is None:
+            time.sleep(0.0002)
+        _, st = srv.wait(tickets[k], j.n)
+        lat.append(st.total_s)
+        batch_jobs.append(st.batch_jobs)
+    makespan = time.perf_counter() - t0
+    th.join()
+    srv.close()
+    sp = sum(j.n * (j.n - 1) // 2 * j.L for j in jobs)
+    lat_ms = np.array(lat) * 1e3
+    return {"jobs": len(jobs), "makespan_s": round(makespan, 3),
+            "value": round(sp / makespan / 1e9, 2), "unit": "G site-pairs/s",
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
+            "mean_batch_jobs": round(float(np.mean(batch_jobs)), 2)}
+
+
+def one_job_per_launch(ctx, jobs, alns) -> dict:
+    """The same jobs one at a time: upload + pack, one distance launch, triangle to the host."""
+    import torch
+    t0 = time.perf_counter()
+    for j, a in zip(jobs, alns):
+        A = ctx.alignment(a)
+        ctx.distances(A, j.model).cpu()
+        A.close()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    sp = sum(j.n * (j.n - 1) // 2 * j.L for j in jobs)
+    return {"makespan_s": round(secs, 3), "value": round(sp / secs / 1e9, 2),
+            "unit": "G site-pairs/s"}
+
+
+def bootstrap_compare(ctx, dev) -> dict:
+    """1000 x 5000 K2P, 100 column-weight replicates: one launch vs one launch per replicate."""
+    import torch
+    from rapidnj_amd.workloads import WORKLOADS, bootstrap_weights, simulate_alignment
+    wl = WORKLOADS["bootstrap"]
+    aln = simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    A = ctx.alignment(aln)
+    w = bootstrap_weights(wl.L, wl.replicates, seed0=0)
+    pairs = wl.n * (wl.n - 1) // 2
+    out = torch.empty((wl.replicates, pairs), dtype=torch.float32, device=dev)
+    for _ in range(2):
+        ctx.bootstrap(A, w, wl.model, out=out)
+    torch.cuda.synchronize()
+    k = 5
+    t0 = time.perf_counter()
+    for _ in range(k):
+        ctx.bootstrap(A, w, wl.model, out=out)
+    torch.cuda.synchronize()
+    batched = (time.perf_counter() - t0) / k
+    ref = out.clone()
+    for r in range(wl.replicates):
+        ctx.bootstrap(A, w[r:r + 1], wl.model, out=out[r:r + 1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        for r in range(wl.replicates):
+            ctx.bootstrap(A, w[r:r + 1], wl.model, out=out[r:r + 1])
+    torch.cuda.synchronize()
+    single = (time.perf_counter() - t0) / 2
+    same = bool(torch.equal(out, ref))
+    sp = pairs * wl.L * wl.replicates
+    A.close()
+    return {"workload": wl.name, "batched_ms": round(batched * 1e3, 3),
+            "batched_value": round(sp / batched / 1e9, 2),
+            "one_per_launch_ms": round(single * 1e3, 3),
+            "one_per_launch_value": round(sp / single / 1e9, 2),
+            "speedup": round(single / batched, 2), "identical_results": same,
+            "unit": "G site-pairs/s"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large",
-                    choices=["large", "plain", "small_jc", "bootstrap"])
+                    choices=["large", "plain", "small_jc", "bootstrap", "served"])
+    ap.add_argument("--served-jobs", type=int, default=0,
+                    help="jobs of the served stream (default: 512 for --workload served, 128 as "
+                         "the side measurement of the default run)")
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 2000 x 10000 side run")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (a known-traffic kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per "
               "box); run with --gpus 1", file=sys.stderr)
         return 2
 
-    import numpy as np
+    import numpy as np  # noqa: F401
     import torch
 
     from rapidnj_amd import native as N
-    from rapidnj_amd.workloads import WORKLOADS, bootstrap_weights, simulate_alignment
+    from rapidnj_amd.workloads import WORKLOADS, bootstrap_weights, served_stream, simulate_alignment
+
+    served_jobs, served_alns = [], []
+    if args.workload == "served" or not args.no_extra:
+        count = args.served_jobs or (512 if args.workload == "served" else 128)
+        served_jobs = served_stream()[:count]
+        log(f"served stream: generating {count} jobs in a process pool")
+        served_alns = generate_stream(served_jobs, cpu_cores())
+
+    if args.workload == "served":
+        return served_main(args, served_jobs, served_alns)
 
     wl = WORKLOADS[args.workload]
     log(f"workload {wl.name}: generating {wl.n} x {wl.L} (seed {wl.seed})")
@@ -156,6 +277,17 @@ def main() -> int:
                 "hbm": {"achieved": round(algo_bytes / (kern_ms / 1e3) / 1e9, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "algo_bytes": algo_bytes}}
 
+    if args.calibrate_copy_mib:
+        nbytes = args.calibrate_copy_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB "
+            "written by one copy kernel")
+        del src, dst
+
     extra = {}
     if not args.no_extra and args.workload == "large":
         pw = WORKLOADS["plain"]
@@ -177,11 +309,21 @@ def main() -> int:
                                          "value": round(p_pairs * pw.L / (p_ms / 1e3) / 1e9, 2),
                                          "unit": "G site-pairs/s"}
         PA.close()
+        del p_out
+        log("side run: batched bootstrap vs one launch per replicate")
+        extra["bootstrap"] = bootstrap_compare(ctx, dev)
+        log(f"side run: served stream, {len(served_jobs)} jobs")
+        extra["served"] = {
+            "stream": f"first {len(served_jobs)} jobs of the 512-job stream (Poisson 20/s; "
+                      "taxa log-uniform 50-5000, sites 500-30000, gaps 0-8%, JC/K2P 50/50)",
+            "open_loop": served_run(ctx, served_jobs, served_alns, open_loop=True),
+            "closed_loop": served_run(ctx, served_jobs, served_alns, open_loop=False),
+            "one_job_per_launch": one_job_per_launch(ctx, served_jobs, served_alns)}
 
     base = None
     if not args.no_cpu_baseline:
         threads = cpu_cores()
-        sample_n = min(n, 8000) if L >= 5000 else min(n, 12000)
+        sample_n = min(n, 20000)  # the whole 20000 x 5000 workload: ~4 s on 16 host threads
         log(f"CPU baseline on {sample_n} taxa with {threads} threads")
         base = cpu_baseline(aln, wl.model, sample_n, threads)
 
@@ -199,6 +341,29 @@ def main() -> int:
         line["extra"] = extra
     print(json.dumps(line), flush=True)
     A.close()
+    ctx.close()
+    return 0
+
+
+def served_main(args, jobs, alns) -> int:
+    """--workload served: the served stream as the measured workload."""
+    from rapidnj_amd import native as N
+    ctx = N.Context(0)
+    warm = served_run(ctx, jobs[:8], alns[:8], open_loop=False)  # tunes shape classes once
+    log(f"warm: {warm}")
+    opened = served_run(ctx, jobs, alns, open_loop=True)
+    closed = served_run(ctx, jobs, alns, open_loop=False)
+    single = one_job_per_launch(ctx, jobs, alns)
+    line = {
+        "metric": METRIC, "value": closed["value"], "unit": "G site-pairs/s", "n_gpus": 1,
+        "steps": len(jobs), "warmup": 8, "ms_per_step": round(closed["makespan_s"] * 1e3 / len(jobs), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded generator per job)",
+        "config": {"workload": f"served stream, {len(jobs)} jobs, Poisson 20/s",
+                   "parallelism": "single GPU"},
+        "served": {"open_loop": opened, "closed_loop": closed, "one_job_per_launch": single},
+    }
+    print(json.dumps(line), flush=True)
     ctx.close()
     return 0
 

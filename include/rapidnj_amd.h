@@ -59,7 +59,10 @@ enum {
 
 /* flags for rnj_distances_device / rnj_bootstrap_device */
 enum {
-  RNJ_FLAG_RAW = 1 /* keep -1 for saturated pairs (no 2*max post-process) */
+  RNJ_FLAG_RAW = 1,  /* keep -1 for saturated pairs (no 2*max post-process) */
+  RNJ_FLAG_EXACT = 2 /* the library's compensated FP64 log (<= 1 ulp). Default: a leaner FP64
+                        log (~2 ulp of double), which rounds to the same float except in rare
+                        ties; counts, saturated pairs and zeros are exact either way */
 };
 
 typedef struct rnj_context rnj_context;
@@ -152,7 +155,8 @@ RNJ_EXPORT int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uin
 /* Host hand-off: rows[i] are n caller-owned arrays of n floats (e.g. new distType[n]), filled
  * with the post-processed symmetric matrix, diagonal 0, sentinels patched: the in-memory matrix
  * handed to distMatrixReader (src/main.cpp:419-437, distMatrixReader.cpp:14-23). Synchronous.
- * max_out (host, may be null) receives the maximum used for the sentinel. */
+ * max_out (host, may be null) receives the maximum used for the sentinel. The host hand-offs
+ * (this, rnj_distances_stream_rows and the group-1 entry points) use the exact FP64 logs. */
 RNJ_EXPORT int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
                                   float** rows, float* max_out);
 
@@ -182,6 +186,8 @@ typedef struct {
   unsigned slots;           /* batches in flight (default 2) */
   unsigned max_batch_jobs;  /* jobs per launch (default 64) */
   size_t hbm_budget;        /* bytes the server may hold; 0 = 80% of free HBM at creation */
+  unsigned batch_window_us; /* hold a batch up to this long after its first job is ready, to
+                               gather more jobs into the launch (default 0: launch at once) */
 } rnj_server_config;
 
 typedef struct {

@@ -28,8 +28,10 @@ enum Model : int { kJC = 0, kK2P = 1, kCounts = 2 };
 enum Layout : int { kTriangle = 0, kSquare = 1 };
 
 // One distance matrix inside a launch. A launch covers jobs[0..njobs); the work items of job k
-// (tiles of TI x TJ taxon pairs over the upper triangle of tile blocks) are numbered
-// item_base .. item_base + tiles - 1.
+// (tiles of TI x TJ taxon pairs over the upper triangle of tile blocks, each cut into `splits`
+// ranges of words) are numbered item_base .. item_base + tiles * splits - 1, item = tile * splits
+// + split. A split job leaves exact partial counts in `partial` and the finish kernel, numbering
+// that job's tiles from fin_base, sums them and applies the epilogue.
 struct JobDesc {
   const uint32_t* planes;     // [3][W_pad][n_pad]
   unsigned long long plane_stride;  // W_pad * n_pad
@@ -37,6 +39,10 @@ struct JobDesc {
   unsigned nblk;              // ceil(n / TI) for the launch's tile edge (TI == TJ)
   unsigned tiles;             // nblk * (nblk + 1) / 2
   unsigned item_base;
+  unsigned splits;            // word ranges per tile (1 = finish in the tile kernel)
+  unsigned words_per_split;   // multiple of the configuration's chunk
+  unsigned fin_base;          // first finish-kernel item of this job
+  uint32_t* partial;          // [tiles][splits][3][threads][micro_i * micro_j] when splits > 1
   int model;                  // Model
   int layout;                 // Layout (distances only)
   unsigned long long ld;      // row stride in elements for kSquare
@@ -75,8 +81,10 @@ hipError_t launch_pack_legacy(const uint32_t* bits, const uint32_t* gaps, size_t
                               unsigned words16, unsigned n, unsigned n_pad, unsigned W_pad,
                               uint32_t* planes, hipStream_t stream);
 hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
-hipError_t launch_tiles(int config, bool jc_only, const JobDesc* jobs, unsigned njobs,
-                        unsigned total_items, hipStream_t stream);
+// exact: every log in FP64 as the reference writes it; otherwise FP32 logs away from saturation.
+hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
+                        unsigned total_items, unsigned finish_items, hipStream_t stream);
+unsigned tile_threads(int config);  // work-group size of a configuration
 hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
 hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
                                   unsigned L, uint8_t* codes, hipStream_t stream);

@@ -17,6 +17,8 @@ const TileConfig kTileConfigs[] = {
     {"t64_m4x4_c8", 64, 4, 4, 8},
     {"t128_m8x8_c16", 128, 8, 8, 16},
     {"t128_m8x4_c16", 128, 8, 4, 16},
+    {"t32_m4x4_c16", 32, 4, 4, 16},
+    {"t32_m4x4_c8", 32, 4, 4, 8},
 };
 const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 
@@ -44,30 +46,85 @@ using gptr = const __attribute__((address_space(1))) T*;
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 // ---- corrections ---------------------------------------------------------------------------
-// Kimura 2-parameter, KimuraDistance.cpp:339-348. The proportions are rounded to float first,
-// then promoted; the logs are double; -1 marks saturation and valid == 0.
+// Kimura 2-parameter (KimuraDistance.cpp:339-348) and Jukes-Cantor with alpha = 4
+// (JCdistance.cpp:170-183). What decides the result's class is kept exactly as the reference
+// computes it:
+//  * the proportions: (float)((double)a / b) of integers below 2^24 equals the correctly rounded
+//    float quotient (a double rounding 53 -> 24 bits is innocuous for a quotient), so the float
+//    division is used, and the double one above 2^24;
+//  * the arguments of the logarithms, 1-2P-Q and 1-2Q (K2P) and 1-p/0.75 (JC), formed in double
+//    from the float proportions in the reference's order (exact for K2P), and the saturation
+//    tests on them, so the saturated pairs (-1) are the same;
+//  * zero distances: ts = tv = 0 gives -0.0f in both models, as the reference's -c*log(1.0) does.
+// The logarithms of those double arguments use log_unit (below) unless kExact, which calls the
+// library's compensated FP64 log; either way the reference's FP64 expression is evaluated.
 #pragma clang fp contract(off)
-__device__ __noinline__ float k2p_distance(uint32_t ts, uint32_t tv, uint32_t valid) {
-  float d = -1.0f;
-  if (valid != 0) {
-    const float P = (float)((double)ts / (double)valid);
-    const float Q = (float)((double)tv / (double)valid);
-    const float t1 = (float)(1.0 - 2.0 * (double)P - (double)Q);
-    const float t2 = (float)(1.0 - 2.0 * (double)Q);
-    if (!(t1 <= 0.0f || t2 <= 0.0f))
-      d = (float)(-0.5 * log(1.0 - 2.0 * (double)P - (double)Q) - 0.25 * log(1.0 - 2.0 * (double)Q));
-  }
-  return d;
+__device__ __forceinline__ float proportion(uint32_t a, uint32_t valid) {
+  if (valid < (1u << 24)) return (float)a / (float)valid;
+  return (float)((double)a / (double)valid);
 }
 
-// Jukes-Cantor with alpha = 4, JCdistance.cpp:170-183: float proportion, double log; valid == 0
-// gives p = 0 (and so -0.0f), not a sentinel.
+// Natural log for 0 < x <= 1 (a normal double), in plain FP64 arithmetic: x = 2^k * m with
+// m in [sqrt(1/2), sqrt(2)), s = (m - 1) / (m + 1), log(m) = 2 atanh(s) = 2s (1 + s^2/3 + ... +
+// s^20/21), which leaves a truncation below 2^-56 for |s| <= 0.1716, and k ln 2 with ln 2 split
+// into a 32-bit head (k * head is exact) and a tail. The quotient uses the hardware reciprocal
+// refined by two Newton steps. The result is within about 2 ulp of double, so after rounding to
+// float it equals the reference's float value except in rare ties to a float rounding boundary;
+// it costs about a third of the instructions of the compensated library log.
+__device__ __forceinline__ double log_unit(double x) {
+  const double kLn2Hi = 0x1.62e42ff000000p-1;
+  const double kLn2Lo = -0x1.718432a1b0e26p-35;
+  int k;
+  double m = frexp(x, &k);            // m in [0.5, 1)
+  if (m < 0.70710678118654757) {      // m in [sqrt(1/2), sqrt(2))
+    m = m * 2.0;
+    k -= 1;
+  }
+  const double f = m - 1.0;           // exact
+  const double d = 2.0 + f;
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  const double sd = f * r;
+  const double t = sd * sd;
+  double p = 1.0 / 21.0;
+  p = fma(p, t, 1.0 / 19.0);
+  p = fma(p, t, 1.0 / 17.0);
+  p = fma(p, t, 1.0 / 15.0);
+  p = fma(p, t, 1.0 / 13.0);
+  p = fma(p, t, 1.0 / 11.0);
+  p = fma(p, t, 1.0 / 9.0);
+  p = fma(p, t, 1.0 / 7.0);
+  p = fma(p, t, 1.0 / 5.0);
+  p = fma(p, t, 1.0 / 3.0);
+  const double two_s = 2.0 * sd;
+  const double lm = fma(two_s * t, p, two_s);
+  const double kd = (double)k;
+  return fma(kd, kLn2Hi, fma(kd, kLn2Lo, lm));
+}
+
+template <bool kExact>
+__device__ __noinline__ float k2p_distance(uint32_t ts, uint32_t tv, uint32_t valid) {
+  if (valid == 0) return -1.0f;
+  if ((ts | tv) == 0) return -0.0f;
+  const float P = proportion(ts, valid);
+  const float Q = proportion(tv, valid);
+  const double x = 1.0 - 2.0 * (double)P - (double)Q;
+  const double y = 1.0 - 2.0 * (double)Q;
+  if ((float)x <= 0.0f || (float)y <= 0.0f) return -1.0f;
+  if (kExact) return (float)(-0.5 * log(x) - 0.25 * log(y));
+  return (float)(-0.5 * log_unit(x) - 0.25 * log_unit(y));
+}
+
+template <bool kExact>
 __device__ __noinline__ float jc_distance(uint32_t mismatches, uint32_t valid) {
   const double b = 0.75;  // (alpha - 1) / alpha
-  float p = 0.0f;
-  if (valid != 0) p = (float)mismatches / (float)valid;
+  if (valid == 0 || mismatches == 0) return -0.0f;
+  const float p = proportion(mismatches, valid);
   if ((double)p / b >= 1.0) return -1.0f;
-  return (float)(-b * log(1.0 - (double)p / b));
+  const double x = 1.0 - (double)p / b;
+  if (kExact) return (float)(-b * log(x));
+  return (float)(-b * log_unit(x));
 }
 #pragma clang fp contract(on)
 
@@ -122,13 +179,74 @@ __device__ __forceinline__ void fetch_chunk(v4u (&pI)[EP], v4u (&pJ)[EP], gptr<v
   }
 }
 
+// ---- epilogue shared by the tile kernel and the split-finish kernel ------------------------
+// Thread (ty, tx) of tile (bi, bj) holds the counts of its MI x MJ pairs: a = tv (mismatches for
+// the JC-only loop), b = ts, v = valid. Writes distances (or counts), records saturated pairs and
+// folds the wave's maximum into the job's running maximum.
+template <int TILE, int MI, int MJ, bool JC6, bool kExact>
+__device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, unsigned bj,
+                                              unsigned ty, unsigned tx, unsigned tid,
+                                              const uint32_t (&acc_a)[MI][MJ],
+                                              const uint32_t (&acc_b)[MI][MJ],
+                                              const uint32_t (&acc_v)[MI][MJ]) {
+  const unsigned n = J.n;
+  const int model = J.model;
+  float lmax = 0.0f;
+#pragma unroll
+  for (int a = 0; a < MI; ++a) {
+    const unsigned i = bi * TILE + ty * MI + a;
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) {
+      const unsigned j = bj * TILE + tx * MJ + b;
+      if (j >= n) continue;
+      if (i < j) {
+        if (model == kCounts) {
+          const unsigned long long p = tri_index(i, j, n);
+          J.out_tv[p] = acc_a[a][b];
+          J.out_ts[p] = acc_b[a][b];
+          J.out_valid[p] = acc_v[a][b];
+          continue;
+        }
+        float d;
+        if (model == kJC)
+          d = jc_distance<kExact>(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b], acc_v[a][b]);
+        else
+          d = k2p_distance<kExact>(acc_b[a][b], acc_a[a][b], acc_v[a][b]);
+        if (d == -1.0f) push_sentinel(J, i, j);
+        else if (d > lmax) lmax = d;
+        if (J.layout == kSquare) {
+          J.out[(size_t)i * J.ld + j] = d;
+          J.out[(size_t)j * J.ld + i] = d;
+        } else {
+          J.out[tri_index(i, j, n)] = d;
+        }
+      } else if (i == j && model != kCounts && J.layout == kSquare) {
+        J.out[(size_t)i * J.ld + i] = 0.0f;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
+  if ((tid & 63) == 0 && lmax > 0.0f) atomicMax(J.maxbits, __float_as_uint(lmax));
+}
+
+// Partial counts of one (tile, split): [3][NT][MI*MJ] u32, thread-major so that every thread
+// stores and loads its own contiguous 4*MI*MJ bytes.
+template <int NT, int MI, int MJ>
+__device__ __forceinline__ uint32_t* partial_slot(const JobDesc& J, unsigned tile,
+                                                  unsigned split) {
+  return J.partial + ((size_t)tile * J.splits + split) * 3 * NT * MI * MJ;
+}
+
 // ---- the pair-tile kernel ------------------------------------------------------------------
-// One work-group computes a TILE x TILE block of taxon pairs (bi <= bj); thread (ty, tx) owns the
-// MI x MJ micro-tile i = bi*TILE + ty*MI + a, j = bj*TILE + tx*MJ + b. Sites are walked in chunks
-// of CW words: the chunk's H/L/V words of both taxon blocks are staged in LDS (layout
+// One work-group computes a TILE x TILE block of taxon pairs (bi <= bj) over one range of words
+// (the whole alignment, or split `split` of J.splits); thread (ty, tx) owns the MI x MJ
+// micro-tile i = bi*TILE + ty*MI + a, j = bj*TILE + tx*MJ + b. Sites are walked in chunks of CW
+// words: the chunk's H/L/V words of both taxon blocks are staged in LDS (layout
 // [plane][word][taxon], written linearly), the next chunk is prefetched into registers while
-// the current one is counted.
-template <int TILE, int MI, int MJ, int CW, bool JC6>
+// the current one is counted. Unsplit jobs finish in place; split jobs leave exact partial
+// counts for finish_kernel.
+template <int TILE, int MI, int MJ, int CW, bool JC6, bool kExact>
 __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
     const JobDesc* __restrict__ jobs, unsigned njobs) {
   constexpr int NTX = TILE / MJ;
@@ -144,8 +262,10 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
   const unsigned item = xcd_item(blockIdx.x, gridDim.x);
   const unsigned k = find_job(jobs, njobs, item);
   const JobDesc& J = jobs[k];
+  const unsigned local = item - J.item_base;
+  const unsigned tile = local / J.splits, split = local % J.splits;
   unsigned bi, bj;
-  tile_coords(item - J.item_base, J.nblk, bi, bj);
+  tile_coords(tile, J.nblk, bi, bj);
 
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
@@ -160,10 +280,13 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
 #pragma unroll
     for (int b = 0; b < MJ; ++b) acc_a[a][b] = acc_b[a][b] = acc_v[a][b] = 0;
 
+  const unsigned w0 = split * J.words_per_split;
+  const unsigned w1 = min(J.W, w0 + J.words_per_split);
+  const unsigned c0 = w0 / CW, c1 = w1 / CW;             // word ranges are multiples of CW
+
   v4u pI[EP], pJ[EP];
-  const unsigned nchunks = J.W / CW;
-  if (nchunks > 0) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, 0, row4, plane4);
-  for (unsigned c = 0; c < nchunks; ++c) {
+  if (c0 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c0, row4, plane4);
+  for (unsigned c = c0; c < c1; ++c) {
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EP; ++e) {
@@ -172,7 +295,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
       sJ[tid + e * NT] = pJ[e];
     }
     __syncthreads();
-    if (c + 1 < nchunks) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
+    if (c + 1 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
 
 #pragma unroll 4
     for (int w = 0; w < CW; ++w) {
@@ -214,56 +337,78 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
     }
   }
 
-  // ---- epilogue: correction, output, running max, saturated pairs -------------------------
-  const unsigned n = J.n;
-  const int model = J.model;
-  float lmax = 0.0f;
+  if (J.splits > 1) {
+    uint32_t* slot = partial_slot<NT, MI, MJ>(J, tile, split) + (size_t)tid * MI * MJ;
+    const size_t plane = (size_t)NT * MI * MJ;
 #pragma unroll
-  for (int a = 0; a < MI; ++a) {
-    const unsigned i = bi * TILE + ty * MI + a;
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
-    for (int b = 0; b < MJ; ++b) {
-      const unsigned j = bj * TILE + tx * MJ + b;
-      if (j >= n) continue;
-      if (i < j) {
-        if (model == kCounts) {
-          const unsigned long long p = tri_index(i, j, n);
-          J.out_tv[p] = acc_a[a][b];
-          J.out_ts[p] = acc_b[a][b];
-          J.out_valid[p] = acc_v[a][b];
-          continue;
-        }
-        float d;
-        if (model == kJC)
-          d = jc_distance(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b], acc_v[a][b]);
-        else
-          d = k2p_distance(acc_b[a][b], acc_a[a][b], acc_v[a][b]);
-        if (d == -1.0f) push_sentinel(J, i, j);
-        else if (d > lmax) lmax = d;
-        if (J.layout == kSquare) {
-          J.out[(size_t)i * J.ld + j] = d;
-          J.out[(size_t)j * J.ld + i] = d;
-        } else {
-          J.out[tri_index(i, j, n)] = d;
-        }
-      } else if (i == j && model != kCounts && J.layout == kSquare) {
-        J.out[(size_t)i * J.ld + i] = 0.0f;
+      for (int b = 0; b < MJ; ++b) {
+        slot[a * MJ + b] = acc_a[a][b];
+        slot[plane + a * MJ + b] = acc_b[a][b];
+        slot[2 * plane + a * MJ + b] = acc_v[a][b];
       }
-    }
+    return;
   }
+  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
+}
+
+// Sums the partial counts of the splits of one tile of a split job and finishes the tile. Work
+// items are numbered per job from fin_base over the fin_tiles (= tiles when split, else 0).
+template <int TILE, int MI, int MJ, bool JC6, bool kExact>
+__global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
+    const JobDesc* __restrict__ jobs, unsigned njobs) {
+  constexpr int NTX = TILE / MJ;
+  constexpr int NT = (TILE / MI) * NTX;
+  const unsigned item = blockIdx.x;
+  unsigned lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi + 1) / 2;
+    if (jobs[mid].fin_base <= item) lo = mid; else hi = mid - 1;
+  }
+  const JobDesc& J = jobs[lo];
+  const unsigned tile = item - J.fin_base;
+  if (J.splits <= 1 || tile >= J.tiles) return;
+  unsigned bi, bj;
+  tile_coords(tile, J.nblk, bi, bj);
+  const unsigned tid = threadIdx.x;
+  const unsigned ty = tid / NTX, tx = tid % NTX;
+  uint32_t acc_a[MI][MJ], acc_b[MI][MJ], acc_v[MI][MJ];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
-  if ((tid & 63) == 0 && lmax > 0.0f) atomicMax(J.maxbits, __float_as_uint(lmax));
+  for (int a = 0; a < MI; ++a)
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) acc_a[a][b] = acc_b[a][b] = acc_v[a][b] = 0;
+  const size_t plane = (size_t)NT * MI * MJ;
+  for (unsigned sp = 0; sp < J.splits; ++sp) {
+    const uint32_t* slot = partial_slot<NT, MI, MJ>(J, tile, sp) + (size_t)tid * MI * MJ;
+#pragma unroll
+    for (int a = 0; a < MI; ++a)
+#pragma unroll
+      for (int b = 0; b < MJ; ++b) {
+        acc_a[a][b] += slot[a * MJ + b];
+        acc_b[a][b] += slot[plane + a * MJ + b];
+        acc_v[a][b] += slot[2 * plane + a * MJ + b];
+      }
+  }
+  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
 }
 
 template <int TILE, int MI, int MJ, int CW>
-hipError_t launch_cfg(bool jc6, const JobDesc* jobs, unsigned njobs, unsigned total,
-                      hipStream_t s) {
+hipError_t launch_cfg(bool jc6, bool exact, const JobDesc* jobs, unsigned njobs, unsigned total,
+                      unsigned finish_items, hipStream_t s) {
   constexpr int NT = (TILE / MI) * (TILE / MJ);
-  if (jc6)
-    tile_kernel<TILE, MI, MJ, CW, true><<<dim3(total), dim3(NT), 0, s>>>(jobs, njobs);
-  else
-    tile_kernel<TILE, MI, MJ, CW, false><<<dim3(total), dim3(NT), 0, s>>>(jobs, njobs);
+  const dim3 grid(total), block(NT);
+  if (jc6 && exact) tile_kernel<TILE, MI, MJ, CW, true, true><<<grid, block, 0, s>>>(jobs, njobs);
+  else if (jc6) tile_kernel<TILE, MI, MJ, CW, true, false><<<grid, block, 0, s>>>(jobs, njobs);
+  else if (exact) tile_kernel<TILE, MI, MJ, CW, false, true><<<grid, block, 0, s>>>(jobs, njobs);
+  else tile_kernel<TILE, MI, MJ, CW, false, false><<<grid, block, 0, s>>>(jobs, njobs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || finish_items == 0) return e;
+  const dim3 fgrid(finish_items);
+  if (jc6 && exact) finish_kernel<TILE, MI, MJ, true, true><<<fgrid, block, 0, s>>>(jobs, njobs);
+  else if (jc6) finish_kernel<TILE, MI, MJ, true, false><<<fgrid, block, 0, s>>>(jobs, njobs);
+  else if (exact) finish_kernel<TILE, MI, MJ, false, true><<<fgrid, block, 0, s>>>(jobs, njobs);
+  else finish_kernel<TILE, MI, MJ, false, false><<<fgrid, block, 0, s>>>(jobs, njobs);
   return hipGetLastError();
 }
 
@@ -493,16 +638,24 @@ hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_tiles(int config, bool jc_only, const JobDesc* jobs, unsigned njobs,
-                        unsigned total_items, hipStream_t stream) {
+hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
+                        unsigned total_items, unsigned finish_items, hipStream_t stream) {
   if (total_items == 0) return hipSuccess;
+  const unsigned f = finish_items;
   switch (config) {
-    case 0: return launch_cfg<64, 4, 4, 16>(jc_only, jobs, njobs, total_items, stream);
-    case 1: return launch_cfg<64, 4, 4, 8>(jc_only, jobs, njobs, total_items, stream);
-    case 2: return launch_cfg<128, 8, 8, 16>(jc_only, jobs, njobs, total_items, stream);
-    case 3: return launch_cfg<128, 8, 4, 16>(jc_only, jobs, njobs, total_items, stream);
+    case 0: return launch_cfg<64, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 1: return launch_cfg<64, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 2: return launch_cfg<128, 8, 8, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 3: return launch_cfg<128, 8, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 4: return launch_cfg<32, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 5: return launch_cfg<32, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+unsigned tile_threads(int config) {
+  const TileConfig& c = kTileConfigs[config];
+  return (c.tile / c.micro_i) * (c.tile / c.micro_j);
 }
 
 hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream) {

@@ -66,12 +66,26 @@ void free_workspace(Workspace* ws) {
   if (ws->d_jobs) (void)hipFree(ws->d_jobs);
   if (ws->d_stats) (void)hipFree(ws->d_stats);
   if (ws->d_sent) (void)hipFree(ws->d_sent);
+  if (ws->d_partial) (void)hipFree(ws->d_partial);
   ws->h_jobs = nullptr;
   ws->d_jobs = nullptr;
   ws->d_stats = nullptr;
   ws->d_sent = nullptr;
+  ws->d_partial = nullptr;
   ws->cap_jobs = 0;
   ws->cap_sent = 0;
+  ws->cap_partial = 0;
+}
+
+int grow_partial(Workspace* ws, size_t elems) {
+  if (elems <= ws->cap_partial) return RNJ_OK;
+  const size_t cap = std::max(elems, ws->cap_partial + ws->cap_partial / 2);
+  if (ws->d_partial) (void)hipFree(ws->d_partial);
+  ws->d_partial = nullptr;
+  ws->cap_partial = 0;
+  RNJ_HIP(hipMalloc((void**)&ws->d_partial, cap * sizeof(uint32_t)));
+  ws->cap_partial = cap;
+  return RNJ_OK;
 }
 
 int grow_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
@@ -133,10 +147,51 @@ int ilog2(unsigned long long x) {
   return r;
 }
 
-// Fill item numbering for tile edge `tile` and bind scratch. Returns total work items.
-unsigned long long number_jobs(std::vector<JobSpec>& specs, unsigned tile, Workspace* ws,
-                               size_t* sent_offsets) {
-  unsigned long long base = 0;
+// Work decomposition of one launch for one tile configuration. When the launch has fewer tiles
+// than about 8 per CU, every job's words are cut into up to 16 ranges (each at least two LDS
+// chunks), so that the device fills; the split tiles are finished by the finish kernel.
+struct LaunchPlan {
+  std::vector<unsigned> splits, wps, tiles;
+  std::vector<size_t> partial_off;
+  size_t partial_elems = 0;
+  unsigned long long items = 0, finish_items = 0;
+};
+
+LaunchPlan plan_launch(const std::vector<JobSpec>& specs, int config, unsigned cus) {
+  const TileConfig& c = kTileConfigs[config];
+  LaunchPlan P;
+  unsigned long long total_tiles = 0;
+  for (const JobSpec& s : specs) {
+    const unsigned nb = (s.n + c.tile - 1) / c.tile;
+    P.tiles.push_back(nb * (nb + 1) / 2);
+    total_tiles += P.tiles.back();
+  }
+  const unsigned long long target = 8ull * std::max(1u, cus);
+  unsigned want = 1;
+  if (total_tiles > 0 && total_tiles < target)
+    want = (unsigned)std::min<unsigned long long>(16, (target + total_tiles - 1) / total_tiles);
+  const size_t per_split = (size_t)3 * c.tile * c.tile;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    const unsigned chunks = specs[k].W / c.chunk;
+    unsigned S = std::max(1u, std::min(want, chunks / 2));
+    const unsigned cps = (chunks + S - 1) / S;  // chunks per split
+    S = (chunks + cps - 1) / cps;
+    P.splits.push_back(S);
+    P.wps.push_back(cps * c.chunk);
+    P.partial_off.push_back(P.partial_elems);
+    if (S > 1) {
+      P.partial_elems += (size_t)P.tiles[k] * S * per_split;
+      P.finish_items += P.tiles[k];
+    }
+    P.items += (unsigned long long)P.tiles[k] * S;
+  }
+  return P;
+}
+
+// Fill the job table for `plan` and bind scratch.
+void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, Workspace* ws,
+                 size_t* sent_caps) {
+  unsigned long long base = 0, fin = 0;
   size_t soff = 0;
   for (size_t k = 0; k < specs.size(); ++k) {
     const JobSpec& s = specs[k];
@@ -147,9 +202,13 @@ unsigned long long number_jobs(std::vector<JobSpec>& specs, unsigned tile, Works
     d.n = s.n;
     d.n_pad = s.n_pad;
     d.W = s.W;
-    d.nblk = (s.n + tile - 1) / tile;
-    d.tiles = d.nblk * (d.nblk + 1) / 2;
+    d.nblk = (s.n + kTileConfigs[config].tile - 1) / kTileConfigs[config].tile;
+    d.tiles = P.tiles[k];
     d.item_base = (unsigned)base;
+    d.splits = P.splits[k];
+    d.words_per_split = P.wps[k];
+    d.fin_base = (unsigned)fin;
+    d.partial = P.splits[k] > 1 ? ws->d_partial + P.partial_off[k] : nullptr;
     d.model = s.model;
     d.layout = s.layout;
     d.ld = s.ld;
@@ -160,21 +219,24 @@ unsigned long long number_jobs(std::vector<JobSpec>& specs, unsigned tile, Works
     d.maxbits = ws->d_stats + 2 * k;
     d.sent_count = ws->d_stats + 2 * k + 1;
     d.sent = ws->d_sent + soff;
-    d.sent_cap = (unsigned)sent_offsets[k];
-    soff += sent_offsets[k];
-    base += d.tiles;
+    d.sent_cap = (unsigned)sent_caps[k];
+    soff += sent_caps[k];
+    base += (unsigned long long)d.tiles * d.splits;
+    if (d.splits > 1) fin += d.tiles;
   }
-  return base;
 }
 
 }  // namespace
 
 void free_workspace_public(Workspace* ws) { free_workspace(ws); }
 
-// Default configuration before (or without) tuning: the 64-edge tile with 4x4 micro-tiles.
+// Configuration before (or without) tuning, from measurements on MI355X: below ~4096 taxa the
+// 32-taxon tile fills the device better (2000 x 10000: 0.21 ms vs 0.24 ms), above it the
+// 64-taxon tile with 8-word chunks is fastest (20000 x 5000: 7.2 ms).
 static int default_config(unsigned n, unsigned W, bool jc_only) {
-  (void)n; (void)W; (void)jc_only;
-  return 0;
+  (void)W;
+  (void)jc_only;
+  return n <= 4096 ? 4 : 1;
 }
 
 int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
@@ -189,25 +251,32 @@ int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
 // launch's shape class. Only for launches big enough that the timing means something and only
 // outside stream capture.
 static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, size_t* sent_caps,
-                    bool jc_only, unsigned n_key, unsigned W_key, hipStream_t stream, int* best) {
+                    bool jc_only, bool exact, unsigned n_key, unsigned W_key, hipStream_t stream,
+                    int* best) {
   hipEvent_t e0, e1;
   RNJ_HIP(hipEventCreate(&e0));
   RNJ_HIP(hipEventCreate(&e1));
   float best_ms = 1e30f;
   *best = default_config(n_key, W_key, jc_only);
+  const unsigned cus = (unsigned)ctx->prop.multiProcessorCount;
   for (int c = 0; c < kNumTileConfigs; ++c) {
-    const unsigned long long items = number_jobs(specs, kTileConfigs[c].tile, ws, sent_caps);
-    if (items >= (1ull << 31)) continue;
+    const LaunchPlan P = plan_launch(specs, c, cus);
+    if (P.items >= (1ull << 31)) continue;
+    RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
+    number_jobs(specs, c, P, ws, sent_caps);
     RNJ_HIP(hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
                            hipMemcpyHostToDevice, stream));
-    float ms = 0;
-    for (int rep = 0; rep < 2; ++rep) {
+    float ms = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
       RNJ_HIP(launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream));
       RNJ_HIP(hipEventRecord(e0, stream));
-      RNJ_HIP(launch_tiles(c, jc_only, ws->d_jobs, (unsigned)specs.size(), (unsigned)items, stream));
+      RNJ_HIP(launch_tiles(c, jc_only, exact, ws->d_jobs, (unsigned)specs.size(),
+                           (unsigned)P.items, (unsigned)P.finish_items, stream));
       RNJ_HIP(hipEventRecord(e1, stream));
       RNJ_HIP(hipEventSynchronize(e1));
-      RNJ_HIP(hipEventElapsedTime(&ms, e0, e1));
+      float t = 0;
+      RNJ_HIP(hipEventElapsedTime(&t, e0, e1));
+      if (rep > 0) ms = std::min(ms, t);  // the first launch of a configuration warms it up
     }
     if (ms < best_ms) {
       best_ms = ms;
@@ -221,7 +290,8 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, si
   return RNJ_OK;
 }
 
-int launch_jobs(Context* ctx, std::vector<JobSpec>& specs, bool raw, hipStream_t stream) {
+int launch_jobs(Context* ctx, std::vector<JobSpec>& specs, bool raw, bool exact,
+                hipStream_t stream, bool allow_tune) {
   if (specs.empty()) return RNJ_OK;
   bool jc_only = true;
   unsigned n_key = 0, W_key = 0;
@@ -252,25 +322,33 @@ int launch_jobs(Context* ctx, std::vector<JobSpec>& specs, bool raw, hipStream_t
     if (!cached) {
       hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
       (void)hipStreamIsCapturing(stream, &cap);
-      // tune only launches worth >= ~2^33 word-pairs (a few ms): small ones keep the default
-      if (cap == hipStreamCaptureStatusNone && work >= (1ull << 34)) {
-        rc = autotune(ctx, specs, ws, caps.data(), jc_only, n_key, W_key, stream, &config);
+      // tune only launches worth >= 2^28 taxon^2-words (~0.1 ms of counting): smaller ones
+      // keep the default
+      if (allow_tune && cap == hipStreamCaptureStatusNone && work >= (1ull << 28)) {
+        rc = autotune(ctx, specs, ws, caps.data(), jc_only, exact, n_key, W_key, stream, &config);
         if (rc != RNJ_OK) { release_workspace(ctx, ws, stream); return rc; }
       } else {
         config = default_config(n_key, W_key, jc_only);
       }
     }
   }
-  const unsigned long long items = number_jobs(specs, kTileConfigs[config].tile, ws, caps.data());
-  if (items >= (1ull << 31)) {
+  const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
+  if (P.items >= (1ull << 31)) {
     release_workspace(ctx, ws, stream);
     return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
   }
+  rc = grow_partial(ws, P.partial_elems);
+  if (rc != RNJ_OK) {
+    release_workspace(ctx, ws, stream);
+    return rc;
+  }
+  number_jobs(specs, config, P, ws, caps.data());
   hipError_t e = hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
                                 hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream);
   if (e == hipSuccess)
-    e = launch_tiles(config, jc_only, ws->d_jobs, (unsigned)specs.size(), (unsigned)items, stream);
+    e = launch_tiles(config, jc_only, exact, ws->d_jobs, (unsigned)specs.size(),
+                     (unsigned)P.items, (unsigned)P.finish_items, stream);
   if (e == hipSuccess && !raw) e = launch_patch(ws->d_jobs, (unsigned)specs.size(), stream);
   for (size_t k = 0; e == hipSuccess && k < specs.size(); ++k)
     if (specs[k].max_out)
@@ -572,7 +650,8 @@ int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, int model, 
   if (rc != RNJ_OK) return rc;
   std::vector<JobSpec> jobs{spec_for(aln, model, layout, out, ld)};
   jobs[0].max_out = max_out;
-  return launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, stream_of(ctx, stream));
+  return launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, (flags & RNJ_FLAG_EXACT) != 0,
+                     stream_of(ctx, stream));
 }
 
 int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, uint32_t* tv,
@@ -586,7 +665,7 @@ int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, 
   jobs[0].out_ts = ts;
   jobs[0].out_tv = tv;
   jobs[0].out_valid = valid;
-  return launch_jobs(ctx, jobs, true, stream_of(ctx, stream));
+  return launch_jobs(ctx, jobs, true, false, stream_of(ctx, stream));
 }
 
 }  // extern "C"
@@ -610,7 +689,7 @@ static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, f
   RNJ_HIP(hipMallocAsync((void**)&dmax, sizeof(float), s));
   std::vector<JobSpec> jobs{spec_for(aln, model, kSquare, dmat, n)};
   jobs[0].max_out = dmax;
-  rc = launch_jobs(ctx, jobs, false, s);
+  rc = launch_jobs(ctx, jobs, false, true, s);
   const size_t slab_rows = std::max<size_t>(1, ((size_t)32 << 20) / ((size_t)n * sizeof(float)));
   float* pinned[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -737,7 +816,7 @@ int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
       j.out = out + (size_t)r * matrix_stride;
       jobs[r] = j;
     }
-    rc = launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, s);
+    rc = launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, (flags & RNJ_FLAG_EXACT) != 0, s);
   } else {
     rc = hip_fail(e, "rnj_bootstrap_device");
   }
@@ -833,7 +912,8 @@ void computeDistancesDNA2_gpu(float* results) {
   float* d = nullptr;
   hipError_t e = hipMalloc((void**)&d, (size_t)n * n * sizeof(float));
   if (e != hipSuccess) legacy_check(hip_fail(e, "hipMalloc"), where);
-  legacy_check(rnj_distances_device(g_k2p.ctx, g_k2p.aln, RNJ_K2P, RNJ_SQUARE, d, n, RNJ_FLAG_RAW,
+  legacy_check(rnj_distances_device(g_k2p.ctx, g_k2p.aln, RNJ_K2P, RNJ_SQUARE, d, n,
+                                    RNJ_FLAG_RAW | RNJ_FLAG_EXACT,
                                     nullptr, nullptr),
                where);
   e = hipMemcpy(results, d, (size_t)n * n * sizeof(float), hipMemcpyDeviceToHost);

@@ -30,6 +30,12 @@ int hip_fail(hipError_t e, const char* what);
     if (_e != hipSuccess) return hip_fail(_e, #call);   \
   } while (0)
 
+#define RNJ_HIP_STATUS(call)                            \
+  do {                                                  \
+    const int _rc = (call);                             \
+    if (_rc != RNJ_OK) return _rc;                      \
+  } while (0)
+
 // ---- scratch for one launch -----------------------------------------------------------------
 // Job table (pinned host copy + device copy), per-job statistics (max bits, saturated count) and
 // the saturated-pair list. Reused once the event recorded after its last launch has completed.
@@ -40,6 +46,8 @@ struct Workspace {
   unsigned* d_stats = nullptr;  // [cap_jobs][2]: maxbits, sent_count
   unsigned long long* d_sent = nullptr;
   size_t cap_sent = 0;
+  uint32_t* d_partial = nullptr;  // partial counts of split tiles
+  size_t cap_partial = 0;         // u32 elements
   hipEvent_t done = nullptr;
   bool recorded = false;
   bool in_use = false;
@@ -85,8 +93,12 @@ size_t sentinel_capacity(unsigned n);
 int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out);
 
 // Enqueue a multi-job launch on `stream`: reset statistics, tile kernel, saturation patch
-// (unless raw), max copies. All jobs must share one tile configuration (chosen here).
-int launch_jobs(Context* ctx, std::vector<JobSpec>& jobs, bool raw, hipStream_t stream);
+// (unless raw), max copies. All jobs must share one tile configuration (chosen here). exact:
+// every logarithm in FP64 as the reference writes it (else FP32 away from saturation).
+// allow_tune: the first launch of an untuned shape class may time every configuration
+// (synchronously); without it such launches use the size-based default.
+int launch_jobs(Context* ctx, std::vector<JobSpec>& jobs, bool raw, bool exact,
+                hipStream_t stream, bool allow_tune = true);
 
 int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only);
 void free_workspace_public(Workspace* ws);

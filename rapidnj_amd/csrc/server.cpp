@@ -1,14 +1,21 @@
 // Served distance-matrix jobs: a request loop that admits jobs by HBM budget and batches them.
 //
-// Submitted jobs wait in a FIFO queue. A worker thread forms a batch from the queue head as long
-// as the jobs' device memory (characters, packed planes, triangle, saturated-pair list) fits the
-// remaining budget and a stream slot is free, then enqueues on that slot's stream: upload, pack,
-// ONE multi-job tile launch for the whole batch, saturation patch and download of every job's
-// triangle into pinned memory. Up to `slots` batches are on the device at once, so jobs that
-// arrive while others run are admitted as soon as a slot and memory free up, and their uploads,
-// counting and downloads overlap with the running batches. Each job's result is bit-identical to
-// running it alone: the counts are exact integers and every job keeps its own maximum and list
-// of saturated pairs.
+// Admission. A submitted job is admitted when its device memory (characters, packed planes,
+// triangle, saturated-pair list) fits what the budget has left and no earlier job is waiting.
+// An admitted job is uploaded from the caller's buffer on the submitting thread (copy on the
+// server's upload stream, then the packing kernel) and becomes ready when that stream reaches
+// its event; the caller's buffer is free when submit returns. A job that does not fit is copied
+// to host memory and queued; the worker admits queued jobs in FIFO order as batches retire and
+// memory frees up.
+//
+// Batching. The worker forms a batch of up to max_batch_jobs ready jobs per free stream slot
+// and enqueues on that slot's stream: a wait for each job's upload, ONE multi-job tile launch
+// and the saturation patch. Up to `slots` batches are on the device at once, so uploads of new
+// jobs overlap the counting of running batches. A finished job's triangle stays resident in HBM
+// until rnj_server_wait copies it, once, straight into the caller's buffer on the caller's
+// thread; its memory returns to the budget then. Each job's result is bit-identical to running it
+// alone: the counts are exact integers and every job keeps its own maximum and saturated-pair
+// list.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -23,35 +30,37 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-enum class State { Queued, Admitted, Done };
+enum class State { Queued, Uploading, Ready, Running, Done };
 
 struct ServedJob {
   uint64_t id = 0;
-  unsigned n = 0, L = 0;
+  unsigned n = 0, L = 0, n_pad = 0, W = 0;
   int model = 0;
-  std::vector<char> chars;
-  std::vector<float> result;
+  std::vector<char> chars;  // host copy, only while queued
   State state = State::Queued;
   int status = RNJ_OK;
   std::string error;
-  Clock::time_point t_submit, t_admit, t_done;
+  Clock::time_point t_submit, t_admit, t_ready, t_done;
   unsigned batch_id = 0, batch_jobs = 0;
   size_t need = 0;
+  uint32_t* planes = nullptr;  // device, from admission until its batch retires
+  float* tri = nullptr;
+  hipEvent_t ready = nullptr;  // recorded after packing
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  float* pinned = nullptr;  // downloads of the batch's triangles
-  size_t pinned_floats = 0;
   bool busy = false;
   unsigned batch_id = 0;
   std::vector<ServedJob*> jobs;
-  std::vector<size_t> offsets;  // job k's triangle at pinned + offsets[k]
-  size_t bytes = 0;
+  std::string error;
+  int error_status = RNJ_OK;
 };
 
 size_t pairs_of(unsigned n) { return (size_t)n * (n - 1) / 2; }
+
+size_t tri_bytes(unsigned n) { return pairs_of(n) * 4; }
 
 size_t job_bytes(unsigned n, unsigned L) {
   const size_t n_pad = round_up(n, kTaxaPad);
@@ -65,9 +74,13 @@ struct rnj_server {
   Context* ctx = nullptr;
   rnj_server_config cfg{};
   std::vector<Slot> slots;
+  hipStream_t upload_stream = nullptr;
+  hipStream_t download_stream = nullptr;
   std::mutex mu;
+  std::mutex upload_mu;  // one upload at a time on the upload stream
   std::condition_variable cv_work, cv_done;
-  std::deque<ServedJob*> queue;
+  std::deque<ServedJob*> queue;  // waiting for admission (host copies)
+  std::deque<ServedJob*> ready;  // admitted: uploading or uploaded, in arrival order
   std::map<uint64_t, ServedJob*> jobs;
   size_t used = 0;
   uint64_t next_id = 1;
@@ -76,165 +89,205 @@ struct rnj_server {
   std::thread worker;
 
   void run();
-  void retire(Slot& s);
-  void launch(Slot& s, std::vector<ServedJob*>& batch);
-  void fail_batch(Slot& s, int status, const std::string& msg);
+  int upload(ServedJob* j, const char* chars, size_t pitch);
+  bool launch(Slot& s);
+  void release_job_memory(ServedJob* j, hipStream_t stream);
 };
 
-void rnj_server::fail_batch(Slot& s, int status, const std::string& msg) {
-  for (ServedJob* j : s.jobs) {
-    j->status = status;
-    j->error = msg;
-    j->state = State::Done;
-    j->t_done = Clock::now();
+// Upload and pack one admitted job (no server lock held). chars: n rows at `pitch` on the host.
+// Returns when the caller's buffer has been read.
+int rnj_server::upload(ServedJob* j, const char* chars, size_t pitch) {
+  std::lock_guard<std::mutex> ul(upload_mu);
+  hipStream_t s = upload_stream;
+  uint8_t* dchars = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dchars, (size_t)j->n * j->L, s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&j->planes, (size_t)3 * j->W * j->n_pad * 4, s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&j->tri, pairs_of(j->n) * 4, s);
+  if (e == hipSuccess) {
+    if (pitch == j->L)  // one contiguous copy: a 2D copy from pageable memory goes row by row
+      e = hipMemcpyAsync(dchars, chars, (size_t)j->n * j->L, hipMemcpyHostToDevice, s);
+    else
+      e = hipMemcpy2DAsync(dchars, j->L, chars, pitch, j->L, j->n, hipMemcpyHostToDevice, s);
   }
+  if (e == hipSuccess) e = launch_pack_chars(dchars, j->L, j->n, j->L, j->n_pad, j->W, j->planes, s);
+  if (dchars) (void)hipFreeAsync(dchars, s);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&j->ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(j->ready, s);
+  // the source is the caller's (pageable) memory: wait until it has been read
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    j->error = std::string("upload: ") + hipGetErrorString(e);
+    return RNJ_E_HIP;
+  }
+  return RNJ_OK;
 }
 
-// Enqueue one batch on slot s (called with the lock held; only enqueues work).
-void rnj_server::launch(Slot& s, std::vector<ServedJob*>& batch) {
-  s.jobs = batch;
-  s.batch_id = next_batch++;
-  s.bytes = 0;
-  s.offsets.assign(batch.size(), 0);
-  size_t floats = 0;
-  for (size_t k = 0; k < batch.size(); ++k) {
-    s.offsets[k] = floats;
-    floats += pairs_of(batch[k]->n);
-    s.bytes += batch[k]->need;
-  }
-  const Clock::time_point now = Clock::now();
-  for (ServedJob* j : batch) {
-    j->state = State::Admitted;
-    j->t_admit = now;
-    j->batch_id = s.batch_id;
-    j->batch_jobs = (unsigned)batch.size();
-  }
-  used += s.bytes;
-  s.busy = true;
+void rnj_server::release_job_memory(ServedJob* j, hipStream_t stream) {
+  if (j->planes) (void)hipFreeAsync(j->planes, stream);
+  if (j->tri) (void)hipFreeAsync(j->tri, stream);
+  j->planes = nullptr;
+  j->tri = nullptr;
+}
+
+// Memory a job holds after its batch ran: only its triangle (freed by rnj_server_wait).
+static size_t held_after_batch(const ServedJob* j) { return tri_bytes(j->n); }
+
+// Enqueue the batch in s.jobs on slot s (worker thread, no server lock). Returns false with the
+// slot's error set if enqueueing failed.
+bool rnj_server::launch(Slot& s) {
   hipError_t e = hipSuccess;
-  if (floats > s.pinned_floats) {
-    if (s.pinned) (void)hipHostFree(s.pinned);
-    s.pinned = nullptr;
-    s.pinned_floats = 0;
-    e = hipHostMalloc((void**)&s.pinned, floats * sizeof(float), hipHostMallocDefault);
-    if (e == hipSuccess) s.pinned_floats = floats;
-  }
-  std::vector<void*> allocs;
   std::vector<JobSpec> specs;
-  std::vector<float*> tris;
-  for (size_t k = 0; e == hipSuccess && k < batch.size(); ++k) {
-    ServedJob* j = batch[k];
-    const unsigned n_pad = round_up(j->n, kTaxaPad), W = words_for_sites(j->L);
-    uint8_t* dchars = nullptr;
-    uint32_t* planes = nullptr;
-    float* tri = nullptr;
-    e = hipMallocAsync((void**)&dchars, (size_t)j->n * j->L, s.stream);
-    if (e == hipSuccess) allocs.push_back(dchars);
-    if (e == hipSuccess) e = hipMallocAsync((void**)&planes, (size_t)3 * W * n_pad * 4, s.stream);
-    if (e == hipSuccess) allocs.push_back(planes);
-    if (e == hipSuccess) e = hipMallocAsync((void**)&tri, pairs_of(j->n) * 4, s.stream);
-    if (e == hipSuccess) allocs.push_back(tri);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(dchars, j->chars.data(), (size_t)j->n * j->L, hipMemcpyHostToDevice,
-                         s.stream);
-    if (e == hipSuccess) e = launch_pack_chars(dchars, j->L, j->n, j->L, n_pad, W, planes, s.stream);
+  for (ServedJob* j : s.jobs) {
+    if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, j->ready, 0);
     JobSpec js{};
-    js.planes = planes;
+    js.planes = j->planes;
     js.n = j->n;
-    js.n_pad = n_pad;
-    js.W_pad = W;
-    js.W = W;
+    js.n_pad = j->n_pad;
+    js.W_pad = j->W;
+    js.W = j->W;
     js.model = j->model;
     js.layout = kTriangle;
-    js.out = tri;
+    js.out = j->tri;
     specs.push_back(js);
-    tris.push_back(tri);
   }
   int rc = RNJ_OK;
-  if (e == hipSuccess) rc = launch_jobs(ctx, specs, false, s.stream);
-  for (size_t k = 0; e == hipSuccess && rc == RNJ_OK && k < batch.size(); ++k)
-    e = hipMemcpyAsync(s.pinned + s.offsets[k], tris[k], pairs_of(batch[k]->n) * 4,
-                       hipMemcpyDeviceToHost, s.stream);
-  for (void* p : allocs) (void)hipFreeAsync(p, s.stream);
+  if (e == hipSuccess) rc = launch_jobs(ctx, specs, false, false, s.stream, false);
+  for (ServedJob* j : s.jobs) {  // the packed planes are no longer needed; the triangle stays
+    if (j->planes) (void)hipFreeAsync(j->planes, s.stream);
+    j->planes = nullptr;
+  }
   if (e == hipSuccess && rc == RNJ_OK) e = hipEventRecord(s.done, s.stream);
-  if (e != hipSuccess || rc != RNJ_OK) {
-    const std::string msg = e != hipSuccess ? std::string("batch launch: ") + hipGetErrorString(e)
-                                            : std::string(last_error_cstr());
-    fail_batch(s, e != hipSuccess ? RNJ_E_HIP : rc, msg);
-    (void)hipStreamSynchronize(s.stream);
-    used -= s.bytes;
-    s.busy = false;
-    s.jobs.clear();
-    cv_done.notify_all();
-  }
-}
-
-void rnj_server::retire(Slot& s) {
-  const Clock::time_point now = Clock::now();
-  for (size_t k = 0; k < s.jobs.size(); ++k) {
-    ServedJob* j = s.jobs[k];
-    j->result.assign(s.pinned + s.offsets[k], s.pinned + s.offsets[k] + pairs_of(j->n));
-    j->chars.clear();
-    j->chars.shrink_to_fit();
-    j->status = RNJ_OK;
-    j->state = State::Done;
-    j->t_done = now;
-  }
-  used -= s.bytes;
-  s.busy = false;
-  s.jobs.clear();
-  cv_done.notify_all();
+  if (e == hipSuccess && rc == RNJ_OK) return true;
+  s.error = e != hipSuccess ? std::string("batch launch: ") + hipGetErrorString(e)
+                            : std::string(last_error_cstr());
+  s.error_status = e != hipSuccess ? RNJ_E_HIP : rc;
+  (void)hipStreamSynchronize(s.stream);
+  return false;
 }
 
 void rnj_server::run() {
   (void)hipSetDevice(ctx->device);
   std::unique_lock<std::mutex> lock(mu);
   for (;;) {
-    bool any_busy = false;
+    // 1. batches that finished on the device
+    std::vector<Slot*> finished, failed;
     for (Slot& s : slots) {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.done);
-      if (q == hipSuccess) retire(s);
-      else if (q != hipErrorNotReady) {
-        fail_batch(s, RNJ_E_HIP, std::string("batch failed: ") + hipGetErrorString(q));
-        used -= s.bytes;
-        s.busy = false;
-        s.jobs.clear();
-        cv_done.notify_all();
-      } else {
-        any_busy = true;
+      if (q == hipSuccess) {
+        finished.push_back(&s);
+      } else if (q != hipErrorNotReady) {
+        s.error = std::string("batch failed: ") + hipGetErrorString(q);
+        s.error_status = RNJ_E_HIP;
+        failed.push_back(&s);
       }
     }
-    // admission: FIFO, as many head jobs as fit the budget, one batch per free slot
-    for (Slot& s : slots) {
-      if (s.busy || queue.empty()) continue;
-      std::vector<ServedJob*> batch;
-      size_t add = 0;
-      while (!queue.empty() && batch.size() < cfg.max_batch_jobs) {
-        ServedJob* j = queue.front();
-        if (j->need > cfg.hbm_budget) {  // can never fit
-          queue.pop_front();
-          j->status = RNJ_E_ADMISSION;
-          j->error = "job needs more HBM than the server budget";
-          j->state = State::Done;
-          j->t_done = Clock::now();
-          cv_done.notify_all();
-          continue;
-        }
-        if (used + add + j->need > cfg.hbm_budget) break;
-        add += j->need;
-        batch.push_back(j);
+    // 2. queued jobs that now fit (FIFO); they are uploaded below, outside the lock
+    std::vector<ServedJob*> to_upload;
+    while (!queue.empty()) {
+      ServedJob* j = queue.front();
+      if (j->need > cfg.hbm_budget) {  // can never fit
         queue.pop_front();
+        j->status = RNJ_E_ADMISSION;
+        j->error = "job needs more HBM than the server budget";
+        j->state = State::Done;
+        j->t_done = Clock::now();
+        cv_done.notify_all();
+        continue;
       }
-      if (batch.empty()) break;
-      launch(s, batch);
-      if (s.busy) any_busy = true;
+      if (used + j->need > cfg.hbm_budget) break;
+      queue.pop_front();
+      used += j->need;
+      j->state = State::Uploading;
+      j->t_admit = Clock::now();
+      to_upload.push_back(j);
     }
-    if (stopping && queue.empty() && !any_busy) break;
-    if (any_busy)
-      cv_work.wait_for(lock, std::chrono::microseconds(100));
-    else
-      cv_work.wait(lock, [this] { return stopping || !queue.empty(); });
+    // 3. batches of ready jobs (arrival order) for free slots
+    std::vector<Slot*> launching;
+    for (Slot& s : slots) {
+      if (s.busy || ready.empty() || ready.front()->state != State::Ready) continue;
+      if (cfg.batch_window_us && !stopping) {  // gather more jobs while the window is open
+        size_t n_ready = 0;
+        for (ServedJob* j : ready) n_ready += j->state == State::Ready;
+        const auto age = Clock::now() - ready.front()->t_ready;
+        if (n_ready < cfg.max_batch_jobs && age < std::chrono::microseconds(cfg.batch_window_us))
+          break;
+      }
+      s.jobs.clear();
+      while (!ready.empty() && ready.front()->state == State::Ready &&
+             s.jobs.size() < cfg.max_batch_jobs) {
+        ServedJob* j = ready.front();
+        ready.pop_front();
+        j->state = State::Running;
+        s.jobs.push_back(j);
+      }
+      s.batch_id = next_batch++;
+      for (ServedJob* j : s.jobs) {
+        j->batch_id = s.batch_id;
+        j->batch_jobs = (unsigned)s.jobs.size();
+      }
+      s.busy = true;
+      launching.push_back(&s);
+    }
+    bool busy = false;
+    for (Slot& s : slots) busy = busy || s.busy;
+    const bool idle = queue.empty() && ready.empty() && to_upload.empty() && finished.empty() &&
+                      failed.empty();
+    if (stopping && idle && !busy) break;
+    lock.unlock();
+
+    // 4. host-side work without the lock
+    for (ServedJob* j : to_upload) {
+      j->status = upload(j, j->chars.data(), j->L);
+      j->chars.clear();
+      j->chars.shrink_to_fit();
+    }
+    for (Slot* s : launching)
+      if (!launch(*s)) failed.push_back(s);
+
+    lock.lock();
+    const Clock::time_point now = Clock::now();
+    for (ServedJob* j : to_upload) {
+      if (j->status == RNJ_OK) {
+        j->state = State::Ready;
+        j->t_ready = now;
+        ready.push_back(j);
+      } else {
+        release_job_memory(j, upload_stream);
+        used -= j->need;
+        j->state = State::Done;
+        j->t_done = now;
+      }
+    }
+    for (Slot* s : finished) {
+      for (ServedJob* j : s->jobs) {
+        j->status = RNJ_OK;
+        j->state = State::Done;
+        j->t_done = now;
+        used -= j->need - held_after_batch(j);
+      }
+      s->busy = false;
+      s->jobs.clear();
+    }
+    for (Slot* s : failed) {
+      for (ServedJob* j : s->jobs) {
+        j->status = s->error_status;
+        j->error = s->error;
+        j->state = State::Done;
+        j->t_done = now;
+        release_job_memory(j, s->stream);
+        used -= j->need;
+      }
+      s->busy = false;
+      s->jobs.clear();
+    }
+    if (!finished.empty() || !failed.empty() || !to_upload.empty()) cv_done.notify_all();
+    bool busy_now = false;
+    for (Slot& s : slots) busy_now = busy_now || s.busy;
+    if (busy_now || !ready.empty())  // a batch, an upload or a batch window in flight
+      cv_work.wait_for(lock, std::chrono::microseconds(50));
+    else if (!stopping && queue.empty())
+      cv_work.wait(lock, [this] { return stopping || !queue.empty() || !ready.empty(); });
   }
 }
 
@@ -258,17 +311,21 @@ int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server
     srv->cfg.hbm_budget = (size_t)(0.8 * (double)free_b);
   }
   srv->slots.resize(srv->cfg.slots);
+  hipError_t e = hipStreamCreateWithFlags(&srv->upload_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&srv->download_stream, hipStreamNonBlocking);
   for (Slot& s : srv->slots) {
-    hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
-    if (e != hipSuccess) {
-      for (Slot& t : srv->slots) {
-        if (t.stream) (void)hipStreamDestroy(t.stream);
-        if (t.done) (void)hipEventDestroy(t.done);
-      }
-      delete srv;
-      return hip_fail(e, "rnj_server_create");
+  }
+  if (e != hipSuccess) {
+    for (Slot& t : srv->slots) {
+      if (t.stream) (void)hipStreamDestroy(t.stream);
+      if (t.done) (void)hipEventDestroy(t.done);
     }
+    if (srv->upload_stream) (void)hipStreamDestroy(srv->upload_stream);
+    if (srv->download_stream) (void)hipStreamDestroy(srv->download_stream);
+    delete srv;
+    return hip_fail(e, "rnj_server_create");
   }
   srv->worker = std::thread([srv] { srv->run(); });
   *out = srv;
@@ -284,11 +341,12 @@ int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned
   auto* j = new ServedJob();
   j->n = n;
   j->L = L;
+  j->n_pad = round_up(n, kTaxaPad);
+  j->W = words_for_sites(L);
   j->model = model;
-  j->chars.resize((size_t)n * L);
-  for (unsigned t = 0; t < n; ++t) std::memcpy(&j->chars[(size_t)t * L], chars + (size_t)t * pitch, L);
   j->need = job_bytes(n, L);
   j->t_submit = Clock::now();
+  bool admit_now = false;
   {
     std::lock_guard<std::mutex> lock(srv->mu);
     if (srv->stopping) {
@@ -297,8 +355,42 @@ int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned
     }
     j->id = srv->next_id++;
     srv->jobs[j->id] = j;
-    srv->queue.push_back(j);
     *ticket = j->id;
+    if (srv->queue.empty() && j->need <= srv->cfg.hbm_budget &&
+        srv->used + j->need <= srv->cfg.hbm_budget) {
+      admit_now = true;
+      srv->used += j->need;
+      j->state = State::Uploading;
+      j->t_admit = j->t_submit;
+      srv->ready.push_back(j);  // keeps arrival order; batched once Ready
+    }
+  }
+  if (admit_now) {
+    (void)hipSetDevice(srv->ctx->device);
+    const int rc = srv->upload(j, chars, pitch);
+    std::lock_guard<std::mutex> lock(srv->mu);
+    if (rc == RNJ_OK) {
+      j->state = State::Ready;
+      j->t_ready = Clock::now();
+    } else {
+      for (auto it = srv->ready.begin(); it != srv->ready.end(); ++it)
+        if (*it == j) {
+          srv->ready.erase(it);
+          break;
+        }
+      srv->release_job_memory(j, srv->upload_stream);
+      srv->used -= j->need;
+      j->status = rc;
+      j->state = State::Done;
+      j->t_done = Clock::now();
+      srv->cv_done.notify_all();
+    }
+  } else {
+    j->chars.resize((size_t)n * L);
+    for (unsigned t = 0; t < n; ++t)
+      std::memcpy(&j->chars[(size_t)t * L], chars + (size_t)t * pitch, L);
+    std::lock_guard<std::mutex> lock(srv->mu);
+    srv->queue.push_back(j);
   }
   srv->cv_work.notify_one();
   return RNJ_OK;
@@ -321,18 +413,38 @@ int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_st
   srv->cv_done.wait(lock, [j] { return j->state == State::Done; });
   srv->jobs.erase(it);
   lock.unlock();
-  const int status = j->status;
-  if (status == RNJ_OK && tri_out) std::memcpy(tri_out, j->result.data(), j->result.size() * 4);
+  int status = j->status;
+  if (j->tri) {  // finished: the triangle is resident; copy it once into the caller's buffer
+    hipError_t e = hipSuccess;
+    if (status == RNJ_OK && tri_out) {
+      e = hipMemcpyAsync(tri_out, j->tri, tri_bytes(j->n), hipMemcpyDeviceToHost,
+                         srv->download_stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(srv->download_stream);
+    }
+    (void)hipFreeAsync(j->tri, srv->download_stream);
+    j->tri = nullptr;
+    {
+      std::lock_guard<std::mutex> relock(srv->mu);
+      srv->used -= held_after_batch(j);
+    }
+    srv->cv_work.notify_one();  // memory came back: queued jobs may fit now
+    if (e != hipSuccess) {
+      status = RNJ_E_HIP;
+      j->error = std::string("download: ") + hipGetErrorString(e);
+    }
+  }
+  const Clock::time_point copied = Clock::now();
   if (stats) {
     auto sec = [](Clock::duration d) { return std::chrono::duration<double>(d).count(); };
     const bool admitted = j->t_admit.time_since_epoch().count() != 0;
     stats->queued_s = admitted ? sec(j->t_admit - j->t_submit) : sec(j->t_done - j->t_submit);
-    stats->device_s = admitted ? sec(j->t_done - j->t_admit) : 0.0;
-    stats->total_s = sec(j->t_done - j->t_submit);
+    stats->device_s = admitted ? sec(copied - j->t_admit) : 0.0;
+    stats->total_s = sec(copied - j->t_submit);
     stats->batch_id = j->batch_id;
     stats->batch_jobs = j->batch_jobs;
   }
   if (status != RNJ_OK) set_error(j->error);
+  if (j->ready) (void)hipEventDestroy(j->ready);
   delete j;
   return status;
 }
@@ -362,7 +474,7 @@ int rnj_server_destroy(rnj_server* srv) {
   {
     std::lock_guard<std::mutex> lock(srv->mu);
     srv->stopping = true;
-    // jobs not yet admitted are cancelled; admitted batches finish
+    // jobs not yet admitted are cancelled; admitted ones finish
     for (ServedJob* j : srv->queue) {
       j->status = RNJ_E_CANCELLED;
       j->error = "server destroyed";
@@ -375,11 +487,19 @@ int rnj_server_destroy(rnj_server* srv) {
   if (srv->worker.joinable()) srv->worker.join();
   for (Slot& s : srv->slots) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
-    if (s.pinned) (void)hipHostFree(s.pinned);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.done) (void)hipEventDestroy(s.done);
   }
-  for (auto& kv : srv->jobs) delete kv.second;
+  for (auto& kv : srv->jobs) {  // results never waited for
+    srv->release_job_memory(kv.second, srv->download_stream);
+    if (kv.second->ready) (void)hipEventDestroy(kv.second->ready);
+    delete kv.second;
+  }
+  for (hipStream_t st : {srv->upload_stream, srv->download_stream})
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   delete srv;
   return RNJ_OK;
 }

@@ -21,6 +21,7 @@ RNJ_E_ADMISSION, RNJ_E_CANCELLED, RNJ_E_NOT_READY, RNJ_E_NO_DEVICE = 5, 6, 7, 8
 JC, K2P = 0, 1
 TRIANGLE, SQUARE = 0, 1
 FLAG_RAW = 1
+FLAG_EXACT = 2
 
 _vp = ctypes.c_void_p
 _u = ctypes.c_uint
@@ -71,7 +72,8 @@ SIGNATURES = {
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("slots", _u), ("max_batch_jobs", _u), ("hbm_budget", _sz)]
+    _fields_ = [("slots", _u), ("max_batch_jobs", _u), ("hbm_budget", _sz),
+                ("batch_window_us", _u)]
 
 
 class JobStats(ctypes.Structure):
@@ -258,7 +260,7 @@ class Context:
 
     # ---- distances ----------------------------------------------------------------------------
     def distances(self, aln: Alignment, model="k2p", layout: str = "triangle", out=None,
-                  raw: bool = False, max_out=None, stream=None):
+                  raw: bool = False, max_out=None, stream=None, exact: bool = False):
         """Distance matrix into a device tensor (float32): triangle (n(n-1)/2) or square (n, n)."""
         import torch
         n = aln.n
@@ -268,7 +270,8 @@ class Context:
             out = torch.empty(shape, dtype=torch.float32, device=f"cuda:{self.device}")
         ld = out.stride(0) if lay == SQUARE else 0
         check(lib().rnj_distances_device(self.handle, aln.handle, _model(model), lay,
-                                         out.data_ptr(), ld, FLAG_RAW if raw else 0,
+                                         out.data_ptr(), ld,
+                                         (FLAG_RAW if raw else 0) | (FLAG_EXACT if exact else 0),
                                          max_out.data_ptr() if max_out is not None else None,
                                          _stream_handle(stream)))
         return out
@@ -294,7 +297,8 @@ class Context:
         return mat, mx.value
 
     def bootstrap(self, aln: Alignment, weights: np.ndarray, model="k2p",
-                  layout: str = "triangle", out=None, raw: bool = False, stream=None):
+                  layout: str = "triangle", out=None, raw: bool = False, stream=None,
+                  exact: bool = False):
         """R replicate matrices from (R, L) uint16 column weights, in one launch."""
         import torch
         w = np.ascontiguousarray(weights, dtype=np.uint16)
@@ -309,19 +313,22 @@ class Context:
         ld = out.stride(1) if lay == SQUARE else 0
         check(lib().rnj_bootstrap_device(self.handle, aln.handle, _model(model), w.ctypes.data, R,
                                          lay, out.data_ptr(), mstride, ld,
-                                         FLAG_RAW if raw else 0, _stream_handle(stream)))
+                                         (FLAG_RAW if raw else 0) | (FLAG_EXACT if exact else 0),
+                                         _stream_handle(stream)))
         return out
 
-    def server(self, slots: int = 2, max_batch_jobs: int = 64, hbm_budget: int = 0) -> "Server":
-        return Server(self, slots, max_batch_jobs, hbm_budget)
+    def server(self, slots: int = 2, max_batch_jobs: int = 64, hbm_budget: int = 0,
+               batch_window_us: int = 0) -> "Server":
+        return Server(self, slots, max_batch_jobs, hbm_budget, batch_window_us)
 
 
 class Server:
     """Served jobs: admission by HBM budget, batching, overlap of batches on the device."""
 
-    def __init__(self, ctx: Context, slots: int, max_batch_jobs: int, hbm_budget: int):
+    def __init__(self, ctx: Context, slots: int, max_batch_jobs: int, hbm_budget: int,
+                 batch_window_us: int = 0):
         self.ctx = ctx
-        cfg = ServerConfig(slots, max_batch_jobs, hbm_budget)
+        cfg = ServerConfig(slots, max_batch_jobs, hbm_budget, batch_window_us)
         h = _vp()
         check(lib().rnj_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value

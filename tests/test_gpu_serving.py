@@ -36,7 +36,7 @@ def _jobs(count, seed=0):
 def test_alone_equals_in_mix(ctx):
     jobs = _jobs(24)
     alone = [ctx.distances(ctx.alignment(a), m).cpu().numpy() for a, m in jobs]
-    with ctx.server(slots=2, max_batch_jobs=8) as srv:
+    with ctx.server(slots=2, max_batch_jobs=8, batch_window_us=500_000) as srv:
         tickets = [srv.submit(a, m) for a, m in jobs]
         results = [srv.wait(t, a.shape[0]) for t, (a, _) in zip(tickets, jobs)]
     batched = 0
@@ -44,7 +44,7 @@ def test_alone_equals_in_mix(ctx):
         np.testing.assert_array_equal(tri.view(np.uint32), ref.view(np.uint32))
         assert st.total_s >= st.device_s >= 0 and st.batch_jobs >= 1
         batched += st.batch_jobs > 1
-    assert batched > 0  # jobs submitted together shared launches
+    assert batched == len(jobs)  # inside the batch window every launch held several jobs
 
 
 def test_jobs_arriving_while_others_run(ctx):
