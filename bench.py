@@ -163,9 +163,22 @@ def bootstrap_compare(ctx, dev) -> dict:
     torch.cuda.synchronize()
     single = (time.perf_counter() - t0) / 2
     same = bool(torch.equal(out, ref))
+    # the same launch captured once into a HIP graph, replayed with fresh weights per call
+    sets = [bootstrap_weights(wl.L, wl.replicates, seed0=1000 * (i + 1)) for i in range(k)]
+    with ctx.bootstrap_plan(A, wl.replicates, wl.model, out=out) as plan:
+        plan.run(w)
+        torch.cuda.synchronize()
+        same = same and bool(torch.equal(out, ref))
+        t0 = time.perf_counter()
+        for ws_ in sets:
+            plan.run(ws_)
+        torch.cuda.synchronize()
+        graphed = (time.perf_counter() - t0) / k
     sp = pairs * wl.L * wl.replicates
     A.close()
-    return {"workload": wl.name, "batched_ms": round(batched * 1e3, 3),
+    return {"workload": wl.name, "graph_ms": round(graphed * 1e3, 3),
+            "graph_value": round(sp / graphed / 1e9, 2),
+            "batched_ms": round(batched * 1e3, 3),
             "batched_value": round(sp / batched / 1e9, 2),
             "one_per_launch_ms": round(single * 1e3, 3),
             "one_per_launch_value": round(sp / single / 1e9, 2),

@@ -68,6 +68,7 @@ enum {
 typedef struct rnj_context rnj_context;
 typedef struct rnj_alignment rnj_alignment;
 typedef struct rnj_server rnj_server;
+typedef struct rnj_plan rnj_plan;
 
 /* ---- reference device boundary (group 1) ------------------------------------------------ */
 
@@ -176,6 +177,20 @@ RNJ_EXPORT int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* 
 RNJ_EXPORT int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
                          const uint16_t* weights, unsigned R, int layout, float* out,
                          size_t matrix_stride, size_t ld, int flags, void* stream);
+
+/* Captured batched bootstrap: the launch of rnj_bootstrap_device for fixed (alignment, model, R,
+ * layout, out) captured once into a HIP graph and replayed per call with new weights. All
+ * device memory is allocated at creation; a run allocates nothing. max_length bounds the sum
+ * of every replicate's weights (0 = L, the multinomial case); a run whose weights exceed it is
+ * rejected. Runs of one plan are serialised by the plan. No counterpart in the reference (its
+ * bootstrap recomputes replicates one by one, src/main.cpp:519-532). */
+RNJ_EXPORT int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int model,
+                                         unsigned R, unsigned max_length, int layout, float* out,
+                                         size_t matrix_stride, size_t ld, int flags,
+                                         rnj_plan** plan);
+RNJ_EXPORT int rnj_bootstrap_plan_run(rnj_plan* plan, const uint16_t* weights, void* stream);
+RNJ_EXPORT int rnj_plan_tile_config(const rnj_plan* plan); /* configuration the graph uses */
+RNJ_EXPORT int rnj_plan_destroy(rnj_plan* plan);
 
 /* ---- served jobs ----------------------------------------------------------------------------
  * A request loop on its own thread admits submitted jobs while their HBM need fits the budget,

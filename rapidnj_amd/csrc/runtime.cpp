@@ -290,73 +290,81 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, si
   return RNJ_OK;
 }
 
-int launch_jobs(Context* ctx, std::vector<JobSpec>& specs, bool raw, bool exact,
-                hipStream_t stream, bool allow_tune) {
-  if (specs.empty()) return RNJ_OK;
-  bool jc_only = true;
-  unsigned n_key = 0, W_key = 0;
-  unsigned long long work = 0;
-  std::vector<size_t> caps(specs.size());
-  size_t nsent = 0;
+LaunchShape shape_of(const std::vector<JobSpec>& specs) {
+  LaunchShape sh;
+  sh.caps.resize(specs.size());
   for (size_t k = 0; k < specs.size(); ++k) {
     const JobSpec& s = specs[k];
-    if (s.model != kJC) jc_only = false;
-    n_key = std::max(n_key, s.n);
-    W_key = std::max(W_key, s.W);
-    work += (unsigned long long)s.n * s.n * s.W;
-    caps[k] = s.model == kCounts ? 0 : sentinel_capacity(s.n);
-    nsent += caps[k];
+    if (s.model != kJC) sh.jc_only = false;
+    sh.n_key = std::max(sh.n_key, s.n);
+    sh.W_key = std::max(sh.W_key, s.W);
+    sh.work += (unsigned long long)s.n * s.n * s.W;
+    sh.caps[k] = s.model == kCounts ? 0 : sentinel_capacity(s.n);
+    sh.nsent += sh.caps[k];
   }
-  Workspace* ws = nullptr;
-  int rc = acquire_workspace(ctx, (unsigned)specs.size(), std::max<size_t>(nsent, 1), &ws);
-  if (rc != RNJ_OK) return rc;
+  return sh;
+}
 
-  int config = ctx->tile_config;
-  if (config < 0) {
-    bool cached = false;
-    {
-      std::lock_guard<std::mutex> lock(ctx->mu);
-      auto it = ctx->tuned.find(std::make_tuple(ilog2(n_key), ilog2(W_key), (int)jc_only));
-      if (it != ctx->tuned.end()) { config = it->second; cached = true; }
-    }
-    if (!cached) {
-      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-      (void)hipStreamIsCapturing(stream, &cap);
-      // tune only launches worth >= 2^28 taxon^2-words (~0.1 ms of counting): smaller ones
-      // keep the default
-      if (allow_tune && cap == hipStreamCaptureStatusNone && work >= (1ull << 28)) {
-        rc = autotune(ctx, specs, ws, caps.data(), jc_only, exact, n_key, W_key, stream, &config);
-        if (rc != RNJ_OK) { release_workspace(ctx, ws, stream); return rc; }
-      } else {
-        config = default_config(n_key, W_key, jc_only);
-      }
+int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Workspace* ws,
+                  bool exact, hipStream_t stream, bool allow_tune, int* config) {
+  *config = ctx->tile_config;
+  if (*config >= 0) return RNJ_OK;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto it = ctx->tuned.find(std::make_tuple(ilog2(sh.n_key), ilog2(sh.W_key), (int)sh.jc_only));
+    if (it != ctx->tuned.end()) {
+      *config = it->second;
+      return RNJ_OK;
     }
   }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(stream, &cap);
+  // tune only launches worth >= 2^28 taxon^2-words (~0.1 ms of counting): smaller ones keep
+  // the default
+  if (allow_tune && cap == hipStreamCaptureStatusNone && sh.work >= (1ull << 28))
+    return autotune(ctx, specs, ws, sh.caps.data(), sh.jc_only, exact, sh.n_key, sh.W_key, stream,
+                    config);
+  *config = default_config(sh.n_key, sh.W_key, sh.jc_only);
+  return RNJ_OK;
+}
+
+int prepare_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
+  return grow_workspace(ws, njobs, std::max<size_t>(nsent, 1));
+}
+
+int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
+                   Workspace* ws, bool raw, bool exact, hipStream_t stream) {
   const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
-  if (P.items >= (1ull << 31)) {
-    release_workspace(ctx, ws, stream);
-    return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
-  }
-  rc = grow_partial(ws, P.partial_elems);
-  if (rc != RNJ_OK) {
-    release_workspace(ctx, ws, stream);
-    return rc;
-  }
-  number_jobs(specs, config, P, ws, caps.data());
+  if (P.items >= (1ull << 31)) return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
+  RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
+  number_jobs(specs, config, P, ws, sh.caps.data());
   hipError_t e = hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
                                 hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream);
   if (e == hipSuccess)
-    e = launch_tiles(config, jc_only, exact, ws->d_jobs, (unsigned)specs.size(),
+    e = launch_tiles(config, sh.jc_only, exact, ws->d_jobs, (unsigned)specs.size(),
                      (unsigned)P.items, (unsigned)P.finish_items, stream);
   if (e == hipSuccess && !raw) e = launch_patch(ws->d_jobs, (unsigned)specs.size(), stream);
   for (size_t k = 0; e == hipSuccess && k < specs.size(); ++k)
     if (specs[k].max_out)
       e = hipMemcpyAsync(specs[k].max_out, ws->d_stats + 2 * k, sizeof(float),
                          hipMemcpyDeviceToDevice, stream);
-  release_workspace(ctx, ws, stream);
-  if (e != hipSuccess) return hip_fail(e, "launch_jobs");
+  if (e != hipSuccess) return hip_fail(e, "launch");
   return RNJ_OK;
+}
+
+int launch_jobs(Context* ctx, std::vector<JobSpec>& specs, bool raw, bool exact,
+                hipStream_t stream, bool allow_tune) {
+  if (specs.empty()) return RNJ_OK;
+  LaunchShape sh = shape_of(specs);
+  Workspace* ws = nullptr;
+  int rc = acquire_workspace(ctx, (unsigned)specs.size(), std::max<size_t>(sh.nsent, 1), &ws);
+  if (rc != RNJ_OK) return rc;
+  int config = 0;
+  rc = select_config(ctx, specs, sh, ws, exact, stream, allow_tune, &config);
+  if (rc == RNJ_OK) rc = enqueue_launch(ctx, specs, sh, config, ws, raw, exact, stream);
+  release_workspace(ctx, ws, stream);
+  return rc;
 }
 
 }  // namespace rnj

@@ -101,6 +101,21 @@ int launch_jobs(Context* ctx, std::vector<JobSpec>& jobs, bool raw, bool exact,
                 hipStream_t stream, bool allow_tune = true);
 
 int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only);
+
+// The pieces of launch_jobs, for callers that own their workspace (captured plans).
+struct LaunchShape {
+  bool jc_only = true;
+  unsigned n_key = 0, W_key = 0;
+  unsigned long long work = 0;
+  std::vector<size_t> caps;  // saturated-pair list capacity per job
+  size_t nsent = 0;
+};
+LaunchShape shape_of(const std::vector<JobSpec>& specs);
+int prepare_workspace(Workspace* ws, unsigned njobs, size_t nsent);
+int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Workspace* ws,
+                  bool exact, hipStream_t stream, bool allow_tune, int* config);
+int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
+                   Workspace* ws, bool raw, bool exact, hipStream_t stream);
 void free_workspace_public(Workspace* ws);
 
 }  // namespace rnj

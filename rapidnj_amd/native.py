@@ -61,6 +61,12 @@ SIGNATURES = {
     "rnj_distances_stream_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "rnj_bootstrap_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _u, ctypes.c_int, _vp,
                                             _sz, _sz, ctypes.c_int, _vp]),
+    "rnj_bootstrap_plan_create": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u, _u, ctypes.c_int,
+                                                 _vp, _sz, _sz, ctypes.c_int,
+                                                 ctypes.POINTER(_vp)]),
+    "rnj_bootstrap_plan_run": (ctypes.c_int, [_vp, _vp, _vp]),
+    "rnj_plan_tile_config": (ctypes.c_int, [_vp]),
+    "rnj_plan_destroy": (ctypes.c_int, [_vp]),
     "rnj_server_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
     "rnj_server_submit": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_uint64)]),
@@ -317,9 +323,58 @@ class Context:
                                          _stream_handle(stream)))
         return out
 
+    def bootstrap_plan(self, aln: Alignment, R: int, model="k2p", layout: str = "triangle",
+                       out=None, max_length: int = 0, raw: bool = False,
+                       exact: bool = False) -> "BootstrapPlan":
+        """A captured (HIP graph) batched-bootstrap launch writing R matrices into `out`."""
+        return BootstrapPlan(self, aln, R, model, layout, out, max_length, raw, exact)
+
     def server(self, slots: int = 2, max_batch_jobs: int = 64, hbm_budget: int = 0,
                batch_window_us: int = 0) -> "Server":
         return Server(self, slots, max_batch_jobs, hbm_budget, batch_window_us)
+
+
+class BootstrapPlan:
+    """rnj_bootstrap_plan_*: one graph per (alignment, model, R, layout, output)."""
+
+    def __init__(self, ctx: Context, aln: Alignment, R: int, model, layout: str, out,
+                 max_length: int, raw: bool, exact: bool):
+        import torch
+        self.ctx, self.aln, self.R = ctx, aln, R
+        n = aln.n
+        lay = {"triangle": TRIANGLE, "square": SQUARE}[layout]
+        if out is None:
+            shape = (R, n * (n - 1) // 2) if lay == TRIANGLE else (R, n, n)
+            out = torch.empty(shape, dtype=torch.float32, device=f"cuda:{ctx.device}")
+        self.out = out
+        ld = out.stride(1) if lay == SQUARE else 0
+        h = _vp()
+        check(lib().rnj_bootstrap_plan_create(ctx.handle, aln.handle, _model(model), R,
+                                              max_length, lay, out.data_ptr(), out.stride(0), ld,
+                                              (FLAG_RAW if raw else 0) |
+                                              (FLAG_EXACT if exact else 0), ctypes.byref(h)))
+        self.handle = h.value
+
+    @property
+    def tile_config(self) -> int:
+        return lib().rnj_plan_tile_config(self.handle)
+
+    def run(self, weights: np.ndarray, stream=None):
+        w = np.ascontiguousarray(weights, dtype=np.uint16)
+        assert w.shape == (self.R, self.aln.L), "one weight per replicate and column"
+        check(lib().rnj_bootstrap_plan_run(self.handle, w.ctypes.data, _stream_handle(stream)))
+        return self.out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rnj_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Server:
