@@ -1,0 +1,237 @@
+// Device code shared by the VALU tile kernels (kernels.hip) and the MFMA tile kernel
+// (mfma.hip): bit helpers, the JC / K2P corrections, tile and job mapping, and the per-pair
+// epilogue. Included once per translation unit (everything has internal linkage).
+#pragma once
+
+#include "internal.hpp"
+
+#include <math.h>
+
+namespace rnj {
+namespace {
+
+// (a ^ b) & c in one v_bitop3_b32 (truth table over a=0xF0, b=0xCC, c=0xAA).
+__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x28u);
+}
+
+// popcount(x) + acc as one v_bcnt_u32_b32. Written out because LLVM otherwise re-associates
+// the per-word accumulations into v_bcnt + v_add3 trees (1.5 extra VALU ops per pair and word).
+__device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+  return r;
+}
+
+// Pointer in the global address space, so loads compile to global_load rather than flat_load
+// (the job table hands over generic pointers).
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+// Native 4 x u32 vector (ext_vector_type) for 128-bit global and LDS accesses.
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// ---- corrections ---------------------------------------------------------------------------
+// Kimura 2-parameter (KimuraDistance.cpp:339-348) and Jukes-Cantor with alpha = 4
+// (JCdistance.cpp:170-183). What decides the result's class is kept exactly as the reference
+// computes it:
+//  * the proportions: (float)((double)a / b) of integers below 2^24 equals the correctly rounded
+//    float quotient (a double rounding 53 -> 24 bits is innocuous for a quotient), so the float
+//    division is used, and the double one above 2^24;
+//  * the arguments of the logarithms, 1-2P-Q and 1-2Q (K2P) and 1-p/0.75 (JC), formed in double
+//    from the float proportions in the reference's order (exact for K2P), and the saturation
+//    tests on them, so the saturated pairs (-1) are the same;
+//  * zero distances: ts = tv = 0 gives -0.0f in both models, as the reference's -c*log(1.0) does.
+// The logarithms of those double arguments use log_unit (below) unless kExact, which calls the
+// library's compensated FP64 log; either way the reference's FP64 expression is evaluated.
+#pragma clang fp contract(off)
+__device__ __forceinline__ float proportion(uint32_t a, uint32_t valid) {
+  if (valid < (1u << 24)) return (float)a / (float)valid;
+  return (float)((double)a / (double)valid);
+}
+
+// Natural log for 0 < x <= 1 (a normal double), in plain FP64 arithmetic: x = 2^k * m with
+// m in [sqrt(1/2), sqrt(2)), s = (m - 1) / (m + 1), log(m) = 2 atanh(s) = 2s (1 + s^2/3 + ... +
+// s^20/21), which leaves a truncation below 2^-56 for |s| <= 0.1716, and k ln 2 with ln 2 split
+// into a 32-bit head (k * head is exact) and a tail. The quotient uses the hardware reciprocal
+// refined by two Newton steps. The result is within about 2 ulp of double, so after rounding to
+// float it equals the reference's float value except in rare ties to a float rounding boundary;
+// it costs about a third of the instructions of the compensated library log.
+__device__ __forceinline__ double log_unit(double x) {
+  const double kLn2Hi = 0x1.62e42ff000000p-1;
+  const double kLn2Lo = -0x1.718432a1b0e26p-35;
+  int k;
+  double m = frexp(x, &k);            // m in [0.5, 1)
+  const bool low = m < 0.70710678118654757;
+  m = low ? m * 2.0 : m;              // m in [sqrt(1/2), sqrt(2))
+  k = low ? k - 1 : k;
+  const double f = m - 1.0;           // exact
+  const double d = 2.0 + f;
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  const double sd = f * r;
+  const double t = sd * sd;
+  double p = 1.0 / 21.0;
+  p = fma(p, t, 1.0 / 19.0);
+  p = fma(p, t, 1.0 / 17.0);
+  p = fma(p, t, 1.0 / 15.0);
+  p = fma(p, t, 1.0 / 13.0);
+  p = fma(p, t, 1.0 / 11.0);
+  p = fma(p, t, 1.0 / 9.0);
+  p = fma(p, t, 1.0 / 7.0);
+  p = fma(p, t, 1.0 / 5.0);
+  p = fma(p, t, 1.0 / 3.0);
+  const double two_s = 2.0 * sd;
+  const double lm = fma(two_s * t, p, two_s);
+  const double kd = (double)k;
+  return fma(kd, kLn2Hi, fma(kd, kLn2Lo, lm));
+}
+
+// The corrections as straight-line code (every case computed, the result selected), so that a
+// caller can interleave several pairs. kSmall: valid < 2^24 is known (fewer than 2^24 sites).
+template <bool kExact, bool kSmall>
+__device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t valid) {
+  const uint32_t v = valid ? valid : 1u;
+  const float P = kSmall ? (float)ts / (float)v : proportion(ts, v);
+  const float Q = kSmall ? (float)tv / (float)v : proportion(tv, v);
+  const double x = 1.0 - 2.0 * (double)P - (double)Q;
+  const double y = 1.0 - 2.0 * (double)Q;
+  const bool sat = valid == 0 || (float)x <= 0.0f || (float)y <= 0.0f;
+  const double xs = sat ? 1.0 : x, ys = sat ? 1.0 : y;
+  const float d = kExact ? (float)(-0.5 * log(xs) - 0.25 * log(ys))
+                         : (float)(-0.5 * log_unit(xs) - 0.25 * log_unit(ys));
+  // valid == 0 implies ts = tv = 0
+  return (ts | tv) == 0 ? (valid == 0 ? -1.0f : -0.0f) : (sat ? -1.0f : d);
+}
+
+template <bool kExact, bool kSmall>
+__device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid) {
+  const double b = 0.75;  // (alpha - 1) / alpha
+  const uint32_t v = valid ? valid : 1u;
+  const float p = kSmall ? (float)mismatches / (float)v : proportion(mismatches, v);
+  const double q = (double)p / b;
+  const bool sat = q >= 1.0;
+  const double x = sat ? 1.0 : 1.0 - q;
+  const float d = kExact ? (float)(-b * log(x)) : (float)(-b * log_unit(x));
+  return (valid == 0 || mismatches == 0) ? -0.0f : (sat ? -1.0f : d);
+}
+
+template <bool kExact>
+__device__ __noinline__ float k2p_distance(uint32_t ts, uint32_t tv, uint32_t valid) {
+  return k2p_value<kExact, false>(ts, tv, valid);
+}
+
+template <bool kExact>
+__device__ __noinline__ float jc_distance(uint32_t mismatches, uint32_t valid) {
+  return jc_value<kExact, false>(mismatches, valid);
+}
+#pragma clang fp contract(on)
+
+// Row r of the block triangle holds tiles (r, r..nb-1); it starts at r*nb - r*(r-1)/2.
+__device__ __forceinline__ void tile_coords(unsigned t, unsigned nb, unsigned& bi, unsigned& bj) {
+  const double f = 2.0 * nb + 1.0;
+  long long r = (long long)((f - sqrt(f * f - 8.0 * (double)t)) * 0.5);
+  if (r < 0) r = 0;
+  if (r > (long long)nb - 1) r = nb - 1;
+  auto start = [nb](long long q) { return q * (long long)nb - q * (q - 1) / 2; };
+  while (r + 1 < (long long)nb && start(r + 1) <= (long long)t) ++r;
+  while (r > 0 && start(r) > (long long)t) --r;
+  bi = (unsigned)r;
+  bj = (unsigned)(r + ((long long)t - start(r)));
+}
+
+// Patch order of the pair tiles (bi <= bj) of an nb x nb block grid: block rows in bands of 8,
+// each band walked column by column (the band's diagonal triangle first), so that any 32
+// consecutive tiles touch about 8 row blocks and 4 column blocks. Row-major order would give
+// 32 tiles over 33 blocks; here a group of work-groups resident on one XCD re-reads the same
+// blocks from its L2. Bijective onto the upper triangle (checked on the host).
+__host__ __device__ __forceinline__ void patch_coords(unsigned t, unsigned nb, unsigned& bi,
+                                                      unsigned& bj) {
+  unsigned R0 = 0, r = 0;
+  for (;;) {
+    r = nb - R0 < 8u ? nb - R0 : 8u;
+    const unsigned cnt = r * (r + 1) / 2 + (nb - R0 - r) * r;
+    if (t < cnt || R0 + r >= nb) break;
+    t -= cnt;
+    R0 += 8;
+  }
+  const unsigned tri = r * (r + 1) / 2;
+  if (t < tri) {
+    unsigned c = 0;
+    while ((c + 1) * (c + 2) / 2 <= t) ++c;
+    bi = R0 + (t - c * (c + 1) / 2);
+    bj = R0 + c;
+  } else {
+    t -= tri;
+    bi = R0 + t % r;
+    bj = R0 + r + t / r;
+  }
+}
+
+// Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
+// items so that tiles sharing a block row also share that XCD's L2.
+__device__ __forceinline__ unsigned xcd_item(unsigned bid, unsigned grid) {
+  const unsigned q = grid / 8, r = grid % 8, x = bid % 8, k = bid / 8;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+__device__ __forceinline__ unsigned find_job(const JobDesc* jobs, unsigned njobs, unsigned item) {
+  unsigned lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi + 1) / 2;
+    if (jobs[mid].item_base <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void push_sentinel(const JobDesc& J, unsigned i, unsigned j) {
+  const unsigned k = atomicAdd(J.sent_count, 1u);
+  if (k < J.sent_cap) J.sent[k] = ((unsigned long long)i << 32) | j;
+}
+
+// One finished pair (i < j < n) of job J: counts, or the corrected distance with the reference's
+// saturation marker, written to the job's layout; lmax tracks the largest positive distance.
+// mismatch_only: `tv` holds ts + tv (the JC-only count loop) and ts is unused.
+template <bool kExact>
+__device__ __forceinline__ void finish_pair(const JobDesc& J, unsigned i, unsigned j, uint32_t ts,
+                                            uint32_t tv, uint32_t valid, bool mismatch_only,
+                                            float& lmax) {
+  const unsigned n = J.n;
+  if (J.model == kCounts) {
+    const unsigned long long p = tri_index(i, j, n);
+    J.out_tv[p] = tv;
+    J.out_ts[p] = ts;
+    J.out_valid[p] = valid;
+    return;
+  }
+  float d;
+  if (J.model == kJC)
+    d = jc_distance<kExact>(mismatch_only ? tv : tv + ts, valid);
+  else
+    d = k2p_distance<kExact>(ts, tv, valid);
+  if (d == -1.0f) push_sentinel(J, i, j);
+  else if (d > lmax) lmax = d;
+  if (J.layout == kSquare) {
+    J.out[(size_t)i * J.ld + j] = d;
+    J.out[(size_t)j * J.ld + i] = d;
+  } else {
+    J.out[tri_index(i, j, n)] = d;
+  }
+}
+
+// Zero diagonal of a square output (the reference's post-process writes 0 on i == i).
+__device__ __forceinline__ void finish_diagonal(const JobDesc& J, unsigned i) {
+  if (J.model != kCounts && J.layout == kSquare) J.out[(size_t)i * J.ld + i] = 0.0f;
+}
+
+// Fold a thread's maximum into the job's running maximum: wave reduction, one atomic per wave.
+// Every lane of the wave must call it.
+__device__ __forceinline__ void fold_max(const JobDesc& J, float lmax, unsigned tid) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
+  if ((tid & 63) == 0 && lmax > 0.0f) atomicMax(J.maxbits, __float_as_uint(lmax));
+}
+
+}  // namespace
+}  // namespace rnj

@@ -6,161 +6,23 @@
 //   tv += bcnt(t); ts += bcnt(s); valid += bcnt(g)
 // i.e. 7 VALU instructions (v_and, 2x v_bitop3, v_xor, 3x v_bcnt with accumulate). JC needs only
 // ts + tv, which costs 6: the mismatch mask (Hi^Hj | Li^Lj) & g and g itself.
-#include "internal.hpp"
-
-#include <math.h>
+#include "device_common.hpp"
 
 namespace rnj {
 
 const TileConfig kTileConfigs[] = {
-    {"t64_m4x4_c16", 64, 4, 4, 16},
-    {"t64_m4x4_c8", 64, 4, 4, 8},
-    {"t128_m8x8_c16", 128, 8, 8, 16},
-    {"t128_m8x4_c16", 128, 8, 4, 16},
-    {"t32_m4x4_c16", 32, 4, 4, 16},
-    {"t32_m4x4_c8", 32, 4, 4, 8},
+    {"t64_m4x4_c16", 64, 4, 4, 16, false, 0},
+    {"t64_m4x4_c8", 64, 4, 4, 8, false, 0},
+    {"t128_m8x8_c16", 128, 8, 8, 16, false, 0},
+    {"t128_m8x4_c16", 128, 8, 4, 16, false, 0},
+    {"t32_m4x4_c16", 32, 4, 4, 16, false, 0},
+    {"t32_m4x4_c8", 32, 4, 4, 8, false, 0},
+    {"mfma_fp4_t128_d2", 128, 0, 0, 2, true, 2},
+    {"mfma_fp4_t128_d4", 128, 0, 0, 2, true, 4},
 };
 const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 
 namespace {
-
-// (a ^ b) & c in one v_bitop3_b32 (truth table over a=0xF0, b=0xCC, c=0xAA).
-__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x28u);
-}
-
-// popcount(x) + acc as one v_bcnt_u32_b32. Written out because LLVM otherwise re-associates
-// the per-word accumulations into v_bcnt + v_add3 trees (1.5 extra VALU ops per pair and word).
-__device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {
-  uint32_t r;
-  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
-  return r;
-}
-
-// Pointer in the global address space, so loads compile to global_load rather than flat_load
-// (the job table hands over generic pointers).
-template <class T>
-using gptr = const __attribute__((address_space(1))) T*;
-
-// Native 4 x u32 vector (ext_vector_type) for 128-bit global and LDS accesses.
-typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-
-// ---- corrections ---------------------------------------------------------------------------
-// Kimura 2-parameter (KimuraDistance.cpp:339-348) and Jukes-Cantor with alpha = 4
-// (JCdistance.cpp:170-183). What decides the result's class is kept exactly as the reference
-// computes it:
-//  * the proportions: (float)((double)a / b) of integers below 2^24 equals the correctly rounded
-//    float quotient (a double rounding 53 -> 24 bits is innocuous for a quotient), so the float
-//    division is used, and the double one above 2^24;
-//  * the arguments of the logarithms, 1-2P-Q and 1-2Q (K2P) and 1-p/0.75 (JC), formed in double
-//    from the float proportions in the reference's order (exact for K2P), and the saturation
-//    tests on them, so the saturated pairs (-1) are the same;
-//  * zero distances: ts = tv = 0 gives -0.0f in both models, as the reference's -c*log(1.0) does.
-// The logarithms of those double arguments use log_unit (below) unless kExact, which calls the
-// library's compensated FP64 log; either way the reference's FP64 expression is evaluated.
-#pragma clang fp contract(off)
-__device__ __forceinline__ float proportion(uint32_t a, uint32_t valid) {
-  if (valid < (1u << 24)) return (float)a / (float)valid;
-  return (float)((double)a / (double)valid);
-}
-
-// Natural log for 0 < x <= 1 (a normal double), in plain FP64 arithmetic: x = 2^k * m with
-// m in [sqrt(1/2), sqrt(2)), s = (m - 1) / (m + 1), log(m) = 2 atanh(s) = 2s (1 + s^2/3 + ... +
-// s^20/21), which leaves a truncation below 2^-56 for |s| <= 0.1716, and k ln 2 with ln 2 split
-// into a 32-bit head (k * head is exact) and a tail. The quotient uses the hardware reciprocal
-// refined by two Newton steps. The result is within about 2 ulp of double, so after rounding to
-// float it equals the reference's float value except in rare ties to a float rounding boundary;
-// it costs about a third of the instructions of the compensated library log.
-__device__ __forceinline__ double log_unit(double x) {
-  const double kLn2Hi = 0x1.62e42ff000000p-1;
-  const double kLn2Lo = -0x1.718432a1b0e26p-35;
-  int k;
-  double m = frexp(x, &k);            // m in [0.5, 1)
-  if (m < 0.70710678118654757) {      // m in [sqrt(1/2), sqrt(2))
-    m = m * 2.0;
-    k -= 1;
-  }
-  const double f = m - 1.0;           // exact
-  const double d = 2.0 + f;
-  double r = __builtin_amdgcn_rcp(d);
-  r = fma(r, fma(-d, r, 1.0), r);
-  r = fma(r, fma(-d, r, 1.0), r);
-  const double sd = f * r;
-  const double t = sd * sd;
-  double p = 1.0 / 21.0;
-  p = fma(p, t, 1.0 / 19.0);
-  p = fma(p, t, 1.0 / 17.0);
-  p = fma(p, t, 1.0 / 15.0);
-  p = fma(p, t, 1.0 / 13.0);
-  p = fma(p, t, 1.0 / 11.0);
-  p = fma(p, t, 1.0 / 9.0);
-  p = fma(p, t, 1.0 / 7.0);
-  p = fma(p, t, 1.0 / 5.0);
-  p = fma(p, t, 1.0 / 3.0);
-  const double two_s = 2.0 * sd;
-  const double lm = fma(two_s * t, p, two_s);
-  const double kd = (double)k;
-  return fma(kd, kLn2Hi, fma(kd, kLn2Lo, lm));
-}
-
-template <bool kExact>
-__device__ __noinline__ float k2p_distance(uint32_t ts, uint32_t tv, uint32_t valid) {
-  if (valid == 0) return -1.0f;
-  if ((ts | tv) == 0) return -0.0f;
-  const float P = proportion(ts, valid);
-  const float Q = proportion(tv, valid);
-  const double x = 1.0 - 2.0 * (double)P - (double)Q;
-  const double y = 1.0 - 2.0 * (double)Q;
-  if ((float)x <= 0.0f || (float)y <= 0.0f) return -1.0f;
-  if (kExact) return (float)(-0.5 * log(x) - 0.25 * log(y));
-  return (float)(-0.5 * log_unit(x) - 0.25 * log_unit(y));
-}
-
-template <bool kExact>
-__device__ __noinline__ float jc_distance(uint32_t mismatches, uint32_t valid) {
-  const double b = 0.75;  // (alpha - 1) / alpha
-  if (valid == 0 || mismatches == 0) return -0.0f;
-  const float p = proportion(mismatches, valid);
-  if ((double)p / b >= 1.0) return -1.0f;
-  const double x = 1.0 - (double)p / b;
-  if (kExact) return (float)(-b * log(x));
-  return (float)(-b * log_unit(x));
-}
-#pragma clang fp contract(on)
-
-// Row r of the block triangle holds tiles (r, r..nb-1); it starts at r*nb - r*(r-1)/2.
-__device__ __forceinline__ void tile_coords(unsigned t, unsigned nb, unsigned& bi, unsigned& bj) {
-  const double f = 2.0 * nb + 1.0;
-  long long r = (long long)((f - sqrt(f * f - 8.0 * (double)t)) * 0.5);
-  if (r < 0) r = 0;
-  if (r > (long long)nb - 1) r = nb - 1;
-  auto start = [nb](long long q) { return q * (long long)nb - q * (q - 1) / 2; };
-  while (r + 1 < (long long)nb && start(r + 1) <= (long long)t) ++r;
-  while (r > 0 && start(r) > (long long)t) --r;
-  bi = (unsigned)r;
-  bj = (unsigned)(r + ((long long)t - start(r)));
-}
-
-// Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
-// items so that tiles sharing a block row also share that XCD's L2.
-__device__ __forceinline__ unsigned xcd_item(unsigned bid, unsigned grid) {
-  const unsigned q = grid / 8, r = grid % 8, x = bid % 8, k = bid / 8;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
-__device__ __forceinline__ unsigned find_job(const JobDesc* jobs, unsigned njobs, unsigned item) {
-  unsigned lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const unsigned mid = (lo + hi + 1) / 2;
-    if (jobs[mid].item_base <= item) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ void push_sentinel(const JobDesc& J, unsigned i, unsigned j) {
-  const unsigned k = atomicAdd(J.sent_count, 1u);
-  if (k < J.sent_cap) J.sent[k] = ((unsigned long long)i << 32) | j;
-}
 
 // Chunk `chunk` of the i and j taxon blocks into registers: linear element lin = q + Q*(w + CW*p)
 // of a block is word row (chunk*CW + w) of plane p, uint4 column q.
@@ -190,7 +52,6 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
                                               const uint32_t (&acc_b)[MI][MJ],
                                               const uint32_t (&acc_v)[MI][MJ]) {
   const unsigned n = J.n;
-  const int model = J.model;
   float lmax = 0.0f;
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
@@ -199,35 +60,11 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
     for (int b = 0; b < MJ; ++b) {
       const unsigned j = bj * TILE + tx * MJ + b;
       if (j >= n) continue;
-      if (i < j) {
-        if (model == kCounts) {
-          const unsigned long long p = tri_index(i, j, n);
-          J.out_tv[p] = acc_a[a][b];
-          J.out_ts[p] = acc_b[a][b];
-          J.out_valid[p] = acc_v[a][b];
-          continue;
-        }
-        float d;
-        if (model == kJC)
-          d = jc_distance<kExact>(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b], acc_v[a][b]);
-        else
-          d = k2p_distance<kExact>(acc_b[a][b], acc_a[a][b], acc_v[a][b]);
-        if (d == -1.0f) push_sentinel(J, i, j);
-        else if (d > lmax) lmax = d;
-        if (J.layout == kSquare) {
-          J.out[(size_t)i * J.ld + j] = d;
-          J.out[(size_t)j * J.ld + i] = d;
-        } else {
-          J.out[tri_index(i, j, n)] = d;
-        }
-      } else if (i == j && model != kCounts && J.layout == kSquare) {
-        J.out[(size_t)i * J.ld + i] = 0.0f;
-      }
+      if (i < j) finish_pair<kExact>(J, i, j, acc_b[a][b], acc_a[a][b], acc_v[a][b], JC6, lmax);
+      else if (i == j) finish_diagonal(J, i);
     }
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
-  if ((tid & 63) == 0 && lmax > 0.0f) atomicMax(J.maxbits, __float_as_uint(lmax));
+  fold_max(J, lmax, tid);
 }
 
 // Partial counts of one (tile, split): [3][NT][MI*MJ] u32, thread-major so that every thread
@@ -649,12 +486,16 @@ hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* job
     case 3: return launch_cfg<128, 8, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
     case 4: return launch_cfg<32, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
     case 5: return launch_cfg<32, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    default: return hipErrorInvalidValue;
+    default:
+      if (config < kNumTileConfigs && kTileConfigs[config].mfma)
+        return launch_mfma(kTileConfigs[config].depth, exact, jobs, njobs, total_items, stream);
+      return hipErrorInvalidValue;
   }
 }
 
 unsigned tile_threads(int config) {
   const TileConfig& c = kTileConfigs[config];
+  if (c.mfma) return 256;
   return (c.tile / c.micro_i) * (c.tile / c.micro_j);
 }
 
