@@ -107,14 +107,10 @@ RNJ_EXPORT int rnj_create(int device, rnj_context** out);
 RNJ_EXPORT int rnj_destroy(rnj_context* ctx);
 RNJ_EXPORT int rnj_synchronize(rnj_context* ctx); /* waits for all work the context enqueued */
 
-/* Tile-kernel configurations: -1 = choose per shape with the built-in autotuner (default).
- * The VALU configurations count with 32-bit bit-plane logic; "mfma_fp4_t128" counts the same
- * exact integers as Gram products of {-1,0,+1} FP4 site embeddings on the matrix cores (used for
- * alignments of >= 1024 taxa and < 2^24 sites, when matrix cores are enabled). */
+/* Tile-kernel configurations: -1 = choose per shape with the built-in autotuner (default). */
 RNJ_EXPORT int rnj_tile_config_count(void);
 RNJ_EXPORT const char* rnj_tile_config_name(int config);
 RNJ_EXPORT int rnj_set_tile_config(rnj_context* ctx, int config);
-RNJ_EXPORT int rnj_use_matrix_cores(rnj_context* ctx, int enable); /* default 1 */
 RNJ_EXPORT int rnj_get_tile_config(rnj_context* ctx, unsigned n, unsigned L, int model, int* config);
 
 /* Alignments packed once into HBM as H/L/V bit planes.

@@ -1,6 +1,5 @@
-// Device code shared by the VALU tile kernels (kernels.hip) and the MFMA tile kernel
-// (mfma.hip): bit helpers, the JC / K2P corrections, tile and job mapping, and the per-pair
-// epilogue. Included once per translation unit (everything has internal linkage).
+// Device code of the pair-tile kernels: bit helpers, the JC / K2P corrections, tile and job
+// mapping, and the per-pair epilogue. Included by one translation unit (internal linkage).
 #pragma once
 
 #include "internal.hpp"

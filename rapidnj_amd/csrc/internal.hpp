@@ -34,7 +34,6 @@ enum Layout : int { kTriangle = 0, kSquare = 1 };
 // that job's tiles from fin_base, sums them and applies the epilogue.
 struct JobDesc {
   const uint32_t* planes;     // [3][W_pad][n_pad]
-  const uint32_t* emb;        // FP4 fragments for the matrix-core kernel (mfma.hip), or null
   unsigned long long plane_stride;  // W_pad * n_pad
   unsigned n, n_pad, W;       // W: words to process (multiple of kWordPad, <= W_pad)
   unsigned nblk;              // ceil(n / TI) for the launch's tile edge (TI == TJ)
@@ -70,8 +69,6 @@ struct TileConfig {
   unsigned micro_i;  // pairs per thread along i
   unsigned micro_j;  // pairs per thread along j
   unsigned chunk;    // words of 32 sites staged in LDS per step
-  bool mfma;         // the matrix-core kernel (needs the FP4 fragments; never split)
-  unsigned depth;    // matrix-core kernel: 64-site blocks loaded ahead of the MFMAs
 };
 extern const TileConfig kTileConfigs[];
 extern const int kNumTileConfigs;
@@ -88,12 +85,7 @@ hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t s
 hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
                         unsigned total_items, unsigned finish_items, hipStream_t stream);
 unsigned tile_threads(int config);  // work-group size of a configuration
-// Matrix-core path (mfma.hip): FP4 fragments of an alignment's planes (n_pad * W_pad * 64 bytes)
-// and the 128 x 128 pair-tile kernel over them.
-hipError_t launch_pack_fp4(const uint32_t* planes, unsigned n_pad, unsigned W_pad, uint32_t* emb,
-                           hipStream_t stream);
-hipError_t launch_mfma(unsigned depth, bool exact, const JobDesc* jobs, unsigned njobs,
-                       unsigned total, hipStream_t stream);
+
 hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
 hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
                                   unsigned L, uint8_t* codes, hipStream_t stream);

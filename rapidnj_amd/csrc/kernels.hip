@@ -11,14 +11,12 @@
 namespace rnj {
 
 const TileConfig kTileConfigs[] = {
-    {"t64_m4x4_c16", 64, 4, 4, 16, false, 0},
-    {"t64_m4x4_c8", 64, 4, 4, 8, false, 0},
-    {"t128_m8x8_c16", 128, 8, 8, 16, false, 0},
-    {"t128_m8x4_c16", 128, 8, 4, 16, false, 0},
-    {"t32_m4x4_c16", 32, 4, 4, 16, false, 0},
-    {"t32_m4x4_c8", 32, 4, 4, 8, false, 0},
-    {"mfma_fp4_t128_d2", 128, 0, 0, 2, true, 2},
-    {"mfma_fp4_t128_d4", 128, 0, 0, 2, true, 4},
+    {"t64_m4x4_c16", 64, 4, 4, 16},
+    {"t64_m4x4_c8", 64, 4, 4, 8},
+    {"t128_m8x8_c16", 128, 8, 8, 16},
+    {"t128_m8x4_c16", 128, 8, 4, 16},
+    {"t32_m4x4_c16", 32, 4, 4, 16},
+    {"t32_m4x4_c8", 32, 4, 4, 8},
 };
 const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 
@@ -102,7 +100,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
   const unsigned local = item - J.item_base;
   const unsigned tile = local / J.splits, split = local % J.splits;
   unsigned bi, bj;
-  tile_coords(tile, J.nblk, bi, bj);
+  patch_coords(tile, J.nblk, bi, bj);
 
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
@@ -207,7 +205,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
   const unsigned tile = item - J.fin_base;
   if (J.splits <= 1 || tile >= J.tiles) return;
   unsigned bi, bj;
-  tile_coords(tile, J.nblk, bi, bj);
+  patch_coords(tile, J.nblk, bi, bj);
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
   uint32_t acc_a[MI][MJ], acc_b[MI][MJ], acc_v[MI][MJ];
@@ -486,16 +484,12 @@ hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* job
     case 3: return launch_cfg<128, 8, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
     case 4: return launch_cfg<32, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
     case 5: return launch_cfg<32, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    default:
-      if (config < kNumTileConfigs && kTileConfigs[config].mfma)
-        return launch_mfma(kTileConfigs[config].depth, exact, jobs, njobs, total_items, stream);
-      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
   }
 }
 
 unsigned tile_threads(int config) {
   const TileConfig& c = kTileConfigs[config];
-  if (c.mfma) return 256;
   return (c.tile / c.micro_i) * (c.tile / c.micro_j);
 }
 

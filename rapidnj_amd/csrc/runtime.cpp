@@ -173,7 +173,7 @@ LaunchPlan plan_launch(const std::vector<JobSpec>& specs, int config, unsigned c
   const size_t per_split = (size_t)3 * c.tile * c.tile;
   for (size_t k = 0; k < specs.size(); ++k) {
     const unsigned chunks = specs[k].W / c.chunk;
-    unsigned S = c.mfma ? 1u : std::max(1u, std::min(want, chunks / 2));
+    unsigned S = std::max(1u, std::min(want, chunks / 2));
     const unsigned cps = (chunks + S - 1) / S;  // chunks per split
     S = (chunks + cps - 1) / cps;
     P.splits.push_back(S);
@@ -198,7 +198,6 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
     JobDesc& d = ws->h_jobs[k];
     std::memset(&d, 0, sizeof d);
     d.planes = s.planes;
-    d.emb = s.emb;
     d.plane_stride = (unsigned long long)s.W_pad * s.n_pad;
     d.n = s.n;
     d.n_pad = s.n_pad;
@@ -231,35 +230,6 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
 
 void free_workspace_public(Workspace* ws) { free_workspace(ws); }
 
-// The matrix-core kernel needs every job's FP4 fragments and exact FP32 sums (< 2^24 sites).
-static bool mfma_eligible(Context* ctx, const std::vector<JobSpec>& specs) {
-  if (!ctx->matrix_cores) return false;
-  for (const JobSpec& s : specs)
-    if (!s.emb || s.W >= (1u << 19)) return false;
-  return true;
-}
-
-void ensure_fp4(Context* ctx, Alignment* a, hipStream_t stream) {
-  if (!ctx->matrix_cores || a->emb || a->n < 1024 || a->W >= (1u << 19)) return;
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  if (a->emb) return;
-  // 64 bytes per taxon and 32-site word (5.3x the planes); only where a quarter of the free
-  // memory covers it
-  const size_t bytes = (size_t)a->n_pad * a->W * 64;
-  size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > free_b / 4) return;
-  uint32_t* emb = nullptr;
-  if (hipMalloc((void**)&emb, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    return;
-  }
-  if (launch_pack_fp4(a->planes, a->n_pad, a->W, emb, stream) != hipSuccess) {
-    (void)hipFree(emb);
-    return;
-  }
-  a->emb = emb;
-}
-
 // Configuration before (or without) tuning, from measurements on MI355X: below ~4096 taxa the
 // 32-taxon tile fills the device better (2000 x 10000: 0.21 ms vs 0.24 ms), above it the
 // 64-taxon tile with 8-word chunks is fastest (20000 x 5000: 7.2 ms).
@@ -273,7 +243,7 @@ int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
   if (ctx->tile_config >= 0) return ctx->tile_config;
   std::lock_guard<std::mutex> lock(ctx->mu);
   auto it = ctx->tuned.find(std::make_tuple(ilog2(n), ilog2(W), (int)jc_only));
-  if (it != ctx->tuned.end()) return ctx->matrix_cores ? it->second.first : it->second.second;
+  if (it != ctx->tuned.end()) return it->second;
   return default_config(n, W, jc_only);
 }
 
@@ -286,12 +256,10 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, si
   hipEvent_t e0, e1;
   RNJ_HIP(hipEventCreate(&e0));
   RNJ_HIP(hipEventCreate(&e1));
-  float best_ms = 1e30f, valu_ms = 1e30f;
+  float best_ms = 1e30f;
   *best = default_config(n_key, W_key, jc_only);
-  int best_valu = *best;
   const unsigned cus = (unsigned)ctx->prop.multiProcessorCount;
   for (int c = 0; c < kNumTileConfigs; ++c) {
-    if (kTileConfigs[c].mfma && !mfma_eligible(ctx, specs)) continue;
     const LaunchPlan P = plan_launch(specs, c, cus);
     if (P.items >= (1ull << 31)) continue;
     RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
@@ -314,15 +282,11 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, si
       best_ms = ms;
       *best = c;
     }
-    if (!kTileConfigs[c].mfma && ms < valu_ms) {
-      valu_ms = ms;
-      best_valu = c;
-    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lock(ctx->mu);
-  ctx->tuned[std::make_tuple(ilog2(n_key), ilog2(W_key), (int)jc_only)] = {*best, best_valu};
+  ctx->tuned[std::make_tuple(ilog2(n_key), ilog2(W_key), (int)jc_only)] = *best;
   return RNJ_OK;
 }
 
@@ -344,18 +308,12 @@ LaunchShape shape_of(const std::vector<JobSpec>& specs) {
 int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Workspace* ws,
                   bool exact, hipStream_t stream, bool allow_tune, int* config) {
   *config = ctx->tile_config;
-  if (*config >= 0) {
-    if (kTileConfigs[*config].mfma && !mfma_eligible(ctx, specs))
-      return fail(RNJ_E_INVALID, "the matrix-core configuration needs FP4 fragments (>= 1024 "
-                                 "taxa, < 2^24 sites, matrix cores enabled)");
-    return RNJ_OK;
-  }
+  if (*config >= 0) return RNJ_OK;
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
     auto it = ctx->tuned.find(std::make_tuple(ilog2(sh.n_key), ilog2(sh.W_key), (int)sh.jc_only));
     if (it != ctx->tuned.end()) {
-      const int c = it->second.first;
-      *config = !kTileConfigs[c].mfma || mfma_eligible(ctx, specs) ? c : it->second.second;
+      *config = it->second;
       return RNJ_OK;
     }
   }
@@ -460,7 +418,6 @@ int check_aln(rnj_context* ctx, const rnj_alignment* aln) {
 JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size_t ld) {
   JobSpec s{};
   s.planes = a->planes;
-  s.emb = a->emb;
   s.n = a->n;
   s.n_pad = a->n_pad;
   s.W_pad = a->W;
@@ -547,12 +504,6 @@ int rnj_tile_config_count(void) { return kNumTileConfigs; }
 const char* rnj_tile_config_name(int config) {
   if (config < 0 || config >= kNumTileConfigs) return nullptr;
   return kTileConfigs[config].name;
-}
-
-int rnj_use_matrix_cores(rnj_context* ctx, int enable) {
-  if (!ctx) return fail(RNJ_E_INVALID, "null context");
-  ctx->matrix_cores = enable != 0;
-  return RNJ_OK;
 }
 
 int rnj_set_tile_config(rnj_context* ctx, int config) {
@@ -668,7 +619,6 @@ int rnj_alignment_free(rnj_alignment* aln) {
     (void)hipDeviceSynchronize();
     (void)hipFree(aln->planes);
   }
-  if (aln->emb) (void)hipFree(aln->emb);
   delete aln;
   return RNJ_OK;
 }
@@ -706,7 +656,6 @@ int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, int model, 
   if (layout == RNJ_SQUARE && ld < aln->n) return fail(RNJ_E_INVALID, "ld smaller than n");
   rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
-  ensure_fp4(ctx, const_cast<rnj_alignment*>(aln), stream_of(ctx, stream));
   std::vector<JobSpec> jobs{spec_for(aln, model, layout, out, ld)};
   jobs[0].max_out = max_out;
   return launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, (flags & RNJ_FLAG_EXACT) != 0,
@@ -720,7 +669,6 @@ int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, 
   if (!ts || !tv || !valid) return fail(RNJ_E_INVALID, "null output");
   rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
-  ensure_fp4(ctx, const_cast<rnj_alignment*>(aln), stream_of(ctx, stream));
   std::vector<JobSpec> jobs{spec_for(aln, kCounts, kTriangle, nullptr, 0)};
   jobs[0].out_ts = ts;
   jobs[0].out_tv = tv;
@@ -747,7 +695,6 @@ static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, f
   float* dmax = nullptr;
   RNJ_HIP(hipMallocAsync((void**)&dmat, (size_t)n * n * sizeof(float), s));
   RNJ_HIP(hipMallocAsync((void**)&dmax, sizeof(float), s));
-  ensure_fp4(ctx, const_cast<rnj_alignment*>(aln), s);
   std::vector<JobSpec> jobs{spec_for(aln, model, kSquare, dmat, n)};
   jobs[0].max_out = dmax;
   rc = launch_jobs(ctx, jobs, false, true, s);

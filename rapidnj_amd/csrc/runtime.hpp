@@ -57,13 +57,10 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   int tile_config = -1;  // -1: autotune per shape class
-  bool matrix_cores = true;  // the autotuner may pick the matrix-core (MFMA) kernel
   std::mutex mu;
   std::vector<std::unique_ptr<Workspace>> pool;
   // autotune cache: (log2 n bucket, log2 W bucket, jc_only) -> config
-  // per shape class: the fastest configuration and the fastest VALU one (for launches the
-  // matrix-core kernel cannot serve)
-  std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
+  std::map<std::tuple<int, int, int>, int> tuned;
   hipDeviceProp_t prop{};
 };
 
@@ -73,14 +70,12 @@ struct Alignment {
   unsigned n_pad = 0;        // multiple of kTaxaPad
   unsigned W = 0;            // 32-site words, multiple of kWordPad (= W_pad of the planes)
   uint32_t* planes = nullptr;  // [3][W][n_pad]
-  uint32_t* emb = nullptr;     // FP4 fragments for the matrix-core kernel, built on first use
   size_t bytes() const { return (size_t)3 * W * n_pad * sizeof(uint32_t); }
 };
 
 // One matrix of a launch, before the runtime numbers tiles and binds scratch.
 struct JobSpec {
   const uint32_t* planes;
-  const uint32_t* emb;  // FP4 fragments or null (then the matrix-core kernel is not eligible)
   unsigned n, n_pad, W_pad, W;
   int model;   // Model
   int layout;  // Layout
@@ -106,9 +101,6 @@ int launch_jobs(Context* ctx, std::vector<JobSpec>& jobs, bool raw, bool exact,
                 hipStream_t stream, bool allow_tune = true);
 
 int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only);
-// Build the alignment's FP4 fragments if the matrix-core kernel could serve it (enough taxa,
-// fewer than 2^24 sites, matrix cores enabled); failures only leave it ineligible.
-void ensure_fp4(Context* ctx, Alignment* a, hipStream_t stream);
 
 // The pieces of launch_jobs, for callers that own their workspace (captured plans).
 struct LaunchShape {

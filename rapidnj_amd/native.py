@@ -45,7 +45,6 @@ SIGNATURES = {
     "rnj_tile_config_count": (ctypes.c_int, []),
     "rnj_tile_config_name": (ctypes.c_char_p, [ctypes.c_int]),
     "rnj_set_tile_config": (ctypes.c_int, [_vp, ctypes.c_int]),
-    "rnj_use_matrix_cores": (ctypes.c_int, [_vp, ctypes.c_int]),
     "rnj_get_tile_config": (ctypes.c_int, [_vp, _u, _u, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rnj_alignment_from_chars": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int, ctypes.c_int,
                                                 ctypes.POINTER(_vp)]),
@@ -218,9 +217,6 @@ class Context:
 
     def set_tile_config(self, config: int) -> None:
         check(lib().rnj_set_tile_config(self.handle, config))
-
-    def use_matrix_cores(self, enable: bool) -> None:
-        check(lib().rnj_use_matrix_cores(self.handle, int(enable)))
 
     def tile_config(self, n: int, L: int, model="k2p") -> int:
         c = ctypes.c_int()
