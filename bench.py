@@ -68,6 +68,90 @@ def cpu_baseline(aln, model: str, sample_n: int, threads: int) -> dict:
                       f"compute+post-process {secs:.2f} s"}
 
 
+def _pmc_pass(counter: str, workload: str, tile_config: int) -> dict | None:
+    """One rocprofv3 counter pass over a child `bench.py --probe-traffic` run: per-dispatch values
+    of `counter` (KiB) for the last tile-kernel dispatch and for the 1 GiB elementwise kernel
+    that ends the probe. None when rocprofv3 is missing or the pass fails."""
+    import csv
+    import shutil
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        cmd = [prof, "--pmc", counter, "-d", tmp, "-o", "run", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--probe-traffic", "--workload", workload,
+               "--tile-config", str(tile_config)]
+        env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=ROOT)
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, check=True, timeout=240,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        except (subprocess.SubprocessError, OSError) as e:
+            log(f"rocprofv3 --pmc {counter} failed: {e}")
+            return None
+        files = []
+        for root, _, names in os.walk(tmp):
+            files += [os.path.join(root, f) for f in names if f.endswith("counter_collection.csv")]
+        if not files:
+            return None
+        per_dispatch: dict[int, list] = {}
+        with open(files[0]) as fh:
+            for r in csv.DictReader(fh):
+                d = int(r["Dispatch_Id"])
+                ent = per_dispatch.setdefault(d, [r["Kernel_Name"], 0.0])
+                ent[1] += float(r["Counter_Value"])
+    order = sorted(per_dispatch)
+    tiles = [d for d in order if "tile_kernel" in per_dispatch[d][0]]
+    if not tiles:
+        return None
+    return {"tile_kib": per_dispatch[tiles[-1]][1], "copy_kib": per_dispatch[order[-1]][1],
+            "tile_kernel": per_dispatch[tiles[-1]][0].split("(")[0]}
+
+
+def measure_traffic(workload: str, tile_config: int) -> dict | None:
+    """HBM bytes of one tile-kernel launch: FETCH_SIZE and WRITE_SIZE (two counter passes), each
+    scaled by what the counter reads for a 1 GiB -> 1 GiB elementwise kernel in the same pass."""
+    f = _pmc_pass("FETCH_SIZE", workload, tile_config)
+    w = _pmc_pass("WRITE_SIZE", workload, tile_config)
+    if not f or not w or f["copy_kib"] <= 0 or w["copy_kib"] <= 0:
+        return None
+    gib_kib = float(1 << 20)
+    fetch_scale, write_scale = gib_kib / f["copy_kib"], gib_kib / w["copy_kib"]
+    fetch_b = f["tile_kib"] * fetch_scale * 1024
+    write_b = w["tile_kib"] * write_scale * 1024
+    return {"bytes": fetch_b + write_b, "fetch_bytes": fetch_b, "write_bytes": write_b,
+            "fetch_scale": round(fetch_scale, 3), "write_scale": round(write_scale, 3),
+            "counters": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE (KiB per dispatch), each "
+                        "scaled by its reading for a 1 GiB -> 1 GiB elementwise kernel in the "
+                        "same run"}
+
+
+def probe_traffic(args) -> int:
+    """Child of measure_traffic: two raw launches of the workload, then one elementwise kernel that
+    reads 1 GiB and writes 1 GiB (ordinary vector loads and stores, like the tile kernel's)."""
+    import torch
+    from rapidnj_amd import native as N
+    from rapidnj_amd.workloads import WORKLOADS, simulate_alignment
+    wl = WORKLOADS[args.workload]
+    aln = simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    ctx = N.Context(0)
+    if args.tile_config >= 0:
+        ctx.set_tile_config(args.tile_config)
+    A = ctx.alignment(aln)
+    pairs = wl.n * (wl.n - 1) // 2
+    out = torch.empty(pairs, dtype=torch.float32, device="cuda:0")
+    for _ in range(2):
+        ctx.distances(A, wl.model, out=out, raw=True)
+    torch.cuda.synchronize()
+    src = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 2.0, out=dst)
+    torch.cuda.synchronize()
+    A.close()
+    ctx.close()
+    return 0
+
+
 def _gen_job(job):
     from rapidnj_amd.workloads import simulate_alignment
     return simulate_alignment(job.n, job.L, job.gap, job.seed)
@@ -202,7 +286,12 @@ def main() -> int:
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-traffic kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM bytes per launch")
+    ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.probe_traffic:
+        return probe_traffic(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per "
               "box); run with --gpus 1", file=sys.stderr)
@@ -332,6 +421,15 @@ def main() -> int:
             "open_loop": served_run(ctx, served_jobs, served_alns, open_loop=True),
             "closed_loop": served_run(ctx, served_jobs, served_alns, open_loop=False),
             "one_job_per_launch": one_job_per_launch(ctx, served_jobs, served_alns)}
+
+    if not args.no_traffic and not wl.replicates:
+        log("HBM traffic: two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE)")
+        t = measure_traffic(args.workload, cfg)
+        if t is not None:
+            roofline["traffic"] = round(t["bytes"])
+            roofline["traffic_detail"] = {k: (round(v) if isinstance(v, float) and v > 1e3 else v)
+                                          for k, v in t.items() if k != "bytes"}
+            roofline["traffic_detail"]["algo_bytes"] = algo_bytes
 
     base = None
     if not args.no_cpu_baseline:

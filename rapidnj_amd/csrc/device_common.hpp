@@ -184,8 +184,18 @@ __device__ __forceinline__ unsigned find_job(const JobDesc* jobs, unsigned njobs
   return lo;
 }
 
+// Append (i, j) to the job's saturated-pair list. The lanes of a wave that reach this call
+// together take consecutive slots with one atomic on the list's counter (saturated pairs come
+// in clusters of distant clades, and one counter hit by every such pair serialises).
 __device__ __forceinline__ void push_sentinel(const JobDesc& J, unsigned i, unsigned j) {
-  const unsigned k = atomicAdd(J.sent_count, 1u);
+  const unsigned long long active = __ballot(1);
+  const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(active >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((unsigned)active, 0u));
+  const int leader = __ffsll((long long)active) - 1;
+  unsigned base = 0;
+  if (below == 0) base = atomicAdd(J.sent_count, (unsigned)__popcll(active));
+  base = __shfl(base, leader);
+  const unsigned k = base + below;
   if (k < J.sent_cap) J.sent[k] = ((unsigned long long)i << 32) | j;
 }
 

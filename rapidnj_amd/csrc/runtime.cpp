@@ -249,42 +249,53 @@ int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
 
 // Time every configuration on this launch (stream-synchronous) and remember the fastest for the
 // launch's shape class. Only for launches big enough that the timing means something and only
-// outside stream capture.
+// outside stream capture. The device is warmed first with two launches of the default (a cold
+// device runs its first milliseconds at lower clocks), then every configuration is timed in two
+// rounds of two launches, and each keeps its fastest launch, so that a clock change during the
+// sweep does not decide the choice.
 static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, size_t* sent_caps,
                     bool jc_only, bool exact, unsigned n_key, unsigned W_key, hipStream_t stream,
                     int* best) {
   hipEvent_t e0, e1;
   RNJ_HIP(hipEventCreate(&e0));
   RNJ_HIP(hipEventCreate(&e1));
-  float best_ms = 1e30f;
   *best = default_config(n_key, W_key, jc_only);
   const unsigned cus = (unsigned)ctx->prop.multiProcessorCount;
-  for (int c = 0; c < kNumTileConfigs; ++c) {
+  std::vector<float> ms(kNumTileConfigs, 1e30f);
+  auto run = [&](int c, bool timed) -> int {
     const LaunchPlan P = plan_launch(specs, c, cus);
-    if (P.items >= (1ull << 31)) continue;
+    if (P.items >= (1ull << 31)) return RNJ_OK;
     RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
     number_jobs(specs, c, P, ws, sent_caps);
     RNJ_HIP(hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
                            hipMemcpyHostToDevice, stream));
-    float ms = 1e30f;
-    for (int rep = 0; rep < 3; ++rep) {
-      RNJ_HIP(launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream));
-      RNJ_HIP(hipEventRecord(e0, stream));
-      RNJ_HIP(launch_tiles(c, jc_only, exact, ws->d_jobs, (unsigned)specs.size(),
-                           (unsigned)P.items, (unsigned)P.finish_items, stream));
-      RNJ_HIP(hipEventRecord(e1, stream));
-      RNJ_HIP(hipEventSynchronize(e1));
-      float t = 0;
-      RNJ_HIP(hipEventElapsedTime(&t, e0, e1));
-      if (rep > 0) ms = std::min(ms, t);  // the first launch of a configuration warms it up
+    RNJ_HIP(launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream));
+    RNJ_HIP(hipEventRecord(e0, stream));
+    RNJ_HIP(launch_tiles(c, jc_only, exact, ws->d_jobs, (unsigned)specs.size(), (unsigned)P.items,
+                         (unsigned)P.finish_items, stream));
+    RNJ_HIP(hipEventRecord(e1, stream));
+    RNJ_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    RNJ_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (timed) ms[c] = std::min(ms[c], t);
+    return RNJ_OK;
+  };
+  int rc = RNJ_OK;
+  for (int k = 0; k < 2 && rc == RNJ_OK; ++k) rc = run(*best, false);
+  for (int round = 0; round < 2 && rc == RNJ_OK; ++round)
+    for (int c = 0; c < kNumTileConfigs && rc == RNJ_OK; ++c) {
+      if (round == 0) rc = run(c, false);  // first launch of a configuration: code and caches warm
+      for (int k = 0; k < 2 && rc == RNJ_OK; ++k) rc = run(c, true);
     }
-    if (ms < best_ms) {
-      best_ms = ms;
-      *best = c;
-    }
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc != RNJ_OK) return rc;
+  float best_ms = 1e30f;
+  for (int c = 0; c < kNumTileConfigs; ++c)
+    if (ms[c] < best_ms) {
+      best_ms = ms[c];
+      *best = c;
+    }
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->tuned[std::make_tuple(ilog2(n_key), ilog2(W_key), (int)jc_only)] = *best;
   return RNJ_OK;
