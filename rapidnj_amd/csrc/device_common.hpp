@@ -42,7 +42,8 @@ typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 //    tests on them, so the saturated pairs (-1) are the same;
 //  * zero distances: ts = tv = 0 gives -0.0f in both models, as the reference's -c*log(1.0) does.
 // The logarithms of those double arguments use log_unit (below) unless kExact, which calls the
-// library's compensated FP64 log; either way the reference's FP64 expression is evaluated.
+// library's compensated FP64 log on the reference's expression; k2p_value / jc_value state how
+// the default path regroups them and what that costs in accuracy.
 #pragma clang fp contract(off)
 __device__ __forceinline__ float proportion(uint32_t a, uint32_t valid) {
   if (valid < (1u << 24)) return (float)a / (float)valid;
@@ -87,19 +88,54 @@ __device__ __forceinline__ double log_unit(double x) {
   return fma(kd, kLn2Hi, fma(kd, kLn2Lo, lm));
 }
 
+// Two proportions a/v and b/v as correctly rounded float quotients, from one FP64 reciprocal
+// of v (hardware estimate + two Newton steps: within about 2^-52 relative, and a * r adds one
+// more rounding). For integers below 2^24 that is enough: a quotient a/v of such integers is
+// never a float rounding midpoint (that would need a 25-bit odd numerator a/v' with v' | a), and
+// its distance to the nearest midpoint is at least 1 / (v * 2^25) >= 2^-49 relative, so the
+// rounded product lands on the same float as the exact quotient. Above 2^24 the double
+// division of proportion() is used.
+__device__ __forceinline__ void proportions2(uint32_t a, uint32_t b, uint32_t v, float& pa,
+                                             float& pb) {
+  if (v < (1u << 24)) {
+    const double dv = (double)v;
+    double r = __builtin_amdgcn_rcp(dv);
+    r = fma(r, fma(-dv, r, 1.0), r);
+    r = fma(r, fma(-dv, r, 1.0), r);
+    pa = (float)((double)a * r);
+    pb = (float)((double)b * r);
+  } else {
+    pa = (float)((double)a / (double)v);
+    pb = (float)((double)b / (double)v);
+  }
+}
+
 // The corrections as straight-line code (every case computed, the result selected), so that a
 // caller can interleave several pairs. kSmall: valid < 2^24 is known (fewer than 2^24 sites).
+// The proportions and the saturation tests are always the reference's. For the logarithms,
+// kExact evaluates the reference's expression with the library log; otherwise
+//  * K2P takes one log: -0.5 ln x - 0.25 ln y = -0.25 ln(x^2 y), with x^2 y rounded twice
+//    (2^-52 relative, i.e. 2^-54 absolute in the distance: below 1e-9 relative of any nonzero
+//    distance, which is at least about 2^-24), and
+//  * JC forms 1 - p/0.75 as fma(-p, 4/3, 1) (2^-52 absolute).
+// Either way the float result equals the reference's unless the exact value lies within about
+// 1e-9 relative of a float rounding boundary; it is always well inside 1e-6.
 template <bool kExact, bool kSmall>
 __device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t valid) {
   const uint32_t v = valid ? valid : 1u;
-  const float P = kSmall ? (float)ts / (float)v : proportion(ts, v);
-  const float Q = kSmall ? (float)tv / (float)v : proportion(tv, v);
+  float P, Q;
+  if (kExact) {
+    P = kSmall ? (float)ts / (float)v : proportion(ts, v);
+    Q = kSmall ? (float)tv / (float)v : proportion(tv, v);
+  } else {
+    proportions2(ts, tv, v, P, Q);
+  }
   const double x = 1.0 - 2.0 * (double)P - (double)Q;
   const double y = 1.0 - 2.0 * (double)Q;
   const bool sat = valid == 0 || (float)x <= 0.0f || (float)y <= 0.0f;
   const double xs = sat ? 1.0 : x, ys = sat ? 1.0 : y;
   const float d = kExact ? (float)(-0.5 * log(xs) - 0.25 * log(ys))
-                         : (float)(-0.5 * log_unit(xs) - 0.25 * log_unit(ys));
+                         : (float)(-0.25 * log_unit(xs * xs * ys));
   // valid == 0 implies ts = tv = 0
   return (ts | tv) == 0 ? (valid == 0 ? -1.0f : -0.0f) : (sat ? -1.0f : d);
 }
@@ -109,10 +145,16 @@ __device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid) {
   const double b = 0.75;  // (alpha - 1) / alpha
   const uint32_t v = valid ? valid : 1u;
   const float p = kSmall ? (float)mismatches / (float)v : proportion(mismatches, v);
-  const double q = (double)p / b;
-  const bool sat = q >= 1.0;
-  const double x = sat ? 1.0 : 1.0 - q;
-  const float d = kExact ? (float)(-b * log(x)) : (float)(-b * log_unit(x));
+  float d;
+  bool sat;
+  if (kExact) {
+    const double q = (double)p / b;
+    sat = q >= 1.0;
+    d = (float)(-b * log(sat ? 1.0 : 1.0 - q));
+  } else {
+    sat = p >= 0.75f;  // the same test: a float p < 0.75 gives p / 0.75 < 1 - 2^-25 in double
+    d = (float)(-b * log_unit(sat ? 1.0 : fma(-(double)p, 4.0 / 3.0, 1.0)));
+  }
   return (valid == 0 || mismatches == 0) ? -0.0f : (sat ? -1.0f : d);
 }
 

@@ -13,7 +13,6 @@ namespace rnj {
 const TileConfig kTileConfigs[] = {
     {"t64_m4x4_c16", 64, 4, 4, 16},
     {"t64_m4x4_c8", 64, 4, 4, 8},
-    {"t128_m8x8_c16", 128, 8, 8, 16},
     {"t128_m8x4_c16", 128, 8, 4, 16},
     {"t32_m4x4_c16", 32, 4, 4, 16},
     {"t32_m4x4_c8", 32, 4, 4, 8},
@@ -51,15 +50,47 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
                                               const uint32_t (&acc_v)[MI][MJ]) {
   const unsigned n = J.n;
   float lmax = 0.0f;
+  if (J.model == kCounts) {
+#pragma unroll
+    for (int a = 0; a < MI; ++a) {
+      const unsigned i = bi * TILE + ty * MI + a;
+#pragma unroll
+      for (int b = 0; b < MJ; ++b) {
+        const unsigned j = bj * TILE + tx * MJ + b;
+        if (j < n && i < j)
+          finish_pair<kExact>(J, i, j, acc_b[a][b], acc_a[a][b], acc_v[a][b], JC6, lmax);
+      }
+    }
+    return;
+  }
+  // One row of MJ corrections as straight-line code (interleaved by the compiler), then stores.
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
+    float d[MJ];
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) {
+      if (J.model == kJC)
+        d[b] = jc_value<kExact, false>(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b], acc_v[a][b]);
+      else
+        d[b] = k2p_value<kExact, false>(acc_b[a][b], acc_a[a][b], acc_v[a][b]);
+    }
 #pragma unroll
     for (int b = 0; b < MJ; ++b) {
       const unsigned j = bj * TILE + tx * MJ + b;
       if (j >= n) continue;
-      if (i < j) finish_pair<kExact>(J, i, j, acc_b[a][b], acc_a[a][b], acc_v[a][b], JC6, lmax);
-      else if (i == j) finish_diagonal(J, i);
+      if (i < j) {
+        if (d[b] == -1.0f) push_sentinel(J, i, j);
+        else lmax = fmaxf(lmax, d[b]);
+        if (J.layout == kSquare) {
+          J.out[(size_t)i * J.ld + j] = d[b];
+          J.out[(size_t)j * J.ld + i] = d[b];
+        } else {
+          J.out[tri_index(i, j, n)] = d[b];
+        }
+      } else if (i == j) {
+        finish_diagonal(J, i);
+      }
     }
   }
   fold_max(J, lmax, tid);
@@ -480,10 +511,9 @@ hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* job
   switch (config) {
     case 0: return launch_cfg<64, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
     case 1: return launch_cfg<64, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 2: return launch_cfg<128, 8, 8, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 3: return launch_cfg<128, 8, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 4: return launch_cfg<32, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 5: return launch_cfg<32, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 2: return launch_cfg<128, 8, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 3: return launch_cfg<32, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 4: return launch_cfg<32, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -54,6 +54,10 @@ int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignm
     delete a;
     return hip_fail(e, "hipMalloc(alignment planes)");
   }
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->alignments.insert(a);
+  }
   *out = a;
   return RNJ_OK;
 }
@@ -236,7 +240,7 @@ void free_workspace_public(Workspace* ws) { free_workspace(ws); }
 static int default_config(unsigned n, unsigned W, bool jc_only) {
   (void)W;
   (void)jc_only;
-  return n <= 4096 ? 4 : 1;
+  return n <= 4096 ? 3 : 1;
 }
 
 int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
@@ -392,7 +396,10 @@ hipStream_t stream_of(rnj_context* ctx, void* stream) {
   return (hipStream_t)stream;
 }
 
+// Every entry point starts here: it also clears the thread's last-error slot, so that an error
+// left by an unrelated earlier HIP call is not reported by this call's launch checks.
 int set_device(rnj_context* ctx) {
+  (void)hipGetLastError();
   RNJ_HIP(hipSetDevice(ctx->device));
   return RNJ_OK;
 }
@@ -501,6 +508,11 @@ int rnj_destroy(rnj_context* ctx) {
   if (!ctx) return RNJ_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
+  for (Alignment* a : ctx->alignments) {
+    if (a->planes) (void)hipFree(a->planes);
+    delete a;
+  }
+  ctx->alignments.clear();
   for (auto& w : ctx->pool) {
     free_workspace_public(w.get());
     if (w->done) (void)hipEventDestroy(w->done);
@@ -564,8 +576,7 @@ int rnj_alignment_from_chars(rnj_context* ctx, const char* chars, size_t pitch, 
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (staging) (void)hipFree(staging);
   if (e != hipSuccess) {
-    (void)hipFree(a->planes);
-    delete a;
+    rnj_alignment_free(static_cast<rnj_alignment*>(a));
     return hip_fail(e, "rnj_alignment_from_chars");
   }
   *out = static_cast<rnj_alignment*>(a);
@@ -615,8 +626,7 @@ int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned int* const* b
   if (db) (void)hipFree(db);
   if (dg) (void)hipFree(dg);
   if (e != hipSuccess) {
-    (void)hipFree(a->planes);
-    delete a;
+    rnj_alignment_free(static_cast<rnj_alignment*>(a));
     return hip_fail(e, "rnj_alignment_from_bitstrings");
   }
   *out = static_cast<rnj_alignment*>(a);
@@ -625,7 +635,11 @@ int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned int* const* b
 
 int rnj_alignment_free(rnj_alignment* aln) {
   if (!aln) return RNJ_OK;
-  if (aln->ctx) (void)hipSetDevice(aln->ctx->device);
+  if (aln->ctx) {
+    (void)hipSetDevice(aln->ctx->device);
+    std::lock_guard<std::mutex> lock(aln->ctx->mu);
+    aln->ctx->alignments.erase(aln);
+  }
   if (aln->planes) {
     (void)hipDeviceSynchronize();
     (void)hipFree(aln->planes);

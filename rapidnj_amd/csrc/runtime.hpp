@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstdint>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -62,6 +63,7 @@ struct Context {
   // autotune cache: (log2 n bucket, log2 W bucket, jc_only) -> config
   std::map<std::tuple<int, int, int>, int> tuned;
   hipDeviceProp_t prop{};
+  std::set<struct Alignment*> alignments;  // live alignments (freed by rnj_destroy if left)
 };
 
 struct Alignment {

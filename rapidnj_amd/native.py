@@ -8,6 +8,7 @@ and stream plumbing only).
 from __future__ import annotations
 
 import ctypes
+import weakref
 import os
 from dataclasses import dataclass
 
@@ -162,6 +163,7 @@ class Alignment:
     def __init__(self, ctx: "Context", handle: int):
         self.ctx = ctx
         self.handle = handle
+        ctx._adopt(self)
         n, L, W, b = _u(), _u(), _u(), _sz()
         check(lib().rnj_alignment_info(handle, ctypes.byref(n), ctypes.byref(L), ctypes.byref(W),
                                        ctypes.byref(b)))
@@ -200,9 +202,19 @@ class Context:
         check(lib().rnj_create(device, ctypes.byref(h)))
         self.handle = h.value
         self.device = device
+        self._children = weakref.WeakSet()  # alignments, plans and servers made from this context
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def close(self) -> None:
+        """Destroy the context; servers, plans and alignments still open are closed first."""
         if self.handle:
+            live = list(self._children)
+            for kind in (Server, BootstrapPlan, Alignment):
+                for c in live:
+                    if isinstance(c, kind):
+                        c.close()
             lib().rnj_destroy(self.handle)
             self.handle = None
 
@@ -354,6 +366,7 @@ class BootstrapPlan:
                                               (FLAG_RAW if raw else 0) |
                                               (FLAG_EXACT if exact else 0), ctypes.byref(h)))
         self.handle = h.value
+        ctx._adopt(self)
 
     @property
     def tile_config(self) -> int:
@@ -387,6 +400,7 @@ class Server:
         h = _vp()
         check(lib().rnj_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
+        ctx._adopt(self)
 
     def submit(self, aln: np.ndarray, model="k2p") -> int:
         a = np.ascontiguousarray(aln, dtype=np.uint8)

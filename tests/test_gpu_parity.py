@@ -285,6 +285,29 @@ def test_large_properties(ctx, N):
             assert sqn[a, b] == pytest.approx(dk, rel=DIST_RTOL)
 
 
+def test_context_close_releases_children(N):
+    """Closing a context closes what was made from it; later closes and collection are no-ops,
+    and a new context works (no stale device state or HIP error is left behind)."""
+    import gc
+    c = N.Context(0)
+    aln = simulate_alignment(40, 300, 0.1, 4)
+    A = c.alignment(aln)
+    B = c.alignment(aln)
+    plan = c.bootstrap_plan(A, 2, "k2p")
+    ref = _np(c.distances(A, "k2p"))
+    c.close()
+    assert A.handle is None and B.handle is None and plan.handle is None
+    A.close()
+    del A, B, plan
+    gc.collect()
+    c2 = N.Context(0)
+    try:
+        got = _np(c2.distances(c2.alignment(aln), "k2p"))
+        np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+    finally:
+        c2.close()
+
+
 def test_error_paths(ctx, N):
     with pytest.raises(N.RnjError) as e:
         ctx.alignment(np.frombuffer(b"ACGT", dtype=np.uint8).reshape(1, 4))
