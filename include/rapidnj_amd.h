@@ -132,6 +132,24 @@ RNJ_EXPORT int rnj_alignment_from_rows(rnj_context* ctx, const char* const* rows
 RNJ_EXPORT int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned int* const* bitStrings,
                                   const unsigned int* const* gapFilters, unsigned n,
                                   unsigned bitStringsCount, rnj_alignment** out);
+/* FASTA parsed on the device: the reader dataloaderFasta::parseData / storeSequence
+ * (dataloaderFasta.hpp:78-133, 201-276) restated as parallel passes over the file image in HBM,
+ * followed by the packer. `data` holds the file's `len` bytes, in host memory or, with
+ * data_on_device, in device memory. The same input gives the same sequences and names as the
+ * reference: ';' lines are comments, '>' opens a record whose name ends at the first byte below
+ * 32, sequence lines keep bytes 33..127, empty records are not stored but keep their name, data
+ * before the first header forms a nameless record. Errors as the reference's exits: fewer than
+ * 2 sequences, or a sequence whose length differs from the first (RNJ_E_INVALID, message names
+ * the sequence), first sequence not DNA (RNJ_E_NOT_DNA). Synchronous. */
+RNJ_EXPORT int rnj_alignment_from_fasta(rnj_context* ctx, const char* data, size_t len,
+                                        int data_on_device, rnj_alignment** out);
+/* The same from a file: read through pinned buffers and uploaded while reading
+ * (dataloaderFasta::load, dataloaderFasta.hpp:24-52). */
+RNJ_EXPORT int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignment** out);
+/* Names of an alignment read from FASTA, one per '>' record in file order
+ * (dataloaderFasta::getSequenceNames, dataloaderFasta.hpp:298-300); 0 / NULL otherwise. */
+RNJ_EXPORT unsigned rnj_alignment_name_count(const rnj_alignment* aln);
+RNJ_EXPORT const char* rnj_alignment_name(const rnj_alignment* aln, unsigned k);
 RNJ_EXPORT int rnj_alignment_free(rnj_alignment* aln);
 /* n taxa, L sites, W 32-site words counted per taxon, bytes of HBM held. */
 RNJ_EXPORT int rnj_alignment_info(const rnj_alignment* aln, unsigned* n, unsigned* L, unsigned* W,

@@ -59,6 +59,9 @@ def lib() -> ctypes.CDLL:
         L.rnj_oracle_distance_matrix.argtypes = [_u32p, _u32p, ctypes.c_uint, ctypes.c_uint,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.rnj_oracle_parse_fasta.restype = ctypes.c_int
+        L.rnj_oracle_parse_fasta.argtypes = [ctypes.c_char_p, ctypes.c_size_t] + \
+            [ctypes.c_void_p] * 9
         L.rnj_oracle_correct_triangle.restype = ctypes.c_int
         L.rnj_oracle_correct_triangle.argtypes = [_u32p, _u32p, _u32p, ctypes.c_uint, ctypes.c_int,
                                                   ctypes.c_void_p, ctypes.c_void_p,
@@ -182,6 +185,41 @@ def triangle_to_full(tri: np.ndarray, n: int, maxd: float | None = None) -> np.n
     full[iu] = t
     full[(iu[1], iu[0])] = t
     return full
+
+
+class FastaError(ValueError):
+    """The reference's exits while reading FASTA: kind 1 = a stored record's length differs from
+    the first one's (seq, length), kind 2 = fewer than 2 sequences."""
+
+    def __init__(self, kind: int, seq: int = 0, length: int = 0):
+        super().__init__(f"fasta error {kind}: sequence {seq} length {length}")
+        self.kind, self.seq, self.length = kind, seq, length
+
+
+def parse_fasta(data: bytes) -> tuple[list[bytes], np.ndarray]:
+    """(names of every '>' record in file order, (n, L) uint8 kept bytes of the stored records)."""
+    raw = bytes(data)
+    u = ctypes.c_uint
+    n, L, H, bad, bad_len = u(), u(), u(), u(), u()
+    nb = ctypes.c_size_t()
+    rc = lib().rnj_oracle_parse_fasta(raw, len(raw), ctypes.byref(n), ctypes.byref(L),
+                                      ctypes.byref(H), ctypes.byref(nb), ctypes.byref(bad),
+                                      ctypes.byref(bad_len), None, None, None)
+    if rc == 1:
+        raise FastaError(1, bad.value, bad_len.value)
+    if rc == 2:
+        raise FastaError(2, n.value)
+    rows = np.zeros((n.value, L.value), dtype=np.uint8)
+    names = ctypes.create_string_buffer(max(nb.value, 1))
+    lens = (ctypes.c_uint * max(H.value, 1))()
+    lib().rnj_oracle_parse_fasta(raw, len(raw), ctypes.byref(n), ctypes.byref(L), ctypes.byref(H),
+                                 ctypes.byref(nb), ctypes.byref(bad), ctypes.byref(bad_len),
+                                 rows.ctypes.data_as(ctypes.c_void_p), names, lens)
+    out, o = [], 0
+    for k in range(H.value):
+        out.append(names.raw[o:o + lens[k]])
+        o += lens[k]
+    return out, rows
 
 
 def ref_harness_available() -> bool:

@@ -14,6 +14,10 @@
 // Modes (all binary outputs are little-endian):
 //   pack   FASTA OUT                  u32 n, L, bitStringsCount; then n*bsc*4 u32 bit strings,
 //                                     then n*bsc*4 u32 gap filters
+//   fasta  FASTA OUT                  u32 n, L, bitStringsCount, H; H names (u32 length + bytes,
+//                                     every '>' record in file order); then bit strings and gap
+//                                     filters as in `pack` (the reference exits 1 on bad input)
+//   load   FASTA REPS                 prints {"best_s":..} of parse + pack (readData, fastdist)
 //   counts FASTA OUT [naive]          u32 n, L; then for i<j (row-major) u32 ts, tv, valid
 //   matrix FASTA jc|kim T OUT [naive] u32 n; then n*n f32 (post-processed matrix); prints seconds
 //   time   FASTA jc|kim T REPS        prints {"best_s":..,"mean_s":..,"n":..,"L":..,"threads":..}
@@ -95,6 +99,44 @@ int mode_pack(int argc, char** argv) {
   for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getBitStrings()[i]), bsc * 16);
   for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getGapFilters()[i]), bsc * 16);
   delete dl;
+  return 0;
+}
+
+int mode_fasta(int argc, char** argv) {
+  if (argc < 4) die("fasta FASTA OUT");
+  dataloaderFasta* dl = new dataloaderFasta();
+  dl->readData(argv[2], true, false, UNKNOWN, false);  // exits 1 itself on malformed input
+  if (dl->type != DNA) die("input is not detected as DNA");
+  uint32_t n = dl->getSequenceCount(), L = dl->getSequenceLength(), bsc = dl->getBitStringsCount();
+  vector<string>* names = dl->getSequenceNames();
+  std::ofstream o = open_out(argv[3]);
+  put_u32(o, n); put_u32(o, L); put_u32(o, bsc); put_u32(o, (uint32_t)names->size());
+  for (const string& s : *names) {
+    put_u32(o, (uint32_t)s.size());
+    o.write(s.data(), s.size());
+  }
+  for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getBitStrings()[i]), bsc * 16);
+  for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getGapFilters()[i]), bsc * 16);
+  delete dl;
+  return 0;
+}
+
+int mode_load(int argc, char** argv) {
+  if (argc < 4) die("load FASTA REPS");
+  int reps = atoi(argv[3]);
+  double best = 1e30;
+  uint32_t n = 0, L = 0;
+  for (int r = 0; r < reps; r++) {
+    auto t0 = std::chrono::steady_clock::now();
+    dataloaderFasta* dl = new dataloaderFasta();
+    dl->readData(argv[2], true, false, UNKNOWN, false);
+    auto t1 = std::chrono::steady_clock::now();
+    best = std::min(best, std::chrono::duration<double>(t1 - t0).count());
+    n = dl->getSequenceCount();
+    L = dl->getSequenceLength();
+    delete dl;
+  }
+  std::cout << "{\"best_s\": " << best << ", \"n\": " << n << ", \"L\": " << L << "}" << std::endl;
   return 0;
 }
 
@@ -199,9 +241,11 @@ int mode_tree(int argc, char** argv) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2) die("usage: ref_harness pack|counts|matrix|time|nj|tree ...");
+  if (argc < 2) die("usage: ref_harness pack|fasta|load|counts|matrix|time|nj|tree ...");
   std::string mode = argv[1];
   if (mode == "pack") return mode_pack(argc, argv);
+  if (mode == "fasta") return mode_fasta(argc, argv);
+  if (mode == "load") return mode_load(argc, argv);
   if (mode == "counts") return mode_counts(argc, argv);
   if (mode == "matrix") return mode_matrix(argc, argv);
   if (mode == "time") return mode_time(argc, argv);

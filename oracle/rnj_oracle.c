@@ -346,3 +346,75 @@ int rnj_oracle_correct_triangle(const uint32_t* ts, const uint32_t* tv, const ui
   if (!tri) free(t);
   return 0;
 }
+
+/* ---- FASTA reader (dataloaderFasta.hpp:78-133, 201-276) ----------------------------------- */
+typedef struct {
+  const unsigned char* d;
+  size_t len, pos;
+  int eof;
+} fasta_reader;
+
+/* getNextChar (dataloaderFasta.hpp:118-126): the next byte as a signed char; past the end,
+ * 10 ('\n'), and the main loop stops (fillBuffer leaves buf NULL). */
+static int fasta_get(fasta_reader* r) {
+  if (r->pos < r->len) return (int)(signed char)r->d[r->pos++];
+  r->eof = 1;
+  return 10;
+}
+
+int rnj_oracle_parse_fasta(const unsigned char* data, size_t len, unsigned* n, unsigned* L,
+                           unsigned* H, size_t* name_bytes, unsigned* bad, unsigned* bad_len,
+                           unsigned char* rows, unsigned char* names, unsigned* name_len) {
+  fasta_reader r = {data, len, 0, 0};
+  unsigned nseq = 0, nh = 0, first_len = 0, have_first = 0;
+  size_t nb = 0, cur = 0;
+  const unsigned row_len = rows ? *L : 0;
+#define FASTA_STORE()                                   \
+  do {                                                  \
+    if (cur) {                                          \
+      if (!have_first) {                                \
+        first_len = (unsigned)cur;                      \
+        have_first = 1;                                 \
+      } else if (cur != first_len) {                    \
+        *bad = nseq;                                    \
+        *bad_len = (unsigned)cur;                       \
+        return 1;                                       \
+      }                                                 \
+      nseq++;                                           \
+      cur = 0;                                          \
+    }                                                   \
+  } while (0)
+  while (!r.eof) {                                      /* parseData */
+    int c = fasta_get(&r);
+    if (c == ';') {                                     /* parseComment */
+      while (c != 10) c = fasta_get(&r);
+    } else if (c == '>') {
+      FASTA_STORE();
+      c = fasta_get(&r);
+      size_t s = 0;                                     /* parseSequenceName */
+      while (c > 31) {
+        if (names) names[nb + s] = (unsigned char)c;
+        s++;
+        c = fasta_get(&r);
+      }
+      if (name_len) name_len[nh] = (unsigned)s;
+      nh++;
+      nb += s;
+    } else {                                            /* parseSequenceLine */
+      while (c != 10) {
+        if (c > 32) {
+          if (rows && cur < row_len) rows[(size_t)nseq * row_len + cur] = (unsigned char)c;
+          cur++;
+        }
+        c = fasta_get(&r);
+      }
+    }
+  }
+  FASTA_STORE();
+#undef FASTA_STORE
+  *n = nseq;
+  if (!rows) *L = first_len;
+  *H = nh;
+  *name_bytes = nb;
+  return nseq < 2 ? 2 : 0;
+}

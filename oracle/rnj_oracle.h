@@ -19,6 +19,7 @@
 #ifndef RNJ_ORACLE_H
 #define RNJ_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -28,6 +29,17 @@ extern "C" {
 enum { RNJ_ORACLE_JC = 0, RNJ_ORACLE_K2P = 1 };
 
 /* bitStringsCount for a DNA alignment of length L (dataloaderFasta.hpp:222). */
+/* FASTA reader, dataloaderFasta::parseData / storeSequence restated (dataloaderFasta.hpp:78-133,
+ * 201-276), bytes as signed chars. Two calls: with rows == NULL it returns the shape (*n
+ * sequences of *L sites, *H names holding *name_bytes bytes); with buffers sized from that it
+ * fills rows (n * L bytes, the kept bytes of every stored record), names (the name bytes
+ * back to back) and name_len (H lengths). Returns 0, 1 when a stored record's length differs
+ * from the first one's (*bad = its sequence index, *bad_len = its length; the reference exits
+ * there) or 2 with fewer than 2 sequences. */
+int rnj_oracle_parse_fasta(const unsigned char* data, size_t len, unsigned* n, unsigned* L,
+                           unsigned* H, size_t* name_bytes, unsigned* bad, unsigned* bad_len,
+                           unsigned char* rows, unsigned char* names, unsigned* name_len);
+
 unsigned rnj_oracle_bitstrings_count(unsigned L);
 
 /* 1 if the first sequence classifies the alignment as DNA (dataloaderFasta.hpp:54-76). */

@@ -1,0 +1,578 @@
+// FASTA ingest on the device: a file image in HBM -> record names + packed H/L/V planes.
+//
+// The reference reads FASTA with a byte-at-a-time state machine (dataloaderFasta.hpp:78-133,
+// 201-217, 253-276), bytes taken as signed chars:
+//   * the main loop reads one byte: ';' skips to the end of the line, '>' ends the current
+//     record and starts a name (the next byte is its first candidate), anything else is a
+//     sequence line;
+//   * a name takes bytes > 31 and stops at the first other byte, which is consumed; if that was
+//     not the line's '\n', the main loop resumes right after it, on the same line;
+//   * a sequence line appends every byte > 32 up to its '\n' (so ';' and '>' inside a line are
+//     data, and spaces, tabs, '\r' and bytes >= 128 are dropped);
+//   * end of input acts as a final '\n';
+//   * a record is stored only if it holds at least one byte; the first stored record fixes the
+//     sequence length and every later one must match it; every '>' contributes a name, stored
+//     or not, so names follow headers and not stored sequences.
+//
+// The device restates that per line, in parallel: each physical line is a chain of zero or more
+// headers (every one but the last ended by a control byte other than '\n') followed by at most
+// one data span (to the end of the line) or a comment. Data bytes go to the record opened by the
+// last header before them. Passes (all over HBM, 16 KB per work-group):
+//   1. count and list the '\n' positions (per-block counts, scan, ordered write);
+//   2. one wave per line: its header count, data span and kept-byte count;
+//   3. scans: header index and record of each line, byte offset of each line in the data stream;
+//   4. records: length and first byte offset (atomics), stored-record numbering, length check;
+//   5. one wave per data line: kept bytes scattered into an n x L character matrix;
+//   6. the character matrix is packed into planes by the alignment packer; names are gathered.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace rnj {
+namespace {
+
+constexpr unsigned kThreads = 256;
+constexpr unsigned kRounds = 4;                       // 16-byte loads per thread and block
+constexpr size_t kChunk = (size_t)kThreads * kRounds * 16;  // bytes per work-group in pass 1
+
+__device__ __forceinline__ int sbyte(const uint8_t* b, size_t p) { return (int)(int8_t)b[p]; }
+
+// Bytes of x equal to 0x0A ('\n') as set high bits (exact: no false positives).
+__device__ __forceinline__ uint32_t nl_bits(uint32_t x) {
+  const uint32_t v = x ^ 0x0A0A0A0Au;
+  return ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v | 0x7F7F7F7Fu);
+}
+
+// 16 bytes at p (p < N); bytes at or past N read as 0 (not '\n').
+__device__ __forceinline__ uint4 load16(const uint8_t* b, size_t N, size_t p) {
+  if (p + 16 <= N) return *reinterpret_cast<const uint4*>(b + p);
+  uint32_t w[4] = {0, 0, 0, 0};
+  for (size_t k = p; k < N; ++k) w[(k - p) / 4] |= (uint32_t)b[k] << (8 * ((k - p) % 4));
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ unsigned count16(const uint4& q) {
+  return __popc(nl_bits(q.x)) + __popc(nl_bits(q.y)) + __popc(nl_bits(q.z)) + __popc(nl_bits(q.w));
+}
+
+// Pass 1a: '\n' count of each 16 KB block.
+__global__ __launch_bounds__(kThreads) void nl_count_kernel(const uint8_t* __restrict__ b, size_t N,
+                                                            uint32_t* __restrict__ blk) {
+  const size_t base = (size_t)blockIdx.x * kChunk;
+  unsigned c = 0;
+#pragma unroll
+  for (unsigned r = 0; r < kRounds; ++r) {
+    const size_t p = base + ((size_t)r * kThreads + threadIdx.x) * 16;
+    if (p < N) c += count16(load16(b, N, p));
+  }
+  __shared__ unsigned s[kThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) blk[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
+}
+
+// Exclusive prefix of v over the work-group (256 threads), and the total.
+__device__ __forceinline__ unsigned block_exclusive(unsigned v, unsigned& total) {
+  __shared__ unsigned s[kThreads / 64];
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned y = __shfl_up(x, off);
+    if (lane >= (unsigned)off) x += y;
+  }
+  if (lane == 63) s[wave] = x;
+  __syncthreads();
+  unsigned before = 0;
+  for (unsigned w = 0; w < wave; ++w) before += s[w];
+  total = s[0] + s[1] + s[2] + s[3];
+  __syncthreads();
+  return before + x - v;
+}
+
+// Pass 1b: positions of the '\n' bytes, in file order.
+__global__ __launch_bounds__(kThreads) void nl_write_kernel(const uint8_t* __restrict__ b, size_t N,
+                                                            const unsigned long long* __restrict__ blk_base,
+                                                            unsigned long long* __restrict__ pos) {
+  const size_t base = (size_t)blockIdx.x * kChunk;
+  unsigned long long out = blk_base[blockIdx.x];
+  for (unsigned r = 0; r < kRounds; ++r) {
+    const size_t p = base + ((size_t)r * kThreads + threadIdx.x) * 16;
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if (p < N) q = load16(b, N, p);
+    const uint32_t w[4] = {nl_bits(q.x), nl_bits(q.y), nl_bits(q.z), nl_bits(q.w)};
+    const unsigned c = __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
+    unsigned total;
+    const unsigned ex = block_exclusive(c, total);
+    unsigned k = 0;
+    for (int d = 0; d < 4; ++d) {
+      uint32_t m = w[d];
+      while (m) {
+        const int bit = __ffs(m) - 1;
+        pos[out + ex + k++] = p + 4 * d + bit / 8;
+        m &= m - 1;
+      }
+    }
+    out += total;
+  }
+}
+
+struct Line {
+  unsigned long long start, end;  // [start, end): the line without its '\n'
+};
+
+__device__ __forceinline__ Line line_at(const unsigned long long* nl, unsigned long long n_nl,
+                                        size_t N, unsigned long long l) {
+  Line L;
+  L.start = l == 0 ? 0 : nl[l - 1] + 1;
+  L.end = l < n_nl ? nl[l] : N;
+  return L;
+}
+
+// Walks the header chain of a line (every lane the same, so control flow stays uniform).
+// Returns the header count; *ds/*de receive the data span (empty if none). If spans is not
+// null the name spans [s, e) are written to spans[0..count).
+__device__ __forceinline__ unsigned walk_line(const uint8_t* b, Line L, unsigned long long* ds,
+                                              unsigned long long* de,
+                                              unsigned long long (*spans)[2]) {
+  unsigned h = 0;
+  unsigned long long p = L.start;
+  *ds = *de = L.start;
+  while (p < L.end) {
+    const int c = sbyte(b, p);
+    if (c == ';') break;                       // comment: the rest of the line is skipped
+    if (c != '>') {                            // data to the end of the line
+      *ds = p;
+      *de = L.end;
+      break;
+    }
+    unsigned long long q = p + 1;              // name: bytes > 31 after '>'
+    while (q < L.end && sbyte(b, q) > 31) ++q;
+    if (spans) {
+      spans[h][0] = p + 1;
+      spans[h][1] = q;
+    }
+    ++h;
+    if (q >= L.end) break;                     // the name ended at '\n' (or the end of input)
+    p = q + 1;                                 // another control byte: resume after it
+  }
+  return h;
+}
+
+// Pass 2: one wave per line.
+__global__ __launch_bounds__(kThreads) void line_kernel(const uint8_t* __restrict__ b, size_t N,
+                                                        const unsigned long long* __restrict__ nl,
+                                                        unsigned long long n_nl,
+                                                        unsigned long long M,
+                                                        uint32_t* __restrict__ hcount,
+                                                        unsigned long long* __restrict__ dspan,
+                                                        uint32_t* __restrict__ dcount) {
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned long long waves = (unsigned long long)gridDim.x * (kThreads / 64);
+  for (unsigned long long l = (unsigned long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+       l < M; l += waves) {
+    const Line L = line_at(nl, n_nl, N, l);
+    unsigned long long ds, de;
+    const unsigned h = walk_line(b, L, &ds, &de, nullptr);
+    unsigned c = 0;
+    for (unsigned long long p = ds + lane; p < de + lane; p += 64) {
+      const bool keep = p < de && sbyte(b, p) > 32;
+      c += (unsigned)__popcll(__ballot(keep));
+    }
+    if (lane == 0) {
+      hcount[l] = h;
+      dspan[2 * l] = ds;
+      dspan[2 * l + 1] = de;
+      dcount[l] = c;
+    }
+  }
+}
+
+// Name spans of every header, at their header index.
+__global__ void header_kernel(const uint8_t* __restrict__ b, size_t N,
+                              const unsigned long long* __restrict__ nl, unsigned long long n_nl,
+                              unsigned long long M, const uint32_t* __restrict__ hcount,
+                              const uint32_t* __restrict__ hbase,
+                              unsigned long long* __restrict__ spans) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M || hcount[l] == 0) return;
+  const Line L = line_at(nl, n_nl, N, l);
+  unsigned h = 0;
+  unsigned long long p = L.start;
+  while (p < L.end && h < hcount[l]) {
+    unsigned long long q = p + 1;
+    while (q < L.end && sbyte(b, q) > 31) ++q;
+    spans[2 * (hbase[l] + h)] = p + 1;
+    spans[2 * (hbase[l] + h) + 1] = q;
+    ++h;
+    p = q + 1;
+  }
+}
+
+// Pass 4a: record of each data line; record length and first byte offset.
+__global__ void record_kernel(unsigned long long M, const uint32_t* __restrict__ hcount,
+                              const uint32_t* __restrict__ hbase,
+                              const uint32_t* __restrict__ dcount,
+                              const unsigned long long* __restrict__ off,
+                              unsigned long long* __restrict__ rec_len,
+                              unsigned long long* __restrict__ rec_base) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M || dcount[l] == 0) return;
+  const unsigned r = hbase[l] + hcount[l];  // record 0 holds data before the first header
+  atomicAdd(&rec_len[r], (unsigned long long)dcount[l]);
+  atomicMin(&rec_base[r], off[l]);
+}
+
+__global__ void stored_flag_kernel(unsigned R, const unsigned long long* __restrict__ rec_len,
+                                   uint32_t* __restrict__ stored, unsigned* __restrict__ first) {
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  stored[r] = rec_len[r] > 0;
+  if (rec_len[r] > 0) atomicMin(first, r);
+}
+
+// Pass 4b: the first stored record whose length differs from the first one (as a sequence index).
+__global__ void length_check_kernel(unsigned R, const unsigned long long* __restrict__ rec_len,
+                                    const uint32_t* __restrict__ seq, const unsigned* __restrict__ first,
+                                    unsigned* __restrict__ bad) {
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R || rec_len[r] == 0 || *first >= R) return;
+  if (rec_len[r] != rec_len[*first]) atomicMin(bad, seq[r]);
+}
+
+// Pass 5: one wave per data line; kept bytes to row seq[r] of the n x L character matrix.
+__global__ __launch_bounds__(kThreads) void scatter_kernel(const uint8_t* __restrict__ b,
+                                                           unsigned long long M,
+                                                           const uint32_t* __restrict__ hcount,
+                                                           const uint32_t* __restrict__ hbase,
+                                                           const unsigned long long* __restrict__ dspan,
+                                                           const uint32_t* __restrict__ dcount,
+                                                           const unsigned long long* __restrict__ off,
+                                                           const unsigned long long* __restrict__ rec_base,
+                                                           const uint32_t* __restrict__ seq,
+                                                           unsigned long long L,
+                                                           uint8_t* __restrict__ chars) {
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned long long waves = (unsigned long long)gridDim.x * (kThreads / 64);
+  for (unsigned long long l = (unsigned long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+       l < M; l += waves) {
+    if (dcount[l] == 0) continue;
+    const unsigned r = hbase[l] + hcount[l];
+    uint8_t* row = chars + (unsigned long long)seq[r] * L + (off[l] - rec_base[r]);
+    const unsigned long long ds = dspan[2 * l], de = dspan[2 * l + 1];
+    unsigned long long w = 0;
+    for (unsigned long long p = ds + lane; p < de + lane; p += 64) {
+      const uint8_t c = p < de ? b[p] : 0;
+      const bool keep = p < de && (int)(int8_t)c > 32;
+      const unsigned long long m = __ballot(keep);
+      const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      if (keep) row[w + below] = c;
+      w += (unsigned)__popcll(m);
+    }
+  }
+}
+
+__global__ void name_gather_kernel(const uint8_t* __restrict__ b, unsigned H,
+                                   const unsigned long long* __restrict__ spans,
+                                   const unsigned long long* __restrict__ noff,
+                                   uint8_t* __restrict__ blob) {
+  const unsigned h = blockIdx.x;
+  if (h >= H) return;
+  const unsigned long long s = spans[2 * h], e = spans[2 * h + 1];
+  for (unsigned long long p = s + threadIdx.x; p < e; p += blockDim.x) blob[noff[h] + (p - s)] = b[p];
+}
+
+__global__ void name_len_kernel(unsigned H, const unsigned long long* __restrict__ spans,
+                                unsigned long long* __restrict__ len) {
+  const unsigned h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < H) len[h] = spans[2 * h + 1] - spans[2 * h];
+}
+
+// ---- exclusive scan (three phases, recursive over block totals) ------------------------------
+template <class Tin, class Tout>
+__global__ __launch_bounds__(kThreads) void scan_blocks_kernel(const Tin* __restrict__ in,
+                                                               unsigned long long n,
+                                                               Tout* __restrict__ out,
+                                                               Tout* __restrict__ sums) {
+  constexpr unsigned kItems = 8;
+  const unsigned long long base = (unsigned long long)blockIdx.x * kThreads * kItems;
+  const unsigned long long mine = base + (unsigned long long)threadIdx.x * kItems;
+  Tout v[kItems];
+  Tout acc = 0;
+#pragma unroll
+  for (unsigned k = 0; k < kItems; ++k) {
+    v[k] = mine + k < n ? (Tout)in[mine + k] : (Tout)0;
+    acc += v[k];
+  }
+  __shared__ Tout s[kThreads];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (unsigned d = 1; d < kThreads; d <<= 1) {  // inclusive Hillis-Steele over thread totals
+    const Tout y = threadIdx.x >= d ? s[threadIdx.x - d] : (Tout)0;
+    __syncthreads();
+    s[threadIdx.x] += y;
+    __syncthreads();
+  }
+  Tout run = s[threadIdx.x] - acc;
+#pragma unroll
+  for (unsigned k = 0; k < kItems; ++k) {
+    if (mine + k < n) out[mine + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kThreads - 1) sums[blockIdx.x] = s[kThreads - 1];
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void scan_add_kernel(T* __restrict__ out,
+                                                            unsigned long long n,
+                                                            const T* __restrict__ base) {
+  constexpr unsigned kItems = 8;
+  const unsigned long long i0 = (unsigned long long)blockIdx.x * kThreads * kItems;
+  const T add = base[blockIdx.x];
+  for (unsigned k = threadIdx.x; k < kThreads * kItems; k += kThreads)
+    if (i0 + k < n) out[i0 + k] += add;
+}
+
+// out[i] = in[0] + .. + in[i-1]; *total (device) = the sum of all (written at out[n] is not done).
+template <class Tin, class Tout>
+hipError_t exclusive_scan(const Tin* in, unsigned long long n, Tout* out, Tout* total,
+                          hipStream_t s) {
+  constexpr unsigned long long kPer = (unsigned long long)kThreads * 8;
+  const unsigned long long blocks = n == 0 ? 1 : (n + kPer - 1) / kPer;
+  Tout* sums = nullptr;
+  hipError_t e = hipMallocAsync((void**)&sums, (blocks + 1) * sizeof(Tout), s);
+  if (e != hipSuccess) return e;
+  scan_blocks_kernel<Tin, Tout><<<dim3((unsigned)blocks), dim3(kThreads), 0, s>>>(in, n, out, sums);
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    if (blocks > 1) {
+      Tout* sums_ex = nullptr;
+      e = hipMallocAsync((void**)&sums_ex, blocks * sizeof(Tout), s);
+      if (e == hipSuccess) e = exclusive_scan<Tout, Tout>(sums, blocks, sums_ex, total, s);
+      if (e == hipSuccess) {
+        scan_add_kernel<Tout><<<dim3((unsigned)blocks), dim3(kThreads), 0, s>>>(out, n, sums_ex);
+        e = hipGetLastError();
+      }
+      if (sums_ex) (void)hipFreeAsync(sums_ex, s);
+    } else if (total) {
+      e = hipMemcpyAsync(total, sums, sizeof(Tout), hipMemcpyDeviceToDevice, s);
+    }
+  }
+  (void)hipFreeAsync(sums, s);
+  return e;
+}
+
+unsigned grid_for(unsigned long long items, unsigned per_block, unsigned cap = 0x7fffffffu) {
+  const unsigned long long g = (items + per_block - 1) / per_block;
+  return (unsigned)(g == 0 ? 1 : (g > cap ? cap : g));
+}
+
+// Temporary device buffers of one parse, released in order on the stream.
+struct Temps {
+  hipStream_t s;
+  std::vector<void*> ptrs;
+  template <class T>
+  hipError_t get(T** p, unsigned long long count) {
+    *p = nullptr;
+    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
+    if (e == hipSuccess) ptrs.push_back(*p);
+    return e;
+  }
+  ~Temps() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+  }
+};
+
+#define FA_HIP(call)                                 \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+template <class T>
+T read_scalar(const T* d, hipStream_t s, hipError_t* e) {
+  T v{};
+  *e = hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, s);
+  if (*e == hipSuccess) *e = hipStreamSynchronize(s);
+  return v;
+}
+
+}  // namespace
+
+int fasta_to_alignment(Context* ctx, const uint8_t* b, size_t N, hipStream_t s, Alignment** out) {
+  Temps t{s, {}};
+  hipError_t e;
+  // 1. '\n' positions
+  const unsigned long long nblk = N == 0 ? 1 : (N + kChunk - 1) / kChunk;
+  uint32_t* blk = nullptr;
+  unsigned long long *blk_base = nullptr, *n_nl_d = nullptr;
+  FA_HIP(t.get(&blk, nblk));
+  FA_HIP(t.get(&blk_base, nblk));
+  FA_HIP(t.get(&n_nl_d, 1));
+  if (N > 0) {
+    nl_count_kernel<<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(b, N, blk);
+    FA_HIP(hipGetLastError());
+  } else {
+    FA_HIP(hipMemsetAsync(blk, 0, sizeof(uint32_t), s));
+  }
+  FA_HIP((exclusive_scan<uint32_t, unsigned long long>(blk, nblk, blk_base, n_nl_d, s)));
+  const unsigned long long n_nl = read_scalar(n_nl_d, s, &e);
+  FA_HIP(e);
+  unsigned long long* nl = nullptr;
+  FA_HIP(t.get(&nl, n_nl));
+  if (N > 0 && n_nl > 0) {
+    nl_write_kernel<<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(b, N, blk_base, nl);
+    FA_HIP(hipGetLastError());
+  }
+  uint8_t last = '\n';
+  if (N > 0) {
+    FA_HIP(hipMemcpyAsync(&last, b + N - 1, 1, hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+  }
+  const unsigned long long M = n_nl + (last != '\n' ? 1 : 0);  // lines
+  if (M == 0) return fail(RNJ_E_INVALID, "empty FASTA input: no sequences");
+
+  // 2. lines
+  uint32_t *hcount = nullptr, *dcount = nullptr, *hbase = nullptr, *H_d = nullptr;
+  unsigned long long *dspan = nullptr, *off = nullptr, *total_d = nullptr;
+  FA_HIP(t.get(&hcount, M));
+  FA_HIP(t.get(&dcount, M));
+  FA_HIP(t.get(&dspan, 2 * M));
+  line_kernel<<<dim3(grid_for(M, kThreads / 64, 1u << 20)), dim3(kThreads), 0, s>>>(b, N, nl, n_nl, M, hcount,
+                                                                          dspan, dcount);
+  FA_HIP(hipGetLastError());
+
+  // 3. scans
+  FA_HIP(t.get(&hbase, M));
+  FA_HIP(t.get(&H_d, 1));
+  FA_HIP(t.get(&off, M));
+  FA_HIP(t.get(&total_d, 1));
+  FA_HIP((exclusive_scan<uint32_t, uint32_t>(hcount, M, hbase, H_d, s)));
+  FA_HIP((exclusive_scan<uint32_t, unsigned long long>(dcount, M, off, total_d, s)));
+  const unsigned H = read_scalar(H_d, s, &e);
+  FA_HIP(e);
+  const unsigned R = H + 1;  // record 0: data before the first header
+
+  // 4. records
+  unsigned long long *rec_len = nullptr, *rec_base = nullptr;
+  uint32_t *stored = nullptr, *seq = nullptr, *nseq_d = nullptr;
+  unsigned *first_d = nullptr, *bad_d = nullptr;
+  FA_HIP(t.get(&rec_len, R));
+  FA_HIP(t.get(&rec_base, R));
+  FA_HIP(t.get(&stored, R));
+  FA_HIP(t.get(&seq, R));
+  FA_HIP(t.get(&nseq_d, 1));
+  FA_HIP(t.get(&first_d, 1));
+  FA_HIP(t.get(&bad_d, 1));
+  FA_HIP(hipMemsetAsync(rec_len, 0, R * sizeof(unsigned long long), s));
+  FA_HIP(hipMemsetAsync(rec_base, 0xff, R * sizeof(unsigned long long), s));
+  FA_HIP(hipMemsetAsync(first_d, 0xff, sizeof(unsigned), s));
+  FA_HIP(hipMemsetAsync(bad_d, 0xff, sizeof(unsigned), s));
+  record_kernel<<<dim3(grid_for(M, 256, 0x7fffffffu)), dim3(256), 0, s>>>(M, hcount, hbase, dcount,
+                                                                           off, rec_len, rec_base);
+  FA_HIP(hipGetLastError());
+  stored_flag_kernel<<<dim3(grid_for(R, 256, 0x7fffffffu)), dim3(256), 0, s>>>(R, rec_len, stored,
+                                                                                first_d);
+  FA_HIP(hipGetLastError());
+  FA_HIP((exclusive_scan<uint32_t, uint32_t>(stored, R, seq, nseq_d, s)));
+  length_check_kernel<<<dim3(grid_for(R, 256, 0x7fffffffu)), dim3(256), 0, s>>>(R, rec_len, seq,
+                                                                                 first_d, bad_d);
+  FA_HIP(hipGetLastError());
+  const unsigned nseq = read_scalar(nseq_d, s, &e);
+  FA_HIP(e);
+  const unsigned first = read_scalar(first_d, s, &e);
+  FA_HIP(e);
+  const unsigned bad = read_scalar(bad_d, s, &e);
+  FA_HIP(e);
+  if (nseq < 2)
+    return fail(RNJ_E_INVALID, "found " + std::to_string(nseq) +
+                                   " sequences in the FASTA input; at least 2 are needed");
+  unsigned long long L = 0;
+  FA_HIP(hipMemcpyAsync(&L, rec_len + first, sizeof(L), hipMemcpyDeviceToHost, s));
+  FA_HIP(hipStreamSynchronize(s));
+  if (bad != 0xffffffffu) {
+    // the reference reports the first offending sequence with its length
+    std::vector<unsigned long long> lens(R);
+    std::vector<uint32_t> seqs(R);
+    FA_HIP(hipMemcpyAsync(lens.data(), rec_len, R * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(seqs.data(), seq, R * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    unsigned long long blen = 0;
+    for (unsigned r = 0; r < R; ++r)
+      if (lens[r] > 0 && seqs[r] == bad) blen = lens[r];
+    return fail(RNJ_E_INVALID, "sequence " + std::to_string(bad) + " has length " +
+                                   std::to_string(blen) + " while the preceding sequences have length " +
+                                   std::to_string(L));
+  }
+  if (L >= (1ull << 32) - 64) return fail(RNJ_E_INVALID, "sequences longer than 2^32 sites");
+
+  // 5. character matrix
+  uint8_t* chars = nullptr;
+  FA_HIP(t.get(&chars, (unsigned long long)nseq * L));
+  scatter_kernel<<<dim3(grid_for(M, kThreads / 64, 1u << 20)), dim3(kThreads), 0, s>>>(
+      b, M, hcount, hbase, dspan, dcount, off, rec_base, seq, L, chars);
+  FA_HIP(hipGetLastError());
+  std::vector<char> row0(L);
+  FA_HIP(hipMemcpyAsync(row0.data(), chars, L, hipMemcpyDeviceToHost, s));
+  FA_HIP(hipStreamSynchronize(s));
+  if (!first_row_is_dna(row0.data(), (unsigned)L))
+    return fail(RNJ_E_NOT_DNA, "first sequence is not DNA; only DNA distances are supported");
+
+  // 6. pack + names
+  Alignment* a = nullptr;
+  int rc = alloc_alignment(ctx, nseq, (unsigned)L, (unsigned)L, &a);
+  if (rc != RNJ_OK) return rc;
+  e = launch_pack_chars(chars, L, nseq, (unsigned)L, a->n_pad, a->W, a->planes, s);
+  if (e == hipSuccess && H > 0) {
+    unsigned long long *spans = nullptr, *nlen = nullptr, *noff = nullptr, *ntot = nullptr;
+    uint8_t* blob = nullptr;
+    e = t.get(&spans, 2ull * H);
+    if (e == hipSuccess) e = t.get(&nlen, H);
+    if (e == hipSuccess) e = t.get(&noff, H);
+    if (e == hipSuccess) e = t.get(&ntot, 1);
+    if (e == hipSuccess) {
+      header_kernel<<<dim3(grid_for(M, 256, 0x7fffffffu)), dim3(256), 0, s>>>(b, N, nl, n_nl, M, hcount,
+                                                                               hbase, spans);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      name_len_kernel<<<dim3(grid_for(H, 256, 0x7fffffffu)), dim3(256), 0, s>>>(H, spans, nlen);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = exclusive_scan<unsigned long long, unsigned long long>(nlen, H, noff, ntot, s);
+    unsigned long long total = 0;
+    if (e == hipSuccess) total = read_scalar(ntot, s, &e);
+    if (e == hipSuccess) e = t.get(&blob, total);
+    if (e == hipSuccess && total > 0) {
+      name_gather_kernel<<<dim3(H), dim3(64), 0, s>>>(b, H, spans, noff, blob);
+      e = hipGetLastError();
+    }
+    std::vector<unsigned long long> hoff(H), hlen(H);
+    std::vector<char> hblob(total);
+    if (e == hipSuccess) e = hipMemcpyAsync(hoff.data(), noff, H * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hlen.data(), nlen, H * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && total > 0)
+      e = hipMemcpyAsync(hblob.data(), blob, total, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) {
+      a->names.reserve(H);
+      for (unsigned h = 0; h < H; ++h) a->names.emplace_back(hblob.data() + hoff[h], hlen[h]);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    rnj_alignment_free(static_cast<rnj_alignment*>(a));
+    return hip_fail(e, "FASTA parse");
+  }
+  *out = a;
+  return RNJ_OK;
+}
+
+}  // namespace rnj

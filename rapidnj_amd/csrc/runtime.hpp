@@ -72,6 +72,7 @@ struct Alignment {
   unsigned n_pad = 0;        // multiple of kTaxaPad
   unsigned W = 0;            // 32-site words, multiple of kWordPad (= W_pad of the planes)
   uint32_t* planes = nullptr;  // [3][W][n_pad]
+  std::vector<std::string> names;  // FASTA record names in file order (alignments from FASTA)
   size_t bytes() const { return (size_t)3 * W * n_pad * sizeof(uint32_t); }
 };
 
@@ -93,6 +94,13 @@ size_t sentinel_capacity(unsigned n);
 
 // Allocate planes for an alignment of n taxa and `sites` counted sites (zeroed).
 int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out);
+
+// dataloaderFasta.hpp:54-76 on the first sequence: DNA unless a byte >= 'A' is not ACGTUN.
+bool first_row_is_dna(const char* row, unsigned L);
+
+// Parse a FASTA image already in device memory (16-byte aligned) and pack it (fasta.hip).
+int fasta_to_alignment(Context* ctx, const uint8_t* bytes, size_t len, hipStream_t stream,
+                       Alignment** out);
 
 // Enqueue a multi-job launch on `stream`: reset statistics, tile kernel, saturation patch
 // (unless raw), max copies. All jobs must share one tile configuration (chosen here). exact:

@@ -51,6 +51,10 @@ SIGNATURES = {
                                                 ctypes.POINTER(_vp)]),
     "rnj_alignment_from_rows": (ctypes.c_int, [_vp, _vp, _u, _u, ctypes.POINTER(_vp)]),
     "rnj_alignment_from_bitstrings": (ctypes.c_int, [_vp, _vp, _vp, _u, _u, ctypes.POINTER(_vp)]),
+    "rnj_alignment_from_fasta": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "rnj_alignment_from_fasta_file": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "rnj_alignment_name_count": (_u, [_vp]),
+    "rnj_alignment_name": (ctypes.c_char_p, [_vp, _u]),
     "rnj_alignment_free": (ctypes.c_int, [_vp]),
     "rnj_alignment_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u), ctypes.POINTER(_u),
                                           ctypes.POINTER(_u), ctypes.POINTER(_sz)]),
@@ -177,6 +181,12 @@ class Alignment:
     def L(self) -> int:
         return self.info.L
 
+    @property
+    def names(self) -> list[bytes]:
+        """FASTA record names in file order (empty for alignments not read from FASTA)."""
+        k = lib().rnj_alignment_name_count(self.handle)
+        return [lib().rnj_alignment_name(self.handle, i) for i in range(k)]
+
     def planes(self) -> np.ndarray:
         out = np.zeros((3, self.info.W, self.info.n), dtype=np.uint32)
         check(lib().rnj_alignment_planes(self.handle, out.ctypes.data))
@@ -274,6 +284,25 @@ class Context:
         check(lib().rnj_alignment_from_bitstrings(self.handle, ctypes.cast(pb, _vp),
                                                   ctypes.cast(pg, _vp), n, words // 4,
                                                   ctypes.byref(h)))
+        return Alignment(self, h.value)
+
+    def alignment_from_fasta(self, data) -> Alignment:
+        """Parse a FASTA image on the device (bytes / uint8 numpy on the host, or a uint8 torch
+        tensor on the device) and pack it; record names in Alignment.names."""
+        h = _vp()
+        if hasattr(data, "data_ptr"):
+            check(lib().rnj_alignment_from_fasta(self.handle, data.data_ptr(), data.numel(), 1,
+                                                 ctypes.byref(h)))
+        else:
+            buf = np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray) \
+                else np.ascontiguousarray(data, dtype=np.uint8)
+            check(lib().rnj_alignment_from_fasta(self.handle, buf.ctypes.data, buf.size, 0,
+                                                 ctypes.byref(h)))
+        return Alignment(self, h.value)
+
+    def alignment_from_fasta_file(self, path: str) -> Alignment:
+        h = _vp()
+        check(lib().rnj_alignment_from_fasta_file(self.handle, os.fsencode(path), ctypes.byref(h)))
         return Alignment(self, h.value)
 
     # ---- distances ----------------------------------------------------------------------------
