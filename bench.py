@@ -152,6 +152,53 @@ def probe_traffic(args) -> int:
     return 0
 
 
+def ingest_compare(ctx, aln) -> dict:
+    """FASTA -> packed alignment in HBM for the benchmark alignment (60-column lines): from host
+    bytes (upload + device parse + pack), from a device-resident file image (parse + pack only),
+    and from a file (pinned read + upload + parse + pack), next to the reference's reader
+    (dataloaderFasta, readData with fastdist) on the same file."""
+    import torch
+    from oracle import oracle as O
+    lines = []
+    for i, row in enumerate(aln):
+        lines.append(b">t%d" % i)
+        b = row.tobytes()
+        lines += [b[k:k + 60] for k in range(0, len(b), 60)]
+    data = b"\n".join(lines) + b"\n"
+    out = {"bytes": len(data), "taxa": int(aln.shape[0]), "sites": int(aln.shape[1])}
+
+    def best(fn, reps=3):
+        t = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            a = fn()
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+            a.close()
+        return min(t)
+    host = best(lambda: ctx.alignment_from_fasta(data))
+    dev = torch.frombuffer(bytearray(data), dtype=torch.uint8).to("cuda:0")
+    device = best(lambda: ctx.alignment_from_fasta(dev))
+    del dev
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "bench.fa")
+        with open(path, "wb") as fh:
+            fh.write(data)
+        from_file = best(lambda: ctx.alignment_from_fasta_file(path))
+        ref = None
+        if O.ref_harness_available():
+            r = subprocess.run([O.REF_HARNESS, "load", path, "1"], check=True, capture_output=True,
+                               text=True)
+            ref = json.loads(r.stdout.strip().splitlines()[-1])["best_s"]
+    out.update({"host_bytes_ms": round(host * 1e3, 2), "device_image_ms": round(device * 1e3, 2),
+                "file_ms": round(from_file * 1e3, 2),
+                "device_image_GBps": round(len(data) / device / 1e9, 1),
+                "cpu_reference_reader_ms": None if ref is None else round(ref * 1e3, 1),
+                "speedup_file_vs_reference": None if ref is None else round(ref / from_file, 1)})
+    return out
+
+
 def _gen_job(job):
     from rapidnj_amd.workloads import simulate_alignment
     return simulate_alignment(job.n, job.L, job.gap, job.seed)
@@ -412,6 +459,8 @@ def main() -> int:
                                          "unit": "G site-pairs/s"}
         PA.close()
         del p_out
+        log("side run: FASTA ingest (device reader) vs the reference reader")
+        extra["ingest"] = ingest_compare(ctx, aln)
         log("side run: batched bootstrap vs one launch per replicate")
         extra["bootstrap"] = bootstrap_compare(ctx, dev)
         log(f"side run: served stream, {len(served_jobs)} jobs")

@@ -60,15 +60,30 @@ __device__ __forceinline__ unsigned count16(const uint4& q) {
   return __popc(nl_bits(q.x)) + __popc(nl_bits(q.y)) + __popc(nl_bits(q.z)) + __popc(nl_bits(q.w));
 }
 
-// Pass 1a: '\n' count of each 16 KB block.
+// Bytes of x in 33..127 (signed chars > 32: what a sequence line keeps) as set high bits.
+__device__ __forceinline__ uint32_t kept_bits(uint32_t x) {
+  return ((x & 0x7F7F7F7Fu) + 0x5F5F5F5Fu) & ~x & 0x80808080u;
+}
+
+__device__ __forceinline__ unsigned kept16(const uint4& q) {
+  return __popc(kept_bits(q.x)) + __popc(kept_bits(q.y)) + __popc(kept_bits(q.z)) +
+         __popc(kept_bits(q.w));
+}
+
+// Pass 1a: '\n' count of each 16 KB block, and the kept-byte count of each 16-byte chunk.
 __global__ __launch_bounds__(kThreads) void nl_count_kernel(const uint8_t* __restrict__ b, size_t N,
-                                                            uint32_t* __restrict__ blk) {
+                                                            uint32_t* __restrict__ blk,
+                                                            uint8_t* __restrict__ kc) {
   const size_t base = (size_t)blockIdx.x * kChunk;
   unsigned c = 0;
 #pragma unroll
   for (unsigned r = 0; r < kRounds; ++r) {
     const size_t p = base + ((size_t)r * kThreads + threadIdx.x) * 16;
-    if (p < N) c += count16(load16(b, N, p));
+    if (p < N) {
+      const uint4 q = load16(b, N, p);
+      c += count16(q);
+      kc[p / 16] = (uint8_t)kept16(q);
+    }
   }
   __shared__ unsigned s[kThreads / 64];
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
@@ -164,33 +179,36 @@ __device__ __forceinline__ unsigned walk_line(const uint8_t* b, Line L, unsigned
   return h;
 }
 
-// Pass 2: one wave per line.
-__global__ __launch_bounds__(kThreads) void line_kernel(const uint8_t* __restrict__ b, size_t N,
-                                                        const unsigned long long* __restrict__ nl,
-                                                        unsigned long long n_nl,
-                                                        unsigned long long M,
-                                                        uint32_t* __restrict__ hcount,
-                                                        unsigned long long* __restrict__ dspan,
-                                                        uint32_t* __restrict__ dcount) {
-  const unsigned lane = threadIdx.x & 63;
-  const unsigned long long waves = (unsigned long long)gridDim.x * (kThreads / 64);
-  for (unsigned long long l = (unsigned long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-       l < M; l += waves) {
-    const Line L = line_at(nl, n_nl, N, l);
-    unsigned long long ds, de;
-    const unsigned h = walk_line(b, L, &ds, &de, nullptr);
-    unsigned c = 0;
-    for (unsigned long long p = ds + lane; p < de + lane; p += 64) {
-      const bool keep = p < de && sbyte(b, p) > 32;
-      c += (unsigned)__popcll(__ballot(keep));
-    }
-    if (lane == 0) {
-      hcount[l] = h;
-      dspan[2 * l] = ds;
-      dspan[2 * l + 1] = de;
-      dcount[l] = c;
-    }
-  }
+// Kept bytes (33..127) in [0, x): chunk prefix Kc (Kc[N/16 rounded up] = total) plus the
+// chunk's bytes before x.
+__device__ __forceinline__ unsigned long long kept_before(const uint8_t* b,
+                                                          const unsigned long long* Kc,
+                                                          unsigned long long x) {
+  const unsigned long long c = x / 16;
+  unsigned long long k = Kc[c];
+  for (unsigned long long p = c * 16; p < x; ++p) k += (unsigned)(sbyte(b, p) > 32);
+  return k;
+}
+
+// Pass 2: one thread per line: header count, data span, kept bytes of the span, and the kept
+// count before the span (for the scatter's in-line rank).
+__global__ void line_kernel(const uint8_t* __restrict__ b, size_t N,
+                            const unsigned long long* __restrict__ nl, unsigned long long n_nl,
+                            unsigned long long M, const unsigned long long* __restrict__ Kc,
+                            uint32_t* __restrict__ hcount, unsigned long long* __restrict__ dspan,
+                            uint32_t* __restrict__ dcount, unsigned long long* __restrict__ kds) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M) return;
+  const Line L = line_at(nl, n_nl, N, l);
+  unsigned long long ds, de;
+  const unsigned h = walk_line(b, L, &ds, &de, nullptr);
+  const unsigned long long k0 = kept_before(b, Kc, ds);
+  const unsigned long long k1 = ds == de ? k0 : kept_before(b, Kc, de);
+  hcount[l] = h;
+  dspan[2 * l] = ds;
+  dspan[2 * l + 1] = de;
+  dcount[l] = (uint32_t)(k1 - k0);
+  kds[l] = k0;
 }
 
 // Name spans of every header, at their header index.
@@ -214,18 +232,25 @@ __global__ void header_kernel(const uint8_t* __restrict__ b, size_t N,
   }
 }
 
-// Pass 4a: record of each data line; record length and first byte offset.
-__global__ void record_kernel(unsigned long long M, const uint32_t* __restrict__ hcount,
-                              const uint32_t* __restrict__ hbase,
-                              const uint32_t* __restrict__ dcount,
-                              const unsigned long long* __restrict__ off,
-                              unsigned long long* __restrict__ rec_len,
-                              unsigned long long* __restrict__ rec_base) {
+// Pass 4a: first byte offset of every record. Records are contiguous runs of lines (the record
+// of a line, hbase + hcount, never decreases); the records opened on line l start at its
+// offset, which also makes records opened and closed on one line empty.
+__global__ void record_base_kernel(unsigned long long M, const uint32_t* __restrict__ hcount,
+                                   const uint32_t* __restrict__ hbase,
+                                   const unsigned long long* __restrict__ off,
+                                   unsigned long long* __restrict__ rec_base) {
   const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= M || dcount[l] == 0) return;
-  const unsigned r = hbase[l] + hcount[l];  // record 0 holds data before the first header
-  atomicAdd(&rec_len[r], (unsigned long long)dcount[l]);
-  atomicMin(&rec_base[r], off[l]);
+  if (l >= M) return;
+  const unsigned hi = hbase[l] + hcount[l];
+  const unsigned lo = l == 0 ? 0u : hbase[l - 1] + hcount[l - 1] + 1u;
+  for (unsigned r = lo; r <= hi; ++r) rec_base[r] = off[l];
+}
+
+__global__ void record_len_kernel(unsigned R, const unsigned long long* __restrict__ rec_base,
+                                  const unsigned long long* __restrict__ total,
+                                  unsigned long long* __restrict__ rec_len) {
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) rec_len[r] = (r + 1 < R ? rec_base[r + 1] : *total) - rec_base[r];
 }
 
 __global__ void stored_flag_kernel(unsigned R, const unsigned long long* __restrict__ rec_len,
@@ -245,36 +270,66 @@ __global__ void length_check_kernel(unsigned R, const unsigned long long* __rest
   if (rec_len[r] != rec_len[*first]) atomicMin(bad, seq[r]);
 }
 
-// Pass 5: one wave per data line; kept bytes to row seq[r] of the n x L character matrix.
-__global__ __launch_bounds__(kThreads) void scatter_kernel(const uint8_t* __restrict__ b,
-                                                           unsigned long long M,
-                                                           const uint32_t* __restrict__ hcount,
-                                                           const uint32_t* __restrict__ hbase,
-                                                           const unsigned long long* __restrict__ dspan,
-                                                           const uint32_t* __restrict__ dcount,
-                                                           const unsigned long long* __restrict__ off,
-                                                           const unsigned long long* __restrict__ rec_base,
-                                                           const uint32_t* __restrict__ seq,
-                                                           unsigned long long L,
-                                                           uint8_t* __restrict__ chars) {
+// Pass 5a: destination of each data line's first kept byte in the n x L matrix, less the kept
+// count before the line's span (so that a byte's destination is dbase + its kept rank in the file).
+__global__ void line_dest_kernel(unsigned long long M, const uint32_t* __restrict__ hcount,
+                                 const uint32_t* __restrict__ hbase,
+                                 const unsigned long long* __restrict__ dspan,
+                                 const unsigned long long* __restrict__ kds,
+                                 const unsigned long long* __restrict__ off,
+                                 const unsigned long long* __restrict__ rec_base,
+                                 const uint32_t* __restrict__ seq, unsigned long long L,
+                                 unsigned long long* __restrict__ dbase) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M) return;
+  const unsigned r = hbase[l] + hcount[l];
+  dbase[l] = dspan[2 * l] < dspan[2 * l + 1]
+                 ? (unsigned long long)seq[r] * L + (off[l] - rec_base[r]) - kds[l]
+                 : 0ull;
+}
+
+// Pass 5b: one wave per 4 KB of the file, lanes on consecutive bytes (coalesced loads, and
+// consecutive destinations within a line). The wave finds the line of its first byte (binary
+// search over the '\n' positions); per 64-byte window, ballots give every lane its line (line
+// at the window start + '\n' bytes below it) and its kept rank (kept bytes before the window +
+// kept bytes below it). A kept byte inside its line's data span goes to dbase[line] + rank.
+constexpr unsigned kScatterBytes = 4096;
+
+__global__ __launch_bounds__(256) void scatter_kernel(const uint8_t* __restrict__ b, size_t N,
+                                                      const unsigned long long* __restrict__ nl,
+                                                      unsigned long long n_nl,
+                                                      const unsigned long long* __restrict__ Kc,
+                                                      const unsigned long long* __restrict__ dspan,
+                                                      const unsigned long long* __restrict__ dbase,
+                                                      uint8_t* __restrict__ chars) {
   const unsigned lane = threadIdx.x & 63;
-  const unsigned long long waves = (unsigned long long)gridDim.x * (kThreads / 64);
-  for (unsigned long long l = (unsigned long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-       l < M; l += waves) {
-    if (dcount[l] == 0) continue;
-    const unsigned r = hbase[l] + hcount[l];
-    uint8_t* row = chars + (unsigned long long)seq[r] * L + (off[l] - rec_base[r]);
-    const unsigned long long ds = dspan[2 * l], de = dspan[2 * l + 1];
-    unsigned long long w = 0;
-    for (unsigned long long p = ds + lane; p < de + lane; p += 64) {
-      const uint8_t c = p < de ? b[p] : 0;
-      const bool keep = p < de && (int)(int8_t)c > 32;
-      const unsigned long long m = __ballot(keep);
-      const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-      if (keep) row[w + below] = c;
-      w += (unsigned)__popcll(m);
+  const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const unsigned long long p0 = wave * kScatterBytes;
+  if (p0 >= N) return;  // uniform over the wave
+  unsigned long long lo = 0, hi = n_nl;
+  while (lo < hi) {
+    const unsigned long long mid = (lo + hi) / 2;
+    if (nl[mid] < p0) lo = mid + 1; else hi = mid;
+  }
+  unsigned long long line = lo, rank = Kc[p0 / 16];
+  const unsigned long long pend = p0 + kScatterBytes < N ? p0 + kScatterBytes : N;
+  for (unsigned long long w = p0; w < pend; w += 64) {
+    const unsigned long long p = w + lane;
+    const bool in = p < pend;
+    const uint8_t c = in ? b[p] : 0;
+    const bool is_nl = in && c == '\n';
+    const bool kept = in && (int)(int8_t)c > 32;
+    const unsigned long long nlm = __ballot(is_nl), km = __ballot(kept);
+    const unsigned nl_below = __builtin_amdgcn_mbcnt_hi((unsigned)(nlm >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)nlm, 0u));
+    const unsigned k_below = __builtin_amdgcn_mbcnt_hi((unsigned)(km >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((unsigned)km, 0u));
+    if (kept) {
+      const unsigned long long my = line + nl_below;
+      if (p >= dspan[2 * my] && p < dspan[2 * my + 1]) chars[dbase[my] + rank + k_below] = c;
     }
+    line += (unsigned)__popcll(nlm);
+    rank += (unsigned)__popcll(km);
   }
 }
 
@@ -339,55 +394,81 @@ __global__ __launch_bounds__(kThreads) void scan_add_kernel(T* __restrict__ out,
     if (i0 + k < n) out[i0 + k] += add;
 }
 
-// out[i] = in[0] + .. + in[i-1]; *total (device) = the sum of all (written at out[n] is not done).
-template <class Tin, class Tout>
-hipError_t exclusive_scan(const Tin* in, unsigned long long n, Tout* out, Tout* total,
-                          hipStream_t s) {
-  constexpr unsigned long long kPer = (unsigned long long)kThreads * 8;
-  const unsigned long long blocks = n == 0 ? 1 : (n + kPer - 1) / kPer;
-  Tout* sums = nullptr;
-  hipError_t e = hipMallocAsync((void**)&sums, (blocks + 1) * sizeof(Tout), s);
-  if (e != hipSuccess) return e;
-  scan_blocks_kernel<Tin, Tout><<<dim3((unsigned)blocks), dim3(kThreads), 0, s>>>(in, n, out, sums);
-  e = hipGetLastError();
-  if (e == hipSuccess) {
-    if (blocks > 1) {
-      Tout* sums_ex = nullptr;
-      e = hipMallocAsync((void**)&sums_ex, blocks * sizeof(Tout), s);
-      if (e == hipSuccess) e = exclusive_scan<Tout, Tout>(sums, blocks, sums_ex, total, s);
-      if (e == hipSuccess) {
-        scan_add_kernel<Tout><<<dim3((unsigned)blocks), dim3(kThreads), 0, s>>>(out, n, sums_ex);
-        e = hipGetLastError();
-      }
-      if (sums_ex) (void)hipFreeAsync(sums_ex, s);
-    } else if (total) {
-      e = hipMemcpyAsync(total, sums, sizeof(Tout), hipMemcpyDeviceToDevice, s);
-    }
-  }
-  (void)hipFreeAsync(sums, s);
-  return e;
-}
-
 unsigned grid_for(unsigned long long items, unsigned per_block, unsigned cap = 0x7fffffffu) {
   const unsigned long long g = (items + per_block - 1) / per_block;
   return (unsigned)(g == 0 ? 1 : (g > cap ? cap : g));
 }
 
-// Temporary device buffers of one parse, released in order on the stream.
+// Temporary device buffers of one parse. Blocks come from the context's cache of earlier parses
+// (power-of-two sizes) or from hipMalloc, and go back to the cache when the parse ends (after
+// its stream has drained); the cache keeps at most kScratchCache bytes.
+constexpr size_t kScratchCache = (size_t)4 << 30;
+
 struct Temps {
+  Context* ctx;
   hipStream_t s;
-  std::vector<void*> ptrs;
+  std::vector<std::pair<size_t, void*>> held;
   template <class T>
   hipError_t get(T** p, unsigned long long count) {
-    *p = nullptr;
-    hipError_t e = hipMallocAsync((void**)p, (count ? count : 1) * sizeof(T), s);
-    if (e == hipSuccess) ptrs.push_back(*p);
-    return e;
+    size_t bytes = 256;
+    while (bytes < (count ? count : 1) * sizeof(T)) bytes <<= 1;
+    {
+      std::lock_guard<std::mutex> lock(ctx->scratch_mu);
+      auto it = ctx->scratch_free.find(bytes);
+      if (it != ctx->scratch_free.end()) {
+        *p = static_cast<T*>(it->second);
+        ctx->scratch_free.erase(it);
+        ctx->scratch_bytes -= bytes;
+        held.emplace_back(bytes, *p);
+        return hipSuccess;
+      }
+    }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return e;
+    *p = static_cast<T*>(q);
+    held.emplace_back(bytes, q);
+    return hipSuccess;
   }
   ~Temps() {
-    for (void* p : ptrs) (void)hipFreeAsync(p, s);
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lock(ctx->scratch_mu);
+    for (auto& b : held) {
+      if (ctx->scratch_bytes + b.first <= kScratchCache) {
+        ctx->scratch_free.emplace(b.first, b.second);
+        ctx->scratch_bytes += b.first;
+      } else {
+        (void)hipFree(b.second);
+      }
+    }
   }
 };
+
+// out[i] = in[0] + .. + in[i-1], and *total (device, may be null) = the sum of all.
+template <class Tin, class Tout>
+hipError_t exclusive_scan(const Tin* in, unsigned long long n, Tout* out, Tout* total,
+                          hipStream_t s, Temps& t) {
+  constexpr unsigned long long kPer = (unsigned long long)kThreads * 8;
+  const unsigned long long blocks = n == 0 ? 1 : (n + kPer - 1) / kPer;
+  Tout* sums = nullptr;
+  hipError_t e = t.get(&sums, blocks + 1);
+  if (e != hipSuccess) return e;
+  scan_blocks_kernel<Tin, Tout><<<dim3((unsigned)blocks), dim3(kThreads), 0, s>>>(in, n, out, sums);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (blocks > 1) {
+    Tout* sums_ex = nullptr;
+    e = t.get(&sums_ex, blocks);
+    if (e == hipSuccess) e = exclusive_scan<Tout, Tout>(sums, blocks, sums_ex, total, s, t);
+    if (e == hipSuccess) {
+      scan_add_kernel<Tout><<<dim3((unsigned)blocks), dim3(kThreads), 0, s>>>(out, n, sums_ex);
+      e = hipGetLastError();
+    }
+    return e;
+  }
+  if (total) e = hipMemcpyAsync(total, sums, sizeof(Tout), hipMemcpyDeviceToDevice, s);
+  return e;
+}
 
 #define FA_HIP(call)                                 \
   do {                                               \
@@ -405,23 +486,34 @@ T read_scalar(const T* d, hipStream_t s, hipError_t* e) {
 
 }  // namespace
 
-int fasta_to_alignment(Context* ctx, const uint8_t* b, size_t N, hipStream_t s, Alignment** out) {
-  Temps t{s, {}};
+int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind stage,
+                       hipStream_t s, Alignment** out) {
+  Temps t{ctx, s, {}};
   hipError_t e;
+  const uint8_t* b = src;
+  if (stage == hipMemcpyHostToDevice || stage == hipMemcpyDeviceToDevice) {
+    uint8_t* copy = nullptr;
+    FA_HIP(t.get(&copy, N));
+    if (N) FA_HIP(hipMemcpyAsync(copy, src, N, stage, s));
+    b = copy;
+  }
   // 1. '\n' positions
-  const unsigned long long nblk = N == 0 ? 1 : (N + kChunk - 1) / kChunk;
+  if (N == 0) return fail(RNJ_E_INVALID, "empty FASTA input: no sequences");
+  const unsigned long long nblk = (N + kChunk - 1) / kChunk;
+  const unsigned long long nchunks = (N + 15) / 16;
   uint32_t* blk = nullptr;
-  unsigned long long *blk_base = nullptr, *n_nl_d = nullptr;
+  uint8_t* kc = nullptr;
+  unsigned long long *blk_base = nullptr, *n_nl_d = nullptr, *Kc = nullptr;
   FA_HIP(t.get(&blk, nblk));
   FA_HIP(t.get(&blk_base, nblk));
   FA_HIP(t.get(&n_nl_d, 1));
-  if (N > 0) {
-    nl_count_kernel<<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(b, N, blk);
-    FA_HIP(hipGetLastError());
-  } else {
-    FA_HIP(hipMemsetAsync(blk, 0, sizeof(uint32_t), s));
-  }
-  FA_HIP((exclusive_scan<uint32_t, unsigned long long>(blk, nblk, blk_base, n_nl_d, s)));
+  FA_HIP(t.get(&kc, nchunks));
+  FA_HIP(t.get(&Kc, nchunks + 1));
+  nl_count_kernel<<<dim3((unsigned)nblk), dim3(kThreads), 0, s>>>(b, N, blk, kc);
+  FA_HIP(hipGetLastError());
+  FA_HIP((exclusive_scan<uint32_t, unsigned long long>(blk, nblk, blk_base, n_nl_d, s, t)));
+  // kept bytes before every 16-byte chunk; Kc[nchunks] = all kept bytes
+  FA_HIP((exclusive_scan<uint8_t, unsigned long long>(kc, nchunks, Kc, Kc + nchunks, s, t)));
   const unsigned long long n_nl = read_scalar(n_nl_d, s, &e);
   FA_HIP(e);
   unsigned long long* nl = nullptr;
@@ -431,21 +523,20 @@ int fasta_to_alignment(Context* ctx, const uint8_t* b, size_t N, hipStream_t s, 
     FA_HIP(hipGetLastError());
   }
   uint8_t last = '\n';
-  if (N > 0) {
-    FA_HIP(hipMemcpyAsync(&last, b + N - 1, 1, hipMemcpyDeviceToHost, s));
-    FA_HIP(hipStreamSynchronize(s));
-  }
+  FA_HIP(hipMemcpyAsync(&last, b + N - 1, 1, hipMemcpyDeviceToHost, s));
+  FA_HIP(hipStreamSynchronize(s));
   const unsigned long long M = n_nl + (last != '\n' ? 1 : 0);  // lines
   if (M == 0) return fail(RNJ_E_INVALID, "empty FASTA input: no sequences");
 
   // 2. lines
   uint32_t *hcount = nullptr, *dcount = nullptr, *hbase = nullptr, *H_d = nullptr;
-  unsigned long long *dspan = nullptr, *off = nullptr, *total_d = nullptr;
+  unsigned long long *dspan = nullptr, *kds = nullptr, *off = nullptr, *total_d = nullptr;
   FA_HIP(t.get(&hcount, M));
   FA_HIP(t.get(&dcount, M));
   FA_HIP(t.get(&dspan, 2 * M));
-  line_kernel<<<dim3(grid_for(M, kThreads / 64, 1u << 20)), dim3(kThreads), 0, s>>>(b, N, nl, n_nl, M, hcount,
-                                                                          dspan, dcount);
+  FA_HIP(t.get(&kds, M));
+  line_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(b, N, nl, n_nl, M, Kc, hcount, dspan,
+                                                           dcount, kds);
   FA_HIP(hipGetLastError());
 
   // 3. scans
@@ -453,8 +544,8 @@ int fasta_to_alignment(Context* ctx, const uint8_t* b, size_t N, hipStream_t s, 
   FA_HIP(t.get(&H_d, 1));
   FA_HIP(t.get(&off, M));
   FA_HIP(t.get(&total_d, 1));
-  FA_HIP((exclusive_scan<uint32_t, uint32_t>(hcount, M, hbase, H_d, s)));
-  FA_HIP((exclusive_scan<uint32_t, unsigned long long>(dcount, M, off, total_d, s)));
+  FA_HIP((exclusive_scan<uint32_t, uint32_t>(hcount, M, hbase, H_d, s, t)));
+  FA_HIP((exclusive_scan<uint32_t, unsigned long long>(dcount, M, off, total_d, s, t)));
   const unsigned H = read_scalar(H_d, s, &e);
   FA_HIP(e);
   const unsigned R = H + 1;  // record 0: data before the first header
@@ -470,18 +561,17 @@ int fasta_to_alignment(Context* ctx, const uint8_t* b, size_t N, hipStream_t s, 
   FA_HIP(t.get(&nseq_d, 1));
   FA_HIP(t.get(&first_d, 1));
   FA_HIP(t.get(&bad_d, 1));
-  FA_HIP(hipMemsetAsync(rec_len, 0, R * sizeof(unsigned long long), s));
-  FA_HIP(hipMemsetAsync(rec_base, 0xff, R * sizeof(unsigned long long), s));
   FA_HIP(hipMemsetAsync(first_d, 0xff, sizeof(unsigned), s));
   FA_HIP(hipMemsetAsync(bad_d, 0xff, sizeof(unsigned), s));
-  record_kernel<<<dim3(grid_for(M, 256, 0x7fffffffu)), dim3(256), 0, s>>>(M, hcount, hbase, dcount,
-                                                                           off, rec_len, rec_base);
+  record_base_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(M, hcount, hbase, off, rec_base);
   FA_HIP(hipGetLastError());
-  stored_flag_kernel<<<dim3(grid_for(R, 256, 0x7fffffffu)), dim3(256), 0, s>>>(R, rec_len, stored,
+  record_len_kernel<<<dim3(grid_for(R, 256)), dim3(256), 0, s>>>(R, rec_base, total_d, rec_len);
+  FA_HIP(hipGetLastError());
+  stored_flag_kernel<<<dim3(grid_for(R, 256)), dim3(256), 0, s>>>(R, rec_len, stored,
                                                                                 first_d);
   FA_HIP(hipGetLastError());
-  FA_HIP((exclusive_scan<uint32_t, uint32_t>(stored, R, seq, nseq_d, s)));
-  length_check_kernel<<<dim3(grid_for(R, 256, 0x7fffffffu)), dim3(256), 0, s>>>(R, rec_len, seq,
+  FA_HIP((exclusive_scan<uint32_t, uint32_t>(stored, R, seq, nseq_d, s, t)));
+  length_check_kernel<<<dim3(grid_for(R, 256)), dim3(256), 0, s>>>(R, rec_len, seq,
                                                                                  first_d, bad_d);
   FA_HIP(hipGetLastError());
   const unsigned nseq = read_scalar(nseq_d, s, &e);
@@ -516,8 +606,14 @@ int fasta_to_alignment(Context* ctx, const uint8_t* b, size_t N, hipStream_t s, 
   // 5. character matrix
   uint8_t* chars = nullptr;
   FA_HIP(t.get(&chars, (unsigned long long)nseq * L));
-  scatter_kernel<<<dim3(grid_for(M, kThreads / 64, 1u << 20)), dim3(kThreads), 0, s>>>(
-      b, M, hcount, hbase, dspan, dcount, off, rec_base, seq, L, chars);
+  unsigned long long* dbase = nullptr;
+  FA_HIP(t.get(&dbase, M));
+  line_dest_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(M, hcount, hbase, dspan, kds, off,
+                                                                rec_base, seq, L, dbase);
+  FA_HIP(hipGetLastError());
+  const unsigned long long scatter_waves = (N + kScatterBytes - 1) / kScatterBytes;
+  scatter_kernel<<<dim3(grid_for(scatter_waves, 4)), dim3(256), 0, s>>>(b, N, nl, n_nl, Kc, dspan,
+                                                                        dbase, chars);
   FA_HIP(hipGetLastError());
   std::vector<char> row0(L);
   FA_HIP(hipMemcpyAsync(row0.data(), chars, L, hipMemcpyDeviceToHost, s));
@@ -538,15 +634,15 @@ int fasta_to_alignment(Context* ctx, const uint8_t* b, size_t N, hipStream_t s, 
     if (e == hipSuccess) e = t.get(&noff, H);
     if (e == hipSuccess) e = t.get(&ntot, 1);
     if (e == hipSuccess) {
-      header_kernel<<<dim3(grid_for(M, 256, 0x7fffffffu)), dim3(256), 0, s>>>(b, N, nl, n_nl, M, hcount,
+      header_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(b, N, nl, n_nl, M, hcount,
                                                                                hbase, spans);
       e = hipGetLastError();
     }
     if (e == hipSuccess) {
-      name_len_kernel<<<dim3(grid_for(H, 256, 0x7fffffffu)), dim3(256), 0, s>>>(H, spans, nlen);
+      name_len_kernel<<<dim3(grid_for(H, 256)), dim3(256), 0, s>>>(H, spans, nlen);
       e = hipGetLastError();
     }
-    if (e == hipSuccess) e = exclusive_scan<unsigned long long, unsigned long long>(nlen, H, noff, ntot, s);
+    if (e == hipSuccess) e = exclusive_scan<unsigned long long, unsigned long long>(nlen, H, noff, ntot, s, t);
     unsigned long long total = 0;
     if (e == hipSuccess) total = read_scalar(ntot, s, &e);
     if (e == hipSuccess) e = t.get(&blob, total);

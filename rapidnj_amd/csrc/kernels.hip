@@ -289,29 +289,59 @@ __device__ __forceinline__ void char_bits(uint8_t c, uint32_t& h, uint32_t& l, u
   l = (lc == 'g') | (lc == 'c');
 }
 
-__global__ void pack_chars_kernel(const uint8_t* __restrict__ chars, size_t pitch, unsigned n,
-                                  unsigned L, unsigned n_pad, unsigned W_pad,
-                                  uint32_t* __restrict__ planes) {
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_pad) return;
-  for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
-  uint32_t H = 0, Lo = 0, V = 0;
-  if (t < n) {
-    const uint8_t* row = chars + (size_t)t * pitch;
-    for (unsigned k = 0; k < 32; ++k) {
-      const unsigned s = w * 32 + k;
-      if (s >= L) break;
-      uint32_t h, l, v;
-      char_bits(row[s], h, l, v);
-      H |= h << k;
-      Lo |= l << k;
-      V |= v << k;
-    }
-  }
+// One work-group packs a tile of 64 taxa x 1024 sites (32 words): the tile's bytes are read row
+// by row (consecutive lanes, consecutive bytes: coalesced) into LDS rows padded by one word, then
+// lane t of every wave packs taxon t's words from LDS (rows one word apart, so the 64 lanes hit
+// 64 different banks) and stores them (consecutive lanes, consecutive taxa: coalesced). Sites at
+// or past L and taxa at or past n pack as 0 (not counted).
+constexpr unsigned kPackTaxa = 64, kPackWords = 32, kPackRow = kPackWords * 8 + 1;
+
+__global__ __launch_bounds__(256) void pack_chars_kernel(const uint8_t* __restrict__ chars,
+                                                         size_t pitch, unsigned n, unsigned L,
+                                                         unsigned n_pad, unsigned W_pad,
+                                                         uint32_t* __restrict__ planes) {
+  __shared__ uint32_t tile[kPackTaxa * kPackRow];
+  const unsigned t0 = blockIdx.x * kPackTaxa;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const size_t ps = (size_t)W_pad * n_pad;
-  planes[0 * ps + (size_t)w * n_pad + t] = H;
-  planes[1 * ps + (size_t)w * n_pad + t] = Lo;
-  planes[2 * ps + (size_t)w * n_pad + t] = V;
+  const unsigned t = t0 + lane;
+  for (unsigned w0 = blockIdx.y * kPackWords; w0 < W_pad; w0 += gridDim.y * kPackWords) {
+    for (unsigned e = threadIdx.x; e < kPackTaxa * kPackWords * 8; e += blockDim.x) {
+      const unsigned r = e / (kPackWords * 8), q = e % (kPackWords * 8);
+      const size_t s = (size_t)w0 * 32 + 4 * q;
+      uint32_t x = 0;
+      if (t0 + r < n) {
+        const uint8_t* row = chars + (size_t)(t0 + r) * pitch;
+#pragma unroll
+        for (unsigned k = 0; k < 4; ++k)
+          if (s + k < L) x |= (uint32_t)row[s + k] << (8 * k);
+      }
+      tile[r * kPackRow + q] = x;
+    }
+    __syncthreads();
+    for (unsigned ww = wave; ww < kPackWords; ww += blockDim.x / 64) {
+      const unsigned w = w0 + ww;
+      if (w >= W_pad) break;
+      uint32_t H = 0, Lo = 0, V = 0;
+#pragma unroll
+      for (unsigned j = 0; j < 8; ++j) {
+        const uint32_t x = tile[lane * kPackRow + ww * 8 + j];
+#pragma unroll
+        for (unsigned k = 0; k < 4; ++k) {
+          uint32_t h, l, v;
+          char_bits((uint8_t)(x >> (8 * k)), h, l, v);
+          H |= h << (4 * j + k);
+          Lo |= l << (4 * j + k);
+          V |= v << (4 * j + k);
+        }
+      }
+      if (t < n_pad) {
+        planes[0 * ps + (size_t)w * n_pad + t] = H;
+        planes[1 * ps + (size_t)w * n_pad + t] = Lo;
+        planes[2 * ps + (size_t)w * n_pad + t] = V;
+      }
+    }
+    __syncthreads();  // the tile is rewritten by the next word block
   }
 }
 
@@ -486,7 +516,8 @@ static unsigned ygrid(unsigned rows) { return rows < 1 ? 1 : (rows > 65535 ? 655
 hipError_t launch_pack_chars(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
                              unsigned n_pad, unsigned W_pad, uint32_t* planes,
                              hipStream_t stream) {
-  pack_chars_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(W_pad)), dim3(256), 0, stream>>>(
+  const unsigned wblocks = (W_pad + kPackWords - 1) / kPackWords;
+  pack_chars_kernel<<<dim3((n_pad + kPackTaxa - 1) / kPackTaxa, ygrid(wblocks)), dim3(256), 0, stream>>>(
       chars, pitch, n, L, n_pad, W_pad, planes);
   return hipGetLastError();
 }

@@ -521,6 +521,8 @@ int rnj_destroy(rnj_context* ctx) {
     delete a;
   }
   ctx->alignments.clear();
+  for (auto& kv : ctx->scratch_free) (void)hipFree(kv.second);
+  ctx->scratch_free.clear();
   for (auto& w : ctx->pool) {
     free_workspace_public(w.get());
     if (w->done) (void)hipEventDestroy(w->done);
@@ -661,26 +663,15 @@ int rnj_alignment_from_fasta(rnj_context* ctx, const char* data, size_t len, int
   if (!ctx || (!data && len) || !out) return fail(RNJ_E_INVALID, "null argument");
   int rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
-  const uint8_t* d = reinterpret_cast<const uint8_t*>(data);
-  uint8_t* staging = nullptr;
   // device input may still be in flight on another stream of the caller's
   if (data_on_device) RNJ_HIP(hipDeviceSynchronize());
   // the parser reads 16-byte words: host input is uploaded, unaligned device input is copied
-  if (!data_on_device || (reinterpret_cast<uintptr_t>(data) & 15u) != 0) {
-    RNJ_HIP(hipMalloc((void**)&staging, len ? len : 1));
-    hipError_t e = hipMemcpyAsync(staging, data, len,
-                                  data_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                  ctx->stream);
-    if (e != hipSuccess) {
-      (void)hipFree(staging);
-      return hip_fail(e, "rnj_alignment_from_fasta upload");
-    }
-    d = staging;
-  }
+  const bool aligned = (reinterpret_cast<uintptr_t>(data) & 15u) == 0;
+  const hipMemcpyKind stage = !data_on_device ? hipMemcpyHostToDevice
+                              : aligned       ? hipMemcpyDefault
+                                              : hipMemcpyDeviceToDevice;
   Alignment* a = nullptr;
-  rc = fasta_to_alignment(ctx, d, len, ctx->stream, &a);
-  (void)hipStreamSynchronize(ctx->stream);
-  if (staging) (void)hipFree(staging);
+  rc = fasta_to_alignment(ctx, reinterpret_cast<const uint8_t*>(data), len, stage, ctx->stream, &a);
   if (rc != RNJ_OK) return rc;
   *out = static_cast<rnj_alignment*>(a);
   return RNJ_OK;
@@ -738,7 +729,7 @@ int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignm
                       : hip_fail(e, "rnj_alignment_from_fasta_file upload");
   }
   Alignment* a = nullptr;
-  rc = fasta_to_alignment(ctx, dev, len, ctx->stream, &a);
+  rc = fasta_to_alignment(ctx, dev, len, hipMemcpyDefault, ctx->stream, &a);
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(dev);
   if (rc != RNJ_OK) return rc;

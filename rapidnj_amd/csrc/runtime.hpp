@@ -64,6 +64,10 @@ struct Context {
   std::map<std::tuple<int, int, int>, int> tuned;
   hipDeviceProp_t prop{};
   std::set<struct Alignment*> alignments;  // live alignments (freed by rnj_destroy if left)
+  // device blocks of finished FASTA parses, by power-of-two size, reused by later parses
+  std::mutex scratch_mu;
+  std::multimap<size_t, void*> scratch_free;
+  size_t scratch_bytes = 0;
 };
 
 struct Alignment {
@@ -98,9 +102,11 @@ int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignm
 // dataloaderFasta.hpp:54-76 on the first sequence: DNA unless a byte >= 'A' is not ACGTUN.
 bool first_row_is_dna(const char* row, unsigned L);
 
-// Parse a FASTA image already in device memory (16-byte aligned) and pack it (fasta.hip).
-int fasta_to_alignment(Context* ctx, const uint8_t* bytes, size_t len, hipStream_t stream,
-                       Alignment** out);
+// Parse a FASTA image and pack it (fasta.hip). `bytes` is in device memory, 16-byte aligned,
+// unless `stage` is hipMemcpyHostToDevice or hipMemcpyDeviceToDevice: then it is first copied
+// into a (cached) device buffer of the parse.
+int fasta_to_alignment(Context* ctx, const uint8_t* bytes, size_t len, hipMemcpyKind stage,
+                       hipStream_t stream, Alignment** out);
 
 // Enqueue a multi-job launch on `stream`: reset statistics, tile kernel, saturation patch
 // (unless raw), max copies. All jobs must share one tile configuration (chosen here). exact:
