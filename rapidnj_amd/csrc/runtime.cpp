@@ -6,6 +6,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <thread>
+#include <unistd.h>
+#include <sys/stat.h>
+#include <fcntl.h>
 #include <cstdlib>
 #include <cstring>
 
@@ -523,6 +527,10 @@ int rnj_destroy(rnj_context* ctx) {
   ctx->alignments.clear();
   for (auto& kv : ctx->scratch_free) (void)hipFree(kv.second);
   ctx->scratch_free.clear();
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->file_pin[k]) (void)hipHostFree(ctx->file_pin[k]);
+    if (ctx->file_ev[k]) (void)hipEventDestroy(ctx->file_ev[k]);
+  }
   for (auto& w : ctx->pool) {
     free_workspace_public(w.get());
     if (w->done) (void)hipEventDestroy(w->done);
@@ -681,48 +689,63 @@ int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignm
   if (!ctx || !path || !out) return fail(RNJ_E_INVALID, "null argument");
   int rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return fail(RNJ_E_INVALID, std::string("cannot read file: ") + path);
-  std::fseek(f, 0, SEEK_END);
-  const long end = std::ftell(f);
-  std::fseek(f, 0, SEEK_SET);
-  if (end < 0) {
-    std::fclose(f);
+  const int fd = ::open(path, O_RDONLY);
+  if (fd < 0) return fail(RNJ_E_INVALID, std::string("cannot read file: ") + path);
+  struct stat st;
+  if (::fstat(fd, &st) != 0) {
+    ::close(fd);
     return fail(RNJ_E_INVALID, std::string("cannot size file: ") + path);
   }
-  const size_t len = (size_t)end;
-  // two pinned chunks: the disk read of one overlaps the upload of the other
-  constexpr size_t kChunkBytes = 32u << 20;
-  uint8_t* dev = nullptr;
-  uint8_t* pin[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  hipError_t e = hipMalloc((void**)&dev, len ? len : 1);
+  const size_t len = (size_t)st.st_size;
+  // Two pinned chunks kept by the context: several threads read one chunk (page-cache copies
+  // scale with threads) while the other uploads.
+  std::lock_guard<std::mutex> file_lock(ctx->file_mu);
+  hipError_t e = hipSuccess;
   for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-    e = hipHostMalloc((void**)&pin[k], kChunkBytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
+    if (!ctx->file_pin[k]) e = hipHostMalloc((void**)&ctx->file_pin[k], kFileChunk, hipHostMallocDefault);
+    if (e == hipSuccess && !ctx->file_ev[k])
+      e = hipEventCreateWithFlags(&ctx->file_ev[k], hipEventDisableTiming);
   }
+  uint8_t* dev = nullptr;
+  if (e == hipSuccess) e = hipMalloc((void**)&dev, len ? len : 1);
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nthreads = std::max(1u, std::min(8u, hw / 2));
   size_t done = 0;
   int k = 0;
   bool short_read = false;
   while (e == hipSuccess && done < len) {
-    e = hipEventSynchronize(ev[k]);  // the upload that last used pin[k] has finished
-    const size_t want = std::min(kChunkBytes, len - done);
-    if (e == hipSuccess && std::fread(pin[k], 1, want, f) != want) {
+    e = hipEventSynchronize(ctx->file_ev[k]);  // the upload that last used this chunk is done
+    if (e != hipSuccess) break;
+    const size_t want = std::min(kFileChunk, len - done);
+    const size_t part = (want + nthreads - 1) / nthreads;
+    std::vector<std::thread> readers;
+    std::atomic<bool> bad{false};
+    for (unsigned t = 0; t < nthreads && t * part < want; ++t) {
+      readers.emplace_back([&, t] {
+        size_t at = t * part;
+        const size_t stop = std::min(want, at + part);
+        while (at < stop) {
+          const ssize_t got = ::pread(fd, ctx->file_pin[k] + at, stop - at, (off_t)(done + at));
+          if (got <= 0) {
+            bad = true;
+            return;
+          }
+          at += (size_t)got;
+        }
+      });
+    }
+    for (auto& r : readers) r.join();
+    if (bad) {
       short_read = true;
       break;
     }
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(dev + done, pin[k], want, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[k], ctx->stream);
+    e = hipMemcpyAsync(dev + done, ctx->file_pin[k], want, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->file_ev[k], ctx->stream);
     done += want;
     k ^= 1;
   }
-  std::fclose(f);
+  ::close(fd);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  for (int j = 0; j < 2; ++j) {
-    if (pin[j]) (void)hipHostFree(pin[j]);
-    if (ev[j]) (void)hipEventDestroy(ev[j]);
-  }
   if (e != hipSuccess || short_read) {
     if (dev) (void)hipFree(dev);
     return short_read ? fail(RNJ_E_INVALID, std::string("short read: ") + path)

@@ -68,7 +68,13 @@ struct Context {
   std::mutex scratch_mu;
   std::multimap<size_t, void*> scratch_free;
   size_t scratch_bytes = 0;
+  // pinned staging of FASTA file reads (allocated on first use, kept)
+  std::mutex file_mu;
+  uint8_t* file_pin[2] = {nullptr, nullptr};
+  hipEvent_t file_ev[2] = {nullptr, nullptr};
 };
+
+constexpr size_t kFileChunk = (size_t)32 << 20;  // bytes per pinned file-read chunk
 
 struct Alignment {
   Context* ctx = nullptr;
