@@ -199,6 +199,34 @@ def ingest_compare(ctx, aln) -> dict:
     return out
 
 
+def handoff_compare(ctx, A) -> dict:
+    """The in-memory NJ hand-off of the benchmark matrix: distances computed in HBM, then
+    post-processed rows in host memory, as full rows (rnj_distances_rows) and as the lower
+    triangle rapidNJMem keeps (rnj_distances_rows_lower)."""
+    import ctypes
+    import numpy as np
+    from rapidnj_amd import native as N
+    n = A.n
+    full = np.empty((n, n), dtype=np.float32)
+    lower = [np.empty(i + 1, dtype=np.float32) for i in range(n)]
+    pf = (ctypes.c_void_p * n)(*[full.ctypes.data + i * full.strides[0] for i in range(n)])
+    pl = (ctypes.c_void_p * n)(*[r.ctypes.data for r in lower])
+    mx = ctypes.c_float()
+    out = {"taxa": n}
+    for name, fn, rows, nbytes in (("full_rows", N.lib().rnj_distances_rows, pf, n * n * 4),
+                                   ("lower_rows", N.lib().rnj_distances_rows_lower, pl,
+                                    n * (n + 1) // 2 * 4)):
+        best = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            N.check(fn(ctx.handle, A.handle, N.K2P, ctypes.cast(rows, ctypes.c_void_p),
+                       ctypes.byref(mx)))
+            best = min(best, time.perf_counter() - t0)
+        out[name + "_ms"] = round(best * 1e3, 1)
+        out[name + "_host_GBps"] = round(nbytes / best / 1e9, 1)
+    return out
+
+
 def _gen_job(job):
     from rapidnj_amd.workloads import simulate_alignment
     return simulate_alignment(job.n, job.L, job.gap, job.seed)
@@ -459,6 +487,8 @@ def main() -> int:
                                          "unit": "G site-pairs/s"}
         PA.close()
         del p_out
+        log("side run: host hand-off of the 20000 x 5000 matrix (full rows, lower triangle)")
+        extra["handoff"] = handoff_compare(ctx, A)
         log("side run: FASTA ingest (device reader) vs the reference reader")
         extra["ingest"] = ingest_compare(ctx, aln)
         log("side run: batched bootstrap vs one launch per replicate")

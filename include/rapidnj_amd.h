@@ -178,12 +178,19 @@ RNJ_EXPORT int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uin
  * (this, rnj_distances_stream_rows and the group-1 entry points) use the exact FP64 logs. */
 RNJ_EXPORT int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
                                   float** rows, float* max_out);
+/* The same hand-off into the rows rapidNJMem keeps: rows[i] holds i + 1 floats, row i of the
+ * post-processed matrix up to and including the diagonal (the lower triangle that
+ * distMatrixReader::initializeData cuts out for halfMatrix, distMatrixReader.cpp:26-35). Half
+ * the bytes of rnj_distances_rows. */
+RNJ_EXPORT int rnj_distances_rows_lower(rnj_context* ctx, const rnj_alignment* aln, int model,
+                                        float** rows, float* max_out);
 
 /* Row streaming for the disk-backed path: the matrix is computed once in HBM, then every row
  * (post-processed, full length n) is passed to `sink` in row order, as
  * KimuraDistance::computeDistanceMatrixMT hands rows to diskMatrix::writeArray
  * (KimuraDistance.cpp:267-281, diskMatrix.cpp:96-101). A non-zero return from the sink stops
- * the stream with RNJ_E_CANCELLED. `values` is valid only during the call. */
+ * the stream with RNJ_E_CANCELLED. `values` is valid only during the call. The sink is called
+ * from one library thread, in row order, never concurrently. */
 typedef int (*rnj_row_sink)(void* user, unsigned row, const float* values, unsigned n);
 RNJ_EXPORT int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
                                          rnj_row_sink sink, void* user, float* max_out);

@@ -92,6 +92,11 @@ class DistanceCalculator {
       : model_(model), verbose_(verbose), fastdist_(fastdist), loader_(loader), matrix_(rows),
         dm_(dm), max_(0.0f) {}
 
+  // Rows the calculator allocates itself become the lower triangle with the diagonal
+  // (row i: i + 1 values), which is all rapidNJMem reads; distMatrixReader::initializeData
+  // with halfMatrix copies exactly those (distMatrixReader.cpp:26-35). Half the transfer.
+  void set_lower_rows(bool lower) { lower_rows_ = lower; }
+
   void compute() {
     if (matrix_ != nullptr && dm_ != nullptr) {
       std::cerr << "ERROR: computation of bootstrap values is not supported for trees which use "
@@ -106,11 +111,17 @@ class DistanceCalculator {
     rnj_alignment* aln = upload_alignment(loader_, fastdist_);
     int rc;
     if (dm_ == nullptr) {
-      if (matrix_ == nullptr) {
+      if (matrix_ == nullptr && lower_rows_) {
         matrix_ = new distType*[n];
-        for (unsigned i = 0; i < n; ++i) matrix_[i] = new distType[n];
+        for (unsigned i = 0; i < n; ++i) matrix_[i] = new distType[i + 1];
+        rc = rnj_distances_rows_lower(shared_context(), aln, model_, matrix_, &max_);
+      } else {
+        if (matrix_ == nullptr) {
+          matrix_ = new distType*[n];
+          for (unsigned i = 0; i < n; ++i) matrix_[i] = new distType[n];
+        }
+        rc = rnj_distances_rows(shared_context(), aln, model_, matrix_, &max_);
       }
-      rc = rnj_distances_rows(shared_context(), aln, model_, matrix_, &max_);
     } else {
       rc = rnj_distances_stream_rows(shared_context(), aln, model_, &write_row_to_disk<DiskMatrix>,
                                      dm_, &max_);
@@ -125,6 +136,7 @@ class DistanceCalculator {
  private:
   int model_;
   bool verbose_, fastdist_;
+  bool lower_rows_ = false;
   Loader* loader_;
   distType** matrix_;
   DiskMatrix* dm_;
@@ -143,6 +155,8 @@ class KimuraDistance {
     impl_.compute();
   }
   void computeDistancesGPU() { impl_.compute(); }
+  // For callers that build rapidNJMem (halfMatrix): rows hold the lower triangle only.
+  void useLowerTriangleRows(bool lower) { impl_.set_lower_rows(lower); }
   distType** getDistanceMatrix() { return impl_.matrix(); }
   distType getMaxDistance() const { return impl_.max_distance(); }
 
@@ -159,6 +173,7 @@ class JCdistance {
     (void)numThreads;
     impl_.compute();
   }
+  void useLowerTriangleRows(bool lower) { impl_.set_lower_rows(lower); }
   distType** getDistanceMatrix() { return impl_.matrix(); }
   distType getMaxDistance() const { return impl_.max_distance(); }
 

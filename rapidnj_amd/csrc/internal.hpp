@@ -84,6 +84,8 @@ hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t s
 // exact: every log in FP64 as the reference writes it; otherwise FP32 logs away from saturation.
 hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
                         unsigned total_items, unsigned finish_items, hipStream_t stream);
+hipError_t launch_lower_pack(const float* sq, size_t ld, unsigned n, float* out,
+                             hipStream_t stream);
 unsigned tile_threads(int config);  // work-group size of a configuration
 
 hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream);

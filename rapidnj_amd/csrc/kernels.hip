@@ -554,6 +554,24 @@ unsigned tile_threads(int config) {
   return (c.tile / c.micro_i) * (c.tile / c.micro_j);
 }
 
+// Row r of the lower triangle with its diagonal (r + 1 values) from the square matrix, rows back
+// to back at r (r + 1) / 2: the rows rapidNJMem keeps (distMatrixReader.cpp:26-35).
+__global__ void lower_pack_kernel(const float* __restrict__ sq, size_t ld, unsigned n,
+                                  float* __restrict__ out) {
+  for (unsigned r = blockIdx.x; r < n; r += gridDim.x) {
+    const float* row = sq + (size_t)r * ld;
+    float* dst = out + (size_t)r * (r + 1) / 2;
+    for (unsigned j = threadIdx.x; j <= r; j += blockDim.x) dst[j] = row[j];
+  }
+}
+
+hipError_t launch_lower_pack(const float* sq, size_t ld, unsigned n, float* out,
+                             hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  lower_pack_kernel<<<dim3(n < 65536 ? n : 65536), dim3(256), 0, stream>>>(sq, ld, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream) {
   if (njobs == 0) return hipSuccess;
   patch_kernel<<<dim3(64, njobs), dim3(256), 0, stream>>>(jobs);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <thread>
+#include <condition_variable>
 #include <unistd.h>
 #include <sys/stat.h>
 #include <fcntl.h>
@@ -528,8 +529,8 @@ int rnj_destroy(rnj_context* ctx) {
   for (auto& kv : ctx->scratch_free) (void)hipFree(kv.second);
   ctx->scratch_free.clear();
   for (int k = 0; k < 2; ++k) {
-    if (ctx->file_pin[k]) (void)hipHostFree(ctx->file_pin[k]);
-    if (ctx->file_ev[k]) (void)hipEventDestroy(ctx->file_ev[k]);
+    if (ctx->pin[k]) (void)hipHostFree(ctx->pin[k]);
+    if (ctx->pin_ev[k]) (void)hipEventDestroy(ctx->pin_ev[k]);
   }
   for (auto& w : ctx->pool) {
     free_workspace_public(w.get());
@@ -699,13 +700,8 @@ int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignm
   const size_t len = (size_t)st.st_size;
   // Two pinned chunks kept by the context: several threads read one chunk (page-cache copies
   // scale with threads) while the other uploads.
-  std::lock_guard<std::mutex> file_lock(ctx->file_mu);
-  hipError_t e = hipSuccess;
-  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-    if (!ctx->file_pin[k]) e = hipHostMalloc((void**)&ctx->file_pin[k], kFileChunk, hipHostMallocDefault);
-    if (e == hipSuccess && !ctx->file_ev[k])
-      e = hipEventCreateWithFlags(&ctx->file_ev[k], hipEventDisableTiming);
-  }
+  std::lock_guard<std::mutex> file_lock(ctx->pin_mu);
+  hipError_t e = ensure_pinned(ctx);
   uint8_t* dev = nullptr;
   if (e == hipSuccess) e = hipMalloc((void**)&dev, len ? len : 1);
   const unsigned hw = std::thread::hardware_concurrency();
@@ -714,9 +710,9 @@ int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignm
   int k = 0;
   bool short_read = false;
   while (e == hipSuccess && done < len) {
-    e = hipEventSynchronize(ctx->file_ev[k]);  // the upload that last used this chunk is done
+    e = hipEventSynchronize(ctx->pin_ev[k]);  // the upload that last used this chunk is done
     if (e != hipSuccess) break;
-    const size_t want = std::min(kFileChunk, len - done);
+    const size_t want = std::min(kPinBytes, len - done);
     const size_t part = (want + nthreads - 1) / nthreads;
     std::vector<std::thread> readers;
     std::atomic<bool> bad{false};
@@ -725,7 +721,7 @@ int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignm
         size_t at = t * part;
         const size_t stop = std::min(want, at + part);
         while (at < stop) {
-          const ssize_t got = ::pread(fd, ctx->file_pin[k] + at, stop - at, (off_t)(done + at));
+          const ssize_t got = ::pread(fd, ctx->pin[k] + at, stop - at, (off_t)(done + at));
           if (got <= 0) {
             bad = true;
             return;
@@ -739,8 +735,8 @@ int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignm
       short_read = true;
       break;
     }
-    e = hipMemcpyAsync(dev + done, ctx->file_pin[k], want, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ctx->file_ev[k], ctx->stream);
+    e = hipMemcpyAsync(dev + done, ctx->pin[k], want, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->pin_ev[k], ctx->stream);
     done += want;
     k ^= 1;
   }
@@ -825,66 +821,134 @@ int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, 
 }  // extern "C"
 
 // ---- host hand-off ------------------------------------------------------------------------------
-// Compute the square matrix in HBM once, then stream row slabs through two pinned buffers: the
-// D2H of one slab overlaps the host-side delivery of the previous one.
+namespace rnj {
+hipError_t ensure_pinned(Context* ctx) {
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    if (!ctx->pin[k]) e = hipHostMalloc((void**)&ctx->pin[k], kPinBytes, hipHostMallocDefault);
+    if (e == hipSuccess && !ctx->pin_ev[k])
+      e = hipEventCreateWithFlags(&ctx->pin_ev[k], hipEventDisableTiming);
+  }
+  return e;
+}
+}  // namespace rnj
+
+// The post-processed square matrix is computed in HBM once. The rows handed over (full rows, or
+// the lower triangle with the diagonal that rapidNJMem keeps, distMatrixReader.cpp:26-35) lie
+// back to back on the device (the lower triangle packed by launch_lower_pack), and go to the
+// host in slabs of whole rows through the context's two pinned buffers: the copy of slab k+1
+// overlaps the delivery of slab k. Delivery runs on `threads` library threads, each taking every
+// threads-th row of a slab (copies from pinned into the caller's pageable rows are the slow
+// part); with one thread the rows arrive in order.
+namespace {
+struct RowShape {
+  bool lower;
+  unsigned n;
+  size_t off(unsigned r) const { return lower ? (size_t)r * (r + 1) / 2 : (size_t)r * n; }
+  size_t len(unsigned r) const { return lower ? (size_t)r + 1 : (size_t)n; }
+};
+}  // namespace
+
 template <class Deliver>
 static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, float* max_out,
-                        Deliver deliver) {
+                        bool lower, unsigned threads, Deliver deliver) {
   int rc = check_aln(ctx, aln);
   if (rc == RNJ_OK) rc = check_model(model);
   if (rc != RNJ_OK) return rc;
   rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
   const unsigned n = aln->n;
+  const RowShape shape{lower, n};
+  const size_t cap = kPinBytes / sizeof(float);
+  if ((size_t)n > cap) return fail(RNJ_E_INVALID, "rows longer than the pinned staging buffer");
   hipStream_t s = ctx->stream;
-  float* dmat = nullptr;
-  float* dmax = nullptr;
+  float *dmat = nullptr, *dlow = nullptr, *dmax = nullptr;
   RNJ_HIP(hipMallocAsync((void**)&dmat, (size_t)n * n * sizeof(float), s));
   RNJ_HIP(hipMallocAsync((void**)&dmax, sizeof(float), s));
   std::vector<JobSpec> jobs{spec_for(aln, model, kSquare, dmat, n)};
   jobs[0].max_out = dmax;
   rc = launch_jobs(ctx, jobs, false, true, s);
-  const size_t slab_rows = std::max<size_t>(1, ((size_t)32 << 20) / ((size_t)n * sizeof(float)));
-  float* pinned[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
   hipError_t e = hipSuccess;
-  for (int b = 0; rc == RNJ_OK && b < 2 && e == hipSuccess; ++b) {
-    e = hipHostMalloc((void**)&pinned[b], slab_rows * n * sizeof(float), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming);
+  const float* src = dmat;
+  if (rc == RNJ_OK && lower) {
+    e = hipMallocAsync((void**)&dlow, shape.off(n) * sizeof(float), s);
+    if (e == hipSuccess) e = launch_lower_pack(dmat, n, n, dlow, s);
+    src = dlow;
   }
-  size_t prev_rows = 0, prev_first = 0;
-  int prev_buf = -1, buf = 0;
-  auto flush = [&](int b, size_t first, size_t cnt) -> int {
-    hipError_t w = hipEventSynchronize(ev[b]);
-    if (w != hipSuccess) return hip_fail(w, "hipEventSynchronize");
-    for (size_t q = 0; q < cnt; ++q) {
-      const int r = deliver((unsigned)(first + q), pinned[b] + q * n);
-      if (r != RNJ_OK) return r;
+  // slabs of whole rows
+  std::vector<std::pair<unsigned, unsigned>> slabs;
+  for (unsigned r0 = 0, r = 0; r <= n; ++r) {
+    if (r == n || shape.off(r + 1) - shape.off(r0) > cap) {
+      if (r > r0 || r == n) slabs.emplace_back(r0, r == n ? n : r);
+      if (r == n) break;
+      r0 = r;
     }
-    return RNJ_OK;
+  }
+  std::lock_guard<std::mutex> pin_lock(ctx->pin_mu);
+  if (rc == RNJ_OK && e == hipSuccess) e = ensure_pinned(ctx);
+  std::mutex m;
+  std::condition_variable cv;
+  long long issued = -1;
+  std::vector<unsigned> consumed(slabs.size(), 0);
+  bool stop = rc != RNJ_OK || e != hipSuccess;
+  int deliver_rc = RNJ_OK;
+  const unsigned T = std::max(1u, threads);
+  auto worker = [&](unsigned t) {
+    for (size_t k = 0; k < slabs.size(); ++k) {
+      {
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return issued >= (long long)k || stop; });
+        if (stop) return;
+      }
+      int wrc = RNJ_OK;
+      if (hipEventSynchronize(ctx->pin_ev[k & 1]) != hipSuccess)
+        wrc = fail(RNJ_E_HIP, "row hand-off copy failed");
+      const float* base = reinterpret_cast<const float*>(ctx->pin[k & 1]);
+      const size_t off0 = shape.off(slabs[k].first);
+      for (unsigned r = slabs[k].first + t; wrc == RNJ_OK && r < slabs[k].second; r += T)
+        wrc = deliver(r, base + (shape.off(r) - off0), shape.len(r));
+      std::lock_guard<std::mutex> lk(m);
+      if (wrc != RNJ_OK) {
+        if (deliver_rc == RNJ_OK) deliver_rc = wrc;
+        stop = true;
+      }
+      ++consumed[k];
+      cv.notify_all();
+      if (stop) return;
+    }
   };
-  for (size_t r = 0; rc == RNJ_OK && e == hipSuccess && r < n; r += slab_rows, buf ^= 1) {
-    const size_t cnt = std::min(slab_rows, (size_t)n - r);
-    e = hipMemcpyAsync(pinned[buf], dmat + r * n, cnt * n * sizeof(float), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipEventRecord(ev[buf], s);
-    if (e == hipSuccess && prev_buf >= 0) rc = flush(prev_buf, prev_first, prev_rows);
-    prev_buf = buf;
-    prev_rows = cnt;
-    prev_first = r;
+  std::vector<std::thread> pool;
+  if (!stop)
+    for (unsigned t = 0; t < T; ++t) pool.emplace_back(worker, t);
+  for (size_t k = 0; !stop && k < slabs.size(); ++k) {
+    if (k >= 2) {
+      std::unique_lock<std::mutex> lk(m);
+      cv.wait(lk, [&] { return consumed[k - 2] == T || stop; });
+      if (stop) break;
+    }
+    const size_t first = shape.off(slabs[k].first), cnt = shape.off(slabs[k].second) - first;
+    e = hipMemcpyAsync(ctx->pin[k & 1], src + first, cnt * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(ctx->pin_ev[k & 1], s);
+    std::lock_guard<std::mutex> lk(m);
+    if (e != hipSuccess) stop = true;
+    else issued = (long long)k;
+    cv.notify_all();
   }
-  if (rc == RNJ_OK && e == hipSuccess && prev_buf >= 0) rc = flush(prev_buf, prev_first, prev_rows);
-  if (rc == RNJ_OK && e == hipSuccess && max_out)
-    e = hipMemcpy(max_out, dmax, sizeof(float), hipMemcpyDeviceToHost);
+  for (auto& th : pool) th.join();
+  if (rc == RNJ_OK && e == hipSuccess && deliver_rc == RNJ_OK && max_out)
+    e = hipMemcpyAsync(max_out, dmax, sizeof(float), hipMemcpyDeviceToHost, s);
   (void)hipStreamSynchronize(s);
-  for (int b = 0; b < 2; ++b) {
-    if (pinned[b]) (void)hipHostFree(pinned[b]);
-    if (ev[b]) (void)hipEventDestroy(ev[b]);
-  }
   (void)hipFreeAsync(dmat, s);
+  if (dlow) (void)hipFreeAsync(dlow, s);
   (void)hipFreeAsync(dmax, s);
   if (rc != RNJ_OK) return rc;
+  if (deliver_rc != RNJ_OK) return deliver_rc;
   if (e != hipSuccess) return hip_fail(e, "row hand-off");
   return RNJ_OK;
+}
+
+static unsigned copy_threads() {
+  return std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
 }
 
 extern "C" {
@@ -894,21 +958,34 @@ int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model, fl
   if (!aln || !rows) return fail(RNJ_E_INVALID, "null argument");
   for (unsigned i = 0; i < aln->n; ++i)
     if (!rows[i]) return fail(RNJ_E_INVALID, "null row");
-  const unsigned n = aln->n;
-  return rows_handoff(ctx, aln, model, max_out, [rows, n](unsigned row, const float* v) {
-    std::memcpy(rows[row], v, (size_t)n * sizeof(float));
-    return (int)RNJ_OK;
-  });
+  return rows_handoff(ctx, aln, model, max_out, false, copy_threads(),
+                      [rows](unsigned row, const float* v, size_t len) {
+                        std::memcpy(rows[row], v, len * sizeof(float));
+                        return (int)RNJ_OK;
+                      });
+}
+
+int rnj_distances_rows_lower(rnj_context* ctx, const rnj_alignment* aln, int model, float** rows,
+                             float* max_out) {
+  if (!aln || !rows) return fail(RNJ_E_INVALID, "null argument");
+  for (unsigned i = 0; i < aln->n; ++i)
+    if (!rows[i]) return fail(RNJ_E_INVALID, "null row");
+  return rows_handoff(ctx, aln, model, max_out, true, copy_threads(),
+                      [rows](unsigned row, const float* v, size_t len) {
+                        std::memcpy(rows[row], v, len * sizeof(float));
+                        return (int)RNJ_OK;
+                      });
 }
 
 int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
                               rnj_row_sink sink, void* user, float* max_out) {
   if (!aln || !sink) return fail(RNJ_E_INVALID, "null argument");
-  const unsigned n = aln->n;
-  return rows_handoff(ctx, aln, model, max_out, [sink, user, n](unsigned row, const float* v) {
-    if (sink(user, row, v, n) != 0) return fail(RNJ_E_CANCELLED, "row sink stopped the stream");
-    return (int)RNJ_OK;
-  });
+  return rows_handoff(ctx, aln, model, max_out, false, 1,
+                      [sink, user](unsigned row, const float* v, size_t len) {
+                        if (sink(user, row, v, (unsigned)len) != 0)
+                          return fail(RNJ_E_CANCELLED, "row sink stopped the stream");
+                        return (int)RNJ_OK;
+                      });
 }
 
 // ---- bootstrap ------------------------------------------------------------------------------------

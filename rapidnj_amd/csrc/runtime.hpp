@@ -68,13 +68,16 @@ struct Context {
   std::mutex scratch_mu;
   std::multimap<size_t, void*> scratch_free;
   size_t scratch_bytes = 0;
-  // pinned staging of FASTA file reads (allocated on first use, kept)
-  std::mutex file_mu;
-  uint8_t* file_pin[2] = {nullptr, nullptr};
-  hipEvent_t file_ev[2] = {nullptr, nullptr};
+  // two pinned staging buffers (FASTA file reads, row hand-offs), allocated on first use, kept
+  std::mutex pin_mu;
+  uint8_t* pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
 
-constexpr size_t kFileChunk = (size_t)32 << 20;  // bytes per pinned file-read chunk
+constexpr size_t kPinBytes = (size_t)32 << 20;  // bytes per pinned staging buffer
+
+// Allocate the context's pinned staging buffers if needed (caller holds pin_mu).
+hipError_t ensure_pinned(Context* ctx);
 
 struct Alignment {
   Context* ctx = nullptr;

@@ -63,6 +63,7 @@ SIGNATURES = {
                                             ctypes.c_int, _vp, _vp]),
     "rnj_counts_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rnj_distances_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
+    "rnj_distances_rows_lower": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "rnj_distances_stream_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
     "rnj_bootstrap_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _u, ctypes.c_int, _vp,
                                             _sz, _sz, ctypes.c_int, _vp]),
@@ -342,6 +343,16 @@ class Context:
         check(lib().rnj_distances_rows(self.handle, aln.handle, _model(model),
                                        ctypes.cast(rows, _vp), ctypes.byref(mx)))
         return mat, mx.value
+
+    def distance_rows_lower(self, aln: Alignment, model="k2p") -> tuple[list[np.ndarray], float]:
+        """Lower-triangle hand-off: row i holds the i + 1 values up to the diagonal."""
+        n = aln.n
+        out = [np.empty(i + 1, dtype=np.float32) for i in range(n)]
+        rows = (_vp * n)(*[r.ctypes.data for r in out])
+        mx = ctypes.c_float()
+        check(lib().rnj_distances_rows_lower(self.handle, aln.handle, _model(model),
+                                             ctypes.cast(rows, _vp), ctypes.byref(mx)))
+        return out, mx.value
 
     def bootstrap(self, aln: Alignment, weights: np.ndarray, model="k2p",
                   layout: str = "triangle", out=None, raw: bool = False, stream=None,

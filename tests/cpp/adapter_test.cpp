@@ -75,7 +75,7 @@ static bool read_input(const char* path, MemoryLoader& ld) {
 
 int main(int argc, char** argv) {
   if (argc < 6) {
-    std::cerr << "usage: adapter_test IN OUT kim|jc mem|disk|reuse fast|chars" << std::endl;
+    std::cerr << "usage: adapter_test IN OUT kim|jc mem|disk|reuse|half fast|chars" << std::endl;
     return 2;
   }
   MemoryLoader ld;
@@ -114,11 +114,13 @@ int main(int argc, char** argv) {
       }
     }
     distType** m = nullptr;
+    const bool half = mode == "half";  // lower-triangle rows, as rapidNJMem keeps them
     if (model == "kim") {
       auto* alg = rows ? new rapidnj_amd::KimuraDistance<MemoryLoader, MemoryDiskMatrix>(
                              false, fast, &ld, rows, nullptr)
                        : new rapidnj_amd::KimuraDistance<MemoryLoader, MemoryDiskMatrix>(
                              false, fast, &ld, nullptr);
+      alg->useLowerTriangleRows(half);
       alg->computeDistances(8);
       m = alg->getDistanceMatrix();
       maxd = alg->getMaxDistance();
@@ -126,6 +128,7 @@ int main(int argc, char** argv) {
     } else {
       auto* alg = new rapidnj_amd::JCdistance<MemoryLoader, MemoryDiskMatrix>(false, fast, &ld,
                                                                               nullptr);
+      alg->useLowerTriangleRows(half);
       alg->computeDistanceMatrix(8);
       m = alg->getDistanceMatrix();
       maxd = alg->getMaxDistance();
@@ -136,9 +139,13 @@ int main(int argc, char** argv) {
       return 3;
     }
     for (unsigned i = 0; i < n; ++i) {
-      std::memcpy(&out[(size_t)i * n], m[i], n * sizeof(distType));
-      delete[] m[i];  // the NJ builders own and delete the rows (rapidNJ.cpp:432-437)
+      if (half) {  // rapidNJMem reads matrix[max(i,j)][min(i,j)] (rapidNJMem.hpp:62-68)
+        for (unsigned j = 0; j < n; ++j) out[(size_t)i * n + j] = j <= i ? m[i][j] : m[j][i];
+      } else {
+        std::memcpy(&out[(size_t)i * n], m[i], n * sizeof(distType));
+      }
     }
+    for (unsigned i = 0; i < n; ++i) delete[] m[i];  // the NJ builders own the rows (rapidNJ.cpp:432-437)
     delete[] m;
   }
   std::ofstream o(argv[2], std::ios::binary);

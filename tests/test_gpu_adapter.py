@@ -44,7 +44,7 @@ def _run(tmp_path, d, model, mode, source):
 
 @pytest.mark.parametrize("case", ["lgt70.npz", "c1.npz", "saturated.npz", "edge_L129.npz"])
 @pytest.mark.parametrize("model", ["kim", "jc"])
-@pytest.mark.parametrize("mode", ["mem", "disk", "reuse"])
+@pytest.mark.parametrize("mode", ["mem", "disk", "reuse", "half"])
 def test_host_classes_match_reference(tmp_path, case, model, mode):
     if not os.path.exists(DRIVER):
         pytest.fail(f"{DRIVER} missing: run build()")

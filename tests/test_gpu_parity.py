@@ -125,6 +125,23 @@ def test_host_rows_handoff(ctx):
         mat, mx = ctx.distance_rows(ctx.alignment(d["aln"]), model)
         assert_distances_match(mat, d[model])
         assert np.array_equal(mat, mat.T)
+        low, mx2 = ctx.distance_rows_lower(ctx.alignment(d["aln"]), model)
+        assert mx2 == mx and [len(r) for r in low] == list(range(1, mat.shape[0] + 1))
+        for i, r in enumerate(low):
+            np.testing.assert_array_equal(r.view(np.uint32), mat[i, : i + 1].view(np.uint32))
+
+
+def test_host_rows_handoff_many_slabs(ctx):
+    """5000 taxa: the square rows span several pinned slabs, the lower rows two; both equal the
+    device square matrix bit for bit."""
+    A = ctx.alignment(simulate_alignment(5000, 700, 0.05, 19))
+    sq = _np(ctx.distances(A, "k2p", layout="square", exact=True))  # the hand-offs use exact logs
+    mat, mx = ctx.distance_rows(A, "k2p")
+    np.testing.assert_array_equal(mat.view(np.uint32), sq.view(np.uint32))
+    low, _ = ctx.distance_rows_lower(A, "k2p")
+    for i in (0, 1, 2047, 2048, 3333, 4999):
+        np.testing.assert_array_equal(low[i].view(np.uint32), sq[i, : i + 1].view(np.uint32))
+    assert all(len(low[i]) == i + 1 for i in range(5000))
 
 
 @pytest.mark.parametrize("path", EDGE[:6] + [os.path.join(GOLDEN, "c1.npz")], ids=os.path.basename)
