@@ -227,6 +227,24 @@ def handoff_compare(ctx, A) -> dict:
     return out
 
 
+def side_main(args) -> int:
+    """--side: one side measurement alone (for profiling it)."""
+    from rapidnj_amd import native as N
+    from rapidnj_amd.workloads import WORKLOADS, simulate_alignment
+    wl = WORKLOADS[args.workload]
+    aln = simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    ctx = N.Context(0)
+    if args.side == "ingest":
+        res = ingest_compare(ctx, aln)
+    else:
+        A = ctx.alignment(aln)
+        res = handoff_compare(ctx, A)
+        A.close()
+    print(json.dumps({"side": args.side, "workload": wl.name, **res}), flush=True)
+    ctx.close()
+    return 0
+
+
 def _gen_job(job):
     from rapidnj_amd.workloads import simulate_alignment
     return simulate_alignment(job.n, job.L, job.gap, job.seed)
@@ -364,9 +382,13 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM bytes per launch")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--side", choices=["ingest", "handoff"],
+                    help="run only this side measurement on the workload and print its JSON")
     args = ap.parse_args()
     if args.probe_traffic:
         return probe_traffic(args)
+    if args.side:
+        return side_main(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per "
               "box); run with --gpus 1", file=sys.stderr)
