@@ -59,6 +59,16 @@ def lib() -> ctypes.CDLL:
         L.rnj_oracle_distance_matrix.argtypes = [_u32p, _u32p, ctypes.c_uint, ctypes.c_uint,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.rnj_oracle_count_protein.restype = None
+        L.rnj_oracle_count_protein.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint,
+                                               _u64_3]
+        for f in (L.rnj_oracle_kimura_protein, L.rnj_oracle_jc_protein):
+            f.restype = ctypes.c_float
+            f.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.rnj_oracle_protein_matrix.restype = ctypes.c_int
+        L.rnj_oracle_protein_matrix.argtypes = [ctypes.c_char_p, ctypes.c_uint, ctypes.c_uint,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.POINTER(ctypes.c_float)]
         L.rnj_oracle_parse_fasta.restype = ctypes.c_int
         L.rnj_oracle_parse_fasta.argtypes = [ctypes.c_char_p, ctypes.c_size_t] + \
             [ctypes.c_void_p] * 9
@@ -185,6 +195,47 @@ def triangle_to_full(tri: np.ndarray, n: int, maxd: float | None = None) -> np.n
     full[iu] = t
     full[(iu[1], iu[0])] = t
     return full
+
+
+def count_protein(a: bytes, b: bytes) -> tuple[int, int, int]:
+    out = _u64_3()
+    lib().rnj_oracle_count_protein(a, b, len(a), out)
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def kimura_protein(dist: int, valid: int) -> float:
+    return lib().rnj_oracle_kimura_protein(dist, valid)
+
+
+def jc_protein(dist: int, valid: int) -> float:
+    return lib().rnj_oracle_jc_protein(dist, valid)
+
+
+def protein_counts_triangle(aln: np.ndarray):
+    """(dist, valid) upper triangles (uint64) of an (n, L) protein character alignment."""
+    aln = np.ascontiguousarray(aln, dtype=np.uint8)
+    n = aln.shape[0]
+    rows = [aln[i].tobytes() for i in range(n)]
+    dist, valid = [], []
+    for i in range(n):
+        for j in range(i + 1, n):
+            d, _, v = count_protein(rows[i], rows[j])
+            dist.append(d)
+            valid.append(v)
+    return np.array(dist, dtype=np.uint64), np.array(valid, dtype=np.uint64)
+
+
+def protein_matrix(aln: np.ndarray, model: str):
+    """(full post-processed n x n matrix, max) for an (n, L) protein character alignment."""
+    aln = np.ascontiguousarray(aln, dtype=np.uint8)
+    n, L = aln.shape
+    full = np.zeros((n, n), dtype=np.float32)
+    mx = ctypes.c_float()
+    rc = lib().rnj_oracle_protein_matrix(aln.tobytes(), n, L, model_id(model), 1,
+                                         full.ctypes.data, ctypes.byref(mx))
+    if rc != 0:
+        raise RuntimeError("oracle protein matrix failed")
+    return full, mx.value
 
 
 class FastaError(ValueError):

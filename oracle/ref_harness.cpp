@@ -6,6 +6,7 @@
 // Entry points it drives, all in the reference tree:
 //   dataloaderFasta::readData / getBitStrings / getGapFilters   dataloaderFasta.hpp:24-99,219-251
 //   bitDistanceGap::computeDistance                             bitDistanceGap.cpp:60-98
+//   bitDistanceProtein::computeDistance                         bitDistanceProtein.cpp:31-61
 //   simpleDistanceCalculator::computeDistance (the -f path)     simpleDistanceCalculator.cpp:8-16
 //   KimuraDistance::computeDistances / getDistanceMatrix        KimuraDistance.cpp:75-99,397-399
 //   JCdistance::computeDistanceMatrix / getDistanceMatrix       JCdistance.cpp:24-44,232-234
@@ -18,6 +19,9 @@
 //                                     every '>' record in file order); then bit strings and gap
 //                                     filters as in `pack` (the reference exits 1 on bad input)
 //   load   FASTA REPS                 prints {"best_s":..} of parse + pack (readData, fastdist)
+//   pcounts FASTA OUT                 protein (bitDistanceProtein): u32 n, L; then for i<j
+//                                     u64 data[0], data[1], data[2] as the estimator returns them
+//   pmatrix FASTA jc|kim OUT          protein distances: u32 n; then n*n f32 (post-processed)
 //   counts FASTA OUT [naive]          u32 n, L; then for i<j (row-major) u32 ts, tv, valid
 //   matrix FASTA jc|kim T OUT [naive] u32 n; then n*n f32 (post-processed matrix); prints seconds
 //   time   FASTA jc|kim T REPS        prints {"best_s":..,"mean_s":..,"n":..,"L":..,"threads":..}
@@ -28,6 +32,7 @@
 #include "stdinclude.h"
 #include "dataloaderFasta.hpp"
 #include "bitDistanceGap.hpp"
+#include "bitDistanceProtein.hpp"
 #include "simpleDistanceCalculator.hpp"
 #include "KimuraDistance.hpp"
 #include "JCdistance.hpp"
@@ -140,6 +145,43 @@ int mode_load(int argc, char** argv) {
   return 0;
 }
 
+dataloaderFasta* load_protein(const std::string& path) {
+  dataloaderFasta* dl = new dataloaderFasta();
+  dl->readData(path, true, false, UNKNOWN, false);
+  if (dl->type != PROTEIN) die("input is not detected as protein");
+  return dl;
+}
+
+int mode_pcounts(int argc, char** argv) {
+  if (argc < 4) die("pcounts FASTA OUT");
+  dataloaderFasta* dl = load_protein(argv[2]);
+  bitDistanceProtein est(dl);
+  uint32_t n = dl->getSequenceCount(), L = dl->getSequenceLength();
+  std::ofstream o = open_out(argv[3]);
+  put_u32(o, n); put_u32(o, L);
+  unsigned long long d[3];
+  for (uint32_t i = 0; i < n; i++)
+    for (uint32_t j = i + 1; j < n; j++) {
+      est.computeDistance(i, j, d);
+      o.write(reinterpret_cast<const char*>(d), sizeof d);
+    }
+  delete dl;
+  return 0;
+}
+
+int mode_pmatrix(int argc, char** argv) {
+  if (argc < 5) die("pmatrix FASTA jc|kim OUT");
+  dataloaderFasta* dl = load_protein(argv[2]);
+  int n = dl->getSequenceCount();
+  distType** m = compute_matrix(dl, argv[3], 4, true, NULL);
+  std::ofstream o = open_out(argv[4]);
+  put_u32(o, (uint32_t)n);
+  for (int i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(m[i]), sizeof(distType) * n);
+  free_matrix(m, n);
+  delete dl;
+  return 0;
+}
+
 int mode_counts(int argc, char** argv) {
   if (argc < 4) die("counts FASTA OUT [naive]");
   bool naive = argc > 4 && std::string(argv[4]) == "naive";
@@ -246,6 +288,8 @@ int main(int argc, char** argv) {
   if (mode == "pack") return mode_pack(argc, argv);
   if (mode == "fasta") return mode_fasta(argc, argv);
   if (mode == "load") return mode_load(argc, argv);
+  if (mode == "pcounts") return mode_pcounts(argc, argv);
+  if (mode == "pmatrix") return mode_pmatrix(argc, argv);
   if (mode == "counts") return mode_counts(argc, argv);
   if (mode == "matrix") return mode_matrix(argc, argv);
   if (mode == "time") return mode_time(argc, argv);

@@ -418,3 +418,93 @@ int rnj_oracle_parse_fasta(const unsigned char* data, size_t len, unsigned* n, u
   *name_bytes = nb;
   return nseq < 2 ? 2 : 0;
 }
+
+/* ---- protein (dataloaderFasta.hpp:184-199, bitDistanceProtein.cpp:31-119, KimuraDistance.cpp
+ * :30-49,322-338, JCdistance.cpp:155-183) --------------------------------------------------- */
+char rnj_oracle_protein_resolve(char c) {
+  switch (c) {
+    case '-': case '.': case 'X': case 'x': case 'z': case 'Z': case 'b': case 'B':
+    case 'J': case 'j': case '?':
+      return '-';
+    default:
+      return c;
+  }
+}
+
+void rnj_oracle_count_protein(const char* a, const char* b, unsigned L, uint64_t out[3]) {
+  const uint64_t walked = 128ull * (L / 128 + 1);     /* (bitStringsCount / 8) * 8 * 16 chars */
+  const uint64_t padded = ((uint64_t)L + 127) & ~127ull;
+  uint64_t dist = 0, gaps = walked - L;                /* padding is '-' */
+  for (unsigned k = 0; k < L; k++) {
+    const char x = rnj_oracle_protein_resolve(a[k]), y = rnj_oracle_protein_resolve(b[k]);
+    if (x == '-' || y == '-') gaps++;
+    else if (x != y) dist++;
+  }
+  out[0] = dist;
+  out[1] = 0;
+  out[2] = padded - gaps;
+}
+
+/* the Dayhoff PAM distances (x100) for p = 0.750 .. 0.930, KimuraDistance.cpp:30-49 */
+static const int kDayhoffPams[] = {
+  195, 196, 197, 198, 199, 200, 200, 201, 202, 203, 204, 205, 206, 207, 208, 209, 209, 210,
+  211, 212, 213, 214, 215, 216, 217, 218, 219, 220, 221, 222, 223, 224, 226, 227, 228, 229,
+  230, 231, 232, 233, 234, 236, 237, 238, 239, 240, 241, 243, 244, 245, 246, 248, 249, 250,
+  252, 253, 254, 255, 257, 258, 260, 261, 262, 264, 265, 267, 268, 270, 271, 273, 274, 276,
+  277, 279, 281, 282, 284, 285, 287, 289, 291, 292, 294, 296, 298, 299, 301, 303, 305, 307,
+  309, 311, 313, 315, 317, 319, 321, 323, 325, 328, 330, 332, 335, 337, 339, 342, 344, 347,
+  349, 352, 354, 357, 360, 362, 365, 368, 371, 374, 377, 380, 383, 386, 389, 393, 396, 399,
+  403, 407, 410, 414, 418, 422, 426, 430, 434, 438, 442, 447, 451, 456, 461, 466, 471, 476,
+  482, 487, 493, 498, 504, 511, 517, 524, 531, 538, 545, 553, 560, 569, 577, 586, 595, 605,
+  615, 626, 637, 649, 661, 675, 688, 703, 719, 736, 754, 775, 796, 819, 845, 874, 907, 945, 988};
+
+float rnj_oracle_kimura_protein(uint64_t dist, uint64_t valid) {
+  float distance = -1.0f;
+  const long total = (long)dist;
+  if (valid != 0) distance = total / (float)valid;
+  if (distance < 0.75) {
+    if (distance > 0.0) distance = (float)(-log(1.0 - distance - (distance * distance * 0.20)));
+  } else {
+    if (distance > 0.930) {
+      distance = 10.0f;
+    } else {
+      const int idx = (int)((distance * 1000.0) - 750.0);
+      distance = (float)(kDayhoffPams[idx] / 100.0);
+    }
+  }
+  return distance;
+}
+
+float rnj_oracle_jc_protein(uint64_t dist, uint64_t valid) {
+  const double alpha = 20.0;
+  const long total = (long)dist;
+  float distance;
+  if (valid == 0) distance = 0;
+  else distance = total / (float)valid;
+  if (distance / ((alpha - 1) / alpha) >= 1.0) distance = -1;
+  else distance = (float)(-((alpha - 1) / alpha) * log(1.0 - distance / ((alpha - 1) / alpha)));
+  return distance;
+}
+
+int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model, int threads,
+                              float* full, float* max_out) {
+  (void)threads;
+  const size_t npairs = (size_t)n * (n - 1) / 2;
+  float* t = (float*)malloc((npairs ? npairs : 1) * sizeof(float));
+  if (!t) return -1;
+  float maxd = 0.0f;
+  size_t k = 0;
+  for (unsigned i = 0; i < n; i++)
+    for (unsigned j = i + 1; j < n; j++, k++) {
+      uint64_t c[3];
+      rnj_oracle_count_protein(aln + (size_t)i * L, aln + (size_t)j * L, L, c);
+      const float d = model == RNJ_ORACLE_K2P ? rnj_oracle_kimura_protein(c[0], c[2])
+                                              : rnj_oracle_jc_protein(c[0], c[2]);
+      if (d > maxd) maxd = d;
+      t[k] = d;
+    }
+  post_process(t, n, maxd, full);
+  if (max_out) *max_out = maxd;
+  free(t);
+  return 0;
+}

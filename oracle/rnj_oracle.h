@@ -40,6 +40,22 @@ int rnj_oracle_parse_fasta(const unsigned char* data, size_t len, unsigned* n, u
                            unsigned* H, size_t* name_bytes, unsigned* bad, unsigned* bad_len,
                            unsigned char* rows, unsigned char* names, unsigned* name_len);
 
+/* Protein (PROTEIN type): resolveChar (dataloaderFasta.hpp:184-199) maps - . X x z Z b B J j ?
+ * to '-', and the byte compare of bitDistanceProtein::computeDistance (bitDistanceProtein.cpp
+ * :31-119) over the resolved residues: out[0] = positions where both are residues and differ,
+ * out[1] = 0, out[2] = roundup(L, 128) - gap positions among the 128 * (L / 128 + 1) positions
+ * the estimator walks (padding counts as gap), in unsigned 64-bit arithmetic. */
+char rnj_oracle_protein_resolve(char c);
+void rnj_oracle_count_protein(const char* a, const char* b, unsigned L, uint64_t out[3]);
+/* Kimura protein with the Dayhoff PAM table (KimuraDistance.cpp:30-49,322-338) and JC with
+ * alpha = 20 (JCdistance.cpp:155-183) on those counts. */
+float rnj_oracle_kimura_protein(uint64_t dist, uint64_t valid);
+float rnj_oracle_jc_protein(uint64_t dist, uint64_t valid);
+/* Post-processed n x n protein matrix (model RNJ_ORACLE_K2P = Kimura protein, RNJ_ORACLE_JC) of
+ * an (n, L) character alignment (raw characters; resolved here). */
+int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model, int threads,
+                              float* full, float* max_out);
+
 unsigned rnj_oracle_bitstrings_count(unsigned L);
 
 /* 1 if the first sequence classifies the alignment as DNA (dataloaderFasta.hpp:54-76). */

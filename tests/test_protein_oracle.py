@@ -1,0 +1,74 @@
+"""The protein oracle (oracle/rnj_oracle.c) against the reference's protein engine.
+
+tests/golden/protein.npz holds, per case, what the reference's bitDistanceProtein returned for
+every pair (data[0..2]) and its post-processed Kimura-protein and JC (alpha = 20) matrices
+(tests/golden/make_protein_golden.py). The restatement must give the same u64 counts, including
+the estimator's valid count that is 128 short (and wraps) when L is a multiple of 128, and the
+same float matrices.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return dict(np.load(os.path.join(GOLDEN, "protein.npz"), allow_pickle=False))
+
+
+def cases(g):
+    return sorted({k.rsplit("_", 1)[0] for k in g if k.endswith("_aln")})
+
+
+def test_cases_present(golden):
+    assert set(CASES) <= set(cases(golden))
+
+
+CASES = ["pf15224", "p12x1", "p20x128", "p30x256", "p25x300", "p16x640", "p10x200gap"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_counts_and_matrices(golden, case):
+    aln = golden[f"{case}_aln"]
+    d0, d2 = O.protein_counts_triangle(aln)
+    np.testing.assert_array_equal(d0, golden[f"{case}_d0"])
+    assert not golden[f"{case}_d1"].any()
+    np.testing.assert_array_equal(d2, golden[f"{case}_d2"])
+    for model, key in (("k2p", "kim"), ("jc", "jc")):
+        full, _ = O.protein_matrix(aln, model)
+        np.testing.assert_array_equal(full.view(np.uint32), golden[f"{case}_{key}"].view(np.uint32))
+
+
+def test_regimes_covered(golden):
+    """The fixtures reach every branch: the Dayhoff table, the 10.0 cap, saturated and empty
+    pairs, and the wrapped valid count."""
+    kim = np.concatenate([golden[f"{c}_kim"].ravel() for c in cases(golden)])
+    assert np.any(kim == np.float32(10.0))
+    table = (kim >= 1.95) & (kim <= 9.88)
+    assert np.any(table)
+    d2 = np.concatenate([golden[f"{c}_d2"] for c in cases(golden)])
+    assert np.any(d2 > np.uint64(1) << np.uint64(63))        # 128 short, wrapped
+    assert np.any(d2 == 0)
+
+
+def test_dayhoff_boundaries():
+    """p just below 0.75 uses the formula, 0.75 the first table entry, 0.93 the last, above it 10."""
+    assert O.kimura_protein(0, 10) == 0.0
+    assert O.kimura_protein(749, 1000) == pytest.approx(-np.log(1 - 0.749 - 0.749 ** 2 * 0.2), rel=1e-6)
+    assert O.kimura_protein(750, 1000) == np.float32(1.95)
+    # 930/1000 in float is just above 0.930: capped. Below it the table is nondecreasing, from
+    # 1.95 PAM/100 at p = 0.75; the formula just below 0.75 lands near the same value.
+    assert O.kimura_protein(930, 1000) == np.float32(10.0)
+    table = [O.kimura_protein(k, 1000) for k in range(750, 930)]
+    assert all(a <= b for a, b in zip(table, table[1:]))
+    assert 9.0 < table[-1] < 10.0
+    assert 1.9 < O.kimura_protein(749, 1000) < 2.0
+    assert O.kimura_protein(5, 0) == -1.0
+    # 95/100 in float is just below 0.95: finite; above it saturated
+    assert O.jc_protein(95, 100) > 17.0 and O.jc_protein(96, 100) == -1.0
+    assert O.jc_protein(0, 0) == 0.0
