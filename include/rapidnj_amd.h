@@ -1,4 +1,5 @@
-/* rapidnj_amd: MI355X (gfx950) pairwise evolutionary-distance engine for DNA alignments.
+/* rapidnj_amd: MI355X (gfx950) pairwise evolutionary-distance engine for DNA and protein
+ * alignments.
  *
  * C ABI of librapidnj_amd.so. Plain pointers and sizes only. Two groups of entry points:
  *
@@ -16,6 +17,11 @@
  * taxa hold one of ACGTacgt). Distances follow the reference's CPU recipes in FP64
  * (KimuraDistance.cpp:339-348, JCdistance.cpp:170-183); saturated pairs carry -1 until the
  * reference post-process replaces them by 2 * max (KimuraDistance.cpp:366-395).
+ * Protein alignments (type discovered as the reference's readers do, or set with
+ * rnj_set_input_type) count mismatched residues and the estimator's valid sites
+ * (bitDistanceProtein.cpp:31-61); RNJ_K2P is then the Kimura protein distance with the Dayhoff
+ * table (KimuraDistance.cpp:322-338) and RNJ_JC the Jukes-Cantor distance with alpha = 20
+ * (JCdistance.cpp:155-158).
  */
 #ifndef RAPIDNJ_AMD_H
 #define RAPIDNJ_AMD_H
@@ -41,7 +47,7 @@ enum {
   RNJ_E_INVALID = 1,    /* bad argument (null pointer, n < 2, unknown model, ...) */
   RNJ_E_HIP = 2,        /* a HIP runtime call failed */
   RNJ_E_NOMEM = 3,      /* device or pinned host allocation failed */
-  RNJ_E_NOT_DNA = 4,    /* first sequence is not DNA (dataloaderFasta.hpp:54-76) */
+  RNJ_E_NOT_DNA = 4,    /* a DNA-only operation (bootstrap, served jobs) on protein input */
   RNJ_E_ADMISSION = 5,  /* a served job can never fit the server's HBM budget */
   RNJ_E_CANCELLED = 6,  /* a served job was cancelled */
   RNJ_E_NOT_READY = 7,  /* a served job has not finished (rnj_server_poll) */
@@ -50,6 +56,9 @@ enum {
 
 /* models: rnj_*_distances */
 enum { RNJ_JC = 0, RNJ_K2P = 1 };
+
+/* alignment types (the reference's InputType and its -i d|p option, src/main.cpp:397-399) */
+enum { RNJ_TYPE_AUTO = 0, RNJ_TYPE_DNA = 1, RNJ_TYPE_PROTEIN = 2 };
 
 /* output layouts for device distance matrices */
 enum {
@@ -113,12 +122,20 @@ RNJ_EXPORT const char* rnj_tile_config_name(int config);
 RNJ_EXPORT int rnj_set_tile_config(rnj_context* ctx, int config);
 RNJ_EXPORT int rnj_get_tile_config(rnj_context* ctx, unsigned n, unsigned L, int model, int* config);
 
-/* Alignments packed once into HBM as H/L/V bit planes.
+/* Type of the alignments this context reads from characters, rows and FASTA from now on:
+ * RNJ_TYPE_AUTO (default) classifies the first sequence as the reference's readers do
+ * (dataloaderFasta.hpp:54-76: DNA unless a byte >= 'A' is not one of ACGTUN of either case, else
+ * protein); RNJ_TYPE_DNA / RNJ_TYPE_PROTEIN force the type, like the reference's `-i d|p`. */
+RNJ_EXPORT int rnj_set_input_type(rnj_context* ctx, int type);
+
+/* Alignments packed once into HBM as bit planes.
  * from_chars: n rows of L bytes, row t at chars + t*pitch, on the host (chars_on_device = 0) or
- *   already in device memory (chars_on_device = 1). Character rules of the reference's FASTA
- *   packer: ACGT of either case are counted, everything else ('-', N, U, IUPAC, '.', '?') is not
- *   (dataloaderFasta.hpp:147-182). The first row must classify as DNA (dataloaderFasta.hpp:54-76)
- *   unless `skip_type_check`; n >= 2 and L >= 1 (dataloaderFasta.hpp:32-35).
+ *   already in device memory (chars_on_device = 1). DNA: the character rules of the reference's
+ *   FASTA packer, ACGT of either case are counted, everything else ('-', N, U, IUPAC, '.', '?')
+ *   is not (dataloaderFasta.hpp:147-182). Protein: every byte is a residue except the gap
+ *   characters - . X x z Z b B J j ? (dataloader::resolveChar, as bitDistanceProtein uses it).
+ *   The type is the context's input type; `skip_type_check` reads the rows as DNA regardless.
+ *   n >= 2 and L >= 1 (dataloaderFasta.hpp:32-35).
  * from_rows: one pointer per taxon, like dataloader::getSequences (dataloader.hpp:26).
  * from_bitstrings: the reference's packed strings, per taxon bitStringsCount x 128 bits of codes
  *   and gap filters, like dataloader::getBitStrings / getGapFilters (dataloader.hpp:20-24,
@@ -140,7 +157,7 @@ RNJ_EXPORT int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned in
  * 32, sequence lines keep bytes 33..127, empty records are not stored but keep their name, data
  * before the first header forms a nameless record. Errors as the reference's exits: fewer than
  * 2 sequences, or a sequence whose length differs from the first (RNJ_E_INVALID, message names
- * the sequence), first sequence not DNA (RNJ_E_NOT_DNA). Synchronous. */
+ * the sequence). The type as for from_chars. Synchronous. */
 RNJ_EXPORT int rnj_alignment_from_fasta(rnj_context* ctx, const char* data, size_t len,
                                         int data_on_device, rnj_alignment** out);
 /* The same from a file: read through pinned buffers and uploaded while reading
@@ -154,7 +171,11 @@ RNJ_EXPORT int rnj_alignment_free(rnj_alignment* aln);
 /* n taxa, L sites, W 32-site words counted per taxon, bytes of HBM held. */
 RNJ_EXPORT int rnj_alignment_info(const rnj_alignment* aln, unsigned* n, unsigned* L, unsigned* W,
                        size_t* bytes);
-/* Copy the packed planes [3][W][n] (H, L, V; site s of word w at bit s%32) to the host. */
+/* RNJ_TYPE_DNA or RNJ_TYPE_PROTEIN, and the number of bit planes: 3 for DNA (H, L, V), 1 + B for
+ * protein (gap bits, then the B bits of each site's residue code; B = 5, 6 or 8 for up to 31,
+ * 63 or 255 distinct residue bytes, which are numbered 1.. in byte order, gaps 0). */
+RNJ_EXPORT int rnj_alignment_type(const rnj_alignment* aln, int* type, unsigned* planes);
+/* Copy the packed planes [planes][W][n] (site s of word w at bit s%32) to the host. */
 RNJ_EXPORT int rnj_alignment_planes(const rnj_alignment* aln, uint32_t* host_planes);
 
 /* One distance matrix into device memory `out` (layout RNJ_TRIANGLE or RNJ_SQUARE with row
@@ -167,7 +188,9 @@ RNJ_EXPORT int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, 
                          float* out, size_t ld, int flags, float* max_out, void* stream);
 
 /* Exact ts / tv / valid of every pair into device arrays of n(n-1)/2 u32, upper triangle
- * row-major (DistanceEstimate::computeDistance, DistanceEstimate.hpp:14). */
+ * row-major (DistanceEstimate::computeDistance, DistanceEstimate.hpp:14). Protein: ts holds the
+ * mismatches (data[0]), tv 0 (data[1]) and valid the low 32 bits of the 64-bit data[2], which
+ * is 128 short of the residue pairs when L is a multiple of 128 (bitDistanceProtein.cpp:31-61). */
 RNJ_EXPORT int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, uint32_t* tv,
                       uint32_t* valid, void* stream);
 
@@ -195,7 +218,8 @@ typedef int (*rnj_row_sink)(void* user, unsigned row, const float* values, unsig
 RNJ_EXPORT int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
                                          rnj_row_sink sink, void* user, float* max_out);
 
-/* Batched bootstrap: R replicates given as column-weight vectors weights[r*L + c] (host u16;
+/* Batched bootstrap (DNA alignments; RNJ_E_NOT_DNA for protein): R replicates given as
+ * column-weight vectors weights[r*L + c] (host u16;
  * replicate r repeats column c weights[r][c] times, the expanded alignment of
  * dataloader::sample_fast_dist, dataloader.cpp:16-58). All replicates run in one tile launch.
  * out holds R matrices, matrix r at out + r*matrix_stride floats (layout as above). */
@@ -239,7 +263,8 @@ typedef struct {
 } rnj_job_stats;
 
 RNJ_EXPORT int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server** out);
-/* chars: n rows of L bytes (host), row t at chars + t*pitch; copied before returning. */
+/* chars: n rows of L bytes (host), row t at chars + t*pitch; copied before returning. DNA only:
+ * rows whose type (rnj_set_input_type of the server's context) is protein give RNJ_E_NOT_DNA. */
 RNJ_EXPORT int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned n, unsigned L,
                       int model, uint64_t* ticket);
 /* Blocks until the job ends; tri_out (host, n(n-1)/2 floats, may be null) receives the

@@ -30,6 +30,31 @@ using gptr = const __attribute__((address_space(1))) T*;
 // Native 4 x u32 vector (ext_vector_type) for 128-bit global and LDS accesses.
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
+// Chunk `chunk` of the i and j taxon blocks into registers: linear element lin = q + Q*(w + CW*p)
+// of a block is word row (chunk*CW + w) of plane p, uint4 column q (E = planes * CW * Q).
+template <int EP, int NT, int Q, int CW, int E>
+__device__ __forceinline__ void fetch_chunk(v4u (&pI)[EP], v4u (&pJ)[EP], gptr<v4u> gI,
+                                            gptr<v4u> gJ, unsigned tid, unsigned chunk,
+                                            size_t row4, size_t plane4) {
+#pragma unroll
+  for (int e = 0; e < EP; ++e) {
+    const unsigned lin = tid + e * NT;
+    if (E % NT != 0 && lin >= (unsigned)E) break;
+    const unsigned q = lin % Q, w = (lin / Q) % CW, p = lin / (Q * CW);
+    const size_t off = p * plane4 + (size_t)(chunk * CW + w) * row4 + q;
+    pI[e] = gI[off];
+    pJ[e] = gJ[off];
+  }
+}
+
+// Partial counts of one (tile, split): [3][NT][MI*MJ] u32, thread-major so that every thread
+// stores and loads its own contiguous 4*MI*MJ bytes.
+template <int NT, int MI, int MJ>
+__device__ __forceinline__ uint32_t* partial_slot(const JobDesc& J, unsigned tile,
+                                                  unsigned split) {
+  return J.partial + ((size_t)tile * J.splits + split) * 3 * NT * MI * MJ;
+}
+
 // ---- corrections ---------------------------------------------------------------------------
 // Kimura 2-parameter (KimuraDistance.cpp:339-348) and Jukes-Cantor with alpha = 4
 // (JCdistance.cpp:170-183). What decides the result's class is kept exactly as the reference

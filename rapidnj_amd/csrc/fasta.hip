@@ -618,15 +618,14 @@ int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind
   std::vector<char> row0(L);
   FA_HIP(hipMemcpyAsync(row0.data(), chars, L, hipMemcpyDeviceToHost, s));
   FA_HIP(hipStreamSynchronize(s));
-  if (!first_row_is_dna(row0.data(), (unsigned)L))
-    return fail(RNJ_E_NOT_DNA, "first sequence is not DNA; only DNA distances are supported");
+  const int kind = alignment_kind(ctx, row0.data(), (unsigned)L);
 
   // 6. pack + names
   Alignment* a = nullptr;
-  int rc = alloc_alignment(ctx, nseq, (unsigned)L, (unsigned)L, &a);
+  int rc = pack_alignment(ctx, chars, L, nseq, (unsigned)L, kind, s, &a);
   if (rc != RNJ_OK) return rc;
-  e = launch_pack_chars(chars, L, nseq, (unsigned)L, a->n_pad, a->W, a->planes, s);
-  if (e == hipSuccess && H > 0) {
+  e = hipSuccess;
+  if (H > 0) {
     unsigned long long *spans = nullptr, *nlen = nullptr, *noff = nullptr, *ntot = nullptr;
     uint8_t* blob = nullptr;
     e = t.get(&spans, 2ull * H);

@@ -10,6 +10,9 @@
 // contributes nothing to any count. For a taxon pair (i, j) and one word:
 //   g = Vi & Vj, tv = (Hi ^ Hj) & g, ts = (Li ^ Lj) & (g & ~tv)
 // which is the reference's per-word logic (bitDistanceGap.cpp:138-161) on split planes.
+//
+// Protein alignments use 1 + B planes instead (protein.hip): planes[0] the gap bits, planes[1..B]
+// the bits of each site's residue code, plus a per-taxon count of gap sites.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,6 +28,7 @@ constexpr unsigned kBlockThreads = 256;
 constexpr uint32_t kSentinelBits = 0xBF800000u;  // -1.0f, the reference's saturation marker
 
 enum Model : int { kJC = 0, kK2P = 1, kCounts = 2 };
+enum Kind : int { kDNA = 0, kProtein = 1 };
 enum Layout : int { kTriangle = 0, kSquare = 1 };
 
 // One distance matrix inside a launch. A launch covers jobs[0..njobs); the work items of job k
@@ -33,7 +37,7 @@ enum Layout : int { kTriangle = 0, kSquare = 1 };
 // + split. A split job leaves exact partial counts in `partial` and the finish kernel, numbering
 // that job's tiles from fin_base, sums them and applies the epilogue.
 struct JobDesc {
-  const uint32_t* planes;     // [3][W_pad][n_pad]
+  const uint32_t* planes;     // [3][W_pad][n_pad] (DNA) or [1 + B][W_pad][n_pad] (protein)
   unsigned long long plane_stride;  // W_pad * n_pad
   unsigned n, n_pad, W;       // W: words to process (multiple of kWordPad, <= W_pad)
   unsigned nblk;              // ceil(n / TI) for the launch's tile edge (TI == TJ)
@@ -54,6 +58,8 @@ struct JobDesc {
   unsigned long long* sent;   // (i << 32 | j) of saturated pairs
   unsigned sent_cap;
   unsigned* sent_count;
+  const uint32_t* gaps;       // protein: gap sites per taxon [n_pad]
+  unsigned L;                 // protein: alignment length (sites)
 };
 
 // Upper-triangle position of pair (i, j), i < j: row i holds j = i+1 .. n-1 (row-major).
@@ -84,6 +90,18 @@ hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t s
 // exact: every log in FP64 as the reference writes it; otherwise FP32 logs away from saturation.
 hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
                         unsigned total_items, unsigned finish_items, hipStream_t stream);
+// Protein (protein.hip). census: sets bit v of present[8] for every byte value v in the n x L
+// characters (present must start at zero). pack: characters -> gap plane + B code planes using
+// codes[256] (code of each byte value, 0 for gaps), gaps[t] += gap sites of taxon t (gaps must
+// start at zero). B is 5, 6 or 8.
+hipError_t launch_symbol_census(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                                uint32_t* present, hipStream_t stream);
+hipError_t launch_pack_protein(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                               unsigned n_pad, unsigned W_pad, const uint8_t* codes, unsigned B,
+                               uint32_t* planes, uint32_t* gaps, hipStream_t stream);
+hipError_t launch_protein_tiles(int config, unsigned B, bool exact, const JobDesc* jobs,
+                                unsigned njobs, unsigned total_items, unsigned finish_items,
+                                hipStream_t stream);
 hipError_t launch_lower_pack(const float* sq, size_t ld, unsigned n, float* out,
                              hipStream_t stream);
 unsigned tile_threads(int config);  // work-group size of a configuration

@@ -21,23 +21,6 @@ const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 
 namespace {
 
-// Chunk `chunk` of the i and j taxon blocks into registers: linear element lin = q + Q*(w + CW*p)
-// of a block is word row (chunk*CW + w) of plane p, uint4 column q.
-template <int EP, int NT, int Q, int CW, int E>
-__device__ __forceinline__ void fetch_chunk(v4u (&pI)[EP], v4u (&pJ)[EP], gptr<v4u> gI,
-                                            gptr<v4u> gJ, unsigned tid, unsigned chunk,
-                                            size_t row4, size_t plane4) {
-#pragma unroll
-  for (int e = 0; e < EP; ++e) {
-    const unsigned lin = tid + e * NT;
-    if (E % NT != 0 && lin >= (unsigned)E) break;
-    const unsigned q = lin % Q, w = (lin / Q) % CW, p = lin / (Q * CW);
-    const size_t off = p * plane4 + (size_t)(chunk * CW + w) * row4 + q;
-    pI[e] = gI[off];
-    pJ[e] = gJ[off];
-  }
-}
-
 // ---- epilogue shared by the tile kernel and the split-finish kernel ------------------------
 // Thread (ty, tx) of tile (bi, bj) holds the counts of its MI x MJ pairs: a = tv (mismatches for
 // the JC-only loop), b = ts, v = valid. Writes distances (or counts), records saturated pairs and
@@ -94,14 +77,6 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
     }
   }
   fold_max(J, lmax, tid);
-}
-
-// Partial counts of one (tile, split): [3][NT][MI*MJ] u32, thread-major so that every thread
-// stores and loads its own contiguous 4*MI*MJ bytes.
-template <int NT, int MI, int MJ>
-__device__ __forceinline__ uint32_t* partial_slot(const JobDesc& J, unsigned tile,
-                                                  unsigned split) {
-  return J.partial + ((size_t)tile * J.splits + split) * 3 * NT * MI * MJ;
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------

@@ -89,6 +89,8 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
                               size_t ld, int flags, rnj_plan** plan) {
   if (!ctx || !aln || !out || !plan) return fail(RNJ_E_INVALID, "null argument");
   if (aln->ctx != ctx) return fail(RNJ_E_INVALID, "alignment belongs to another context");
+  if (aln->kind != kDNA)
+    return fail(RNJ_E_NOT_DNA, "bootstrap replicates are built for DNA alignments only");
   if (model != RNJ_JC && model != RNJ_K2P) return fail(RNJ_E_INVALID, "model must be RNJ_JC or RNJ_K2P");
   if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");
   if (R == 0) return fail(RNJ_E_INVALID, "a plan needs at least one replicate");

@@ -1,0 +1,441 @@
+// HIP kernels for protein alignments on gfx950: the residue census, packing into code bit
+// planes, and the pair-tile kernel with the fused Kimura-protein / Jukes-Cantor (alpha = 20)
+// correction.
+//
+// The reference compares residues byte by byte after mapping the gap characters
+// - . X x z Z b B J j ? to '-' (bitDistanceProtein.cpp:31-61 over dataloader::resolveChar):
+// a site is a mismatch when neither character is a gap and they differ, and the estimator's
+// third count is the sites where neither is a gap. Here every byte value present in the
+// alignment gets a code (gaps 0, the other values 1..S in byte order), stored bit-sliced:
+//   planes[0][w][t]      gap bits: 1 where site 32w + b of taxon t is a gap
+//   planes[1 + p][w][t]  bit p of the site's code, p < B (B = 5, 6 or 8 for S < 32, 64, 256)
+// plus gaps[t], the taxon's gap sites. Padding (sites at or past L, taxa at or past n) is code 0
+// without the gap bit. For a pair and 32 sites the kernel counts
+//   x = OR_p (ci_p ^ cj_p)   sites whose codes differ (mismatches, and one-sided gaps)
+//   g = gi & gj              sites where both are gaps
+// in B + 3 VALU instructions (v_xor, B - 1 v_bitop3, v_and, 2 v_bcnt), and the epilogue recovers
+//   mismatches = #x - (gaps_i + gaps_j - 2 #g),  residue pairs = L - gaps_i - gaps_j + #g.
+// There is no matrix-core work: the counting is bitwise and integer.
+#include "device_common.hpp"
+
+#include <algorithm>
+
+namespace rnj {
+namespace {
+
+// The Dayhoff PAM distances for p = 0.750, 0.751, .., 0.930 as the reference tabulates them
+// (KimuraDistance.cpp:30-49, integers in PAM/100 units), converted as the reference converts
+// them, (float)(pam / 100.0). The values have to be these to give the reference's distances.
+#define RNJ_PAM(x) (float)((x) / 100.0)
+__constant__ float kDayhoff[181] = {
+    RNJ_PAM(195), RNJ_PAM(196), RNJ_PAM(197), RNJ_PAM(198), RNJ_PAM(199), RNJ_PAM(200),
+    RNJ_PAM(200), RNJ_PAM(201), RNJ_PAM(202), RNJ_PAM(203), RNJ_PAM(204), RNJ_PAM(205),
+    RNJ_PAM(206), RNJ_PAM(207), RNJ_PAM(208), RNJ_PAM(209), RNJ_PAM(209), RNJ_PAM(210),
+    RNJ_PAM(211), RNJ_PAM(212), RNJ_PAM(213), RNJ_PAM(214), RNJ_PAM(215), RNJ_PAM(216),
+    RNJ_PAM(217), RNJ_PAM(218), RNJ_PAM(219), RNJ_PAM(220), RNJ_PAM(221), RNJ_PAM(222),
+    RNJ_PAM(223), RNJ_PAM(224), RNJ_PAM(226), RNJ_PAM(227), RNJ_PAM(228), RNJ_PAM(229),
+    RNJ_PAM(230), RNJ_PAM(231), RNJ_PAM(232), RNJ_PAM(233), RNJ_PAM(234), RNJ_PAM(236),
+    RNJ_PAM(237), RNJ_PAM(238), RNJ_PAM(239), RNJ_PAM(240), RNJ_PAM(241), RNJ_PAM(243),
+    RNJ_PAM(244), RNJ_PAM(245), RNJ_PAM(246), RNJ_PAM(248), RNJ_PAM(249), RNJ_PAM(250),
+    RNJ_PAM(252), RNJ_PAM(253), RNJ_PAM(254), RNJ_PAM(255), RNJ_PAM(257), RNJ_PAM(258),
+    RNJ_PAM(260), RNJ_PAM(261), RNJ_PAM(262), RNJ_PAM(264), RNJ_PAM(265), RNJ_PAM(267),
+    RNJ_PAM(268), RNJ_PAM(270), RNJ_PAM(271), RNJ_PAM(273), RNJ_PAM(274), RNJ_PAM(276),
+    RNJ_PAM(277), RNJ_PAM(279), RNJ_PAM(281), RNJ_PAM(282), RNJ_PAM(284), RNJ_PAM(285),
+    RNJ_PAM(287), RNJ_PAM(289), RNJ_PAM(291), RNJ_PAM(292), RNJ_PAM(294), RNJ_PAM(296),
+    RNJ_PAM(298), RNJ_PAM(299), RNJ_PAM(301), RNJ_PAM(303), RNJ_PAM(305), RNJ_PAM(307),
+    RNJ_PAM(309), RNJ_PAM(311), RNJ_PAM(313), RNJ_PAM(315), RNJ_PAM(317), RNJ_PAM(319),
+    RNJ_PAM(321), RNJ_PAM(323), RNJ_PAM(325), RNJ_PAM(328), RNJ_PAM(330), RNJ_PAM(332),
+    RNJ_PAM(335), RNJ_PAM(337), RNJ_PAM(339), RNJ_PAM(342), RNJ_PAM(344), RNJ_PAM(347),
+    RNJ_PAM(349), RNJ_PAM(352), RNJ_PAM(354), RNJ_PAM(357), RNJ_PAM(360), RNJ_PAM(362),
+    RNJ_PAM(365), RNJ_PAM(368), RNJ_PAM(371), RNJ_PAM(374), RNJ_PAM(377), RNJ_PAM(380),
+    RNJ_PAM(383), RNJ_PAM(386), RNJ_PAM(389), RNJ_PAM(393), RNJ_PAM(396), RNJ_PAM(399),
+    RNJ_PAM(403), RNJ_PAM(407), RNJ_PAM(410), RNJ_PAM(414), RNJ_PAM(418), RNJ_PAM(422),
+    RNJ_PAM(426), RNJ_PAM(430), RNJ_PAM(434), RNJ_PAM(438), RNJ_PAM(442), RNJ_PAM(447),
+    RNJ_PAM(451), RNJ_PAM(456), RNJ_PAM(461), RNJ_PAM(466), RNJ_PAM(471), RNJ_PAM(476),
+    RNJ_PAM(482), RNJ_PAM(487), RNJ_PAM(493), RNJ_PAM(498), RNJ_PAM(504), RNJ_PAM(511),
+    RNJ_PAM(517), RNJ_PAM(524), RNJ_PAM(531), RNJ_PAM(538), RNJ_PAM(545), RNJ_PAM(553),
+    RNJ_PAM(560), RNJ_PAM(569), RNJ_PAM(577), RNJ_PAM(586), RNJ_PAM(595), RNJ_PAM(605),
+    RNJ_PAM(615), RNJ_PAM(626), RNJ_PAM(637), RNJ_PAM(649), RNJ_PAM(661), RNJ_PAM(675),
+    RNJ_PAM(688), RNJ_PAM(703), RNJ_PAM(719), RNJ_PAM(736), RNJ_PAM(754), RNJ_PAM(775),
+    RNJ_PAM(796), RNJ_PAM(819), RNJ_PAM(845), RNJ_PAM(874), RNJ_PAM(907), RNJ_PAM(945),
+    RNJ_PAM(988)};
+#undef RNJ_PAM
+
+// ---- corrections ---------------------------------------------------------------------------
+// mism and valid are the reference's data[0] (data[1] is 0 for protein) and data[2], valid being
+// its 64-bit count. Proportions are float quotients of the float-converted counts, as the
+// reference's `total / (distType) data[2]` computes them. kExact takes the library log; the
+// default log_unit rounds to the same float except in rare ties (see log_unit).
+#pragma clang fp contract(off)
+// KimuraDistance.cpp:322-338: -1 when valid is 0, -log(1 - p - 0.2 p^2) below 0.75 (p * p in
+// float, the rest in double), the Dayhoff table up to 0.930 and 10 above.
+template <bool kExact>
+__device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid) {
+  const float p = valid != 0 ? (float)mism / (float)valid : -1.0f;
+  const bool formula = p < 0.75f && p > 0.0f;
+  const float pp = p * p;
+  const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
+  const double lg = kExact ? log(arg) : log_unit(arg);
+  int idx = (int)((double)p * 1000.0 - 750.0);
+  idx = idx < 0 ? 0 : (idx > 180 ? 180 : idx);
+  const float table = (double)p > 0.930 ? 10.0f : kDayhoff[idx];
+  return p < 0.75f ? (formula ? (float)(-lg) : p) : table;
+}
+
+// JCdistance.cpp:170-183 with alpha = 20: p = 0 when valid is 0, -1 when p / 0.95 >= 1.
+template <bool kExact>
+__device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long long valid) {
+  const double b = 19.0 / 20.0;  // (alpha - 1) / alpha
+  const float p = valid != 0 ? (float)mism / (float)valid : 0.0f;
+  const double q = (double)p / b;
+  const bool sat = q >= 1.0;
+  const double arg = sat ? 1.0 : 1.0 - q;
+  const double lg = kExact ? log(arg) : log_unit(arg);
+  return sat ? -1.0f : (float)(-b * lg);
+}
+#pragma clang fp contract(on)
+
+// ---- epilogue ------------------------------------------------------------------------------
+// Thread (ty, tx) of tile (bi, bj) holds x (differing sites) and g (gap-gap sites) of its
+// MI x MJ pairs. The reference walks 128 * (L/128 + 1) positions and counts the padding past L
+// as gaps, then subtracts from L rounded up to 128 (bitDistanceProtein.cpp:31-61): its valid
+// count is the residue pairs, less 128 when L is a multiple of 128 (modulo 2^64).
+template <int TILE, int MI, int MJ, bool kExact>
+__device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned bi, unsigned bj,
+                                                 unsigned ty, unsigned tx, unsigned tid,
+                                                 const uint32_t (&acc_x)[MI][MJ],
+                                                 const uint32_t (&acc_g)[MI][MJ]) {
+  const unsigned n = J.n;
+  const unsigned long long L = J.L, bias = (J.L % 128u == 0) ? 128ull : 0ull;
+  uint32_t gi[MI], gj[MJ];
+#pragma unroll
+  for (int a = 0; a < MI; ++a) gi[a] = J.gaps[bi * TILE + ty * MI + a];
+#pragma unroll
+  for (int b = 0; b < MJ; ++b) gj[b] = J.gaps[bj * TILE + tx * MJ + b];
+  float lmax = 0.0f;
+#pragma unroll
+  for (int a = 0; a < MI; ++a) {
+    const unsigned i = bi * TILE + ty * MI + a;
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) {
+      const unsigned j = bj * TILE + tx * MJ + b;
+      if (j >= n || i > j) continue;
+      if (i == j) {
+        finish_diagonal(J, i);
+        continue;
+      }
+      const unsigned long long gsum = (unsigned long long)gi[a] + gj[b];
+      const uint32_t mism = (uint32_t)(acc_x[a][b] + 2ull * acc_g[a][b] - gsum);
+      const unsigned long long valid = L - gsum + acc_g[a][b] - bias;
+      if (J.model == kCounts) {
+        const unsigned long long p = tri_index(i, j, n);
+        J.out_ts[p] = mism;
+        J.out_tv[p] = 0;
+        J.out_valid[p] = (uint32_t)valid;
+        continue;
+      }
+      const float d = J.model == kJC ? jc_protein_value<kExact>(mism, valid)
+                                     : kimura_protein_value<kExact>(mism, valid);
+      if (d == -1.0f) push_sentinel(J, i, j);
+      else lmax = fmaxf(lmax, d);
+      if (J.layout == kSquare) {
+        J.out[(size_t)i * J.ld + j] = d;
+        J.out[(size_t)j * J.ld + i] = d;
+      } else {
+        J.out[tri_index(i, j, n)] = d;
+      }
+    }
+  }
+  if (J.model != kCounts) fold_max(J, lmax, tid);
+}
+
+// ---- the pair-tile kernel ------------------------------------------------------------------
+// The DNA tile kernel's schedule (kernels.hip) over 1 + B planes: one work-group per TILE x TILE
+// block of pairs and range of words, chunks of CW words of both blocks staged in LDS as
+// [plane][word][taxon], the next chunk prefetched into registers while this one is counted.
+template <int TILE, int MI, int MJ, int CW, int B, bool kExact>
+__global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel(
+    const JobDesc* __restrict__ jobs, unsigned njobs) {
+  constexpr int P = 1 + B;
+  constexpr int NTX = TILE / MJ;
+  constexpr int NT = (TILE / MI) * NTX;
+  constexpr int Q = TILE / 4;
+  constexpr int E = P * CW * Q;
+  static_assert(MI % 4 == 0 && MJ % 4 == 0, "micro-tiles are read as uint4");
+  constexpr int EP = (E + NT - 1) / NT;
+
+  __shared__ v4u sI[E];
+  __shared__ v4u sJ[E];
+
+  const unsigned item = xcd_item(blockIdx.x, gridDim.x);
+  const unsigned k = find_job(jobs, njobs, item);
+  const JobDesc& J = jobs[k];
+  const unsigned local = item - J.item_base;
+  const unsigned tile = local / J.splits, split = local % J.splits;
+  unsigned bi, bj;
+  patch_coords(tile, J.nblk, bi, bj);
+
+  const unsigned tid = threadIdx.x;
+  const unsigned ty = tid / NTX, tx = tid % NTX;
+  const gptr<v4u> gI = (gptr<v4u>)(J.planes + (size_t)bi * TILE);
+  const gptr<v4u> gJ = (gptr<v4u>)(J.planes + (size_t)bj * TILE);
+  const size_t row4 = J.n_pad / 4;
+  const size_t plane4 = J.plane_stride / 4;
+
+  uint32_t acc_x[MI][MJ], acc_g[MI][MJ];
+#pragma unroll
+  for (int a = 0; a < MI; ++a)
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) acc_x[a][b] = acc_g[a][b] = 0;
+
+  const unsigned w0 = split * J.words_per_split;
+  const unsigned w1 = min(J.W, w0 + J.words_per_split);
+  const unsigned c0 = w0 / CW, c1 = w1 / CW;
+
+  v4u pI[EP], pJ[EP];
+  if (c0 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c0, row4, plane4);
+  for (unsigned c = c0; c < c1; ++c) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EP; ++e) {
+      if (E % NT != 0 && tid + e * NT >= (unsigned)E) break;
+      sI[tid + e * NT] = pI[e];
+      sJ[tid + e * NT] = pJ[e];
+    }
+    __syncthreads();
+    if (c + 1 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
+
+#pragma unroll 2
+    for (int w = 0; w < CW; ++w) {
+      uint32_t ci[P][MI], cj[P][MJ];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+#pragma unroll
+        for (int r = 0; r < MI / 4; ++r) {
+          const v4u v = sI[(p * CW + w) * Q + ty * (MI / 4) + r];
+          ci[p][4 * r] = v.x; ci[p][4 * r + 1] = v.y; ci[p][4 * r + 2] = v.z; ci[p][4 * r + 3] = v.w;
+        }
+#pragma unroll
+        for (int r = 0; r < MJ / 4; ++r) {
+          const v4u v = sJ[(p * CW + w) * Q + tx * (MJ / 4) + r];
+          cj[p][4 * r] = v.x; cj[p][4 * r + 1] = v.y; cj[p][4 * r + 2] = v.z; cj[p][4 * r + 3] = v.w;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < MI; ++a)
+#pragma unroll
+        for (int b = 0; b < MJ; ++b) {
+          uint32_t x = ci[1][a] ^ cj[1][b];
+#pragma unroll
+          for (int p = 2; p < P; ++p)  // x | (ci ^ cj)
+            x = __builtin_amdgcn_bitop3_b32(x, ci[p][a], cj[p][b], 0xF6u);
+          acc_x[a][b] = bcnt_add(x, acc_x[a][b]);
+          acc_g[a][b] = bcnt_add(ci[0][a] & cj[0][b], acc_g[a][b]);
+        }
+    }
+  }
+
+  if (J.splits > 1) {
+    uint32_t* slot = partial_slot<NT, MI, MJ>(J, tile, split) + (size_t)tid * MI * MJ;
+    const size_t plane = (size_t)NT * MI * MJ;
+#pragma unroll
+    for (int a = 0; a < MI; ++a)
+#pragma unroll
+      for (int b = 0; b < MJ; ++b) {
+        slot[a * MJ + b] = acc_x[a][b];
+        slot[plane + a * MJ + b] = acc_g[a][b];
+      }
+    return;
+  }
+  protein_epilogue<TILE, MI, MJ, kExact>(J, bi, bj, ty, tx, tid, acc_x, acc_g);
+}
+
+// Sums the partial counts of the splits of one tile and finishes it (kernels.hip numbering).
+template <int TILE, int MI, int MJ, bool kExact>
+__global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_finish_kernel(
+    const JobDesc* __restrict__ jobs, unsigned njobs) {
+  constexpr int NTX = TILE / MJ;
+  constexpr int NT = (TILE / MI) * NTX;
+  const unsigned item = blockIdx.x;
+  unsigned lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi + 1) / 2;
+    if (jobs[mid].fin_base <= item) lo = mid; else hi = mid - 1;
+  }
+  const JobDesc& J = jobs[lo];
+  const unsigned tile = item - J.fin_base;
+  if (J.splits <= 1 || tile >= J.tiles) return;
+  unsigned bi, bj;
+  patch_coords(tile, J.nblk, bi, bj);
+  const unsigned tid = threadIdx.x;
+  const unsigned ty = tid / NTX, tx = tid % NTX;
+  uint32_t acc_x[MI][MJ], acc_g[MI][MJ];
+#pragma unroll
+  for (int a = 0; a < MI; ++a)
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) acc_x[a][b] = acc_g[a][b] = 0;
+  const size_t plane = (size_t)NT * MI * MJ;
+  for (unsigned sp = 0; sp < J.splits; ++sp) {
+    const uint32_t* slot = partial_slot<NT, MI, MJ>(J, tile, sp) + (size_t)tid * MI * MJ;
+#pragma unroll
+    for (int a = 0; a < MI; ++a)
+#pragma unroll
+      for (int b = 0; b < MJ; ++b) {
+        acc_x[a][b] += slot[a * MJ + b];
+        acc_g[a][b] += slot[plane + a * MJ + b];
+      }
+  }
+  protein_epilogue<TILE, MI, MJ, kExact>(J, bi, bj, ty, tx, tid, acc_x, acc_g);
+}
+
+template <int TILE, int MI, int MJ, int CW, int B>
+hipError_t launch_protein_cfg(bool exact, const JobDesc* jobs, unsigned njobs, unsigned total,
+                              unsigned finish_items, hipStream_t s) {
+  constexpr int NT = (TILE / MI) * (TILE / MJ);
+  const dim3 grid(total), block(NT);
+  if (exact) protein_tile_kernel<TILE, MI, MJ, CW, B, true><<<grid, block, 0, s>>>(jobs, njobs);
+  else protein_tile_kernel<TILE, MI, MJ, CW, B, false><<<grid, block, 0, s>>>(jobs, njobs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || finish_items == 0) return e;
+  const dim3 fgrid(finish_items);
+  if (exact) protein_finish_kernel<TILE, MI, MJ, true><<<fgrid, block, 0, s>>>(jobs, njobs);
+  else protein_finish_kernel<TILE, MI, MJ, false><<<fgrid, block, 0, s>>>(jobs, njobs);
+  return hipGetLastError();
+}
+
+template <int B>
+hipError_t launch_protein_b(int config, bool exact, const JobDesc* jobs, unsigned njobs,
+                            unsigned total, unsigned f, hipStream_t s) {
+  switch (config) {  // the configurations of kTileConfigs
+    case 0: return launch_protein_cfg<64, 4, 4, 16, B>(exact, jobs, njobs, total, f, s);
+    case 1: return launch_protein_cfg<64, 4, 4, 8, B>(exact, jobs, njobs, total, f, s);
+    case 2: return launch_protein_cfg<128, 8, 4, 16, B>(exact, jobs, njobs, total, f, s);
+    case 3: return launch_protein_cfg<32, 4, 4, 16, B>(exact, jobs, njobs, total, f, s);
+    case 4: return launch_protein_cfg<32, 4, 4, 8, B>(exact, jobs, njobs, total, f, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---- census and packing ----------------------------------------------------------------------
+// Byte values present in the characters: a 256-bit set per work-group in LDS (a bit is tested
+// before the atomic, so a work-group issues about one atomic per distinct value), or-ed into
+// `present` at the end.
+__global__ __launch_bounds__(256) void symbol_census_kernel(const uint8_t* __restrict__ chars,
+                                                            size_t pitch, unsigned n, unsigned L,
+                                                            uint32_t* __restrict__ present) {
+  __shared__ uint32_t seen[8];
+  if (threadIdx.x < 8) seen[threadIdx.x] = 0;
+  __syncthreads();
+  for (unsigned r = blockIdx.y; r < n; r += gridDim.y) {
+    const uint8_t* row = chars + (size_t)r * pitch;
+    for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < L; c += gridDim.x * blockDim.x) {
+      const unsigned v = row[c];
+      const uint32_t bit = 1u << (v & 31);
+      if (!(seen[v >> 5] & bit)) atomicOr(&seen[v >> 5], bit);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 && seen[threadIdx.x]) atomicOr(&present[threadIdx.x], seen[threadIdx.x]);
+}
+
+// One work-group packs 64 taxa x 1024 sites, staged through LDS like the DNA packer
+// (kernels.hip): rows read coalesced into padded LDS rows, lane t of each wave builds taxon t's
+// plane words, stores coalesced over taxa, and adds the taxon's gap sites to gaps[t].
+constexpr unsigned kPTaxa = 64, kPWords = 32, kPRow = kPWords * 8 + 1;
+
+__global__ __launch_bounds__(256) void pack_protein_kernel(
+    const uint8_t* __restrict__ chars, size_t pitch, unsigned n, unsigned L, unsigned n_pad,
+    unsigned W_pad, const uint8_t* __restrict__ codes, unsigned B, uint32_t* __restrict__ planes,
+    uint32_t* __restrict__ gaps) {
+  __shared__ uint32_t tile[kPTaxa * kPRow];
+  __shared__ uint8_t table[256];
+  table[threadIdx.x] = codes[threadIdx.x];
+  const unsigned t0 = blockIdx.x * kPTaxa;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t ps = (size_t)W_pad * n_pad;
+  const unsigned t = t0 + lane;
+  uint32_t gap_sites = 0;
+  for (unsigned w0 = blockIdx.y * kPWords; w0 < W_pad; w0 += gridDim.y * kPWords) {
+    __syncthreads();  // table written / the previous word block's tile consumed
+    for (unsigned e = threadIdx.x; e < kPTaxa * kPWords * 8; e += blockDim.x) {
+      const unsigned r = e / (kPWords * 8), q = e % (kPWords * 8);
+      const size_t s = (size_t)w0 * 32 + 4 * q;
+      uint32_t x = 0;
+      if (t0 + r < n) {
+        const uint8_t* row = chars + (size_t)(t0 + r) * pitch;
+#pragma unroll
+        for (unsigned k = 0; k < 4; ++k)
+          if (s + k < L) x |= (uint32_t)row[s + k] << (8 * k);
+      }
+      tile[r * kPRow + q] = x;
+    }
+    __syncthreads();
+    for (unsigned ww = wave; ww < kPWords; ww += blockDim.x / 64) {
+      const unsigned w = w0 + ww;
+      if (w >= W_pad) break;
+      uint32_t G = 0, C[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (unsigned j = 0; j < 8; ++j) {
+        const uint32_t x = tile[lane * kPRow + ww * 8 + j];
+#pragma unroll
+        for (unsigned k = 0; k < 4; ++k) {
+          const uint32_t c = table[(x >> (8 * k)) & 255u];
+          const unsigned bit = 4 * j + k;
+          G |= (uint32_t)(c == 0) << bit;
+#pragma unroll
+          for (unsigned p = 0; p < 8; ++p) C[p] |= ((c >> p) & 1u) << bit;
+        }
+      }
+      // sites past L and taxa past n: code 0, no gap
+      const unsigned long long first = (unsigned long long)w * 32;
+      uint32_t keep = first >= L ? 0u : (L - first >= 32 ? ~0u : (1u << (L - first)) - 1u);
+      if (t >= n) keep = 0;
+      G &= keep;
+      gap_sites += __popc(G);
+      if (t < n_pad) {
+        planes[(size_t)w * n_pad + t] = G;
+#pragma unroll
+        for (unsigned p = 0; p < 8; ++p)
+          if (p < B) planes[(1 + p) * ps + (size_t)w * n_pad + t] = C[p] & keep;
+      }
+    }
+  }
+  if (t < n && gap_sites) atomicAdd(&gaps[t], gap_sites);
+}
+
+}  // namespace
+
+// ---- launchers ---------------------------------------------------------------------------------
+static unsigned ygrid_p(unsigned rows) { return rows < 1 ? 1 : (rows > 65535 ? 65535 : rows); }
+
+hipError_t launch_symbol_census(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                                uint32_t* present, hipStream_t stream) {
+  const unsigned gx = std::min(32u, (L + 255) / 256);
+  const unsigned gy = ygrid_p(std::min(n, 4096u));
+  symbol_census_kernel<<<dim3(gx, gy), dim3(256), 0, stream>>>(chars, pitch, n, L, present);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_protein(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                               unsigned n_pad, unsigned W_pad, const uint8_t* codes, unsigned B,
+                               uint32_t* planes, uint32_t* gaps, hipStream_t stream) {
+  if (B != 5 && B != 6 && B != 8) return hipErrorInvalidValue;
+  const unsigned wblocks = (W_pad + kPWords - 1) / kPWords;
+  pack_protein_kernel<<<dim3((n_pad + kPTaxa - 1) / kPTaxa, ygrid_p(wblocks)), dim3(256), 0,
+                        stream>>>(chars, pitch, n, L, n_pad, W_pad, codes, B, planes, gaps);
+  return hipGetLastError();
+}
+
+hipError_t launch_protein_tiles(int config, unsigned B, bool exact, const JobDesc* jobs,
+                                unsigned njobs, unsigned total_items, unsigned finish_items,
+                                hipStream_t stream) {
+  if (total_items == 0) return hipSuccess;
+  switch (B) {
+    case 5: return launch_protein_b<5>(config, exact, jobs, njobs, total_items, finish_items, stream);
+    case 6: return launch_protein_b<6>(config, exact, jobs, njobs, total_items, finish_items, stream);
+    case 8: return launch_protein_b<8>(config, exact, jobs, njobs, total_items, finish_items, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace rnj

@@ -47,15 +47,23 @@ size_t sentinel_capacity(unsigned n) {
   return std::min<size_t>(std::max<size_t>(pairs / 64, 1024), (size_t)1 << 24);
 }
 
-int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out) {
+int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out,
+                    int kind, unsigned code_bits) {
   auto* a = new rnj_alignment();
   a->ctx = ctx;
   a->n = n;
   a->L = L;
   a->n_pad = round_up(n, kTaxaPad);
   a->W = words_for_sites(sites);
+  a->kind = kind;
+  a->code_bits = code_bits;
   hipError_t e = hipMalloc((void**)&a->planes, a->bytes());
+  if (e == hipSuccess && kind == kProtein) {
+    a->gaps = a->planes + (size_t)a->nplanes() * a->W * a->n_pad;
+    e = hipMemsetAsync(a->gaps, 0, (size_t)a->n_pad * sizeof(uint32_t), ctx->stream);
+  }
   if (e != hipSuccess) {
+    if (a->planes) (void)hipFree(a->planes);
     delete a;
     return hip_fail(e, "hipMalloc(alignment planes)");
   }
@@ -225,6 +233,8 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
     d.out_ts = s.out_ts;
     d.out_tv = s.out_tv;
     d.out_valid = s.out_valid;
+    d.gaps = s.gaps;
+    d.L = s.L;
     d.maxbits = ws->d_stats + 2 * k;
     d.sent_count = ws->d_stats + 2 * k + 1;
     d.sent = ws->d_sent + soff;
@@ -248,6 +258,21 @@ static int default_config(unsigned n, unsigned W, bool jc_only) {
   return n <= 4096 ? 3 : 1;
 }
 
+// Autotune classes: DNA launches by whether every job is JC (the 6-instruction loop), protein
+// launches by code width.
+static int tune_variant(const LaunchShape& sh) {
+  return sh.kind == kProtein ? 16 + (int)sh.code_bits : (int)sh.jc_only;
+}
+
+static hipError_t launch_counting(int config, const LaunchShape& sh, bool exact,
+                                  const JobDesc* jobs, unsigned njobs, unsigned items,
+                                  unsigned finish_items, hipStream_t stream) {
+  if (sh.kind == kProtein)
+    return launch_protein_tiles(config, sh.code_bits, exact, jobs, njobs, items, finish_items,
+                                stream);
+  return launch_tiles(config, sh.jc_only, exact, jobs, njobs, items, finish_items, stream);
+}
+
 int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
   if (ctx->tile_config >= 0) return ctx->tile_config;
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -262,9 +287,11 @@ int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
 // device runs its first milliseconds at lower clocks), then every configuration is timed in two
 // rounds of two launches, and each keeps its fastest launch, so that a clock change during the
 // sweep does not decide the choice.
-static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, size_t* sent_caps,
-                    bool jc_only, bool exact, unsigned n_key, unsigned W_key, hipStream_t stream,
-                    int* best) {
+static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws,
+                    const LaunchShape& sh, bool exact, hipStream_t stream, int* best) {
+  size_t* sent_caps = const_cast<size_t*>(sh.caps.data());
+  const bool jc_only = sh.jc_only;
+  const unsigned n_key = sh.n_key, W_key = sh.W_key;
   hipEvent_t e0, e1;
   RNJ_HIP(hipEventCreate(&e0));
   RNJ_HIP(hipEventCreate(&e1));
@@ -280,8 +307,8 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, si
                            hipMemcpyHostToDevice, stream));
     RNJ_HIP(launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream));
     RNJ_HIP(hipEventRecord(e0, stream));
-    RNJ_HIP(launch_tiles(c, jc_only, exact, ws->d_jobs, (unsigned)specs.size(), (unsigned)P.items,
-                         (unsigned)P.finish_items, stream));
+    RNJ_HIP(launch_counting(c, sh, exact, ws->d_jobs, (unsigned)specs.size(), (unsigned)P.items,
+                            (unsigned)P.finish_items, stream));
     RNJ_HIP(hipEventRecord(e1, stream));
     RNJ_HIP(hipEventSynchronize(e1));
     float t = 0;
@@ -313,15 +340,20 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws, si
     if (tc > tb || (tc == tb && ms[c] < ms[*best])) *best = c;
   }
   std::lock_guard<std::mutex> lock(ctx->mu);
-  ctx->tuned[std::make_tuple(ilog2(n_key), ilog2(W_key), (int)jc_only)] = *best;
+  ctx->tuned[std::make_tuple(ilog2(n_key), ilog2(W_key), tune_variant(sh))] = *best;
   return RNJ_OK;
 }
 
 LaunchShape shape_of(const std::vector<JobSpec>& specs) {
   LaunchShape sh;
   sh.caps.resize(specs.size());
+  if (!specs.empty()) {
+    sh.kind = specs[0].kind;
+    sh.code_bits = specs[0].code_bits;
+  }
   for (size_t k = 0; k < specs.size(); ++k) {
     const JobSpec& s = specs[k];
+    if (s.kind != sh.kind || s.code_bits != sh.code_bits) sh.mixed = true;
     if (s.model != kJC) sh.jc_only = false;
     sh.n_key = std::max(sh.n_key, s.n);
     sh.W_key = std::max(sh.W_key, s.W);
@@ -338,7 +370,7 @@ int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Wo
   if (*config >= 0) return RNJ_OK;
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
-    auto it = ctx->tuned.find(std::make_tuple(ilog2(sh.n_key), ilog2(sh.W_key), (int)sh.jc_only));
+    auto it = ctx->tuned.find(std::make_tuple(ilog2(sh.n_key), ilog2(sh.W_key), tune_variant(sh)));
     if (it != ctx->tuned.end()) {
       *config = it->second;
       return RNJ_OK;
@@ -349,8 +381,7 @@ int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Wo
   // tune only launches worth >= 2^28 taxon^2-words (~0.1 ms of counting): smaller ones keep
   // the default
   if (allow_tune && cap == hipStreamCaptureStatusNone && sh.work >= (1ull << 28))
-    return autotune(ctx, specs, ws, sh.caps.data(), sh.jc_only, exact, sh.n_key, sh.W_key, stream,
-                    config);
+    return autotune(ctx, specs, ws, sh, exact, stream, config);
   *config = default_config(sh.n_key, sh.W_key, sh.jc_only);
   return RNJ_OK;
 }
@@ -361,6 +392,7 @@ int prepare_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
 
 int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
                    Workspace* ws, bool raw, bool exact, hipStream_t stream) {
+  if (sh.mixed) return fail(RNJ_E_INVALID, "one launch cannot mix DNA and protein jobs");
   const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
   if (P.items >= (1ull << 31)) return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
   RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
@@ -369,8 +401,8 @@ int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, i
                                 hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream);
   if (e == hipSuccess)
-    e = launch_tiles(config, sh.jc_only, exact, ws->d_jobs, (unsigned)specs.size(),
-                     (unsigned)P.items, (unsigned)P.finish_items, stream);
+    e = launch_counting(config, sh, exact, ws->d_jobs, (unsigned)specs.size(),
+                        (unsigned)P.items, (unsigned)P.finish_items, stream);
   if (e == hipSuccess && !raw) e = launch_patch(ws->d_jobs, (unsigned)specs.size(), stream);
   for (size_t k = 0; e == hipSuccess && k < specs.size(); ++k)
     if (specs[k].max_out)
@@ -411,6 +443,72 @@ bool first_row_is_dna(const char* row, unsigned L) {
   return true;
 }
 
+int alignment_kind(const Context* ctx, const char* row0, unsigned L) {
+  if (ctx->input_type == RNJ_TYPE_DNA) return kDNA;
+  if (ctx->input_type == RNJ_TYPE_PROTEIN) return kProtein;
+  return first_row_is_dna(row0, L) ? kDNA : kProtein;
+}
+
+// The characters dataloader::resolveChar maps to the gap for protein alignments (the ones
+// bitDistanceProtein never counts as residues).
+static bool protein_gap_byte(unsigned v) {
+  switch (v) {
+    case '-': case '.': case 'X': case 'x': case 'z': case 'Z': case 'b': case 'B': case 'J':
+    case 'j': case '?':
+      return true;
+    default:
+      return false;
+  }
+}
+
+int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                   int kind, hipStream_t s, Alignment** out) {
+  Alignment* a = nullptr;
+  if (kind == kDNA) {
+    RNJ_HIP_STATUS(alloc_alignment(ctx, n, L, L, &a));
+    hipError_t e = launch_pack_chars(chars, pitch, n, L, a->n_pad, a->W, a->planes, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      rnj_alignment_free(static_cast<rnj_alignment*>(a));
+      return hip_fail(e, "pack DNA alignment");
+    }
+    *out = a;
+    return RNJ_OK;
+  }
+  // protein: census of the byte values, residue codes 1..S in byte order, gaps 0
+  uint32_t* scratch = nullptr;  // present[8], then codes[256]
+  hipError_t e = hipMalloc((void**)&scratch, 8 * sizeof(uint32_t) + 256);
+  if (e == hipSuccess) e = hipMemsetAsync(scratch, 0, 8 * sizeof(uint32_t), s);
+  if (e == hipSuccess) e = launch_symbol_census(chars, pitch, n, L, scratch, s);
+  uint32_t present[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(present, scratch, sizeof present, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  uint8_t codes[256];
+  unsigned S = 0;
+  for (unsigned v = 0; v < 256; ++v) {
+    const bool here = (present[v >> 5] >> (v & 31)) & 1u;
+    codes[v] = (here && !protein_gap_byte(v)) ? (uint8_t)++S : (uint8_t)0;
+  }
+  const unsigned B = S < 32 ? 5 : (S < 64 ? 6 : 8);
+  int rc = RNJ_OK;
+  if (e == hipSuccess) rc = alloc_alignment(ctx, n, L, L, &a, kProtein, B);
+  uint8_t* dcodes = reinterpret_cast<uint8_t*>(scratch + 8);
+  if (e == hipSuccess && rc == RNJ_OK)
+    e = hipMemcpyAsync(dcodes, codes, sizeof codes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && rc == RNJ_OK)
+    e = launch_pack_protein(chars, pitch, n, L, a->n_pad, a->W, dcodes, B, a->planes, a->gaps, s);
+  if (e == hipSuccess && rc == RNJ_OK) e = hipStreamSynchronize(s);
+  if (scratch) (void)hipFree(scratch);
+  if (rc == RNJ_OK && e != hipSuccess) rc = hip_fail(e, "pack protein alignment");
+  if (rc != RNJ_OK) {
+    if (a) rnj_alignment_free(static_cast<rnj_alignment*>(a));
+    return rc;
+  }
+  *out = a;
+  return RNJ_OK;
+}
+
 }  // namespace rnj
 
 using namespace rnj;
@@ -448,6 +546,10 @@ int check_aln(rnj_context* ctx, const rnj_alignment* aln) {
 
 JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size_t ld) {
   JobSpec s{};
+  s.kind = a->kind;
+  s.code_bits = a->code_bits;
+  s.gaps = a->gaps;
+  s.L = a->L;
   s.planes = a->planes;
   s.n = a->n;
   s.n_pad = a->n_pad;
@@ -543,6 +645,14 @@ int rnj_destroy(rnj_context* ctx) {
 
 int rnj_tile_config_count(void) { return kNumTileConfigs; }
 
+int rnj_set_input_type(rnj_context* ctx, int type) {
+  if (!ctx) return fail(RNJ_E_INVALID, "null context");
+  if (type != RNJ_TYPE_AUTO && type != RNJ_TYPE_DNA && type != RNJ_TYPE_PROTEIN)
+    return fail(RNJ_E_INVALID, "type must be RNJ_TYPE_AUTO, RNJ_TYPE_DNA or RNJ_TYPE_PROTEIN");
+  ctx->input_type = type;
+  return RNJ_OK;
+}
+
 const char* rnj_tile_config_name(int config) {
   if (config < 0 || config >= kNumTileConfigs) return nullptr;
   return kTileConfigs[config].name;
@@ -571,33 +681,32 @@ int rnj_alignment_from_chars(rnj_context* ctx, const char* chars, size_t pitch, 
   if (pitch < L) return fail(RNJ_E_INVALID, "pitch smaller than sequence length");
   int rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
+  int kind = kDNA;
   if (!skip_type_check) {
     std::vector<char> first(L);
     if (chars_on_device) RNJ_HIP(hipMemcpy(first.data(), chars, L, hipMemcpyDeviceToHost));
     else std::memcpy(first.data(), chars, L);
-    if (!first_row_is_dna(first.data(), L))
-      return fail(RNJ_E_NOT_DNA, "first sequence is not DNA; only DNA distances are supported");
+    kind = alignment_kind(ctx, first.data(), L);
   }
-  Alignment* a = nullptr;
-  rc = alloc_alignment(ctx, n, L, L, &a);
-  if (rc != RNJ_OK) return rc;
   const uint8_t* dchars = (const uint8_t*)chars;
   void* staging = nullptr;
-  hipError_t e = hipSuccess;
   if (!chars_on_device) {
-    e = hipMalloc(&staging, (size_t)n * pitch);
+    hipError_t e = hipMalloc(&staging, (size_t)n * pitch);
     if (e == hipSuccess)
       e = hipMemcpyAsync(staging, chars, (size_t)n * pitch, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      if (staging) (void)hipFree(staging);
+      return hip_fail(e, "rnj_alignment_from_chars upload");
+    }
     dchars = (const uint8_t*)staging;
   }
-  if (e == hipSuccess)
-    e = launch_pack_chars(dchars, pitch, n, L, a->n_pad, a->W, a->planes, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (staging) (void)hipFree(staging);
-  if (e != hipSuccess) {
-    rnj_alignment_free(static_cast<rnj_alignment*>(a));
-    return hip_fail(e, "rnj_alignment_from_chars");
+  Alignment* a = nullptr;
+  rc = pack_alignment(ctx, dchars, pitch, n, L, kind, ctx->stream, &a);
+  if (staging) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(staging);
   }
+  if (rc != RNJ_OK) return rc;
   *out = static_cast<rnj_alignment*>(a);
   return RNJ_OK;
 }
@@ -775,11 +884,18 @@ int rnj_alignment_info(const rnj_alignment* aln, unsigned* n, unsigned* L, unsig
   return RNJ_OK;
 }
 
+int rnj_alignment_type(const rnj_alignment* aln, int* type, unsigned* planes) {
+  if (!aln) return fail(RNJ_E_INVALID, "null alignment");
+  if (type) *type = aln->kind == kProtein ? RNJ_TYPE_PROTEIN : RNJ_TYPE_DNA;
+  if (planes) *planes = aln->nplanes();
+  return RNJ_OK;
+}
+
 int rnj_alignment_planes(const rnj_alignment* aln, uint32_t* host_planes) {
   if (!aln || !host_planes) return fail(RNJ_E_INVALID, "null argument");
   RNJ_HIP(hipSetDevice(aln->ctx->device));
   RNJ_HIP(hipStreamSynchronize(aln->ctx->stream));
-  for (unsigned p = 0; p < 3; ++p)
+  for (unsigned p = 0; p < aln->nplanes(); ++p)
     RNJ_HIP(hipMemcpy2D(host_planes + (size_t)p * aln->W * aln->n, aln->n * sizeof(uint32_t),
                         aln->planes + (size_t)p * aln->W * aln->n_pad,
                         aln->n_pad * sizeof(uint32_t), aln->n * sizeof(uint32_t), aln->W,
@@ -995,6 +1111,8 @@ int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
   int rc = check_aln(ctx, aln);
   if (rc == RNJ_OK) rc = check_model(model);
   if (rc != RNJ_OK) return rc;
+  if (aln->kind != kDNA)
+    return fail(RNJ_E_NOT_DNA, "bootstrap replicates are built for DNA alignments only");
   if (!weights || !out) return fail(RNJ_E_INVALID, "null argument");
   if (R == 0) return RNJ_OK;
   if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");

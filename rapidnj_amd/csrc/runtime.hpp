@@ -72,6 +72,7 @@ struct Context {
   std::mutex pin_mu;
   uint8_t* pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  int input_type = RNJ_TYPE_AUTO;  // type of alignments read from characters / FASTA
 };
 
 constexpr size_t kPinBytes = (size_t)32 << 20;  // bytes per pinned staging buffer
@@ -84,13 +85,23 @@ struct Alignment {
   unsigned n = 0, L = 0;     // taxa, sites
   unsigned n_pad = 0;        // multiple of kTaxaPad
   unsigned W = 0;            // 32-site words, multiple of kWordPad (= W_pad of the planes)
-  uint32_t* planes = nullptr;  // [3][W][n_pad]
+  int kind = kDNA;           // Kind
+  unsigned code_bits = 0;    // protein: B code planes (5, 6 or 8)
+  uint32_t* planes = nullptr;  // [nplanes()][W][n_pad], then (protein) gaps
+  uint32_t* gaps = nullptr;    // protein: gap sites per taxon [n_pad], inside the planes block
   std::vector<std::string> names;  // FASTA record names in file order (alignments from FASTA)
-  size_t bytes() const { return (size_t)3 * W * n_pad * sizeof(uint32_t); }
+  unsigned nplanes() const { return kind == kProtein ? 1 + code_bits : 3; }
+  size_t bytes() const {
+    return ((size_t)nplanes() * W + (kind == kProtein ? 1 : 0)) * n_pad * sizeof(uint32_t);
+  }
 };
 
 // One matrix of a launch, before the runtime numbers tiles and binds scratch.
 struct JobSpec {
+  int kind;               // Kind; the jobs of one launch share kind and code_bits
+  unsigned code_bits;     // protein
+  const uint32_t* gaps;   // protein: gap sites per taxon
+  unsigned L;             // protein: sites
   const uint32_t* planes;
   unsigned n, n_pad, W_pad, W;
   int model;   // Model
@@ -105,11 +116,23 @@ unsigned round_up(unsigned x, unsigned m);
 unsigned words_for_sites(unsigned sites);  // ceil(sites/32) rounded up to kWordPad
 size_t sentinel_capacity(unsigned n);
 
-// Allocate planes for an alignment of n taxa and `sites` counted sites (zeroed).
-int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out);
+// Allocate planes for an alignment of n taxa and `sites` counted sites (not initialised; the
+// packers write every word, a protein alignment's gap counts are zeroed on ctx->stream).
+int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out,
+                    int kind = kDNA, unsigned code_bits = 0);
 
 // dataloaderFasta.hpp:54-76 on the first sequence: DNA unless a byte >= 'A' is not ACGTUN.
 bool first_row_is_dna(const char* row, unsigned L);
+
+// Kind of an alignment whose first sequence is `row0`: the context's input type, or when that
+// is RNJ_TYPE_AUTO the reference's discovery on the first sequence.
+int alignment_kind(const Context* ctx, const char* row0, unsigned L);
+
+// Pack n x L characters in device memory (row t at chars + t * pitch) into a new alignment of
+// `kind`, enqueued on `stream` and synchronised. Protein alignments first take a census of the
+// byte values to number the residues.
+int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
+                   int kind, hipStream_t stream, Alignment** out);
 
 // Parse a FASTA image and pack it (fasta.hip). `bytes` is in device memory, 16-byte aligned,
 // unless `stage` is hipMemcpyHostToDevice or hipMemcpyDeviceToDevice: then it is first copied
@@ -130,6 +153,9 @@ int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only);
 // The pieces of launch_jobs, for callers that own their workspace (captured plans).
 struct LaunchShape {
   bool jc_only = true;
+  int kind = kDNA;
+  unsigned code_bits = 0;
+  bool mixed = false;  // jobs of different kinds or code widths (not launchable together)
   unsigned n_key = 0, W_key = 0;
   unsigned long long work = 0;
   std::vector<size_t> caps;  // saturated-pair list capacity per job

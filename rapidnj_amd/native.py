@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "librapidnj_amd.so")
 RNJ_OK, RNJ_E_INVALID, RNJ_E_HIP, RNJ_E_NOMEM, RNJ_E_NOT_DNA = 0, 1, 2, 3, 4
 RNJ_E_ADMISSION, RNJ_E_CANCELLED, RNJ_E_NOT_READY, RNJ_E_NO_DEVICE = 5, 6, 7, 8
 JC, K2P = 0, 1
+TYPE_AUTO, TYPE_DNA, TYPE_PROTEIN = 0, 1, 2
 TRIANGLE, SQUARE = 0, 1
 FLAG_RAW = 1
 FLAG_EXACT = 2
@@ -46,6 +47,7 @@ SIGNATURES = {
     "rnj_tile_config_count": (ctypes.c_int, []),
     "rnj_tile_config_name": (ctypes.c_char_p, [ctypes.c_int]),
     "rnj_set_tile_config": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "rnj_set_input_type": (ctypes.c_int, [_vp, ctypes.c_int]),
     "rnj_get_tile_config": (ctypes.c_int, [_vp, _u, _u, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rnj_alignment_from_chars": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int, ctypes.c_int,
                                                 ctypes.POINTER(_vp)]),
@@ -58,6 +60,7 @@ SIGNATURES = {
     "rnj_alignment_free": (ctypes.c_int, [_vp]),
     "rnj_alignment_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u), ctypes.POINTER(_u),
                                           ctypes.POINTER(_u), ctypes.POINTER(_sz)]),
+    "rnj_alignment_type": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_u)]),
     "rnj_alignment_planes": (ctypes.c_int, [_vp, _vp]),
     "rnj_distances_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _sz,
                                             ctypes.c_int, _vp, _vp]),
@@ -160,10 +163,20 @@ class AlignmentInfo:
     L: int
     W: int
     bytes: int
+    type: int = TYPE_DNA
+    nplanes: int = 3
+
+
+def _type(t) -> int:
+    if isinstance(t, str):
+        return {"auto": TYPE_AUTO, "dna": TYPE_DNA, "d": TYPE_DNA, "protein": TYPE_PROTEIN,
+                "p": TYPE_PROTEIN}[t.lower()]
+    return int(t)
 
 
 class Alignment:
-    """A DNA alignment packed into HBM as H/L/V bit planes (owned by a :class:`Context`)."""
+    """An alignment packed into HBM as bit planes (owned by a :class:`Context`): H/L/V for DNA,
+    gap bits + residue code bits for protein."""
 
     def __init__(self, ctx: "Context", handle: int):
         self.ctx = ctx
@@ -172,7 +185,13 @@ class Alignment:
         n, L, W, b = _u(), _u(), _u(), _sz()
         check(lib().rnj_alignment_info(handle, ctypes.byref(n), ctypes.byref(L), ctypes.byref(W),
                                        ctypes.byref(b)))
-        self.info = AlignmentInfo(n.value, L.value, W.value, b.value)
+        t, P = ctypes.c_int(), _u()
+        check(lib().rnj_alignment_type(handle, ctypes.byref(t), ctypes.byref(P)))
+        self.info = AlignmentInfo(n.value, L.value, W.value, b.value, t.value, P.value)
+
+    @property
+    def is_protein(self) -> bool:
+        return self.info.type == TYPE_PROTEIN
 
     @property
     def n(self) -> int:
@@ -189,7 +208,7 @@ class Alignment:
         return [lib().rnj_alignment_name(self.handle, i) for i in range(k)]
 
     def planes(self) -> np.ndarray:
-        out = np.zeros((3, self.info.W, self.info.n), dtype=np.uint32)
+        out = np.zeros((self.info.nplanes, self.info.W, self.info.n), dtype=np.uint32)
         check(lib().rnj_alignment_planes(self.handle, out.ctypes.data))
         return out
 
@@ -247,6 +266,10 @@ class Context:
         return c.value
 
     # ---- alignments ---------------------------------------------------------------------------
+    def set_input_type(self, type) -> None:
+        """"auto" (the reference's discovery on the first sequence), "dna" or "protein"."""
+        check(lib().rnj_set_input_type(self.handle, _type(type)))
+
     def alignment(self, aln, skip_type_check: bool = False) -> Alignment:
         """Pack an (n, L) uint8 alignment (numpy on the host, or a uint8 torch tensor on the device)."""
         h = _vp()

@@ -330,9 +330,8 @@ def test_error_paths(ctx, N):
         ctx.alignment(np.frombuffer(b"ACGT", dtype=np.uint8).reshape(1, 4))
     assert e.value.status == N.RNJ_E_INVALID
     prot = np.frombuffer(b"MKVLAAGG" * 2, dtype=np.uint8).reshape(2, 8)
-    with pytest.raises(N.RnjError) as e:
-        ctx.alignment(prot)
-    assert e.value.status == N.RNJ_E_NOT_DNA
+    assert ctx.alignment(prot).is_protein
+    assert not ctx.alignment(prot, skip_type_check=True).is_protein
     A = ctx.alignment(simulate_alignment(10, 50, 0.0, 3))
     import torch
     small = torch.empty((10, 9), dtype=torch.float32, device="cuda:0")
