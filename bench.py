@@ -7,7 +7,7 @@ saturation patch (2 * max) into a float32 upper triangle that stays in HBM (the 
 computeDistances region, KimuraDistance.cpp:75-99, minus parsing). Default workload: 20000 taxa x
 5000 sites, 5% gaps, K2P. Prints one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--workload large|plain|small_jc|bootstrap]
+  python bench.py [--steps K] [--warmup W] [--workload large|plain|small_jc|bootstrap|protein]
 """
 from __future__ import annotations
 
@@ -41,7 +41,7 @@ def cpu_cores() -> int:
     return max(1, min(16, n))
 
 
-def cpu_baseline(aln, model: str, sample_n: int, threads: int) -> dict:
+def cpu_baseline(aln, model: str, sample_n: int, threads: int, protein: bool = False) -> dict:
     """Reference CPU engine (compiled from its sources into oracle/_ref) on the first sample_n
     taxa of the same alignment; falls back to the oracle's restatement when the harness is absent."""
     from oracle import oracle as O
@@ -53,18 +53,24 @@ def cpu_baseline(aln, model: str, sample_n: int, threads: int) -> dict:
         with tempfile.TemporaryDirectory() as tmp:
             fa = os.path.join(tmp, "sample.fa")
             write_fasta(fa, sub)
-            out = subprocess.run([O.REF_HARNESS, "time", fa, "kim" if model == "k2p" else "jc",
-                                  str(threads), "1"], check=True, capture_output=True, text=True)
+            out = subprocess.run([O.REF_HARNESS, "ptime" if protein else "time", fa,
+                                  "kim" if model == "k2p" else "jc", str(threads), "1"],
+                                 check=True, capture_output=True, text=True)
             secs = json.loads(out.stdout.strip().splitlines()[-1])["best_s"]
         kind = "reference"
     else:
         t0 = time.perf_counter()
-        O.distance_matrix(sub, model, threads=threads)
+        if protein:
+            O.protein_matrix(sub, model)
+            threads = 1
+        else:
+            O.distance_matrix(sub, model, threads=threads)
         secs = time.perf_counter() - t0
         kind = "port"
+    label = ("Kimura protein" if model == "k2p" else "JC alpha=20") if protein else model.upper()
     return {"value": round(site_pairs / secs / 1e9, 3), "unit": "G site-pairs/s",
             "cores": threads, "kind": kind,
-            "sample": f"first {n} taxa x {L} sites of the same alignment, {model.upper()}, "
+            "sample": f"first {n} taxa x {L} sites of the same alignment, {label}, "
                       f"compute+post-process {secs:.2f} s"}
 
 
@@ -130,9 +136,9 @@ def probe_traffic(args) -> int:
     reads 1 GiB and writes 1 GiB (ordinary vector loads and stores, like the tile kernel's)."""
     import torch
     from rapidnj_amd import native as N
-    from rapidnj_amd.workloads import WORKLOADS, simulate_alignment
+    from rapidnj_amd.workloads import WORKLOADS, make_alignment
     wl = WORKLOADS[args.workload]
-    aln = simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    aln = make_alignment(wl)
     ctx = N.Context(0)
     if args.tile_config >= 0:
         ctx.set_tile_config(args.tile_config)
@@ -227,12 +233,64 @@ def handoff_compare(ctx, A) -> dict:
     return out
 
 
+def ops_per_word_pair(A, model: str) -> int:
+    """VALU instructions per 32 sites and pair of the tile kernel that counts alignment A."""
+    if A.is_protein:
+        return A.info.nplanes + 2  # B code planes: v_xor, B - 1 v_bitop3, v_and, 2 v_bcnt
+    return OPS_PER_WORD_PAIR[model]
+
+
+def protein_side(ctx, stream, cpu: bool) -> dict:
+    """Kimura-protein distances of the 20000 x 1250 protein workload (resident, float32 upper
+    triangle in HBM): throughput, the protein tile kernel's VALU roofline, and the reference's
+    CPU protein engine (bitDistanceProtein + KimuraDistance) on the first 10000 taxa."""
+    import torch
+    from rapidnj_amd import native as N
+    from rapidnj_amd.workloads import WORKLOADS, make_alignment
+    wl = WORKLOADS["protein"]
+    aln = make_alignment(wl)
+    A = ctx.alignment(aln)
+    n, L = wl.n, wl.L
+    pairs = n * (n - 1) // 2
+    out = torch.empty(pairs, dtype=torch.float32, device="cuda:0")
+    for _ in range(3):  # the first launch tunes the tile configuration
+        ctx.distances(A, wl.model, out=out, stream=stream)
+    torch.cuda.synchronize()
+    k = 10
+    t0 = time.perf_counter()
+    for _ in range(k):
+        ctx.distances(A, wl.model, out=out, stream=stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / k * 1e3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        ctx.distances(A, wl.model, out=out, raw=True, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kern_ms = e0.elapsed_time(e1) / k
+    ops = ops_per_word_pair(A, wl.model) * pairs * (L / 32.0)
+    tops = ops / (kern_ms / 1e3) / 1e12
+    res = {"workload": wl.name, "code_planes": A.info.nplanes - 1,
+           "ms_per_step": round(ms, 3), "value": round(pairs * L / (ms / 1e3) / 1e9, 2),
+           "unit": "G site-pairs/s", "kernel_ms": round(kern_ms, 3),
+           "valu": {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
+                    "frac": round(tops / VALU_PEAK_TOPS, 4),
+                    "ops_per_32_site_pairs": ops_per_word_pair(A, wl.model)},
+           "tile_config": N.tile_configs()[ctx.tile_config_for(A, wl.model)]}
+    if cpu:
+        res["cpu_reference"] = cpu_baseline(aln, wl.model, 10000, cpu_cores(), protein=True)
+        res["speedup_vs_cpu"] = round(res["value"] / res["cpu_reference"]["value"], 1)
+    A.close()
+    return res
+
+
 def side_main(args) -> int:
     """--side: one side measurement alone (for profiling it)."""
     from rapidnj_amd import native as N
-    from rapidnj_amd.workloads import WORKLOADS, simulate_alignment
+    from rapidnj_amd.workloads import WORKLOADS, make_alignment
     wl = WORKLOADS[args.workload]
-    aln = simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    aln = make_alignment(wl)
     ctx = N.Context(0)
     if args.side == "ingest":
         res = ingest_compare(ctx, aln)
@@ -369,7 +427,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large",
-                    choices=["large", "plain", "small_jc", "bootstrap", "served"])
+                    choices=["large", "plain", "small_jc", "bootstrap", "served", "protein"])
     ap.add_argument("--served-jobs", type=int, default=0,
                     help="jobs of the served stream (default: 512 for --workload served, 128 as "
                          "the side measurement of the default run)")
@@ -398,7 +456,8 @@ def main() -> int:
     import torch
 
     from rapidnj_amd import native as N
-    from rapidnj_amd.workloads import WORKLOADS, bootstrap_weights, served_stream, simulate_alignment
+    from rapidnj_amd.workloads import (WORKLOADS, bootstrap_weights, make_alignment, served_stream,
+                                       simulate_alignment)
 
     served_jobs, served_alns = [], []
     if args.workload == "served" or not args.no_extra:
@@ -412,7 +471,7 @@ def main() -> int:
 
     wl = WORKLOADS[args.workload]
     log(f"workload {wl.name}: generating {wl.n} x {wl.L} (seed {wl.seed})")
-    aln = simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    aln = make_alignment(wl)
     ctx = N.Context(0)
     if args.tile_config >= 0:
         ctx.set_tile_config(args.tile_config)
@@ -465,10 +524,11 @@ def main() -> int:
         kern_label = "bootstrap launch (gather + tile kernel)"
     else:
         kern_label = "tile_kernel (+ stats reset)"
-    ops = OPS_PER_WORD_PAIR[wl.model] * pairs * (L / 32.0) * R
+    planes = A.info.nplanes
+    ops = ops_per_word_pair(A, wl.model) * pairs * (L / 32.0) * R
     achieved = ops / (kern_ms / 1e3) / 1e12
-    algo_bytes = pairs * 4 * R + n * 3 * L / 8 * R
-    cfg = ctx.tile_config(n, L, wl.model)
+    algo_bytes = pairs * 4 * R + n * planes * L / 8 * R
+    cfg = ctx.tile_config_for(A, wl.model)
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
                 "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                 "kernel": kern_label, "kernel_ms": round(kern_ms, 4),
@@ -515,6 +575,8 @@ def main() -> int:
         extra["ingest"] = ingest_compare(ctx, aln)
         log("side run: batched bootstrap vs one launch per replicate")
         extra["bootstrap"] = bootstrap_compare(ctx, dev)
+        log("side run: protein distances (Kimura protein, 20000 x 1250)")
+        extra["protein"] = protein_side(ctx, stream, not args.no_cpu_baseline)
         log(f"side run: served stream, {len(served_jobs)} jobs")
         extra["served"] = {
             "stream": f"first {len(served_jobs)} jobs of the 512-job stream (Poisson 20/s; "
@@ -537,13 +599,14 @@ def main() -> int:
         threads = cpu_cores()
         sample_n = min(n, 20000)  # the whole 20000 x 5000 workload: ~4 s on 16 host threads
         log(f"CPU baseline on {sample_n} taxa with {threads} threads")
-        base = cpu_baseline(aln, wl.model, sample_n, threads)
+        base = cpu_baseline(aln, wl.model, sample_n, threads, protein=wl.protein)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "G site-pairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-        "data": "synthetic (seeded K2P-evolved alignment on a random-joining tree)",
+        "data": "synthetic (seeded %s-evolved alignment on a random-joining tree)"
+                % ("20-state" if wl.protein else "K2P"),
         "config": {"workload": wl.name, "taxa": n, "sites": L, "gap": wl.gap, "model": wl.model,
                    "replicates": wl.replicates, "output": "float32 upper triangle in HBM",
                    "parallelism": "single GPU"},

@@ -121,6 +121,10 @@ RNJ_EXPORT int rnj_tile_config_count(void);
 RNJ_EXPORT const char* rnj_tile_config_name(int config);
 RNJ_EXPORT int rnj_set_tile_config(rnj_context* ctx, int config);
 RNJ_EXPORT int rnj_get_tile_config(rnj_context* ctx, unsigned n, unsigned L, int model, int* config);
+/* The configuration the next launch of this alignment and model uses (tuned per shape class and
+ * alignment type: DNA, or protein of a code width). */
+RNJ_EXPORT int rnj_get_tile_config_for(rnj_context* ctx, const rnj_alignment* aln, int model,
+                                       int* config);
 
 /* Type of the alignments this context reads from characters, rows and FASTA from now on:
  * RNJ_TYPE_AUTO (default) classifies the first sequence as the reference's readers do

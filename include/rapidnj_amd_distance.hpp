@@ -10,8 +10,13 @@
 //
 // KimuraDistance.hpp:12-17 and JCdistance.hpp:14-18 declare the originals. The templates take the
 // loader and disk-matrix types so the header needs none of the reference's headers: a Loader
-// provides the getters of dataloader (dataloader.hpp:20-29) and a DiskMatrix provides
+// provides the getters of dataloader (dataloader.hpp:20-29) and, like dataloader, may have a
+// `type` member holding the InputType (dataloader.hpp:84); a DiskMatrix provides
 // writeArray(distType*, int row, int size) (diskMatrix.h:13).
+//
+// Protein loaders (type == PROTEIN) get the reference's protein distances: Kimura protein with
+// the Dayhoff table (KimuraDistance.cpp:322-338) and JC with alpha = 20 (JCdistance.cpp:155-158),
+// counted as bitDistanceProtein counts (bitDistanceProtein.cpp:31-61).
 //
 // Behaviour kept from the reference:
 //  * without a disk matrix the result is n rows allocated with new distType[n] (ownership passes
@@ -56,22 +61,42 @@ inline void fail_exit(const char* where) {
   std::exit(1);
 }
 
-// Packs the loader's alignment into HBM. With fastdist the loader's bit strings are used, which
-// are exactly what the reference's bitDistanceGap counts; otherwise its character rows.
+// The loader's InputType (stdinclude.h:26: DNA, PROTEIN, UNKNOWN are 0, 1, 2) as an RNJ_TYPE_*.
+// A loader without a `type` member is classified by the library from its first sequence.
+template <class Loader>
+auto loader_type(Loader* loader, int) -> decltype((int)loader->type) {
+  const int t = (int)loader->type;
+  return t == 0 ? RNJ_TYPE_DNA : (t == 1 ? RNJ_TYPE_PROTEIN : RNJ_TYPE_AUTO);
+}
+template <class Loader>
+int loader_type(Loader*, long) {
+  return RNJ_TYPE_AUTO;
+}
+
+// Packs the loader's alignment into HBM. DNA with fastdist: the loader's bit strings, which are
+// exactly what the reference's bitDistanceGap counts. Protein with fastdist: the same bit
+// strings hold the resolved characters four to a word (dataloaderFasta.hpp
+// encodeProteinSequence), so their first L bytes are the row. Otherwise the character rows.
+// The shared context's input type is set for the upload and reset after it (the reference's
+// callers are single-threaded).
 template <class Loader>
 rnj_alignment* upload_alignment(Loader* loader, bool fastdist) {
   rnj_alignment* aln = nullptr;
   const unsigned n = loader->getSequenceCount();
+  const int type = loader_type(loader, 0);
   int rc;
-  if (fastdist) {
+  if (type == RNJ_TYPE_PROTEIN || !fastdist) {
+    const char* const* rows =
+        fastdist ? reinterpret_cast<const char* const*>(loader->getBitStrings())
+                 : const_cast<const char* const*>(loader->getSequences()->data());
+    rnj_set_input_type(shared_context(), type);
+    rc = rnj_alignment_from_rows(shared_context(), rows, n, loader->getSequenceLength(), &aln);
+    rnj_set_input_type(shared_context(), RNJ_TYPE_AUTO);
+  } else {
     rc = rnj_alignment_from_bitstrings(shared_context(),
                                        (const unsigned int* const*)loader->getBitStrings(),
                                        (const unsigned int* const*)loader->getGapFilters(), n,
                                        loader->getBitStringsCount(), &aln);
-  } else {
-    std::vector<char*>* seqs = loader->getSequences();
-    rc = rnj_alignment_from_rows(shared_context(), (const char* const*)seqs->data(), n,
-                                 loader->getSequenceLength(), &aln);
   }
   if (rc != RNJ_OK) fail_exit("upload");
   return aln;

@@ -25,6 +25,7 @@
 //   counts FASTA OUT [naive]          u32 n, L; then for i<j (row-major) u32 ts, tv, valid
 //   matrix FASTA jc|kim T OUT [naive] u32 n; then n*n f32 (post-processed matrix); prints seconds
 //   time   FASTA jc|kim T REPS        prints {"best_s":..,"mean_s":..,"n":..,"L":..,"threads":..}
+//   ptime  FASTA jc|kim T REPS        the same for a protein alignment
 //   nj     MATRIX OUT                 MATRIX = u32 n + n*n f32; writes Newick (rapidNJ) with
 //                                     leaf names "t<i>"
 //   tree   FASTA jc|kim T OUT         distance matrix + rapidNJ; writes Newick
@@ -219,8 +220,8 @@ int mode_matrix(int argc, char** argv) {
 }
 
 int mode_time(int argc, char** argv) {
-  if (argc < 6) die("time FASTA jc|kim T REPS");
-  dataloaderFasta* dl = load(argv[2], true);
+  if (argc < 6) die(std::string(argv[1]) + " FASTA jc|kim T REPS");
+  dataloaderFasta* dl = std::string(argv[1]) == "ptime" ? load_protein(argv[2]) : load(argv[2], true);
   int n = dl->getSequenceCount();
   int threads = atoi(argv[4]), reps = atoi(argv[5]);
   double best = 1e30, total = 0;
@@ -283,7 +284,7 @@ int mode_tree(int argc, char** argv) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2) die("usage: ref_harness pack|fasta|load|counts|matrix|time|nj|tree ...");
+  if (argc < 2) die("usage: ref_harness pack|fasta|load|pcounts|pmatrix|counts|matrix|time|ptime|nj|tree ...");
   std::string mode = argv[1];
   if (mode == "pack") return mode_pack(argc, argv);
   if (mode == "fasta") return mode_fasta(argc, argv);
@@ -292,7 +293,7 @@ int main(int argc, char** argv) {
   if (mode == "pmatrix") return mode_pmatrix(argc, argv);
   if (mode == "counts") return mode_counts(argc, argv);
   if (mode == "matrix") return mode_matrix(argc, argv);
-  if (mode == "time") return mode_time(argc, argv);
+  if (mode == "time" || mode == "ptime") return mode_time(argc, argv);
   if (mode == "nj") return mode_nj(argc, argv);
   if (mode == "tree") return mode_tree(argc, argv);
   die("unknown mode " + mode);

@@ -67,11 +67,27 @@ __constant__ float kDayhoff[181] = {
 // reference's `total / (distType) data[2]` computes them. kExact takes the library log; the
 // default log_unit rounds to the same float except in rare ties (see log_unit).
 #pragma clang fp contract(off)
+// (float)mism / (float)valid. Below 2^24 both convert exactly and the float division is the
+// correctly rounded quotient of the integers, which the default path forms from the FP64
+// reciprocal as proportions2 does (same result, without the division sequence).
+template <bool kExact>
+__device__ __forceinline__ float protein_proportion(uint32_t mism, unsigned long long valid) {
+  if (!kExact && valid < (1u << 24) && mism < (1u << 24)) {
+    const double dv = (double)valid;
+    double r = __builtin_amdgcn_rcp(dv);
+    r = fma(r, fma(-dv, r, 1.0), r);
+    r = fma(r, fma(-dv, r, 1.0), r);
+    return (float)((double)mism * r);
+  }
+  return (float)mism / (float)valid;
+}
+
 // KimuraDistance.cpp:322-338: -1 when valid is 0, -log(1 - p - 0.2 p^2) below 0.75 (p * p in
-// float, the rest in double), the Dayhoff table up to 0.930 and 10 above.
+// float, the rest in double), the Dayhoff table up to 0.930 and 10 above. Straight-line (every
+// case computed, the result selected) so that a caller can interleave several pairs.
 template <bool kExact>
 __device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid) {
-  const float p = valid != 0 ? (float)mism / (float)valid : -1.0f;
+  const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : -1.0f;
   const bool formula = p < 0.75f && p > 0.0f;
   const float pp = p * p;
   const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
@@ -82,11 +98,12 @@ __device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned lo
   return p < 0.75f ? (formula ? (float)(-lg) : p) : table;
 }
 
-// JCdistance.cpp:170-183 with alpha = 20: p = 0 when valid is 0, -1 when p / 0.95 >= 1.
+// JCdistance.cpp:170-183 with alpha = 20: p = 0 when valid is 0, -1 when p / 0.95 >= 1 (the
+// quotient in double, as the reference forms it, so that the saturated pairs are the same).
 template <bool kExact>
 __device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long long valid) {
   const double b = 19.0 / 20.0;  // (alpha - 1) / alpha
-  const float p = valid != 0 ? (float)mism / (float)valid : 0.0f;
+  const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : 0.0f;
   const double q = (double)p / b;
   const bool sat = q >= 1.0;
   const double arg = sat ? 1.0 : 1.0 - q;
@@ -112,41 +129,62 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned bi, 
   for (int a = 0; a < MI; ++a) gi[a] = J.gaps[bi * TILE + ty * MI + a];
 #pragma unroll
   for (int b = 0; b < MJ; ++b) gj[b] = J.gaps[bj * TILE + tx * MJ + b];
-  float lmax = 0.0f;
+  auto counts = [&](int a, int b, uint32_t& mism, unsigned long long& valid) {
+    const unsigned long long gsum = (unsigned long long)gi[a] + gj[b];
+    mism = (uint32_t)(acc_x[a][b] + 2ull * acc_g[a][b] - gsum);
+    valid = L - gsum + acc_g[a][b] - bias;
+  };
+  if (J.model == kCounts) {
 #pragma unroll
-  for (int a = 0; a < MI; ++a) {
-    const unsigned i = bi * TILE + ty * MI + a;
+    for (int a = 0; a < MI; ++a) {
+      const unsigned i = bi * TILE + ty * MI + a;
 #pragma unroll
-    for (int b = 0; b < MJ; ++b) {
-      const unsigned j = bj * TILE + tx * MJ + b;
-      if (j >= n || i > j) continue;
-      if (i == j) {
-        finish_diagonal(J, i);
-        continue;
-      }
-      const unsigned long long gsum = (unsigned long long)gi[a] + gj[b];
-      const uint32_t mism = (uint32_t)(acc_x[a][b] + 2ull * acc_g[a][b] - gsum);
-      const unsigned long long valid = L - gsum + acc_g[a][b] - bias;
-      if (J.model == kCounts) {
+      for (int b = 0; b < MJ; ++b) {
+        const unsigned j = bj * TILE + tx * MJ + b;
+        if (j >= n || i >= j) continue;
+        uint32_t mism;
+        unsigned long long valid;
+        counts(a, b, mism, valid);
         const unsigned long long p = tri_index(i, j, n);
         J.out_ts[p] = mism;
         J.out_tv[p] = 0;
         J.out_valid[p] = (uint32_t)valid;
-        continue;
       }
-      const float d = J.model == kJC ? jc_protein_value<kExact>(mism, valid)
-                                     : kimura_protein_value<kExact>(mism, valid);
-      if (d == -1.0f) push_sentinel(J, i, j);
-      else lmax = fmaxf(lmax, d);
-      if (J.layout == kSquare) {
-        J.out[(size_t)i * J.ld + j] = d;
-        J.out[(size_t)j * J.ld + i] = d;
-      } else {
-        J.out[tri_index(i, j, n)] = d;
+    }
+    return;
+  }
+  float lmax = 0.0f;
+#pragma unroll
+  for (int a = 0; a < MI; ++a) {
+    const unsigned i = bi * TILE + ty * MI + a;
+    float d[MJ];
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) {
+      uint32_t mism;
+      unsigned long long valid;
+      counts(a, b, mism, valid);
+      d[b] = J.model == kJC ? jc_protein_value<kExact>(mism, valid)
+                            : kimura_protein_value<kExact>(mism, valid);
+    }
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) {
+      const unsigned j = bj * TILE + tx * MJ + b;
+      if (j >= n) continue;
+      if (i < j) {
+        if (d[b] == -1.0f) push_sentinel(J, i, j);
+        else lmax = fmaxf(lmax, d[b]);
+        if (J.layout == kSquare) {
+          J.out[(size_t)i * J.ld + j] = d[b];
+          J.out[(size_t)j * J.ld + i] = d[b];
+        } else {
+          J.out[tri_index(i, j, n)] = d[b];
+        }
+      } else if (i == j) {
+        finish_diagonal(J, i);
       }
     }
   }
-  if (J.model != kCounts) fold_max(J, lmax, tid);
+  fold_max(J, lmax, tid);
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------

@@ -273,6 +273,15 @@ static hipError_t launch_counting(int config, const LaunchShape& sh, bool exact,
   return launch_tiles(config, sh.jc_only, exact, jobs, njobs, items, finish_items, stream);
 }
 
+// The configuration the next launch of this shape class uses (without tuning it).
+static int current_config(Context* ctx, const LaunchShape& sh) {
+  if (ctx->tile_config >= 0) return ctx->tile_config;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto it = ctx->tuned.find(std::make_tuple(ilog2(sh.n_key), ilog2(sh.W_key), tune_variant(sh)));
+  if (it != ctx->tuned.end()) return it->second;
+  return default_config(sh.n_key, sh.W_key, sh.jc_only);
+}
+
 int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
   if (ctx->tile_config >= 0) return ctx->tile_config;
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -644,6 +653,16 @@ int rnj_destroy(rnj_context* ctx) {
 }
 
 int rnj_tile_config_count(void) { return kNumTileConfigs; }
+
+int rnj_get_tile_config_for(rnj_context* ctx, const rnj_alignment* aln, int model, int* config) {
+  int rc = check_aln(ctx, aln);
+  if (rc == RNJ_OK) rc = check_model(model);
+  if (rc != RNJ_OK) return rc;
+  if (!config) return fail(RNJ_E_INVALID, "null argument");
+  std::vector<JobSpec> jobs{spec_for(aln, model, kTriangle, nullptr, 0)};
+  *config = current_config(ctx, shape_of(jobs));
+  return RNJ_OK;
+}
 
 int rnj_set_input_type(rnj_context* ctx, int type) {
   if (!ctx) return fail(RNJ_E_INVALID, "null context");

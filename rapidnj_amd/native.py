@@ -47,6 +47,7 @@ SIGNATURES = {
     "rnj_tile_config_count": (ctypes.c_int, []),
     "rnj_tile_config_name": (ctypes.c_char_p, [ctypes.c_int]),
     "rnj_set_tile_config": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "rnj_get_tile_config_for": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rnj_set_input_type": (ctypes.c_int, [_vp, ctypes.c_int]),
     "rnj_get_tile_config": (ctypes.c_int, [_vp, _u, _u, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rnj_alignment_from_chars": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int, ctypes.c_int,
@@ -263,6 +264,12 @@ class Context:
     def tile_config(self, n: int, L: int, model="k2p") -> int:
         c = ctypes.c_int()
         check(lib().rnj_get_tile_config(self.handle, n, L, _model(model), ctypes.byref(c)))
+        return c.value
+
+    def tile_config_for(self, aln: "Alignment", model="k2p") -> int:
+        """Configuration the next launch of `aln` with `model` uses."""
+        c = ctypes.c_int()
+        check(lib().rnj_get_tile_config_for(self.handle, aln.handle, _model(model), ctypes.byref(c)))
         return c.value
 
     # ---- alignments ---------------------------------------------------------------------------

@@ -1,4 +1,5 @@
-"""Seeded synthetic DNA alignments and bootstrap weights for the benchmark configurations.
+"""Seeded synthetic DNA and protein alignments and bootstrap weights for the benchmark
+configurations.
 
 The reference ships no generator for its benchmark inputs (its ``sim_seq.cpp`` is a time-seeded
 star model, ``src/distanceCalculation/sim_seq.cpp:12-167``), so this module fixes one:
@@ -14,6 +15,11 @@ star model, ``src/distanceCalculation/sim_seq.cpp:12-167``), so this module fixe
 Bases are held in the reference's 2-bit code (A=0, G=1, T=2, C=3, ``src/stdinclude.h:25``): a
 transition flips the low bit, a transversion flips the high bit. Sequences are returned as an
 ``(n, L)`` uint8 array of ASCII letters (uppercase ``ACGT`` and ``-``).
+
+Protein alignments (:func:`simulate_protein_alignment`) use the same topology and branch lengths
+with a uniform 20-state model: root residues uniform over the 20 amino acids, and along a branch
+of length t a site changes with probability (19/20)(1 - exp(-20t/19)) to one of the other 19
+residues, uniformly; gaps as above.
 """
 from __future__ import annotations
 
@@ -88,6 +94,36 @@ def simulate_alignment(n: int, L: int, gap: float, seed: int, kappa: float = 2.0
     return np.ascontiguousarray(leaves)
 
 
+AMINO_ACIDS = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", dtype=np.uint8)
+
+
+def simulate_protein_alignment(n: int, L: int, gap: float, seed: int,
+                               mean_branch: float = 0.015) -> np.ndarray:
+    """Return an ``(n, L)`` uint8 ASCII protein alignment (20 amino acids and ``-``)."""
+    if n < 2 or L < 1:
+        raise ValueError("need n >= 2 and L >= 1")
+    rng = np.random.default_rng(seed)
+    parent = _random_joining_tree(n, rng)
+    nodes = 2 * n - 1
+    blen = rng.exponential(mean_branch, size=nodes)
+    depth = np.zeros(nodes, dtype=np.int64)
+    for v in range(nodes - 2, -1, -1):
+        depth[v] = depth[parent[v]] + 1
+    order = np.argsort(depth, kind="stable")
+    levels = np.split(order, np.flatnonzero(np.diff(depth[order])) + 1)
+    codes = np.empty((nodes, L), dtype=np.uint8)
+    codes[nodes - 1] = rng.integers(0, 20, size=L, dtype=np.uint8)
+    for lev in levels[1:]:
+        p_change = 0.95 * (1.0 - np.exp(-blen[lev][:, None] / 0.95))
+        change = rng.random((lev.size, L)) < p_change
+        step = rng.integers(1, 20, size=(lev.size, L), dtype=np.uint8)  # to another residue
+        codes[lev] = (codes[parent[lev]] + np.where(change, step, 0)) % 20
+    leaves = AMINO_ACIDS[codes[:n]]
+    if gap > 0:
+        leaves[rng.random((n, L)) < gap] = GAP
+    return np.ascontiguousarray(leaves)
+
+
 def bootstrap_weights(L: int, replicates: int, seed0: int = 0) -> np.ndarray:
     """``(replicates, L)`` uint16 column weights; replicate r ~ Multinomial(L; uniform), seed r+seed0."""
     out = np.empty((replicates, L), dtype=np.uint16)
@@ -152,9 +188,10 @@ class Workload:
     n: int
     L: int
     gap: float
-    model: str  # "jc" or "k2p"
+    model: str  # "jc" or "k2p" (Kimura protein for protein workloads)
     seed: int
     replicates: int = 0
+    protein: bool = False
 
     @property
     def site_pairs(self) -> int:
@@ -166,7 +203,16 @@ WORKLOADS = {
     "plain": Workload("2000x10000 K2P, 5% gaps", 2000, 10000, 0.05, "k2p", 2),
     "large": Workload("20000x5000 K2P, 5% gaps", 20000, 5000, 0.05, "k2p", 3),
     "bootstrap": Workload("1000x5000 K2P, 100 bootstrap replicates", 1000, 5000, 0.05, "k2p", 4, 100),
+    "protein": Workload("20000x1250 Kimura protein, 10% gaps", 20000, 1250, 0.10, "k2p", 6,
+                        protein=True),
 }
+
+
+def make_alignment(wl: Workload) -> np.ndarray:
+    """The workload's alignment (DNA or protein generator)."""
+    if wl.protein:
+        return simulate_protein_alignment(wl.n, wl.L, wl.gap, wl.seed)
+    return simulate_alignment(wl.n, wl.L, wl.gap, wl.seed)
 
 
 @dataclass(frozen=True)

@@ -3,8 +3,10 @@
 // matrix, KimuraDistance / JCdistance, getDistanceMatrix. Test infrastructure for
 // tests/test_gpu_adapter.py.
 //
-//   adapter_test IN OUT kim|jc mem|disk|reuse fast|chars
+//   adapter_test IN OUT kim|jc mem|disk|reuse|half fast|chars [protein]
 // IN:  u32 n, L, bitStringsCount; n*L chars; n*bsc*4 u32 bit strings; n*bsc*4 u32 gap filters
+//      (protein: the bit strings hold the resolved characters, 16 per 4 words, as the reference's
+//      loader stores them; the gap filters are unused)
 // OUT: u32 n; n*n f32 matrix (row-major); f32 maximum distance
 #include <cstdint>
 #include <cstring>
@@ -27,6 +29,7 @@ class MemoryLoader {
   std::vector<char*> seqs;
   std::vector<unsigned int*> bit_rows, gap_rows;
   std::vector<std::string> names;
+  int type = 0;  // InputType (stdinclude.h:26): DNA = 0, PROTEIN = 1
 
   unsigned int** getBitStrings() { return bit_rows.data(); }
   unsigned int** getGapFilters() { return gap_rows.data(); }
@@ -75,7 +78,8 @@ static bool read_input(const char* path, MemoryLoader& ld) {
 
 int main(int argc, char** argv) {
   if (argc < 6) {
-    std::cerr << "usage: adapter_test IN OUT kim|jc mem|disk|reuse|half fast|chars" << std::endl;
+    std::cerr << "usage: adapter_test IN OUT kim|jc mem|disk|reuse|half fast|chars [protein]"
+              << std::endl;
     return 2;
   }
   MemoryLoader ld;
@@ -85,6 +89,7 @@ int main(int argc, char** argv) {
   }
   const std::string model = argv[3], mode = argv[4];
   const bool fast = std::string(argv[5]) == "fast";
+  if (argc > 6 && std::string(argv[6]) == "protein") ld.type = 1;
   const unsigned n = ld.n;
   std::vector<distType> out((size_t)n * n);
   distType maxd = 0;

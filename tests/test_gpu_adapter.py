@@ -29,11 +29,12 @@ def _write_input(path, d):
         np.ascontiguousarray(gaps, dtype=np.uint32).tofile(f)
 
 
-def _run(tmp_path, d, model, mode, source):
+def _run(tmp_path, d, model, mode, source, protein=False):
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     _write_input(inp, d)
-    r = subprocess.run([DRIVER, str(inp), str(out), model, mode, source], capture_output=True,
-                       text=True, timeout=120)
+    extra = ["protein"] if protein else []
+    r = subprocess.run([DRIVER, str(inp), str(out), model, mode, source] + extra,
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = np.fromfile(out, dtype=np.uint32)
     n = int(raw[0])
@@ -66,3 +67,36 @@ def test_character_rows_path(tmp_path):
     d = dict(np.load(os.path.join(GOLDEN, "c1.npz"), allow_pickle=False))
     mat, _ = _run(tmp_path, d, "kim", "mem", "chars")
     np.testing.assert_allclose(mat, d["k2p"], rtol=1e-6, atol=0)
+
+
+GAP_CHARS = b"-.XxzZbBJj?"
+
+
+def _protein_loader_data(aln):
+    """The arrays the reference's FASTA loader keeps for a protein alignment: resolved characters
+    (gap characters as '-', dataloaderFasta.hpp resolveChar), padded with '-' to bitStringsCount
+    = L/16 + 8 blocks of 16, four to a u32 (encodeProteinSequence)."""
+    n, L = aln.shape
+    res = aln.copy()
+    res[np.isin(res, np.frombuffer(GAP_CHARS, dtype=np.uint8))] = ord("-")
+    bsc = L // 16 + 8
+    padded = np.full((n, bsc * 16), ord("-"), dtype=np.uint8)
+    padded[:, :L] = res
+    bits = padded.view("<u4").reshape(n, bsc * 4)
+    return {"aln": res, "bits": bits, "gaps": np.zeros_like(bits)}
+
+
+@pytest.mark.parametrize("model,key", [("kim", "kim"), ("jc", "jc")])
+@pytest.mark.parametrize("mode", ["mem", "disk", "half"])
+@pytest.mark.parametrize("source", ["fast", "chars"])
+def test_protein_loader(tmp_path, model, key, mode, source):
+    """A protein loader (type PROTEIN) gets the reference's protein distances through the same
+    classes: PF15224 from the reference's test data, and an alignment with a multiple-of-128
+    length and an all-gap row."""
+    g = dict(np.load(os.path.join(GOLDEN, "protein.npz"), allow_pickle=False))
+    for case in ("pf15224", "p20x128", "p10x200gap"):
+        d = _protein_loader_data(g[f"{case}_aln"])
+        mat, _ = _run(tmp_path, d, model, mode, source, protein=True)
+        want = g[f"{case}_{key}"]
+        np.testing.assert_allclose(mat, want, rtol=1e-6, atol=0)
+        np.testing.assert_array_equal(np.signbit(mat), np.signbit(want))
