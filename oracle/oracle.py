@@ -76,6 +76,12 @@ def lib() -> ctypes.CDLL:
         L.rnj_oracle_correct_triangle.argtypes = [_u32p, _u32p, _u32p, ctypes.c_uint, ctypes.c_int,
                                                   ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.POINTER(ctypes.c_float)]
+        L.rnj_oracle_nj.restype = ctypes.c_int
+        L.rnj_oracle_nj.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int] + \
+            [ctypes.c_void_p] * 5 + [ctypes.POINTER(ctypes.c_float)]
+        L.rnj_oracle_newick.restype = ctypes.c_size_t
+        L.rnj_oracle_newick.argtypes = [ctypes.c_uint] + [ctypes.c_void_p] * 5 + \
+            [ctypes.c_float, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -236,6 +242,39 @@ def protein_matrix(aln: np.ndarray, model: str):
     if rc != 0:
         raise RuntimeError("oracle protein matrix failed")
     return full, mx.value
+
+
+def nj(matrix: np.ndarray, negative_branches: bool = False):
+    """rapidNJ's joins on an n x n float32 matrix: (left, right, llen, rlen, last (2,), last_dist)."""
+    m = np.ascontiguousarray(matrix, dtype=np.float32)
+    n = m.shape[0]
+    k = max(n - 2, 1)
+    left, right = np.zeros(k, dtype=np.int32), np.zeros(k, dtype=np.int32)
+    llen, rlen = np.zeros(k, dtype=np.float64), np.zeros(k, dtype=np.float64)
+    last = np.zeros(2, dtype=np.int32)
+    ld = ctypes.c_float()
+    rc = lib().rnj_oracle_nj(m.ctypes.data, n, int(negative_branches), left.ctypes.data,
+                             right.ctypes.data, llen.ctypes.data, rlen.ctypes.data,
+                             last.ctypes.data, ctypes.byref(ld))
+    if rc != 0:
+        raise RuntimeError(f"oracle nj failed ({rc})")
+    return left[: n - 2], right[: n - 2], llen[: n - 2], rlen[: n - 2], last, ld.value
+
+
+def newick(n: int, tree, names=None) -> bytes:
+    """polytree::serialize_tree text of a tree from :func:`nj` (names default to t0..t{n-1})."""
+    left, right, llen, rlen, last, ld = tree
+    names = names or [b"t%d" % i for i in range(n)]
+    arr = (ctypes.c_char_p * n)(*names)
+    args = [n, np.ascontiguousarray(left, dtype=np.int32).ctypes.data,
+            np.ascontiguousarray(right, dtype=np.int32).ctypes.data,
+            np.ascontiguousarray(llen, dtype=np.float64).ctypes.data,
+            np.ascontiguousarray(rlen, dtype=np.float64).ctypes.data,
+            np.ascontiguousarray(last, dtype=np.int32).ctypes.data, ctypes.c_float(ld), arr]
+    need = lib().rnj_oracle_newick(*args, None, 0)
+    buf = ctypes.create_string_buffer(need + 1)
+    lib().rnj_oracle_newick(*args, buf, need + 1)
+    return buf.raw[:need]
 
 
 class FastaError(ValueError):

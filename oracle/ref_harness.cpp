@@ -26,7 +26,7 @@
 //   matrix FASTA jc|kim T OUT [naive] u32 n; then n*n f32 (post-processed matrix); prints seconds
 //   time   FASTA jc|kim T REPS        prints {"best_s":..,"mean_s":..,"n":..,"L":..,"threads":..}
 //   ptime  FASTA jc|kim T REPS        the same for a protein alignment
-//   nj     MATRIX OUT                 MATRIX = u32 n + n*n f32; writes Newick (rapidNJ) with
+//   nj     MATRIX OUT [neg]           MATRIX = u32 n + n*n f32; writes Newick (rapidNJ, negative_branches with neg) with
 //                                     leaf names "t<i>"
 //   tree   FASTA jc|kim T OUT         distance matrix + rapidNJ; writes Newick
 
@@ -238,7 +238,7 @@ int mode_time(int argc, char** argv) {
   return 0;
 }
 
-void run_nj(distType** m, int n, std::ostream& out) {
+void run_nj(distType** m, int n, std::ostream& out, bool negative_branches = false) {
   vector<string>* names = new vector<string>;
   for (int i = 0; i < n; i++) {
     std::ostringstream s;
@@ -248,14 +248,14 @@ void run_nj(distType** m, int n, std::ostream& out) {
   ProgressBar* pb = new ProgressBar();
   distMatrixReader* reader = new distMatrixReader(false, n, false, names, m);
   reader->initializeData();
-  rapidNJ* nj = new rapidNJ(reader, n, false, pb);
+  rapidNJ* nj = new rapidNJ(reader, n, negative_branches, pb);
   polytree* tree = nj->run();
   tree->serialize_tree(out);
   out.flush();
 }
 
 int mode_nj(int argc, char** argv) {
-  if (argc < 4) die("nj MATRIX OUT");
+  if (argc < 4) die("nj MATRIX OUT [neg]");
   std::ifstream in(argv[2], std::ios::binary);
   if (!in) die(std::string("cannot read ") + argv[2]);
   uint32_t n = 0;
@@ -267,7 +267,8 @@ int mode_nj(int argc, char** argv) {
   }
   if (!in) die("short matrix file");
   std::ofstream o(argv[3]);
-  run_nj(m, (int)n, o);  // rapidNJ's destructor path owns the rows; the process exits next
+  // rapidNJ's destructor path owns the rows; the process exits next
+  run_nj(m, (int)n, o, argc > 4 && std::string(argv[4]) == "neg");
   return 0;
 }
 

@@ -56,6 +56,19 @@ float rnj_oracle_jc_protein(uint64_t dist, uint64_t valid);
 int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model, int threads,
                               float* full, float* max_out);
 
+/* rapidNJ's neighbour joining (rapidNJ.cpp, restated in rnj_oracle_nj.c) on an n x n
+ * row-major matrix (copied): joins k = 0..n-3 as polytree::addInternalNode receives them
+ * (left[k], right[k] = the joined clusters' ids, llen/rlen their branch lengths; the new cluster
+ * is id n + k), then the last two cluster ids and their distance
+ * (polytree::set_serialization_indices). Returns 0. */
+int rnj_oracle_nj(const float* matrix, unsigned n, int negative_branches, int* left, int* right,
+                  double* llen, double* rlen, int* last, float* last_dist);
+/* polytree::serialize_tree (polytree.cpp:402-458, no bootstrap counts) of such a tree into out
+ * (NUL-terminated when it fits); returns the length the text needs. */
+size_t rnj_oracle_newick(unsigned n, const int* left, const int* right, const double* llen,
+                         const double* rlen, const int* last, float last_dist,
+                         const char* const* names, char* out, size_t cap);
+
 unsigned rnj_oracle_bitstrings_count(unsigned L);
 
 /* 1 if the first sequence classifies the alignment as DNA (dataloaderFasta.hpp:54-76). */

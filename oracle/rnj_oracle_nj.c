@@ -1,0 +1,369 @@
+/* CPU restatement of rapidNJ's neighbour joining (rapidNJ.cpp) and its Newick writer
+ * (polytree.cpp). TEST INFRASTRUCTURE ONLY, like rnj_oracle.c: only tests/ and bench.py's CPU leg
+ * load it, as the checker.
+ *
+ * rapidNJ keeps every row's partners sorted by distance and stops a row's scan at a bound; the
+ * global minimum it finds is the minimum of the criterion over all live pairs, and the pair it
+ * returns is the first one reaching it in its visiting order. This file restates that result
+ * directly (every live pair evaluated, no sorted rows), with the reference's arithmetic and
+ * visiting order:
+ *   - rows are visited by index, cyclically from the row the previous search cached
+ *     (rapidNJ.cpp:290-319), and a row's pairs in increasing distance (its sorted list, built by
+ *     std::sort on the distance only, cluster_pair.h:8); pairs of one row with equal distance
+ *     AND equal criterion are taken in increasing partner index here (the reference's order for
+ *     such exact ties is std::sort's);
+ *   - a pair (a, b) lives in the row of the cluster with the larger id, against the partner with
+ *     the smaller id (rows are built from lower ids, new clusters get the next id), and its
+ *     criterion is the float (d - sep[partner]) - sep[row] (rapidNJ.cpp:336);
+ *   - a row with redundant copies first offers the copy join at -2 * sep[row]
+ *     (rapidNJ.cpp:418-430);
+ *   - the caches of the rows where the minimum was last improved follow rapidNJ.cpp:337-345.
+ * Merging (rapidNJ.cpp:240-281), the update of distances and separations (rapidNJ.cpp:166-238, in
+ * id order, with float arithmetic and the sequential float sum of the new separation sum),
+ * redundancy detection (rapidNJ.cpp:91-138,155-164) and the last join (rapidNJ.cpp:52-85) are
+ * restated step for step. Pinned against the reference's own trees (tests/test_nj_oracle.py,
+ * tests/golden/nj.npz written by the compiled reference).
+ */
+#include <float.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "rnj_oracle.h"
+
+typedef struct {
+  unsigned n;
+  float* m;  /* n x n by index, updated in place */
+  float* sepsum;
+  float* sep;
+  int *index_to_id, *id_to_index, *red_count, *rtm, *red_head, *red_next;
+  int cluster_count, current_id;
+  int min1, min2, cache1, cache2;
+  float global_min;
+  int new_row, obs_row;
+  int negative_branches;
+} nj_state;
+
+/* redundantMap[i] as a singly linked list: push_front / front / pop_front */
+static void red_push(nj_state* s, int i, int j) {
+  s->red_next[j] = s->red_head[i];
+  s->red_head[i] = j;
+}
+static int red_pop(nj_state* s, int i) {
+  const int j = s->red_head[i];
+  s->red_head[i] = s->red_next[j];
+  return j;
+}
+
+/* rapidNJ.cpp:155-164 */
+static void handle_redundancy(nj_state* s, int i, int j) {
+  const unsigned n = s->n;
+  for (unsigned k = 0; k < n; k++)
+    if (s->m[(size_t)i * n + k] != s->m[(size_t)j * n + k]) return;
+  s->index_to_id[j] = -1;
+  s->rtm[j] = i;
+  red_push(s, i, j);
+  s->red_count[i]++;
+}
+
+/* rapidNJ.cpp:91-138 (the sorted rows are not built: the search below evaluates every pair) */
+static void initialize(nj_state* s) {
+  const unsigned n = s->n;
+  s->cluster_count = (int)n;
+  s->current_id = (int)n;
+  s->cache1 = s->cache2 = 0;
+  s->min1 = s->min2 = 0;
+  for (unsigned i = 0; i < n; i++) {
+    s->index_to_id[i] = (int)i;
+    s->id_to_index[i] = (int)i;
+    s->red_count[i] = 1;
+    s->rtm[i] = (int)i;
+    s->red_head[i] = -1;
+    float sum = 0;
+    for (unsigned j = 0; j < n; j++) sum += s->m[(size_t)i * n + j];
+    s->sepsum[i] = sum;
+    s->sep[i] = sum / (float)(s->cluster_count - 2);
+  }
+  for (int i = (int)n - 1; i >= 0; i--) {
+    if (s->index_to_id[i] == -1) continue;
+    for (int j = 0; j < i; j++)
+      if (s->m[(size_t)i * n + j] == 0) handle_redundancy(s, i, j);
+  }
+}
+
+/* a new minimum: rapidNJ.cpp:337-345 */
+static void improve(nj_state* s, int i, int j, float value) {
+  if (i != s->min1) {
+    s->cache2 = s->cache1;
+    s->cache1 = s->min1;
+  }
+  s->global_min = value;
+  s->min1 = i;
+  s->min2 = j;
+}
+
+/* rapidNJ.cpp:290-361 (findMin, findRowMin, searchRedundantEntries) as described above */
+static void find_min(nj_state* s) {
+  const unsigned n = s->n;
+  s->global_min = FLT_MAX;
+  const int start = s->index_to_id[s->cache1] != -1 ? s->cache1 : s->cache2;
+  int i = start;
+  do {
+    if (s->index_to_id[i] != -1) {
+      const float row_sep = s->sep[i];
+      if (s->red_count[i] > 1) {
+        const float value = -2 * s->sep[i];
+        if (value < s->global_min) improve(s, i, s->red_head[i], value);
+      }
+      const int id_i = s->index_to_id[i];
+      int best = -1;
+      float best_v = 0, best_d = 0;
+      for (unsigned j = 0; j < n; j++) {
+        const int id_j = s->index_to_id[j];
+        if (id_j == -1 || id_j >= id_i) continue;
+        const float d = s->m[(size_t)i * n + j];
+        float v = d - s->sep[s->rtm[j]];
+        v = v - row_sep;
+        if (best < 0 || v < best_v || (v == best_v && d < best_d)) {
+          best = (int)j;
+          best_v = v;
+          best_d = d;
+        }
+      }
+      if (best >= 0 && best_v < s->global_min) improve(s, i, best, best_v);
+    }
+    i = (i + 1) % (int)n;
+  } while (i != start);
+}
+
+/* rapidNJ.cpp:240-281; the join goes to the caller's arrays */
+static void merge_min_nodes(nj_state* s, int* left, int* right, double* llen, double* rlen,
+                            int step) {
+  const unsigned n = s->n;
+  const int r1 = s->rtm[s->min1], r2 = s->rtm[s->min2];
+  const double dist = s->m[(size_t)r1 * n + r2];
+  const double sep1 = s->sep[r1];
+  const double sep2 = s->sep[r2];
+  double dist1 = (0.5 * dist) + (0.5 * (sep1 - sep2));
+  double dist2 = (0.5 * dist) + (0.5 * (sep2 - sep1));
+  if (s->negative_branches) {
+    if (dist1 < 0.0) {
+      dist2 += dist1;
+      dist1 = 0;
+    }
+    if (dist2 < 0.0) {
+      dist1 += dist2;
+      if (dist1 < 0) dist1 = 0;
+      dist2 = 0;
+    }
+  }
+  s->new_row = r1;
+  s->obs_row = r2;
+  if (s->red_count[r1] != 1) {
+    s->new_row = red_pop(s, r1);
+    s->red_count[r1]--;
+    s->index_to_id[s->new_row] = s->new_row;
+  }
+  if (s->red_count[s->rtm[s->min2]] != 1) {
+    const int m2 = s->rtm[s->min2];
+    s->obs_row = red_pop(s, m2);
+    s->red_count[m2]--;
+    s->index_to_id[s->obs_row] = s->obs_row;
+  }
+  left[step] = s->index_to_id[s->new_row];
+  right[step] = s->index_to_id[s->obs_row];
+  llen[step] = dist1;
+  rlen[step] = dist2;
+}
+
+/* rapidNJ.cpp:166-238 (the sorted row of the new cluster is not built) */
+static void update_data(nj_state* s) {
+  const unsigned n = s->n;
+  float new_sepsum = 0;
+  const int r1 = s->rtm[s->min1], r2 = s->rtm[s->min2];
+  const float mutual = s->m[(size_t)r1 * n + r2];
+  const float* row1 = s->m + (size_t)r1 * n;
+  const float* row2 = s->m + (size_t)r2 * n;
+  float* new_row = s->m + (size_t)s->new_row * n;
+  for (int cur = 0; cur < s->current_id; cur++) {
+    const int i = s->id_to_index[cur];
+    if (i == -1 || s->index_to_id[i] == -1) continue;
+    if (i == s->new_row || i == s->obs_row) {
+      new_row[i] = 0;
+    } else {
+      const float val1 = row1[i];
+      const float val2 = row2[i];
+      const float t = val1 + val2 - mutual;
+      float dist = (float)((double)t / 2.0);
+      if (s->negative_branches && dist < 0) dist = 0;
+      new_sepsum += dist * (float)s->red_count[i];
+      s->sepsum[i] += (dist - val1 - val2);
+      s->sep[i] = s->sepsum[i] / (float)(s->cluster_count - 2);
+      new_row[i] = dist;
+      s->m[(size_t)i * n + s->new_row] = dist;
+    }
+  }
+  s->id_to_index[s->index_to_id[s->new_row]] = -1;
+  s->id_to_index[s->index_to_id[s->obs_row]] = -1;
+  s->index_to_id[s->obs_row] = -1;
+  s->sepsum[s->new_row] = new_sepsum;
+  s->sepsum[s->obs_row] = 0;
+  s->sep[s->new_row] = new_sepsum / (float)(s->cluster_count - 2);
+  s->sep[s->obs_row] = FLT_MIN;
+  s->rtm[s->new_row] = s->new_row;
+  s->rtm[s->obs_row] = s->obs_row;
+  const int next_id = s->current_id++;
+  s->index_to_id[s->new_row] = next_id;
+  s->id_to_index[next_id] = s->new_row;
+}
+
+int rnj_oracle_nj(const float* matrix, unsigned n, int negative_branches, int* left, int* right,
+                  double* llen, double* rlen, int* last, float* last_dist) {
+  if (n < 2) return -1;
+  nj_state s;
+  memset(&s, 0, sizeof s);
+  s.n = n;
+  s.negative_branches = negative_branches;
+  s.m = (float*)malloc((size_t)n * n * sizeof(float));
+  s.sepsum = (float*)malloc(n * sizeof(float));
+  s.sep = (float*)malloc(n * sizeof(float));
+  s.index_to_id = (int*)malloc(n * sizeof(int));
+  s.id_to_index = (int*)malloc(2 * (size_t)n * sizeof(int));
+  s.red_count = (int*)malloc(n * sizeof(int));
+  s.rtm = (int*)malloc(n * sizeof(int));
+  s.red_head = (int*)malloc(n * sizeof(int));
+  s.red_next = (int*)malloc(n * sizeof(int));
+  if (!s.m || !s.sepsum || !s.sep || !s.index_to_id || !s.id_to_index || !s.red_count || !s.rtm ||
+      !s.red_head || !s.red_next)
+    return -2;
+  memcpy(s.m, matrix, (size_t)n * n * sizeof(float));
+  for (size_t k = 0; k < 2 * (size_t)n; k++) s.id_to_index[k] = -1;
+  initialize(&s);
+  int step = 0;
+  while (s.cluster_count > 2) {
+    find_min(&s);
+    merge_min_nodes(&s, left, right, llen, rlen, step++);
+    s.cluster_count--;
+    update_data(&s);
+  }
+  /* rapidNJ.cpp:52-85 */
+  int index1 = -1, index2 = -1;
+  for (unsigned i = 0; i < n; i++) {
+    if (s.index_to_id[i] != -1) {
+      if (index1 == -1) {
+        if (s.red_count[i] == 2) {
+          index1 = (int)i;
+          index2 = s.red_head[i];
+          s.index_to_id[index2] = index2;
+          break;
+        } else {
+          index1 = (int)i;
+        }
+      } else {
+        index2 = (int)i;
+        break;
+      }
+    }
+  }
+  int rc = 0;
+  if (index1 == -1 || index2 == -1) {
+    rc = -3;
+  } else {
+    last[0] = s.index_to_id[index1];
+    last[1] = s.index_to_id[index2];
+    *last_dist = s.m[(size_t)index1 * n + index2];
+  }
+  free(s.m);
+  free(s.sepsum);
+  free(s.sep);
+  free(s.index_to_id);
+  free(s.id_to_index);
+  free(s.red_count);
+  free(s.rtm);
+  free(s.red_head);
+  free(s.red_next);
+  return rc;
+}
+
+/* ---- Newick (polytree.cpp:39-51, 402-458 without bootstrap counts) ------------------------ */
+typedef struct {
+  char* out;
+  size_t cap, len;
+} sbuf;
+
+static void put(sbuf* b, const char* s, size_t k) {
+  if (b->len + k < b->cap) memcpy(b->out + b->len, s, k);
+  b->len += k;
+}
+static void puts_(sbuf* b, const char* s) { put(b, s, strlen(s)); }
+static void put_len(sbuf* b, double x) {
+  char tmp[64];
+  const int k = snprintf(tmp, sizeof tmp, "%.5g", x);
+  put(b, tmp, (size_t)k);
+}
+static void put_leaf(sbuf* b, const char* name, double len) {
+  puts_(b, "'");
+  puts_(b, name);
+  puts_(b, "':");
+  put_len(b, len);
+}
+
+typedef struct {
+  unsigned n;
+  const int *left, *right;
+  const double* dist; /* by node id (leaves 0..n-1, internal n..2n-2) */
+  const char* const* names;
+} tree_view;
+
+static void node(const tree_view* t, sbuf* b, int l, int r) {
+  const unsigned n = t->n;
+  if ((unsigned)l < n) {
+    put_leaf(b, t->names[l], t->dist[l]);
+  } else {
+    puts_(b, "(");
+    node(t, b, t->left[l - n], t->right[l - n]);
+    puts_(b, "):");
+    put_len(b, t->dist[l]);
+  }
+  puts_(b, ",");
+  if ((unsigned)r < n) {
+    put_leaf(b, t->names[r], t->dist[r]);
+  } else {
+    puts_(b, "(");
+    node(t, b, t->left[r - n], t->right[r - n]);
+    puts_(b, "):");
+    put_len(b, t->dist[r]);
+  }
+}
+
+size_t rnj_oracle_newick(unsigned n, const int* left, const int* right, const double* llen,
+                         const double* rlen, const int* last, float last_dist,
+                         const char* const* names, char* out, size_t cap) {
+  double* dist = (double*)calloc(2 * (size_t)n, sizeof(double));
+  if (!dist) return 0;
+  for (unsigned k = 0; k + 2 < n; k++) {  /* addInternalNode: distances by child id */
+    dist[left[k]] = llen[k];
+    dist[right[k]] = rlen[k];
+  }
+  tree_view t = {n, left, right, dist, names};
+  sbuf b = {out, cap, 0};
+  const int sl = last[0], sr = last[1];
+  puts_(&b, "(");
+  if ((unsigned)sl < n) {
+    put_leaf(&b, names[sl], last_dist);
+  } else {
+    if ((unsigned)sr >= n) puts_(&b, "(");
+    node(&t, &b, left[sl - n], right[sl - n]);
+    if ((unsigned)sr >= n) {
+      puts_(&b, "):");
+      put_len(&b, last_dist);
+    }
+  }
+  puts_(&b, ",");
+  if ((unsigned)sr < n) put_leaf(&b, names[sr], last_dist);
+  else node(&t, &b, left[sr - n], right[sr - n]);
+  puts_(&b, ");\n");
+  if (b.len < b.cap) b.out[b.len] = 0;
+  free(dist);
+  return b.len;
+}

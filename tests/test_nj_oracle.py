@@ -1,0 +1,64 @@
+"""The neighbour-joining oracle (oracle/rnj_oracle_nj.c) against the reference's own trees.
+
+tests/golden/nj.npz holds distance matrices computed by the reference and the Newick text its
+rapidNJ builds on them (tests/golden/make_nj_golden.py), with negative_branches off and on;
+tests/golden/lgt70.npz holds the reference's trees of its own LGT_70 test alignment. The
+oracle's text must be byte-identical: same joins in the same order, same branch lengths to the
+printed precision, same layout.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def nj_golden():
+    return dict(np.load(os.path.join(GOLDEN, "nj.npz"), allow_pickle=False))
+
+
+def matrix_of(g, case):
+    if f"{case}_matrix" in g:
+        return g[f"{case}_matrix"]
+    return O.distance_matrix(g[f"{case}_aln"], "k2p")[1]  # the reference's K2P bits
+
+
+@pytest.mark.parametrize("case", ["dup", "short", "sat", "sim300", "sim300jc", "sim800"])
+@pytest.mark.parametrize("neg", [False, True])
+def test_reference_trees(nj_golden, case, neg):
+    m = matrix_of(nj_golden, case)
+    want = bytes(nj_golden[f"{case}_newick_neg" if neg else f"{case}_newick"])
+    assert O.newick(m.shape[0], O.nj(m, neg)) == want
+
+
+@pytest.mark.parametrize("model", ["k2p", "jc"])
+def test_lgt70(model):
+    d = np.load(os.path.join(GOLDEN, "lgt70.npz"), allow_pickle=False)
+    m = d[model]
+    assert O.newick(m.shape[0], O.nj(m)).rstrip(b"\n") == bytes(d[f"newick_{model}"])
+
+
+def test_redundant_copies_join_at_zero_length(nj_golden):
+    """The fixture has identical rows, and copies are joined to each other with both branches of
+    length 0 (equal separations, distance 0)."""
+    m = nj_golden["dup_matrix"]
+    left, right, llen, rlen, last, _ = O.nj(m)
+    assert len({tuple(r) for r in m}) < m.shape[0]
+    assert np.any((llen == 0) & (rlen == 0))
+
+
+def test_join_invariants():
+    """Every id is joined exactly once; new clusters are numbered n, n+1, ..."""
+    rng = np.random.default_rng(3)
+    n = 40
+    x = rng.random((n, 5))
+    m = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)).astype(np.float32)
+    left, right, llen, rlen, last, _ = O.nj(m)
+    ids = list(left) + list(right) + list(last)
+    assert sorted(ids) == list(range(2 * n - 2))
+    for k in range(n - 2):
+        assert left[k] < n + k and right[k] < n + k
