@@ -245,6 +245,35 @@ RNJ_EXPORT int rnj_bootstrap_plan_run(rnj_plan* plan, const uint16_t* weights, v
 RNJ_EXPORT int rnj_plan_tile_config(const rnj_plan* plan); /* configuration the graph uses */
 RNJ_EXPORT int rnj_plan_destroy(rnj_plan* plan);
 
+/* ---- neighbour joining on the device ----------------------------------------------------------
+ * rapidNJ::run (src/rapidNJ.cpp:45-88) with the matrix kept in HBM: the same joins in the same
+ * order (the criterion's float arithmetic, rapidNJ's visiting order and row caches, redundant
+ * copies, negative_branches), giving polytree's Newick text (src/polytree.cpp:402-458). For
+ * exact ties in both distance and criterion within one row the reference's order is std::sort's;
+ * here the partner with the lower index comes first. Replaces `new rapidNJ(reader, n,
+ * negative_branches, pb)->run()` (src/main.cpp) for in-memory matrices. */
+typedef struct rnj_tree rnj_tree;
+/* matrix: n rows of ld floats in device memory (post-processed, symmetric, diagonal 0), read
+ * only; enqueued on `stream` after the work already there, synchronous. n >= 3. */
+RNJ_EXPORT int rnj_nj_device(rnj_context* ctx, const float* matrix, unsigned n, size_t ld,
+                             int negative_branches, void* stream, rnj_tree** out);
+/* Distances (model RNJ_K2P / RNJ_JC, with the exact FP64 logs the host hand-off uses) and
+ * neighbour joining without the matrix leaving HBM; leaf names are the alignment's FASTA names. */
+RNJ_EXPORT int rnj_nj_alignment(rnj_context* ctx, const rnj_alignment* aln, int model,
+                                int negative_branches, rnj_tree** out);
+RNJ_EXPORT unsigned rnj_tree_size(const rnj_tree* tree); /* leaves */
+/* Join k (0 <= k < n-2) joins clusters left[k] and right[k] (leaves 0..n-1, join k creates id
+ * n+k) with branch lengths left_len[k], right_len[k] (polytree::addInternalNode); last[0..1] and
+ * last_dist are the final two clusters and their distance. Any pointer may be NULL. */
+RNJ_EXPORT int rnj_tree_joins(const rnj_tree* tree, int* left, int* right, double* left_len,
+                              double* right_len, int* last, float* last_dist);
+/* Newick text as polytree::serialize_tree writes it (branch lengths "%.5g", names quoted, ";\n"):
+ * names[i] (or, when names is NULL, the alignment's names, else "t<i>"). Writes up to cap-1
+ * bytes and a NUL; *length receives the full length. */
+RNJ_EXPORT int rnj_tree_newick(const rnj_tree* tree, const char* const* names, char* out,
+                               size_t cap, size_t* length);
+RNJ_EXPORT int rnj_tree_free(rnj_tree* tree);
+
 /* ---- served jobs ----------------------------------------------------------------------------
  * A request loop on its own thread admits submitted jobs while their HBM need fits the budget,
  * batches the admitted jobs into one multi-job tile launch per batch, keeps up to `slots` batches

@@ -1,0 +1,69 @@
+// Device neighbour joining (nj.hip) and its host driver (nj.cpp). Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rnj {
+
+// Scalar state of one run, in device memory (rapidNJ's members, rapidNJ.h).
+struct NjState {
+  int n;
+  int used;           // slots in use (slot order = cluster id order)
+  int alive;          // live slots after a compaction
+  int cluster_count;  // clusterCount (copies included)
+  int current_id;     // next cluster id
+  int min1, min2, cache1, cache2;  // findMin's result and row caches (rapidNJ.cpp:290-345)
+  int new_row, obs_row;
+  int step;           // joins made
+  int negative_branches;
+  int error;          // set when an iteration finds no pair or runs out of slots (then all stop)
+};
+
+// Device arrays of one run, passed by value to the kernels.
+struct NjDevice {
+  NjState* st;
+  size_t ld;           // row stride of the slot matrices (= slot capacity)
+  // by matrix index (n), rapidNJ's bookkeeping
+  int* index_to_id;
+  int* id_to_index;    // 2n
+  int* red_count;      // redundantCount
+  int* rtm;            // redundantToMasterMap
+  int* red_head;       // redundantMap[i] as a linked list: head, next
+  int* red_next;
+  int* slot_of_index;  // slot of a live index, -1 otherwise
+  // by slot (capacity ld)
+  int* slot_index;
+  int* alive;
+  float* sepsum;       // separationsums
+  float* sep;          // separations
+  float* row_v;        // best criterion of each row (scan)
+  int* row_p;          // its partner index
+  float* newcol;       // new cluster's distances (step)
+  float* term;         // dist * redundantCount terms of the new separation sum
+  int* map;            // compaction: live slots in order
+  // joins (polytree::addInternalNode arguments), n - 2 of them
+  int* left;
+  int* right;
+  double* llen;
+  double* rlen;
+};
+
+hipError_t nj_launch_row_sums(const float* M, size_t ldm, int n, float* sums, hipStream_t s);
+// Lists pairs (i, j), j < i, with M[i][j] == 0 (up to cap; *count = all of them).
+hipError_t nj_launch_zero_pairs(const float* M, size_t ldm, int n, int* pairs, int cap, int* count,
+                                hipStream_t s);
+// same[k] = 1 when the rows of pair k are identical.
+hipError_t nj_launch_same_rows(const float* M, size_t ldm, int n, const int* pairs, int npairs,
+                               int* same, hipStream_t s);
+hipError_t nj_launch_gather(const float* M, size_t ldm, const int* vis, int v, float* S, size_t ld,
+                            hipStream_t s);
+// One iteration (scan + step) over slots < used_bound.
+hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, hipStream_t s);
+// Rebuild the slot matrix into S2 from the live slots (at most alive_bound of them).
+hipError_t nj_launch_compact(const NjDevice& D, const float* S, float* S2, int alive_bound,
+                             hipStream_t s);
+
+}  // namespace rnj

@@ -77,6 +77,12 @@ SIGNATURES = {
     "rnj_bootstrap_plan_run": (ctypes.c_int, [_vp, _vp, _vp]),
     "rnj_plan_tile_config": (ctypes.c_int, [_vp]),
     "rnj_plan_destroy": (ctypes.c_int, [_vp]),
+    "rnj_nj_device": (ctypes.c_int, [_vp, _vp, _u, _sz, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    "rnj_nj_alignment": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "rnj_tree_size": (_u, [_vp]),
+    "rnj_tree_joins": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rnj_tree_newick": (ctypes.c_int, [_vp, _vp, _vp, _sz, ctypes.POINTER(_sz)]),
+    "rnj_tree_free": (ctypes.c_int, [_vp]),
     "rnj_server_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
     "rnj_server_submit": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_uint64)]),
@@ -411,9 +417,69 @@ class Context:
         """A captured (HIP graph) batched-bootstrap launch writing R matrices into `out`."""
         return BootstrapPlan(self, aln, R, model, layout, out, max_length, raw, exact)
 
+    def nj(self, matrix, negative_branches: bool = False, stream=None) -> Tree:
+        """rapidNJ on a square float32 device matrix (torch tensor (n, ld>=n) on this device)."""
+        import torch
+        assert matrix.dtype == torch.float32 and matrix.is_cuda and matrix.dim() == 2
+        assert matrix.stride(1) == 1
+        n = matrix.shape[0]
+        h = _vp()
+        check(lib().rnj_nj_device(self.handle, matrix.data_ptr(), n, matrix.stride(0),
+                                  int(negative_branches), _stream_handle(stream), ctypes.byref(h)))
+        return Tree(h.value)
+
+    def nj_alignment(self, aln: Alignment, model="k2p", negative_branches: bool = False) -> Tree:
+        """Distances and neighbour joining with the matrix kept in HBM."""
+        h = _vp()
+        check(lib().rnj_nj_alignment(self.handle, aln.handle, _model(model),
+                                     int(negative_branches), ctypes.byref(h)))
+        return Tree(h.value)
+
     def server(self, slots: int = 2, max_batch_jobs: int = 64, hbm_budget: int = 0,
                batch_window_us: int = 0) -> "Server":
         return Server(self, slots, max_batch_jobs, hbm_budget, batch_window_us)
+
+
+class Tree:
+    """A neighbour-joining tree built on the device (rnj_nj_device / rnj_nj_alignment)."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+        self.n = lib().rnj_tree_size(handle)
+
+    def joins(self):
+        """(left, right, left_len, right_len, last (2,), last_dist) as polytree receives them."""
+        k = max(self.n - 2, 1)
+        left, right = np.zeros(k, dtype=np.int32), np.zeros(k, dtype=np.int32)
+        ll, rl = np.zeros(k, dtype=np.float64), np.zeros(k, dtype=np.float64)
+        last = np.zeros(2, dtype=np.int32)
+        ld = ctypes.c_float()
+        check(lib().rnj_tree_joins(self.handle, left.ctypes.data, right.ctypes.data,
+                                   ll.ctypes.data, rl.ctypes.data, last.ctypes.data,
+                                   ctypes.byref(ld)))
+        m = self.n - 2
+        return left[:m], right[:m], ll[:m], rl[:m], last, ld.value
+
+    def newick(self, names=None) -> bytes:
+        arr = None
+        if names is not None:
+            arr = (ctypes.c_char_p * self.n)(*[bytes(x) for x in names])
+        length = _sz()
+        check(lib().rnj_tree_newick(self.handle, arr, None, 0, ctypes.byref(length)))
+        buf = ctypes.create_string_buffer(length.value + 1)
+        check(lib().rnj_tree_newick(self.handle, arr, buf, length.value + 1, ctypes.byref(length)))
+        return buf.raw[: length.value]
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rnj_tree_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
 
 
 class BootstrapPlan:

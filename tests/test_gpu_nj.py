@@ -1,0 +1,123 @@
+"""Neighbour joining on the device (rnj_nj_device / rnj_nj_alignment) against the reference's
+own trees and the oracle.
+
+tests/golden/nj.npz and lgt70.npz hold matrices the reference computed and the Newick text its
+rapidNJ built on them (tests/golden/make_nj_golden.py): the device text must be byte-identical,
+with negative_branches off and on, across redundant copies, equal distances, saturated pairs and
+slot-matrix compactions. Larger trees are compared join by join with the oracle
+(oracle/rnj_oracle_nj.c), itself pinned to those trees by tests/test_nj_oracle.py.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rapidnj_amd.workloads import simulate_alignment
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def N():
+    from rapidnj_amd import native
+    return native
+
+
+@pytest.fixture(scope="module")
+def ctx(N):
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def nj_golden():
+    return dict(np.load(os.path.join(GOLDEN, "nj.npz"), allow_pickle=False))
+
+
+def device(m):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(m, dtype=np.float32)).to("cuda:0")
+
+
+def same_joins(tree, ref):
+    left, right, ll, rl, last, ld = tree.joins()
+    rleft, rright, rll, rrl, rlast, rld = ref
+    np.testing.assert_array_equal(left, rleft)
+    np.testing.assert_array_equal(right, rright)
+    np.testing.assert_array_equal(ll.view(np.uint64), rll.view(np.uint64))
+    np.testing.assert_array_equal(rl.view(np.uint64), rrl.view(np.uint64))
+    np.testing.assert_array_equal(last, rlast)
+    assert np.float32(ld).tobytes() == np.float32(rld).tobytes()
+
+
+@pytest.mark.parametrize("case", ["dup", "short", "sat", "sim300", "sim300jc"])
+@pytest.mark.parametrize("neg", [False, True])
+def test_reference_trees(ctx, nj_golden, case, neg):
+    m = nj_golden[f"{case}_matrix"]
+    want = bytes(nj_golden[f"{case}_newick_neg" if neg else f"{case}_newick"])
+    tree = ctx.nj(device(m), negative_branches=neg)
+    assert tree.newick() == want
+    same_joins(tree, O.nj(m, neg))
+
+
+@pytest.mark.parametrize("model", ["k2p", "jc"])
+def test_lgt70(ctx, model):
+    d = np.load(os.path.join(GOLDEN, "lgt70.npz"), allow_pickle=False)
+    tree = ctx.nj(device(d[model]))
+    assert tree.newick().rstrip(b"\n") == bytes(d[f"newick_{model}"])
+    # the whole pipeline from the alignment, matrix kept in HBM
+    A = ctx.alignment(d["aln"])
+    assert ctx.nj_alignment(A, model).newick().rstrip(b"\n") == bytes(d[f"newick_{model}"])
+
+
+def test_alignment_pipeline_800(ctx, nj_golden):
+    A = ctx.alignment(nj_golden["sim800_aln"])
+    for neg in (False, True):
+        want = bytes(nj_golden["sim800_newick_neg" if neg else "sim800_newick"])
+        assert ctx.nj_alignment(A, "k2p", negative_branches=neg).newick() == want
+
+
+@pytest.mark.parametrize("n,L,seed", [(1500, 600, 7), (700, 60, 8)])
+def test_oracle_larger(ctx, n, L, seed):
+    """Join-by-join equality with the oracle on larger trees (many compactions; the short
+    alignment has many equal distances)."""
+    aln = simulate_alignment(n, L, 0.03, seed)
+    aln[5] = aln[17]          # redundant copies
+    aln[n - 1] = aln[17]
+    _, m, _ = O.distance_matrix(aln, "k2p")
+    same_joins(ctx.nj(device(m)), O.nj(m))
+
+
+def test_strided_matrix_and_names(ctx):
+    import torch
+    d = np.load(os.path.join(GOLDEN, "lgt70.npz"), allow_pickle=False)
+    m = d["k2p"]
+    n = m.shape[0]
+    big = torch.full((n, n + 37), 7.0, dtype=torch.float32, device="cuda:0")
+    big[:, :n] = device(m)
+    tree = ctx.nj(big)
+    assert tree.newick().rstrip(b"\n") == bytes(d["newick_k2p"])
+    names = [b"seq_%d" % i for i in range(n)]
+    text = tree.newick(names)
+    assert b"'seq_5'" in text and b"'t5'" not in text
+
+
+def test_fasta_names(ctx):
+    aln = simulate_alignment(12, 200, 0.0, 9)
+    fa = b"".join(b">taxon_%d desc\n" % i + r.tobytes() + b"\n" for i, r in enumerate(aln))
+    A = ctx.alignment_from_fasta(fa)
+    text = ctx.nj_alignment(A, "k2p").newick()
+    for i in range(12):  # a FASTA name runs to the first byte below 32, spaces included
+        assert b"'taxon_%d desc'" % i in text
+
+
+def test_small_and_invalid(ctx, N):
+    m = np.array([[0, 1, 2], [1, 0, 3], [2, 3, 0]], dtype=np.float32)
+    tree = ctx.nj(device(m))
+    assert tree.newick() == O.newick(3, O.nj(m))
+    with pytest.raises(N.RnjError):
+        ctx.nj(device(np.zeros((2, 2), dtype=np.float32)))
