@@ -39,7 +39,9 @@ struct DevBuf {
 
 int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_branches,
            hipStream_t s, rnj_tree* t) {
-  const size_t cap = (size_t)n + std::max<size_t>(64, n / 2);  // slots
+  // slots; a multiple of 64 so that every row starts 256-byte aligned and 4-wide loads stay in
+  // their row
+  const size_t cap = ((size_t)n + std::max<size_t>(64, n / 2) + 63) / 64 * 64;
   // device state
   DevBuf<NjState> st;
   DevBuf<int> index_to_id, id_to_index, red_count, rtm, red_head, red_next, slot_of_index;
@@ -210,14 +212,18 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   float* S = S0.p;
   float* S2 = S1.p;
   size_t used_bound = (size_t)v;
+  bool pending = false;  // the last join's new cluster still needs its separation and row
   for (unsigned it = 0; it + 2 < n; ++it) {
     const size_t cc = n - it;  // clusterCount before this join
     if (used_bound + 1 > cap || (used_bound > 256 && used_bound * 4 > cc * 5)) {
+      if (pending) NJ_HIP(nj_launch_finish(D, S, s));
+      pending = false;
       NJ_HIP(nj_launch_compact(D, S, S2, (int)std::min(cc, used_bound), s));
       std::swap(S, S2);
       used_bound = std::min(cc, used_bound);
     }
-    NJ_HIP(nj_launch_iteration(D, S, (int)used_bound, s));
+    NJ_HIP(nj_launch_iteration(D, S, (int)used_bound, pending, s));
+    pending = true;
     ++used_bound;
   }
   // results

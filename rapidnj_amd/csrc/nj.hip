@@ -25,7 +25,7 @@
 namespace rnj {
 namespace {
 
-constexpr int kScanThreads = 256;
+constexpr int kScanThreads = 256;  // 4 waves, one row each
 constexpr int kStepThreads = 1024;
 
 // Floats as order-preserving unsigned keys (after -0 -> +0, so that equal values compare equal
@@ -35,73 +35,179 @@ __device__ __forceinline__ uint32_t fkey(float x) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-struct Cand {  // lexicographic (v, d, p)
-  uint32_t v, d;
+// A row's candidate: criterion v, distance d, partner index p; ordered by (v, d, p) with NaN v
+// (gone partners) never better.
+struct Cand {
+  float v, d;
   int p;
 };
 
 __device__ __forceinline__ bool cand_less(const Cand& x, const Cand& y) {
-  if (x.v != y.v) return x.v < y.v;
-  if (x.d != y.d) return x.d < y.d;
+  if (x.v < y.v) return true;
+  if (!(x.v == y.v)) return false;
+  if (x.d < y.d) return true;
+  if (!(x.d == y.d)) return false;
   return x.p < y.p;
 }
 
-// Best pair of every live row slot: the row's first candidate reaching its minimum, in the
-// reference's order within a row. row_v = +inf marks a row without candidates.
-__global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const float* __restrict__ S) {
-  const NjState& st = *D.st;
-  const int a = blockIdx.x;
-  if (a >= st.used || st.error) return;
-  const int tid = threadIdx.x;
-  if (!D.alive[a]) {
-    if (tid == 0) D.row_v[a] = __int_as_float(0x7f800000);
-    return;
+__device__ __forceinline__ Cand wave_min(Cand c) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    Cand o{__shfl_xor(c.v, off), __shfl_xor(c.d, off), __shfl_xor(c.p, off)};
+    if (cand_less(o, c)) c = o;
   }
-  const float* row = S + (size_t)a * D.ld;
-  const float s_a = D.sep[a];
-  Cand best{0xffffffffu, 0xffffffffu, 0x7fffffff};
-  float best_v = __int_as_float(0x7f800000);
-  for (int b = tid; b < a; b += kScanThreads) {
-    if (!D.alive[b]) continue;
-    const float d = row[b];
-    const float v = __fsub_rn(__fsub_rn(d, D.sep[b]), s_a);
-    const Cand c{fkey(v), fkey(d), D.slot_index[b]};
-    if (cand_less(c, best)) {
-      best = c;
-      best_v = v;
+  return c;
+}
+
+constexpr float kInf = __builtin_inff();
+
+// The row's answer as rapidNJ's findRowMin gives it: the redundant-copy join first when the row
+// has copies (searchRedundantEntries), then the best pair unless it is not strictly better.
+__device__ __forceinline__ void write_row(const NjDevice& D, int a, const Cand& best, float s_a) {
+  const int idx = D.slot_index[a];
+  float v = best.p >= 0 ? best.v : kInf;
+  int p = best.p;
+  if (D.red_count[idx] > 1) {
+    const float rv = -2.0f * s_a;
+    if (!(best.p >= 0 && best.v < rv)) {
+      v = rv;
+      p = D.red_head[idx];
     }
   }
-  __shared__ Cand sc[kScanThreads];
-  __shared__ float sv[kScanThreads];
-  sc[tid] = best;
-  sv[tid] = best_v;
-  __syncthreads();
-  for (int off = kScanThreads / 2; off > 0; off >>= 1) {
-    if (tid < off && cand_less(sc[tid + off], sc[tid])) {
-      sc[tid] = sc[tid + off];
-      sv[tid] = sv[tid + off];
+  D.row_v[a] = v;
+  D.row_p[a] = p;
+}
+
+// One wave scans row slot a: partners are the slots b < a (NaN separation: gone).
+__device__ void scan_row(const NjDevice& D, const float* __restrict__ S, int a, int lane) {
+  const float s_a = D.sep[a];
+  const float* row = S + (size_t)a * D.ld;
+  Cand best{kInf, kInf, -1};
+  int best_b = -1;
+  for (int b0 = 4 * lane; b0 < a; b0 += 256) {
+    const float4 dv = *reinterpret_cast<const float4*>(row + b0);
+    const float4 sv = *reinterpret_cast<const float4*>(D.sep + b0);
+    const float dd[4] = {dv.x, dv.y, dv.z, dv.w};
+    const float ss[4] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int b = b0 + e;
+      if (b >= a) break;
+      const float v = __fsub_rn(__fsub_rn(dd[e], ss[e]), s_a) + 0.0f;
+      const float d = dd[e] + 0.0f;
+      if (v < best.v || (v == best.v && d < best.d)) {
+        best.v = v;
+        best.d = d;
+        best_b = b;
+        best.p = 0x7fffffff;  // index looked up below (ties only need it)
+      } else if (v == best.v && d == best.d && best_b >= 0) {
+        // exact tie within this lane: the lower partner index first
+        if (D.slot_index[b] < D.slot_index[best_b]) best_b = b;
+      }
+    }
+  }
+  best.p = best_b >= 0 ? D.slot_index[best_b] : 0x7fffffff;
+  if (best_b < 0) best.v = kInf;
+  best = wave_min(best);
+  if (lane == 0) {
+    if (!(best.v < kInf)) best.p = -1;
+    write_row(D, a, best, s_a);
+  }
+}
+
+// Finish the previous join's new cluster (slot used - 1), on one work-group: the reference's
+// sequential float sum of its separation sum in id order (terms of gone or merged slots are +0,
+// which leaves a sum that is never -0 unchanged), its separation, and its row's best pair.
+__device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ S) {
+  NjState& st = *D.st;
+  const int ns = st.used - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int kChunk = 4096;
+  __shared__ float terms[kChunk];
+  __shared__ float s_sum;
+  float acc = 0.0f;
+  for (int base = 0; base < ns; base += kChunk) {
+    const int cnt = min(kChunk, ns - base);
+    for (int k = tid; k < kChunk; k += kScanThreads) terms[k] = k < cnt ? D.term[base + k] : 0.0f;
+    __syncthreads();
+    if (wave == 0) {
+      const float4* t4 = reinterpret_cast<const float4*>(terms);
+      for (int k = 0; k < (cnt + 3) / 4; ++k) {
+        const float4 x = t4[k];
+        acc = __fadd_rn(acc, x.x);
+        acc = __fadd_rn(acc, x.y);
+        acc = __fadd_rn(acc, x.z);
+        acc = __fadd_rn(acc, x.w);
+      }
     }
     __syncthreads();
   }
   if (tid == 0) {
-    const int idx = D.slot_index[a];
-    float v = sv[0];
-    int p = sc[0].p;
-    int red = 0;
-    if (D.red_count[idx] > 1) {  // searchRedundantEntries comes first in the row
-      const float rv = -2.0f * s_a;
-      if (!(sc[0].v != 0xffffffffu && v < rv)) {
-        v = rv;
-        p = D.red_head[idx];
-        red = 1;
-      }
-    }
-    D.row_v[a] = sc[0].v == 0xffffffffu && !red ? __int_as_float(0x7f800000) : v;
-    D.row_p[a] = p;
+    s_sum = acc;
+    D.sepsum[ns] = acc;
+    D.sep[ns] = acc / (float)(st.cluster_count - 2);
   }
+  __syncthreads();
+  const float s_ns = D.sep[ns];
+  // the new row: partners are every live slot below it
+  Cand best{kInf, kInf, -1};
+  int best_b = -1;
+  for (int b = tid; b < ns; b += kScanThreads) {
+    const float d = D.newcol[b] + 0.0f;
+    const float v = __fsub_rn(__fsub_rn(d, D.sep[b]), s_ns) + 0.0f;
+    if (v < best.v || (v == best.v && d < best.d)) {
+      best.v = v;
+      best.d = d;
+      best_b = b;
+    }  // within a thread b increases with the index order only by chance: ties resolved below
+    else if (v == best.v && d == best.d && best_b >= 0 && D.slot_index[b] < D.slot_index[best_b]) {
+      best_b = b;
+    }
+  }
+  best.p = best_b >= 0 ? D.slot_index[best_b] : 0x7fffffff;
+  if (best_b < 0) best.v = kInf;
+  best = wave_min(best);
+  __shared__ Cand wbest[kScanThreads / 64];
+  if (lane == 0) wbest[wave] = best;
+  __syncthreads();
+  if (tid == 0) {
+    Cand b = wbest[0];
+    for (int w = 1; w < kScanThreads / 64; ++w)
+      if (cand_less(wbest[w], b)) b = wbest[w];
+    if (!(b.v < kInf)) b.p = -1;
+    write_row(D, ns, b, s_ns);
+  }
+  (void)s_sum;
 }
 
-// ---- block reductions for the one-work-group step ----------------------------------------------
+// Launch A: work-group 0 finishes the previous join's new cluster when asked; the others scan
+// one row per wave (rows of the live slots except that new one).
+__global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const float* __restrict__ S,
+                                                               int finish) {
+  const NjState& st = *D.st;
+  if (st.error) return;
+  int blk = blockIdx.x;
+  if (finish) {
+    if (blk == 0) {
+      finish_new_cluster(D, S);
+      return;
+    }
+    --blk;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int a = blk * (kScanThreads / 64) + wave;
+  const int used = st.used;
+  if (a >= used || (finish && a == used - 1)) return;
+  if (!D.alive[a]) return;
+  scan_row(D, S, a, lane);
+}
+
+__global__ __launch_bounds__(kScanThreads) void nj_finish_kernel(NjDevice D, const float* __restrict__ S) {
+  if (D.st->error) return;
+  finish_new_cluster(D, S);
+}
+
+// ---- the one-work-group step ------------------------------------------------------------------
 struct RowKey {  // (v, cyc) lexicographic; slot -1 = none
   uint32_t v;
   int cyc, slot;
@@ -114,30 +220,34 @@ __device__ __forceinline__ bool rk_less(const RowKey& x, const RowKey& y) {
   return x.cyc < y.cyc;
 }
 
+// Block-wide minimum: shuffles within waves, then one pass over the 16 wave results.
 __device__ RowKey block_min(RowKey k, RowKey* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = k;
-  __syncthreads();
-  for (int off = kStepThreads / 2; off > 0; off >>= 1) {
-    if (tid < off && rk_less(sh[tid + off], sh[tid])) sh[tid] = sh[tid + off];
-    __syncthreads();
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    RowKey o{(uint32_t)__shfl_xor((int)k.v, off), __shfl_xor(k.cyc, off), __shfl_xor(k.slot, off)};
+    if (rk_less(o, k)) k = o;
   }
-  const RowKey r = sh[0];
+  if (lane == 0) sh[wave] = k;
+  __syncthreads();
+  RowKey r = sh[0];
+  for (int w = 1; w < kStepThreads / 64; ++w)
+    if (rk_less(sh[w], r)) r = sh[w];
   __syncthreads();
   return r;
 }
 
-// The first row (in visiting order) reaching the minimum among live rows with candidates whose
-// cyclic position is below `below`.
-__device__ RowKey best_row_before(const NjDevice& D, const NjState& st, int start, int below,
+// The first row (in visiting order) reaching the minimum among live rows with a candidate below
+// FLT_MAX whose cyclic position is below `below`.
+__device__ RowKey best_row_before(const NjDevice& D, int used, int start, int n, int below,
                                   RowKey* sh) {
   RowKey k{0xffffffffu, 0x7fffffff, -1};
-  for (int b = threadIdx.x; b < st.used; b += kStepThreads) {
+  for (int b = threadIdx.x; b < used; b += kStepThreads) {
     if (!D.alive[b]) continue;
     const float v = D.row_v[b];
-    if (!(v < FLT_MAX)) continue;  // no candidate, or none that beats the initial FLT_MAX
-    const int idx = D.slot_index[b];
-    const int cyc = (idx - start + st.n) % st.n;
+    if (!(v < FLT_MAX)) continue;
+    int cyc = D.slot_index[b] - start;
+    if (cyc < 0) cyc += n;
     if (cyc >= below) continue;
     const RowKey c{fkey(v), cyc, b};
     if (rk_less(c, k)) k = c;
@@ -151,121 +261,122 @@ __device__ __forceinline__ int red_pop(const NjDevice& D, int i) {
   return j;
 }
 
-// One iteration after the scan: findMin's result and caches (rapidNJ.cpp:290-345),
-// mergeMinNodes (:240-281), clusterCount--, updateData (:166-238).
+// Launch B, one work-group: findMin's result and caches (rapidNJ.cpp:290-345), mergeMinNodes
+// (:240-281), clusterCount--, updateData (:166-238) except the new separation sum, which the
+// next launch A finishes.
 __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float* __restrict__ S) {
   NjState& st = *D.st;
   const int tid = threadIdx.x;
-  __shared__ RowKey sh[kStepThreads];
-  __shared__ int s_new_slot, s_rows[2], s_skip[2];
-  __shared__ float s_mutual, s_newsum;
-  __shared__ int s_fail;
+  __shared__ RowKey sh[kStepThreads / 64];
+  __shared__ int s_rows[2], s_skip[2], s_fail;
+  __shared__ float s_mutual;
   if (st.error) return;
-
+  const int n = st.n, used = st.used;
   const int start = D.index_to_id[st.cache1] != -1 ? st.cache1 : st.cache2;
   // records: rows where the running minimum improved, last first
   RowKey rec[4];
   int nrec = 0;
-  int below = st.n + 1;
+  int below = n + 1;
   for (; nrec < 4; ++nrec) {
-    rec[nrec] = best_row_before(D, st, start, below, sh);
+    rec[nrec] = best_row_before(D, used, start, n, below, sh);
     if (rec[nrec].slot < 0) break;
     below = rec[nrec].cyc;
   }
   if (tid == 0) {
     // the winner must name live rows with slots, and the new cluster needs a free slot
-    bool bad = nrec == 0 || st.used >= (int)D.ld;
+    bool bad = nrec == 0 || used >= (int)D.ld;
+    int min1 = 0, min2 = 0, r1 = 0, r2 = 0, s1 = 0, s2 = 0;
     if (!bad) {
-      const int w1 = D.slot_index[rec[0].slot], w2 = D.row_p[rec[0].slot];
-      bad = w2 < 0 || w2 >= st.n;
+      min1 = D.slot_index[rec[0].slot];
+      min2 = D.row_p[rec[0].slot];
+      bad = min2 < 0 || min2 >= n;
       if (!bad) {
-        const int q1 = D.rtm[w1], q2 = D.rtm[w2];
-        bad = q1 < 0 || q1 >= st.n || q2 < 0 || q2 >= st.n || D.slot_of_index[q1] < 0 ||
-              D.slot_of_index[q2] < 0;
+        r1 = D.rtm[min1];
+        r2 = D.rtm[min2];
+        bad = r1 < 0 || r1 >= n || r2 < 0 || r2 >= n;
+        if (!bad) {
+          s1 = D.slot_of_index[r1];
+          s2 = D.slot_of_index[r2];
+          bad = s1 < 0 || s2 < 0;
+        }
       }
     }
     s_fail = bad;
-    if (bad) st.error = 1;
+    if (bad) {
+      st.error = 1;
+    } else {
+      // caches: replay the improvements in visiting order (records are listed last first; with
+      // 4 or more records only the last three matter, and they are distinct rows)
+      int m = st.min1, c1 = st.cache1, c2 = st.cache2;
+      for (int t = nrec - 1; t >= 0; --t) {
+        const int row = D.slot_index[rec[t].slot];
+        if (row != m) {
+          c2 = c1;
+          c1 = m;
+          m = row;
+        }
+      }
+      st.min1 = min1;
+      st.min2 = min2;
+      st.cache1 = c1;
+      st.cache2 = c2;
+      // mergeMinNodes
+      const float mutual = S[(size_t)s1 * D.ld + s2];
+      const double dist = mutual;
+      const double sep1 = D.sep[s1], sep2 = D.sep[s2];
+      double dist1 = (0.5 * dist) + (0.5 * (sep1 - sep2));
+      double dist2 = (0.5 * dist) + (0.5 * (sep2 - sep1));
+      if (st.negative_branches) {
+        if (dist1 < 0.0) {
+          dist2 += dist1;
+          dist1 = 0;
+        }
+        if (dist2 < 0.0) {
+          dist1 += dist2;
+          if (dist1 < 0) dist1 = 0;
+          dist2 = 0;
+        }
+      }
+      int new_row = r1, obs_row = r2;
+      if (D.red_count[r1] != 1) {
+        new_row = red_pop(D, r1);
+        D.red_count[r1]--;
+        D.index_to_id[new_row] = new_row;
+      }
+      const int m2 = D.rtm[min2];
+      if (D.red_count[m2] != 1) {
+        obs_row = red_pop(D, m2);
+        D.red_count[m2]--;
+        D.index_to_id[obs_row] = obs_row;
+      }
+      const int k = st.step;
+      D.left[k] = D.index_to_id[new_row];
+      D.right[k] = D.index_to_id[obs_row];
+      D.llen[k] = dist1;
+      D.rlen[k] = dist2;
+      st.new_row = new_row;
+      st.obs_row = obs_row;
+      st.cluster_count--;
+      s_rows[0] = s1;
+      s_rows[1] = s2;
+      s_mutual = mutual;
+      s_skip[0] = new_row;
+      s_skip[1] = obs_row;
+    }
   }
   __syncthreads();
   if (s_fail) return;
-  if (tid == 0) {
-    // caches: replay the improvements in visiting order (records are listed last first)
-    int m = st.min1, c1 = st.cache1, c2 = st.cache2;
-    for (int t = nrec - 1; t >= 0; --t) {
-      const int row = D.slot_index[rec[t].slot];
-      if (row != m) {
-        c2 = c1;
-        c1 = m;
-        m = row;
-      }
-    }
-    // with 4 or more records only the last three matter, and they are distinct rows
-    const int wslot = rec[0].slot;
-    const int min1 = D.slot_index[wslot];
-    const int min2 = D.row_p[wslot];
-    st.min1 = min1;
-    st.min2 = min2;
-    st.cache1 = c1;
-    st.cache2 = c2;
-    (void)m;
-    // mergeMinNodes
-    const int r1 = D.rtm[min1], r2 = D.rtm[min2];
-    const int s1 = D.slot_of_index[r1], s2 = D.slot_of_index[r2];
-    const float mutual = S[(size_t)s1 * D.ld + s2];
-    const double dist = mutual;
-    const double sep1 = D.sep[s1], sep2 = D.sep[s2];
-    double dist1 = (0.5 * dist) + (0.5 * (sep1 - sep2));
-    double dist2 = (0.5 * dist) + (0.5 * (sep2 - sep1));
-    if (st.negative_branches) {
-      if (dist1 < 0.0) {
-        dist2 += dist1;
-        dist1 = 0;
-      }
-      if (dist2 < 0.0) {
-        dist1 += dist2;
-        if (dist1 < 0) dist1 = 0;
-        dist2 = 0;
-      }
-    }
-    int new_row = r1, obs_row = r2;
-    if (D.red_count[r1] != 1) {
-      new_row = red_pop(D, r1);
-      D.red_count[r1]--;
-      D.index_to_id[new_row] = new_row;
-    }
-    const int m2 = D.rtm[min2];
-    if (D.red_count[m2] != 1) {
-      obs_row = red_pop(D, m2);
-      D.red_count[m2]--;
-      D.index_to_id[obs_row] = obs_row;
-    }
-    const int k = st.step;
-    D.left[k] = D.index_to_id[new_row];
-    D.right[k] = D.index_to_id[obs_row];
-    D.llen[k] = dist1;
-    D.rlen[k] = dist2;
-    st.new_row = new_row;
-    st.obs_row = obs_row;
-    st.cluster_count--;
-    s_rows[0] = s1;
-    s_rows[1] = s2;
-    s_mutual = mutual;
-    s_skip[0] = new_row;
-    s_skip[1] = obs_row;
-    s_new_slot = st.used;
-  }
-  __syncthreads();
 
-  // updateData: every live cluster in id order (= live slots in order)
-  const int used = st.used;
+  // updateData over every live cluster in id order (= live slots in order); the merged rows'
+  // slots die (separation NaN)
   const float mutual = s_mutual;
   const float* row1 = S + (size_t)s_rows[0] * D.ld;
   const float* row2 = S + (size_t)s_rows[1] * D.ld;
   const float denom = (float)(st.cluster_count - 2);
+  const int ns = used;
+  float* new_row_p = S + (size_t)ns * D.ld;
   for (int b = tid; b < used; b += kStepThreads) {
     float dist = 0.0f, term = 0.0f;
-    int use = 0;
     if (D.alive[b]) {
       const int i = D.slot_index[b];
       if (i != s_skip[0] && i != s_skip[1]) {
@@ -274,47 +385,22 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
         dist = (float)((double)t / 2.0);
         if (st.negative_branches && dist < 0) dist = 0;
         term = __fmul_rn(dist, (float)D.red_count[i]);
-        D.sepsum[b] = __fadd_rn(D.sepsum[b], __fsub_rn(__fsub_rn(dist, val1), val2));
-        D.sep[b] = D.sepsum[b] / denom;
-        use = 1;
+        const float ss = __fadd_rn(D.sepsum[b], __fsub_rn(__fsub_rn(dist, val1), val2));
+        D.sepsum[b] = ss;
+        D.sep[b] = ss / denom;
+      } else {
+        D.alive[b] = 0;
+        D.sep[b] = __int_as_float(0x7fc00000);
       }
     }
     D.newcol[b] = dist;
-    D.term[b] = use ? term : __int_as_float(0x7fc00000);  // NaN marks "not added"
-  }
-  __syncthreads();
-  // newSeparationsum: the reference's sequential float sum, in id order, on one wave
-  if (tid < 64) {
-    float acc = 0.0f;
-    for (int base = 0; base < used; base += 64) {
-      const int b = base + tid;
-      const float t = b < used ? D.term[b] : __int_as_float(0x7fc00000);
-      const unsigned long long present = __ballot(t == t);
-      const int ti = __float_as_int(t);
-#pragma unroll
-      for (int l = 0; l < 64; ++l) {
-        const float x = __int_as_float(__builtin_amdgcn_readlane(ti, l));
-        if ((present >> l) & 1ull) acc = __fadd_rn(acc, x);
-      }
-    }
-    if (tid == 0) s_newsum = acc;
-  }
-  __syncthreads();
-  // the new cluster: slot `used`, its row and column; the merged rows' slots die
-  const int ns = s_new_slot;
-  float* new_row_p = S + (size_t)ns * D.ld;
-  for (int b = tid; b < used; b += kStepThreads) {
-    const float x = D.alive[b] ? D.newcol[b] : 0.0f;
-    new_row_p[b] = x;
-    S[(size_t)b * D.ld + ns] = x;
+    D.term[b] = term;
+    new_row_p[b] = dist;
+    S[(size_t)b * D.ld + ns] = dist;
   }
   if (tid == 0) {
     new_row_p[ns] = 0.0f;
     const int new_row = st.new_row, obs_row = st.obs_row;
-    const int old_new_slot = D.slot_of_index[new_row];
-    if (old_new_slot >= 0) D.alive[old_new_slot] = 0;
-    const int obs_slot = D.slot_of_index[obs_row];
-    if (obs_slot >= 0) D.alive[obs_slot] = 0;
     D.slot_of_index[obs_row] = -1;
     D.id_to_index[D.index_to_id[new_row]] = -1;
     D.id_to_index[D.index_to_id[obs_row]] = -1;
@@ -327,8 +413,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
     D.alive[ns] = 1;
     D.slot_index[ns] = new_row;
     D.slot_of_index[new_row] = ns;
-    D.sepsum[ns] = s_newsum;
-    D.sep[ns] = s_newsum / denom;
+    D.sep[ns] = __int_as_float(0x7fc00000);  // until launch A finishes it
     st.used = ns + 1;
     st.step++;
   }
@@ -490,11 +575,21 @@ hipError_t nj_launch_gather(const float* M, size_t ldm, const int* vis, int v, f
   return hipGetLastError();
 }
 
-hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, hipStream_t s) {
-  nj_scan_kernel<<<dim3(used_bound), dim3(kScanThreads), 0, s>>>(D, S);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool finish_previous,
+                               hipStream_t s) {
+  const int rows_blocks = (used_bound + 3) / 4;
+  const int grid = rows_blocks + (finish_previous ? 1 : 0);
+  if (grid > 0) {
+    nj_scan_kernel<<<dim3(grid), dim3(kScanThreads), 0, s>>>(D, S, finish_previous ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   nj_step_kernel<<<dim3(1), dim3(kStepThreads), 0, s>>>(D, S);
+  return hipGetLastError();
+}
+
+hipError_t nj_launch_finish(const NjDevice& D, const float* S, hipStream_t s) {
+  nj_finish_kernel<<<dim3(1), dim3(kScanThreads), 0, s>>>(D, S);
   return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ struct NjState {
   int negative_branches;
   int error;          // set when an iteration finds no pair or runs out of slots (then all stop)
 };
+// A slot's separation is NaN once its cluster is gone, so that scans skip it without a flag.
 
 // Device arrays of one run, passed by value to the kernels.
 struct NjDevice {
@@ -60,8 +61,12 @@ hipError_t nj_launch_same_rows(const float* M, size_t ldm, int n, const int* pai
                                int* same, hipStream_t s);
 hipError_t nj_launch_gather(const float* M, size_t ldm, const int* vis, int v, float* S, size_t ld,
                             hipStream_t s);
-// One iteration (scan + step) over slots < used_bound.
-hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, hipStream_t s);
+// One iteration over slots < used_bound: the scan (after finishing the previous join's new
+// cluster when `finish_previous`), then the step.
+hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool finish_previous,
+                               hipStream_t s);
+// Finish the last join's new cluster alone (before a compaction).
+hipError_t nj_launch_finish(const NjDevice& D, const float* S, hipStream_t s);
 // Rebuild the slot matrix into S2 from the live slots (at most alive_bound of them).
 hipError_t nj_launch_compact(const NjDevice& D, const float* S, float* S2, int alive_bound,
                              hipStream_t s);
