@@ -285,6 +285,47 @@ def protein_side(ctx, stream, cpu: bool) -> dict:
     return res
 
 
+def nj_side(ctx, A, cpu: bool, sample_n: int = 5000) -> dict:
+    """Neighbour joining on the device over the benchmark's 20000 x 5000 K2P matrix kept in HBM
+    (rnj_nj_device), and on its leading sample_n x sample_n block next to the reference's own
+    rapidNJ (compiled from its sources, ``ref_harness nj``) on the same block; the two Newick
+    texts are compared byte for byte."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    n = A.n
+    d = torch.empty((n, n), dtype=torch.float32, device="cuda:0")
+    ctx.distances(A, "k2p", layout="square", out=d, exact=True)
+    torch.cuda.synchronize()
+    out = {"taxa": n}
+    t0 = time.perf_counter()
+    tree = ctx.nj(d)
+    out["device_s"] = round(time.perf_counter() - t0, 3)
+    tree.close()
+    sub = d[:sample_n, :sample_n].contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree = ctx.nj(sub)
+    out[f"device_{sample_n}_s"] = round(time.perf_counter() - t0, 3)
+    text = tree.newick()
+    tree.close()
+    if cpu and O.ref_harness_available():
+        with tempfile.TemporaryDirectory() as tmp:
+            mf, nwk = os.path.join(tmp, "m.bin"), os.path.join(tmp, "t.nwk")
+            m = sub.cpu().numpy()
+            with open(mf, "wb") as fh:
+                np.array([sample_n], dtype=np.uint32).tofile(fh)
+                m.tofile(fh)
+            t0 = time.perf_counter()
+            subprocess.run([O.REF_HARNESS, "nj", mf, nwk], check=True, capture_output=True)
+            out[f"cpu_reference_{sample_n}_s"] = round(time.perf_counter() - t0, 3)
+            with open(nwk, "rb") as fh:
+                out[f"same_tree_as_reference_{sample_n}"] = fh.read() == text
+        out[f"speedup_{sample_n}"] = round(out[f"cpu_reference_{sample_n}_s"] / out[f"device_{sample_n}_s"], 1)
+    del d, sub
+    return out
+
+
 def side_main(args) -> int:
     """--side: one side measurement alone (for profiling it)."""
     from rapidnj_amd import native as N
@@ -294,6 +335,10 @@ def side_main(args) -> int:
     ctx = N.Context(0)
     if args.side == "ingest":
         res = ingest_compare(ctx, aln)
+    elif args.side == "nj":
+        A = ctx.alignment(aln)
+        res = nj_side(ctx, A, not args.no_cpu_baseline)
+        A.close()
     else:
         A = ctx.alignment(aln)
         res = handoff_compare(ctx, A)
@@ -440,7 +485,7 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM bytes per launch")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--side", choices=["ingest", "handoff"],
+    ap.add_argument("--side", choices=["ingest", "handoff", "nj"],
                     help="run only this side measurement on the workload and print its JSON")
     args = ap.parse_args()
     if args.probe_traffic:
@@ -575,6 +620,8 @@ def main() -> int:
         extra["ingest"] = ingest_compare(ctx, aln)
         log("side run: batched bootstrap vs one launch per replicate")
         extra["bootstrap"] = bootstrap_compare(ctx, dev)
+        log("side run: neighbour joining on the device (20000 taxa; 5000-taxa block vs the reference)")
+        extra["nj"] = nj_side(ctx, A, not args.no_cpu_baseline)
         log("side run: protein distances (Kimura protein, 20000 x 1250)")
         extra["protein"] = protein_side(ctx, stream, not args.no_cpu_baseline)
         log(f"side run: served stream, {len(served_jobs)} jobs")

@@ -257,6 +257,11 @@ typedef struct rnj_tree rnj_tree;
  * only; enqueued on `stream` after the work already there, synchronous. n >= 3. */
 RNJ_EXPORT int rnj_nj_device(rnj_context* ctx, const float* matrix, unsigned n, size_t ld,
                              int negative_branches, void* stream, rnj_tree** out);
+/* The same from host rows: rows[i] holds row i of the n x n matrix (distType** as
+ * distMatrixReader::getMatrix returns it, distMatrixReader.cpp:155-157); uploaded through the
+ * context's pinned buffers. */
+RNJ_EXPORT int rnj_nj_rows(rnj_context* ctx, const float* const* rows, unsigned n,
+                           int negative_branches, rnj_tree** out);
 /* Distances (model RNJ_K2P / RNJ_JC, with the exact FP64 logs the host hand-off uses) and
  * neighbour joining without the matrix leaving HBM; leaf names are the alignment's FASTA names. */
 RNJ_EXPORT int rnj_nj_alignment(rnj_context* ctx, const rnj_alignment* aln, int model,

@@ -386,6 +386,50 @@ int rnj_nj_device(rnj_context* ctx, const float* matrix, unsigned n, size_t ld,
   return RNJ_OK;
 }
 
+int rnj_nj_rows(rnj_context* ctx, const float* const* rows, unsigned n, int negative_branches,
+                rnj_tree** out) {
+  if (!ctx || !rows || !out) return fail(RNJ_E_INVALID, "null argument");
+  if (n < 3) return fail(RNJ_E_INVALID, "need at least 3 taxa");
+  for (unsigned i = 0; i < n; ++i)
+    if (!rows[i]) return fail(RNJ_E_INVALID, "null matrix row");
+  (void)hipGetLastError();
+  RNJ_HIP(hipSetDevice(ctx->device));
+  float* d = nullptr;
+  RNJ_HIP(hipMalloc((void**)&d, (size_t)n * n * sizeof(float)));
+  hipError_t e = hipSuccess;
+  {
+    // rows packed into the two pinned buffers in turn, each uploaded while the other fills
+    std::lock_guard<std::mutex> lock(ctx->pin_mu);
+    e = ensure_pinned(ctx);
+    const size_t row_bytes = (size_t)n * sizeof(float);
+    const size_t per = std::max<size_t>(1, kPinBytes / row_bytes);
+    int k = 0;
+    for (unsigned r0 = 0; e == hipSuccess && r0 < n; r0 += (unsigned)per, k ^= 1) {
+      const unsigned r1 = (unsigned)std::min<size_t>(n, r0 + per);
+      e = hipEventSynchronize(ctx->pin_ev[k]);
+      if (e != hipSuccess) break;
+      if (row_bytes <= kPinBytes) {
+        for (unsigned r = r0; r < r1; ++r)
+          std::memcpy(ctx->pin[k] + (size_t)(r - r0) * row_bytes, rows[r], row_bytes);
+        e = hipMemcpyAsync(d + (size_t)r0 * n, ctx->pin[k], (size_t)(r1 - r0) * row_bytes,
+                           hipMemcpyHostToDevice, ctx->stream);
+      } else {
+        e = hipMemcpyAsync(d + (size_t)r0 * n, rows[r0], row_bytes, hipMemcpyHostToDevice,
+                           ctx->stream);
+      }
+      if (e == hipSuccess) e = hipEventRecord(ctx->pin_ev[k], ctx->stream);
+    }
+  }
+  rnj_tree* t = nullptr;
+  int rc = e == hipSuccess ? rnj_nj_device(ctx, d, n, n, negative_branches, ctx->stream, &t)
+                           : hip_fail(e, "rnj_nj_rows upload");
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (rc != RNJ_OK) return rc;
+  *out = t;
+  return RNJ_OK;
+}
+
 int rnj_nj_alignment(rnj_context* ctx, const rnj_alignment* aln, int model, int negative_branches,
                      rnj_tree** out) {
   if (!ctx || !aln || !out) return fail(RNJ_E_INVALID, "null argument");

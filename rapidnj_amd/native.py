@@ -78,6 +78,7 @@ SIGNATURES = {
     "rnj_plan_tile_config": (ctypes.c_int, [_vp]),
     "rnj_plan_destroy": (ctypes.c_int, [_vp]),
     "rnj_nj_device": (ctypes.c_int, [_vp, _vp, _u, _sz, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    "rnj_nj_rows": (ctypes.c_int, [_vp, _vp, _u, ctypes.c_int, ctypes.POINTER(_vp)]),
     "rnj_nj_alignment": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "rnj_tree_size": (_u, [_vp]),
     "rnj_tree_joins": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -426,6 +427,16 @@ class Context:
         h = _vp()
         check(lib().rnj_nj_device(self.handle, matrix.data_ptr(), n, matrix.stride(0),
                                   int(negative_branches), _stream_handle(stream), ctypes.byref(h)))
+        return Tree(h.value)
+
+    def nj_rows(self, matrix: np.ndarray, negative_branches: bool = False) -> Tree:
+        """rapidNJ on a host matrix handed over as row pointers (rnj_nj_rows)."""
+        m = np.ascontiguousarray(matrix, dtype=np.float32)
+        n = m.shape[0]
+        rows = (_vp * n)(*[m.ctypes.data + i * m.strides[0] for i in range(n)])
+        h = _vp()
+        check(lib().rnj_nj_rows(self.handle, ctypes.cast(rows, _vp), n, int(negative_branches),
+                                ctypes.byref(h)))
         return Tree(h.value)
 
     def nj_alignment(self, aln: Alignment, model="k2p", negative_branches: bool = False) -> Tree:

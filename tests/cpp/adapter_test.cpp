@@ -3,7 +3,8 @@
 // matrix, KimuraDistance / JCdistance, getDistanceMatrix. Test infrastructure for
 // tests/test_gpu_adapter.py.
 //
-//   adapter_test IN OUT kim|jc mem|disk|reuse|half fast|chars [protein]
+//   adapter_test IN OUT kim|jc mem|disk|reuse|half|tree fast|chars [protein]
+// (tree: OUT is the Newick text of rapidnj_amd::NJFromAlignment on the loader)
 // IN:  u32 n, L, bitStringsCount; n*L chars; n*bsc*4 u32 bit strings; n*bsc*4 u32 gap filters
 //      (protein: the bit strings hold the resolved characters, 16 per 4 words, as the reference's
 //      loader stores them; the gap filters are unused)
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "rapidnj_amd_distance.hpp"
+#include "rapidnj_amd_nj.hpp"
 
 using rapidnj_amd::distType;
 
@@ -91,6 +93,15 @@ int main(int argc, char** argv) {
   const bool fast = std::string(argv[5]) == "fast";
   if (argc > 6 && std::string(argv[6]) == "protein") ld.type = 1;
   const unsigned n = ld.n;
+  if (mode == "tree") {
+    rapidnj_amd::NJFromAlignment<MemoryLoader> nj(fast, &ld, model == "kim" ? RNJ_K2P : RNJ_JC,
+                                                  false);
+    rapidnj_amd::Tree* tree = nj.run();
+    std::ofstream o(argv[2]);
+    tree->serialize_tree(o);
+    delete tree;
+    return o ? 0 : 4;
+  }
   std::vector<distType> out((size_t)n * n);
   distType maxd = 0;
   if (mode == "disk") {

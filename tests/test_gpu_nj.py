@@ -121,3 +121,51 @@ def test_small_and_invalid(ctx, N):
     assert tree.newick() == O.newick(3, O.nj(m))
     with pytest.raises(N.RnjError):
         ctx.nj(device(np.zeros((2, 2), dtype=np.float32)))
+
+
+DRIVER = os.path.join(os.path.dirname(__file__), "cpp", "_build", "nj_test")
+ADAPTER = os.path.join(os.path.dirname(__file__), "cpp", "_build", "adapter_test")
+
+
+@pytest.mark.parametrize("case", ["dup", "sat", "sim300"])
+@pytest.mark.parametrize("neg", [False, True])
+def test_cpp_rapidNJ_class(tmp_path, nj_golden, case, neg):
+    """rapidnj_amd::rapidNJ<Reader>(reader, n, neg, pb)->run()->serialize_tree(out), the
+    reference's call sequence, over host rows (rnj_nj_rows)."""
+    import subprocess
+    m = nj_golden[f"{case}_matrix"]
+    mf, out = tmp_path / "m.bin", tmp_path / "t.nwk"
+    with open(mf, "wb") as fh:
+        np.array([m.shape[0]], dtype=np.uint32).tofile(fh)
+        m.astype(np.float32).tofile(fh)
+    r = subprocess.run([DRIVER, str(mf), str(out)] + (["neg"] if neg else []), capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = bytes(nj_golden[f"{case}_newick_neg" if neg else f"{case}_newick"])
+    assert out.read_bytes() == want
+
+
+def test_host_rows(ctx, nj_golden):
+    m = nj_golden["sim300_matrix"]
+    assert ctx.nj_rows(m).newick() == bytes(nj_golden["sim300_newick"])
+
+
+@pytest.mark.parametrize("model", ["kim", "jc"])
+def test_cpp_tree_from_loader(tmp_path, model):
+    """rapidnj_amd::NJFromAlignment on a loader (LGT_70): distances and joins on the device."""
+    import subprocess
+    d = dict(np.load(os.path.join(GOLDEN, "lgt70.npz"), allow_pickle=False))
+    aln = np.ascontiguousarray(d["aln"], dtype=np.uint8)
+    n, L = aln.shape
+    bits, gaps, _ = O.pack(aln)
+    inp, out = tmp_path / "in.bin", tmp_path / "t.nwk"
+    with open(inp, "wb") as f:
+        np.array([n, L, bits.shape[1] // 4], dtype=np.uint32).tofile(f)
+        aln.tofile(f)
+        np.ascontiguousarray(bits, dtype=np.uint32).tofile(f)
+        np.ascontiguousarray(gaps, dtype=np.uint32).tofile(f)
+    r = subprocess.run([ADAPTER, str(inp), str(out), model, "tree", "fast"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    key = "newick_k2p" if model == "kim" else "newick_jc"
+    assert out.read_bytes().rstrip(b"\n") == bytes(d[key])
