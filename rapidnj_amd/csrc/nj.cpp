@@ -46,7 +46,8 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   DevBuf<NjState> st;
   DevBuf<int> index_to_id, id_to_index, red_count, rtm, red_head, red_next, slot_of_index;
   DevBuf<int> slot_index, alive, row_p, map, left, right;
-  DevBuf<float> sepsum, sep, row_v, newcol, term, sums, S0, S1;
+  DevBuf<float> sepsum, sep, newcol, term, sums, S0, S1;
+  DevBuf<unsigned long long> row_key;
   DevBuf<double> llen, rlen;
   NJ_HIP(st.alloc(1));
   NJ_HIP(index_to_id.alloc(n));
@@ -64,7 +65,7 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   NJ_HIP(right.alloc(n));
   NJ_HIP(sepsum.alloc(cap));
   NJ_HIP(sep.alloc(cap));
-  NJ_HIP(row_v.alloc(cap));
+  NJ_HIP(row_key.alloc(cap));
   NJ_HIP(newcol.alloc(cap));
   NJ_HIP(term.alloc(cap));
   NJ_HIP(sums.alloc(n));
@@ -196,7 +197,7 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   D.alive = alive.p;
   D.sepsum = sepsum.p;
   D.sep = sep.p;
-  D.row_v = row_v.p;
+  D.row_key = row_key.p;
   D.row_p = row_p.p;
   D.newcol = newcol.p;
   D.term = term.p;
@@ -208,14 +209,14 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
 
   // rapidNJ::run: n - 2 joins. used_bound bounds the slots in use (exact until the first
   // compaction, then the live count's bound clusterCount plus the slots added since). The slot
-  // matrix is compacted when that bound passes 1.25 x clusterCount (or the capacity).
+  // matrix is compacted when that bound passes 1.125 x clusterCount (or the capacity).
   float* S = S0.p;
   float* S2 = S1.p;
   size_t used_bound = (size_t)v;
   bool pending = false;  // the last join's new cluster still needs its separation and row
   for (unsigned it = 0; it + 2 < n; ++it) {
     const size_t cc = n - it;  // clusterCount before this join
-    if (used_bound + 1 > cap || (used_bound > 256 && used_bound * 4 > cc * 5)) {
+    if (used_bound + 1 > cap || (used_bound > 256 && used_bound * 8 > cc * 9)) {
       if (pending) NJ_HIP(nj_launch_finish(D, S, s));
       pending = false;
       NJ_HIP(nj_launch_compact(D, S, S2, (int)std::min(cc, used_bound), s));

@@ -25,7 +25,9 @@
 namespace rnj {
 namespace {
 
-constexpr int kScanThreads = 256;  // 4 waves, one row each
+constexpr int kScanThreads = 1024;  // 16 waves, one row each
+constexpr int kScanRows = kScanThreads / 64;
+constexpr int kSepChunk = 4096;     // separations staged in LDS per step of a row scan
 constexpr int kStepThreads = 1024;
 
 // Floats as order-preserving unsigned keys (after -0 -> +0, so that equal values compare equal
@@ -60,9 +62,11 @@ __device__ __forceinline__ Cand wave_min(Cand c) {
 }
 
 constexpr float kInf = __builtin_inff();
+constexpr unsigned long long kNoKey = ~0ull;
 
 // The row's answer as rapidNJ's findRowMin gives it: the redundant-copy join first when the row
 // has copies (searchRedundantEntries), then the best pair unless it is not strictly better.
+// Rows whose answer does not beat FLT_MAX (the search's start value) get no key.
 __device__ __forceinline__ void write_row(const NjDevice& D, int a, const Cand& best, float s_a) {
   const int idx = D.slot_index[a];
   float v = best.p >= 0 ? best.v : kInf;
@@ -74,44 +78,21 @@ __device__ __forceinline__ void write_row(const NjDevice& D, int a, const Cand& 
       p = D.red_head[idx];
     }
   }
-  D.row_v[a] = v;
+  D.row_key[a] = v < FLT_MAX ? ((unsigned long long)fkey(v) << 32) | (uint32_t)idx : kNoKey;
   D.row_p[a] = p;
 }
 
-// One wave scans row slot a: partners are the slots b < a (NaN separation: gone).
-__device__ void scan_row(const NjDevice& D, const float* __restrict__ S, int a, int lane) {
-  const float s_a = D.sep[a];
-  const float* row = S + (size_t)a * D.ld;
-  Cand best{kInf, kInf, -1};
-  int best_b = -1;
-  for (int b0 = 4 * lane; b0 < a; b0 += 256) {
-    const float4 dv = *reinterpret_cast<const float4*>(row + b0);
-    const float4 sv = *reinterpret_cast<const float4*>(D.sep + b0);
-    const float dd[4] = {dv.x, dv.y, dv.z, dv.w};
-    const float ss[4] = {sv.x, sv.y, sv.z, sv.w};
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int b = b0 + e;
-      if (b >= a) break;
-      const float v = __fsub_rn(__fsub_rn(dd[e], ss[e]), s_a) + 0.0f;
-      const float d = dd[e] + 0.0f;
-      if (v < best.v || (v == best.v && d < best.d)) {
-        best.v = v;
-        best.d = d;
-        best_b = b;
-        best.p = 0x7fffffff;  // index looked up below (ties only need it)
-      } else if (v == best.v && d == best.d && best_b >= 0) {
-        // exact tie within this lane: the lower partner index first
-        if (D.slot_index[b] < D.slot_index[best_b]) best_b = b;
-      }
-    }
-  }
-  best.p = best_b >= 0 ? D.slot_index[best_b] : 0x7fffffff;
-  if (best_b < 0) best.v = kInf;
-  best = wave_min(best);
-  if (lane == 0) {
-    if (!(best.v < kInf)) best.p = -1;
-    write_row(D, a, best, s_a);
+// Candidate (d, separation) of partner b into this lane's best.
+__device__ __forceinline__ void consider(const NjDevice& D, float d_raw, float s_b, float s_a,
+                                         int b, Cand& best, int& best_b) {
+  const float v = __fsub_rn(__fsub_rn(d_raw, s_b), s_a) + 0.0f;
+  const float d = d_raw + 0.0f;
+  if (v < best.v || (v == best.v && d < best.d)) {
+    best.v = v;
+    best.d = d;
+    best_b = b;
+  } else if (v == best.v && d == best.d && best_b >= 0 && D.slot_index[b] < D.slot_index[best_b]) {
+    best_b = b;  // exact tie: the lower partner index first
   }
 }
 
@@ -124,7 +105,7 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr int kChunk = 4096;
   __shared__ float terms[kChunk];
-  __shared__ float s_sum;
+  __shared__ Cand wbest[kScanThreads / 64];
   float acc = 0.0f;
   for (int base = 0; base < ns; base += kChunk) {
     const int cnt = min(kChunk, ns - base);
@@ -143,7 +124,6 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
     __syncthreads();
   }
   if (tid == 0) {
-    s_sum = acc;
     D.sepsum[ns] = acc;
     D.sep[ns] = acc / (float)(st.cluster_count - 2);
   }
@@ -152,22 +132,10 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
   // the new row: partners are every live slot below it
   Cand best{kInf, kInf, -1};
   int best_b = -1;
-  for (int b = tid; b < ns; b += kScanThreads) {
-    const float d = D.newcol[b] + 0.0f;
-    const float v = __fsub_rn(__fsub_rn(d, D.sep[b]), s_ns) + 0.0f;
-    if (v < best.v || (v == best.v && d < best.d)) {
-      best.v = v;
-      best.d = d;
-      best_b = b;
-    }  // within a thread b increases with the index order only by chance: ties resolved below
-    else if (v == best.v && d == best.d && best_b >= 0 && D.slot_index[b] < D.slot_index[best_b]) {
-      best_b = b;
-    }
-  }
+  for (int b = tid; b < ns; b += kScanThreads) consider(D, D.newcol[b], D.sep[b], s_ns, b, best, best_b);
   best.p = best_b >= 0 ? D.slot_index[best_b] : 0x7fffffff;
   if (best_b < 0) best.v = kInf;
   best = wave_min(best);
-  __shared__ Cand wbest[kScanThreads / 64];
   if (lane == 0) wbest[wave] = best;
   __syncthreads();
   if (tid == 0) {
@@ -177,11 +145,11 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
     if (!(b.v < kInf)) b.p = -1;
     write_row(D, ns, b, s_ns);
   }
-  (void)s_sum;
 }
 
-// Launch A: work-group 0 finishes the previous join's new cluster when asked; the others scan
-// one row per wave (rows of the live slots except that new one).
+// Launch A: work-group 0 finishes the previous join's new cluster when asked; every other
+// work-group scans 16 consecutive row slots, one per wave, over partner slots b < row. The
+// separations of the partners are staged in LDS once per work-group and shared by its rows.
 __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const float* __restrict__ S,
                                                                int finish) {
   const NjState& st = *D.st;
@@ -194,12 +162,44 @@ __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const
     }
     --blk;
   }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int a = blk * (kScanThreads / 64) + wave;
+  __shared__ float ssep[kSepChunk];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int used = st.used;
-  if (a >= used || (finish && a == used - 1)) return;
-  if (!D.alive[a]) return;
-  scan_row(D, S, a, lane);
+  const int a0 = blk * kScanRows;
+  if (a0 >= used) return;  // uniform over the work-group
+  const int a = a0 + wave;
+  const bool mine = a < used && !(finish && a == used - 1) && D.alive[a];
+  const int reach = min(a0 + kScanRows, used) - 1;  // longest row prefix in this group
+  const float s_a = mine ? D.sep[a] : 0.0f;
+  const float* row = S + (size_t)a * D.ld;
+  Cand best{kInf, kInf, -1};
+  int best_b = -1;
+  for (int c0 = 0; c0 < reach; c0 += kSepChunk) {
+    const int cnt = min(kSepChunk, reach - c0);
+    for (int k = 4 * tid; k < cnt; k += 4 * kScanThreads)
+      *reinterpret_cast<float4*>(ssep + k) = *reinterpret_cast<const float4*>(D.sep + c0 + k);
+    __syncthreads();
+    if (mine) {
+      const int end = min(a, c0 + cnt);
+      for (int b0 = c0 + 4 * lane; b0 < end; b0 += 256) {
+        const float4 dv = *reinterpret_cast<const float4*>(row + b0);
+        const float4 sv = *reinterpret_cast<const float4*>(ssep + (b0 - c0));
+        consider(D, dv.x, sv.x, s_a, b0, best, best_b);
+        if (b0 + 1 < end) consider(D, dv.y, sv.y, s_a, b0 + 1, best, best_b);
+        if (b0 + 2 < end) consider(D, dv.z, sv.z, s_a, b0 + 2, best, best_b);
+        if (b0 + 3 < end) consider(D, dv.w, sv.w, s_a, b0 + 3, best, best_b);
+      }
+    }
+    __syncthreads();
+  }
+  if (!mine) return;
+  best.p = best_b >= 0 ? D.slot_index[best_b] : 0x7fffffff;
+  if (best_b < 0) best.v = kInf;
+  best = wave_min(best);
+  if (lane == 0) {
+    if (!(best.v < kInf)) best.p = -1;
+    write_row(D, a, best, s_a);
+  }
 }
 
 __global__ __launch_bounds__(kScanThreads) void nj_finish_kernel(NjDevice D, const float* __restrict__ S) {
@@ -237,19 +237,18 @@ __device__ RowKey block_min(RowKey k, RowKey* sh) {
   return r;
 }
 
-// The first row (in visiting order) reaching the minimum among live rows with a candidate below
-// FLT_MAX whose cyclic position is below `below`.
+// The first row (in visiting order) reaching the minimum among rows with a key (live, answer
+// below FLT_MAX) whose cyclic position is below `below`.
 __device__ RowKey best_row_before(const NjDevice& D, int used, int start, int n, int below,
                                   RowKey* sh) {
   RowKey k{0xffffffffu, 0x7fffffff, -1};
   for (int b = threadIdx.x; b < used; b += kStepThreads) {
-    if (!D.alive[b]) continue;
-    const float v = D.row_v[b];
-    if (!(v < FLT_MAX)) continue;
-    int cyc = D.slot_index[b] - start;
+    const unsigned long long key = D.row_key[b];
+    if (key == kNoKey) continue;
+    int cyc = (int)(uint32_t)key - start;
     if (cyc < 0) cyc += n;
     if (cyc >= below) continue;
-    const RowKey c{fkey(v), cyc, b};
+    const RowKey c{(uint32_t)(key >> 32), cyc, b};
     if (rk_less(c, k)) k = c;
   }
   return block_min(k, sh);
@@ -287,7 +286,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
     bool bad = nrec == 0 || used >= (int)D.ld;
     int min1 = 0, min2 = 0, r1 = 0, r2 = 0, s1 = 0, s2 = 0;
     if (!bad) {
-      min1 = D.slot_index[rec[0].slot];
+      min1 = (int)(uint32_t)D.row_key[rec[0].slot];
       min2 = D.row_p[rec[0].slot];
       bad = min2 < 0 || min2 >= n;
       if (!bad) {
@@ -309,7 +308,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
       // 4 or more records only the last three matter, and they are distinct rows)
       int m = st.min1, c1 = st.cache1, c2 = st.cache2;
       for (int t = nrec - 1; t >= 0; --t) {
-        const int row = D.slot_index[rec[t].slot];
+        const int row = (int)(uint32_t)D.row_key[rec[t].slot];
         if (row != m) {
           c2 = c1;
           c1 = m;
@@ -391,6 +390,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
       } else {
         D.alive[b] = 0;
         D.sep[b] = __int_as_float(0x7fc00000);
+        D.row_key[b] = kNoKey;
       }
     }
     D.newcol[b] = dist;
@@ -414,6 +414,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
     D.slot_index[ns] = new_row;
     D.slot_of_index[new_row] = ns;
     D.sep[ns] = __int_as_float(0x7fc00000);  // until launch A finishes it
+    D.row_key[ns] = kNoKey;
     st.used = ns + 1;
     st.step++;
   }
@@ -577,7 +578,7 @@ hipError_t nj_launch_gather(const float* M, size_t ldm, const int* vis, int v, f
 
 hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool finish_previous,
                                hipStream_t s) {
-  const int rows_blocks = (used_bound + 3) / 4;
+  const int rows_blocks = (used_bound + kScanRows - 1) / kScanRows;
   const int grid = rows_blocks + (finish_previous ? 1 : 0);
   if (grid > 0) {
     nj_scan_kernel<<<dim3(grid), dim3(kScanThreads), 0, s>>>(D, S, finish_previous ? 1 : 0);

@@ -40,7 +40,7 @@ struct NjDevice {
   int* alive;
   float* sepsum;       // separationsums
   float* sep;          // separations
-  float* row_v;        // best criterion of each row (scan)
+  unsigned long long* row_key;  // each row's best: (criterion key << 32) | row index; ~0 = none
   int* row_p;          // its partner index
   float* newcol;       // new cluster's distances (step)
   float* term;         // dist * redundantCount terms of the new separation sum
