@@ -399,12 +399,13 @@ def served_run(ctx, jobs, alns, open_loop: bool, slots: int = 2, max_batch: int 
 
 
 def one_job_per_launch(ctx, jobs, alns) -> dict:
-    """The same jobs one at a time: upload + pack, one distance launch, triangle to the host."""
+    """The same jobs one at a time: upload + pack, one distance launch, triangle to the host
+    (through the same pinned staging as the server's results)."""
     import torch
     t0 = time.perf_counter()
     for j, a in zip(jobs, alns):
         A = ctx.alignment(a)
-        ctx.distances(A, j.model).cpu()
+        ctx.to_host(ctx.distances(A, j.model))
         A.close()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0

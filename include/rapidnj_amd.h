@@ -198,6 +198,15 @@ RNJ_EXPORT int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, 
 RNJ_EXPORT int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, uint32_t* tv,
                       uint32_t* valid, void* stream);
 
+/* Copy `bytes` of device memory (e.g. a triangle from rnj_distances_device) into caller-owned
+ * pageable host memory, after the work already enqueued on `stream` (null: the context's
+ * stream). The bytes pass through the context's pinned staging chunks, emptied by library
+ * threads while the next chunk's DMA runs. Synchronous. The reference copies its GPU results
+ * with one synchronous cudaMemcpy per launch into malloc'ed memory (kimuraDNA_gpu.cu:224);
+ * this is that copy for any device result. */
+RNJ_EXPORT int rnj_copy_to_host(rnj_context* ctx, void* dst, const void* src, size_t bytes,
+                                void* stream);
+
 /* Host hand-off: rows[i] are n caller-owned arrays of n floats (e.g. new distType[n]), filled
  * with the post-processed symmetric matrix, diagonal 0, sentinels patched: the in-memory matrix
  * handed to distMatrixReader (src/main.cpp:419-437, distMatrixReader.cpp:14-23). Synchronous.

@@ -494,7 +494,10 @@ int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind
   if (stage == hipMemcpyHostToDevice || stage == hipMemcpyDeviceToDevice) {
     uint8_t* copy = nullptr;
     FA_HIP(t.get(&copy, N));
-    if (N) FA_HIP(hipMemcpyAsync(copy, src, N, stage, s));
+    if (N && stage == hipMemcpyHostToDevice)
+      FA_HIP(ctx->staging.upload(copy, reinterpret_cast<const char*>(src), N, N, 1, s));
+    else if (N)
+      FA_HIP(hipMemcpyAsync(copy, src, N, stage, s));
     b = copy;
   }
   // 1. '\n' positions

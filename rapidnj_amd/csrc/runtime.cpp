@@ -708,19 +708,24 @@ int rnj_alignment_from_chars(rnj_context* ctx, const char* chars, size_t pitch, 
     kind = alignment_kind(ctx, first.data(), L);
   }
   const uint8_t* dchars = (const uint8_t*)chars;
+  size_t dpitch = pitch;
   void* staging = nullptr;
-  if (!chars_on_device) {
-    hipError_t e = hipMalloc(&staging, (size_t)n * pitch);
+  if (!chars_on_device) {  // rows land back to back through the pinned staging chunks
+    hipError_t e = hipMalloc(&staging, (size_t)n * L);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(staging, chars, (size_t)n * pitch, hipMemcpyHostToDevice, ctx->stream);
+      e = ctx->staging.upload((uint8_t*)staging, chars, pitch, L, n, ctx->stream);
     if (e != hipSuccess) {
-      if (staging) (void)hipFree(staging);
+      if (staging) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(staging);
+      }
       return hip_fail(e, "rnj_alignment_from_chars upload");
     }
     dchars = (const uint8_t*)staging;
+    dpitch = L;
   }
   Alignment* a = nullptr;
-  rc = pack_alignment(ctx, dchars, pitch, n, L, kind, ctx->stream, &a);
+  rc = pack_alignment(ctx, dchars, dpitch, n, L, kind, ctx->stream, &a);
   if (staging) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(staging);
@@ -937,6 +942,14 @@ int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, int model, 
   jobs[0].max_out = max_out;
   return launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, (flags & RNJ_FLAG_EXACT) != 0,
                      stream_of(ctx, stream));
+}
+
+int rnj_copy_to_host(rnj_context* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+  if (!ctx || ((!dst || !src) && bytes)) return fail(RNJ_E_INVALID, "null argument");
+  const int rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  RNJ_HIP(ctx->staging.download(dst, src, bytes, stream_of(ctx, stream)));
+  return RNJ_OK;
 }
 
 int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, uint32_t* tv,

@@ -54,6 +54,38 @@ struct Workspace {
   bool in_use = false;
 };
 
+// ---- copies between pageable host memory and the device (upload.cpp) ------------------------
+// Pinned staging chunks: library threads fill (or empty) one chunk while a neighbouring chunk's
+// DMA runs. Calls on one Stager are serialised.
+class CopyPool;
+class Stager {
+ public:
+  Stager();
+  Stager(const Stager&) = delete;
+  Stager& operator=(const Stager&) = delete;
+  ~Stager();
+  // `rows` rows of `len` bytes (row t at src + t * pitch, host memory) into dst (device memory,
+  // rows back to back), enqueued on `stream`. Returns once every source byte has been read;
+  // work enqueued on `stream` afterwards sees the data.
+  hipError_t upload(uint8_t* dst, const char* src, size_t pitch, size_t len, size_t rows,
+                    hipStream_t stream);
+  // `bytes` from device memory into host memory, through DMAs on `stream` (after the work
+  // already enqueued there). Returns when the bytes are in dst_host.
+  hipError_t download(void* dst_host, const void* src_dev, size_t bytes, hipStream_t stream);
+  static constexpr int kChunks = 4;
+  static constexpr size_t kChunkBytes = (size_t)8 << 20;
+  static constexpr size_t kParallelBytes = (size_t)1 << 20;  // smaller chunks: one thread copies
+
+ private:
+  hipError_t ensure();
+  std::mutex mu_;
+  uint8_t* chunk_[kChunks] = {};
+  hipEvent_t ev_[kChunks] = {};
+  int next_ = 0;
+  unsigned parts_ = 1;
+  std::unique_ptr<CopyPool> pool_;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -73,6 +105,7 @@ struct Context {
   uint8_t* pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   int input_type = RNJ_TYPE_AUTO;  // type of alignments read from characters / FASTA
+  Stager staging;  // characters and FASTA images from host memory, results to it
 };
 
 constexpr size_t kPinBytes = (size_t)32 << 20;  // bytes per pinned staging buffer

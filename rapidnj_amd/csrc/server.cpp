@@ -2,9 +2,10 @@
 //
 // Admission. A submitted job is admitted when its device memory (characters, packed planes,
 // triangle, saturated-pair list) fits what the budget has left and no earlier job is waiting.
-// An admitted job is uploaded from the caller's buffer on the submitting thread (copy on the
-// server's upload stream, then the packing kernel) and becomes ready when that stream reaches
-// its event; the caller's buffer is free when submit returns. A job that does not fit is copied
+// An admitted job is uploaded from the caller's buffer on the submitting thread (library threads
+// copy it into pinned staging chunks whose DMAs run on the server's upload stream, then the
+// packing kernel) and becomes ready when that stream reaches its event; the caller's buffer is
+// free when submit returns, without waiting for the DMAs. A job that does not fit is copied
 // to host memory and queued; the worker admits queued jobs in FIFO order as batches retire and
 // memory frees up.
 //
@@ -78,6 +79,8 @@ struct rnj_server {
   hipStream_t download_stream = nullptr;
   std::mutex mu;
   std::mutex upload_mu;  // one upload at a time on the upload stream
+  Stager uploader;       // pinned staging for the callers' characters (submitting threads)
+  Stager downloader;     // and for their results (waiting threads)
   std::condition_variable cv_work, cv_done;
   std::deque<ServedJob*> queue;  // waiting for admission (host copies)
   std::deque<ServedJob*> ready;  // admitted: uploading or uploaded, in arrival order
@@ -95,7 +98,8 @@ struct rnj_server {
 };
 
 // Upload and pack one admitted job (no server lock held). chars: n rows at `pitch` on the host.
-// Returns when the caller's buffer has been read.
+// Returns when the caller's buffer has been read; the copies and the packing run on the upload
+// stream and the job's ready event follows them.
 int rnj_server::upload(ServedJob* j, const char* chars, size_t pitch) {
   std::lock_guard<std::mutex> ul(upload_mu);
   hipStream_t s = upload_stream;
@@ -103,19 +107,14 @@ int rnj_server::upload(ServedJob* j, const char* chars, size_t pitch) {
   hipError_t e = hipMallocAsync((void**)&dchars, (size_t)j->n * j->L, s);
   if (e == hipSuccess) e = hipMallocAsync((void**)&j->planes, (size_t)3 * j->W * j->n_pad * 4, s);
   if (e == hipSuccess) e = hipMallocAsync((void**)&j->tri, pairs_of(j->n) * 4, s);
-  if (e == hipSuccess) {
-    if (pitch == j->L)  // one contiguous copy: a 2D copy from pageable memory goes row by row
-      e = hipMemcpyAsync(dchars, chars, (size_t)j->n * j->L, hipMemcpyHostToDevice, s);
-    else
-      e = hipMemcpy2DAsync(dchars, j->L, chars, pitch, j->L, j->n, hipMemcpyHostToDevice, s);
-  }
+  // through the pinned staging chunks; returns once the caller's rows have been read
+  if (e == hipSuccess) e = uploader.upload(dchars, chars, pitch, j->L, j->n, s);
   if (e == hipSuccess) e = launch_pack_chars(dchars, j->L, j->n, j->L, j->n_pad, j->W, j->planes, s);
   if (dchars) (void)hipFreeAsync(dchars, s);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&j->ready, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventRecord(j->ready, s);
-  // the source is the caller's (pageable) memory: wait until it has been read
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
+    (void)hipStreamSynchronize(s);
     j->error = std::string("upload: ") + hipGetErrorString(e);
     return RNJ_E_HIP;
   }
@@ -418,11 +417,8 @@ int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_st
   int status = j->status;
   if (j->tri) {  // finished: the triangle is resident; copy it once into the caller's buffer
     hipError_t e = hipSuccess;
-    if (status == RNJ_OK && tri_out) {
-      e = hipMemcpyAsync(tri_out, j->tri, tri_bytes(j->n), hipMemcpyDeviceToHost,
-                         srv->download_stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(srv->download_stream);
-    }
+    if (status == RNJ_OK && tri_out)
+      e = srv->downloader.download(tri_out, j->tri, tri_bytes(j->n), srv->download_stream);
     (void)hipFreeAsync(j->tri, srv->download_stream);
     j->tri = nullptr;
     {

@@ -66,6 +66,7 @@ SIGNATURES = {
     "rnj_distances_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _sz,
                                             ctypes.c_int, _vp, _vp]),
     "rnj_counts_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rnj_copy_to_host": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "rnj_distances_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "rnj_distances_rows_lower": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "rnj_distances_stream_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
@@ -163,6 +164,15 @@ def _stream_handle(stream) -> int | None:
     if hasattr(stream, "cuda_stream"):
         return stream.cuda_stream
     return int(stream)
+
+
+def _host_rows(aln: np.ndarray) -> tuple[np.ndarray, int]:
+    """An (n, L) uint8 host alignment and its row pitch in bytes: a view whose rows are
+    contiguous (e.g. a column slice of a wider buffer) is passed as it lies, with its pitch."""
+    if aln.dtype == np.uint8 and aln.ndim == 2 and aln.strides[1] == 1 and aln.strides[0] >= aln.shape[1]:
+        return aln, aln.strides[0]
+    a = np.ascontiguousarray(aln, dtype=np.uint8)
+    return a, a.shape[1]
 
 
 @dataclass
@@ -288,9 +298,9 @@ class Context:
         """Pack an (n, L) uint8 alignment (numpy on the host, or a uint8 torch tensor on the device)."""
         h = _vp()
         if isinstance(aln, np.ndarray):
-            a = np.ascontiguousarray(aln, dtype=np.uint8)
+            a, pitch = _host_rows(aln)
             n, L = a.shape
-            check(lib().rnj_alignment_from_chars(self.handle, a.ctypes.data, L, n, L, 0,
+            check(lib().rnj_alignment_from_chars(self.handle, a.ctypes.data, pitch, n, L, 0,
                                                  int(skip_type_check), ctypes.byref(h)))
         else:
             import torch
@@ -359,6 +369,16 @@ class Context:
                                          (FLAG_RAW if raw else 0) | (FLAG_EXACT if exact else 0),
                                          max_out.data_ptr() if max_out is not None else None,
                                          _stream_handle(stream)))
+        return out
+
+    def to_host(self, t, stream=None) -> np.ndarray:
+        """A contiguous device tensor as a numpy array, copied through the library's pinned
+        staging chunks (rnj_copy_to_host) after the work enqueued on `stream`."""
+        import torch
+        assert t.is_cuda and t.is_contiguous()
+        out = np.empty(tuple(t.shape), dtype=torch.empty(0, dtype=t.dtype).numpy().dtype)
+        check(lib().rnj_copy_to_host(self.handle, out.ctypes.data, t.data_ptr(),
+                                     t.numel() * t.element_size(), _stream_handle(stream)))
         return out
 
     def counts(self, aln: Alignment, stream=None):
@@ -550,10 +570,10 @@ class Server:
         ctx._adopt(self)
 
     def submit(self, aln: np.ndarray, model="k2p") -> int:
-        a = np.ascontiguousarray(aln, dtype=np.uint8)
+        a, pitch = _host_rows(aln)
         n, L = a.shape
         t = ctypes.c_uint64()
-        check(lib().rnj_server_submit(self.handle, a.ctypes.data, L, n, L, _model(model),
+        check(lib().rnj_server_submit(self.handle, a.ctypes.data, pitch, n, L, _model(model),
                                       ctypes.byref(t)))
         return t.value
 
