@@ -107,12 +107,15 @@ hipError_t launch_lower_pack(const float* sq, size_t ld, unsigned n, float* out,
 unsigned tile_threads(int config);  // work-group size of a configuration
 
 hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
+// Column gathers of bootstrap replicates read per-site codes: one nibble per taxon (V<<2|H<<1|L),
+// two taxa per byte, site-major: codes_bytes(n_pad, L) bytes.
+inline size_t codes_bytes(unsigned n_pad, unsigned L) { return (size_t)L * n_pad / 2; }
 hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
-                                  unsigned L, uint8_t* codes, hipStream_t stream);
+                                  unsigned L, uint32_t* codes, hipStream_t stream);
 hipError_t launch_bootstrap_index(const uint16_t* weights, unsigned R, unsigned L,
                                   unsigned* col_index, unsigned idx_stride, unsigned* lengths,
                                   hipStream_t stream);
-hipError_t launch_bootstrap_gather(const uint8_t* codes, unsigned n_pad, const unsigned* col_index,
+hipError_t launch_bootstrap_gather(const uint32_t* codes, unsigned n_pad, const unsigned* col_index,
                                    unsigned idx_stride, const unsigned* lengths, unsigned R,
                                    unsigned W_pad, uint32_t* planes, hipStream_t stream);
 

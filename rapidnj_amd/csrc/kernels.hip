@@ -399,18 +399,27 @@ __global__ void patch_kernel(const JobDesc* __restrict__ jobs) {
 }
 
 // codes[s][t] = V << 2 | H << 1 | L for site s < L of taxon t.
+// Per-site codes of every taxon for the column gathers: nibbles, two taxa per byte, site-major
+// (codes[s][t / 2], taxon t in the low nibble when even), nibble = V << 2 | H << 1 | L. One
+// thread per (site, 8 taxa): one u32 of output.
 __global__ void planes_to_codes_kernel(const uint32_t* __restrict__ planes, unsigned n_pad,
-                                       unsigned W_pad, unsigned L, uint8_t* __restrict__ codes) {
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_pad) return;
+                                       unsigned W_pad, unsigned L, uint32_t* __restrict__ codes) {
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;  // group of 8 taxa
+  const unsigned groups = n_pad / 8;
+  if (g >= groups) return;
   const size_t ps = (size_t)W_pad * n_pad;
   for (unsigned s = blockIdx.y; s < L; s += gridDim.y) {
-    const size_t o = (size_t)(s / 32) * n_pad + t;
+    const size_t o = (size_t)(s / 32) * n_pad + 8 * g;
     const unsigned sh = s % 32;
-    const uint32_t h = (planes[o] >> sh) & 1u;
-    const uint32_t l = (planes[ps + o] >> sh) & 1u;
-    const uint32_t v = (planes[2 * ps + o] >> sh) & 1u;
-    codes[(size_t)s * n_pad + t] = (uint8_t)(v << 2 | h << 1 | l);
+    uint32_t x = 0;
+#pragma unroll
+    for (unsigned j = 0; j < 8; ++j) {
+      const uint32_t h = (planes[o + j] >> sh) & 1u;
+      const uint32_t l = (planes[ps + o + j] >> sh) & 1u;
+      const uint32_t v = (planes[2 * ps + o + j] >> sh) & 1u;
+      x |= (v << 2 | h << 1 | l) << (4 * j);
+    }
+    codes[(size_t)s * groups + g] = x;
   }
 }
 
@@ -453,32 +462,56 @@ __global__ __launch_bounds__(1024) void bootstrap_index_kernel(
   if (tid == 0) lengths[r] = carry_s;
 }
 
+// In-register transpose of a 32 x 32 bit matrix, bit c of a[r] <-> bit r of a[c] (recursive
+// swap of the off-diagonal blocks: 16 x 16, then 8 x 8 inside each, ... down to 1 x 1).
+__device__ __forceinline__ void transpose32(uint32_t (&a)[32]) {
+#pragma unroll
+  for (int j = 16, m = 0x0000FFFF; j != 0; j >>= 1, m ^= m << j) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      if (k & j) continue;
+      const uint32_t t = ((a[k] >> j) ^ a[k + j]) & (uint32_t)m;
+      a[k] ^= t << j;
+      a[k + j] ^= t;
+    }
+  }
+}
+
 // Replicate r's packed planes from the expanded column list: target site s takes source column
-// col_index[r][s]. Sites at or beyond lengths[r] are zero (not counted).
-__global__ void bootstrap_gather_kernel(const uint8_t* __restrict__ codes, unsigned n_pad,
-                                        const unsigned* __restrict__ col_index,
-                                        unsigned idx_stride, const unsigned* __restrict__ lengths,
-                                        unsigned W_pad, uint32_t* __restrict__ planes) {
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+// col_index[r][s]. Sites at or beyond lengths[r] are zero (not counted). A thread takes 8 taxa
+// and one 32-site word: 32 u32 loads (one per site, the 8 taxa's nibbles), a bit transpose that
+// turns them into 32 words of site bits (word 4j + p: taxon j's plane p), and 6 16-byte stores.
+__global__ __launch_bounds__(64) void bootstrap_gather_kernel(
+    const uint32_t* __restrict__ codes, unsigned n_pad, const unsigned* __restrict__ col_index,
+    unsigned idx_stride, const unsigned* __restrict__ lengths, unsigned W_pad,
+    uint32_t* __restrict__ planes) {
+  const unsigned groups = n_pad / 8;
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned r = blockIdx.z;
-  if (t >= n_pad) return;
+  if (g >= groups) return;
   const unsigned len = min(lengths[r], idx_stride);
   const unsigned* idx = col_index + (size_t)r * idx_stride;
-  for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
-  uint32_t H = 0, Lo = 0, V = 0;
-  for (unsigned k = 0; k < 32; ++k) {
-    const unsigned s = w * 32 + k;
-    if (s >= len) break;
-    const uint32_t c = codes[(size_t)idx[s] * n_pad + t];
-    Lo |= (c & 1u) << k;
-    H |= ((c >> 1) & 1u) << k;
-    V |= ((c >> 2) & 1u) << k;
-  }
   const size_t ps = (size_t)W_pad * n_pad;
   uint32_t* pr = planes + (size_t)r * 3 * ps;
-  pr[0 * ps + (size_t)w * n_pad + t] = H;
-  pr[1 * ps + (size_t)w * n_pad + t] = Lo;
-  pr[2 * ps + (size_t)w * n_pad + t] = V;
+  for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
+    uint32_t a[32];
+#pragma unroll
+    for (unsigned k = 0; k < 32; ++k) {
+      const unsigned s = w * 32 + k;
+      a[k] = s < len ? codes[(size_t)idx[s] * groups + g] : 0u;
+    }
+    transpose32(a);
+    // a[4j + p] = taxon (8g + j), plane p (0: L, 1: H, 2: V) over the word's 32 sites
+    const size_t o = (size_t)w * n_pad + 8 * g;
+    uint4* h = reinterpret_cast<uint4*>(pr + o);
+    uint4* l = reinterpret_cast<uint4*>(pr + ps + o);
+    uint4* v = reinterpret_cast<uint4*>(pr + 2 * ps + o);
+    h[0] = make_uint4(a[1], a[5], a[9], a[13]);
+    h[1] = make_uint4(a[17], a[21], a[25], a[29]);
+    l[0] = make_uint4(a[0], a[4], a[8], a[12]);
+    l[1] = make_uint4(a[16], a[20], a[24], a[28]);
+    v[0] = make_uint4(a[2], a[6], a[10], a[14]);
+    v[1] = make_uint4(a[18], a[22], a[26], a[30]);
   }
 }
 
@@ -554,8 +587,9 @@ hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream)
 }
 
 hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
-                                  unsigned L, uint8_t* codes, hipStream_t stream) {
-  planes_to_codes_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(L)), dim3(256), 0, stream>>>(
+                                  unsigned L, uint32_t* codes, hipStream_t stream) {
+  const unsigned groups = n_pad / 8;
+  planes_to_codes_kernel<<<dim3((groups + 255) / 256, ygrid(L)), dim3(256), 0, stream>>>(
       planes, n_pad, W_pad, L, codes);
   return hipGetLastError();
 }
@@ -568,11 +602,12 @@ hipError_t launch_bootstrap_index(const uint16_t* weights, unsigned R, unsigned 
   return hipGetLastError();
 }
 
-hipError_t launch_bootstrap_gather(const uint8_t* codes, unsigned n_pad, const unsigned* col_index,
+hipError_t launch_bootstrap_gather(const uint32_t* codes, unsigned n_pad, const unsigned* col_index,
                                    unsigned idx_stride, const unsigned* lengths, unsigned R,
                                    unsigned W_pad, uint32_t* planes, hipStream_t stream) {
-  bootstrap_gather_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(W_pad), R), dim3(256), 0,
-                            stream>>>(codes, n_pad, col_index, idx_stride, lengths, W_pad, planes);
+  const unsigned groups = n_pad / 8;
+  bootstrap_gather_kernel<<<dim3((groups + 63) / 64, ygrid(W_pad), R), dim3(64), 0, stream>>>(
+      codes, n_pad, col_index, idx_stride, lengths, W_pad, planes);
   return hipGetLastError();
 }
 

@@ -8,8 +8,11 @@
 //   -> job table copy -> statistics reset -> tile kernel (-> finish kernel) -> saturation patch.
 // Every replicate uses the word count of the plan's maximum replicate length; the gather writes
 // zero words past each replicate's own length, so shorter replicates count exactly their sites.
-// rnj_bootstrap_plan_run copies new weights into the pinned buffer and replays the graph. The
-// graph holds no memset node: statistics are cleared by a kernel.
+// The body is captured twice, once per pinned weight buffer. rnj_bootstrap_plan_run copies new
+// weights into the buffer of its turn (after the replay that last read it, two runs back, has
+// finished) and replays that buffer's graph behind the previous replay (a device-side wait, so
+// runs on different streams still follow one another while the host prepares the next run).
+// The graph holds no memset node: statistics are cleared by a kernel.
 #include <cstring>
 
 #include "runtime.hpp"
@@ -20,11 +23,11 @@ struct rnj_plan {
   Context* ctx = nullptr;
   const rnj_alignment* aln = nullptr;
   unsigned R = 0, L = 0, Lmax = 0, Wr = 0;
-  uint16_t* h_weights = nullptr;  // pinned
+  uint16_t* h_weights[2] = {nullptr, nullptr};  // pinned, one per captured graph
   uint16_t* d_weights = nullptr;
   unsigned* d_idx = nullptr;
   unsigned* d_len = nullptr;
-  uint8_t* d_codes = nullptr;
+  uint32_t* d_codes = nullptr;
   uint32_t* d_planes = nullptr;
   Workspace ws;
   std::vector<JobSpec> specs;
@@ -32,9 +35,10 @@ struct rnj_plan {
   int config = 0;
   bool raw = false, exact = false;
   hipStream_t stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipEvent_t done = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};  // recorded after the replay of each graph
+  int turn = 0;                             // graph of the next run
 };
 
 namespace {
@@ -43,23 +47,27 @@ void destroy_plan(rnj_plan* p) {
   if (!p) return;
   if (p->ctx) (void)hipSetDevice(p->ctx->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
-  if (p->done) (void)hipEventSynchronize(p->done);
-  if (p->exec) (void)hipGraphExecDestroy(p->exec);
-  if (p->graph) (void)hipGraphDestroy(p->graph);
-  if (p->h_weights) (void)hipHostFree(p->h_weights);
+  for (int k = 0; k < 2; ++k) {
+    if (p->done[k]) (void)hipEventSynchronize(p->done[k]);
+    if (p->exec[k]) (void)hipGraphExecDestroy(p->exec[k]);
+    if (p->graph[k]) (void)hipGraphDestroy(p->graph[k]);
+    if (p->h_weights[k]) (void)hipHostFree(p->h_weights[k]);
+  }
   for (void* d : {(void*)p->d_weights, (void*)p->d_idx, (void*)p->d_len, (void*)p->d_codes,
                   (void*)p->d_planes})
     if (d) (void)hipFree(d);
   free_workspace_public(&p->ws);
-  if (p->done) (void)hipEventDestroy(p->done);
+  for (hipEvent_t ev : p->done)
+    if (ev) (void)hipEventDestroy(ev);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
 
-// The launch body, enqueued on p->stream (eagerly or under capture).
-int enqueue_plan(rnj_plan* p) {
+// The launch body with pinned weight buffer `buf`, enqueued on p->stream (eagerly or under
+// capture).
+int enqueue_plan(rnj_plan* p, int buf) {
   const rnj_alignment* a = p->aln;
-  RNJ_HIP(hipMemcpyAsync(p->d_weights, p->h_weights, (size_t)p->R * p->L * sizeof(uint16_t),
+  RNJ_HIP(hipMemcpyAsync(p->d_weights, p->h_weights[buf], (size_t)p->R * p->L * sizeof(uint16_t),
                          hipMemcpyHostToDevice, p->stream));
   RNJ_HIP(launch_bootstrap_index(p->d_weights, p->R, p->L, p->d_idx, p->Lmax, p->d_len, p->stream));
   RNJ_HIP(launch_bootstrap_gather(p->d_codes, a->n_pad, p->d_idx, p->Lmax, p->d_len, p->R, p->Wr,
@@ -110,13 +118,16 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
   p->exact = (flags & RNJ_FLAG_EXACT) != 0;
   const size_t plane_words = (size_t)3 * p->Wr * aln->n_pad;
   hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done, hipEventDisableTiming);
-  if (e == hipSuccess)
-    e = hipHostMalloc((void**)&p->h_weights, (size_t)R * L * sizeof(uint16_t), hipHostMallocDefault);
+  for (int k = 0; k < 2; ++k) {
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done[k], hipEventDisableTiming);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&p->h_weights[k], (size_t)R * L * sizeof(uint16_t),
+                        hipHostMallocDefault);
+  }
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_weights, (size_t)R * L * sizeof(uint16_t));
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_idx, (size_t)R * p->Lmax * sizeof(unsigned));
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_len, (size_t)R * sizeof(unsigned));
-  if (e == hipSuccess) e = hipMalloc((void**)&p->d_codes, (size_t)L * aln->n_pad);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->d_codes, codes_bytes(aln->n_pad, L));
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_planes, (size_t)R * plane_words * 4);
   if (e == hipSuccess)
     e = launch_planes_to_codes(aln->planes, aln->n_pad, aln->W, L, p->d_codes, p->stream);
@@ -140,27 +151,28 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
   p->shape = shape_of(p->specs);
   int rc = prepare_workspace(&p->ws, R, p->shape.nsent);
   // the original alignment as every replicate: a valid eager run that settles the configuration
-  for (size_t k = 0; k < (size_t)R * L; ++k) p->h_weights[k] = 1;
+  for (size_t k = 0; k < (size_t)R * L; ++k) p->h_weights[0][k] = 1;
   if (rc == RNJ_OK)
     rc = select_config(ctx, p->specs, p->shape, &p->ws, p->exact, p->stream, true, &p->config);
-  if (rc == RNJ_OK) rc = enqueue_plan(p);
+  if (rc == RNJ_OK) rc = enqueue_plan(p, 0);
   if (rc == RNJ_OK && hipStreamSynchronize(p->stream) != hipSuccess)
     rc = fail(RNJ_E_HIP, "eager plan run failed");
-  if (rc == RNJ_OK) {
+  for (int k = 0; k < 2 && rc == RNJ_OK; ++k) {
     e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed);
-    if (e != hipSuccess) rc = hip_fail(e, "hipStreamBeginCapture");
-  }
-  if (rc == RNJ_OK) {
-    const int body = enqueue_plan(p);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "hipStreamBeginCapture");
+      break;
+    }
+    const int body = enqueue_plan(p, k);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(p->stream, &g);
-    p->graph = g;
+    p->graph[k] = g;
     if (body != RNJ_OK) rc = body;
     else if (e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
-  }
-  if (rc == RNJ_OK) {
-    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    if (rc == RNJ_OK) {
+      e = hipGraphInstantiate(&p->exec[k], p->graph[k], nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
   }
   if (rc != RNJ_OK) {
     destroy_plan(p);
@@ -175,11 +187,14 @@ int rnj_bootstrap_plan_run(rnj_plan* p, const uint16_t* weights, void* stream) {
   int rc = check_weights(p, weights);
   if (rc != RNJ_OK) return rc;
   RNJ_HIP(hipSetDevice(p->ctx->device));
-  RNJ_HIP(hipEventSynchronize(p->done));  // the previous replay has read the pinned weights
-  std::memcpy(p->h_weights, weights, (size_t)p->R * p->L * sizeof(uint16_t));
+  const int k = p->turn;
+  RNJ_HIP(hipEventSynchronize(p->done[k]));  // the replay two runs back has read buffer k
+  std::memcpy(p->h_weights[k], weights, (size_t)p->R * p->L * sizeof(uint16_t));
   hipStream_t s = (hipStream_t)stream;
-  RNJ_HIP(hipGraphLaunch(p->exec, s));
-  RNJ_HIP(hipEventRecord(p->done, s));
+  RNJ_HIP(hipStreamWaitEvent(s, p->done[1 - k], 0));  // after the previous replay, on any stream
+  RNJ_HIP(hipGraphLaunch(p->exec[k], s));
+  RNJ_HIP(hipEventRecord(p->done[k], s));
+  p->turn = 1 - k;
   return RNJ_OK;
 }
 

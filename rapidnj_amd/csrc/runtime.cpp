@@ -1169,12 +1169,12 @@ int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
   const size_t plane_words = (size_t)3 * Wr * aln->n_pad;
   uint16_t* dw = nullptr;
   unsigned *didx = nullptr, *dlen = nullptr;
-  uint8_t* dcodes = nullptr;
+  uint32_t* dcodes = nullptr;
   uint32_t* dplanes = nullptr;
   hipError_t e = hipMallocAsync((void**)&dw, (size_t)R * L * sizeof(uint16_t), s);
   if (e == hipSuccess) e = hipMallocAsync((void**)&didx, (size_t)R * maxlen * sizeof(unsigned), s);
   if (e == hipSuccess) e = hipMallocAsync((void**)&dlen, (size_t)R * sizeof(unsigned), s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&dcodes, (size_t)L * aln->n_pad, s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dcodes, codes_bytes(aln->n_pad, L), s);
   if (e == hipSuccess) e = hipMallocAsync((void**)&dplanes, (size_t)R * plane_words * 4, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(dw, weights, (size_t)R * L * sizeof(uint16_t), hipMemcpyHostToDevice, s);
