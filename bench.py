@@ -379,13 +379,14 @@ def served_run(ctx, jobs, alns, open_loop: bool, slots: int = 2, max_batch: int 
 
     th = threading.Thread(target=submitter)
     th.start()
-    lat, batch_jobs = [], []
+    lat, batch_jobs, stages = [], [], []
     for k, j in enumerate(jobs):
         while tickets[k] is None:
             time.sleep(0.0002)
         _, st = srv.wait(tickets[k], j.n)
         lat.append(st.total_s)
         batch_jobs.append(st.batch_jobs)
+        stages.append((st.queued_s, st.upload_s, st.launch_s, st.run_s, st.download_s))
     makespan = time.perf_counter() - t0
     th.join()
     srv.close()
@@ -395,7 +396,10 @@ def served_run(ctx, jobs, alns, open_loop: bool, slots: int = 2, max_batch: int 
             "value": round(sp / makespan / 1e9, 2), "unit": "G site-pairs/s",
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
             "p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
-            "mean_batch_jobs": round(float(np.mean(batch_jobs)), 2)}
+            "mean_batch_jobs": round(float(np.mean(batch_jobs)), 2),
+            "stage_p50_ms": dict(zip(("queued", "upload", "launch", "run", "download"),
+                                     (round(float(v), 3) for v in
+                                      np.percentile(np.array(stages) * 1e3, 50, axis=0))))}
 
 
 def one_job_per_launch(ctx, jobs, alns) -> dict:

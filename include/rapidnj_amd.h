@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define RNJ_API_VERSION 1
+#define RNJ_API_VERSION 2
 
 /* status codes */
 enum {
@@ -307,6 +307,11 @@ typedef struct {
   double total_s;    /* submit -> results on the host */
   unsigned batch_id; /* launch the job ran in */
   unsigned batch_jobs;
+  /* stages of device_s */
+  double upload_s;   /* admitted -> rows read into the staging chunks, copies and packing enqueued */
+  double launch_s;   /* -> the job's batch enqueued on a slot */
+  double run_s;      /* -> the batch seen finished by the request loop */
+  double download_s; /* the copy of the triangle into tri_out, inside rnj_server_wait */
 } rnj_job_stats;
 
 RNJ_EXPORT int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server** out);

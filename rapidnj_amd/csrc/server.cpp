@@ -41,11 +41,12 @@ struct ServedJob {
   State state = State::Queued;
   int status = RNJ_OK;
   std::string error;
-  Clock::time_point t_submit, t_admit, t_ready, t_done;
+  Clock::time_point t_submit, t_admit, t_ready, t_launch, t_done;
   unsigned batch_id = 0, batch_jobs = 0;
   size_t need = 0;
-  uint32_t* planes = nullptr;  // device, from admission until its batch retires
-  float* tri = nullptr;
+  uint8_t* chars_dev = nullptr;  // device blocks from the server's cache: characters and planes
+  uint32_t* planes = nullptr;    // from admission until the job's batch retires,
+  float* tri = nullptr;          // the triangle until rnj_server_wait has copied it
   hipEvent_t ready = nullptr;  // recorded after packing
 };
 
@@ -62,11 +63,82 @@ struct Slot {
 size_t pairs_of(unsigned n) { return (size_t)n * (n - 1) / 2; }
 
 size_t tri_bytes(unsigned n) { return pairs_of(n) * 4; }
+size_t chars_bytes(unsigned n, unsigned L) { return (size_t)n * L; }
+size_t planes_bytes(unsigned n, unsigned L) {
+  return (size_t)3 * words_for_sites(L) * round_up(n, kTaxaPad) * 4;
+}
 
+// Device blocks by size class, kept after a job for the next job of a similar size. HIP's
+// stream-ordered allocator costs 0.1-0.3 ms per call when sizes vary from call to call (three
+// calls per job: more than a small job's whole run). A block goes back to the cache only once
+// its last use on the device is known to be complete. Size classes: four per power of two
+// (at most 25 % above the request), at least 64 KiB. Idle blocks beyond `cap` bytes are freed
+// at once; a failed allocation frees every idle block and tries again.
+class DeviceCache {
+ public:
+  static size_t bucket(size_t bytes) {
+    constexpr size_t kMin = (size_t)64 << 10;
+    if (bytes <= kMin) return kMin;
+    const int lg = 63 - __builtin_clzll((unsigned long long)(bytes - 1));
+    const size_t step = (size_t)1 << (lg - 2);
+    return (bytes + step - 1) / step * step;
+  }
+  void set_cap(size_t cap) { cap_ = cap; }
+  hipError_t get(size_t bytes, void** out) {
+    const size_t b = bucket(bytes);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto it = free_.find(b);
+      if (it != free_.end()) {
+        *out = it->second;
+        free_.erase(it);
+        idle_ -= b;
+        return hipSuccess;
+      }
+    }
+    hipError_t e = hipMalloc(out, b);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(out, b);
+    }
+    if (e != hipSuccess) *out = nullptr;
+    return e;
+  }
+  void put(void* p, size_t bytes) {
+    if (!p) return;
+    const size_t b = bucket(bytes);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      if (idle_ + b <= cap_) {
+        free_.emplace(b, p);
+        idle_ += b;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+  void trim() {
+    std::multimap<size_t, void*> blocks;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      blocks.swap(free_);
+      idle_ = 0;
+    }
+    for (auto& kv : blocks) (void)hipFree(kv.second);
+  }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+  size_t idle_ = 0, cap_ = 0;
+};
+
+// HBM a job holds from admission until its batch retires (in the cache's size classes; the
+// saturated-pair list lives in the launch's workspace).
 size_t job_bytes(unsigned n, unsigned L) {
-  const size_t n_pad = round_up(n, kTaxaPad);
-  return (size_t)n * L + (size_t)3 * words_for_sites(L) * n_pad * 4 + pairs_of(n) * 4 +
-         sentinel_capacity(n) * 8;
+  return DeviceCache::bucket(chars_bytes(n, L)) + DeviceCache::bucket(planes_bytes(n, L)) +
+         DeviceCache::bucket(tri_bytes(n)) + sentinel_capacity(n) * 8;
 }
 
 }  // namespace
@@ -81,6 +153,7 @@ struct rnj_server {
   std::mutex upload_mu;  // one upload at a time on the upload stream
   Stager uploader;       // pinned staging for the callers' characters (submitting threads)
   Stager downloader;     // and for their results (waiting threads)
+  DeviceCache blocks;    // jobs' device buffers
   std::condition_variable cv_work, cv_done;
   std::deque<ServedJob*> queue;  // waiting for admission (host copies)
   std::deque<ServedJob*> ready;  // admitted: uploading or uploaded, in arrival order
@@ -94,7 +167,7 @@ struct rnj_server {
   void run();
   int upload(ServedJob* j, const char* chars, size_t pitch);
   bool launch(Slot& s);
-  void release_job_memory(ServedJob* j, hipStream_t stream);
+  void release_job_memory(ServedJob* j);
 };
 
 // Upload and pack one admitted job (no server lock held). chars: n rows at `pitch` on the host.
@@ -103,14 +176,13 @@ struct rnj_server {
 int rnj_server::upload(ServedJob* j, const char* chars, size_t pitch) {
   std::lock_guard<std::mutex> ul(upload_mu);
   hipStream_t s = upload_stream;
-  uint8_t* dchars = nullptr;
-  hipError_t e = hipMallocAsync((void**)&dchars, (size_t)j->n * j->L, s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&j->planes, (size_t)3 * j->W * j->n_pad * 4, s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&j->tri, pairs_of(j->n) * 4, s);
+  hipError_t e = blocks.get(chars_bytes(j->n, j->L), (void**)&j->chars_dev);
+  if (e == hipSuccess) e = blocks.get(planes_bytes(j->n, j->L), (void**)&j->planes);
+  if (e == hipSuccess) e = blocks.get(tri_bytes(j->n), (void**)&j->tri);
   // through the pinned staging chunks; returns once the caller's rows have been read
-  if (e == hipSuccess) e = uploader.upload(dchars, chars, pitch, j->L, j->n, s);
-  if (e == hipSuccess) e = launch_pack_chars(dchars, j->L, j->n, j->L, j->n_pad, j->W, j->planes, s);
-  if (dchars) (void)hipFreeAsync(dchars, s);
+  if (e == hipSuccess) e = uploader.upload(j->chars_dev, chars, pitch, j->L, j->n, s);
+  if (e == hipSuccess)
+    e = launch_pack_chars(j->chars_dev, j->L, j->n, j->L, j->n_pad, j->W, j->planes, s);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&j->ready, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventRecord(j->ready, s);
   if (e != hipSuccess) {
@@ -121,15 +193,19 @@ int rnj_server::upload(ServedJob* j, const char* chars, size_t pitch) {
   return RNJ_OK;
 }
 
-void rnj_server::release_job_memory(ServedJob* j, hipStream_t stream) {
-  if (j->planes) (void)hipFreeAsync(j->planes, stream);
-  if (j->tri) (void)hipFreeAsync(j->tri, stream);
+// Every device block of the job back to the cache. Callers ensure that no work on the device
+// still uses them.
+void rnj_server::release_job_memory(ServedJob* j) {
+  blocks.put(j->chars_dev, chars_bytes(j->n, j->L));
+  blocks.put(j->planes, planes_bytes(j->n, j->L));
+  blocks.put(j->tri, tri_bytes(j->n));
+  j->chars_dev = nullptr;
   j->planes = nullptr;
   j->tri = nullptr;
 }
 
-// Memory a job holds after its batch ran: only its triangle (freed by rnj_server_wait).
-static size_t held_after_batch(const ServedJob* j) { return tri_bytes(j->n); }
+// Memory a job holds after its batch ran: only its triangle (released by rnj_server_wait).
+static size_t held_after_batch(const ServedJob* j) { return DeviceCache::bucket(tri_bytes(j->n)); }
 
 // Enqueue the batch in s.jobs on slot s (worker thread, no server lock). Returns false with the
 // slot's error set if enqueueing failed.
@@ -151,10 +227,6 @@ bool rnj_server::launch(Slot& s) {
   }
   int rc = RNJ_OK;
   if (e == hipSuccess) rc = launch_jobs(ctx, specs, false, false, s.stream, false);
-  for (ServedJob* j : s.jobs) {  // the packed planes are no longer needed; the triangle stays
-    if (j->planes) (void)hipFreeAsync(j->planes, s.stream);
-    j->planes = nullptr;
-  }
   if (e == hipSuccess && rc == RNJ_OK) e = hipEventRecord(s.done, s.stream);
   if (e == hipSuccess && rc == RNJ_OK) return true;
   s.error = e != hipSuccess ? std::string("batch launch: ") + hipGetErrorString(e)
@@ -168,13 +240,29 @@ void rnj_server::run() {
   (void)hipSetDevice(ctx->device);
   std::unique_lock<std::mutex> lock(mu);
   for (;;) {
-    // 1. batches that finished on the device
-    std::vector<Slot*> finished, failed;
+    // 1. batches that finished on the device: their jobs are done at once (before any host
+    //    work of this pass), and the slot takes a new batch in step 3
+    std::vector<Slot*> failed;
+    bool finished = false;
     for (Slot& s : slots) {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.done);
       if (q == hipSuccess) {
-        finished.push_back(&s);
+        const Clock::time_point now = Clock::now();
+        for (ServedJob* j : s.jobs) {
+          j->status = RNJ_OK;
+          j->state = State::Done;
+          j->t_done = now;
+          used -= j->need - held_after_batch(j);
+          // the batch waited for the job's upload and packing: characters and planes are free
+          blocks.put(j->chars_dev, chars_bytes(j->n, j->L));
+          blocks.put(j->planes, planes_bytes(j->n, j->L));
+          j->chars_dev = nullptr;
+          j->planes = nullptr;
+        }
+        s.busy = false;
+        s.jobs.clear();
+        finished = true;
       } else if (q != hipErrorNotReady) {
         s.error = std::string("batch failed: ") + hipGetErrorString(q);
         s.error_status = RNJ_E_HIP;
@@ -230,8 +318,8 @@ void rnj_server::run() {
     }
     bool busy = false;
     for (Slot& s : slots) busy = busy || s.busy;
-    const bool idle = queue.empty() && ready.empty() && to_upload.empty() && finished.empty() &&
-                      failed.empty();
+    if (finished) cv_done.notify_all();
+    const bool idle = queue.empty() && ready.empty() && to_upload.empty() && failed.empty();
     if (stopping && idle && !busy) break;
     lock.unlock();
 
@@ -241,8 +329,11 @@ void rnj_server::run() {
       j->chars.clear();
       j->chars.shrink_to_fit();
     }
-    for (Slot* s : launching)
+    for (Slot* s : launching) {
       if (!launch(*s)) failed.push_back(s);
+      const Clock::time_point t = Clock::now();
+      for (ServedJob* j : s->jobs) j->t_launch = t;
+    }
 
     lock.lock();
     const Clock::time_point now = Clock::now();
@@ -252,35 +343,26 @@ void rnj_server::run() {
         j->t_ready = now;
         ready.push_back(j);
       } else {
-        release_job_memory(j, upload_stream);
+        release_job_memory(j);  // the failed upload synchronised its stream
         used -= j->need;
         j->state = State::Done;
         j->t_done = now;
       }
     }
-    for (Slot* s : finished) {
-      for (ServedJob* j : s->jobs) {
-        j->status = RNJ_OK;
-        j->state = State::Done;
-        j->t_done = now;
-        used -= j->need - held_after_batch(j);
-      }
-      s->busy = false;
-      s->jobs.clear();
-    }
     for (Slot* s : failed) {
+      (void)hipStreamSynchronize(s->stream);  // nothing of the batch still runs
       for (ServedJob* j : s->jobs) {
         j->status = s->error_status;
         j->error = s->error;
         j->state = State::Done;
         j->t_done = now;
-        release_job_memory(j, s->stream);
+        release_job_memory(j);
         used -= j->need;
       }
       s->busy = false;
       s->jobs.clear();
     }
-    if (!finished.empty() || !failed.empty() || !to_upload.empty()) cv_done.notify_all();
+    if (!failed.empty() || !to_upload.empty()) cv_done.notify_all();
     bool busy_now = false;
     for (Slot& s : slots) busy_now = busy_now || s.busy;
     if (busy_now || !ready.empty())  // a batch, an upload or a batch window in flight
@@ -309,6 +391,7 @@ int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server
     }
     srv->cfg.hbm_budget = (size_t)(0.8 * (double)free_b);
   }
+  srv->blocks.set_cap(srv->cfg.hbm_budget / 4);
   srv->slots.resize(srv->cfg.slots);
   hipError_t e = hipStreamCreateWithFlags(&srv->upload_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&srv->download_stream, hipStreamNonBlocking);
@@ -379,7 +462,7 @@ int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned
           srv->ready.erase(it);
           break;
         }
-      srv->release_job_memory(j, srv->upload_stream);
+      srv->release_job_memory(j);  // the failed upload synchronised its stream
       srv->used -= j->need;
       j->status = rc;
       j->state = State::Done;
@@ -415,11 +498,12 @@ int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_st
   srv->jobs.erase(it);
   lock.unlock();
   int status = j->status;
+  const Clock::time_point copy_start = Clock::now();
   if (j->tri) {  // finished: the triangle is resident; copy it once into the caller's buffer
     hipError_t e = hipSuccess;
     if (status == RNJ_OK && tri_out)
       e = srv->downloader.download(tri_out, j->tri, tri_bytes(j->n), srv->download_stream);
-    (void)hipFreeAsync(j->tri, srv->download_stream);
+    srv->blocks.put(j->tri, tri_bytes(j->n));  // the download is complete (or its stream drained)
     j->tri = nullptr;
     {
       std::lock_guard<std::mutex> relock(srv->mu);
@@ -440,6 +524,11 @@ int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_st
     stats->total_s = sec(copied - j->t_submit);
     stats->batch_id = j->batch_id;
     stats->batch_jobs = j->batch_jobs;
+    const bool ran = j->t_launch.time_since_epoch().count() != 0;
+    stats->upload_s = admitted && ran ? sec(j->t_ready - j->t_admit) : 0.0;
+    stats->launch_s = ran ? sec(j->t_launch - j->t_ready) : 0.0;
+    stats->run_s = ran ? sec(j->t_done - j->t_launch) : 0.0;
+    stats->download_s = sec(copied - copy_start);
   }
   if (status != RNJ_OK) set_error(j->error);
   if (j->ready) (void)hipEventDestroy(j->ready);
@@ -483,21 +572,22 @@ int rnj_server_destroy(rnj_server* srv) {
   srv->cv_work.notify_all();
   srv->cv_done.notify_all();
   if (srv->worker.joinable()) srv->worker.join();
-  for (Slot& s : srv->slots) {
+  for (Slot& s : srv->slots)
     if (s.stream) (void)hipStreamSynchronize(s.stream);
-    if (s.stream) (void)hipStreamDestroy(s.stream);
-    if (s.done) (void)hipEventDestroy(s.done);
-  }
+  for (hipStream_t st : {srv->upload_stream, srv->download_stream})
+    if (st) (void)hipStreamSynchronize(st);
   for (auto& kv : srv->jobs) {  // results never waited for
-    srv->release_job_memory(kv.second, srv->download_stream);
+    srv->release_job_memory(kv.second);
     if (kv.second->ready) (void)hipEventDestroy(kv.second->ready);
     delete kv.second;
   }
+  srv->blocks.trim();
+  for (Slot& s : srv->slots) {
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
   for (hipStream_t st : {srv->upload_stream, srv->download_stream})
-    if (st) {
-      (void)hipStreamSynchronize(st);
-      (void)hipStreamDestroy(st);
-    }
+    if (st) (void)hipStreamDestroy(st);
   delete srv;
   return RNJ_OK;
 }

@@ -102,7 +102,9 @@ class ServerConfig(ctypes.Structure):
 
 class JobStats(ctypes.Structure):
     _fields_ = [("queued_s", ctypes.c_double), ("device_s", ctypes.c_double),
-                ("total_s", ctypes.c_double), ("batch_id", _u), ("batch_jobs", _u)]
+                ("total_s", ctypes.c_double), ("batch_id", _u), ("batch_jobs", _u),
+                ("upload_s", ctypes.c_double), ("launch_s", ctypes.c_double),
+                ("run_s", ctypes.c_double), ("download_s", ctypes.c_double)]
 
 
 class RnjError(RuntimeError):

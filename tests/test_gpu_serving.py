@@ -72,12 +72,27 @@ def test_admission_rejects_jobs_that_never_fit(ctx, N):
         assert e.value.status == N.RNJ_E_ADMISSION
 
 
+def _size_class(nbytes):
+    """The server's device blocks: four size classes per power of two, at least 64 KiB."""
+    if nbytes <= 64 << 10:
+        return 64 << 10
+    step = 1 << ((nbytes - 1).bit_length() - 1 - 2)
+    return -(-nbytes // step) * step
+
+
 def _job_bytes(n, L):
-    """HBM a served job holds: characters, planes, triangle, saturated-pair list."""
+    """HBM a served job holds: characters, planes and triangle (in the server's size classes)
+    and the saturated-pair list."""
     n_pad = -(-n // 128) * 128
     W = -(-max(1, -(-L // 32)) // 16) * 16
     pairs = n * (n - 1) // 2
-    return n * L + 3 * W * n_pad * 4 + pairs * 4 + min(max(pairs // 64, 1024), 1 << 24) * 8
+    return (_size_class(n * L) + _size_class(3 * W * n_pad * 4) + _size_class(pairs * 4)
+            + min(max(pairs // 64, 1024), 1 << 24) * 8)
+
+
+def test_size_classes():
+    assert [_size_class(b) for b in (1, 65536, 65537, 100000, 1 << 20, (1 << 20) + 1)] == \
+        [65536, 65536, 81920, 114688, 1 << 20, 1310720]
 
 
 def test_budget_serialises_but_completes(ctx):

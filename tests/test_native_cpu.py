@@ -54,7 +54,7 @@ def test_binding_covers_every_entry_point(native):
 
 def test_calls_without_a_device(native):
     L = native.lib()
-    assert L.rnj_api_version() == 1
+    assert L.rnj_api_version() == 2
     assert L.rnj_status_string(native.RNJ_E_NOT_DNA) == b"RNJ_E_NOT_DNA"
     assert L.rnj_tile_config_count() >= 1
     assert all(native.tile_configs())
