@@ -23,10 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "G site-pairs/s on distance matrix vs CPU; p50/p99 job latency served"
-# int32 VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (peak engine clock)
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 HBM_PEAK_GBS = 8000.0
-OPS_PER_WORD_PAIR = {"k2p": 7, "jc": 6}  # VALU instructions per 32 sites and pair
+# VALU instructions per 32 sites and pair of the count loop, by class: (bitwise, v_bcnt).
+# K2P: v_and, v_bitop3 x3 (one folds the xor) and 3 v_bcnt; JC: v_and, 2 v_bitop3, v_or, 2 v_bcnt.
+OP_CLASSES = {"k2p": (4, 3), "jc": (4, 2)}
 
 
 def log(msg: str) -> None:
@@ -233,11 +233,28 @@ def handoff_compare(ctx, A) -> dict:
     return out
 
 
-def ops_per_word_pair(A, model: str) -> int:
-    """VALU instructions per 32 sites and pair of the tile kernel that counts alignment A."""
+def op_classes(A, model: str) -> tuple[int, int]:
+    """(bitwise, v_bcnt) VALU instructions per 32 sites and pair of the tile kernel that counts
+    alignment A."""
     if A.is_protein:
-        return A.info.nplanes + 2  # B code planes: v_xor, B - 1 v_bitop3, v_and, 2 v_bcnt
-    return OPS_PER_WORD_PAIR[model]
+        return A.info.nplanes, 2  # B code planes: v_xor, B - 1 v_bitop3, v_and; 2 v_bcnt
+    return OP_CLASSES[model]
+
+
+def ops_per_word_pair(A, model: str) -> int:
+    return sum(op_classes(A, model))
+
+
+def valu_peak(rates, classes) -> dict:
+    """Int32 VALU ceiling of a count loop issuing `classes` = (bitwise, bcnt) instructions per
+    32 site-pairs, from the device's measured issue rate of each class (rnj_valu_issue_rates):
+    (b + c) / (b / bitwise + c / bcnt) lane-ops/s. No data sheet states an int32 VALU rate; the
+    classes run at different rates (v_bcnt at about half the bitwise rate) and share the VALU."""
+    bw, bc = rates
+    b, c = classes
+    peak = (b + c) / (b / bw + c / bc)
+    return {"peak_tops": peak / 1e12, "bitwise_tops": bw / 1e12, "bcnt_tops": bc / 1e12,
+            "classes": {"bitwise": b, "bcnt": c}}
 
 
 def protein_side(ctx, stream, cpu: bool) -> dict:
@@ -271,12 +288,15 @@ def protein_side(ctx, stream, cpu: bool) -> dict:
     kern_ms = e0.elapsed_time(e1) / k
     ops = ops_per_word_pair(A, wl.model) * pairs * (L / 32.0)
     tops = ops / (kern_ms / 1e3) / 1e12
+    pk = valu_peak(ctx.valu_issue_rates(), op_classes(A, wl.model))
     res = {"workload": wl.name, "code_planes": A.info.nplanes - 1,
            "ms_per_step": round(ms, 3), "value": round(pairs * L / (ms / 1e3) / 1e9, 2),
            "unit": "G site-pairs/s", "kernel_ms": round(kern_ms, 3),
-           "valu": {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-                    "frac": round(tops / VALU_PEAK_TOPS, 4),
-                    "ops_per_32_site_pairs": ops_per_word_pair(A, wl.model)},
+           "valu": {"achieved": round(tops, 3), "peak": round(pk["peak_tops"], 2), "unit": "TOP/s",
+                    "frac": round(tops / pk["peak_tops"], 4),
+                    "ops_per_32_site_pairs": ops_per_word_pair(A, wl.model),
+                    "peak_model": {k: (round(v, 2) if isinstance(v, float) else v)
+                                   for k, v in pk.items() if k != "peak_tops"}},
            "tile_config": N.tile_configs()[ctx.tile_config_for(A, wl.model)]}
     if cpu:
         res["cpu_reference"] = cpu_baseline(aln, wl.model, 10000, cpu_cores(), protein=True)
@@ -478,9 +498,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large",
                     choices=["large", "plain", "small_jc", "bootstrap", "served", "protein"])
-    ap.add_argument("--served-jobs", type=int, default=0,
-                    help="jobs of the served stream (default: 512 for --workload served, 128 as "
-                         "the side measurement of the default run)")
+    ap.add_argument("--served-jobs", type=int, default=512,
+                    help="jobs of the served stream (the first this many of the 512-job stream)")
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 2000 x 10000 side run")
@@ -511,7 +530,7 @@ def main() -> int:
 
     served_jobs, served_alns = [], []
     if args.workload == "served" or not args.no_extra:
-        count = args.served_jobs or (512 if args.workload == "served" else 128)
+        count = max(1, min(512, args.served_jobs))
         served_jobs = served_stream()[:count]
         log(f"served stream: generating {count} jobs in a process pool")
         served_alns = generate_stream(served_jobs, cpu_cores())
@@ -579,8 +598,11 @@ def main() -> int:
     achieved = ops / (kern_ms / 1e3) / 1e12
     algo_bytes = pairs * 4 * R + n * planes * L / 8 * R
     cfg = ctx.tile_config_for(A, wl.model)
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+    pk = valu_peak(ctx.valu_issue_rates(), op_classes(A, wl.model))
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(pk["peak_tops"], 2),
+                "unit": "TOP/s", "frac": round(achieved / pk["peak_tops"], 4), "traffic": None,
+                "peak_model": {k: (round(v, 2) if isinstance(v, float) else v)
+                               for k, v in pk.items() if k != "peak_tops"},
                 "kernel": kern_label, "kernel_ms": round(kern_ms, 4),
                 "ops_per_launch": ops, "tile_config": N.tile_configs()[cfg],
                 "hbm": {"achieved": round(algo_bytes / (kern_ms / 1e3) / 1e9, 1),
@@ -631,7 +653,7 @@ def main() -> int:
         extra["protein"] = protein_side(ctx, stream, not args.no_cpu_baseline)
         log(f"side run: served stream, {len(served_jobs)} jobs")
         extra["served"] = {
-            "stream": f"first {len(served_jobs)} jobs of the 512-job stream (Poisson 20/s; "
+            "stream": f"{len(served_jobs)} jobs of the 512-job stream (Poisson 20/s; "
                       "taxa log-uniform 50-5000, sites 500-30000, gaps 0-8%, JC/K2P 50/50)",
             "open_loop": served_run(ctx, served_jobs, served_alns, open_loop=True),
             "closed_loop": served_run(ctx, served_jobs, served_alns, open_loop=False),

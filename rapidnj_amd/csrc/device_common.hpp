@@ -266,6 +266,37 @@ __device__ __forceinline__ void push_sentinel(const JobDesc& J, unsigned i, unsi
   if (k < J.sent_cap) J.sent[k] = ((unsigned long long)i << 32) | j;
 }
 
+// The saturated pairs of a thread's MI x MJ micro-tile (bit a * MJ + b of `mask` marks pair
+// (i0 + a, j0 + b)) appended to the job's list with one atomic per wave: a wave-wide exclusive
+// scan of the lanes' counts gives each lane its offset. Per-pair appends would hit the job's one
+// counter up to MI x MJ times per wave, which serialises when most pairs saturate. Every lane of
+// the wave must call it.
+template <int MI, int MJ>
+__device__ __forceinline__ void push_sentinels(const JobDesc& J, uint32_t mask, unsigned i0,
+                                               unsigned j0) {
+  static_assert(MI * MJ <= 32, "one bit per pair of the micro-tile");
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned cnt = (unsigned)__popc(mask);
+  unsigned incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned y = __shfl_up(incl, off);
+    if (lane >= (unsigned)off) incl += y;
+  }
+  const unsigned total = __shfl(incl, 63);
+  if (total == 0) return;  // uniform over the wave
+  unsigned base = 0;
+  if (lane == 63) base = atomicAdd(J.sent_count, total);
+  unsigned k = __shfl(base, 63) + incl - cnt;
+  while (mask) {
+    const unsigned bit = (unsigned)(__ffs(mask) - 1);
+    mask &= mask - 1;
+    if (k < J.sent_cap)
+      J.sent[k] = ((unsigned long long)(i0 + bit / MJ) << 32) | (j0 + bit % MJ);
+    ++k;
+  }
+}
+
 // One finished pair (i < j < n) of job J: counts, or the corrected distance with the reference's
 // saturation marker, written to the job's layout; lmax tracks the largest positive distance.
 // mismatch_only: `tv` holds ts + tv (the JC-only count loop) and ts is unused.

@@ -40,6 +40,7 @@ struct JobDesc {
   const uint32_t* planes;     // [3][W_pad][n_pad] (DNA) or [1 + B][W_pad][n_pad] (protein)
   unsigned long long plane_stride;  // W_pad * n_pad
   unsigned n, n_pad, W;       // W: words to process (multiple of kWordPad, <= W_pad)
+  unsigned W_live;            // words holding sites (<= W): the DNA tile kernel stops there
   unsigned nblk;              // ceil(n / TI) for the launch's tile edge (TI == TJ)
   unsigned tiles;             // nblk * (nblk + 1) / 2
   unsigned item_base;
@@ -87,7 +88,8 @@ hipError_t launch_pack_legacy(const uint32_t* bits, const uint32_t* gaps, size_t
                               unsigned words16, unsigned n, unsigned n_pad, unsigned W_pad,
                               uint32_t* planes, hipStream_t stream);
 hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
-// exact: every log in FP64 as the reference writes it; otherwise FP32 logs away from saturation.
+// exact: the reference's expression with the library's FP64 log; otherwise the leaner FP64 log
+// (device_common.hpp, within ~2 ulp of double).
 hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
                         unsigned total_items, unsigned finish_items, hipStream_t stream);
 // Protein (protein.hip). census: sets bit v of present[8] for every byte value v in the n x L
@@ -105,6 +107,9 @@ hipError_t launch_protein_tiles(int config, unsigned B, bool exact, const JobDes
 hipError_t launch_lower_pack(const float* sq, size_t ld, unsigned n, float* out,
                              hipStream_t stream);
 unsigned tile_threads(int config);  // work-group size of a configuration
+// Issue-rate probe (op 0: v_bitop3_b32, 1: v_bcnt_u32_b32): blocks x 256 threads, each running
+// 16 dependency chains for iters x 32 instructions; out holds blocks x 256 u32.
+hipError_t launch_issue_probe(int op, uint32_t* out, unsigned blocks, int iters, hipStream_t s);
 
 hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
 // Column gathers of bootstrap replicates read per-site codes: one nibble per taxon (V<<2|H<<1|L),

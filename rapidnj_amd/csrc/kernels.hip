@@ -33,6 +33,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
                                               const uint32_t (&acc_v)[MI][MJ]) {
   const unsigned n = J.n;
   float lmax = 0.0f;
+  uint32_t sat = 0;  // saturated pairs of the micro-tile, bit a * MJ + b
   if (J.model == kCounts) {
 #pragma unroll
     for (int a = 0; a < MI; ++a) {
@@ -63,7 +64,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
       const unsigned j = bj * TILE + tx * MJ + b;
       if (j >= n) continue;
       if (i < j) {
-        if (d[b] == -1.0f) push_sentinel(J, i, j);
+        if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
         else lmax = fmaxf(lmax, d[b]);
         if (J.layout == kSquare) {
           J.out[(size_t)i * J.ld + j] = d[b];
@@ -76,10 +77,55 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
       }
     }
   }
+  push_sentinels<MI, MJ>(J, sat, bi * TILE + ty * MI, bj * TILE + tx * MJ);
   fold_max(J, lmax, tid);
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------
+// Word w of the staged chunk into thread (ty, tx)'s MI x MJ accumulators (7 VALU ops per pair
+// for K2P, 6 for JC).
+template <int MI, int MJ, int CW, int Q, bool JC6>
+__device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u* __restrict__ sJ,
+                                           int w, unsigned ty, unsigned tx,
+                                           uint32_t (&acc_a)[MI][MJ], uint32_t (&acc_b)[MI][MJ],
+                                           uint32_t (&acc_v)[MI][MJ]) {
+  uint32_t hi[MI], li[MI], vi[MI], hj[MJ], lj[MJ], vj[MJ];
+#pragma unroll
+  for (int r = 0; r < MI / 4; ++r) {
+    const v4u h = sI[(0 * CW + w) * Q + ty * (MI / 4) + r];
+    const v4u l = sI[(1 * CW + w) * Q + ty * (MI / 4) + r];
+    const v4u v = sI[(2 * CW + w) * Q + ty * (MI / 4) + r];
+    hi[4 * r] = h.x; hi[4 * r + 1] = h.y; hi[4 * r + 2] = h.z; hi[4 * r + 3] = h.w;
+    li[4 * r] = l.x; li[4 * r + 1] = l.y; li[4 * r + 2] = l.z; li[4 * r + 3] = l.w;
+    vi[4 * r] = v.x; vi[4 * r + 1] = v.y; vi[4 * r + 2] = v.z; vi[4 * r + 3] = v.w;
+  }
+#pragma unroll
+  for (int r = 0; r < MJ / 4; ++r) {
+    const v4u h = sJ[(0 * CW + w) * Q + tx * (MJ / 4) + r];
+    const v4u l = sJ[(1 * CW + w) * Q + tx * (MJ / 4) + r];
+    const v4u v = sJ[(2 * CW + w) * Q + tx * (MJ / 4) + r];
+    hj[4 * r] = h.x; hj[4 * r + 1] = h.y; hj[4 * r + 2] = h.z; hj[4 * r + 3] = h.w;
+    lj[4 * r] = l.x; lj[4 * r + 1] = l.y; lj[4 * r + 2] = l.z; lj[4 * r + 3] = l.w;
+    vj[4 * r] = v.x; vj[4 * r + 1] = v.y; vj[4 * r + 2] = v.z; vj[4 * r + 3] = v.w;
+  }
+#pragma unroll
+  for (int a = 0; a < MI; ++a)
+#pragma unroll
+    for (int b = 0; b < MJ; ++b) {
+      const uint32_t g = vi[a] & vj[b];
+      const uint32_t t = xor_and(hi[a], hj[b], g);
+      acc_v[a][b] = bcnt_add(g, acc_v[a][b]);
+      if constexpr (JC6) {
+        const uint32_t m = t | xor_and(li[a], lj[b], g);
+        acc_a[a][b] = bcnt_add(m, acc_a[a][b]);
+      } else {
+        const uint32_t s = xor_and(li[a], lj[b], g ^ t);
+        acc_a[a][b] = bcnt_add(t, acc_a[a][b]);
+        acc_b[a][b] = bcnt_add(s, acc_b[a][b]);
+      }
+    }
+}
+
 // One work-group computes a TILE x TILE block of taxon pairs (bi <= bj) over one range of words
 // (the whole alignment, or split `split` of J.splits); thread (ty, tx) owns the MI x MJ
 // micro-tile i = bi*TILE + ty*MI + a, j = bj*TILE + tx*MJ + b. Sites are walked in chunks of CW
@@ -121,9 +167,11 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
 #pragma unroll
     for (int b = 0; b < MJ; ++b) acc_a[a][b] = acc_b[a][b] = acc_v[a][b] = 0;
 
+  // this range's words that hold sites: whole chunks, then at most one partial chunk (the
+  // padded words past W_live are zero and would count nothing)
   const unsigned w0 = split * J.words_per_split;
-  const unsigned w1 = min(J.W, w0 + J.words_per_split);
-  const unsigned c0 = w0 / CW, c1 = w1 / CW;             // word ranges are multiples of CW
+  const unsigned w1 = min(J.W_live, w0 + J.words_per_split);
+  const unsigned c0 = w0 / CW, c1 = (w1 + CW - 1) / CW;  // ranges start on chunk boundaries
 
   v4u pI[EP], pJ[EP];
   if (c0 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c0, row4, plane4);
@@ -138,43 +186,11 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
     __syncthreads();
     if (c + 1 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
 
+    const int live = (int)min((unsigned)CW, w1 - c * CW);  // uniform over the work-group
 #pragma unroll 4
     for (int w = 0; w < CW; ++w) {
-      uint32_t hi[MI], li[MI], vi[MI], hj[MJ], lj[MJ], vj[MJ];
-#pragma unroll
-      for (int r = 0; r < MI / 4; ++r) {
-        const v4u h = sI[(0 * CW + w) * Q + ty * (MI / 4) + r];
-        const v4u l = sI[(1 * CW + w) * Q + ty * (MI / 4) + r];
-        const v4u v = sI[(2 * CW + w) * Q + ty * (MI / 4) + r];
-        hi[4 * r] = h.x; hi[4 * r + 1] = h.y; hi[4 * r + 2] = h.z; hi[4 * r + 3] = h.w;
-        li[4 * r] = l.x; li[4 * r + 1] = l.y; li[4 * r + 2] = l.z; li[4 * r + 3] = l.w;
-        vi[4 * r] = v.x; vi[4 * r + 1] = v.y; vi[4 * r + 2] = v.z; vi[4 * r + 3] = v.w;
-      }
-#pragma unroll
-      for (int r = 0; r < MJ / 4; ++r) {
-        const v4u h = sJ[(0 * CW + w) * Q + tx * (MJ / 4) + r];
-        const v4u l = sJ[(1 * CW + w) * Q + tx * (MJ / 4) + r];
-        const v4u v = sJ[(2 * CW + w) * Q + tx * (MJ / 4) + r];
-        hj[4 * r] = h.x; hj[4 * r + 1] = h.y; hj[4 * r + 2] = h.z; hj[4 * r + 3] = h.w;
-        lj[4 * r] = l.x; lj[4 * r + 1] = l.y; lj[4 * r + 2] = l.z; lj[4 * r + 3] = l.w;
-        vj[4 * r] = v.x; vj[4 * r + 1] = v.y; vj[4 * r + 2] = v.z; vj[4 * r + 3] = v.w;
-      }
-#pragma unroll
-      for (int a = 0; a < MI; ++a)
-#pragma unroll
-        for (int b = 0; b < MJ; ++b) {
-          const uint32_t g = vi[a] & vj[b];
-          const uint32_t t = xor_and(hi[a], hj[b], g);
-          acc_v[a][b] = bcnt_add(g, acc_v[a][b]);
-          if constexpr (JC6) {
-            const uint32_t m = t | xor_and(li[a], lj[b], g);
-            acc_a[a][b] = bcnt_add(m, acc_a[a][b]);
-          } else {
-            const uint32_t s = xor_and(li[a], lj[b], g ^ t);
-            acc_a[a][b] = bcnt_add(t, acc_a[a][b]);
-            acc_b[a][b] = bcnt_add(s, acc_b[a][b]);
-          }
-        }
+      if (w >= live) break;
+      count_word<MI, MJ, CW, Q, JC6>(sI, sJ, w, ty, tx, acc_a, acc_b, acc_v);
     }
   }
 
@@ -515,7 +531,42 @@ __global__ __launch_bounds__(64) void bootstrap_gather_kernel(
   }
 }
 
+// ---- issue-rate probe ----------------------------------------------------------------------
+// 16 independent dependency chains per lane of one instruction class (OP 0: v_bitop3_b32,
+// OP 1: v_bcnt_u32_b32), 32 instructions per loop iteration; the chains' ends are stored so
+// that nothing is dead.
+template <int OP>
+__global__ __launch_bounds__(256) void issue_probe_kernel(uint32_t* __restrict__ out, int iters,
+                                                          uint32_t seed) {
+  constexpr int K = 16;
+  uint32_t a[K];
+  const uint32_t b = seed * 2654435761u + threadIdx.x, c = seed ^ 0x5bd1e995u;
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = seed + threadIdx.x * K + k;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if constexpr (OP == 0)
+          asm volatile("v_bitop3_b32 %0, %0, %1, %2 bitop3:0x28" : "+v"(a[k]) : "v"(b), "v"(c));
+        else
+          asm volatile("v_bcnt_u32_b32 %0, %1, %0" : "+v"(a[k]) : "v"(b));
+      }
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) s += a[k];
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 }  // namespace
+
+hipError_t launch_issue_probe(int op, uint32_t* out, unsigned blocks, int iters, hipStream_t s) {
+  if (op == 0) issue_probe_kernel<0><<<dim3(blocks), dim3(256), 0, s>>>(out, iters, 1u);
+  else issue_probe_kernel<1><<<dim3(blocks), dim3(256), 0, s>>>(out, iters, 1u);
+  return hipGetLastError();
+}
 
 // ---- launchers ---------------------------------------------------------------------------------
 // grid.y is limited to 65535; the packing kernels loop over the rest.

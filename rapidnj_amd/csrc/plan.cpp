@@ -142,6 +142,7 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
     j.n_pad = aln->n_pad;
     j.W_pad = p->Wr;
     j.W = p->Wr;
+    j.W_live = (p->Lmax + 31) / 32;  // every replicate holds at most Lmax sites
     j.model = model;
     j.layout = layout;
     j.ld = ld;

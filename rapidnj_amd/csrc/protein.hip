@@ -154,6 +154,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned bi, 
     return;
   }
   float lmax = 0.0f;
+  uint32_t sat = 0;  // saturated pairs of the micro-tile, bit a * MJ + b
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
@@ -171,7 +172,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned bi, 
       const unsigned j = bj * TILE + tx * MJ + b;
       if (j >= n) continue;
       if (i < j) {
-        if (d[b] == -1.0f) push_sentinel(J, i, j);
+        if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
         else lmax = fmaxf(lmax, d[b]);
         if (J.layout == kSquare) {
           J.out[(size_t)i * J.ld + j] = d[b];
@@ -184,6 +185,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned bi, 
       }
     }
   }
+  push_sentinels<MI, MJ>(J, sat, bi * TILE + ty * MI, bj * TILE + tx * MJ);
   fold_max(J, lmax, tid);
 }
 
@@ -226,9 +228,11 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel
 #pragma unroll
     for (int b = 0; b < MJ; ++b) acc_x[a][b] = acc_g[a][b] = 0;
 
+  // this range's words that hold sites (padded words are code 0 without the gap bit and would
+  // count nothing); ranges start on chunk boundaries, the last chunk may be partial
   const unsigned w0 = split * J.words_per_split;
-  const unsigned w1 = min(J.W, w0 + J.words_per_split);
-  const unsigned c0 = w0 / CW, c1 = w1 / CW;
+  const unsigned w1 = min(J.W_live, w0 + J.words_per_split);
+  const unsigned c0 = w0 / CW, c1 = (w1 + CW - 1) / CW;
 
   v4u pI[EP], pJ[EP];
   if (c0 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c0, row4, plane4);
@@ -243,8 +247,10 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel
     __syncthreads();
     if (c + 1 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
 
-#pragma unroll 2
+    const int live = (int)min((unsigned)CW, w1 - c * CW);  // uniform over the work-group
+#pragma unroll 1
     for (int w = 0; w < CW; ++w) {
+      if (w >= live) break;
       uint32_t ci[P][MI], cj[P][MJ];
 #pragma unroll
       for (int p = 0; p < P; ++p) {

@@ -219,6 +219,7 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
     d.n = s.n;
     d.n_pad = s.n_pad;
     d.W = s.W;
+    d.W_live = s.W_live ? std::min(s.W_live, s.W) : s.W;
     d.nblk = (s.n + kTileConfigs[config].tile - 1) / kTileConfigs[config].tile;
     d.tiles = P.tiles[k];
     d.item_base = (unsigned)base;
@@ -564,6 +565,7 @@ JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size
   s.n_pad = a->n_pad;
   s.W_pad = a->W;
   s.W = a->W;
+  s.W_live = (a->L + 31) / 32;
   s.model = model;
   s.layout = layout;
   s.ld = ld;
@@ -661,6 +663,39 @@ int rnj_get_tile_config_for(rnj_context* ctx, const rnj_alignment* aln, int mode
   if (!config) return fail(RNJ_E_INVALID, "null argument");
   std::vector<JobSpec> jobs{spec_for(aln, model, kTriangle, nullptr, 0)};
   *config = current_config(ctx, shape_of(jobs));
+  return RNJ_OK;
+}
+
+int rnj_valu_issue_rates(rnj_context* ctx, double* bitwise, double* bcnt) {
+  if (!ctx || !bitwise || !bcnt) return fail(RNJ_E_INVALID, "null argument");
+  const int rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  const unsigned blocks = (unsigned)std::max(1, ctx->prop.multiProcessorCount) * 8;  // 8 waves/SIMD
+  const int iters = 2000;
+  hipStream_t s = ctx->stream;
+  uint32_t* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&out, (size_t)blocks * 256 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double rate[2] = {0, 0};
+  for (int op = 0; op < 2 && e == hipSuccess; ++op) {
+    e = launch_issue_probe(op, out, blocks, iters / 4, s);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    if (e == hipSuccess) e = launch_issue_probe(op, out, blocks, iters, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && ms > 0.0f)
+      rate[op] = (double)blocks * 256 * iters * 32 / ((double)ms * 1e-3);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (out) (void)hipFree(out);
+  RNJ_HIP(e);
+  *bitwise = rate[0];
+  *bcnt = rate[1];
   return RNJ_OK;
 }
 
@@ -1191,6 +1226,7 @@ int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
       j.n_pad = aln->n_pad;
       j.W_pad = Wr;
       j.W = words_for_sites(len[r]);
+      j.W_live = (len[r] + 31) / 32;
       j.model = model;
       j.layout = layout;
       j.ld = ld;

@@ -137,6 +137,7 @@ struct JobSpec {
   unsigned L;             // protein: sites
   const uint32_t* planes;
   unsigned n, n_pad, W_pad, W;
+  unsigned W_live;  // words holding sites (0: W); the DNA tile kernel counts no word beyond
   int model;   // Model
   int layout;  // Layout
   size_t ld;
@@ -175,7 +176,7 @@ int fasta_to_alignment(Context* ctx, const uint8_t* bytes, size_t len, hipMemcpy
 
 // Enqueue a multi-job launch on `stream`: reset statistics, tile kernel, saturation patch
 // (unless raw), max copies. All jobs must share one tile configuration (chosen here). exact:
-// every logarithm in FP64 as the reference writes it (else FP32 away from saturation).
+// the reference's expression with the library's FP64 log (else the leaner FP64 log).
 // allow_tune: the first launch of an untuned shape class may time every configuration
 // (synchronously); without it such launches use the size-based default.
 int launch_jobs(Context* ctx, std::vector<JobSpec>& jobs, bool raw, bool exact,

@@ -220,6 +220,7 @@ bool rnj_server::launch(Slot& s) {
     js.n_pad = j->n_pad;
     js.W_pad = j->W;
     js.W = j->W;
+    js.W_live = (j->L + 31) / 32;
     js.model = j->model;
     js.layout = kTriangle;
     js.out = j->tri;

@@ -67,6 +67,8 @@ SIGNATURES = {
                                             ctypes.c_int, _vp, _vp]),
     "rnj_counts_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rnj_copy_to_host": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
+    "rnj_valu_issue_rates": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]),
     "rnj_distances_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "rnj_distances_rows_lower": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "rnj_distances_stream_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
@@ -372,6 +374,12 @@ class Context:
                                          max_out.data_ptr() if max_out is not None else None,
                                          _stream_handle(stream)))
         return out
+
+    def valu_issue_rates(self) -> tuple[float, float]:
+        """Measured lane-ops/s of the count loop's bitwise class and of v_bcnt on this device."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().rnj_valu_issue_rates(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def to_host(self, t, stream=None) -> np.ndarray:
         """A contiguous device tensor as a numpy array, copied through the library's pinned

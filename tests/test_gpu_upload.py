@@ -116,3 +116,9 @@ def test_copy_to_host_after_work_on_stream(ctx):
     ref = ctx.distances(A, "k2p").cpu().numpy()
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
     A.close()
+
+
+def test_valu_issue_rates(ctx):
+    """The issue-rate probe behind the roofline peak: both classes measured, bitwise faster."""
+    bitwise, bcnt = ctx.valu_issue_rates()
+    assert 1e12 < bcnt < bitwise < 1e15
