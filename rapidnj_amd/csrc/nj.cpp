@@ -265,7 +265,8 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   if (index1 == -1 || index2 == -1) return fail(RNJ_E_INVALID, "an error occurred while constructing the tree");
   // matrix[index1][index2]. A row with a copy left is an input row never rewritten, and the
   // copy's column was never written either, so the input holds the value; otherwise both are
-  // live slots.
+  // live slots, and the pair lives in the row of the larger slot (the step that created that
+  // slot wrote its row; its column is only written by a later scan).
   float last_dist = 0.0f;
   const int sl1 = h_slot_of_index[index1], sl2 = h_slot_of_index[index2];
   if (copy_pair) {
@@ -273,7 +274,8 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
                      hipMemcpyDeviceToHost));
   } else {
     if (sl1 < 0 || sl2 < 0) return fail(RNJ_E_INVALID, "last clusters without slots");
-    NJ_HIP(hipMemcpy(&last_dist, S + (size_t)sl1 * cap + sl2, sizeof(float), hipMemcpyDeviceToHost));
+    const size_t hi = (size_t)std::max(sl1, sl2), lo = (size_t)std::min(sl1, sl2);
+    NJ_HIP(hipMemcpy(&last_dist, S + hi * cap + lo, sizeof(float), hipMemcpyDeviceToHost));
   }
   t->n = n;
   t->left.assign(hl.begin(), hl.begin() + (n - 2));

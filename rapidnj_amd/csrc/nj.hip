@@ -12,12 +12,13 @@
 //     search, and a row's pairs by increasing distance, then increasing partner index (for
 //     exact ties in both distance and criterion the reference's order is std::sort's);
 //   * a row with redundant copies offers the copy join at -2 * sep first.
-// Per iteration: nj_scan_kernel (one work-group per row slot: the row's best pair) and
-// nj_step_kernel (one work-group: the global winner and rapidNJ's row caches, the merge with its
-// redundancy bookkeeping, the update of distances and separations in id order including the
-// sequential float sum of the new separation sum, the new slot's row and column). Every few
-// iterations nj_compact_* rebuilds the slot matrix from the live slots. The host enqueues all
-// iterations without waiting: the iteration count (n - 2) and the slot bounds are known ahead.
+// Per iteration: nj_scan_kernel (16 row slots per work-group, one per wave: each row's best
+// pair; it also writes the previous join's column) and nj_step_kernel (one work-group: the
+// global winner and rapidNJ's row caches, the merge with its redundancy bookkeeping, the update
+// of distances and separations in id order, the new slot's row). The new cluster's separation
+// sum (a sequential float sum in id order) is finished by the next scan. Every few iterations
+// nj_compact_* rebuilds the slot matrix from the live slots. The host enqueues all iterations
+// without waiting: the iteration count (n - 2) and the slot bounds are known ahead.
 #include "nj.hpp"
 
 #include <float.h>
@@ -29,6 +30,7 @@ constexpr int kScanThreads = 1024;  // 16 waves, one row each
 constexpr int kScanRows = kScanThreads / 64;
 constexpr int kSepChunk = 4096;     // separations staged in LDS per step of a row scan
 constexpr int kStepThreads = 1024;
+constexpr int kKeyCache = 20000;    // row keys the step keeps in LDS between its record passes
 
 // Floats as order-preserving unsigned keys (after -0 -> +0, so that equal values compare equal
 // as the reference's float comparisons do).
@@ -150,7 +152,10 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
 // Launch A: work-group 0 finishes the previous join's new cluster when asked; every other
 // work-group scans 16 consecutive row slots, one per wave, over partner slots b < row. The
 // separations of the partners are staged in LDS once per work-group and shared by its rows.
-__global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const float* __restrict__ S,
+// With `finish`, each wave also writes its row's entry in the new slot's column (S[a][ns] =
+// newcol[a], a < ns): the step wrote the new row, and no scan reads that column (row a's
+// partners are the slots below a), so the column is complete before the next step reads it.
+__global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, float* __restrict__ S,
                                                                int finish) {
   const NjState& st = *D.st;
   if (st.error) return;
@@ -168,6 +173,7 @@ __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const
   const int a0 = blk * kScanRows;
   if (a0 >= used) return;  // uniform over the work-group
   const int a = a0 + wave;
+  if (finish && lane == 0 && a < used - 1) S[(size_t)a * D.ld + (used - 1)] = D.newcol[a];
   const bool mine = a < used && !(finish && a == used - 1) && D.alive[a];
   const int reach = min(a0 + kScanRows, used) - 1;  // longest row prefix in this group
   const float s_a = mine ? D.sep[a] : 0.0f;
@@ -180,14 +186,24 @@ __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const
       *reinterpret_cast<float4*>(ssep + k) = *reinterpret_cast<const float4*>(D.sep + c0 + k);
     __syncthreads();
     if (mine) {
+      // four 4-wide loads in flight per lane; a lane's candidates are the same as one at a time
+      // (ties are settled by partner index, not by visiting order)
       const int end = min(a, c0 + cnt);
-      for (int b0 = c0 + 4 * lane; b0 < end; b0 += 256) {
-        const float4 dv = *reinterpret_cast<const float4*>(row + b0);
-        const float4 sv = *reinterpret_cast<const float4*>(ssep + (b0 - c0));
-        consider(D, dv.x, sv.x, s_a, b0, best, best_b);
-        if (b0 + 1 < end) consider(D, dv.y, sv.y, s_a, b0 + 1, best, best_b);
-        if (b0 + 2 < end) consider(D, dv.z, sv.z, s_a, b0 + 2, best, best_b);
-        if (b0 + 3 < end) consider(D, dv.w, sv.w, s_a, b0 + 3, best, best_b);
+      for (int b0 = c0 + 4 * lane; b0 < end; b0 += 1024) {
+        float4 dv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (b0 + 256 * u < end) dv[u] = *reinterpret_cast<const float4*>(row + b0 + 256 * u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int bu = b0 + 256 * u;
+          if (bu >= end) break;
+          const float4 sv = *reinterpret_cast<const float4*>(ssep + (bu - c0));
+          consider(D, dv[u].x, sv.x, s_a, bu, best, best_b);
+          if (bu + 1 < end) consider(D, dv[u].y, sv.y, s_a, bu + 1, best, best_b);
+          if (bu + 2 < end) consider(D, dv[u].z, sv.z, s_a, bu + 2, best, best_b);
+          if (bu + 3 < end) consider(D, dv[u].w, sv.w, s_a, bu + 3, best, best_b);
+        }
       }
     }
     __syncthreads();
@@ -202,8 +218,12 @@ __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, const
   }
 }
 
-__global__ __launch_bounds__(kScanThreads) void nj_finish_kernel(NjDevice D, const float* __restrict__ S) {
+// Launch A's finishing work alone (before a compaction, which copies whole rows and columns):
+// the new cluster's separation and row, and its column.
+__global__ __launch_bounds__(kScanThreads) void nj_finish_kernel(NjDevice D, float* __restrict__ S) {
   if (D.st->error) return;
+  const int ns = D.st->used - 1;
+  for (int b = threadIdx.x; b < ns; b += kScanThreads) S[(size_t)b * D.ld + ns] = D.newcol[b];
   finish_new_cluster(D, S);
 }
 
@@ -238,12 +258,22 @@ __device__ RowKey block_min(RowKey k, RowKey* sh) {
 }
 
 // The first row (in visiting order) reaching the minimum among rows with a key (live, answer
-// below FLT_MAX) whose cyclic position is below `below`.
+// below FLT_MAX) whose cyclic position is below `below`. The first pass (fill) copies the keys of
+// the first kKeyCache slots into `cache`; later passes read them from there (block_min's
+// barriers order the copy before them).
 __device__ RowKey best_row_before(const NjDevice& D, int used, int start, int n, int below,
-                                  RowKey* sh) {
+                                  RowKey* sh, unsigned long long* cache, bool fill) {
   RowKey k{0xffffffffu, 0x7fffffff, -1};
   for (int b = threadIdx.x; b < used; b += kStepThreads) {
-    const unsigned long long key = D.row_key[b];
+    unsigned long long key;
+    if (b >= kKeyCache) {
+      key = D.row_key[b];
+    } else if (fill) {
+      key = D.row_key[b];
+      cache[b] = key;
+    } else {
+      key = cache[b];
+    }
     if (key == kNoKey) continue;
     int cyc = (int)(uint32_t)key - start;
     if (cyc < 0) cyc += n;
@@ -267,19 +297,26 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
   NjState& st = *D.st;
   const int tid = threadIdx.x;
   __shared__ RowKey sh[kStepThreads / 64];
+  __shared__ unsigned long long key_cache[kKeyCache];
   __shared__ int s_rows[2], s_skip[2], s_fail;
   __shared__ float s_mutual;
   if (st.error) return;
   const int n = st.n, used = st.used;
   const int start = D.index_to_id[st.cache1] != -1 ? st.cache1 : st.cache2;
   // records: rows where the running minimum improved, last first
+  // (unrolled with constant indices, so that rec stays in registers; nrec is uniform)
   RowKey rec[4];
   int nrec = 0;
   int below = n + 1;
-  for (; nrec < 4; ++nrec) {
-    rec[nrec] = best_row_before(D, used, start, n, below, sh);
-    if (rec[nrec].slot < 0) break;
-    below = rec[nrec].cyc;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (nrec == r) {
+      rec[r] = best_row_before(D, used, start, n, below, sh, key_cache, r == 0);
+      if (rec[r].slot >= 0) {
+        below = rec[r].cyc;
+        nrec = r + 1;
+      }
+    }
   }
   if (tid == 0) {
     // the winner must name live rows with slots, and the new cluster needs a free slot
@@ -307,7 +344,9 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
       // caches: replay the improvements in visiting order (records are listed last first; with
       // 4 or more records only the last three matter, and they are distinct rows)
       int m = st.min1, c1 = st.cache1, c2 = st.cache2;
-      for (int t = nrec - 1; t >= 0; --t) {
+#pragma unroll
+      for (int t = 3; t >= 0; --t) {
+        if (t >= nrec) continue;
         const int row = (int)(uint32_t)D.row_key[rec[t].slot];
         if (row != m) {
           c2 = c1;
@@ -395,8 +434,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
     }
     D.newcol[b] = dist;
     D.term[b] = term;
-    new_row_p[b] = dist;
-    S[(size_t)b * D.ld + ns] = dist;
+    new_row_p[b] = dist;  // the column S[b][ns] is written by the next launch A
   }
   if (tid == 0) {
     new_row_p[ns] = 0.0f;
@@ -589,7 +627,7 @@ hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool
   return hipGetLastError();
 }
 
-hipError_t nj_launch_finish(const NjDevice& D, const float* S, hipStream_t s) {
+hipError_t nj_launch_finish(const NjDevice& D, float* S, hipStream_t s) {
   nj_finish_kernel<<<dim3(1), dim3(kScanThreads), 0, s>>>(D, S);
   return hipGetLastError();
 }

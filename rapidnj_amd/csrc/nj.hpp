@@ -66,7 +66,7 @@ hipError_t nj_launch_gather(const float* M, size_t ldm, const int* vis, int v, f
 hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool finish_previous,
                                hipStream_t s);
 // Finish the last join's new cluster alone (before a compaction).
-hipError_t nj_launch_finish(const NjDevice& D, const float* S, hipStream_t s);
+hipError_t nj_launch_finish(const NjDevice& D, float* S, hipStream_t s);
 // Rebuild the slot matrix into S2 from the live slots (at most alive_bound of them).
 hipError_t nj_launch_compact(const NjDevice& D, const float* S, float* S2, int alive_bound,
                              hipStream_t s);

@@ -250,13 +250,16 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
 
 void free_workspace_public(Workspace* ws) { free_workspace(ws); }
 
-// Configuration before (or without) tuning, from measurements on MI355X: below ~4096 taxa the
-// 32-taxon tile fills the device better (2000 x 10000: 0.21 ms vs 0.24 ms), above it the
-// 64-taxon tile with 8-word chunks is fastest (20000 x 5000: 7.2 ms).
+// Configuration before (or without) tuning, from measurements on MI355X: the 32-taxon tile with
+// 8-word chunks was fastest or within 1 % of it on every shape timed (K2P 2000 x 10000: 0.181 ms,
+// against 0.191 ms for t32_m4x4_c16 and 0.218 ms for t64_m4x4_c8; K2P 20000 x 5000: 5.59 ms,
+// against 5.74 ms for t64_m4x4_c16; JC 20000 x 5000: 4.49 ms, against 4.48 ms for t64_m4x4_c8;
+// Kimura protein 20000 x 1250: 2.35 ms, against 2.43 ms for t64_m4x4_c16).
 static int default_config(unsigned n, unsigned W, bool jc_only) {
+  (void)n;
   (void)W;
   (void)jc_only;
-  return n <= 4096 ? 3 : 1;
+  return 4;  // t32_m4x4_c8
 }
 
 // Autotune classes: DNA launches by whether every job is JC (the 6-instruction loop), protein
