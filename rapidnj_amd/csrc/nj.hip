@@ -413,28 +413,55 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
   const float denom = (float)(st.cluster_count - 2);
   const int ns = used;
   float* new_row_p = S + (size_t)ns * D.ld;
-  for (int b = tid; b < used; b += kStepThreads) {
-    float dist = 0.0f, term = 0.0f;
-    if (D.alive[b]) {
-      const int i = D.slot_index[b];
-      if (i != s_skip[0] && i != s_skip[1]) {
-        const float val1 = row1[b], val2 = row2[b];
-        const float t = __fsub_rn(__fadd_rn(val1, val2), mutual);
-        dist = (float)((double)t / 2.0);
-        if (st.negative_branches && dist < 0) dist = 0;
-        term = __fmul_rn(dist, (float)D.red_count[i]);
-        const float ss = __fadd_rn(D.sepsum[b], __fsub_rn(__fsub_rn(dist, val1), val2));
-        D.sepsum[b] = ss;
-        D.sep[b] = ss / denom;
-      } else {
-        D.alive[b] = 0;
-        D.sep[b] = __int_as_float(0x7fc00000);
-        D.row_key[b] = kNoKey;
+  // Four slots per thread and pass, their loads issued together (every index is in bounds for
+  // dead slots too: slots below `used`, and slot_index always holds a matrix index).
+  const int skip0 = s_skip[0], skip1 = s_skip[1];
+  const bool neg = st.negative_branches != 0;
+  for (int b0 = tid; b0 < used; b0 += 4 * kStepThreads) {
+    int al[4], si[4], rcnt[4];
+    float v1[4], v2[4], ssum[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * kStepThreads;
+      al[u] = 0;
+      si[u] = 0;
+      v1[u] = v2[u] = ssum[u] = 0.0f;
+      if (b < used) {
+        al[u] = D.alive[b];
+        si[u] = D.slot_index[b];
+        v1[u] = row1[b];
+        v2[u] = row2[b];
+        ssum[u] = D.sepsum[b];
       }
     }
-    D.newcol[b] = dist;
-    D.term[b] = term;
-    new_row_p[b] = dist;  // the column S[b][ns] is written by the next launch A
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rcnt[u] = b0 + u * kStepThreads < used ? D.red_count[si[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * kStepThreads;
+      if (b >= used) break;
+      float dist = 0.0f, term = 0.0f;
+      if (al[u]) {
+        const int i = si[u];
+        if (i != skip0 && i != skip1) {
+          const float val1 = v1[u], val2 = v2[u];
+          const float t = __fsub_rn(__fadd_rn(val1, val2), mutual);
+          dist = (float)((double)t / 2.0);
+          if (neg && dist < 0) dist = 0;
+          term = __fmul_rn(dist, (float)rcnt[u]);
+          const float ss = __fadd_rn(ssum[u], __fsub_rn(__fsub_rn(dist, val1), val2));
+          D.sepsum[b] = ss;
+          D.sep[b] = ss / denom;
+        } else {
+          D.alive[b] = 0;
+          D.sep[b] = __int_as_float(0x7fc00000);
+          D.row_key[b] = kNoKey;
+        }
+      }
+      D.newcol[b] = dist;
+      D.term[b] = term;
+      new_row_p[b] = dist;  // the column S[b][ns] is written by the next launch A
+    }
   }
   if (tid == 0) {
     new_row_p[ns] = 0.0f;

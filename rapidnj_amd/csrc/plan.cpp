@@ -5,7 +5,8 @@
 // and then captures it into a graph on a stream of its own:
 //   H2D copy of the column weights (from the plan's pinned buffer)
 //   -> bootstrap_index (expanded column lists) -> bootstrap_gather (replicate planes)
-//   -> job table copy -> statistics reset -> tile kernel (-> finish kernel) -> saturation patch.
+//   -> statistics reset -> tile kernel (-> finish kernel) -> saturation patch
+// (the job table, which never changes, is uploaded once by the eager run).
 // Every replicate uses the word count of the plan's maximum replicate length; the gather writes
 // zero words past each replicate's own length, so shorter replicates count exactly their sites.
 // The body is captured twice, once per pinned weight buffer. rnj_bootstrap_plan_run copies new
@@ -64,8 +65,8 @@ void destroy_plan(rnj_plan* p) {
 }
 
 // The launch body with pinned weight buffer `buf`, enqueued on p->stream (eagerly or under
-// capture).
-int enqueue_plan(rnj_plan* p, int buf) {
+// capture). The job table never changes: the eager run uploads it, captures leave the copy out.
+int enqueue_plan(rnj_plan* p, int buf, bool upload_table) {
   const rnj_alignment* a = p->aln;
   RNJ_HIP(hipMemcpyAsync(p->d_weights, p->h_weights[buf], (size_t)p->R * p->L * sizeof(uint16_t),
                          hipMemcpyHostToDevice, p->stream));
@@ -73,7 +74,7 @@ int enqueue_plan(rnj_plan* p, int buf) {
   RNJ_HIP(launch_bootstrap_gather(p->d_codes, a->n_pad, p->d_idx, p->Lmax, p->d_len, p->R, p->Wr,
                                   p->d_planes, p->stream));
   return enqueue_launch(p->ctx, p->specs, p->shape, p->config, &p->ws, p->raw, p->exact,
-                        p->stream);
+                        p->stream, upload_table);
 }
 
 int check_weights(const rnj_plan* p, const uint16_t* weights) {
@@ -155,7 +156,7 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
   for (size_t k = 0; k < (size_t)R * L; ++k) p->h_weights[0][k] = 1;
   if (rc == RNJ_OK)
     rc = select_config(ctx, p->specs, p->shape, &p->ws, p->exact, p->stream, true, &p->config);
-  if (rc == RNJ_OK) rc = enqueue_plan(p, 0);
+  if (rc == RNJ_OK) rc = enqueue_plan(p, 0, true);
   if (rc == RNJ_OK && hipStreamSynchronize(p->stream) != hipSuccess)
     rc = fail(RNJ_E_HIP, "eager plan run failed");
   for (int k = 0; k < 2 && rc == RNJ_OK; ++k) {
@@ -164,7 +165,7 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
       rc = hip_fail(e, "hipStreamBeginCapture");
       break;
     }
-    const int body = enqueue_plan(p, k);
+    const int body = enqueue_plan(p, k, false);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(p->stream, &g);
     p->graph[k] = g;

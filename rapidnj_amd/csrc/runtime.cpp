@@ -404,14 +404,16 @@ int prepare_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
 }
 
 int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
-                   Workspace* ws, bool raw, bool exact, hipStream_t stream) {
+                   Workspace* ws, bool raw, bool exact, hipStream_t stream, bool upload_table) {
   if (sh.mixed) return fail(RNJ_E_INVALID, "one launch cannot mix DNA and protein jobs");
   const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
   if (P.items >= (1ull << 31)) return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
   RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
   number_jobs(specs, config, P, ws, sh.caps.data());
-  hipError_t e = hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
-                                hipMemcpyHostToDevice, stream);
+  hipError_t e = hipSuccess;
+  if (upload_table)
+    e = hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
+                       hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream);
   if (e == hipSuccess)
     e = launch_counting(config, sh, exact, ws->d_jobs, (unsigned)specs.size(),

@@ -199,8 +199,11 @@ LaunchShape shape_of(const std::vector<JobSpec>& specs);
 int prepare_workspace(Workspace* ws, unsigned njobs, size_t nsent);
 int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Workspace* ws,
                   bool exact, hipStream_t stream, bool allow_tune, int* config);
+// upload_table: copy the job table to the device first (a caller replaying a captured launch
+// whose table never changes uploads it once and captures without the copy).
 int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
-                   Workspace* ws, bool raw, bool exact, hipStream_t stream);
+                   Workspace* ws, bool raw, bool exact, hipStream_t stream,
+                   bool upload_table = true);
 void free_workspace_public(Workspace* ws);
 
 }  // namespace rnj
