@@ -240,14 +240,31 @@ __device__ __forceinline__ bool rk_less(const RowKey& x, const RowKey& y) {
   return x.cyc < y.cyc;
 }
 
-// Block-wide minimum: shuffles within waves, then one pass over the 16 wave results.
-__device__ RowKey block_min(RowKey k, RowKey* sh) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// Fold the key of slot b into k when it has one and its cyclic position is below `below`.
+__device__ __forceinline__ void fold_key(unsigned long long key, int b, int start, int n, int below,
+                                         RowKey& k) {
+  if (key == kNoKey) return;
+  int cyc = (int)(uint32_t)key - start;
+  if (cyc < 0) cyc += n;
+  if (cyc >= below) return;
+  const RowKey c{(uint32_t)(key >> 32), cyc, b};
+  if (rk_less(c, k)) k = c;
+}
+
+// Wave-wide minimum (every lane gets it).
+__device__ __forceinline__ RowKey wave_min_key(RowKey k) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     RowKey o{(uint32_t)__shfl_xor((int)k.v, off), __shfl_xor(k.cyc, off), __shfl_xor(k.slot, off)};
     if (rk_less(o, k)) k = o;
   }
+  return k;
+}
+
+// Block-wide minimum: shuffles within waves, then one pass over the 16 wave results.
+__device__ RowKey block_min(RowKey k, RowKey* sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  k = wave_min_key(k);
   if (lane == 0) sh[wave] = k;
   __syncthreads();
   RowKey r = sh[0];
@@ -264,22 +281,21 @@ __device__ RowKey block_min(RowKey k, RowKey* sh) {
 __device__ RowKey best_row_before(const NjDevice& D, int used, int start, int n, int below,
                                   RowKey* sh, unsigned long long* cache, bool fill) {
   RowKey k{0xffffffffu, 0x7fffffff, -1};
-  for (int b = threadIdx.x; b < used; b += kStepThreads) {
-    unsigned long long key;
-    if (b >= kKeyCache) {
-      key = D.row_key[b];
-    } else if (fill) {
-      key = D.row_key[b];
-      cache[b] = key;
-    } else {
-      key = cache[b];
+  // four keys per thread and pass, loaded together
+  for (int b0 = threadIdx.x; b0 < used; b0 += 4 * kStepThreads) {
+    unsigned long long key[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * kStepThreads;
+      key[u] = kNoKey;
+      if (b < used) key[u] = (fill || b >= kKeyCache) ? D.row_key[b] : cache[b];
     }
-    if (key == kNoKey) continue;
-    int cyc = (int)(uint32_t)key - start;
-    if (cyc < 0) cyc += n;
-    if (cyc >= below) continue;
-    const RowKey c{(uint32_t)(key >> 32), cyc, b};
-    if (rk_less(c, k)) k = c;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * kStepThreads;
+      if (fill && b < used && b < kKeyCache) cache[b] = key[u];
+      fold_key(key[u], b, start, n, below, k);
+    }
   }
   return block_min(k, sh);
 }
