@@ -78,6 +78,7 @@ typedef struct rnj_context rnj_context;
 typedef struct rnj_alignment rnj_alignment;
 typedef struct rnj_server rnj_server;
 typedef struct rnj_plan rnj_plan;
+typedef struct rnj_batch_plan rnj_batch_plan;
 
 /* ---- reference device boundary (group 1) ------------------------------------------------ */
 
@@ -159,6 +160,13 @@ RNJ_EXPORT int rnj_alignment_from_chars(rnj_context* ctx, const char* chars, siz
                              rnj_alignment** out);
 RNJ_EXPORT int rnj_alignment_from_rows(rnj_context* ctx, const char* const* rows, unsigned n, unsigned L,
                             rnj_alignment** out);
+/* New characters for a DNA alignment of the same n x L (host rows at `pitch`, or device memory),
+ * packed into its planes in place on `stream`: later work on that stream, a captured batch plan's
+ * replay included, sees them. Host characters have been read when the call returns. Updates of
+ * one alignment are not to overlap. No counterpart in the reference (a new alignment there is a
+ * new dataloader). */
+RNJ_EXPORT int rnj_alignment_update_chars(rnj_context* ctx, rnj_alignment* aln, const char* chars,
+                                          size_t pitch, int chars_on_device, void* stream);
 RNJ_EXPORT int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned int* const* bitStrings,
                                   const unsigned int* const* gapFilters, unsigned n,
                                   unsigned bitStringsCount, rnj_alignment** out);
@@ -248,6 +256,21 @@ RNJ_EXPORT int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* 
 RNJ_EXPORT int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
                          const uint16_t* weights, unsigned R, int layout, float* out,
                          size_t matrix_stride, size_t ld, int flags, void* stream);
+
+/* Captured multi-alignment launch: the matrices of `count` alignments of any taxa and site counts
+ * (models[k] RNJ_JC or RNJ_K2P; all DNA, or all protein of one code width) in ONE tile launch,
+ * captured once into a HIP graph. outs[k] (device) receives alignment k's matrix in `layout`
+ * (square rows of lds[k] floats). rnj_batch_plan_run replays it: the alignments' current planes
+ * (see rnj_alignment_update_chars) into the same outputs, after the previous replay, allocating
+ * nothing. Alignments and outputs must outlive the plan. No counterpart in the reference (its
+ * distance calculators handle one alignment per call, KimuraDistance.cpp:75-99). */
+RNJ_EXPORT int rnj_batch_plan_create(rnj_context* ctx, const rnj_alignment* const* alns,
+                                     const int* models, unsigned count, int layout,
+                                     float* const* outs, const size_t* lds, int flags,
+                                     rnj_batch_plan** plan);
+RNJ_EXPORT int rnj_batch_plan_run(rnj_batch_plan* plan, void* stream);
+RNJ_EXPORT int rnj_batch_plan_tile_config(const rnj_batch_plan* plan);
+RNJ_EXPORT int rnj_batch_plan_destroy(rnj_batch_plan* plan);
 
 /* Captured batched bootstrap: the launch of rnj_bootstrap_device for fixed (alignment, model, R,
  * layout, out) captured once into a HIP graph and replayed per call with new weights. All

@@ -559,7 +559,9 @@ int check_aln(rnj_context* ctx, const rnj_alignment* aln) {
   return RNJ_OK;
 }
 
-JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size_t ld) {
+}  // namespace
+
+JobSpec rnj::alignment_spec(const Alignment* a, int model, int layout, float* out, size_t ld) {
   JobSpec s{};
   s.kind = a->kind;
   s.code_bits = a->code_bits;
@@ -578,6 +580,10 @@ JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size
   return s;
 }
 
+namespace {
+JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size_t ld) {
+  return alignment_spec(a, model, layout, out, ld);
+}
 }  // namespace
 
 extern "C" {
@@ -641,6 +647,7 @@ int rnj_destroy(rnj_context* ctx) {
   (void)hipDeviceSynchronize();
   for (Alignment* a : ctx->alignments) {
     if (a->planes) (void)hipFree(a->planes);
+    if (a->chars_dev) (void)hipFree(a->chars_dev);
     delete a;
   }
   ctx->alignments.clear();
@@ -787,6 +794,31 @@ int rnj_alignment_from_rows(rnj_context* ctx, const char* const* rows, unsigned 
   return rnj_alignment_from_chars(ctx, buf.data(), L, n, L, 0, 0, out);
 }
 
+int rnj_alignment_update_chars(rnj_context* ctx, rnj_alignment* aln, const char* chars,
+                               size_t pitch, int chars_on_device, void* stream) {
+  int rc = check_aln(ctx, aln);
+  if (rc != RNJ_OK) return rc;
+  if (!chars) return fail(RNJ_E_INVALID, "null characters");
+  if (aln->kind != kDNA)
+    return fail(RNJ_E_NOT_DNA, "only DNA alignments are updated in place (a protein alignment's "
+                               "residue numbering comes from its own characters)");
+  if (pitch < aln->L) return fail(RNJ_E_INVALID, "pitch smaller than sequence length");
+  rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  hipStream_t s = stream_of(ctx, stream);
+  const uint8_t* src = reinterpret_cast<const uint8_t*>(chars);
+  size_t src_pitch = pitch;
+  if (!chars_on_device) {  // through the pinned staging into the alignment's character buffer
+    if (!aln->chars_dev)
+      RNJ_HIP(hipMalloc((void**)&aln->chars_dev, (size_t)aln->n * aln->L));
+    RNJ_HIP(ctx->staging.upload(aln->chars_dev, chars, pitch, aln->L, aln->n, s));
+    src = aln->chars_dev;
+    src_pitch = aln->L;
+  }
+  RNJ_HIP(launch_pack_chars(src, src_pitch, aln->n, aln->L, aln->n_pad, aln->W, aln->planes, s));
+  return RNJ_OK;
+}
+
 int rnj_alignment_from_bitstrings(rnj_context* ctx, const unsigned int* const* bitStrings,
                                   const unsigned int* const* gapFilters, unsigned n,
                                   unsigned bitStringsCount, rnj_alignment** out) {
@@ -832,9 +864,10 @@ int rnj_alignment_free(rnj_alignment* aln) {
     std::lock_guard<std::mutex> lock(aln->ctx->mu);
     aln->ctx->alignments.erase(aln);
   }
-  if (aln->planes) {
+  if (aln->planes || aln->chars_dev) {
     (void)hipDeviceSynchronize();
-    (void)hipFree(aln->planes);
+    if (aln->planes) (void)hipFree(aln->planes);
+    if (aln->chars_dev) (void)hipFree(aln->chars_dev);
   }
   delete aln;
   return RNJ_OK;

@@ -123,6 +123,7 @@ struct Alignment {
   uint32_t* planes = nullptr;  // [nplanes()][W][n_pad], then (protein) gaps
   uint32_t* gaps = nullptr;    // protein: gap sites per taxon [n_pad], inside the planes block
   std::vector<std::string> names;  // FASTA record names in file order (alignments from FASTA)
+  uint8_t* chars_dev = nullptr;      // n x L characters of in-place updates (allocated by the first)
   unsigned nplanes() const { return kind == kProtein ? 1 + code_bits : 3; }
   size_t bytes() const {
     return ((size_t)nplanes() * W + (kind == kProtein ? 1 : 0)) * n_pad * sizeof(uint32_t);
@@ -145,6 +146,9 @@ struct JobSpec {
   uint32_t *out_ts, *out_tv, *out_valid;
   float* max_out;  // device pointer or null
 };
+
+// The launch description of one alignment's matrix.
+JobSpec alignment_spec(const Alignment* a, int model, int layout, float* out, size_t ld);
 
 unsigned round_up(unsigned x, unsigned m);
 unsigned words_for_sites(unsigned sites);  // ceil(sites/32) rounded up to kWordPad

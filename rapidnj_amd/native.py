@@ -80,6 +80,12 @@ SIGNATURES = {
     "rnj_bootstrap_plan_run": (ctypes.c_int, [_vp, _vp, _vp]),
     "rnj_plan_tile_config": (ctypes.c_int, [_vp]),
     "rnj_plan_destroy": (ctypes.c_int, [_vp]),
+    "rnj_batch_plan_create": (ctypes.c_int, [_vp, _vp, _vp, _u, ctypes.c_int, _vp, _vp,
+                                             ctypes.c_int, ctypes.POINTER(_vp)]),
+    "rnj_batch_plan_run": (ctypes.c_int, [_vp, _vp]),
+    "rnj_batch_plan_tile_config": (ctypes.c_int, [_vp]),
+    "rnj_batch_plan_destroy": (ctypes.c_int, [_vp]),
+    "rnj_alignment_update_chars": (ctypes.c_int, [_vp, _vp, _vp, _sz, ctypes.c_int, _vp]),
     "rnj_nj_device": (ctypes.c_int, [_vp, _vp, _u, _sz, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
     "rnj_nj_rows": (ctypes.c_int, [_vp, _vp, _u, ctypes.c_int, ctypes.POINTER(_vp)]),
     "rnj_nj_alignment": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -229,6 +235,20 @@ class Alignment:
         k = lib().rnj_alignment_name_count(self.handle)
         return [lib().rnj_alignment_name(self.handle, i) for i in range(k)]
 
+    def update(self, chars, stream=None) -> None:
+        """New characters of the same (n, L) shape, packed in place on `stream` (a uint8 numpy
+        array on the host, or a uint8 torch tensor on the device). DNA only."""
+        if isinstance(chars, np.ndarray):
+            a, pitch = _host_rows(chars)
+            assert a.shape == (self.n, self.L), "same shape as the alignment"
+            check(lib().rnj_alignment_update_chars(self.ctx.handle, self.handle, a.ctypes.data,
+                                                   pitch, 0, _stream_handle(stream)))
+        else:
+            assert chars.dtype.itemsize == 1 and chars.is_cuda and chars.stride(1) == 1
+            assert tuple(chars.shape) == (self.n, self.L), "same shape as the alignment"
+            check(lib().rnj_alignment_update_chars(self.ctx.handle, self.handle, chars.data_ptr(),
+                                                   chars.stride(0), 1, _stream_handle(stream)))
+
     def planes(self) -> np.ndarray:
         out = np.zeros((self.info.nplanes, self.info.W, self.info.n), dtype=np.uint32)
         check(lib().rnj_alignment_planes(self.handle, out.ctypes.data))
@@ -263,7 +283,7 @@ class Context:
         """Destroy the context; servers, plans and alignments still open are closed first."""
         if self.handle:
             live = list(self._children)
-            for kind in (Server, BootstrapPlan, Alignment):
+            for kind in (Server, BootstrapPlan, BatchPlan, Alignment):
                 for c in live:
                     if isinstance(c, kind):
                         c.close()
@@ -448,6 +468,11 @@ class Context:
         """A captured (HIP graph) batched-bootstrap launch writing R matrices into `out`."""
         return BootstrapPlan(self, aln, R, model, layout, out, max_length, raw, exact)
 
+    def batch_plan(self, alns: list, models, layout: str = "triangle", outs=None,
+                   raw: bool = False, exact: bool = False) -> "BatchPlan":
+        """Several alignments (any n, L) in one captured tile launch; outs[k] receives matrix k."""
+        return BatchPlan(self, alns, models, layout, outs, raw, exact)
+
     def nj(self, matrix, negative_branches: bool = False, stream=None) -> Tree:
         """rapidNJ on a square float32 device matrix (torch tensor (n, ld>=n) on this device)."""
         import torch
@@ -558,6 +583,53 @@ class BootstrapPlan:
     def close(self) -> None:
         if self.handle:
             lib().rnj_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class BatchPlan:
+    """rnj_batch_plan_*: the matrices of several alignments as one captured launch."""
+
+    def __init__(self, ctx: Context, alns: list, models, layout: str, outs, raw: bool,
+                 exact: bool):
+        import torch
+        self.ctx, self.alns = ctx, list(alns)
+        k = len(self.alns)
+        models = [models] * k if isinstance(models, (str, int)) else list(models)
+        lay = {"triangle": TRIANGLE, "square": SQUARE}[layout]
+        if outs is None:
+            outs = [torch.empty((a.n * (a.n - 1) // 2,) if lay == TRIANGLE else (a.n, a.n),
+                                dtype=torch.float32, device=f"cuda:{ctx.device}")
+                    for a in self.alns]
+        self.outs = list(outs)
+        ha = (_vp * k)(*[a.handle for a in self.alns])
+        hm = (ctypes.c_int * k)(*[_model(m) for m in models])
+        ho = (_vp * k)(*[o.data_ptr() for o in self.outs])
+        hl = (_sz * k)(*[o.stride(0) if lay == SQUARE else 0 for o in self.outs])
+        h = _vp()
+        check(lib().rnj_batch_plan_create(ctx.handle, ctypes.cast(ha, _vp), ctypes.cast(hm, _vp), k,
+                                          lay, ctypes.cast(ho, _vp), ctypes.cast(hl, _vp),
+                                          (FLAG_RAW if raw else 0) | (FLAG_EXACT if exact else 0),
+                                          ctypes.byref(h)))
+        self.handle = h.value
+        ctx._adopt(self)
+
+    @property
+    def tile_config(self) -> int:
+        return lib().rnj_batch_plan_tile_config(self.handle)
+
+    def run(self, stream=None) -> list:
+        check(lib().rnj_batch_plan_run(self.handle, _stream_handle(stream)))
+        return self.outs
+
+    def close(self) -> None:
+        if self.handle:
+            lib().rnj_batch_plan_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

@@ -62,3 +62,73 @@ def test_weights_beyond_the_plan_are_rejected(ctx, N):
             plan.run(w)
         assert e.value.status == N.RNJ_E_INVALID
         plan.run(np.ones((2, 300), dtype=np.uint16))
+
+
+# ---- captured multi-alignment launches ---------------------------------------------------------
+BATCH = [(300, 4000, 0.05, "k2p"), (57, 1001, 0.0, "jc"), (1200, 250, 0.2, "k2p"),
+         (64, 30000, 0.08, "jc"), (2, 33, 0.0, "k2p")]
+
+
+def _batch_alignments(ctx, seed0):
+    alns = [simulate_alignment(n, L, g, seed0 + k) for k, (n, L, g, _) in enumerate(BATCH)]
+    return alns, [ctx.alignment(a) for a in alns], [m for *_, m in BATCH]
+
+
+@pytest.mark.parametrize("layout", ["triangle", "square"])
+def test_batch_plan_uneven_alignments_every_replay(ctx, layout):
+    """Five alignments of different taxa and site counts, JC and K2P, in one captured launch:
+    every replay (with eager work between replays) equals each alignment computed alone."""
+    import torch
+    alns, A, models = _batch_alignments(ctx, 40)
+    refs = [ctx.distances(a, m, layout=layout).cpu().numpy() for a, m in zip(A, models)]
+    with ctx.batch_plan(A, models, layout=layout) as plan:
+        for rep in range(3):
+            for o in plan.outs:
+                o.fill_(7.0)
+            plan.run()
+            torch.cuda.synchronize()
+            for o, r in zip(plan.outs, refs):
+                np.testing.assert_array_equal(o.cpu().numpy().view(np.uint32), r.view(np.uint32))
+            ctx.distances(A[0], "jc")  # eager work between replays
+            torch.cuda.synchronize()
+
+
+def test_batch_plan_sees_updated_characters(ctx):
+    """rnj_alignment_update_chars repacks new characters in place; the next replay computes them
+    (host characters, and device characters on a side stream ordered before the replay)."""
+    import torch
+    alns, A, models = _batch_alignments(ctx, 60)
+    with ctx.batch_plan(A, models) as plan:
+        plan.run()
+        torch.cuda.synchronize()
+        n, L, g, m = BATCH[0]
+        new0 = simulate_alignment(n, L, 0.1, 999)
+        n3, L3, g3, m3 = BATCH[3]
+        new3 = simulate_alignment(n3, L3, 0.02, 998)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):  # the updates and the replay in order on one stream
+            A[0].update(new0, stream=s)
+            d3 = torch.from_numpy(new3).to("cuda:0")
+            A[3].update(d3, stream=s)
+            plan.run(stream=s)
+        s.synchronize()
+        ref0 = ctx.distances(ctx.alignment(new0), m).cpu().numpy()
+        ref3 = ctx.distances(ctx.alignment(new3), m3).cpu().numpy()
+        np.testing.assert_array_equal(plan.outs[0].cpu().numpy().view(np.uint32), ref0.view(np.uint32))
+        np.testing.assert_array_equal(plan.outs[3].cpu().numpy().view(np.uint32), ref3.view(np.uint32))
+        # the untouched alignments still give their own matrices
+        ref1 = ctx.distances(A[1], models[1]).cpu().numpy()
+        np.testing.assert_array_equal(plan.outs[1].cpu().numpy().view(np.uint32), ref1.view(np.uint32))
+        np.testing.assert_array_equal(A[0].planes(), ctx.alignment(new0).planes())
+
+
+def test_batch_plan_rejections(ctx, N):
+    from rapidnj_amd.workloads import simulate_protein_alignment
+    dna = ctx.alignment(simulate_alignment(20, 100, 0.0, 1))
+    prot = ctx.alignment(simulate_protein_alignment(20, 100, 0.0, 2))
+    with pytest.raises(N.RnjError):
+        ctx.batch_plan([dna, prot], "k2p")
+    with pytest.raises(N.RnjError):
+        prot.update(simulate_protein_alignment(20, 100, 0.0, 3))
+    with pytest.raises(N.RnjError):
+        ctx.batch_plan([dna], 7)  # not a model
