@@ -438,6 +438,47 @@ def one_job_per_launch(ctx, jobs, alns) -> dict:
             "unit": "G site-pairs/s"}
 
 
+def uneven_batch_compare(ctx, jobs, alns, count: int = 64) -> dict:
+    """The first `count` alignments of the served stream (uneven taxa and site counts, JC and K2P),
+    resident in HBM: one captured multi-alignment launch (rnj_batch_plan) against one distance
+    launch per alignment, outputs in HBM either way."""
+    import torch
+    jobs, alns = jobs[:count], alns[:count]
+    A = [ctx.alignment(a) for a in alns]
+    models = [j.model for j in jobs]
+    outs = [torch.empty(max(1, j.n * (j.n - 1) // 2), dtype=torch.float32, device="cuda:0")
+            for j in jobs]
+    sp = sum(j.n * (j.n - 1) // 2 * j.L for j in jobs)
+    k = 5
+    for a, m, o in zip(A, models, outs):
+        ctx.distances(a, m, out=o)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        for a, m, o in zip(A, models, outs):
+            ctx.distances(a, m, out=o)
+    torch.cuda.synchronize()
+    single = (time.perf_counter() - t0) / k
+    ref = [o.clone() for o in outs]
+    with ctx.batch_plan(A, models, outs=outs) as plan:
+        plan.run()
+        torch.cuda.synchronize()
+        same = all(bool(torch.equal(o, r)) for o, r in zip(outs, ref))
+        t0 = time.perf_counter()
+        for _ in range(k):
+            plan.run()
+        torch.cuda.synchronize()
+        graphed = (time.perf_counter() - t0) / k
+    for a in A:
+        a.close()
+    return {"alignments": len(jobs), "site_pairs": sp, "graph_ms": round(graphed * 1e3, 3),
+            "graph_value": round(sp / graphed / 1e9, 2),
+            "one_launch_per_alignment_ms": round(single * 1e3, 3),
+            "one_launch_per_alignment_value": round(sp / single / 1e9, 2),
+            "speedup": round(single / graphed, 2), "identical_results": same,
+            "unit": "G site-pairs/s"}
+
+
 def bootstrap_compare(ctx, dev) -> dict:
     """1000 x 5000 K2P, 100 column-weight replicates: one launch vs one launch per replicate."""
     import torch
@@ -658,6 +699,8 @@ def main() -> int:
             "open_loop": served_run(ctx, served_jobs, served_alns, open_loop=True),
             "closed_loop": served_run(ctx, served_jobs, served_alns, open_loop=False),
             "one_job_per_launch": one_job_per_launch(ctx, served_jobs, served_alns)}
+        log("side run: 64 uneven alignments in one captured launch vs one launch each")
+        extra["uneven_batch"] = uneven_batch_compare(ctx, served_jobs, served_alns)
 
     if not args.no_traffic and not wl.replicates:
         log("HBM traffic: two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE)")
