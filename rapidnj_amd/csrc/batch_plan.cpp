@@ -84,7 +84,7 @@ int rnj_batch_plan_create(rnj_context* ctx, const rnj_alignment* const* alns, co
   }
   // the alignments may have been packed on other streams of the context
   (void)hipStreamSynchronize(ctx->stream);
-  int rc = prepare_workspace(&p->ws, count, p->shape.nsent);
+  int rc = prepare_workspace(&p->ws, count);
   if (rc == RNJ_OK)
     rc = select_config(ctx, p->specs, p->shape, &p->ws, p->exact, p->stream, true, &p->config);
   if (rc == RNJ_OK)  // eager run: uploads the job table the graph keeps using

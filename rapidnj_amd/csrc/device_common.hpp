@@ -183,15 +183,6 @@ __device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid) {
   return (valid == 0 || mismatches == 0) ? -0.0f : (sat ? -1.0f : d);
 }
 
-template <bool kExact>
-__device__ __noinline__ float k2p_distance(uint32_t ts, uint32_t tv, uint32_t valid) {
-  return k2p_value<kExact, false>(ts, tv, valid);
-}
-
-template <bool kExact>
-__device__ __noinline__ float jc_distance(uint32_t mismatches, uint32_t valid) {
-  return jc_value<kExact, false>(mismatches, valid);
-}
 #pragma clang fp contract(on)
 
 // Row r of the block triangle holds tiles (r, r..nb-1); it starts at r*nb - r*(r-1)/2.
@@ -251,93 +242,37 @@ __device__ __forceinline__ unsigned find_job(const JobDesc* jobs, unsigned njobs
   return lo;
 }
 
-// Append (i, j) to the job's saturated-pair list. The lanes of a wave that reach this call
-// together take consecutive slots with one atomic on the list's counter (saturated pairs come
-// in clusters of distant clades, and one counter hit by every such pair serialises).
-__device__ __forceinline__ void push_sentinel(const JobDesc& J, unsigned i, unsigned j) {
-  const unsigned long long active = __ballot(1);
-  const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(active >> 32),
-                                                   __builtin_amdgcn_mbcnt_lo((unsigned)active, 0u));
-  const int leader = __ffsll((long long)active) - 1;
-  unsigned base = 0;
-  if (below == 0) base = atomicAdd(J.sent_count, (unsigned)__popcll(active));
-  base = __shfl(base, leader);
-  const unsigned k = base + below;
-  if (k < J.sent_cap) J.sent[k] = ((unsigned long long)i << 32) | j;
-}
-
-// The saturated pairs of a thread's MI x MJ micro-tile (bit a * MJ + b of `mask` marks pair
-// (i0 + a, j0 + b)) appended to the job's list with one atomic per wave: a wave-wide exclusive
-// scan of the lanes' counts gives each lane its offset. Per-pair appends would hit the job's one
-// counter up to MI x MJ times per wave, which serialises when most pairs saturate. Every lane of
-// the wave must call it.
-template <int MI, int MJ>
-__device__ __forceinline__ void push_sentinels(const JobDesc& J, uint32_t mask, unsigned i0,
-                                               unsigned j0) {
-  static_assert(MI * MJ <= 32, "one bit per pair of the micro-tile");
-  const unsigned lane = threadIdx.x & 63;
-  const unsigned cnt = (unsigned)__popc(mask);
-  unsigned incl = cnt;
+// Saturation record of one wave of an epilogue, one 32-bit word per wave unit (unit = tile *
+// waves-per-work-group + wave of the work-group): bit 31 set when a lane of the wave holds a
+// saturated pair, bits 0-30 the wave's largest distance (distances are >= 0, so the bits of the
+// float compare as integers; a -0.0 folds into +0.0). When bit 31 is set, every lane also stores
+// its micro-tile mask (bit a * MJ + b: pair (i0 + a, j0 + b) saturated) at 64 * unit + lane.
+// Only plain stores: job_max_kernel folds the words into the job's maximum and patch_kernel
+// rewrites the marked pairs. (One atomic per wave on the job's counters, which this replaces,
+// serialised at about 11 ns per atomic across the XCDs: 4.4 ms per launch at 20000 taxa.)
+// Every lane of the wave must call it.
+__device__ __forceinline__ void wave_record(const JobDesc& J, unsigned unit, uint32_t mask,
+                                            float lmax) {
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned y = __shfl_up(incl, off);
-    if (lane >= (unsigned)off) incl += y;
-  }
-  const unsigned total = __shfl(incl, 63);
-  if (total == 0) return;  // uniform over the wave
-  unsigned base = 0;
-  if (lane == 63) base = atomicAdd(J.sent_count, total);
-  unsigned k = __shfl(base, 63) + incl - cnt;
-  while (mask) {
-    const unsigned bit = (unsigned)(__ffs(mask) - 1);
-    mask &= mask - 1;
-    if (k < J.sent_cap)
-      J.sent[k] = ((unsigned long long)(i0 + bit / MJ) << 32) | (j0 + bit % MJ);
-    ++k;
-  }
+  for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
+  const bool any = __ballot(mask != 0u) != 0ull;  // uniform over the wave
+  const unsigned lane = threadIdx.x & 63;
+  if (any) J.sat_mask[(size_t)unit * 64 + lane] = mask;
+  if (lane == 0) J.sat_word[unit] = (any ? kSatFlag : 0u) | (__float_as_uint(lmax) & ~kSatFlag);
 }
 
-// One finished pair (i < j < n) of job J: counts, or the corrected distance with the reference's
-// saturation marker, written to the job's layout; lmax tracks the largest positive distance.
-// mismatch_only: `tv` holds ts + tv (the JC-only count loop) and ts is unused.
-template <bool kExact>
-__device__ __forceinline__ void finish_pair(const JobDesc& J, unsigned i, unsigned j, uint32_t ts,
-                                            uint32_t tv, uint32_t valid, bool mismatch_only,
-                                            float& lmax) {
-  const unsigned n = J.n;
-  if (J.model == kCounts) {
-    const unsigned long long p = tri_index(i, j, n);
-    J.out_tv[p] = tv;
-    J.out_ts[p] = ts;
-    J.out_valid[p] = valid;
-    return;
-  }
-  float d;
-  if (J.model == kJC)
-    d = jc_distance<kExact>(mismatch_only ? tv : tv + ts, valid);
-  else
-    d = k2p_distance<kExact>(ts, tv, valid);
-  if (d == -1.0f) push_sentinel(J, i, j);
-  else if (d > lmax) lmax = d;
-  if (J.layout == kSquare) {
-    J.out[(size_t)i * J.ld + j] = d;
-    J.out[(size_t)j * J.ld + i] = d;
-  } else {
-    J.out[tri_index(i, j, n)] = d;
-  }
+// The counts of one finished pair (i < j < n) of a counts job, to its three upper triangles.
+__device__ __forceinline__ void finish_counts(const JobDesc& J, unsigned i, unsigned j, uint32_t ts,
+                                              uint32_t tv, uint32_t valid) {
+  const unsigned long long p = tri_index(i, j, J.n);
+  J.out_tv[p] = tv;
+  J.out_ts[p] = ts;
+  J.out_valid[p] = valid;
 }
 
 // Zero diagonal of a square output (the reference's post-process writes 0 on i == i).
 __device__ __forceinline__ void finish_diagonal(const JobDesc& J, unsigned i) {
   if (J.model != kCounts && J.layout == kSquare) J.out[(size_t)i * J.ld + i] = 0.0f;
-}
-
-// Fold a thread's maximum into the job's running maximum: wave reduction, one atomic per wave.
-// Every lane of the wave must call it.
-__device__ __forceinline__ void fold_max(const JobDesc& J, float lmax, unsigned tid) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
-  if ((tid & 63) == 0 && lmax > 0.0f) atomicMax(J.maxbits, __float_as_uint(lmax));
 }
 
 }  // namespace

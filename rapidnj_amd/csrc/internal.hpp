@@ -26,6 +26,7 @@ constexpr unsigned kTaxaPad = 128;    // n_pad multiple (largest tile edge)
 constexpr unsigned kWordPad = 16;     // W_pad multiple (largest LDS site chunk, in words)
 constexpr unsigned kBlockThreads = 256;
 constexpr uint32_t kSentinelBits = 0xBF800000u;  // -1.0f, the reference's saturation marker
+constexpr uint32_t kSatFlag = 0x80000000u;       // a wave's saturation word: it holds saturated pairs
 
 enum Model : int { kJC = 0, kK2P = 1, kCounts = 2 };
 enum Kind : int { kDNA = 0, kProtein = 1 };
@@ -55,10 +56,9 @@ struct JobDesc {
   uint32_t* out_ts;           // counts (kCounts), upper triangle
   uint32_t* out_tv;
   uint32_t* out_valid;
-  unsigned* maxbits;          // bits of the running maximum distance (non-negative floats)
-  unsigned long long* sent;   // (i << 32 | j) of saturated pairs
-  unsigned sent_cap;
-  unsigned* sent_count;
+  unsigned* maxbits;          // bits of the largest distance (non-negative floats), job_max_kernel
+  uint32_t* sat_word;         // [tiles * waves per work-group]: kSatFlag | wave maximum bits
+  uint32_t* sat_mask;         // [tiles * waves per work-group][64]: micro-tile masks, flagged waves
   const uint32_t* gaps;       // protein: gap sites per taxon [n_pad]
   unsigned L;                 // protein: alignment length (sites)
 };
@@ -111,7 +111,14 @@ unsigned tile_threads(int config);  // work-group size of a configuration
 // 16 dependency chains for iters x 32 instructions; out holds blocks x 256 u32.
 hipError_t launch_issue_probe(int op, uint32_t* out, unsigned blocks, int iters, hipStream_t s);
 
-hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
+// After the tile (and finish) kernels of a launch: job_max_kernel folds each job's wave words
+// into its maximum, then, unless raw, patch_kernel writes 2 * maximum over the saturated pairs
+// the flagged waves marked. max_units: the largest job's wave-unit count; config: the launch's
+// tile configuration (the geometry that maps a wave's mask bits back to pairs).
+hipError_t launch_saturation(const JobDesc* jobs, unsigned njobs, unsigned long long max_units,
+                             int config, bool patch, hipStream_t stream);
+// Wave units of a job of `tiles` pair tiles in configuration `config` (one saturation word each).
+unsigned long long sat_units(unsigned long long tiles, int config);
 // Column gathers of bootstrap replicates read per-site codes: one nibble per taxon (V<<2|H<<1|L),
 // two taxa per byte, site-major: codes_bytes(n_pad, L) bytes.
 inline size_t codes_bytes(unsigned n_pad, unsigned L) { return (size_t)L * n_pad / 2; }

@@ -26,11 +26,13 @@ namespace {
 // the JC-only loop), b = ts, v = valid. Writes distances (or counts), records saturated pairs and
 // folds the wave's maximum into the job's running maximum.
 template <int TILE, int MI, int MJ, bool JC6, bool kExact>
-__device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, unsigned bj,
-                                              unsigned ty, unsigned tx, unsigned tid,
+__device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, unsigned bi,
+                                              unsigned bj, unsigned ty, unsigned tx, unsigned tid,
                                               const uint32_t (&acc_a)[MI][MJ],
                                               const uint32_t (&acc_b)[MI][MJ],
                                               const uint32_t (&acc_v)[MI][MJ]) {
+  constexpr int NT = (TILE / MI) * (TILE / MJ);
+  static_assert(NT % 64 == 0, "whole waves per work-group");
   const unsigned n = J.n;
   float lmax = 0.0f;
   uint32_t sat = 0;  // saturated pairs of the micro-tile, bit a * MJ + b
@@ -41,8 +43,8 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
 #pragma unroll
       for (int b = 0; b < MJ; ++b) {
         const unsigned j = bj * TILE + tx * MJ + b;
-        if (j < n && i < j)
-          finish_pair<kExact>(J, i, j, acc_b[a][b], acc_a[a][b], acc_v[a][b], JC6, lmax);
+        // JC6 launches hold no counts jobs (those need ts and tv apart)
+        if (j < n && i < j) finish_counts(J, i, j, acc_b[a][b], acc_a[a][b], acc_v[a][b]);
       }
     }
     return;
@@ -77,8 +79,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned bi, uns
       }
     }
   }
-  push_sentinels<MI, MJ>(J, sat, bi * TILE + ty * MI, bj * TILE + tx * MJ);
-  fold_max(J, lmax, tid);
+  wave_record(J, tile * (NT / 64) + tid / 64, sat, lmax);
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------
@@ -207,7 +208,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
       }
     return;
   }
-  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
+  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, tile, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
 }
 
 // Sums the partial counts of the splits of one tile of a split job and finishes the tile. Work
@@ -247,7 +248,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
         acc_v[a][b] += slot[2 * plane + a * MJ + b];
       }
   }
-  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
+  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, tile, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
 }
 
 template <int TILE, int MI, int MJ, int CW>
@@ -373,44 +374,71 @@ __global__ void reset_stats_kernel(const JobDesc* __restrict__ jobs, unsigned nj
   const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= njobs) return;
   *jobs[k].maxbits = 0u;
-  *jobs[k].sent_count = 0u;
+}
+
+// The largest distance of each job (grid y = job): the job's wave words folded per work-group
+// (grid-stride over its units), then one atomicMax per work-group into the job's maxbits, which
+// reset_stats_kernel zeroed. waves: waves per tile-kernel work-group.
+__global__ __launch_bounds__(256) void job_max_kernel(const JobDesc* __restrict__ jobs,
+                                                      unsigned waves) {
+  const JobDesc& J = jobs[blockIdx.y];
+  if (J.model == kCounts) return;  // uniform over the work-group
+  const unsigned long long units = (unsigned long long)J.tiles * waves;
+  uint32_t m = 0;
+  for (unsigned long long u = (unsigned long long)blockIdx.x * 256 + threadIdx.x; u < units;
+       u += (unsigned long long)gridDim.x * 256)
+    m = max(m, J.sat_word[u] & ~kSatFlag);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off));
+  __shared__ uint32_t wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+    if (m != 0) atomicMax(J.maxbits, m);
+  }
 }
 
 // The reference post-process (KimuraDistance.cpp:366-395, JCdistance.cpp:201-230) replaces every
-// -1 by 2 * maxDistance. Saturated pairs were listed by the tile kernel; if the list overflowed
-// its capacity, the job's whole output is scanned instead.
-__global__ void patch_kernel(const JobDesc* __restrict__ jobs) {
+// -1 by 2 * maxDistance. Each wave reads 64 wave words at a time (one per lane) and takes the
+// flagged units one by one: lane l reads the mask that lane l of the epilogue's wave stored and
+// maps its bits back to pairs with the launch's tile geometry (tile edge, micro-tile mi x mj).
+__global__ __launch_bounds__(256) void patch_kernel(const JobDesc* __restrict__ jobs,
+                                                    unsigned edge, unsigned mi, unsigned mj) {
   const JobDesc& J = jobs[blockIdx.y];
   if (J.model == kCounts) return;
-  const unsigned cnt = *J.sent_count;
-  if (cnt == 0) return;
+  const unsigned ntx = edge / mj;
+  const unsigned waves = (edge / mi) * ntx / 64;
+  const unsigned long long units = (unsigned long long)J.tiles * waves;
   const float val = __uint_as_float(*J.maxbits) * 2.0f;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const size_t start = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cnt <= J.sent_cap) {
-    for (size_t k = start; k < cnt; k += stride) {
-      const unsigned long long p = J.sent[k];
-      const unsigned i = (unsigned)(p >> 32), j = (unsigned)(p & 0xffffffffu);
-      if (J.layout == kSquare) {
-        J.out[(size_t)i * J.ld + j] = val;
-        J.out[(size_t)j * J.ld + i] = val;
-      } else {
-        J.out[tri_index(i, j, J.n)] = val;
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned long long nw = (unsigned long long)gridDim.x * 4;
+  for (unsigned long long base = ((unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+       base < units; base += nw * 64) {
+    const unsigned long long u = base + lane;
+    const bool flagged = u < units && (J.sat_word[u] & kSatFlag) != 0u;
+    unsigned long long pend = __ballot(flagged);
+    while (pend) {  // uniform over the wave
+      const unsigned long long unit = base + (unsigned)(__ffsll((long long)pend) - 1);
+      pend &= pend - 1;
+      uint32_t mask = J.sat_mask[unit * 64 + lane];
+      const unsigned tile = (unsigned)(unit / waves);
+      const unsigned tid = (unsigned)(unit % waves) * 64 + lane;
+      unsigned bi, bj;
+      patch_coords(tile, J.nblk, bi, bj);
+      const unsigned i0 = bi * edge + (tid / ntx) * mi, j0 = bj * edge + (tid % ntx) * mj;
+      while (mask) {
+        const unsigned bit = (unsigned)(__ffs(mask) - 1);
+        mask &= mask - 1;
+        const unsigned i = i0 + bit / mj, j = j0 + bit % mj;
+        if (J.layout == kSquare) {
+          J.out[(size_t)i * J.ld + j] = val;
+          J.out[(size_t)j * J.ld + i] = val;
+        } else {
+          J.out[tri_index(i, j, J.n)] = val;
+        }
       }
     }
-    return;
-  }
-  uint32_t* o = reinterpret_cast<uint32_t*>(J.out);
-  if (J.layout == kSquare) {
-    const size_t total = (size_t)J.n * J.n;
-    for (size_t k = start; k < total; k += stride) {
-      const size_t p = (k / J.n) * J.ld + (k % J.n);
-      if (o[p] == kSentinelBits) J.out[p] = val;
-    }
-  } else {
-    const size_t total = (size_t)J.n * (J.n - 1) / 2;
-    for (size_t k = start; k < total; k += stride)
-      if (o[k] == kSentinelBits) J.out[k] = val;
   }
 }
 
@@ -631,9 +659,20 @@ hipError_t launch_lower_pack(const float* sq, size_t ld, unsigned n, float* out,
   return hipGetLastError();
 }
 
-hipError_t launch_patch(const JobDesc* jobs, unsigned njobs, hipStream_t stream) {
-  if (njobs == 0) return hipSuccess;
-  patch_kernel<<<dim3(64, njobs), dim3(256), 0, stream>>>(jobs);
+unsigned long long sat_units(unsigned long long tiles, int config) {
+  return tiles * (tile_threads(config) / 64);
+}
+
+hipError_t launch_saturation(const JobDesc* jobs, unsigned njobs, unsigned long long max_units,
+                             int config, bool patch, hipStream_t stream) {
+  if (njobs == 0 || max_units == 0) return hipSuccess;
+  const TileConfig& c = kTileConfigs[config];
+  // about 16 words per thread for the maximum; 256 units per work-group pass for the patch
+  const unsigned mx = (unsigned)std::min<unsigned long long>(1024, (max_units + 4095) / 4096);
+  job_max_kernel<<<dim3(mx, njobs), dim3(256), 0, stream>>>(jobs, tile_threads(config) / 64);
+  if (!patch) return hipGetLastError();
+  const unsigned px = (unsigned)std::min<unsigned long long>(512, (max_units + 255) / 256);
+  patch_kernel<<<dim3(px, njobs), dim3(256), 0, stream>>>(jobs, c.tile, c.micro_i, c.micro_j);
   return hipGetLastError();
 }
 

@@ -151,7 +151,7 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
     p->specs.push_back(j);
   }
   p->shape = shape_of(p->specs);
-  int rc = prepare_workspace(&p->ws, R, p->shape.nsent);
+  int rc = prepare_workspace(&p->ws, R);
   // the original alignment as every replicate: a valid eager run that settles the configuration
   for (size_t k = 0; k < (size_t)R * L; ++k) p->h_weights[0][k] = 1;
   if (rc == RNJ_OK)

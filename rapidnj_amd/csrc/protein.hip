@@ -118,8 +118,9 @@ __device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long l
 // as gaps, then subtracts from L rounded up to 128 (bitDistanceProtein.cpp:31-61): its valid
 // count is the residue pairs, less 128 when L is a multiple of 128 (modulo 2^64).
 template <int TILE, int MI, int MJ, bool kExact>
-__device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned bi, unsigned bj,
-                                                 unsigned ty, unsigned tx, unsigned tid,
+__device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile, unsigned bi,
+                                                 unsigned bj, unsigned ty, unsigned tx,
+                                                 unsigned tid,
                                                  const uint32_t (&acc_x)[MI][MJ],
                                                  const uint32_t (&acc_g)[MI][MJ]) {
   const unsigned n = J.n;
@@ -185,8 +186,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned bi, 
       }
     }
   }
-  push_sentinels<MI, MJ>(J, sat, bi * TILE + ty * MI, bj * TILE + tx * MJ);
-  fold_max(J, lmax, tid);
+  wave_record(J, tile * ((TILE / MI) * (TILE / MJ) / 64) + tid / 64, sat, lmax);
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------
@@ -291,7 +291,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel
       }
     return;
   }
-  protein_epilogue<TILE, MI, MJ, kExact>(J, bi, bj, ty, tx, tid, acc_x, acc_g);
+  protein_epilogue<TILE, MI, MJ, kExact>(J, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
 }
 
 // Sums the partial counts of the splits of one tile and finishes it (kernels.hip numbering).
@@ -329,7 +329,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_finish_kern
         acc_g[a][b] += slot[plane + a * MJ + b];
       }
   }
-  protein_epilogue<TILE, MI, MJ, kExact>(J, bi, bj, ty, tx, tid, acc_x, acc_g);
+  protein_epilogue<TILE, MI, MJ, kExact>(J, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
 }
 
 template <int TILE, int MI, int MJ, int CW, int B>

@@ -42,9 +42,13 @@ unsigned words_for_sites(unsigned sites) {
   return round_up(w == 0 ? 1 : w, kWordPad);
 }
 
-size_t sentinel_capacity(unsigned n) {
-  const size_t pairs = (size_t)n * (n - 1) / 2;
-  return std::min<size_t>(std::max<size_t>(pairs / 64, 1024), (size_t)1 << 24);
+size_t saturation_bytes(unsigned n) {
+  size_t most = 0;
+  for (int c = 0; c < kNumTileConfigs; ++c) {
+    const unsigned long long nb = (n + kTileConfigs[c].tile - 1) / kTileConfigs[c].tile;
+    most = std::max<size_t>(most, sat_units(nb * (nb + 1) / 2, c) * 65 * sizeof(uint32_t));
+  }
+  return most;
 }
 
 int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignment** out,
@@ -82,16 +86,29 @@ void free_workspace(Workspace* ws) {
   if (ws->h_jobs) (void)hipHostFree(ws->h_jobs);
   if (ws->d_jobs) (void)hipFree(ws->d_jobs);
   if (ws->d_stats) (void)hipFree(ws->d_stats);
-  if (ws->d_sent) (void)hipFree(ws->d_sent);
+  if (ws->d_sat) (void)hipFree(ws->d_sat);
   if (ws->d_partial) (void)hipFree(ws->d_partial);
   ws->h_jobs = nullptr;
   ws->d_jobs = nullptr;
   ws->d_stats = nullptr;
-  ws->d_sent = nullptr;
+  ws->d_sat = nullptr;
   ws->d_partial = nullptr;
   ws->cap_jobs = 0;
-  ws->cap_sent = 0;
+  ws->cap_units = 0;
   ws->cap_partial = 0;
+}
+
+// Saturation records for `units` wave units: no stores need clearing, since every epilogue wave
+// writes its word and a flagged wave all 64 masks.
+int grow_sat(Workspace* ws, size_t units) {
+  if (units <= ws->cap_units) return RNJ_OK;
+  const size_t cap = std::max(units, ws->cap_units + ws->cap_units / 2);
+  if (ws->d_sat) (void)hipFree(ws->d_sat);
+  ws->d_sat = nullptr;
+  ws->cap_units = 0;
+  RNJ_HIP(hipMalloc((void**)&ws->d_sat, cap * 65 * sizeof(uint32_t)));
+  ws->cap_units = cap;
+  return RNJ_OK;
 }
 
 int grow_partial(Workspace* ws, size_t elems) {
@@ -105,7 +122,7 @@ int grow_partial(Workspace* ws, size_t elems) {
   return RNJ_OK;
 }
 
-int grow_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
+int grow_workspace(Workspace* ws, unsigned njobs) {
   if (njobs > ws->cap_jobs) {
     const unsigned cap = std::max(njobs, 2 * ws->cap_jobs);
     if (ws->h_jobs) (void)hipHostFree(ws->h_jobs);
@@ -117,21 +134,13 @@ int grow_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
     ws->cap_jobs = 0;
     RNJ_HIP(hipHostMalloc((void**)&ws->h_jobs, cap * sizeof(JobDesc), hipHostMallocDefault));
     RNJ_HIP(hipMalloc((void**)&ws->d_jobs, cap * sizeof(JobDesc)));
-    RNJ_HIP(hipMalloc((void**)&ws->d_stats, cap * 2 * sizeof(unsigned)));
+    RNJ_HIP(hipMalloc((void**)&ws->d_stats, cap * sizeof(unsigned)));
     ws->cap_jobs = cap;
-  }
-  if (nsent > ws->cap_sent) {
-    const size_t cap = std::max(nsent, 2 * ws->cap_sent);
-    if (ws->d_sent) (void)hipFree(ws->d_sent);
-    ws->d_sent = nullptr;
-    ws->cap_sent = 0;
-    RNJ_HIP(hipMalloc((void**)&ws->d_sent, cap * sizeof(unsigned long long)));
-    ws->cap_sent = cap;
   }
   return RNJ_OK;
 }
 
-int acquire_workspace(Context* ctx, unsigned njobs, size_t nsent, Workspace** out) {
+int acquire_workspace(Context* ctx, unsigned njobs, Workspace** out) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   Workspace* ws = nullptr;
   for (auto& w : ctx->pool) {
@@ -145,7 +154,7 @@ int acquire_workspace(Context* ctx, unsigned njobs, size_t nsent, Workspace** ou
     ws = ctx->pool.back().get();
     RNJ_HIP(hipEventCreateWithFlags(&ws->done, hipEventDisableTiming));
   }
-  int rc = grow_workspace(ws, njobs, nsent);
+  int rc = grow_workspace(ws, njobs);
   if (rc != RNJ_OK) return rc;
   ws->in_use = true;
   *out = ws;
@@ -169,9 +178,10 @@ int ilog2(unsigned long long x) {
 // chunks), so that the device fills; the split tiles are finished by the finish kernel.
 struct LaunchPlan {
   std::vector<unsigned> splits, wps, tiles;
-  std::vector<size_t> partial_off;
+  std::vector<size_t> partial_off, sat_off;
   size_t partial_elems = 0;
   unsigned long long items = 0, finish_items = 0;
+  unsigned long long sat_units = 0, max_units = 0;  // wave units: all jobs, the largest job
 };
 
 LaunchPlan plan_launch(const std::vector<JobSpec>& specs, int config, unsigned cus) {
@@ -201,15 +211,19 @@ LaunchPlan plan_launch(const std::vector<JobSpec>& specs, int config, unsigned c
       P.finish_items += P.tiles[k];
     }
     P.items += (unsigned long long)P.tiles[k] * S;
+    P.sat_off.push_back(P.sat_units);
+    if (specs[k].model != kCounts) {
+      const unsigned long long u = sat_units(P.tiles[k], config);
+      P.sat_units += u;
+      P.max_units = std::max(P.max_units, u);
+    }
   }
   return P;
 }
 
 // Fill the job table for `plan` and bind scratch.
-void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, Workspace* ws,
-                 size_t* sent_caps) {
+void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, Workspace* ws) {
   unsigned long long base = 0, fin = 0;
-  size_t soff = 0;
   for (size_t k = 0; k < specs.size(); ++k) {
     const JobSpec& s = specs[k];
     JobDesc& d = ws->h_jobs[k];
@@ -236,11 +250,9 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
     d.out_valid = s.out_valid;
     d.gaps = s.gaps;
     d.L = s.L;
-    d.maxbits = ws->d_stats + 2 * k;
-    d.sent_count = ws->d_stats + 2 * k + 1;
-    d.sent = ws->d_sent + soff;
-    d.sent_cap = (unsigned)sent_caps[k];
-    soff += sent_caps[k];
+    d.maxbits = ws->d_stats + k;
+    d.sat_word = ws->d_sat + P.sat_off[k];
+    d.sat_mask = ws->d_sat + ws->cap_units + 64 * P.sat_off[k];
     base += (unsigned long long)d.tiles * d.splits;
     if (d.splits > 1) fin += d.tiles;
   }
@@ -302,7 +314,6 @@ int choose_config(Context* ctx, unsigned n, unsigned W, bool jc_only) {
 // sweep does not decide the choice.
 static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws,
                     const LaunchShape& sh, bool exact, hipStream_t stream, int* best) {
-  size_t* sent_caps = const_cast<size_t*>(sh.caps.data());
   const bool jc_only = sh.jc_only;
   const unsigned n_key = sh.n_key, W_key = sh.W_key;
   hipEvent_t e0, e1;
@@ -315,7 +326,8 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws,
     const LaunchPlan P = plan_launch(specs, c, cus);
     if (P.items >= (1ull << 31)) return RNJ_OK;
     RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
-    number_jobs(specs, c, P, ws, sent_caps);
+    RNJ_HIP_STATUS(grow_sat(ws, P.sat_units));
+    number_jobs(specs, c, P, ws);
     RNJ_HIP(hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
                            hipMemcpyHostToDevice, stream));
     RNJ_HIP(launch_reset_stats(ws->d_jobs, (unsigned)specs.size(), stream));
@@ -359,7 +371,6 @@ static int autotune(Context* ctx, std::vector<JobSpec>& specs, Workspace* ws,
 
 LaunchShape shape_of(const std::vector<JobSpec>& specs) {
   LaunchShape sh;
-  sh.caps.resize(specs.size());
   if (!specs.empty()) {
     sh.kind = specs[0].kind;
     sh.code_bits = specs[0].code_bits;
@@ -371,8 +382,6 @@ LaunchShape shape_of(const std::vector<JobSpec>& specs) {
     sh.n_key = std::max(sh.n_key, s.n);
     sh.W_key = std::max(sh.W_key, s.W);
     sh.work += (unsigned long long)s.n * s.n * s.W;
-    sh.caps[k] = s.model == kCounts ? 0 : sentinel_capacity(s.n);
-    sh.nsent += sh.caps[k];
   }
   return sh;
 }
@@ -399,9 +408,7 @@ int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Wo
   return RNJ_OK;
 }
 
-int prepare_workspace(Workspace* ws, unsigned njobs, size_t nsent) {
-  return grow_workspace(ws, njobs, std::max<size_t>(nsent, 1));
-}
+int prepare_workspace(Workspace* ws, unsigned njobs) { return grow_workspace(ws, njobs); }
 
 int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
                    Workspace* ws, bool raw, bool exact, hipStream_t stream, bool upload_table) {
@@ -409,7 +416,8 @@ int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, i
   const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
   if (P.items >= (1ull << 31)) return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
   RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
-  number_jobs(specs, config, P, ws, sh.caps.data());
+  RNJ_HIP_STATUS(grow_sat(ws, P.sat_units));
+  number_jobs(specs, config, P, ws);
   hipError_t e = hipSuccess;
   if (upload_table)
     e = hipMemcpyAsync(ws->d_jobs, ws->h_jobs, specs.size() * sizeof(JobDesc),
@@ -418,10 +426,11 @@ int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, i
   if (e == hipSuccess)
     e = launch_counting(config, sh, exact, ws->d_jobs, (unsigned)specs.size(),
                         (unsigned)P.items, (unsigned)P.finish_items, stream);
-  if (e == hipSuccess && !raw) e = launch_patch(ws->d_jobs, (unsigned)specs.size(), stream);
+  if (e == hipSuccess)
+    e = launch_saturation(ws->d_jobs, (unsigned)specs.size(), P.max_units, config, !raw, stream);
   for (size_t k = 0; e == hipSuccess && k < specs.size(); ++k)
     if (specs[k].max_out)
-      e = hipMemcpyAsync(specs[k].max_out, ws->d_stats + 2 * k, sizeof(float),
+      e = hipMemcpyAsync(specs[k].max_out, ws->d_stats + k, sizeof(float),
                          hipMemcpyDeviceToDevice, stream);
   if (e != hipSuccess) return hip_fail(e, "launch");
   return RNJ_OK;
@@ -432,7 +441,7 @@ int launch_jobs(Context* ctx, std::vector<JobSpec>& specs, bool raw, bool exact,
   if (specs.empty()) return RNJ_OK;
   LaunchShape sh = shape_of(specs);
   Workspace* ws = nullptr;
-  int rc = acquire_workspace(ctx, (unsigned)specs.size(), std::max<size_t>(sh.nsent, 1), &ws);
+  int rc = acquire_workspace(ctx, (unsigned)specs.size(), &ws);
   if (rc != RNJ_OK) return rc;
   int config = 0;
   rc = select_config(ctx, specs, sh, ws, exact, stream, allow_tune, &config);

@@ -44,9 +44,9 @@ struct Workspace {
   JobDesc* h_jobs = nullptr;
   JobDesc* d_jobs = nullptr;
   unsigned cap_jobs = 0;
-  unsigned* d_stats = nullptr;  // [cap_jobs][2]: maxbits, sent_count
-  unsigned long long* d_sent = nullptr;
-  size_t cap_sent = 0;
+  unsigned* d_stats = nullptr;  // [cap_jobs]: maxbits
+  uint32_t* d_sat = nullptr;    // [cap_units] wave words, then [cap_units][64] masks
+  size_t cap_units = 0;
   uint32_t* d_partial = nullptr;  // partial counts of split tiles
   size_t cap_partial = 0;         // u32 elements
   hipEvent_t done = nullptr;
@@ -152,7 +152,9 @@ JobSpec alignment_spec(const Alignment* a, int model, int layout, float* out, si
 
 unsigned round_up(unsigned x, unsigned m);
 unsigned words_for_sites(unsigned sites);  // ceil(sites/32) rounded up to kWordPad
-size_t sentinel_capacity(unsigned n);
+// Bytes of a launch workspace's saturation records for one job of n taxa, in the configuration
+// that needs the most (one word per wave of the epilogue, 64 masks per flagged wave).
+size_t saturation_bytes(unsigned n);
 
 // Allocate planes for an alignment of n taxa and `sites` counted sites (not initialised; the
 // packers write every word, a protein alignment's gap counts are zeroed on ctx->stream).
@@ -196,11 +198,9 @@ struct LaunchShape {
   bool mixed = false;  // jobs of different kinds or code widths (not launchable together)
   unsigned n_key = 0, W_key = 0;
   unsigned long long work = 0;
-  std::vector<size_t> caps;  // saturated-pair list capacity per job
-  size_t nsent = 0;
 };
 LaunchShape shape_of(const std::vector<JobSpec>& specs);
-int prepare_workspace(Workspace* ws, unsigned njobs, size_t nsent);
+int prepare_workspace(Workspace* ws, unsigned njobs);
 int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Workspace* ws,
                   bool exact, hipStream_t stream, bool allow_tune, int* config);
 // upload_table: copy the job table to the device first (a caller replaying a captured launch

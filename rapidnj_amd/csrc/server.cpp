@@ -134,11 +134,11 @@ class DeviceCache {
   size_t idle_ = 0, cap_ = 0;
 };
 
-// HBM a job holds from admission until its batch retires (in the cache's size classes; the
-// saturated-pair list lives in the launch's workspace).
+// HBM a job holds from admission until its batch retires (in the cache's size classes; its
+// saturation records live in the launch's workspace).
 size_t job_bytes(unsigned n, unsigned L) {
   return DeviceCache::bucket(chars_bytes(n, L)) + DeviceCache::bucket(planes_bytes(n, L)) +
-         DeviceCache::bucket(tri_bytes(n)) + sentinel_capacity(n) * 8;
+         DeviceCache::bucket(tri_bytes(n)) + saturation_bytes(n);
 }
 
 }  // namespace

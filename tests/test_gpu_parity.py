@@ -212,6 +212,37 @@ def test_every_tile_config_agrees(ctx, N):
         ctx.set_tile_config(-1)
 
 
+@pytest.mark.parametrize("model", ["jc", "k2p"])
+def test_saturation_records_every_config(ctx, N, model):
+    """Alignments where most pairs and where some pairs saturate, in every tile configuration,
+    both layouts and a raw run: the saturated pairs (marked per wave of the epilogue and mapped
+    back through each configuration's tile geometry) become 2*max, every one and only those, and
+    the maximum is the oracle's."""
+    import torch
+    rng = np.random.default_rng(12)
+    cases = [np.frombuffer(b"ACGT-", np.uint8)[rng.integers(0, 5, size=(300, 24))],
+             simulate_alignment(700, 60, 0.3, 13), simulate_alignment(260, 40, 0.5, 14)]
+    try:
+        for aln in cases:
+            n = aln.shape[0]
+            tri_ref, full_ref, mx_ref = O.distance_matrix(aln, model)
+            assert (tri_ref == -1).any()
+            iu = np.triu_indices(n, 1)
+            A = ctx.alignment(aln)
+            for c in range(len(N.tile_configs())):
+                ctx.set_tile_config(c)
+                mx = torch.zeros(1, dtype=torch.float32, device="cuda:0")
+                sq = _np(ctx.distances(A, model, layout="square", max_out=mx))
+                assert_distances_match(sq, full_ref)
+                assert float(mx.item()) == pytest.approx(mx_ref, rel=DIST_RTOL)
+                tri = _np(ctx.distances(A, model))
+                assert_distances_match(tri, full_ref[iu])
+                raw = _np(ctx.distances(A, model, raw=True))
+                np.testing.assert_array_equal(raw == -1, tri_ref == -1, err_msg=f"config {c}")
+    finally:
+        ctx.set_tile_config(-1)
+
+
 def test_bootstrap_matches_reference_on_expanded_columns(ctx):
     d = _load(os.path.join(GOLDEN, "bootstrap.npz"))
     A = ctx.alignment(d["aln"])

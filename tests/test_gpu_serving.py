@@ -1,5 +1,5 @@
 """Served jobs: each result is the same alone and in the mix (bitwise: counts are exact and every
-job keeps its own maximum and saturated-pair list), admission by HBM budget, error paths."""
+job keeps its own maximum and saturation records), admission by HBM budget, error paths."""
 import numpy as np
 import pytest
 
@@ -80,14 +80,19 @@ def _size_class(nbytes):
     return -(-nbytes // step) * step
 
 
+# (tile edge, waves per work-group) of the tile configurations
+_TILE_WAVES = ((64, 4), (64, 4), (128, 8), (32, 1), (32, 1))
+
+
 def _job_bytes(n, L):
     """HBM a served job holds: characters, planes and triangle (in the server's size classes)
-    and the saturated-pair list."""
+    and its saturation records (one word and 64 masks per epilogue wave, in the configuration
+    that needs the most)."""
     n_pad = -(-n // 128) * 128
     W = -(-max(1, -(-L // 32)) // 16) * 16
     pairs = n * (n - 1) // 2
-    return (_size_class(n * L) + _size_class(3 * W * n_pad * 4) + _size_class(pairs * 4)
-            + min(max(pairs // 64, 1024), 1 << 24) * 8)
+    sat = max((-(-n // t)) * (-(-n // t) + 1) // 2 * w * 65 * 4 for t, w in _TILE_WAVES)
+    return (_size_class(n * L) + _size_class(3 * W * n_pad * 4) + _size_class(pairs * 4) + sat)
 
 
 def test_size_classes():
