@@ -246,19 +246,23 @@ __device__ __forceinline__ unsigned find_job(const JobDesc* jobs, unsigned njobs
 // waves-per-work-group + wave of the work-group): bit 31 set when a lane of the wave holds a
 // saturated pair, bits 0-30 the wave's largest distance (distances are >= 0, so the bits of the
 // float compare as integers; a -0.0 folds into +0.0). When bit 31 is set, every lane also stores
-// its micro-tile mask (bit a * MJ + b: pair (i0 + a, j0 + b) saturated) at 64 * unit + lane.
+// its micro-tile mask (bit a * MJ + b: pair (i0 + a, j0 + b) saturated) at 64 * unit + lane, and
+// lane 0 the tile's block coordinates bij = bi << 16 | bj.
 // Only plain stores: job_max_kernel folds the words into the job's maximum and patch_kernel
 // rewrites the marked pairs. (One atomic per wave on the job's counters, which this replaces,
 // serialised at about 11 ns per atomic across the XCDs: 4.4 ms per launch at 20000 taxa.)
 // Every lane of the wave must call it.
-__device__ __forceinline__ void wave_record(const JobDesc& J, unsigned unit, uint32_t mask,
-                                            float lmax) {
+__device__ __forceinline__ void wave_record(const JobDesc& J, unsigned unit, unsigned bij,
+                                            uint32_t mask, float lmax) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, off));
   const bool any = __ballot(mask != 0u) != 0ull;  // uniform over the wave
   const unsigned lane = threadIdx.x & 63;
   if (any) J.sat_mask[(size_t)unit * 64 + lane] = mask;
-  if (lane == 0) J.sat_word[unit] = (any ? kSatFlag : 0u) | (__float_as_uint(lmax) & ~kSatFlag);
+  if (lane == 0) {
+    if (any) J.sat_tile[unit] = bij;
+    J.sat_word[unit] = (any ? kSatFlag : 0u) | (__float_as_uint(lmax) & ~kSatFlag);
+  }
 }
 
 // The counts of one finished pair (i < j < n) of a counts job, to its three upper triangles.

@@ -58,6 +58,7 @@ struct JobDesc {
   uint32_t* out_valid;
   unsigned* maxbits;          // bits of the largest distance (non-negative floats), job_max_kernel
   uint32_t* sat_word;         // [tiles * waves per work-group]: kSatFlag | wave maximum bits
+  uint32_t* sat_tile;         // [tiles * waves per work-group]: bi << 16 | bj, flagged waves
   uint32_t* sat_mask;         // [tiles * waves per work-group][64]: micro-tile masks, flagged waves
   const uint32_t* gaps;       // protein: gap sites per taxon [n_pad]
   unsigned L;                 // protein: alignment length (sites)

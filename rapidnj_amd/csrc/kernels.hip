@@ -79,7 +79,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, u
       }
     }
   }
-  wave_record(J, tile * (NT / 64) + tid / 64, sat, lmax);
+  wave_record(J, tile * (NT / 64) + tid / 64, bi << 16 | bj, sat, lmax);
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------
@@ -400,9 +400,14 @@ __global__ __launch_bounds__(256) void job_max_kernel(const JobDesc* __restrict_
 }
 
 // The reference post-process (KimuraDistance.cpp:366-395, JCdistance.cpp:201-230) replaces every
-// -1 by 2 * maxDistance. Each wave reads 64 wave words at a time (one per lane) and takes the
-// flagged units one by one: lane l reads the mask that lane l of the epilogue's wave stored and
-// maps its bits back to pairs with the launch's tile geometry (tile edge, micro-tile mi x mj).
+// -1 by 2 * maxDistance. Each wave takes kPatchUnits consecutive wave units of the job (grid y)
+// and issues all their loads at once (lane k < kPatchUnits: unit k's word and block
+// coordinates; every lane: its mask of each unit), so that one memory latency covers them. For a
+// flagged unit, lane l's mask is the one lane l of the epilogue's wave stored; its bits map back
+// to pairs with the unit's block coordinates and the launch's tile geometry (tile edge,
+// micro-tile mi x mj).
+constexpr unsigned kPatchUnits = 8;
+
 __global__ __launch_bounds__(256) void patch_kernel(const JobDesc* __restrict__ jobs,
                                                     unsigned edge, unsigned mi, unsigned mj) {
   const JobDesc& J = jobs[blockIdx.y];
@@ -410,33 +415,38 @@ __global__ __launch_bounds__(256) void patch_kernel(const JobDesc* __restrict__ 
   const unsigned ntx = edge / mj;
   const unsigned waves = (edge / mi) * ntx / 64;
   const unsigned long long units = (unsigned long long)J.tiles * waves;
-  const float val = __uint_as_float(*J.maxbits) * 2.0f;
+  const unsigned long long u0 =
+      ((unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * kPatchUnits;
+  if (u0 >= units) return;  // uniform over the wave
   const unsigned lane = threadIdx.x & 63;
-  const unsigned long long nw = (unsigned long long)gridDim.x * 4;
-  for (unsigned long long base = ((unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-       base < units; base += nw * 64) {
-    const unsigned long long u = base + lane;
-    const bool flagged = u < units && (J.sat_word[u] & kSatFlag) != 0u;
-    unsigned long long pend = __ballot(flagged);
-    while (pend) {  // uniform over the wave
-      const unsigned long long unit = base + (unsigned)(__ffsll((long long)pend) - 1);
-      pend &= pend - 1;
-      uint32_t mask = J.sat_mask[unit * 64 + lane];
-      const unsigned tile = (unsigned)(unit / waves);
-      const unsigned tid = (unsigned)(unit % waves) * 64 + lane;
-      unsigned bi, bj;
-      patch_coords(tile, J.nblk, bi, bj);
-      const unsigned i0 = bi * edge + (tid / ntx) * mi, j0 = bj * edge + (tid % ntx) * mj;
-      while (mask) {
-        const unsigned bit = (unsigned)(__ffs(mask) - 1);
-        mask &= mask - 1;
-        const unsigned i = i0 + bit / mj, j = j0 + bit % mj;
-        if (J.layout == kSquare) {
-          J.out[(size_t)i * J.ld + j] = val;
-          J.out[(size_t)j * J.ld + i] = val;
-        } else {
-          J.out[tri_index(i, j, J.n)] = val;
-        }
+  uint32_t word = 0, bij = 0;
+  if (lane < kPatchUnits && u0 + lane < units) {
+    word = J.sat_word[u0 + lane];
+    bij = J.sat_tile[u0 + lane];
+  }
+  uint32_t mask[kPatchUnits];
+#pragma unroll
+  for (unsigned k = 0; k < kPatchUnits; ++k)
+    mask[k] = u0 + k < units ? J.sat_mask[(u0 + k) * 64 + lane] : 0u;
+  const float val = __uint_as_float(*J.maxbits) * 2.0f;
+  const unsigned long long flagged = __ballot((word & kSatFlag) != 0u);
+#pragma unroll
+  for (unsigned k = 0; k < kPatchUnits; ++k) {
+    const unsigned b = (unsigned)__shfl((int)bij, (int)k);
+    if (((flagged >> k) & 1ull) == 0ull) continue;  // uniform over the wave
+    uint32_t m = mask[k];
+    const unsigned tid = (unsigned)((u0 + k) % waves) * 64 + lane;
+    const unsigned i0 = (b >> 16) * edge + (tid / ntx) * mi;
+    const unsigned j0 = (b & 0xffffu) * edge + (tid % ntx) * mj;
+    while (m) {
+      const unsigned bit = (unsigned)(__ffs(m) - 1);
+      m &= m - 1;
+      const unsigned i = i0 + bit / mj, j = j0 + bit % mj;
+      if (J.layout == kSquare) {
+        J.out[(size_t)i * J.ld + j] = val;
+        J.out[(size_t)j * J.ld + i] = val;
+      } else {
+        J.out[tri_index(i, j, J.n)] = val;
       }
     }
   }
@@ -667,11 +677,13 @@ hipError_t launch_saturation(const JobDesc* jobs, unsigned njobs, unsigned long 
                              int config, bool patch, hipStream_t stream) {
   if (njobs == 0 || max_units == 0) return hipSuccess;
   const TileConfig& c = kTileConfigs[config];
-  // about 16 words per thread for the maximum; 256 units per work-group pass for the patch
-  const unsigned mx = (unsigned)std::min<unsigned long long>(1024, (max_units + 4095) / 4096);
+  // about 4 words per thread for the maximum; kPatchUnits units per wave for the patch
+  const unsigned mx = (unsigned)std::min<unsigned long long>(2048, (max_units + 1023) / 1024);
   job_max_kernel<<<dim3(mx, njobs), dim3(256), 0, stream>>>(jobs, tile_threads(config) / 64);
   if (!patch) return hipGetLastError();
-  const unsigned px = (unsigned)std::min<unsigned long long>(512, (max_units + 255) / 256);
+  const unsigned long long per_block = 4ull * kPatchUnits;
+  if ((max_units + per_block - 1) / per_block >= (1ull << 31)) return hipErrorInvalidValue;
+  const unsigned px = (unsigned)((max_units + per_block - 1) / per_block);
   patch_kernel<<<dim3(px, njobs), dim3(256), 0, stream>>>(jobs, c.tile, c.micro_i, c.micro_j);
   return hipGetLastError();
 }

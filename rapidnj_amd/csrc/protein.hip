@@ -186,7 +186,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile
       }
     }
   }
-  wave_record(J, tile * ((TILE / MI) * (TILE / MJ) / 64) + tid / 64, sat, lmax);
+  wave_record(J, tile * ((TILE / MI) * (TILE / MJ) / 64) + tid / 64, bi << 16 | bj, sat, lmax);
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------

@@ -46,7 +46,7 @@ size_t saturation_bytes(unsigned n) {
   size_t most = 0;
   for (int c = 0; c < kNumTileConfigs; ++c) {
     const unsigned long long nb = (n + kTileConfigs[c].tile - 1) / kTileConfigs[c].tile;
-    most = std::max<size_t>(most, sat_units(nb * (nb + 1) / 2, c) * 65 * sizeof(uint32_t));
+    most = std::max<size_t>(most, sat_units(nb * (nb + 1) / 2, c) * 66 * sizeof(uint32_t));
   }
   return most;
 }
@@ -98,15 +98,15 @@ void free_workspace(Workspace* ws) {
   ws->cap_partial = 0;
 }
 
-// Saturation records for `units` wave units: no stores need clearing, since every epilogue wave
-// writes its word and a flagged wave all 64 masks.
+// Saturation records for `units` wave units (words, block coordinates, 64 masks each): nothing
+// needs clearing, since every epilogue wave writes its word and a flagged wave the rest.
 int grow_sat(Workspace* ws, size_t units) {
   if (units <= ws->cap_units) return RNJ_OK;
   const size_t cap = std::max(units, ws->cap_units + ws->cap_units / 2);
   if (ws->d_sat) (void)hipFree(ws->d_sat);
   ws->d_sat = nullptr;
   ws->cap_units = 0;
-  RNJ_HIP(hipMalloc((void**)&ws->d_sat, cap * 65 * sizeof(uint32_t)));
+  RNJ_HIP(hipMalloc((void**)&ws->d_sat, cap * 66 * sizeof(uint32_t)));
   ws->cap_units = cap;
   return RNJ_OK;
 }
@@ -252,7 +252,8 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
     d.L = s.L;
     d.maxbits = ws->d_stats + k;
     d.sat_word = ws->d_sat + P.sat_off[k];
-    d.sat_mask = ws->d_sat + ws->cap_units + 64 * P.sat_off[k];
+    d.sat_tile = ws->d_sat + ws->cap_units + P.sat_off[k];
+    d.sat_mask = ws->d_sat + 2 * ws->cap_units + 64 * P.sat_off[k];
     base += (unsigned long long)d.tiles * d.splits;
     if (d.splits > 1) fin += d.tiles;
   }
@@ -415,6 +416,9 @@ int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, i
   if (sh.mixed) return fail(RNJ_E_INVALID, "one launch cannot mix DNA and protein jobs");
   const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
   if (P.items >= (1ull << 31)) return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
+  for (const JobSpec& s : specs)  // saturation records keep block coordinates in 16 bits each
+    if ((s.n + kTileConfigs[config].tile - 1) / kTileConfigs[config].tile > 65535u)
+      return fail(RNJ_E_INVALID, "too many taxa for one distance launch");
   RNJ_HIP_STATUS(grow_partial(ws, P.partial_elems));
   RNJ_HIP_STATUS(grow_sat(ws, P.sat_units));
   number_jobs(specs, config, P, ws);

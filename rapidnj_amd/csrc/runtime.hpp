@@ -45,7 +45,7 @@ struct Workspace {
   JobDesc* d_jobs = nullptr;
   unsigned cap_jobs = 0;
   unsigned* d_stats = nullptr;  // [cap_jobs]: maxbits
-  uint32_t* d_sat = nullptr;    // [cap_units] wave words, then [cap_units][64] masks
+  uint32_t* d_sat = nullptr;    // [cap_units] wave words, [cap_units] tiles, [cap_units][64] masks
   size_t cap_units = 0;
   uint32_t* d_partial = nullptr;  // partial counts of split tiles
   size_t cap_partial = 0;         // u32 elements

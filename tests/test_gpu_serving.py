@@ -86,12 +86,12 @@ _TILE_WAVES = ((64, 4), (64, 4), (128, 8), (32, 1), (32, 1))
 
 def _job_bytes(n, L):
     """HBM a served job holds: characters, planes and triangle (in the server's size classes)
-    and its saturation records (one word and 64 masks per epilogue wave, in the configuration
-    that needs the most)."""
+    and its saturation records (a word, block coordinates and 64 masks per epilogue wave, in
+    the configuration that needs the most)."""
     n_pad = -(-n // 128) * 128
     W = -(-max(1, -(-L // 32)) // 16) * 16
     pairs = n * (n - 1) // 2
-    sat = max((-(-n // t)) * (-(-n // t) + 1) // 2 * w * 65 * 4 for t, w in _TILE_WAVES)
+    sat = max((-(-n // t)) * (-(-n // t) + 1) // 2 * w * 66 * 4 for t, w in _TILE_WAVES)
     return (_size_class(n * L) + _size_class(3 * W * n_pad * 4) + _size_class(pairs * 4) + sat)
 
 
