@@ -619,7 +619,8 @@ def main() -> int:
     value = site_pairs / (ms / 1e3) / 1e9
     log(f"{ms:.3f} ms/step = {value:.1f} G site-pairs/s")
 
-    # kernel duration: raw launches (statistics reset + tile kernel, no patch) between HIP
+    # kernel duration: raw launches (statistics reset + tile kernel + job maximum, no patch)
+    # between HIP
     # events recorded on the stream the library launches on
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     reps = max(3, args.steps)
@@ -633,7 +634,7 @@ def main() -> int:
     if wl.replicates:  # the bootstrap step also gathers replicate columns; report the whole launch
         kern_label = "bootstrap launch (gather + tile kernel)"
     else:
-        kern_label = "tile_kernel (+ stats reset)"
+        kern_label = "tile_kernel (+ stats reset, job_max_kernel)"
     planes = A.info.nplanes
     ops = ops_per_word_pair(A, wl.model) * pairs * (L / 32.0) * R
     achieved = ops / (kern_ms / 1e3) / 1e12

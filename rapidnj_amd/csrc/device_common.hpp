@@ -82,12 +82,33 @@ __device__ __forceinline__ float proportion(uint32_t a, uint32_t valid) {
 // refined by two Newton steps. The result is within about 2 ulp of double, so after rounding to
 // float it equals the reference's float value except in rare ties to a float rounding boundary;
 // it costs about a third of the instructions of the compensated library log.
-__device__ __forceinline__ double log_unit(double x) {
-  const double kLn2Hi = 0x1.62e42ff000000p-1;
-  const double kLn2Lo = -0x1.718432a1b0e26p-35;
+// log_unit's FP64 constants as values held in SGPRs for a whole epilogue. A VOP3 instruction
+// takes no 64-bit literal, so constants written as literals were rebuilt with two v_mov for
+// every pair an epilogue corrects; an empty asm with an SGPR operand makes each one an opaque
+// uniform value that the FP64 FMAs read straight from its SGPR pair.
+struct LogUnitConsts {
+  double inv_odd[10];  // 1/21, 1/19, ..., 1/3
+  double ln2_hi, ln2_lo, sqrt_half;
+};
+
+__device__ __forceinline__ LogUnitConsts log_unit_consts() {
+  LogUnitConsts K = {{1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0,
+                      1.0 / 9.0, 1.0 / 7.0, 1.0 / 5.0, 1.0 / 3.0},
+                     0x1.62e42ff000000p-1,    // ln 2, 32-bit head: k * head is exact
+                     -0x1.718432a1b0e26p-35,  // ln 2 - head
+                     0.70710678118654757};
+#pragma unroll
+  for (int q = 0; q < 10; ++q) asm volatile("" : "+s"(K.inv_odd[q]));
+  asm volatile("" : "+s"(K.ln2_hi));
+  asm volatile("" : "+s"(K.ln2_lo));
+  asm volatile("" : "+s"(K.sqrt_half));
+  return K;
+}
+
+__device__ __forceinline__ double log_unit(double x, const LogUnitConsts& K) {
   int k;
   double m = frexp(x, &k);            // m in [0.5, 1)
-  const bool low = m < 0.70710678118654757;
+  const bool low = m < K.sqrt_half;
   m = low ? m * 2.0 : m;              // m in [sqrt(1/2), sqrt(2))
   k = low ? k - 1 : k;
   const double f = m - 1.0;           // exact
@@ -97,20 +118,13 @@ __device__ __forceinline__ double log_unit(double x) {
   r = fma(r, fma(-d, r, 1.0), r);
   const double sd = f * r;
   const double t = sd * sd;
-  double p = 1.0 / 21.0;
-  p = fma(p, t, 1.0 / 19.0);
-  p = fma(p, t, 1.0 / 17.0);
-  p = fma(p, t, 1.0 / 15.0);
-  p = fma(p, t, 1.0 / 13.0);
-  p = fma(p, t, 1.0 / 11.0);
-  p = fma(p, t, 1.0 / 9.0);
-  p = fma(p, t, 1.0 / 7.0);
-  p = fma(p, t, 1.0 / 5.0);
-  p = fma(p, t, 1.0 / 3.0);
+  double p = K.inv_odd[0];
+#pragma unroll
+  for (int q = 1; q < 10; ++q) p = fma(p, t, K.inv_odd[q]);
   const double two_s = 2.0 * sd;
   const double lm = fma(two_s * t, p, two_s);
   const double kd = (double)k;
-  return fma(kd, kLn2Hi, fma(kd, kLn2Lo, lm));
+  return fma(kd, K.ln2_hi, fma(kd, K.ln2_lo, lm));
 }
 
 // Two proportions a/v and b/v as correctly rounded float quotients, from one FP64 reciprocal
@@ -146,7 +160,8 @@ __device__ __forceinline__ void proportions2(uint32_t a, uint32_t b, uint32_t v,
 // Either way the float result equals the reference's unless the exact value lies within about
 // 1e-9 relative of a float rounding boundary; it is always well inside 1e-6.
 template <bool kExact, bool kSmall>
-__device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t valid) {
+__device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t valid,
+                                           const LogUnitConsts& K) {
   const uint32_t v = valid ? valid : 1u;
   float P, Q;
   if (kExact) {
@@ -160,13 +175,14 @@ __device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t va
   const bool sat = valid == 0 || (float)x <= 0.0f || (float)y <= 0.0f;
   const double xs = sat ? 1.0 : x, ys = sat ? 1.0 : y;
   const float d = kExact ? (float)(-0.5 * log(xs) - 0.25 * log(ys))
-                         : (float)(-0.25 * log_unit(xs * xs * ys));
+                         : (float)(-0.25 * log_unit(xs * xs * ys, K));
   // valid == 0 implies ts = tv = 0
   return (ts | tv) == 0 ? (valid == 0 ? -1.0f : -0.0f) : (sat ? -1.0f : d);
 }
 
 template <bool kExact, bool kSmall>
-__device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid) {
+__device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid,
+                                          const LogUnitConsts& K) {
   const double b = 0.75;  // (alpha - 1) / alpha
   const uint32_t v = valid ? valid : 1u;
   const float p = kSmall ? (float)mismatches / (float)v : proportion(mismatches, v);
@@ -178,7 +194,7 @@ __device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid) {
     d = (float)(-b * log(sat ? 1.0 : 1.0 - q));
   } else {
     sat = p >= 0.75f;  // the same test: a float p < 0.75 gives p / 0.75 < 1 - 2^-25 in double
-    d = (float)(-b * log_unit(sat ? 1.0 : fma(-(double)p, 4.0 / 3.0, 1.0)));
+    d = (float)(-b * log_unit(sat ? 1.0 : fma(-(double)p, 4.0 / 3.0, 1.0), K));
   }
   return (valid == 0 || mismatches == 0) ? -0.0f : (sat ? -1.0f : d);
 }

@@ -50,6 +50,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, u
     return;
   }
   // One row of MJ corrections as straight-line code (interleaved by the compiler), then stores.
+  const LogUnitConsts K = log_unit_consts();
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
@@ -57,9 +58,10 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, u
 #pragma unroll
     for (int b = 0; b < MJ; ++b) {
       if (J.model == kJC)
-        d[b] = jc_value<kExact, false>(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b], acc_v[a][b]);
+        d[b] = jc_value<kExact, false>(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b], acc_v[a][b],
+                                       K);
       else
-        d[b] = k2p_value<kExact, false>(acc_b[a][b], acc_a[a][b], acc_v[a][b]);
+        d[b] = k2p_value<kExact, false>(acc_b[a][b], acc_a[a][b], acc_v[a][b], K);
     }
 #pragma unroll
     for (int b = 0; b < MJ; ++b) {

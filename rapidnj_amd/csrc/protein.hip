@@ -86,12 +86,13 @@ __device__ __forceinline__ float protein_proportion(uint32_t mism, unsigned long
 // float, the rest in double), the Dayhoff table up to 0.930 and 10 above. Straight-line (every
 // case computed, the result selected) so that a caller can interleave several pairs.
 template <bool kExact>
-__device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid) {
+__device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid,
+                                                     const LogUnitConsts& K) {
   const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : -1.0f;
   const bool formula = p < 0.75f && p > 0.0f;
   const float pp = p * p;
   const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
-  const double lg = kExact ? log(arg) : log_unit(arg);
+  const double lg = kExact ? log(arg) : log_unit(arg, K);
   int idx = (int)((double)p * 1000.0 - 750.0);
   idx = idx < 0 ? 0 : (idx > 180 ? 180 : idx);
   const float table = (double)p > 0.930 ? 10.0f : kDayhoff[idx];
@@ -101,13 +102,14 @@ __device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned lo
 // JCdistance.cpp:170-183 with alpha = 20: p = 0 when valid is 0, -1 when p / 0.95 >= 1 (the
 // quotient in double, as the reference forms it, so that the saturated pairs are the same).
 template <bool kExact>
-__device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long long valid) {
+__device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long long valid,
+                                                 const LogUnitConsts& K) {
   const double b = 19.0 / 20.0;  // (alpha - 1) / alpha
   const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : 0.0f;
   const double q = (double)p / b;
   const bool sat = q >= 1.0;
   const double arg = sat ? 1.0 : 1.0 - q;
-  const double lg = kExact ? log(arg) : log_unit(arg);
+  const double lg = kExact ? log(arg) : log_unit(arg, K);
   return sat ? -1.0f : (float)(-b * lg);
 }
 #pragma clang fp contract(on)
@@ -156,6 +158,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile
   }
   float lmax = 0.0f;
   uint32_t sat = 0;  // saturated pairs of the micro-tile, bit a * MJ + b
+  const LogUnitConsts K = log_unit_consts();
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
@@ -165,8 +168,8 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile
       uint32_t mism;
       unsigned long long valid;
       counts(a, b, mism, valid);
-      d[b] = J.model == kJC ? jc_protein_value<kExact>(mism, valid)
-                            : kimura_protein_value<kExact>(mism, valid);
+      d[b] = J.model == kJC ? jc_protein_value<kExact>(mism, valid, K)
+                            : kimura_protein_value<kExact>(mism, valid, K);
     }
 #pragma unroll
     for (int b = 0; b < MJ; ++b) {
