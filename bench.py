@@ -449,9 +449,10 @@ def uneven_batch_compare(ctx, jobs, alns, count: int = 64) -> dict:
     outs = [torch.empty(max(1, j.n * (j.n - 1) // 2), dtype=torch.float32, device="cuda:0")
             for j in jobs]
     sp = sum(j.n * (j.n - 1) // 2 * j.L for j in jobs)
-    k = 5
-    for a, m, o in zip(A, models, outs):
-        ctx.distances(a, m, out=o)
+    k = 10
+    for _ in range(2):
+        for a, m, o in zip(A, models, outs):
+            ctx.distances(a, m, out=o)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(k):
@@ -464,6 +465,8 @@ def uneven_batch_compare(ctx, jobs, alns, count: int = 64) -> dict:
         plan.run()
         torch.cuda.synchronize()
         same = all(bool(torch.equal(o, r)) for o, r in zip(outs, ref))
+        plan.run()
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(k):
             plan.run()
@@ -489,10 +492,10 @@ def bootstrap_compare(ctx, dev) -> dict:
     w = bootstrap_weights(wl.L, wl.replicates, seed0=0)
     pairs = wl.n * (wl.n - 1) // 2
     out = torch.empty((wl.replicates, pairs), dtype=torch.float32, device=dev)
-    for _ in range(2):
+    for _ in range(3):
         ctx.bootstrap(A, w, wl.model, out=out)
     torch.cuda.synchronize()
-    k = 5
+    k = 20
     t0 = time.perf_counter()
     for _ in range(k):
         ctx.bootstrap(A, w, wl.model, out=out)
@@ -509,15 +512,19 @@ def bootstrap_compare(ctx, dev) -> dict:
     torch.cuda.synchronize()
     single = (time.perf_counter() - t0) / 2
     same = bool(torch.equal(out, ref))
-    # the same launch captured once into a HIP graph, replayed with fresh weights per call
-    sets = [bootstrap_weights(wl.L, wl.replicates, seed0=1000 * (i + 1)) for i in range(k)]
+    # the same launch captured into a HIP graph, replayed with fresh weights per call (the plan
+    # alternates two graphs, one per pinned weight buffer: both replay once before the timing)
+    sets = [bootstrap_weights(wl.L, wl.replicates, seed0=1000 * (i + 1)) for i in range(4)]
     with ctx.bootstrap_plan(A, wl.replicates, wl.model, out=out) as plan:
         plan.run(w)
         torch.cuda.synchronize()
         same = same and bool(torch.equal(out, ref))
-        t0 = time.perf_counter()
-        for ws_ in sets:
+        for ws_ in sets[:2]:
             plan.run(ws_)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(k):
+            plan.run(sets[i % 4])
         torch.cuda.synchronize()
         graphed = (time.perf_counter() - t0) / k
     sp = pairs * wl.L * wl.replicates
