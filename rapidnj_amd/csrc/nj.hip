@@ -150,7 +150,8 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
 }
 
 // Launch A: work-group 0 finishes the previous join's new cluster when asked; every other
-// work-group scans 16 consecutive row slots, one per wave, over partner slots b < row. The
+// work-group scans 16 consecutive row slots, one per wave, over partner slots b < row, the
+// highest slots in the first work-groups. The
 // separations of the partners are staged in LDS once per work-group and shared by its rows.
 // With `finish`, each wave also writes its row's entry in the new slot's column (S[a][ns] =
 // newcol[a], a < ns): the step wrote the new row, and no scan reads that column (row a's
@@ -170,7 +171,10 @@ __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, float
   __shared__ float ssep[kSepChunk];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int used = st.used;
-  const int a0 = blk * kScanRows;
+  // longest rows first: row a has a partners, so the groups go out in decreasing slot order and
+  // the launch ends on its shortest rows
+  const int groups = (int)gridDim.x - (finish ? 1 : 0);
+  const int a0 = (groups - 1 - blk) * kScanRows;
   if (a0 >= used) return;  // uniform over the work-group
   const int a = a0 + wave;
   if (finish && lane == 0 && a < used - 1) S[(size_t)a * D.ld + (used - 1)] = D.newcol[a];
