@@ -565,8 +565,8 @@ def main() -> int:
     if args.side:
         return side_main(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per "
-              "box); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: rapidnj_amd has no multi-GPU path (the distance "
+              "launch runs on one MI355X); run with --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np  # noqa: F401
