@@ -439,7 +439,7 @@ def one_job_per_launch(ctx, jobs, alns) -> dict:
 
 
 def uneven_batch_compare(ctx, jobs, alns, count: int = 64) -> dict:
-    """The first `count` alignments of the served stream (uneven taxa and site counts, JC and K2P),
+    """The first `count` alignments of a job list (uneven taxa and site counts, JC and K2P),
     resident in HBM: one captured multi-alignment launch (rnj_batch_plan) against one distance
     launch per alignment, outputs in HBM either way."""
     import torch
@@ -480,6 +480,21 @@ def uneven_batch_compare(ctx, jobs, alns, count: int = 64) -> dict:
             "one_launch_per_alignment_value": round(sp / single / 1e9, 2),
             "speedup": round(single / graphed, 2), "identical_results": same,
             "unit": "G site-pairs/s"}
+
+
+def small_alignments(count: int = 256, seed: int = 17):
+    """`count` small alignments (20-300 taxa, 200-2000 sites, gaps 0-8 %, JC/K2P alternating),
+    as jobs of the served-stream shape, with their character matrices."""
+    import numpy as np
+    from types import SimpleNamespace
+    from rapidnj_amd.workloads import simulate_alignment
+    rng = np.random.default_rng(seed)
+    jobs, alns = [], []
+    for k in range(count):
+        n, L = int(rng.integers(20, 301)), int(rng.integers(200, 2001))
+        jobs.append(SimpleNamespace(n=n, L=L, model="jc" if k % 2 else "k2p"))
+        alns.append(simulate_alignment(n, L, float(rng.uniform(0, 0.08)), 9000 + k))
+    return jobs, alns
 
 
 def bootstrap_compare(ctx, dev) -> dict:
@@ -709,6 +724,9 @@ def main() -> int:
             "one_job_per_launch": one_job_per_launch(ctx, served_jobs, served_alns)}
         log("side run: 64 uneven alignments in one captured launch vs one launch each")
         extra["uneven_batch"] = uneven_batch_compare(ctx, served_jobs, served_alns)
+        log("side run: 256 small alignments in one captured launch vs one launch each")
+        sj, sa = small_alignments()
+        extra["uneven_batch_small"] = uneven_batch_compare(ctx, sj, sa, count=len(sj))
 
     if not args.no_traffic and not wl.replicates:
         log("HBM traffic: two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE)")
