@@ -51,6 +51,7 @@ int rnj_batch_plan_create(rnj_context* ctx, const rnj_alignment* const* alns, co
                           int flags, rnj_batch_plan** plan) {
   if (!ctx || !alns || !models || !outs || !plan) return fail(RNJ_E_INVALID, "null argument");
   if (count == 0) return fail(RNJ_E_INVALID, "a batch plan needs at least one alignment");
+  if (count > kMaxLaunchJobs) return fail(RNJ_E_INVALID, "more than 65535 alignments in one plan");
   if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");
   if (layout == RNJ_SQUARE && !lds) return fail(RNJ_E_INVALID, "square layout needs row strides");
   std::vector<JobSpec> specs;

@@ -103,6 +103,7 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
   if (model != RNJ_JC && model != RNJ_K2P) return fail(RNJ_E_INVALID, "model must be RNJ_JC or RNJ_K2P");
   if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");
   if (R == 0) return fail(RNJ_E_INVALID, "a plan needs at least one replicate");
+  if (R > kMaxLaunchJobs) return fail(RNJ_E_INVALID, "more than 65535 replicates in one plan");
   const unsigned n = aln->n, L = aln->L;
   if (layout == RNJ_SQUARE && ld < n) return fail(RNJ_E_INVALID, "ld smaller than n");
   const size_t min_stride = layout == RNJ_SQUARE ? (size_t)n * ld : (size_t)n * (n - 1) / 2;

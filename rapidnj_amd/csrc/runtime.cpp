@@ -414,6 +414,7 @@ int prepare_workspace(Workspace* ws, unsigned njobs) { return grow_workspace(ws,
 int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
                    Workspace* ws, bool raw, bool exact, hipStream_t stream, bool upload_table) {
   if (sh.mixed) return fail(RNJ_E_INVALID, "one launch cannot mix DNA and protein jobs");
+  if (specs.size() > kMaxLaunchJobs) return fail(RNJ_E_INVALID, "more than 65535 jobs in one launch");
   const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
   if (P.items >= (1ull << 31)) return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
   for (const JobSpec& s : specs)  // saturation records keep block coordinates in 16 bits each
@@ -1233,6 +1234,7 @@ int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
     return fail(RNJ_E_NOT_DNA, "bootstrap replicates are built for DNA alignments only");
   if (!weights || !out) return fail(RNJ_E_INVALID, "null argument");
   if (R == 0) return RNJ_OK;
+  if (R > kMaxLaunchJobs) return fail(RNJ_E_INVALID, "more than 65535 replicates in one call");
   if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");
   const unsigned n = aln->n, L = aln->L;
   const size_t min_stride = layout == RNJ_SQUARE ? (size_t)n * ld : (size_t)n * (n - 1) / 2;

@@ -155,6 +155,8 @@ unsigned words_for_sites(unsigned sites);  // ceil(sites/32) rounded up to kWord
 // Bytes of a launch workspace's saturation records for one job of n taxa, in the configuration
 // that needs the most (one word per wave of the epilogue, 64 masks per flagged wave).
 size_t saturation_bytes(unsigned n);
+// Jobs in one launch: the statistics, maximum and patch kernels index jobs by grid y.
+constexpr unsigned kMaxLaunchJobs = 65535;
 
 // Allocate planes for an alignment of n taxa and `sites` counted sites (not initialised; the
 // packers write every word, a protein alignment's gap counts are zeroed on ctx->stream).

@@ -383,6 +383,7 @@ int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server
   if (cfg) srv->cfg = *cfg;
   if (srv->cfg.slots == 0) srv->cfg.slots = 2;
   if (srv->cfg.max_batch_jobs == 0) srv->cfg.max_batch_jobs = 64;
+  srv->cfg.max_batch_jobs = std::min(srv->cfg.max_batch_jobs, kMaxLaunchJobs);
   if (srv->cfg.hbm_budget == 0) {
     size_t free_b = 0, total_b = 0;
     hipError_t e = hipMemGetInfo(&free_b, &total_b);

@@ -122,6 +122,27 @@ def test_batch_plan_sees_updated_characters(ctx):
         np.testing.assert_array_equal(A[0].planes(), ctx.alignment(new0).planes())
 
 
+def test_launch_job_limit(ctx, N):
+    """More than 65535 jobs in one launch (the maximum and patch kernels index jobs by grid y)
+    are refused before any work: replicates of a bootstrap call or plan, alignments of a batch
+    plan."""
+    import torch
+    A = ctx.alignment(simulate_alignment(4, 40, 0.0, 5))
+    R = 65536
+    w = np.ones((R, 40), dtype=np.uint16)
+    out = torch.empty((R, 6), dtype=torch.float32, device="cuda:0")
+    with pytest.raises(N.RnjError):
+        ctx.bootstrap(A, w, "k2p", out=out)
+    with pytest.raises(N.RnjError):
+        ctx.bootstrap_plan(A, R, "k2p", out=out)
+    with pytest.raises(N.RnjError):
+        ctx.batch_plan([A] * R, "jc", outs=[out[k] for k in range(R)])
+    # 65535 is accepted
+    got = ctx.bootstrap(A, w[:65535], "k2p", out=out[:65535]).cpu().numpy()
+    ref = ctx.distances(A, "k2p").cpu().numpy()
+    assert np.array_equal(got[65534].view(np.uint32), ref.view(np.uint32))
+
+
 def test_batch_plan_rejections(ctx, N):
     from rapidnj_amd.workloads import simulate_protein_alignment
     dna = ctx.alignment(simulate_alignment(20, 100, 0.0, 1))
