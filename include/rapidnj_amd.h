@@ -141,6 +141,18 @@ RNJ_EXPORT int rnj_valu_issue_rates(rnj_context* ctx, double* bitwise, double* b
  * (dataloaderFasta.hpp:54-76: DNA unless a byte >= 'A' is not one of ACGTUN of either case, else
  * protein); RNJ_TYPE_DNA / RNJ_TYPE_PROTEIN force the type, like the reference's `-i d|p`. */
 RNJ_EXPORT int rnj_set_input_type(rnj_context* ctx, int type);
+/* Character rules of DNA alignments read from characters afterwards (rnj_alignment_from_chars /
+ * _rows). RNJ_CHARS_BITS (default): the counts of the bit-string engine, bitDistanceGap
+ * (bitDistanceGap.cpp:60-336), which counts ACGT of either case. RNJ_CHARS_NAIVE: the counts
+ * of the naive counter the reference's `-f` option selects (src/main.cpp:589,597;
+ * simpleDistanceCalculator::calculateTsTv, simpleDistanceCalculator.cpp:58-98, after the
+ * loader's resolveChar, dataloaderFasta.hpp:184-191): a site counts when neither character is a
+ * gap (any byte but ACGTU of either case), and a mismatch is a transition or transversion only
+ * when the lower-index taxon's character is uppercase A, C, G or T. Such alignments are packed
+ * with a fourth bit plane and accepted by the distance, count, hand-off and batch-plan calls;
+ * bootstrap, served jobs and character updates take bit-rule alignments only. */
+enum { RNJ_CHARS_BITS = 0, RNJ_CHARS_NAIVE = 1 };
+RNJ_EXPORT int rnj_set_char_rules(rnj_context* ctx, int rules);
 
 /* Alignments packed once into HBM as bit planes.
  * from_chars: n rows of L bytes, row t at chars + t*pitch, on the host (chars_on_device = 0) or

@@ -76,7 +76,9 @@ int loader_type(Loader*, long) {
 // Packs the loader's alignment into HBM. DNA with fastdist: the loader's bit strings, which are
 // exactly what the reference's bitDistanceGap counts. Protein with fastdist: the same bit
 // strings hold the resolved characters four to a word (dataloaderFasta.hpp
-// encodeProteinSequence), so their first L bytes are the row. Otherwise the character rows.
+// encodeProteinSequence), so their first L bytes are the row. Otherwise the character rows;
+// DNA rows then count under the naive counter's rules (RNJ_CHARS_NAIVE), as `-f` does in the
+// reference (simpleDistanceCalculator).
 // The shared context's input type is set for the upload and reset after it (the reference's
 // callers are single-threaded).
 template <class Loader>
@@ -90,7 +92,9 @@ rnj_alignment* upload_alignment(Loader* loader, bool fastdist) {
         fastdist ? reinterpret_cast<const char* const*>(loader->getBitStrings())
                  : const_cast<const char* const*>(loader->getSequences()->data());
     rnj_set_input_type(shared_context(), type);
+    rnj_set_char_rules(shared_context(), fastdist ? RNJ_CHARS_BITS : RNJ_CHARS_NAIVE);
     rc = rnj_alignment_from_rows(shared_context(), rows, n, loader->getSequenceLength(), &aln);
+    rnj_set_char_rules(shared_context(), RNJ_CHARS_BITS);
     rnj_set_input_type(shared_context(), RNJ_TYPE_AUTO);
   } else {
     rc = rnj_alignment_from_bitstrings(shared_context(),

@@ -14,6 +14,17 @@ __device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) 
   return __builtin_amdgcn_bitop3_b32(a, b, c, 0x28u);
 }
 
+// ~(a ^ b) & c, (a ^ b) | ~c and a & b & c, each one v_bitop3_b32 (same truth-table inputs).
+__device__ __forceinline__ uint32_t xnor_and(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x82u);
+}
+__device__ __forceinline__ uint32_t xor_or_not(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x7Du);
+}
+__device__ __forceinline__ uint32_t and3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x80u);
+}
+
 // popcount(x) + acc as one v_bcnt_u32_b32. Written out because LLVM otherwise re-associates
 // the per-word accumulations into v_bcnt + v_add3 trees (1.5 extra VALU ops per pair and word).
 __device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {

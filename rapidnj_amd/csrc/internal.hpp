@@ -29,7 +29,11 @@ constexpr uint32_t kSentinelBits = 0xBF800000u;  // -1.0f, the reference's satur
 constexpr uint32_t kSatFlag = 0x80000000u;       // a wave's saturation word: it holds saturated pairs
 
 enum Model : int { kJC = 0, kK2P = 1, kCounts = 2 };
-enum Kind : int { kDNA = 0, kProtein = 1 };
+// kNaive: DNA under the reference's naive character rules (simpleDistanceCalculator, `-f`),
+// packed with a fourth plane.
+enum Kind : int { kDNA = 0, kProtein = 1, kNaive = 2 };
+// Count loops of the DNA tile kernel: K2P (and counts), JC only, naive character rules.
+enum CountMode : int { kModeK2P = 0, kModeJC6 = 1, kModeNaive = 2 };
 enum Layout : int { kTriangle = 0, kSquare = 1 };
 
 // One distance matrix inside a launch. A launch covers jobs[0..njobs); the work items of job k
@@ -82,16 +86,18 @@ extern const TileConfig kTileConfigs[];
 extern const int kNumTileConfigs;
 
 // Launchers (kernels.hip). All enqueue on `stream` and return the launch status.
+// naive: the four planes of the naive character rules (H, L, V, U) instead of H, L, V.
 hipError_t launch_pack_chars(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
                              unsigned n_pad, unsigned W_pad, uint32_t* planes,
-                             hipStream_t stream);
+                             hipStream_t stream, bool naive = false);
 hipError_t launch_pack_legacy(const uint32_t* bits, const uint32_t* gaps, size_t stride,
                               unsigned words16, unsigned n, unsigned n_pad, unsigned W_pad,
                               uint32_t* planes, hipStream_t stream);
 hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
 // exact: the reference's expression with the library's FP64 log; otherwise the leaner FP64 log
 // (device_common.hpp, within ~2 ulp of double).
-hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
+// mode: CountMode (kModeJC6 only when every job is JC; kModeNaive for naive-rule alignments).
+hipError_t launch_tiles(int config, int mode, bool exact, const JobDesc* jobs, unsigned njobs,
                         unsigned total_items, unsigned finish_items, hipStream_t stream);
 // Protein (protein.hip). census: sets bit v of present[8] for every byte value v in the n x L
 // characters (present must start at zero). pack: characters -> gap plane + B code planes using

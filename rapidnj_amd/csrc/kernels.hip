@@ -87,12 +87,25 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, u
 // ---- the pair-tile kernel ------------------------------------------------------------------
 // Word w of the staged chunk into thread (ty, tx)'s MI x MJ accumulators (7 VALU ops per pair
 // for K2P, 6 for JC).
-template <int MI, int MJ, int CW, int Q, bool JC6>
+template <int MI, int MJ, int CW, int Q, int MODE>
 __device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u* __restrict__ sJ,
                                            int w, unsigned ty, unsigned tx,
                                            uint32_t (&acc_a)[MI][MJ], uint32_t (&acc_b)[MI][MJ],
                                            uint32_t (&acc_v)[MI][MJ]) {
   uint32_t hi[MI], li[MI], vi[MI], hj[MJ], lj[MJ], vj[MJ];
+  uint32_t ui[MODE == kModeNaive ? MI : 1], uj[MODE == kModeNaive ? MJ : 1];
+  if constexpr (MODE == kModeNaive) {
+#pragma unroll
+    for (int r = 0; r < MI / 4; ++r) {
+      const v4u u = sI[(3 * CW + w) * Q + ty * (MI / 4) + r];
+      ui[4 * r] = u.x; ui[4 * r + 1] = u.y; ui[4 * r + 2] = u.z; ui[4 * r + 3] = u.w;
+    }
+#pragma unroll
+    for (int r = 0; r < MJ / 4; ++r) {
+      const v4u u = sJ[(3 * CW + w) * Q + tx * (MJ / 4) + r];
+      uj[4 * r] = u.x; uj[4 * r + 1] = u.y; uj[4 * r + 2] = u.z; uj[4 * r + 3] = u.w;
+    }
+  }
 #pragma unroll
   for (int r = 0; r < MI / 4; ++r) {
     const v4u h = sI[(0 * CW + w) * Q + ty * (MI / 4) + r];
@@ -116,9 +129,18 @@ __device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u
 #pragma unroll
     for (int b = 0; b < MJ; ++b) {
       const uint32_t g = vi[a] & vj[b];
-      const uint32_t t = xor_and(hi[a], hj[b], g);
       acc_v[a][b] = bcnt_add(g, acc_v[a][b]);
-      if constexpr (JC6) {
+      if constexpr (MODE == kModeNaive) {
+        // ts: both uppercase ACGT, same H, different L; tv: i uppercase ACGT, j counted, and j
+        // not uppercase ACGT or a different H (simpleDistanceCalculator.cpp:58-98)
+        const uint32_t x = and3(ui[a], uj[b], li[a] ^ lj[b]);
+        const uint32_t y = xor_or_not(hi[a], hj[b], uj[b]);
+        acc_b[a][b] = bcnt_add(xnor_and(hi[a], hj[b], x), acc_b[a][b]);
+        acc_a[a][b] = bcnt_add(and3(ui[a], vj[b], y), acc_a[a][b]);
+        continue;
+      }
+      const uint32_t t = xor_and(hi[a], hj[b], g);
+      if constexpr (MODE == kModeJC6) {
         const uint32_t m = t | xor_and(li[a], lj[b], g);
         acc_a[a][b] = bcnt_add(m, acc_a[a][b]);
       } else {
@@ -136,13 +158,14 @@ __device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u
 // [plane][word][taxon], written linearly), the next chunk is prefetched into registers while
 // the current one is counted. Unsplit jobs finish in place; split jobs leave exact partial
 // counts for finish_kernel.
-template <int TILE, int MI, int MJ, int CW, bool JC6, bool kExact>
+template <int TILE, int MI, int MJ, int CW, int MODE, bool kExact>
 __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
     const JobDesc* __restrict__ jobs, unsigned njobs) {
   constexpr int NTX = TILE / MJ;
   constexpr int NT = (TILE / MI) * NTX;
   constexpr int Q = TILE / 4;                 // uint4 per plane-word row of one block
-  constexpr int E = 3 * CW * Q;               // uint4 per block and chunk
+  constexpr int P = MODE == kModeNaive ? 4 : 3;  // planes
+  constexpr int E = P * CW * Q;               // uint4 per block and chunk
   static_assert(MI % 4 == 0 && MJ % 4 == 0, "micro-tiles are read as uint4");
   constexpr int EP = (E + NT - 1) / NT;       // uint4 per thread and block (last may be partial)
 
@@ -193,7 +216,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
 #pragma unroll 4
     for (int w = 0; w < CW; ++w) {
       if (w >= live) break;
-      count_word<MI, MJ, CW, Q, JC6>(sI, sJ, w, ty, tx, acc_a, acc_b, acc_v);
+      count_word<MI, MJ, CW, Q, MODE>(sI, sJ, w, ty, tx, acc_a, acc_b, acc_v);
     }
   }
 
@@ -210,7 +233,8 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
       }
     return;
   }
-  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, tile, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
+  tile_epilogue<TILE, MI, MJ, MODE == kModeJC6, kExact>(J, tile, bi, bj, ty, tx, tid, acc_a, acc_b,
+                                                        acc_v);
 }
 
 // Sums the partial counts of the splits of one tile of a split job and finishes the tile. Work
@@ -254,17 +278,23 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
 }
 
 template <int TILE, int MI, int MJ, int CW>
-hipError_t launch_cfg(bool jc6, bool exact, const JobDesc* jobs, unsigned njobs, unsigned total,
+hipError_t launch_cfg(int mode, bool exact, const JobDesc* jobs, unsigned njobs, unsigned total,
                       unsigned finish_items, hipStream_t s) {
   constexpr int NT = (TILE / MI) * (TILE / MJ);
   const dim3 grid(total), block(NT);
-  if (jc6 && exact) tile_kernel<TILE, MI, MJ, CW, true, true><<<grid, block, 0, s>>>(jobs, njobs);
-  else if (jc6) tile_kernel<TILE, MI, MJ, CW, true, false><<<grid, block, 0, s>>>(jobs, njobs);
-  else if (exact) tile_kernel<TILE, MI, MJ, CW, false, true><<<grid, block, 0, s>>>(jobs, njobs);
-  else tile_kernel<TILE, MI, MJ, CW, false, false><<<grid, block, 0, s>>>(jobs, njobs);
+#define RNJ_TILES(M, X) tile_kernel<TILE, MI, MJ, CW, M, X><<<grid, block, 0, s>>>(jobs, njobs)
+  if (mode == kModeJC6) {
+    if (exact) RNJ_TILES(kModeJC6, true); else RNJ_TILES(kModeJC6, false);
+  } else if (mode == kModeNaive) {
+    if (exact) RNJ_TILES(kModeNaive, true); else RNJ_TILES(kModeNaive, false);
+  } else {
+    if (exact) RNJ_TILES(kModeK2P, true); else RNJ_TILES(kModeK2P, false);
+  }
+#undef RNJ_TILES
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || finish_items == 0) return e;
   const dim3 fgrid(finish_items);
+  const bool jc6 = mode == kModeJC6;
   if (jc6 && exact) finish_kernel<TILE, MI, MJ, true, true><<<fgrid, block, 0, s>>>(jobs, njobs);
   else if (jc6) finish_kernel<TILE, MI, MJ, true, false><<<fgrid, block, 0, s>>>(jobs, njobs);
   else if (exact) finish_kernel<TILE, MI, MJ, false, true><<<fgrid, block, 0, s>>>(jobs, njobs);
@@ -290,6 +320,21 @@ __device__ __forceinline__ void char_bits(uint8_t c, uint32_t& h, uint32_t& l, u
 // or past L and taxa at or past n pack as 0 (not counted).
 constexpr unsigned kPackTaxa = 64, kPackWords = 32, kPackRow = kPackWords * 8 + 1;
 
+// Bits of one character under the naive counter's rules: the reference's loader first turns
+// every byte but ACGTU of either case into '-' (dataloaderFasta.hpp:184-191), and
+// simpleDistanceCalculator::calculateTsTv (simpleDistanceCalculator.cpp:58-98) counts a site
+// when neither is '-' (V), and classifies a mismatch only when the row's character is uppercase
+// A, C, G or T (U plane), by exact byte: H and L as char_bits for those four.
+__device__ __forceinline__ void naive_bits(uint8_t c, uint32_t& h, uint32_t& l, uint32_t& v,
+                                           uint32_t& u) {
+  const uint8_t lc = c | 0x20;
+  v = (lc == 'a') | (lc == 'c') | (lc == 'g') | (lc == 't') | (lc == 'u');
+  u = (c == 'A') | (c == 'C') | (c == 'G') | (c == 'T');
+  h = (c == 'C') | (c == 'T');
+  l = (c == 'G') | (c == 'C');
+}
+
+template <bool NAIVE>
 __global__ __launch_bounds__(256) void pack_chars_kernel(const uint8_t* __restrict__ chars,
                                                          size_t pitch, unsigned n, unsigned L,
                                                          unsigned n_pad, unsigned W_pad,
@@ -316,23 +361,26 @@ __global__ __launch_bounds__(256) void pack_chars_kernel(const uint8_t* __restri
     for (unsigned ww = wave; ww < kPackWords; ww += blockDim.x / 64) {
       const unsigned w = w0 + ww;
       if (w >= W_pad) break;
-      uint32_t H = 0, Lo = 0, V = 0;
+      uint32_t H = 0, Lo = 0, V = 0, U = 0;
 #pragma unroll
       for (unsigned j = 0; j < 8; ++j) {
         const uint32_t x = tile[lane * kPackRow + ww * 8 + j];
 #pragma unroll
         for (unsigned k = 0; k < 4; ++k) {
-          uint32_t h, l, v;
-          char_bits((uint8_t)(x >> (8 * k)), h, l, v);
+          uint32_t h, l, v, u = 0;
+          if constexpr (NAIVE) naive_bits((uint8_t)(x >> (8 * k)), h, l, v, u);
+          else char_bits((uint8_t)(x >> (8 * k)), h, l, v);
           H |= h << (4 * j + k);
           Lo |= l << (4 * j + k);
           V |= v << (4 * j + k);
+          U |= u << (4 * j + k);
         }
       }
       if (t < n_pad) {
         planes[0 * ps + (size_t)w * n_pad + t] = H;
         planes[1 * ps + (size_t)w * n_pad + t] = Lo;
         planes[2 * ps + (size_t)w * n_pad + t] = V;
+        if constexpr (NAIVE) planes[3 * ps + (size_t)w * n_pad + t] = U;
       }
     }
     __syncthreads();  // the tile is rewritten by the next word block
@@ -614,10 +662,13 @@ static unsigned ygrid(unsigned rows) { return rows < 1 ? 1 : (rows > 65535 ? 655
 
 hipError_t launch_pack_chars(const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
                              unsigned n_pad, unsigned W_pad, uint32_t* planes,
-                             hipStream_t stream) {
+                             hipStream_t stream, bool naive) {
   const unsigned wblocks = (W_pad + kPackWords - 1) / kPackWords;
-  pack_chars_kernel<<<dim3((n_pad + kPackTaxa - 1) / kPackTaxa, ygrid(wblocks)), dim3(256), 0, stream>>>(
-      chars, pitch, n, L, n_pad, W_pad, planes);
+  const dim3 grid((n_pad + kPackTaxa - 1) / kPackTaxa, ygrid(wblocks)), block(256);
+  if (naive)
+    pack_chars_kernel<true><<<grid, block, 0, stream>>>(chars, pitch, n, L, n_pad, W_pad, planes);
+  else
+    pack_chars_kernel<false><<<grid, block, 0, stream>>>(chars, pitch, n, L, n_pad, W_pad, planes);
   return hipGetLastError();
 }
 
@@ -634,16 +685,16 @@ hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_tiles(int config, bool jc_only, bool exact, const JobDesc* jobs, unsigned njobs,
+hipError_t launch_tiles(int config, int mode, bool exact, const JobDesc* jobs, unsigned njobs,
                         unsigned total_items, unsigned finish_items, hipStream_t stream) {
   if (total_items == 0) return hipSuccess;
   const unsigned f = finish_items;
   switch (config) {
-    case 0: return launch_cfg<64, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 1: return launch_cfg<64, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 2: return launch_cfg<128, 8, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 3: return launch_cfg<32, 4, 4, 16>(jc_only, exact, jobs, njobs, total_items, f, stream);
-    case 4: return launch_cfg<32, 4, 4, 8>(jc_only, exact, jobs, njobs, total_items, f, stream);
+    case 0: return launch_cfg<64, 4, 4, 16>(mode, exact, jobs, njobs, total_items, f, stream);
+    case 1: return launch_cfg<64, 4, 4, 8>(mode, exact, jobs, njobs, total_items, f, stream);
+    case 2: return launch_cfg<128, 8, 4, 16>(mode, exact, jobs, njobs, total_items, f, stream);
+    case 3: return launch_cfg<32, 4, 4, 16>(mode, exact, jobs, njobs, total_items, f, stream);
+    case 4: return launch_cfg<32, 4, 4, 8>(mode, exact, jobs, njobs, total_items, f, stream);
     default: return hipErrorInvalidValue;
   }
 }

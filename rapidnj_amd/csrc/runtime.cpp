@@ -278,6 +278,7 @@ static int default_config(unsigned n, unsigned W, bool jc_only) {
 // Autotune classes: DNA launches by whether every job is JC (the 6-instruction loop), protein
 // launches by code width.
 static int tune_variant(const LaunchShape& sh) {
+  if (sh.kind == kNaive) return 8;
   return sh.kind == kProtein ? 16 + (int)sh.code_bits : (int)sh.jc_only;
 }
 
@@ -287,7 +288,8 @@ static hipError_t launch_counting(int config, const LaunchShape& sh, bool exact,
   if (sh.kind == kProtein)
     return launch_protein_tiles(config, sh.code_bits, exact, jobs, njobs, items, finish_items,
                                 stream);
-  return launch_tiles(config, sh.jc_only, exact, jobs, njobs, items, finish_items, stream);
+  const int mode = sh.kind == kNaive ? kModeNaive : (sh.jc_only ? kModeJC6 : kModeK2P);
+  return launch_tiles(config, mode, exact, jobs, njobs, items, finish_items, stream);
 }
 
 // The configuration the next launch of this shape class uses (without tuning it).
@@ -493,9 +495,10 @@ static bool protein_gap_byte(unsigned v) {
 int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
                    int kind, hipStream_t s, Alignment** out) {
   Alignment* a = nullptr;
-  if (kind == kDNA) {
-    RNJ_HIP_STATUS(alloc_alignment(ctx, n, L, L, &a));
-    hipError_t e = launch_pack_chars(chars, pitch, n, L, a->n_pad, a->W, a->planes, s);
+  if (kind == kDNA || kind == kNaive) {
+    RNJ_HIP_STATUS(alloc_alignment(ctx, n, L, L, &a, kind, 0));
+    hipError_t e =
+        launch_pack_chars(chars, pitch, n, L, a->n_pad, a->W, a->planes, s, kind == kNaive);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
       rnj_alignment_free(static_cast<rnj_alignment*>(a));
@@ -725,6 +728,14 @@ int rnj_valu_issue_rates(rnj_context* ctx, double* bitwise, double* bcnt) {
   return RNJ_OK;
 }
 
+int rnj_set_char_rules(rnj_context* ctx, int rules) {
+  if (!ctx) return fail(RNJ_E_INVALID, "null context");
+  if (rules != RNJ_CHARS_BITS && rules != RNJ_CHARS_NAIVE)
+    return fail(RNJ_E_INVALID, "rules must be RNJ_CHARS_BITS or RNJ_CHARS_NAIVE");
+  ctx->char_rules = rules;
+  return RNJ_OK;
+}
+
 int rnj_set_input_type(rnj_context* ctx, int type) {
   if (!ctx) return fail(RNJ_E_INVALID, "null context");
   if (type != RNJ_TYPE_AUTO && type != RNJ_TYPE_DNA && type != RNJ_TYPE_PROTEIN)
@@ -768,6 +779,7 @@ int rnj_alignment_from_chars(rnj_context* ctx, const char* chars, size_t pitch, 
     else std::memcpy(first.data(), chars, L);
     kind = alignment_kind(ctx, first.data(), L);
   }
+  if (kind == kDNA && ctx->char_rules == RNJ_CHARS_NAIVE) kind = kNaive;
   const uint8_t* dchars = (const uint8_t*)chars;
   size_t dpitch = pitch;
   void* staging = nullptr;

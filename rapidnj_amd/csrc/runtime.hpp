@@ -105,6 +105,7 @@ struct Context {
   uint8_t* pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   int input_type = RNJ_TYPE_AUTO;  // type of alignments read from characters / FASTA
+  int char_rules = RNJ_CHARS_BITS;  // rules of DNA alignments read from characters
   Stager staging;  // characters and FASTA images from host memory, results to it
 };
 
@@ -124,7 +125,7 @@ struct Alignment {
   uint32_t* gaps = nullptr;    // protein: gap sites per taxon [n_pad], inside the planes block
   std::vector<std::string> names;  // FASTA record names in file order (alignments from FASTA)
   uint8_t* chars_dev = nullptr;      // n x L characters of in-place updates (allocated by the first)
-  unsigned nplanes() const { return kind == kProtein ? 1 + code_bits : 3; }
+  unsigned nplanes() const { return kind == kProtein ? 1 + code_bits : (kind == kNaive ? 4 : 3); }
   size_t bytes() const {
     return ((size_t)nplanes() * W + (kind == kProtein ? 1 : 0)) * n_pad * sizeof(uint32_t);
   }

@@ -49,6 +49,7 @@ SIGNATURES = {
     "rnj_set_tile_config": (ctypes.c_int, [_vp, ctypes.c_int]),
     "rnj_get_tile_config_for": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rnj_set_input_type": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "rnj_set_char_rules": (ctypes.c_int, [_vp, ctypes.c_int]),
     "rnj_get_tile_config": (ctypes.c_int, [_vp, _u, _u, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "rnj_alignment_from_chars": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int, ctypes.c_int,
                                                 ctypes.POINTER(_vp)]),
@@ -317,6 +318,11 @@ class Context:
     def set_input_type(self, type) -> None:
         """"auto" (the reference's discovery on the first sequence), "dna" or "protein"."""
         check(lib().rnj_set_input_type(self.handle, _type(type)))
+
+    def set_char_rules(self, naive: bool) -> None:
+        """DNA alignments read from characters afterwards count as the reference's bit engine
+        (False, the default) or as its naive `-f` counter (True: RNJ_CHARS_NAIVE)."""
+        check(lib().rnj_set_char_rules(self.handle, 1 if naive else 0))
 
     def alignment(self, aln, skip_type_check: bool = False) -> Alignment:
         """Pack an (n, L) uint8 alignment (numpy on the host, or a uint8 torch tensor on the device)."""

@@ -243,6 +243,72 @@ def test_saturation_records_every_config(ctx, N, model):
         ctx.set_tile_config(-1)
 
 
+@pytest.mark.parametrize("path", FULL_CASES, ids=os.path.basename)
+def test_naive_rules_match_reference_counts(ctx, path):
+    """RNJ_CHARS_NAIVE (the reference's `-f` counter): ts, tv and valid equal the naive counts
+    the reference's simpleDistanceCalculator wrote for the golden alignments."""
+    d = _load(path)
+    ctx.set_char_rules(True)
+    try:
+        A = ctx.alignment(d["aln"])
+    finally:
+        ctx.set_char_rules(False)
+    ts, tv, va = ctx.counts(A)
+    np.testing.assert_array_equal(_u32(ts), d["naive_ts"])
+    np.testing.assert_array_equal(_u32(tv), d["naive_tv"])
+    np.testing.assert_array_equal(_u32(va), d["naive_valid"])
+
+
+def _mixed_case_alignment(n, L, seed):
+    """Lowercase letters, U for T and N among uppercase ACGT and gaps: where the naive counter
+    and the bit engine differ."""
+    aln = simulate_alignment(n, L, 0.05, seed).copy()
+    rng = np.random.default_rng(seed)
+    aln[rng.random(aln.shape) < 0.2] |= 0x20          # lowercase ('-' is unchanged)
+    t = (aln | 0x20) == ord("t")
+    aln[t & (rng.random(aln.shape) < 0.3)] += np.uint8(ord("u") - ord("t"))   # T -> U, t -> u
+    aln[rng.random(aln.shape) < 0.02] = ord("N")
+    aln[0, :40] = np.frombuffer(b"ACGTacgtuU" * 4, np.uint8)       # DNA by the first row
+    return aln
+
+
+def test_naive_rules_every_config(ctx, N):
+    """Mixed-case alignment under the naive rules, in every tile configuration: counts equal the
+    oracle's naive counter on every pair, distances equal its corrections in both layouts, and
+    the naive alignment is refused where only bit-rule alignments go."""
+    n, L = 300, 777
+    aln = _mixed_case_alignment(n, L, 41)
+    iu = np.triu_indices(n, 1)
+    ref = np.array([O.count_naive(aln[i].tobytes(), aln[j].tobytes()) for i, j in zip(*iu)],
+                   dtype=np.uint32).reshape(-1, 3)
+    # the two rules differ on this alignment
+    bits = np.array([O.count_chars(aln[i].tobytes(), aln[j].tobytes())
+                     for i, j in zip(iu[0][:2000], iu[1][:2000])], dtype=np.uint32)
+    assert (bits != ref[:2000]).any()
+    ctx.set_char_rules(True)
+    try:
+        A = ctx.alignment(aln)
+    finally:
+        ctx.set_char_rules(False)
+    try:
+        for c in range(len(N.tile_configs())):
+            ctx.set_tile_config(c)
+            ts, tv, va = ctx.counts(A)
+            np.testing.assert_array_equal(_u32(ts), ref[:, 0], err_msg=f"config {c}")
+            np.testing.assert_array_equal(_u32(tv), ref[:, 1], err_msg=f"config {c}")
+            np.testing.assert_array_equal(_u32(va), ref[:, 2], err_msg=f"config {c}")
+            for model in ("jc", "k2p"):
+                _, full, _ = O.correct_triangle(ref[:, 0], ref[:, 1], ref[:, 2], n, model)
+                assert_distances_match(_np(ctx.distances(A, model, layout="square")), full)
+                assert_distances_match(_np(ctx.distances(A, model)), full[iu])
+    finally:
+        ctx.set_tile_config(-1)
+    with pytest.raises(N.RnjError):
+        ctx.bootstrap(A, np.ones((2, L), dtype=np.uint16), "k2p")
+    with pytest.raises(N.RnjError):
+        A.update(aln)
+
+
 def test_bootstrap_matches_reference_on_expanded_columns(ctx):
     d = _load(os.path.join(GOLDEN, "bootstrap.npz"))
     A = ctx.alignment(d["aln"])
