@@ -257,6 +257,50 @@ def valu_peak(rates, classes) -> dict:
             "classes": {"bitwise": b, "bcnt": c}}
 
 
+def naive_side(ctx, aln, wl, out, stream) -> dict:
+    """The workload's alignment under the naive character rules (rnj_set_char_rules: a fourth
+    plane, 6 bitwise + 3 v_bcnt per 32 site-pairs): step time, and the raw launch (reset + tile
+    kernel + job maximum) against the ceiling of that instruction mix."""
+    import torch
+    ctx.set_char_rules(True)
+    try:
+        An = ctx.alignment(aln)
+    finally:
+        ctx.set_char_rules(False)
+    pairs = wl.n * (wl.n - 1) // 2
+    for _ in range(3):
+        ctx.distances(An, wl.model, out=out, stream=stream)
+    torch.cuda.synchronize()
+    k = 10
+    t0 = time.perf_counter()
+    for _ in range(k):
+        ctx.distances(An, wl.model, out=out, stream=stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / k * 1e3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        ctx.distances(An, wl.model, out=out, raw=True, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kern_ms = e0.elapsed_time(e1) / k
+    pk = valu_peak(ctx.valu_issue_rates(), (6, 3))
+    achieved = 9 * pairs * (wl.L / 32.0) / (kern_ms / 1e3) / 1e12
+    cfg = ctx.tile_config_for(An, wl.model)
+    An.close()
+    return {"ms_per_step": round(ms, 4), "value": round(pairs * wl.L / (ms / 1e3) / 1e9, 2),
+            "unit": "G site-pairs/s", "kernel_ms": round(kern_ms, 4),
+            "valu": {"achieved": round(achieved, 3), "peak": round(pk["peak_tops"], 2),
+                     "unit": "TOP/s", "frac": round(achieved / pk["peak_tops"], 4),
+                     "ops_per_32_site_pairs": 9},
+            "tile_config": tile_name(cfg)}
+
+
+def tile_name(cfg: int) -> str:
+    from rapidnj_amd import native as N
+    return N.tile_configs()[cfg]
+
+
 def protein_side(ctx, stream, cpu: bool) -> dict:
     """Kimura-protein distances of the 20000 x 1250 protein workload (resident, float32 upper
     triangle in HBM): throughput, the protein tile kernel's VALU roofline, and the reference's
@@ -705,6 +749,9 @@ def main() -> int:
                                          "unit": "G site-pairs/s"}
         PA.close()
         del p_out
+        if not wl.protein and not wl.replicates:
+            log(f"side run: naive character rules (the reference's -f counter), {wl.name}")
+            extra["naive_rules"] = naive_side(ctx, aln, wl, out, stream)
         log("side run: host hand-off of the 20000 x 5000 matrix (full rows, lower triangle)")
         extra["handoff"] = handoff_compare(ctx, A)
         log("side run: FASTA ingest (device reader) vs the reference reader")
