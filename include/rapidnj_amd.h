@@ -141,16 +141,22 @@ RNJ_EXPORT int rnj_valu_issue_rates(rnj_context* ctx, double* bitwise, double* b
  * (dataloaderFasta.hpp:54-76: DNA unless a byte >= 'A' is not one of ACGTUN of either case, else
  * protein); RNJ_TYPE_DNA / RNJ_TYPE_PROTEIN force the type, like the reference's `-i d|p`. */
 RNJ_EXPORT int rnj_set_input_type(rnj_context* ctx, int type);
-/* Character rules of DNA alignments read from characters afterwards (rnj_alignment_from_chars /
- * _rows). RNJ_CHARS_BITS (default): the counts of the bit-string engine, bitDistanceGap
- * (bitDistanceGap.cpp:60-336), which counts ACGT of either case. RNJ_CHARS_NAIVE: the counts
- * of the naive counter the reference's `-f` option selects (src/main.cpp:589,597;
- * simpleDistanceCalculator::calculateTsTv, simpleDistanceCalculator.cpp:58-98, after the
- * loader's resolveChar, dataloaderFasta.hpp:184-191): a site counts when neither character is a
- * gap (any byte but ACGTU of either case), and a mismatch is a transition or transversion only
- * when the lower-index taxon's character is uppercase A, C, G or T. Such alignments are packed
- * with a fourth bit plane and accepted by the distance, count, hand-off and batch-plan calls;
- * bootstrap, served jobs and character updates take bit-rule alignments only. */
+/* Character rules of alignments read from characters, rows or FASTA afterwards
+ * (rnj_alignment_from_chars / _rows / _fasta / _fasta_file). RNJ_CHARS_BITS (default): the counts
+ * of the bit-string engines, bitDistanceGap for DNA (bitDistanceGap.cpp:60-336), which counts
+ * ACGT of either case, and bitDistanceProtein for protein (bitDistanceProtein.cpp:31-61).
+ * RNJ_CHARS_NAIVE: the counts of the estimator the reference's `-f` option (fastdist off)
+ * selects, simpleDistanceCalculator (src/main.cpp:589,597; KimuraDistance.cpp:195-208), after the
+ * loader's resolveChar (dataloaderFasta.hpp:184-199):
+ *  - DNA, calculateTsTv (simpleDistanceCalculator.cpp:58-98): a site counts when neither
+ *    character is a gap (any byte but ACGTU of either case), and a mismatch is a transition or
+ *    transversion only when the lower-index taxon's character is uppercase A, C, G or T. Such
+ *    alignments are packed with a fourth bit plane;
+ *  - protein, calculateHammingDistance (simpleDistanceCalculator.cpp:40-55): the valid count is
+ *    the exact number of residue pairs (no 128-position shortfall when L is a multiple of 128).
+ * Naive-rule alignments are accepted by the distance, count, hand-off and batch-plan calls;
+ * bootstrap, served jobs and character updates take bit-rule alignments only (served jobs fail
+ * with RNJ_E_INVALID while the server's context has naive rules set). */
 enum { RNJ_CHARS_BITS = 0, RNJ_CHARS_NAIVE = 1 };
 RNJ_EXPORT int rnj_set_char_rules(rnj_context* ctx, int rules);
 
@@ -223,7 +229,8 @@ RNJ_EXPORT int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, 
 /* Exact ts / tv / valid of every pair into device arrays of n(n-1)/2 u32, upper triangle
  * row-major (DistanceEstimate::computeDistance, DistanceEstimate.hpp:14). Protein: ts holds the
  * mismatches (data[0]), tv 0 (data[1]) and valid the low 32 bits of the 64-bit data[2], which
- * is 128 short of the residue pairs when L is a multiple of 128 (bitDistanceProtein.cpp:31-61). */
+ * is 128 short of the residue pairs when L is a multiple of 128 (bitDistanceProtein.cpp:31-61)
+ * unless the alignment was read under the naive character rules (then the residue pairs). */
 RNJ_EXPORT int rnj_counts_device(rnj_context* ctx, const rnj_alignment* aln, uint32_t* ts, uint32_t* tv,
                       uint32_t* valid, void* stream);
 

@@ -16,7 +16,10 @@
 //
 // Protein loaders (type == PROTEIN) get the reference's protein distances: Kimura protein with
 // the Dayhoff table (KimuraDistance.cpp:322-338) and JC with alpha = 20 (JCdistance.cpp:155-158),
-// counted as bitDistanceProtein counts (bitDistanceProtein.cpp:31-61).
+// counted as bitDistanceProtein counts (bitDistanceProtein.cpp:31-61) with fastdist, and as
+// simpleDistanceCalculator::calculateHammingDistance counts (simpleDistanceCalculator.cpp:40-55)
+// without it, the estimator KimuraDistance::getDistanceEstimateInstance picks
+// (KimuraDistance.cpp:195-208).
 //
 // Behaviour kept from the reference:
 //  * without a disk matrix the result is n rows allocated with new distType[n] (ownership passes
@@ -76,9 +79,9 @@ int loader_type(Loader*, long) {
 // Packs the loader's alignment into HBM. DNA with fastdist: the loader's bit strings, which are
 // exactly what the reference's bitDistanceGap counts. Protein with fastdist: the same bit
 // strings hold the resolved characters four to a word (dataloaderFasta.hpp
-// encodeProteinSequence), so their first L bytes are the row. Otherwise the character rows;
-// DNA rows then count under the naive counter's rules (RNJ_CHARS_NAIVE), as `-f` does in the
-// reference (simpleDistanceCalculator).
+// encodeProteinSequence), so their first L bytes are the row. Otherwise the character rows,
+// packed under the naive counter's rules (RNJ_CHARS_NAIVE), as `-f` does in the reference
+// (simpleDistanceCalculator): calculateTsTv for DNA, calculateHammingDistance for protein.
 // The shared context's input type is set for the upload and reset after it (the reference's
 // callers are single-threaded).
 template <class Loader>

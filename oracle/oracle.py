@@ -59,9 +59,10 @@ def lib() -> ctypes.CDLL:
         L.rnj_oracle_distance_matrix.argtypes = [_u32p, _u32p, ctypes.c_uint, ctypes.c_uint,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
-        L.rnj_oracle_count_protein.restype = None
-        L.rnj_oracle_count_protein.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint,
-                                               _u64_3]
+        for name in ("rnj_oracle_count_protein", "rnj_oracle_count_protein_naive"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint, _u64_3]
         for f in (L.rnj_oracle_kimura_protein, L.rnj_oracle_jc_protein):
             f.restype = ctypes.c_float
             f.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
@@ -175,10 +176,11 @@ def distance_matrix(aln: np.ndarray, model: str, threads: int = 8):
     return tri[:m], full, mx.value
 
 
-def correct_triangle(ts, tv, valid, n: int, model: str):
+def correct_triangle(ts, tv, valid, n: int, model: str, with_full: bool = True):
+    """-> (triangle before post-processing, full post-processed n x n matrix or None, max)."""
     m = n * (n - 1) // 2
     tri = np.zeros(max(m, 1), dtype=np.float32)
-    full = np.zeros((n, n), dtype=np.float32)
+    full = np.zeros((n, n), dtype=np.float32) if with_full else None
     mx = ctypes.c_float()
     ts = np.ascontiguousarray(ts, dtype=np.uint32)
     tv = np.ascontiguousarray(tv, dtype=np.uint32)
@@ -186,7 +188,8 @@ def correct_triangle(ts, tv, valid, n: int, model: str):
     if m == 0:
         ts = tv = valid = np.zeros(1, dtype=np.uint32)
     lib().rnj_oracle_correct_triangle(ts, tv, valid, n, model_id(model), tri.ctypes.data,
-                                      full.ctypes.data, ctypes.byref(mx))
+                                      full.ctypes.data if full is not None else None,
+                                      ctypes.byref(mx))
     return tri[:m], full, mx.value
 
 
@@ -209,6 +212,13 @@ def count_protein(a: bytes, b: bytes) -> tuple[int, int, int]:
     return int(out[0]), int(out[1]), int(out[2])
 
 
+def count_protein_naive(a: bytes, b: bytes) -> tuple[int, int, int]:
+    """The -f path's Hamming counter (simpleDistanceCalculator::calculateHammingDistance)."""
+    out = _u64_3()
+    lib().rnj_oracle_count_protein_naive(a, b, len(a), out)
+    return int(out[0]), int(out[1]), int(out[2])
+
+
 def kimura_protein(dist: int, valid: int) -> float:
     return lib().rnj_oracle_kimura_protein(dist, valid)
 
@@ -217,27 +227,30 @@ def jc_protein(dist: int, valid: int) -> float:
     return lib().rnj_oracle_jc_protein(dist, valid)
 
 
-def protein_counts_triangle(aln: np.ndarray):
-    """(dist, valid) upper triangles (uint64) of an (n, L) protein character alignment."""
+def protein_counts_triangle(aln: np.ndarray, naive: bool = False):
+    """(dist, valid) upper triangles (uint64) of an (n, L) protein character alignment, counted
+    by bitDistanceProtein or (naive) by the -f path's Hamming counter."""
     aln = np.ascontiguousarray(aln, dtype=np.uint8)
     n = aln.shape[0]
     rows = [aln[i].tobytes() for i in range(n)]
+    count = count_protein_naive if naive else count_protein
     dist, valid = [], []
     for i in range(n):
         for j in range(i + 1, n):
-            d, _, v = count_protein(rows[i], rows[j])
+            d, _, v = count(rows[i], rows[j])
             dist.append(d)
             valid.append(v)
     return np.array(dist, dtype=np.uint64), np.array(valid, dtype=np.uint64)
 
 
-def protein_matrix(aln: np.ndarray, model: str):
-    """(full post-processed n x n matrix, max) for an (n, L) protein character alignment."""
+def protein_matrix(aln: np.ndarray, model: str, naive: bool = False):
+    """(full post-processed n x n matrix, max) for an (n, L) protein character alignment
+    (naive: counted by the -f path's Hamming counter)."""
     aln = np.ascontiguousarray(aln, dtype=np.uint8)
     n, L = aln.shape
     full = np.zeros((n, n), dtype=np.float32)
     mx = ctypes.c_float()
-    rc = lib().rnj_oracle_protein_matrix(aln.tobytes(), n, L, model_id(model), 1,
+    rc = lib().rnj_oracle_protein_matrix(aln.tobytes(), n, L, model_id(model), int(naive),
                                          full.ctypes.data, ctypes.byref(mx))
     if rc != 0:
         raise RuntimeError("oracle protein matrix failed")

@@ -19,11 +19,17 @@
 //                                     every '>' record in file order); then bit strings and gap
 //                                     filters as in `pack` (the reference exits 1 on bad input)
 //   load   FASTA REPS                 prints {"best_s":..} of parse + pack (readData, fastdist)
-//   pcounts FASTA OUT                 protein (bitDistanceProtein): u32 n, L; then for i<j
+//   pcounts FASTA OUT [naive]         protein (bitDistanceProtein, or with naive the -f path's
+//                                     simpleDistanceCalculator): u32 n, L; then for i<j
 //                                     u64 data[0], data[1], data[2] as the estimator returns them
-//   pmatrix FASTA jc|kim OUT          protein distances: u32 n; then n*n f32 (post-processed)
+//   pmatrix FASTA jc|kim OUT [naive]  protein distances: u32 n; then n*n f32 (post-processed)
 //   counts FASTA OUT [naive]          u32 n, L; then for i<j (row-major) u32 ts, tv, valid
 //   matrix FASTA jc|kim T OUT [naive] u32 n; then n*n f32 (post-processed matrix); prints seconds
+//   dmatrix FASTA jc|kim T DIR OUT [naive]
+//                                     the disk-matrix path (KimuraDistance / JCdistance with a
+//                                     diskMatrix in DIR, KimuraDistance.cpp:75-99,256-281): every
+//                                     row computed with j from 0 and written with writeArray, then
+//                                     read back with readArray; u32 n; then n*n f32
 //   time   FASTA jc|kim T REPS        prints {"best_s":..,"mean_s":..,"n":..,"L":..,"threads":..}
 //   ptime  FASTA jc|kim T REPS        the same for a protein alignment
 //   nj     MATRIX OUT [neg]           MATRIX = u32 n + n*n f32; writes Newick (rapidNJ, negative_branches with neg) with
@@ -41,6 +47,7 @@
 #include "rapidNJ.h"
 #include "polytree.h"
 #include "ProgressBar.hpp"
+#include "diskMatrix.h"
 
 #include <chrono>
 #include <cstdint>
@@ -146,35 +153,39 @@ int mode_load(int argc, char** argv) {
   return 0;
 }
 
-dataloaderFasta* load_protein(const std::string& path) {
+dataloaderFasta* load_protein(const std::string& path, bool fastdist = true) {
   dataloaderFasta* dl = new dataloaderFasta();
-  dl->readData(path, true, false, UNKNOWN, false);
+  dl->readData(path, fastdist, false, UNKNOWN, false);
   if (dl->type != PROTEIN) die("input is not detected as protein");
   return dl;
 }
 
 int mode_pcounts(int argc, char** argv) {
-  if (argc < 4) die("pcounts FASTA OUT");
-  dataloaderFasta* dl = load_protein(argv[2]);
-  bitDistanceProtein est(dl);
+  if (argc < 4) die("pcounts FASTA OUT [naive]");
+  bool naive = argc > 4 && std::string(argv[4]) == "naive";
+  dataloaderFasta* dl = load_protein(argv[2], !naive);
+  DistanceEstimate* est = naive ? static_cast<DistanceEstimate*>(new simpleDistanceCalculator(dl))
+                                : static_cast<DistanceEstimate*>(new bitDistanceProtein(dl));
   uint32_t n = dl->getSequenceCount(), L = dl->getSequenceLength();
   std::ofstream o = open_out(argv[3]);
   put_u32(o, n); put_u32(o, L);
   unsigned long long d[3];
   for (uint32_t i = 0; i < n; i++)
     for (uint32_t j = i + 1; j < n; j++) {
-      est.computeDistance(i, j, d);
+      est->computeDistance(i, j, d);
       o.write(reinterpret_cast<const char*>(d), sizeof d);
     }
+  delete est;
   delete dl;
   return 0;
 }
 
 int mode_pmatrix(int argc, char** argv) {
-  if (argc < 5) die("pmatrix FASTA jc|kim OUT");
-  dataloaderFasta* dl = load_protein(argv[2]);
+  if (argc < 5) die("pmatrix FASTA jc|kim OUT [naive]");
+  bool naive = argc > 5 && std::string(argv[5]) == "naive";
+  dataloaderFasta* dl = load_protein(argv[2], !naive);
   int n = dl->getSequenceCount();
-  distType** m = compute_matrix(dl, argv[3], 4, true, NULL);
+  distType** m = compute_matrix(dl, argv[3], 4, !naive, NULL);
   std::ofstream o = open_out(argv[4]);
   put_u32(o, (uint32_t)n);
   for (int i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(m[i]), sizeof(distType) * n);
@@ -215,6 +226,37 @@ int mode_matrix(int argc, char** argv) {
   for (int i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(m[i]), sizeof(distType) * n);
   std::cout << "seconds=" << secs << std::endl;
   free_matrix(m, n);
+  delete dl;
+  return 0;
+}
+
+int mode_dmatrix(int argc, char** argv) {
+  if (argc < 7) die("dmatrix FASTA jc|kim T DIR OUT [naive]");
+  bool naive = argc > 7 && std::string(argv[7]) == "naive";
+  dataloaderFasta* dl = load(argv[2], !naive);
+  int n = dl->getSequenceCount(), threads = atoi(argv[4]);
+  diskMatrix* dm = new diskMatrix(argv[5], n);
+  std::string model = argv[3];
+  if (model == "kim") {
+    KimuraDistance* alg = new KimuraDistance(false, !naive, dl, dm);
+    alg->computeDistances(threads);
+    delete alg;
+  } else if (model == "jc") {
+    JCdistance* alg = new JCdistance(false, !naive, dl, dm);
+    alg->computeDistanceMatrix(threads);
+    delete alg;
+  } else {
+    die("model must be jc or kim");
+  }
+  dm->flush();
+  std::ofstream o = open_out(argv[6]);
+  put_u32(o, (uint32_t)n);
+  std::vector<distType> row(n);
+  for (int i = 0; i < n; i++) {
+    dm->readArray(row.data(), i, n);
+    o.write(reinterpret_cast<const char*>(row.data()), sizeof(distType) * n);
+  }
+  delete dm;
   delete dl;
   return 0;
 }
@@ -285,7 +327,7 @@ int mode_tree(int argc, char** argv) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2) die("usage: ref_harness pack|fasta|load|pcounts|pmatrix|counts|matrix|time|ptime|nj|tree ...");
+  if (argc < 2) die("usage: ref_harness pack|fasta|load|pcounts|pmatrix|counts|matrix|dmatrix|time|ptime|nj|tree ...");
   std::string mode = argv[1];
   if (mode == "pack") return mode_pack(argc, argv);
   if (mode == "fasta") return mode_fasta(argc, argv);
@@ -294,6 +336,7 @@ int main(int argc, char** argv) {
   if (mode == "pmatrix") return mode_pmatrix(argc, argv);
   if (mode == "counts") return mode_counts(argc, argv);
   if (mode == "matrix") return mode_matrix(argc, argv);
+  if (mode == "dmatrix") return mode_dmatrix(argc, argv);
   if (mode == "time" || mode == "ptime") return mode_time(argc, argv);
   if (mode == "nj") return mode_nj(argc, argv);
   if (mode == "tree") return mode_tree(argc, argv);

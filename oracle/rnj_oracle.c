@@ -445,6 +445,24 @@ void rnj_oracle_count_protein(const char* a, const char* b, unsigned L, uint64_t
   out[2] = padded - gaps;
 }
 
+/* simpleDistanceCalculator::calculateHammingDistance (simpleDistanceCalculator.cpp:40-55) on the
+ * loader's resolved characters (storeSequence with fastdist off, dataloaderFasta.hpp:268-271):
+ * a position counts when neither resolved character is '.' or '-', and is a difference when they
+ * differ; no padding is walked. */
+void rnj_oracle_count_protein_naive(const char* a, const char* b, unsigned L, uint64_t out[3]) {
+  uint64_t dist = 0, len = 0;
+  for (unsigned k = 0; k < L; k++) {
+    const char x = rnj_oracle_protein_resolve(a[k]), y = rnj_oracle_protein_resolve(b[k]);
+    if (x != '.' && x != '-' && y != '.' && y != '-') {
+      if (x != y) dist++;
+      len++;
+    }
+  }
+  out[0] = dist;
+  out[1] = 0;
+  out[2] = len;
+}
+
 /* the Dayhoff PAM distances (x100) for p = 0.750 .. 0.930, KimuraDistance.cpp:30-49 */
 static const int kDayhoffPams[] = {
   195, 196, 197, 198, 199, 200, 200, 201, 202, 203, 204, 205, 206, 207, 208, 209, 209, 210,
@@ -486,9 +504,8 @@ float rnj_oracle_jc_protein(uint64_t dist, uint64_t valid) {
   return distance;
 }
 
-int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model, int threads,
+int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model, int naive,
                               float* full, float* max_out) {
-  (void)threads;
   const size_t npairs = (size_t)n * (n - 1) / 2;
   float* t = (float*)malloc((npairs ? npairs : 1) * sizeof(float));
   if (!t) return -1;
@@ -497,7 +514,8 @@ int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model
   for (unsigned i = 0; i < n; i++)
     for (unsigned j = i + 1; j < n; j++, k++) {
       uint64_t c[3];
-      rnj_oracle_count_protein(aln + (size_t)i * L, aln + (size_t)j * L, L, c);
+      if (naive) rnj_oracle_count_protein_naive(aln + (size_t)i * L, aln + (size_t)j * L, L, c);
+      else rnj_oracle_count_protein(aln + (size_t)i * L, aln + (size_t)j * L, L, c);
       const float d = model == RNJ_ORACLE_K2P ? rnj_oracle_kimura_protein(c[0], c[2])
                                               : rnj_oracle_jc_protein(c[0], c[2]);
       if (d > maxd) maxd = d;

@@ -47,13 +47,18 @@ int rnj_oracle_parse_fasta(const unsigned char* data, size_t len, unsigned* n, u
  * the estimator walks (padding counts as gap), in unsigned 64-bit arithmetic. */
 char rnj_oracle_protein_resolve(char c);
 void rnj_oracle_count_protein(const char* a, const char* b, unsigned L, uint64_t out[3]);
+/* The -f path's protein counter, simpleDistanceCalculator::calculateHammingDistance
+ * (simpleDistanceCalculator.cpp:40-55), on the resolved characters: out[0] = positions where
+ * both are residues and differ, out[1] = 0, out[2] = positions where both are residues. */
+void rnj_oracle_count_protein_naive(const char* a, const char* b, unsigned L, uint64_t out[3]);
 /* Kimura protein with the Dayhoff PAM table (KimuraDistance.cpp:30-49,322-338) and JC with
  * alpha = 20 (JCdistance.cpp:155-183) on those counts. */
 float rnj_oracle_kimura_protein(uint64_t dist, uint64_t valid);
 float rnj_oracle_jc_protein(uint64_t dist, uint64_t valid);
 /* Post-processed n x n protein matrix (model RNJ_ORACLE_K2P = Kimura protein, RNJ_ORACLE_JC) of
- * an (n, L) character alignment (raw characters; resolved here). */
-int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model, int threads,
+ * an (n, L) character alignment (raw characters; resolved here), counted by bitDistanceProtein
+ * (naive = 0) or by the -f path's Hamming counter (naive = 1). */
+int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model, int naive,
                               float* full, float* max_out);
 
 /* rapidNJ's neighbour joining (rapidNJ.cpp, restated in rnj_oracle_nj.c) on an n x n

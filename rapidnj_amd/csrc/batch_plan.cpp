@@ -67,9 +67,7 @@ int rnj_batch_plan_create(rnj_context* ctx, const rnj_alignment* const* alns, co
     specs.push_back(alignment_spec(a, models[k], layout, outs[k], ld));
   }
   LaunchShape shape = shape_of(specs);
-  if (shape.mixed)
-    return fail(RNJ_E_INVALID, "one launch cannot mix DNA and protein alignments (or protein "
-                               "alignments of different code widths)");
+  if (shape.mixed) return fail(RNJ_E_INVALID, mixed_message(shape));
   RNJ_HIP(hipSetDevice(ctx->device));
   auto* p = new rnj_batch_plan();
   p->ctx = ctx;

@@ -253,6 +253,23 @@ __host__ __device__ __forceinline__ void patch_coords(unsigned t, unsigned nb, u
   }
 }
 
+// Block coordinates of tile `tile` of job J: tiles 0 .. tri-1 cover the upper triangle of the
+// block grid (bi <= bj) in patch order; a kRows job has tri more tiles, the same blocks
+// transposed (bi >= bj), whose pairs (i, j) have i > j and are counted with taxon i as the row's
+// taxon. Returns whether the tile is such a transposed one.
+__device__ __forceinline__ bool job_tile(const JobDesc& J, unsigned tile, unsigned& bi,
+                                         unsigned& bj) {
+  const unsigned long long tri = (unsigned long long)J.nblk * (J.nblk + 1) / 2;
+  const bool lower = tile >= tri;  // only kRows jobs number tiles past tri
+  patch_coords(lower ? (unsigned)(tile - tri) : tile, J.nblk, bi, bj);
+  if (lower) {
+    const unsigned t = bi;
+    bi = bj;
+    bj = t;
+  }
+  return lower;
+}
+
 // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
 // items so that tiles sharing a block row also share that XCD's L2.
 __device__ __forceinline__ unsigned xcd_item(unsigned bid, unsigned grid) {
@@ -303,7 +320,18 @@ __device__ __forceinline__ void finish_counts(const JobDesc& J, unsigned i, unsi
 
 // Zero diagonal of a square output (the reference's post-process writes 0 on i == i).
 __device__ __forceinline__ void finish_diagonal(const JobDesc& J, unsigned i) {
-  if (J.model != kCounts && J.layout == kSquare) J.out[(size_t)i * J.ld + i] = 0.0f;
+  if (J.model != kCounts && J.layout != kTriangle) J.out[(size_t)i * J.ld + i] = 0.0f;
+}
+
+// The distance of pair (i, j) into its output: the upper triangle, both halves of a square, or
+// (kRows) entry (i, j) only.
+__device__ __forceinline__ void store_pair(const JobDesc& J, unsigned i, unsigned j, float d) {
+  if (J.layout == kTriangle) {
+    J.out[tri_index(i, j, J.n)] = d;
+  } else {
+    J.out[(size_t)i * J.ld + j] = d;
+    if (J.layout == kSquare) J.out[(size_t)j * J.ld + i] = d;
+  }
 }
 
 }  // namespace

@@ -34,7 +34,12 @@ enum Model : int { kJC = 0, kK2P = 1, kCounts = 2 };
 enum Kind : int { kDNA = 0, kProtein = 1, kNaive = 2 };
 // Count loops of the DNA tile kernel: K2P (and counts), JC only, naive character rules.
 enum CountMode : int { kModeK2P = 0, kModeJC6 = 1, kModeNaive = 2 };
-enum Layout : int { kTriangle = 0, kSquare = 1 };
+// kRows (internal to the row stream): square rows in which every entry (i, j), i != j, is counted
+// with taxon i as the row's taxon, as the reference's disk-matrix path computes them
+// (KimuraDistance.cpp:300-320 with offset 0). A kRows job covers the upper triangle of tile blocks
+// and then the transposed one (see job_tile); only naive-rule alignments, whose counts are not
+// symmetric, need it.
+enum Layout : int { kTriangle = 0, kSquare = 1, kRows = 2 };
 
 // One distance matrix inside a launch. A launch covers jobs[0..njobs); the work items of job k
 // (tiles of TI x TJ taxon pairs over the upper triangle of tile blocks, each cut into `splits`
@@ -66,6 +71,7 @@ struct JobDesc {
   uint32_t* sat_mask;         // [tiles * waves per work-group][64]: micro-tile masks, flagged waves
   const uint32_t* gaps;       // protein: gap sites per taxon [n_pad]
   unsigned L;                 // protein: alignment length (sites)
+  unsigned valid_bias;        // protein: subtracted from the residue pairs (128 or 0)
 };
 
 // Upper-triangle position of pair (i, j), i < j: row i holds j = i+1 .. n-1 (row-major).

@@ -24,10 +24,12 @@ namespace {
 // ---- epilogue shared by the tile kernel and the split-finish kernel ------------------------
 // Thread (ty, tx) of tile (bi, bj) holds the counts of its MI x MJ pairs: a = tv (mismatches for
 // the JC-only loop), b = ts, v = valid. Writes distances (or counts), records saturated pairs and
-// folds the wave's maximum into the job's running maximum.
+// folds the wave's maximum into the job's running maximum. A tile's pairs are those with i < j,
+// or for a transposed tile of a kRows job (lower) those with i > j.
 template <int TILE, int MI, int MJ, bool JC6, bool kExact>
 __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, unsigned bi,
-                                              unsigned bj, unsigned ty, unsigned tx, unsigned tid,
+                                              unsigned bj, bool lower, unsigned ty, unsigned tx,
+                                              unsigned tid,
                                               const uint32_t (&acc_a)[MI][MJ],
                                               const uint32_t (&acc_b)[MI][MJ],
                                               const uint32_t (&acc_v)[MI][MJ]) {
@@ -66,16 +68,11 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, u
 #pragma unroll
     for (int b = 0; b < MJ; ++b) {
       const unsigned j = bj * TILE + tx * MJ + b;
-      if (j >= n) continue;
-      if (i < j) {
+      if (j >= n || i >= n) continue;
+      if (lower ? i > j : i < j) {
         if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
         else lmax = fmaxf(lmax, d[b]);
-        if (J.layout == kSquare) {
-          J.out[(size_t)i * J.ld + j] = d[b];
-          J.out[(size_t)j * J.ld + i] = d[b];
-        } else {
-          J.out[tri_index(i, j, n)] = d[b];
-        }
+        store_pair(J, i, j, d[b]);
       } else if (i == j) {
         finish_diagonal(J, i);
       }
@@ -178,7 +175,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
   const unsigned local = item - J.item_base;
   const unsigned tile = local / J.splits, split = local % J.splits;
   unsigned bi, bj;
-  patch_coords(tile, J.nblk, bi, bj);
+  const bool lower = job_tile(J, tile, bi, bj);
 
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
@@ -233,8 +230,8 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
       }
     return;
   }
-  tile_epilogue<TILE, MI, MJ, MODE == kModeJC6, kExact>(J, tile, bi, bj, ty, tx, tid, acc_a, acc_b,
-                                                        acc_v);
+  tile_epilogue<TILE, MI, MJ, MODE == kModeJC6, kExact>(J, tile, bi, bj, lower, ty, tx, tid, acc_a,
+                                                        acc_b, acc_v);
 }
 
 // Sums the partial counts of the splits of one tile of a split job and finishes the tile. Work
@@ -254,7 +251,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
   const unsigned tile = item - J.fin_base;
   if (J.splits <= 1 || tile >= J.tiles) return;
   unsigned bi, bj;
-  patch_coords(tile, J.nblk, bi, bj);
+  const bool lower = job_tile(J, tile, bi, bj);
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
   uint32_t acc_a[MI][MJ], acc_b[MI][MJ], acc_v[MI][MJ];
@@ -274,7 +271,8 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
         acc_v[a][b] += slot[2 * plane + a * MJ + b];
       }
   }
-  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, tile, bi, bj, ty, tx, tid, acc_a, acc_b, acc_v);
+  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, tile, bi, bj, lower, ty, tx, tid, acc_a, acc_b,
+                                           acc_v);
 }
 
 template <int TILE, int MI, int MJ, int CW>
@@ -491,13 +489,7 @@ __global__ __launch_bounds__(256) void patch_kernel(const JobDesc* __restrict__ 
     while (m) {
       const unsigned bit = (unsigned)(__ffs(m) - 1);
       m &= m - 1;
-      const unsigned i = i0 + bit / mj, j = j0 + bit % mj;
-      if (J.layout == kSquare) {
-        J.out[(size_t)i * J.ld + j] = val;
-        J.out[(size_t)j * J.ld + i] = val;
-      } else {
-        J.out[tri_index(i, j, J.n)] = val;
-      }
+      store_pair(J, i0 + bit / mj, j0 + bit % mj, val);
     }
   }
 }

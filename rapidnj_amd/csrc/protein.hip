@@ -118,7 +118,9 @@ __device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long l
 // Thread (ty, tx) of tile (bi, bj) holds x (differing sites) and g (gap-gap sites) of its
 // MI x MJ pairs. The reference walks 128 * (L/128 + 1) positions and counts the padding past L
 // as gaps, then subtracts from L rounded up to 128 (bitDistanceProtein.cpp:31-61): its valid
-// count is the residue pairs, less 128 when L is a multiple of 128 (modulo 2^64).
+// count is the residue pairs, less 128 when L is a multiple of 128 (modulo 2^64). J.valid_bias
+// holds that 128 (or 0; always 0 for the -f path's Hamming counter, which counts the residue
+// pairs, simpleDistanceCalculator.cpp:40-55).
 template <int TILE, int MI, int MJ, bool kExact>
 __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile, unsigned bi,
                                                  unsigned bj, unsigned ty, unsigned tx,
@@ -126,7 +128,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile
                                                  const uint32_t (&acc_x)[MI][MJ],
                                                  const uint32_t (&acc_g)[MI][MJ]) {
   const unsigned n = J.n;
-  const unsigned long long L = J.L, bias = (J.L % 128u == 0) ? 128ull : 0ull;
+  const unsigned long long L = J.L, bias = J.valid_bias;
   uint32_t gi[MI], gj[MJ];
 #pragma unroll
   for (int a = 0; a < MI; ++a) gi[a] = J.gaps[bi * TILE + ty * MI + a];
@@ -178,12 +180,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile
       if (i < j) {
         if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
         else lmax = fmaxf(lmax, d[b]);
-        if (J.layout == kSquare) {
-          J.out[(size_t)i * J.ld + j] = d[b];
-          J.out[(size_t)j * J.ld + i] = d[b];
-        } else {
-          J.out[tri_index(i, j, n)] = d[b];
-        }
+        store_pair(J, i, j, d[b]);  // protein jobs are never kRows (their counts are symmetric)
       } else if (i == j) {
         finish_diagonal(J, i);
       }

@@ -190,7 +190,8 @@ LaunchPlan plan_launch(const std::vector<JobSpec>& specs, int config, unsigned c
   unsigned long long total_tiles = 0;
   for (const JobSpec& s : specs) {
     const unsigned nb = (s.n + c.tile - 1) / c.tile;
-    P.tiles.push_back(nb * (nb + 1) / 2);
+    // a kRows job also covers the transposed block triangle (job_tile)
+    P.tiles.push_back(nb * (nb + 1) / 2 * (s.layout == kRows ? 2u : 1u));
     total_tiles += P.tiles.back();
   }
   const unsigned long long target = 8ull * std::max(1u, cus);
@@ -250,6 +251,7 @@ void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, W
     d.out_valid = s.out_valid;
     d.gaps = s.gaps;
     d.L = s.L;
+    d.valid_bias = s.valid_bias;
     d.maxbits = ws->d_stats + k;
     d.sat_word = ws->d_sat + P.sat_off[k];
     d.sat_tile = ws->d_sat + ws->cap_units + P.sat_off[k];
@@ -380,13 +382,29 @@ LaunchShape shape_of(const std::vector<JobSpec>& specs) {
   }
   for (size_t k = 0; k < specs.size(); ++k) {
     const JobSpec& s = specs[k];
-    if (s.kind != sh.kind || s.code_bits != sh.code_bits) sh.mixed = true;
+    if (!sh.mixed && (s.kind != sh.kind || s.code_bits != sh.code_bits)) {
+      sh.mixed = true;
+      sh.mixed_kind = s.kind;
+      sh.mixed_bits = s.code_bits;
+    }
     if (s.model != kJC) sh.jc_only = false;
     sh.n_key = std::max(sh.n_key, s.n);
     sh.W_key = std::max(sh.W_key, s.W);
     sh.work += (unsigned long long)s.n * s.n * s.W;
   }
   return sh;
+}
+
+std::string mixed_message(const LaunchShape& sh) {
+  auto name = [](int kind, unsigned bits) -> std::string {
+    if (kind == kNaive) return "DNA under the naive character rules";
+    if (kind == kDNA) return "DNA under the bit-string rules";
+    return "protein with " + std::to_string(bits) + " code bits";
+  };
+  return "one launch cannot mix " + name(sh.kind, sh.code_bits) + " and " +
+         name(sh.mixed_kind, sh.mixed_bits) +
+         " (alignments of one launch share the count loop: one character-rule set for DNA, one "
+         "code width for protein)";
 }
 
 int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Workspace* ws,
@@ -415,7 +433,7 @@ int prepare_workspace(Workspace* ws, unsigned njobs) { return grow_workspace(ws,
 
 int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, int config,
                    Workspace* ws, bool raw, bool exact, hipStream_t stream, bool upload_table) {
-  if (sh.mixed) return fail(RNJ_E_INVALID, "one launch cannot mix DNA and protein jobs");
+  if (sh.mixed) return fail(RNJ_E_INVALID, mixed_message(sh));
   if (specs.size() > kMaxLaunchJobs) return fail(RNJ_E_INVALID, "more than 65535 jobs in one launch");
   const LaunchPlan P = plan_launch(specs, config, (unsigned)ctx->prop.multiProcessorCount);
   if (P.items >= (1ull << 31)) return fail(RNJ_E_INVALID, "launch too large: more than 2^31 pair tiles");
@@ -475,9 +493,11 @@ bool first_row_is_dna(const char* row, unsigned L) {
 }
 
 int alignment_kind(const Context* ctx, const char* row0, unsigned L) {
-  if (ctx->input_type == RNJ_TYPE_DNA) return kDNA;
-  if (ctx->input_type == RNJ_TYPE_PROTEIN) return kProtein;
-  return first_row_is_dna(row0, L) ? kDNA : kProtein;
+  int kind;
+  if (ctx->input_type == RNJ_TYPE_DNA) kind = kDNA;
+  else if (ctx->input_type == RNJ_TYPE_PROTEIN) kind = kProtein;
+  else kind = first_row_is_dna(row0, L) ? kDNA : kProtein;
+  return (kind == kDNA && ctx->char_rules == RNJ_CHARS_NAIVE) ? kNaive : kind;
 }
 
 // The characters dataloader::resolveChar maps to the gap for protein alignments (the ones
@@ -525,6 +545,7 @@ int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n,
   const unsigned B = S < 32 ? 5 : (S < 64 ? 6 : 8);
   int rc = RNJ_OK;
   if (e == hipSuccess) rc = alloc_alignment(ctx, n, L, L, &a, kProtein, B);
+  if (e == hipSuccess && rc == RNJ_OK) a->hamming = ctx->char_rules == RNJ_CHARS_NAIVE;
   uint8_t* dcodes = reinterpret_cast<uint8_t*>(scratch + 8);
   if (e == hipSuccess && rc == RNJ_OK)
     e = hipMemcpyAsync(dcodes, codes, sizeof codes, hipMemcpyHostToDevice, s);
@@ -584,6 +605,10 @@ JobSpec rnj::alignment_spec(const Alignment* a, int model, int layout, float* ou
   s.code_bits = a->code_bits;
   s.gaps = a->gaps;
   s.L = a->L;
+  // bitDistanceProtein walks 128 * (L/128 + 1) positions and subtracts from L rounded up to 128,
+  // so its valid count is 128 short when L is a multiple of 128 (bitDistanceProtein.cpp:31-61);
+  // the Hamming counter of the -f path counts the residue pairs exactly
+  s.valid_bias = (a->kind == kProtein && !a->hamming && a->L % 128u == 0) ? 128u : 0u;
   s.planes = a->planes;
   s.n = a->n;
   s.n_pad = a->n_pad;
@@ -1096,9 +1121,12 @@ struct RowShape {
 };
 }  // namespace
 
+// row_side: every entry (i, j) counted with taxon i as the row's taxon and the maximum taken over
+// the whole square (the disk-matrix path; differs from the symmetric matrix only for naive-rule
+// alignments).
 template <class Deliver>
 static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, float* max_out,
-                        bool lower, unsigned threads, Deliver deliver) {
+                        bool lower, bool row_side, unsigned threads, Deliver deliver) {
   int rc = check_aln(ctx, aln);
   if (rc == RNJ_OK) rc = check_model(model);
   if (rc != RNJ_OK) return rc;
@@ -1112,7 +1140,8 @@ static int rows_handoff(rnj_context* ctx, const rnj_alignment* aln, int model, f
   float *dmat = nullptr, *dlow = nullptr, *dmax = nullptr;
   RNJ_HIP(hipMallocAsync((void**)&dmat, (size_t)n * n * sizeof(float), s));
   RNJ_HIP(hipMallocAsync((void**)&dmax, sizeof(float), s));
-  std::vector<JobSpec> jobs{spec_for(aln, model, kSquare, dmat, n)};
+  std::vector<JobSpec> jobs{
+      spec_for(aln, model, row_side && aln->kind == kNaive ? kRows : kSquare, dmat, n)};
   jobs[0].max_out = dmax;
   rc = launch_jobs(ctx, jobs, false, true, s);
   hipError_t e = hipSuccess;
@@ -1205,7 +1234,7 @@ int rnj_distances_rows(rnj_context* ctx, const rnj_alignment* aln, int model, fl
   if (!aln || !rows) return fail(RNJ_E_INVALID, "null argument");
   for (unsigned i = 0; i < aln->n; ++i)
     if (!rows[i]) return fail(RNJ_E_INVALID, "null row");
-  return rows_handoff(ctx, aln, model, max_out, false, copy_threads(),
+  return rows_handoff(ctx, aln, model, max_out, false, false, copy_threads(),
                       [rows](unsigned row, const float* v, size_t len) {
                         std::memcpy(rows[row], v, len * sizeof(float));
                         return (int)RNJ_OK;
@@ -1217,7 +1246,7 @@ int rnj_distances_rows_lower(rnj_context* ctx, const rnj_alignment* aln, int mod
   if (!aln || !rows) return fail(RNJ_E_INVALID, "null argument");
   for (unsigned i = 0; i < aln->n; ++i)
     if (!rows[i]) return fail(RNJ_E_INVALID, "null row");
-  return rows_handoff(ctx, aln, model, max_out, true, copy_threads(),
+  return rows_handoff(ctx, aln, model, max_out, true, false, copy_threads(),
                       [rows](unsigned row, const float* v, size_t len) {
                         std::memcpy(rows[row], v, len * sizeof(float));
                         return (int)RNJ_OK;
@@ -1227,7 +1256,7 @@ int rnj_distances_rows_lower(rnj_context* ctx, const rnj_alignment* aln, int mod
 int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
                               rnj_row_sink sink, void* user, float* max_out) {
   if (!aln || !sink) return fail(RNJ_E_INVALID, "null argument");
-  return rows_handoff(ctx, aln, model, max_out, false, 1,
+  return rows_handoff(ctx, aln, model, max_out, false, true, 1,
                       [sink, user](unsigned row, const float* v, size_t len) {
                         if (sink(user, row, v, (unsigned)len) != 0)
                           return fail(RNJ_E_CANCELLED, "row sink stopped the stream");

@@ -121,6 +121,10 @@ struct Alignment {
   unsigned W = 0;            // 32-site words, multiple of kWordPad (= W_pad of the planes)
   int kind = kDNA;           // Kind
   unsigned code_bits = 0;    // protein: B code planes (5, 6 or 8)
+  // protein counted as simpleDistanceCalculator::calculateHammingDistance counts
+  // (simpleDistanceCalculator.cpp:40-55, the `-f` path): the valid count is the residue pairs,
+  // without bitDistanceProtein's 128-position shortfall when L is a multiple of 128
+  bool hamming = false;
   uint32_t* planes = nullptr;  // [nplanes()][W][n_pad], then (protein) gaps
   uint32_t* gaps = nullptr;    // protein: gap sites per taxon [n_pad], inside the planes block
   std::vector<std::string> names;  // FASTA record names in file order (alignments from FASTA)
@@ -137,6 +141,7 @@ struct JobSpec {
   unsigned code_bits;     // protein
   const uint32_t* gaps;   // protein: gap sites per taxon
   unsigned L;             // protein: sites
+  unsigned valid_bias;    // protein: 128 when bitDistanceProtein's valid count is 128 short
   const uint32_t* planes;
   unsigned n, n_pad, W_pad, W;
   unsigned W_live;  // words holding sites (0: W); the DNA tile kernel counts no word beyond
@@ -168,12 +173,14 @@ int alloc_alignment(Context* ctx, unsigned n, unsigned L, unsigned sites, Alignm
 bool first_row_is_dna(const char* row, unsigned L);
 
 // Kind of an alignment whose first sequence is `row0`: the context's input type, or when that
-// is RNJ_TYPE_AUTO the reference's discovery on the first sequence.
+// is RNJ_TYPE_AUTO the reference's discovery on the first sequence; DNA becomes kNaive under the
+// context's naive character rules.
 int alignment_kind(const Context* ctx, const char* row0, unsigned L);
 
 // Pack n x L characters in device memory (row t at chars + t * pitch) into a new alignment of
 // `kind`, enqueued on `stream` and synchronised. Protein alignments first take a census of the
-// byte values to number the residues.
+// byte values to number the residues; under the context's naive character rules they count as
+// the Hamming counter of the -f path does (Alignment::hamming).
 int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
                    int kind, hipStream_t stream, Alignment** out);
 
@@ -199,10 +206,14 @@ struct LaunchShape {
   int kind = kDNA;
   unsigned code_bits = 0;
   bool mixed = false;  // jobs of different kinds or code widths (not launchable together)
+  int mixed_kind = 0;  // the first job's kind and code width that differ from job 0's
+  unsigned mixed_bits = 0;
   unsigned n_key = 0, W_key = 0;
   unsigned long long work = 0;
 };
 LaunchShape shape_of(const std::vector<JobSpec>& specs);
+// Why the jobs of a mixed shape cannot share one launch (names the two kinds involved).
+std::string mixed_message(const LaunchShape& sh);
 int prepare_workspace(Workspace* ws, unsigned njobs);
 int select_config(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, Workspace* ws,
                   bool exact, hipStream_t stream, bool allow_tune, int* config);

@@ -422,6 +422,10 @@ int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned
   if (n < 2) return fail(RNJ_E_INVALID, "need at least 2 sequences");
   if (L < 1 || pitch < L) return fail(RNJ_E_INVALID, "bad sequence length or pitch");
   if (model != RNJ_JC && model != RNJ_K2P) return fail(RNJ_E_INVALID, "model must be RNJ_JC or RNJ_K2P");
+  if (srv->ctx->char_rules != RNJ_CHARS_BITS)
+    return fail(RNJ_E_INVALID,
+                "served jobs count under the bit-string rules only (the server's context has the "
+                "naive character rules set)");
   if (alignment_kind(srv->ctx, chars, L) != kDNA)
     return fail(RNJ_E_NOT_DNA, "served jobs are DNA alignments only");
   auto* j = new ServedJob();

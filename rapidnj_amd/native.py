@@ -104,6 +104,9 @@ SIGNATURES = {
 }
 
 
+_ROW_SINK = ctypes.CFUNCTYPE(ctypes.c_int, _vp, _u, ctypes.POINTER(ctypes.c_float), _u)
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", _u), ("max_batch_jobs", _u), ("hbm_budget", _sz),
                 ("batch_window_us", _u)]
@@ -446,6 +449,25 @@ class Context:
         check(lib().rnj_distances_rows_lower(self.handle, aln.handle, _model(model),
                                              ctypes.cast(rows, _vp), ctypes.byref(mx)))
         return out, mx.value
+
+    def distance_stream_rows(self, aln: Alignment, model="k2p") -> tuple[np.ndarray, float]:
+        """The disk-matrix row stream (rnj_distances_stream_rows): every post-processed row as
+        the library hands it to the sink, in order, gathered into an (n, n) array."""
+        n = aln.n
+        mat = np.empty((n, n), dtype=np.float32)
+        seen = []
+
+        def sink(user, row, values, length):
+            mat[row] = np.ctypeslib.as_array(values, shape=(length,))
+            seen.append(row)
+            return 0
+
+        cb = _ROW_SINK(sink)
+        mx = ctypes.c_float()
+        check(lib().rnj_distances_stream_rows(self.handle, aln.handle, _model(model),
+                                              ctypes.cast(cb, _vp), None, ctypes.byref(mx)))
+        assert seen == list(range(n)), "rows arrive once each, in order"
+        return mat, mx.value
 
     def bootstrap(self, aln: Alignment, weights: np.ndarray, model="k2p",
                   layout: str = "triangle", out=None, raw: bool = False, stream=None,

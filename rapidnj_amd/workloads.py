@@ -94,6 +94,22 @@ def simulate_alignment(n: int, L: int, gap: float, seed: int, kappa: float = 2.0
     return np.ascontiguousarray(leaves)
 
 
+def _simulate_args(args) -> np.ndarray:
+    return simulate_alignment(*args)
+
+
+def simulate_many(specs, procs: int = 8) -> list:
+    """:func:`simulate_alignment` for every ``(n, L, gap, seed)`` of ``specs``, in a pool of
+    ``procs`` spawned interpreter processes (fresh processes: safe after the caller has
+    initialised a GPU, unlike forked ones)."""
+    import multiprocessing as mp
+    specs = list(specs)
+    if procs <= 1 or len(specs) <= 1:
+        return [simulate_alignment(*s) for s in specs]
+    with mp.get_context("spawn").Pool(min(procs, len(specs))) as pool:
+        return pool.map(_simulate_args, specs, chunksize=1)
+
+
 AMINO_ACIDS = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", dtype=np.uint8)
 
 

@@ -15,8 +15,14 @@ over amino acids, lowercase letters and every character the reader resolves to a
 128 short, wrapping modulo 2^64 for pairs with few common residues), 300 and 640, and one at
 length 200 with a row of gaps only (valid count 0) and a row with two residues.
 
+The same cases are also read with fastdist off, the reference's `-f` path: `pcounts ... naive`
+and `pmatrix ... naive` run simpleDistanceCalculator::calculateHammingDistance
+(simpleDistanceCalculator.cpp:40-55) on the loader's resolved characters, whose valid count is
+the exact number of residue pairs (no 128-position shortfall).
+
 npz keys per case <c>: <c>_aln (n x L uint8), <c>_d0, <c>_d1, <c>_d2 (upper triangle, row-major,
-uint64), <c>_kim and <c>_jc (n x n float32).
+uint64), <c>_kim and <c>_jc (n x n float32); the -f path's <c>_nd0, <c>_nd1, <c>_nd2, <c>_nkim and
+<c>_njc.
 """
 from __future__ import annotations
 
@@ -84,17 +90,19 @@ def case(aln: np.ndarray, tmp: str) -> dict:
     names, rows = O.parse_fasta(open(fa, "rb").read())
     assert np.array_equal(rows, aln)
     out = {"aln": aln}
-    cnt = os.path.join(tmp, "c.bin")
-    run("pcounts", fa, cnt)
-    raw = np.fromfile(cnt, dtype=np.uint8)
-    d = np.frombuffer(raw[8:].tobytes(), dtype=np.uint64).reshape(-1, 3)
-    out["d0"], out["d1"], out["d2"] = d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy()
-    for model, key in (("kim", "kim"), ("jc", "jc")):
-        mat = os.path.join(tmp, "m.bin")
-        run("pmatrix", fa, model, mat)
-        raw = np.fromfile(mat, dtype=np.uint8)
-        n = int(np.frombuffer(raw[:4].tobytes(), dtype=np.uint32)[0])
-        out[key] = np.frombuffer(raw[4:].tobytes(), dtype=np.float32).reshape(n, n).copy()
+    for pre, extra in (("", []), ("n", ["naive"])):
+        cnt = os.path.join(tmp, "c.bin")
+        run("pcounts", fa, cnt, *extra)
+        raw = np.fromfile(cnt, dtype=np.uint8)
+        d = np.frombuffer(raw[8:].tobytes(), dtype=np.uint64).reshape(-1, 3)
+        for k in range(3):
+            out[f"{pre}d{k}"] = d[:, k].copy()
+        for model in ("kim", "jc"):
+            mat = os.path.join(tmp, "m.bin")
+            run("pmatrix", fa, model, mat, *extra)
+            raw = np.fromfile(mat, dtype=np.uint8)
+            n = int(np.frombuffer(raw[:4].tobytes(), dtype=np.uint32)[0])
+            out[pre + model] = np.frombuffer(raw[4:].tobytes(), dtype=np.float32).reshape(n, n).copy()
     return out
 
 

@@ -89,17 +89,35 @@ def _protein_loader_data(aln):
     return {"aln": res, "bits": bits, "gaps": np.zeros_like(bits)}
 
 
-@pytest.mark.parametrize("model,key", [("kim", "kim"), ("jc", "jc")])
+@pytest.mark.parametrize("model", ["kim", "jc"])
 @pytest.mark.parametrize("mode", ["mem", "disk", "half"])
 @pytest.mark.parametrize("source", ["fast", "chars"])
-def test_protein_loader(tmp_path, model, key, mode, source):
+def test_protein_loader(tmp_path, model, mode, source):
     """A protein loader (type PROTEIN) gets the reference's protein distances through the same
     classes: PF15224 from the reference's test data, and an alignment with a multiple-of-128
-    length and an all-gap row."""
+    length and an all-gap row. With fastdist on ("fast") the reference counts with
+    bitDistanceProtein; with fastdist off ("chars", the -f path) with
+    simpleDistanceCalculator::calculateHammingDistance, whose valid count differs on every pair
+    of p20x128 (L = 128)."""
     g = dict(np.load(os.path.join(GOLDEN, "protein.npz"), allow_pickle=False))
+    key = model if source == "fast" else "n" + model
     for case in ("pf15224", "p20x128", "p10x200gap"):
         d = _protein_loader_data(g[f"{case}_aln"])
         mat, _ = _run(tmp_path, d, model, mode, source, protein=True)
         want = g[f"{case}_{key}"]
         np.testing.assert_allclose(mat, want, rtol=1e-6, atol=0)
         np.testing.assert_array_equal(np.signbit(mat), np.signbit(want))
+
+
+@pytest.mark.parametrize("case", ["mixed", "sim"])
+@pytest.mark.parametrize("model,key", [("kim", "k2p"), ("jc", "jc")])
+@pytest.mark.parametrize("mode", ["mem", "disk"])
+def test_naive_rows_memory_and_disk(tmp_path, case, model, key, mode):
+    """fastdist off on mixed-case DNA: the in-memory matrix is the mirrored upper triangle, the
+    disk matrix holds every (i, j) counted with i as the row (the reference's two paths, written
+    by the reference in tests/golden/naive_rows.npz)."""
+    g = dict(np.load(os.path.join(GOLDEN, "naive_rows.npz"), allow_pickle=False))
+    mat, _ = _run(tmp_path, {"aln": g[f"{case}_aln"]}, model, mode, "chars")
+    want = g[f"{case}_{mode}_{key}"]
+    np.testing.assert_allclose(mat, want, rtol=1e-6, atol=0)
+    np.testing.assert_array_equal(np.signbit(mat), np.signbit(want))

@@ -107,6 +107,31 @@ def test_golden_distances(ctx, golden, case):
         assert (np.signbit(got) == np.signbit(want)).all()
 
 
+@pytest.mark.parametrize("case", CASES)
+def test_golden_hamming(ctx, golden, case):
+    """Under the naive character rules (the reference's -f path) protein alignments count as
+    simpleDistanceCalculator::calculateHammingDistance: the exact residue pairs, with no
+    128-position shortfall, for the counts and both models."""
+    aln = golden[f"{case}_aln"]
+    ctx.set_char_rules(True)
+    try:
+        A = ctx.alignment(aln)
+    finally:
+        ctx.set_char_rules(False)
+    assert A.is_protein
+    ts, tv, valid = (t.cpu().numpy().view(np.uint32) for t in ctx.counts(A))
+    np.testing.assert_array_equal(ts, golden[f"{case}_nd0"].astype(np.uint32))
+    assert not tv.any()
+    np.testing.assert_array_equal(valid, golden[f"{case}_nd2"].astype(np.uint32))
+    for model, key in (("k2p", "nkim"), ("jc", "njc")):
+        want = golden[f"{case}_{key}"]
+        got = square(ctx, A, model, True)
+        np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+        got = square(ctx, A, model, False)
+        np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
+        assert (np.signbit(got) == np.signbit(want)).all()
+
+
 @pytest.mark.parametrize("n,L", [(300, 1000), (257, 1024), (130, 4096 + 77), (64, 33)])
 def test_oracle_sizes(ctx, n, L):
     aln = protein_alignment(n, L, seed=n + L)

@@ -44,6 +44,37 @@ def test_counts_and_matrices(golden, case):
         np.testing.assert_array_equal(full.view(np.uint32), golden[f"{case}_{key}"].view(np.uint32))
 
 
+@pytest.mark.parametrize("case", CASES)
+def test_hamming_counts_and_matrices(golden, case):
+    """The -f path (fastdist off): simpleDistanceCalculator::calculateHammingDistance counts
+    the residue pairs exactly, so where L is a multiple of 128 its valid count is 128 above
+    bitDistanceProtein's, and elsewhere the two agree."""
+    aln = golden[f"{case}_aln"]
+    d0, d2 = O.protein_counts_triangle(aln, naive=True)
+    np.testing.assert_array_equal(d0, golden[f"{case}_nd0"])
+    assert not golden[f"{case}_nd1"].any()
+    np.testing.assert_array_equal(d2, golden[f"{case}_nd2"])
+    np.testing.assert_array_equal(golden[f"{case}_nd0"], golden[f"{case}_d0"])
+    shift = np.uint64(128 if aln.shape[1] % 128 == 0 else 0)
+    np.testing.assert_array_equal(golden[f"{case}_nd2"] - shift, golden[f"{case}_d2"])
+    for model, key in (("k2p", "nkim"), ("jc", "njc")):
+        full, _ = O.protein_matrix(aln, model, naive=True)
+        np.testing.assert_array_equal(full.view(np.uint32), golden[f"{case}_{key}"].view(np.uint32))
+
+
+def test_hamming_brute_force():
+    """The Hamming counter on tiny rows against a direct count over the gap set."""
+    rng = np.random.default_rng(7)
+    alpha = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWYacdxX-.?zZbBJj*", dtype=np.uint8)
+    gaps = set(b"-.XxzZbBJj?")
+    for _ in range(200):
+        L = int(rng.integers(1, 40))
+        a = alpha[rng.integers(0, len(alpha), L)].tobytes()
+        b = alpha[rng.integers(0, len(alpha), L)].tobytes()
+        both = [(x, y) for x, y in zip(a, b) if x not in gaps and y not in gaps]
+        assert O.count_protein_naive(a, b) == (sum(x != y for x, y in both), 0, len(both))
+
+
 def test_regimes_covered(golden):
     """The fixtures reach every branch: the Dayhoff table, the 10.0 cap, saturated and empty
     pairs, and the wrapped valid count."""
