@@ -226,6 +226,18 @@ RNJ_EXPORT int rnj_alignment_planes(const rnj_alignment* aln, uint32_t* host_pla
 RNJ_EXPORT int rnj_distances_device(rnj_context* ctx, const rnj_alignment* aln, int model, int layout,
                          float* out, size_t ld, int flags, float* max_out, void* stream);
 
+/* rnj_distances_device, same arguments and results, for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. No
+ * counterpart in the reference. It calls no hipMalloc / hipFree and never tunes: the launch's
+ * scratch (job table, maximum, saturation records, split partial counts) is a fixed 64 MiB
+ * device array per GPU, held from the first rnj_create on that GPU until the process ends.
+ * Matrices whose scratch does not fit (above about 22000 taxa) fail with RNJ_E_INVALID before
+ * anything is enqueued. Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other threads, other contexts) race, so the caller orders them. */
+RNJ_EXPORT int rnj_distances_device_noalloc(rnj_context* ctx, const rnj_alignment* aln, int model,
+                                            int layout, float* out, size_t ld, int flags,
+                                            float* max_out, void* stream);
+
 /* Exact ts / tv / valid of every pair into device arrays of n(n-1)/2 u32, upper triangle
  * row-major (DistanceEstimate::computeDistance, DistanceEstimate.hpp:14). Protein: ts holds the
  * mismatches (data[0]), tv 0 (data[1]) and valid the low 32 bits of the 64-bit data[2], which

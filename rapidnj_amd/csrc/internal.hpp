@@ -100,6 +100,10 @@ hipError_t launch_pack_legacy(const uint32_t* bits, const uint32_t* gaps, size_t
                               unsigned words16, unsigned n, unsigned n_pad, unsigned W_pad,
                               uint32_t* planes, hipStream_t stream);
 hipError_t launch_reset_stats(const JobDesc* jobs, unsigned njobs, hipStream_t stream);
+// Writes one job description (a kernel argument) to dst in device memory.
+hipError_t launch_store_job(const JobDesc& d, JobDesc* dst, hipStream_t stream);
+// Writes the float whose bits are *maxbits to dst (device memory).
+hipError_t launch_copy_max(const unsigned* maxbits, float* dst, hipStream_t stream);
 // exact: the reference's expression with the library's FP64 log; otherwise the leaner FP64 log
 // (device_common.hpp, within ~2 ulp of double).
 // mode: CountMode (kModeJC6 only when every job is JC; kModeNaive for naive-rule alignments).

@@ -418,6 +418,16 @@ __global__ void pack_legacy_kernel(const uint32_t* __restrict__ bits,
   }
 }
 
+// One job description, passed by value, into device memory (a launch whose job table lives in
+// memory the caller brings, with no host-to-device copy).
+__global__ void store_job_kernel(JobDesc d, JobDesc* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = d;
+}
+
+__global__ void copy_max_kernel(const unsigned* __restrict__ maxbits, float* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = __uint_as_float(*maxbits);
+}
+
 __global__ void reset_stats_kernel(const JobDesc* __restrict__ jobs, unsigned njobs) {
   const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= njobs) return;
@@ -669,6 +679,16 @@ hipError_t launch_pack_legacy(const uint32_t* bits, const uint32_t* gaps, size_t
                               uint32_t* planes, hipStream_t stream) {
   pack_legacy_kernel<<<dim3(n_pad / 256 + (n_pad % 256 != 0), ygrid(W_pad)), dim3(256), 0, stream>>>(
       bits, gaps, stride, words16, n, n_pad, W_pad, planes);
+  return hipGetLastError();
+}
+
+hipError_t launch_store_job(const JobDesc& d, JobDesc* dst, hipStream_t stream) {
+  store_job_kernel<<<dim3(1), dim3(64), 0, stream>>>(d, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_max(const unsigned* maxbits, float* dst, hipStream_t stream) {
+  copy_max_kernel<<<dim3(1), dim3(64), 0, stream>>>(maxbits, dst);
   return hipGetLastError();
 }
 

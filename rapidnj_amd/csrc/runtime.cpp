@@ -173,6 +173,69 @@ int ilog2(unsigned long long x) {
   return r;
 }
 
+}  // namespace
+
+unsigned job_tiles(const JobSpec& s, int config) {
+  const unsigned nb = (s.n + kTileConfigs[config].tile - 1) / kTileConfigs[config].tile;
+  // a kRows job also covers the transposed block triangle (job_tile)
+  return nb * (nb + 1) / 2 * (s.layout == kRows ? 2u : 1u);
+}
+
+unsigned split_want(unsigned long long total_tiles, unsigned cus) {
+  const unsigned long long target = 8ull * std::max(1u, cus);
+  if (total_tiles == 0 || total_tiles >= target) return 1;
+  return (unsigned)std::min<unsigned long long>(16, (target + total_tiles - 1) / total_tiles);
+}
+
+JobPlan plan_job(const JobSpec& s, int config, unsigned tiles, unsigned want) {
+  const TileConfig& c = kTileConfigs[config];
+  JobPlan jp;
+  jp.tiles = tiles;
+  const unsigned chunks = s.W / c.chunk;
+  unsigned S = std::max(1u, std::min(want, chunks / 2));
+  const unsigned cps = (chunks + S - 1) / S;  // chunks per split
+  jp.splits = (chunks + cps - 1) / cps;
+  jp.wps = cps * c.chunk;
+  jp.partial_elems = jp.splits > 1 ? (size_t)tiles * jp.splits * 3 * c.tile * c.tile : 0;
+  jp.sat_units = s.model != kCounts ? sat_units(tiles, config) : 0;
+  return jp;
+}
+
+void describe_job(JobDesc& d, const JobSpec& s, int config, const JobPlan& jp,
+                  unsigned long long item_base, unsigned long long fin_base, uint32_t* partial,
+                  unsigned* maxbits, uint32_t* sat, size_t cap_units, size_t sat_off) {
+  std::memset(&d, 0, sizeof d);
+  d.planes = s.planes;
+  d.plane_stride = (unsigned long long)s.W_pad * s.n_pad;
+  d.n = s.n;
+  d.n_pad = s.n_pad;
+  d.W = s.W;
+  d.W_live = s.W_live ? std::min(s.W_live, s.W) : s.W;
+  d.nblk = (s.n + kTileConfigs[config].tile - 1) / kTileConfigs[config].tile;
+  d.tiles = jp.tiles;
+  d.item_base = (unsigned)item_base;
+  d.splits = jp.splits;
+  d.words_per_split = jp.wps;
+  d.fin_base = (unsigned)fin_base;
+  d.partial = jp.splits > 1 ? partial : nullptr;
+  d.model = s.model;
+  d.layout = s.layout;
+  d.ld = s.ld;
+  d.out = s.out;
+  d.out_ts = s.out_ts;
+  d.out_tv = s.out_tv;
+  d.out_valid = s.out_valid;
+  d.gaps = s.gaps;
+  d.L = s.L;
+  d.valid_bias = s.valid_bias;
+  d.maxbits = maxbits;
+  d.sat_word = sat + sat_off;
+  d.sat_tile = sat + cap_units + sat_off;
+  d.sat_mask = sat + 2 * cap_units + 64 * sat_off;
+}
+
+namespace {
+
 // Work decomposition of one launch for one tile configuration. When the launch has fewer tiles
 // than about 8 per CU, every job's words are cut into up to 16 ranges (each at least two LDS
 // chunks), so that the device fills; the split tiles are finished by the finish kernel.
@@ -185,39 +248,24 @@ struct LaunchPlan {
 };
 
 LaunchPlan plan_launch(const std::vector<JobSpec>& specs, int config, unsigned cus) {
-  const TileConfig& c = kTileConfigs[config];
   LaunchPlan P;
   unsigned long long total_tiles = 0;
   for (const JobSpec& s : specs) {
-    const unsigned nb = (s.n + c.tile - 1) / c.tile;
-    // a kRows job also covers the transposed block triangle (job_tile)
-    P.tiles.push_back(nb * (nb + 1) / 2 * (s.layout == kRows ? 2u : 1u));
+    P.tiles.push_back(job_tiles(s, config));
     total_tiles += P.tiles.back();
   }
-  const unsigned long long target = 8ull * std::max(1u, cus);
-  unsigned want = 1;
-  if (total_tiles > 0 && total_tiles < target)
-    want = (unsigned)std::min<unsigned long long>(16, (target + total_tiles - 1) / total_tiles);
-  const size_t per_split = (size_t)3 * c.tile * c.tile;
+  const unsigned want = split_want(total_tiles, cus);
   for (size_t k = 0; k < specs.size(); ++k) {
-    const unsigned chunks = specs[k].W / c.chunk;
-    unsigned S = std::max(1u, std::min(want, chunks / 2));
-    const unsigned cps = (chunks + S - 1) / S;  // chunks per split
-    S = (chunks + cps - 1) / cps;
-    P.splits.push_back(S);
-    P.wps.push_back(cps * c.chunk);
+    const JobPlan jp = plan_job(specs[k], config, P.tiles[k], want);
+    P.splits.push_back(jp.splits);
+    P.wps.push_back(jp.wps);
     P.partial_off.push_back(P.partial_elems);
-    if (S > 1) {
-      P.partial_elems += (size_t)P.tiles[k] * S * per_split;
-      P.finish_items += P.tiles[k];
-    }
-    P.items += (unsigned long long)P.tiles[k] * S;
+    P.partial_elems += jp.partial_elems;
+    if (jp.splits > 1) P.finish_items += jp.tiles;
+    P.items += (unsigned long long)jp.tiles * jp.splits;
     P.sat_off.push_back(P.sat_units);
-    if (specs[k].model != kCounts) {
-      const unsigned long long u = sat_units(P.tiles[k], config);
-      P.sat_units += u;
-      P.max_units = std::max(P.max_units, u);
-    }
+    P.sat_units += jp.sat_units;
+    P.max_units = std::max(P.max_units, jp.sat_units);
   }
   return P;
 }
@@ -226,38 +274,14 @@ LaunchPlan plan_launch(const std::vector<JobSpec>& specs, int config, unsigned c
 void number_jobs(std::vector<JobSpec>& specs, int config, const LaunchPlan& P, Workspace* ws) {
   unsigned long long base = 0, fin = 0;
   for (size_t k = 0; k < specs.size(); ++k) {
-    const JobSpec& s = specs[k];
-    JobDesc& d = ws->h_jobs[k];
-    std::memset(&d, 0, sizeof d);
-    d.planes = s.planes;
-    d.plane_stride = (unsigned long long)s.W_pad * s.n_pad;
-    d.n = s.n;
-    d.n_pad = s.n_pad;
-    d.W = s.W;
-    d.W_live = s.W_live ? std::min(s.W_live, s.W) : s.W;
-    d.nblk = (s.n + kTileConfigs[config].tile - 1) / kTileConfigs[config].tile;
-    d.tiles = P.tiles[k];
-    d.item_base = (unsigned)base;
-    d.splits = P.splits[k];
-    d.words_per_split = P.wps[k];
-    d.fin_base = (unsigned)fin;
-    d.partial = P.splits[k] > 1 ? ws->d_partial + P.partial_off[k] : nullptr;
-    d.model = s.model;
-    d.layout = s.layout;
-    d.ld = s.ld;
-    d.out = s.out;
-    d.out_ts = s.out_ts;
-    d.out_tv = s.out_tv;
-    d.out_valid = s.out_valid;
-    d.gaps = s.gaps;
-    d.L = s.L;
-    d.valid_bias = s.valid_bias;
-    d.maxbits = ws->d_stats + k;
-    d.sat_word = ws->d_sat + P.sat_off[k];
-    d.sat_tile = ws->d_sat + ws->cap_units + P.sat_off[k];
-    d.sat_mask = ws->d_sat + 2 * ws->cap_units + 64 * P.sat_off[k];
-    base += (unsigned long long)d.tiles * d.splits;
-    if (d.splits > 1) fin += d.tiles;
+    JobPlan jp;
+    jp.tiles = P.tiles[k];
+    jp.splits = P.splits[k];
+    jp.wps = P.wps[k];
+    describe_job(ws->h_jobs[k], specs[k], config, jp, base, fin, ws->d_partial + P.partial_off[k],
+                 ws->d_stats + k, ws->d_sat, ws->cap_units, P.sat_off[k]);
+    base += (unsigned long long)jp.tiles * jp.splits;
+    if (jp.splits > 1) fin += jp.tiles;
   }
 }
 
@@ -274,7 +298,7 @@ static int default_config(unsigned n, unsigned W, bool jc_only) {
   (void)n;
   (void)W;
   (void)jc_only;
-  return 4;  // t32_m4x4_c8
+  return kDefaultTileConfig;
 }
 
 // Autotune classes: DNA launches by whether every job is JC (the 6-instruction loop), protein
@@ -284,7 +308,7 @@ static int tune_variant(const LaunchShape& sh) {
   return sh.kind == kProtein ? 16 + (int)sh.code_bits : (int)sh.jc_only;
 }
 
-static hipError_t launch_counting(int config, const LaunchShape& sh, bool exact,
+hipError_t launch_counting(int config, const LaunchShape& sh, bool exact,
                                   const JobDesc* jobs, unsigned njobs, unsigned items,
                                   unsigned finish_items, hipStream_t stream) {
   if (sh.kind == kProtein)
@@ -295,7 +319,7 @@ static hipError_t launch_counting(int config, const LaunchShape& sh, bool exact,
 }
 
 // The configuration the next launch of this shape class uses (without tuning it).
-static int current_config(Context* ctx, const LaunchShape& sh) {
+int current_config(Context* ctx, const LaunchShape& sh) {
   if (ctx->tile_config >= 0) return ctx->tile_config;
   std::lock_guard<std::mutex> lock(ctx->mu);
   auto it = ctx->tuned.find(std::make_tuple(ilog2(sh.n_key), ilog2(sh.W_key), tune_variant(sh)));
@@ -669,8 +693,16 @@ int rnj_create(int device, rnj_context** out) {
     }
   }
   if (e != hipSuccess) {
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return hip_fail(e, "rnj_create");
+  }
+  // the device address of rnj_distances_device_noalloc's scratch array, once per device
+  const int rc = scratch_setup(device);
+  if (rc != RNJ_OK) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
   }
   *out = ctx;
   return RNJ_OK;

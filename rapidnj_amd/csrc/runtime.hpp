@@ -224,6 +224,37 @@ int enqueue_launch(Context* ctx, std::vector<JobSpec>& specs, LaunchShape& sh, i
                    bool upload_table = true);
 void free_workspace_public(Workspace* ws);
 
+// One job's share of a launch (the pieces of plan_launch / number_jobs, for callers that plan a
+// single job without the workspace machinery).
+struct JobPlan {
+  unsigned tiles = 0, splits = 1, wps = 0;  // pair tiles, word ranges per tile, words per range
+  size_t partial_elems = 0;                 // u32 partial counts when split
+  unsigned long long sat_units = 0;         // saturation wave units
+};
+unsigned job_tiles(const JobSpec& s, int config);
+// Word ranges per tile that a launch of `total_tiles` tiles asks for (1: no split).
+unsigned split_want(unsigned long long total_tiles, unsigned cus);
+JobPlan plan_job(const JobSpec& s, int config, unsigned tiles, unsigned want);
+// The kernels' description of one job: items from item_base, finish items from fin_base, its
+// partial counts, maximum, and saturation records (regions of cap_units words, then tiles, then
+// 64 masks per unit; this job's units from sat_off).
+void describe_job(JobDesc& d, const JobSpec& s, int config, const JobPlan& jp,
+                  unsigned long long item_base, unsigned long long fin_base, uint32_t* partial,
+                  unsigned* maxbits, uint32_t* sat, size_t cap_units, size_t sat_off);
+// The tile configuration launches get before (or without) tuning: t32_m4x4_c8.
+constexpr int kDefaultTileConfig = 4;
+// Device scratch of rnj_distances_device_noalloc (noalloc.hip): one array of this size per GPU.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+// Takes the device address of that array on `device` (current) once; rnj_create calls it.
+int scratch_setup(int device);
+// The configuration a launch of this shape class gets without tuning (the context's forced
+// configuration, a tuned one, else the default).
+int current_config(Context* ctx, const LaunchShape& sh);
+// The count kernels of a launch (tile kernel, and the finish kernel for split jobs).
+hipError_t launch_counting(int config, const LaunchShape& sh, bool exact, const JobDesc* jobs,
+                           unsigned njobs, unsigned items, unsigned finish_items,
+                           hipStream_t stream);
+
 }  // namespace rnj
 
 struct rnj_context : rnj::Context {};

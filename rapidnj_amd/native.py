@@ -66,6 +66,8 @@ SIGNATURES = {
     "rnj_alignment_planes": (ctypes.c_int, [_vp, _vp]),
     "rnj_distances_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _sz,
                                             ctypes.c_int, _vp, _vp]),
+    "rnj_distances_device_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                                                    _sz, ctypes.c_int, _vp, _vp]),
     "rnj_counts_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rnj_copy_to_host": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "rnj_valu_issue_rates": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),

@@ -62,6 +62,9 @@ def test_calls_without_a_device(native):
     # argument checks come before any device call
     assert L.rnj_distances_device(None, None, 1, 0, None, 0, 0, None, None) == native.RNJ_E_INVALID
     assert b"null" in L.rnj_last_error()
+    assert L.rnj_distances_device_noalloc(None, None, 1, 0, None, 0, 0, None,
+                                          None) == native.RNJ_E_INVALID
+    assert b"null" in L.rnj_last_error()
     assert L.rnj_set_tile_config(None, 0) == native.RNJ_E_INVALID
     assert L.rnj_set_char_rules(None, 1) == native.RNJ_E_INVALID
     assert L.rnj_server_submit(None, None, 0, 0, 0, 0, None) == native.RNJ_E_INVALID
