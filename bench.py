@@ -25,20 +25,49 @@ sys.path.insert(0, ROOT)
 METRIC = "G site-pairs/s on distance matrix vs CPU; p50/p99 job latency served"
 HBM_PEAK_GBS = 8000.0
 # VALU instructions per 32 sites and pair of the count loop, by class: (bitwise, v_bcnt).
-# K2P: v_and, v_bitop3 x3 (one folds the xor) and 3 v_bcnt; JC: v_and, 2 v_bitop3, v_or, 2 v_bcnt.
-OP_CLASSES = {"k2p": (4, 3), "jc": (4, 2)}
+# K2P: v_and, 2 v_bitop3 (same-class sites, then transitions among them) and 3 v_bcnt;
+# JC: v_and, 2 v_bitop3 (same class, then identical bases) and 2 v_bcnt.
+OP_CLASSES = {"k2p": (3, 3), "jc": (3, 2)}
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_cores() -> int:
+def host_cpus() -> dict:
+    """The host CPUs this process may run on: the affinity mask, capped by a cgroup CPU quota
+    (cpu.max) and by the CPU share the environment grants (OMP_NUM_THREADS) when they are set, and
+    the CPU model from /proc/cpuinfo."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    usable = aff if quota is None else max(1, min(aff, int(quota)))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        usable = min(usable, int(share))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"affinity": aff, "cgroup_quota": quota, "omp_num_threads": share or None,
+            "usable": usable, "model": model}
+
+
+def cpu_cores() -> int:
+    """Every host CPU this process may use (worker pools and the CPU baseline)."""
+    return host_cpus()["usable"]
 
 
 def cpu_baseline(aln, model: str, sample_n: int, threads: int, protein: bool = False) -> dict:
@@ -786,10 +815,22 @@ def main() -> int:
 
     base = None
     if not args.no_cpu_baseline:
-        threads = cpu_cores()
-        sample_n = min(n, 20000)  # the whole 20000 x 5000 workload: ~4 s on 16 host threads
-        log(f"CPU baseline on {sample_n} taxa with {threads} threads")
+        cpus = host_cpus()
+        threads = cpus["usable"]
+        # every usable core on the whole workload (a few seconds on 16 cores), then one thread on
+        # the first half of the taxa (a quarter of the pairs: 10-15 s)
+        sample_n = min(n, 20000)
+        log(f"CPU baseline on {sample_n} taxa with {threads} threads ({cpus['model']})")
         base = cpu_baseline(aln, wl.model, sample_n, threads, protein=wl.protein)
+        one_n = min(n, 10000)
+        log(f"CPU baseline on {one_n} taxa with 1 thread")
+        one = cpu_baseline(aln, wl.model, one_n, 1, protein=wl.protein)
+        base.update({"cpu_model": cpus["model"], "cores_affinity": cpus["affinity"],
+                     "cgroup_cpu_quota": cpus["cgroup_quota"],
+                     "omp_num_threads": cpus["omp_num_threads"],
+                     "single_thread": {k: one[k] for k in ("value", "cores", "sample")},
+                     "gpu_speedup": round(value / base["value"], 1),
+                     "gpu_speedup_vs_single_thread": round(value / one["value"], 1)})
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "G site-pairs/s", "n_gpus": 1,

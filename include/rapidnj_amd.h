@@ -131,7 +131,7 @@ RNJ_EXPORT int rnj_get_tile_config_for(rnj_context* ctx, const rnj_alignment* al
  * on this context's device: the bitwise class (v_bitop3_b32, as v_and / v_xor / v_or) and
  * v_bcnt_u32_b32. Each is timed on 16 independent dependency chains per lane at 8 waves per SIMD.
  * The ceiling of a count loop that issues b bitwise and c bcnt instructions per 32 site-pairs is
- * (b + c) / (b / bitwise + c / bcnt) lane-ops per second: K2P issues 4 + 3, JC 4 + 2. Takes a
+ * (b + c) / (b / bitwise + c / bcnt) lane-ops per second: K2P issues 3 + 3, JC 3 + 2. Takes a
  * few milliseconds. No counterpart in the reference; it is the device's int32 VALU peak for this
  * path, which no data sheet states. */
 RNJ_EXPORT int rnj_valu_issue_rates(rnj_context* ctx, double* bitwise, double* bcnt);

@@ -2,10 +2,12 @@
 // correction, the saturation patch, and bootstrap column resampling.
 //
 // The count kernel is integer VALU work: per 32 sites and taxon pair it issues
-//   g = Vi & Vj; t = bitop3((Hi ^ Hj) & g); u = g ^ t; s = bitop3((Li ^ Lj) & u);
-//   tv += bcnt(t); ts += bcnt(s); valid += bcnt(g)
-// i.e. 7 VALU instructions (v_and, 2x v_bitop3, v_xor, 3x v_bcnt with accumulate). JC needs only
-// ts + tv, which costs 6: the mismatch mask (Hi^Hj | Li^Lj) & g and g itself.
+//   g = Vi & Vj; u = bitop3(~(Hi ^ Hj) & g); s = bitop3((Li ^ Lj) & u);
+//   valid += bcnt(g); same += bcnt(u); ts += bcnt(s)
+// i.e. 6 VALU instructions (v_and, 2x v_bitop3, 3x v_bcnt with accumulate): u holds the valid
+// sites whose bases share H (both purines or both pyrimidines), so tv = valid - same, and s the
+// transitions among them. JC needs only ts + tv, which costs 5: g, u, the identical bases
+// ~(Li ^ Lj) & u, and two counts (mismatches = valid - identical).
 #include "device_common.hpp"
 
 namespace rnj {
@@ -82,8 +84,8 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, u
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------
-// Word w of the staged chunk into thread (ty, tx)'s MI x MJ accumulators (7 VALU ops per pair
-// for K2P, 6 for JC).
+// Word w of the staged chunk into thread (ty, tx)'s MI x MJ accumulators (6 VALU ops per pair
+// for K2P, 5 for JC, 9 for the naive rules).
 template <int MI, int MJ, int CW, int Q, int MODE>
 __device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u* __restrict__ sJ,
                                            int w, unsigned ty, unsigned tx,
@@ -136,16 +138,31 @@ __device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u
         acc_a[a][b] = bcnt_add(and3(ui[a], vj[b], y), acc_a[a][b]);
         continue;
       }
-      const uint32_t t = xor_and(hi[a], hj[b], g);
+      // u: sites valid in both with the same H (purine/purine or pyrimidine/pyrimidine); the
+      // transversions are the rest of g, so #tv = #g - #u (counts_from_same turns acc_a into it)
+      const uint32_t u = xnor_and(hi[a], hj[b], g);
       if constexpr (MODE == kModeJC6) {
-        const uint32_t m = t | xor_and(li[a], lj[b], g);
-        acc_a[a][b] = bcnt_add(m, acc_a[a][b]);
+        // identical bases: u with the same L; mismatches = #g - #same
+        acc_a[a][b] = bcnt_add(xnor_and(li[a], lj[b], u), acc_a[a][b]);
       } else {
-        const uint32_t s = xor_and(li[a], lj[b], g ^ t);
-        acc_a[a][b] = bcnt_add(t, acc_a[a][b]);
-        acc_b[a][b] = bcnt_add(s, acc_b[a][b]);
+        acc_a[a][b] = bcnt_add(u, acc_a[a][b]);
+        acc_b[a][b] = bcnt_add(xor_and(li[a], lj[b], u), acc_b[a][b]);  // transitions
       }
     }
+}
+
+// After the word loop of the K2P and JC modes acc_a holds #u (K2P: same-class sites) or #same
+// (JC: identical bases); both complements within the valid sites are what the epilogue takes
+// (K2P: transversions, JC: mismatches). The naive mode counts them directly.
+template <int MI, int MJ, int MODE>
+__device__ __forceinline__ void counts_from_same(uint32_t (&acc_a)[MI][MJ],
+                                                 const uint32_t (&acc_v)[MI][MJ]) {
+  if constexpr (MODE != kModeNaive) {
+#pragma unroll
+    for (int a = 0; a < MI; ++a)
+#pragma unroll
+      for (int b = 0; b < MJ; ++b) acc_a[a][b] = acc_v[a][b] - acc_a[a][b];
+  }
 }
 
 // One work-group computes a TILE x TILE block of taxon pairs (bi <= bj) over one range of words
@@ -216,6 +233,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
       count_word<MI, MJ, CW, Q, MODE>(sI, sJ, w, ty, tx, acc_a, acc_b, acc_v);
     }
   }
+  counts_from_same<MI, MJ, MODE>(acc_a, acc_v);
 
   if (J.splits > 1) {
     uint32_t* slot = partial_slot<NT, MI, MJ>(J, tile, split) + (size_t)tid * MI * MJ;
