@@ -105,6 +105,25 @@ RNJ_EXPORT void computeDistancesDNA_gpu(unsigned int* ts, unsigned int* tv, unsi
 /* distanceCalculatorDNA_gpu.cu:228-236: waits for the device; results were already copied. */
 RNJ_EXPORT void getResultsDNA_gpu(unsigned int* ts, unsigned int* tv, unsigned int* gaps);
 
+/* The protein half (gpu/distanceCalculatorProtein_gpu.cu:139-229, declared by the caller at
+ * KimuraDistance.cpp:20-22 and called by KimuraDistance::computeDistancesProteinGPU at
+ * :113-162). initialiseProtein_gpu: sequenceCount taxa staged as dataloader::initGPUData does
+ * for protein (dataloader.hpp:38-70: bsStride u32 words per taxon, 4 characters each), of which
+ * the first paddedLength characters (the caller passes L rounded up to 128) are compared. */
+RNJ_EXPORT void initialiseProtein_gpu(unsigned int sequenceCount, unsigned int bsStride,
+                                      unsigned int paddedLength);
+/* distanceCalculatorProtein_gpu.cu:139-176: bitStrings is the staged host array
+ * [sequenceCount][bsStride]. For every pair, over the paddedLength positions, a character below
+ * 64 is a gap (the kernel's test, :76-90; the loader's '-' padding and the zero fill beyond it
+ * are gaps); dist[i*n+j] receives the positions where both are residues and differ, gaps[i*n+j]
+ * the positions where either is a gap, for j > i (this implementation also mirrors j < i and
+ * writes 0 on the diagonal). Host arrays of sequenceCount^2 u32. */
+RNJ_EXPORT void computeDistancesProtein_gpu(unsigned int* bitStrings, unsigned int* dist,
+                                            unsigned int* gaps);
+/* distanceCalculatorProtein_gpu.cu:178-187: waits for the device; computeDistancesProtein_gpu
+ * has copied the results (the reference's version prints that it is not implemented and exits). */
+RNJ_EXPORT void getResultsProtein_gpu(unsigned int* dist, unsigned int* gaps);
+
 /* ---- handle API (group 2) --------------------------------------------------------------- */
 
 RNJ_EXPORT const char* rnj_last_error(void);           /* message of this thread's last failure */

@@ -34,6 +34,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <iomanip>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -107,6 +108,21 @@ rnj_alignment* upload_alignment(Loader* loader, bool fastdist) {
   }
   if (rc != RNJ_OK) fail_exit("upload");
   return aln;
+}
+
+// The reference's matrix output (`-o m`), printDistanceMatrix (src/main.cpp:212-221): a tab and
+// the taxon count, then per row the taxon's name, a tab, and every entry in fixed notation with
+// 6 decimals, each followed by a space. For the rows a calculator hands over (names: the
+// loader's getSequenceNames()); the stream keeps std::fixed and the precision afterwards, as the
+// reference's does.
+template <class Names>
+void printDistanceMatrix(std::ostream& out, const Names& names, distType* const* matrix, int n) {
+  out << "\t" << n << std::endl;
+  for (int i = 0; i < n; i++) {
+    out << names[i] << "\t";
+    for (int j = 0; j < n; j++) out << std::setprecision(6) << std::fixed << matrix[i][j] << " ";
+    out << std::endl;
+  }
 }
 
 template <class DiskMatrix>

@@ -63,6 +63,9 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, name)
             f.restype = None
             f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint, _u64_3]
+        L.rnj_oracle_protein_gpu_counts.restype = None
+        L.rnj_oracle_protein_gpu_counts.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                                    ctypes.c_uint, _u64_3]
         for f in (L.rnj_oracle_kimura_protein, L.rnj_oracle_jc_protein):
             f.restype = ctypes.c_float
             f.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
@@ -217,6 +220,33 @@ def count_protein_naive(a: bytes, b: bytes) -> tuple[int, int, int]:
     out = _u64_3()
     lib().rnj_oracle_count_protein_naive(a, b, len(a), out)
     return int(out[0]), int(out[1]), int(out[2])
+
+
+def protein_gpu_counts(a: bytes, b: bytes) -> tuple[int, int]:
+    """The reference's dormant protein kernel on two staged rows: (mismatches, gap positions)."""
+    out = _u64_3()
+    lib().rnj_oracle_protein_gpu_counts(a, b, len(a), out)
+    return int(out[0]), int(out[1])
+
+
+def protein_staging(aln: np.ndarray) -> tuple[np.ndarray, int, int]:
+    """The host array dataloader::initGPUData stages for a protein alignment (dataloader.hpp
+    :38-70): the loader's resolved rows padded with '-' to bitStringsCount = L/16 + 8 blocks of
+    16 characters (dataloaderFasta.hpp storeSequenceFastDist), copied into zeroed rows of bsStride
+    = roundup(ceil(L/4), 128) words, of which min(4 bitStringsCount, bsStride) words are copied.
+    -> (n x bsStride uint32, bsStride, paddedLength = roundup(L, 128))."""
+    aln = np.ascontiguousarray(aln, dtype=np.uint8)
+    n, L = aln.shape
+    res = aln.copy()
+    res[np.isin(res, np.frombuffer(b"-.XxzZbBJj?", dtype=np.uint8))] = ord("-")
+    bsc = L // 16 + 8
+    loader = np.full((n, bsc * 16), ord("-"), dtype=np.uint8)
+    loader[:, :L] = res
+    stride = ((L + 3) // 4 + 127) & ~127
+    staged = np.zeros((n, stride * 4), dtype=np.uint8)
+    words = min(bsc * 4, stride)
+    staged[:, :words * 4] = loader[:, :words * 4]
+    return staged.view("<u4").reshape(n, stride), stride, (L + 127) & ~127
 
 
 def kimura_protein(dist: int, valid: int) -> float:

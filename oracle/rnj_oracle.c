@@ -463,6 +463,24 @@ void rnj_oracle_count_protein_naive(const char* a, const char* b, unsigned L, ui
   out[2] = len;
 }
 
+/* The reference's dormant protein kernel, computeDistanceProtein
+ * (gpu/distanceCalculatorProtein_gpu.cu:52-104), on two staged rows of `padded` bytes (the
+ * loader's resolved characters, its '-' padding and the zero fill of dataloader::initGPUData):
+ * a byte below 64 is a gap (uchar4 components, unsigned); out[0] = positions where neither is a
+ * gap and the bytes differ, out[1] = positions where either is a gap, out[2] = 0. */
+void rnj_oracle_protein_gpu_counts(const unsigned char* a, const unsigned char* b,
+                                   unsigned padded, uint64_t out[3]) {
+  uint64_t r = 0, g = 0;
+  for (unsigned k = 0; k < padded; k++) {
+    const int gap = a[k] < 64 || b[k] < 64;
+    g += gap;
+    r += (a[k] != b[k]) && !gap;
+  }
+  out[0] = r;
+  out[1] = g;
+  out[2] = 0;
+}
+
 /* the Dayhoff PAM distances (x100) for p = 0.750 .. 0.930, KimuraDistance.cpp:30-49 */
 static const int kDayhoffPams[] = {
   195, 196, 197, 198, 199, 200, 200, 201, 202, 203, 204, 205, 206, 207, 208, 209, 209, 210,

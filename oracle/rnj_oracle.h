@@ -51,6 +51,11 @@ void rnj_oracle_count_protein(const char* a, const char* b, unsigned L, uint64_t
  * (simpleDistanceCalculator.cpp:40-55), on the resolved characters: out[0] = positions where
  * both are residues and differ, out[1] = 0, out[2] = positions where both are residues. */
 void rnj_oracle_count_protein_naive(const char* a, const char* b, unsigned L, uint64_t out[3]);
+/* The reference's dormant protein kernel (gpu/distanceCalculatorProtein_gpu.cu:52-104) on two
+ * staged rows of `padded` bytes: bytes below 64 are gaps; out[0] = both residues and differ,
+ * out[1] = either a gap, out[2] = 0. */
+void rnj_oracle_protein_gpu_counts(const unsigned char* a, const unsigned char* b,
+                                   unsigned padded, uint64_t out[3]);
 /* Kimura protein with the Dayhoff PAM table (KimuraDistance.cpp:30-49,322-338) and JC with
  * alpha = 20 (JCdistance.cpp:155-183) on those counts. */
 float rnj_oracle_kimura_protein(uint64_t dist, uint64_t valid);

@@ -537,7 +537,7 @@ static bool protein_gap_byte(unsigned v) {
 }
 
 int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
-                   int kind, hipStream_t s, Alignment** out) {
+                   int kind, hipStream_t s, Alignment** out, int gap_rule) {
   Alignment* a = nullptr;
   if (kind == kDNA || kind == kNaive) {
     RNJ_HIP_STATUS(alloc_alignment(ctx, n, L, L, &a, kind, 0));
@@ -564,12 +564,14 @@ int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n,
   unsigned S = 0;
   for (unsigned v = 0; v < 256; ++v) {
     const bool here = (present[v >> 5] >> (v & 31)) & 1u;
-    codes[v] = (here && !protein_gap_byte(v)) ? (uint8_t)++S : (uint8_t)0;
+    const bool gap = gap_rule == kGapBelow64 ? v < 64 : protein_gap_byte(v);
+    codes[v] = (here && !gap) ? (uint8_t)++S : (uint8_t)0;
   }
   const unsigned B = S < 32 ? 5 : (S < 64 ? 6 : 8);
   int rc = RNJ_OK;
   if (e == hipSuccess) rc = alloc_alignment(ctx, n, L, L, &a, kProtein, B);
-  if (e == hipSuccess && rc == RNJ_OK) a->hamming = ctx->char_rules == RNJ_CHARS_NAIVE;
+  if (e == hipSuccess && rc == RNJ_OK)
+    a->hamming = gap_rule == kGapBelow64 || ctx->char_rules == RNJ_CHARS_NAIVE;
   uint8_t* dcodes = reinterpret_cast<uint8_t*>(scratch + 8);
   if (e == hipSuccess && rc == RNJ_OK)
     e = hipMemcpyAsync(dcodes, codes, sizeof codes, hipMemcpyHostToDevice, s);
@@ -1507,6 +1509,92 @@ void computeDistancesDNA_gpu(unsigned int* ts, unsigned int* tv, unsigned int* g
 void getResultsDNA_gpu(unsigned int* ts, unsigned int* tv, unsigned int* gaps) {
   (void)ts; (void)tv; (void)gaps;
   if (g_counts.ctx) legacy_check(rnj_synchronize(g_counts.ctx), "getResultsDNA_gpu");
+}
+
+}  // extern "C"
+
+// ---- the reference's protein device boundary (gpu/distanceCalculatorProtein_gpu.cu:139-229) ----
+namespace {
+struct ProteinLegacyState {
+  rnj_context* ctx = nullptr;
+  unsigned n = 0, stride = 0, padded = 0;
+};
+ProteinLegacyState g_protein;
+}  // namespace
+
+extern "C" {
+
+void initialiseProtein_gpu(unsigned int sequenceCount, unsigned int bsStride,
+                           unsigned int paddedLength) {
+  const char* where = "initialiseProtein_gpu";
+  ProteinLegacyState& st = g_protein;
+  if (!st.ctx) legacy_check(rnj_create(0, &st.ctx), where);
+  if (sequenceCount < 2) {
+    set_error("need at least 2 sequences");
+    legacy_die(where);
+  }
+  if (paddedLength == 0 || (size_t)paddedLength > (size_t)bsStride * 4) {
+    set_error("paddedLength must be between 1 and 4 * bsStride characters");
+    legacy_die(where);
+  }
+  st.n = sequenceCount;
+  st.stride = bsStride;
+  st.padded = paddedLength;
+}
+
+void computeDistancesProtein_gpu(unsigned int* bitStrings, unsigned int* dist, unsigned int* gaps) {
+  const char* where = "computeDistancesProtein_gpu";
+  ProteinLegacyState& st = g_protein;
+  if (!st.ctx) {
+    set_error("called before initialisation");
+    legacy_die(where);
+  }
+  if (!bitStrings || !dist || !gaps) {
+    set_error("null argument");
+    legacy_die(where);
+  }
+  Context* ctx = st.ctx;
+  const unsigned n = st.n, P = st.padded;
+  const size_t m = (size_t)n * (n - 1) / 2;
+  legacy_check(set_device(st.ctx), where);
+  // the staged rows (4 characters per word, dataloader::initGPUData) as n x P characters
+  uint8_t* dchars = nullptr;
+  uint32_t* d = nullptr;
+  hipError_t e = hipMalloc((void**)&dchars, (size_t)n * P);
+  if (e == hipSuccess) e = hipMalloc((void**)&d, 3 * std::max<size_t>(m, 1) * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = ctx->staging.upload(dchars, reinterpret_cast<const char*>(bitStrings),
+                            (size_t)st.stride * 4, P, n, ctx->stream);
+  if (e != hipSuccess) legacy_check(hip_fail(e, where), where);
+  Alignment* a = nullptr;
+  legacy_check(pack_alignment(ctx, dchars, P, n, P, kProtein, ctx->stream, &a, kGapBelow64), where);
+  rnj_alignment* aln = static_cast<rnj_alignment*>(a);
+  const int rc = rnj_counts_device(st.ctx, aln, d, d + m, d + 2 * m, ctx->stream);
+  std::vector<uint32_t> h(3 * m);
+  if (rc == RNJ_OK)
+    e = hipMemcpyAsync(h.data(), d, 3 * m * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (rc == RNJ_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  rnj_alignment_free(aln);
+  (void)hipFree(dchars);
+  (void)hipFree(d);
+  legacy_check(rc, where);
+  if (e != hipSuccess) legacy_check(hip_fail(e, where), where);
+  // mismatches and the positions where either character is a gap, of the P positions walked
+  size_t k = 0;
+  for (size_t i = 0; i < n; ++i) {
+    dist[i * n + i] = 0;
+    gaps[i * n + i] = 0;
+    for (size_t j = i + 1; j < n; ++j, ++k) {
+      const uint32_t gp = P - h[2 * m + k];
+      dist[i * n + j] = dist[j * n + i] = h[k];
+      gaps[i * n + j] = gaps[j * n + i] = gp;
+    }
+  }
+}
+
+void getResultsProtein_gpu(unsigned int* dist, unsigned int* gaps) {
+  (void)dist; (void)gaps;
+  if (g_protein.ctx) legacy_check(rnj_synchronize(g_protein.ctx), "getResultsProtein_gpu");
 }
 
 }  // extern "C"

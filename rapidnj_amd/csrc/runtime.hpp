@@ -121,9 +121,10 @@ struct Alignment {
   unsigned W = 0;            // 32-site words, multiple of kWordPad (= W_pad of the planes)
   int kind = kDNA;           // Kind
   unsigned code_bits = 0;    // protein: B code planes (5, 6 or 8)
-  // protein counted as simpleDistanceCalculator::calculateHammingDistance counts
-  // (simpleDistanceCalculator.cpp:40-55, the `-f` path): the valid count is the residue pairs,
-  // without bitDistanceProtein's 128-position shortfall when L is a multiple of 128
+  // protein whose valid count is the residue pairs exactly, as
+  // simpleDistanceCalculator::calculateHammingDistance (simpleDistanceCalculator.cpp:40-55, the
+  // `-f` path) and the dormant protein kernel count them, without bitDistanceProtein's
+  // 128-position shortfall when L is a multiple of 128
   bool hamming = false;
   uint32_t* planes = nullptr;  // [nplanes()][W][n_pad], then (protein) gaps
   uint32_t* gaps = nullptr;    // protein: gap sites per taxon [n_pad], inside the planes block
@@ -181,8 +182,13 @@ int alignment_kind(const Context* ctx, const char* row0, unsigned L);
 // `kind`, enqueued on `stream` and synchronised. Protein alignments first take a census of the
 // byte values to number the residues; under the context's naive character rules they count as
 // the Hamming counter of the -f path does (Alignment::hamming).
+// gap_rule (protein): kGapResolved, the characters dataloader::resolveChar maps to the gap
+// (- . X x z Z b B J j ?); kGapBelow64, every byte below 64, as the reference's dormant protein
+// kernel tests them (gpu/distanceCalculatorProtein_gpu.cu:76-90; its valid count is then the
+// residue pairs exactly).
+enum GapRule : int { kGapResolved = 0, kGapBelow64 = 1 };
 int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n, unsigned L,
-                   int kind, hipStream_t stream, Alignment** out);
+                   int kind, hipStream_t stream, Alignment** out, int gap_rule = kGapResolved);
 
 // Parse a FASTA image and pack it (fasta.hip). `bytes` is in device memory, 16-byte aligned,
 // unless `stage` is hipMemcpyHostToDevice or hipMemcpyDeviceToDevice: then it is first copied

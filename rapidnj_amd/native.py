@@ -38,6 +38,9 @@ SIGNATURES = {
     "storeDataDNA_gpu": (None, [_vp, _vp]),
     "computeDistancesDNA_gpu": (None, [_vp, _vp, _vp]),
     "getResultsDNA_gpu": (None, [_vp, _vp, _vp]),
+    "initialiseProtein_gpu": (None, [_u, _u, _u]),
+    "computeDistancesProtein_gpu": (None, [_vp, _vp, _vp]),
+    "getResultsProtein_gpu": (None, [_vp, _vp]),
     "rnj_last_error": (ctypes.c_char_p, []),
     "rnj_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "rnj_api_version": (ctypes.c_int, []),

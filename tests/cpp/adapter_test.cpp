@@ -3,8 +3,9 @@
 // matrix, KimuraDistance / JCdistance, getDistanceMatrix. Test infrastructure for
 // tests/test_gpu_adapter.py.
 //
-//   adapter_test IN OUT kim|jc mem|disk|reuse|half|tree fast|chars [protein]
-// (tree: OUT is the Newick text of rapidnj_amd::NJFromAlignment on the loader)
+//   adapter_test IN OUT kim|jc mem|disk|reuse|half|tree|print fast|chars [protein]
+// (tree: OUT is the Newick text of rapidnj_amd::NJFromAlignment on the loader; print: OUT is the
+// text rapidnj_amd::printDistanceMatrix writes for the in-memory rows, names t0, t1, ..)
 // IN:  u32 n, L, bitStringsCount; n*L chars; n*bsc*4 u32 bit strings; n*bsc*4 u32 gap filters
 //      (protein: the bit strings hold the resolved characters, 16 per 4 words, as the reference's
 //      loader stores them; the gap filters are unused)
@@ -100,6 +101,23 @@ int main(int argc, char** argv) {
     std::ofstream o(argv[2]);
     tree->serialize_tree(o);
     delete tree;
+    return o ? 0 : 4;
+  }
+  if (mode == "print") {
+    distType** m = nullptr;
+    if (model == "kim") {
+      rapidnj_amd::KimuraDistance<MemoryLoader, MemoryDiskMatrix> alg(false, fast, &ld, nullptr);
+      alg.computeDistances(8);
+      m = alg.getDistanceMatrix();
+    } else {
+      rapidnj_amd::JCdistance<MemoryLoader, MemoryDiskMatrix> alg(false, fast, &ld, nullptr);
+      alg.computeDistanceMatrix(8);
+      m = alg.getDistanceMatrix();
+    }
+    std::ofstream o(argv[2]);
+    rapidnj_amd::printDistanceMatrix(o, *ld.getSequenceNames(), m, (int)n);
+    for (unsigned i = 0; i < n; ++i) delete[] m[i];
+    delete[] m;
     return o ? 0 : 4;
   }
   std::vector<distType> out((size_t)n * n);

@@ -121,3 +121,26 @@ def test_naive_rows_memory_and_disk(tmp_path, case, model, key, mode):
     want = g[f"{case}_{mode}_{key}"]
     np.testing.assert_allclose(mat, want, rtol=1e-6, atol=0)
     np.testing.assert_array_equal(np.signbit(mat), np.signbit(want))
+
+
+@pytest.mark.parametrize("case,model", [("c1", "jc"), ("c1", "kim"), ("saturated", "kim"),
+                                        ("lgt70", "kim"), ("pf15224", "kim")])
+def test_printed_matrix_matches_reference_program(tmp_path, case, model):
+    """The reference's `-o m` output (printDistanceMatrix, src/main.cpp:212-221), printed by the
+    reference's own program from its own matrix (tests/golden/matrix_text.npz), equals the text
+    rapidnj_amd::printDistanceMatrix writes for the rows the GPU calculator hands over, byte for
+    byte: 6 fixed decimals absorb the last-ulp differences the 1e-6 tolerance allows."""
+    g = dict(np.load(os.path.join(GOLDEN, "matrix_text.npz"), allow_pickle=False))
+    aln = g[f"{case}_aln"]
+    protein = case == "pf15224"
+    d = _protein_loader_data(aln) if protein else {"aln": aln}
+    inp, out = tmp_path / "in.bin", tmp_path / "out.txt"
+    _write_input(inp, d)
+    r = subprocess.run([DRIVER, str(inp), str(out), model, "print", "fast"] +
+                       (["protein"] if protein else []), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = g[f"{case}_{model}_text"].tobytes()
+    got = out.read_bytes()
+    assert got.split(b"\n")[0] == want.split(b"\n")[0]
+    assert got == want

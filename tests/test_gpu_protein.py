@@ -132,6 +132,56 @@ def test_golden_hamming(ctx, golden, case):
         assert (np.signbit(got) == np.signbit(want)).all()
 
 
+def _dormant_protein_abi(N, staged, n, stride, P):
+    """initialiseProtein_gpu / computeDistancesProtein_gpu / getResultsProtein_gpu through the C
+    ABI, as KimuraDistance::computeDistancesProteinGPU calls them (KimuraDistance.cpp:113-128)."""
+    L = N.lib()
+    dist = np.full(n * n, 0xDEADBEEF, dtype=np.uint32)
+    gaps = np.full(n * n, 0xDEADBEEF, dtype=np.uint32)
+    staged = np.ascontiguousarray(staged, dtype=np.uint32)
+    L.initialiseProtein_gpu(n, stride, P)
+    L.computeDistancesProtein_gpu(staged.ctypes.data, dist.ctypes.data, gaps.ctypes.data)
+    L.getResultsProtein_gpu(dist.ctypes.data, gaps.ctypes.data)
+    return dist.reshape(n, n), gaps.reshape(n, n)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_reference_protein_gpu_abi(N, golden, case):
+    """The reference's dormant protein ABI: per pair the mismatches and the gap positions among
+    the paddedLength staged characters (bytes below 64 are gaps), equal to the oracle's
+    restatement of computeDistanceProtein on every pair, and for rows free of other bytes below 64
+    to the reference-written bitDistanceProtein and Hamming counts."""
+    aln = golden[f"{case}_aln"]
+    n = aln.shape[0]
+    staged, stride, P = O.protein_staging(aln)
+    dist, gaps = _dormant_protein_abi(N, staged, n, stride, P)
+    rows = staged.view(np.uint8).reshape(n, -1)[:, :P]
+    iu = np.triu_indices(n, 1)
+    want = np.array([O.protein_gpu_counts(rows[i].tobytes(), rows[j].tobytes())
+                     for i, j in zip(*iu)], dtype=np.uint32)
+    np.testing.assert_array_equal(dist[iu], want[:, 0])
+    np.testing.assert_array_equal(gaps[iu], want[:, 1])
+    np.testing.assert_array_equal(dist, dist.T)
+    np.testing.assert_array_equal(gaps, gaps.T)
+    assert not np.diag(dist).any() and not np.diag(gaps).any()
+    if case == "pf15224":  # letters and '-' only: the reference's own CPU counts
+        np.testing.assert_array_equal(dist[iu], golden[f"{case}_d0"].astype(np.uint32))
+        np.testing.assert_array_equal(gaps[iu], (P - golden[f"{case}_nd2"]).astype(np.uint32))
+
+
+def test_reference_protein_gpu_abi_large(N):
+    """A 500-taxon, 3000-site alignment with every gap character, '*' and lowercase through the
+    same ABI, against the oracle's restatement (sampled rows)."""
+    aln = protein_alignment(500, 3000, seed=77, alphabet=AMINO + b"*acd", gap=0.1)
+    staged, stride, P = O.protein_staging(aln)
+    dist, gaps = _dormant_protein_abi(N, staged, 500, stride, P)
+    rows = staged.view(np.uint8).reshape(500, -1)[:, :P]
+    for i in (0, 1, 250, 498):
+        for j in range(i + 1, 500, 7):
+            assert (dist[i, j], gaps[i, j]) == O.protein_gpu_counts(rows[i].tobytes(),
+                                                                     rows[j].tobytes())
+
+
 @pytest.mark.parametrize("n,L", [(300, 1000), (257, 1024), (130, 4096 + 77), (64, 33)])
 def test_oracle_sizes(ctx, n, L):
     aln = protein_alignment(n, L, seed=n + L)

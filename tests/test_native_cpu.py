@@ -36,7 +36,8 @@ def test_header_declares_the_reference_boundary():
     names = set(declared_functions())
     for ref in ("initialiseDNA2_gpu", "storeDataDNA2_gpu", "computeDistancesDNA2_gpu",
                 "initialiseDNA_gpu", "storeDataDNA_gpu", "computeDistancesDNA_gpu",
-                "getResultsDNA_gpu"):
+                "getResultsDNA_gpu", "initialiseProtein_gpu", "computeDistancesProtein_gpu",
+                "getResultsProtein_gpu"):
         assert ref in names
 
 

@@ -75,6 +75,32 @@ def test_hamming_brute_force():
         assert O.count_protein_naive(a, b) == (sum(x != y for x, y in both), 0, len(both))
 
 
+@pytest.mark.parametrize("case", CASES)
+def test_dormant_kernel_counts_against_reference(golden, case):
+    """The restatement of the reference's dormant protein kernel (a byte below 64 is a gap) on
+    the rows dataloader::initGPUData stages: for pairs whose resolved rows hold no byte below 64
+    but '-', its mismatches are bitDistanceProtein's data[0] and its gap positions paddedLength
+    minus the Hamming counter's residue pairs, both written by the reference."""
+    aln = golden[f"{case}_aln"]
+    n, L = aln.shape
+    staged, stride, P = O.protein_staging(aln)
+    assert P == -(-L // 128) * 128 and stride * 4 >= P
+    rows = staged.view(np.uint8).reshape(n, -1)[:, :P]
+    res = aln.copy()
+    res[np.isin(res, np.frombuffer(b"-.XxzZbBJj?", dtype=np.uint8))] = ord("-")
+    clean = [not ((r < 64) & (r != ord("-"))).any() for r in res]
+    k = checked = 0
+    for i in range(n):
+        for j in range(i + 1, n):
+            r, gp = O.protein_gpu_counts(rows[i].tobytes(), rows[j].tobytes())
+            assert 0 <= r + gp <= P
+            if clean[i] and clean[j]:
+                assert r == golden[f"{case}_d0"][k] and gp == P - golden[f"{case}_nd2"][k]
+                checked += 1
+            k += 1
+    assert checked >= 3
+
+
 def test_regimes_covered(golden):
     """The fixtures reach every branch: the Dayhoff table, the 10.0 cap, saturated and empty
     pairs, and the wrapped valid count."""
