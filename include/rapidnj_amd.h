@@ -47,7 +47,7 @@ enum {
   RNJ_E_INVALID = 1,    /* bad argument (null pointer, n < 2, unknown model, ...) */
   RNJ_E_HIP = 2,        /* a HIP runtime call failed */
   RNJ_E_NOMEM = 3,      /* device or pinned host allocation failed */
-  RNJ_E_NOT_DNA = 4,    /* a DNA-only operation (bootstrap, served jobs) on protein input */
+  RNJ_E_NOT_DNA = 4,    /* a DNA-only operation (served jobs, character updates) on protein */
   RNJ_E_ADMISSION = 5,  /* a served job can never fit the server's HBM budget */
   RNJ_E_CANCELLED = 6,  /* a served job was cancelled */
   RNJ_E_NOT_READY = 7,  /* a served job has not finished (rnj_server_poll) */
@@ -173,9 +173,9 @@ RNJ_EXPORT int rnj_set_input_type(rnj_context* ctx, int type);
  *    alignments are packed with a fourth bit plane;
  *  - protein, calculateHammingDistance (simpleDistanceCalculator.cpp:40-55): the valid count is
  *    the exact number of residue pairs (no 128-position shortfall when L is a multiple of 128).
- * Naive-rule alignments are accepted by the distance, count, hand-off and batch-plan calls;
- * bootstrap, served jobs and character updates take bit-rule alignments only (served jobs fail
- * with RNJ_E_INVALID while the server's context has naive rules set). */
+ * Naive-rule alignments are accepted by the distance, count, hand-off, bootstrap and batch-plan
+ * calls; served jobs and character updates take bit-rule alignments only (served jobs fail with
+ * RNJ_E_INVALID while the server's context has naive rules set). */
 enum { RNJ_CHARS_BITS = 0, RNJ_CHARS_NAIVE = 1 };
 RNJ_EXPORT int rnj_set_char_rules(rnj_context* ctx, int rules);
 
@@ -298,11 +298,15 @@ typedef int (*rnj_row_sink)(void* user, unsigned row, const float* values, unsig
 RNJ_EXPORT int rnj_distances_stream_rows(rnj_context* ctx, const rnj_alignment* aln, int model,
                                          rnj_row_sink sink, void* user, float* max_out);
 
-/* Batched bootstrap (DNA alignments; RNJ_E_NOT_DNA for protein): R replicates given as
- * column-weight vectors weights[r*L + c] (host u16;
- * replicate r repeats column c weights[r][c] times, the expanded alignment of
- * dataloader::sample_fast_dist, dataloader.cpp:16-58). All replicates run in one tile launch.
- * out holds R matrices, matrix r at out + r*matrix_stride floats (layout as above). */
+/* Batched bootstrap: R replicates given as column-weight vectors weights[r*L + c] (host u16;
+ * replicate r repeats column c weights[r][c] times, the expanded alignment of the reference's
+ * resampling: dataloader::sample_fast_dist for DNA with 2 bits and protein with 8 bits per site,
+ * sample_char for the `-f` character path, dataloader.cpp:16-69). Counts do not depend on the
+ * column order, so replicate r equals the reference's result on its resampled alignment. DNA
+ * under either character rules and protein alignments; a protein replicate of Lr = sum of its
+ * weights sites counts as an alignment of Lr sites (for the multinomial case Lr = L). All
+ * replicates run in one tile launch. out holds R matrices, matrix r at out + r*matrix_stride
+ * floats (layout as above). */
 RNJ_EXPORT int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
                          const uint16_t* weights, unsigned R, int layout, float* out,
                          size_t matrix_stride, size_t ld, int flags, void* stream);

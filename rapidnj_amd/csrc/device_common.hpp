@@ -270,6 +270,21 @@ __device__ __forceinline__ bool job_tile(const JobDesc& J, unsigned tile, unsign
   return lower;
 }
 
+// In-register transpose of a 32 x 32 bit matrix, bit c of a[r] <-> bit r of a[c] (recursive
+// swap of the off-diagonal blocks: 16 x 16, then 8 x 8 inside each, ... down to 1 x 1).
+__device__ __forceinline__ void transpose32(uint32_t (&a)[32]) {
+#pragma unroll
+  for (int j = 16, m = 0x0000FFFF; j != 0; j >>= 1, m ^= m << j) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      if (k & j) continue;
+      const uint32_t t = ((a[k] >> j) ^ a[k + j]) & (uint32_t)m;
+      a[k] ^= t << j;
+      a[k + j] ^= t;
+    }
+  }
+}
+
 // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
 // items so that tiles sharing a block row also share that XCD's L2.
 __device__ __forceinline__ unsigned xcd_item(unsigned bid, unsigned grid) {

@@ -71,7 +71,10 @@ struct JobDesc {
   uint32_t* sat_mask;         // [tiles * waves per work-group][64]: micro-tile masks, flagged waves
   const uint32_t* gaps;       // protein: gap sites per taxon [n_pad]
   unsigned L;                 // protein: alignment length (sites)
-  unsigned valid_bias;        // protein: subtracted from the residue pairs (128 or 0)
+  // protein: 1 for bitDistanceProtein's valid count (128 short of the residue pairs when the
+  // length is a multiple of 128), 0 for the residue pairs
+  unsigned bias_rule;
+  const unsigned* len_dev;    // protein: the length in device memory (bootstrap replicates), or null
 };
 
 // Upper-triangle position of pair (i, j), i < j: row i holds j = i+1 .. n-1 (row-major).
@@ -136,16 +139,31 @@ hipError_t launch_saturation(const JobDesc* jobs, unsigned njobs, unsigned long 
                              int config, bool patch, hipStream_t stream);
 // Wave units of a job of `tiles` pair tiles in configuration `config` (one saturation word each).
 unsigned long long sat_units(unsigned long long tiles, int config);
-// Column gathers of bootstrap replicates read per-site codes: one nibble per taxon (V<<2|H<<1|L),
-// two taxa per byte, site-major: codes_bytes(n_pad, L) bytes.
+// Column gathers of bootstrap replicates read per-site codes: DNA one nibble per taxon
+// (U<<3|V<<2|H<<1|L, U only under the naive rules), two taxa per byte, site-major:
+// codes_bytes(n_pad, L) bytes; protein one byte per taxon, the residue code (0 = gap):
+// protein_codes_bytes(n_pad, L).
 inline size_t codes_bytes(unsigned n_pad, unsigned L) { return (size_t)L * n_pad / 2; }
+inline size_t protein_codes_bytes(unsigned n_pad, unsigned L) { return (size_t)L * n_pad; }
 hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
-                                  unsigned L, uint32_t* codes, hipStream_t stream);
+                                  unsigned L, uint32_t* codes, hipStream_t stream,
+                                  bool naive = false);
 hipError_t launch_bootstrap_index(const uint16_t* weights, unsigned R, unsigned L,
                                   unsigned* col_index, unsigned idx_stride, unsigned* lengths,
                                   hipStream_t stream);
+// naive: four planes per replicate (H, L, V, U) instead of three.
 hipError_t launch_bootstrap_gather(const uint32_t* codes, unsigned n_pad, const unsigned* col_index,
                                    unsigned idx_stride, const unsigned* lengths, unsigned R,
-                                   unsigned W_pad, uint32_t* planes, hipStream_t stream);
+                                   unsigned W_pad, uint32_t* planes, hipStream_t stream,
+                                   bool naive = false);
+// Protein (protein.hip): per-site code bytes of an alignment of 1 + B planes, and the replicate
+// gather: replicate r at planes + r * rep_words, its 1 + B planes of W_pad words, then its
+// per-taxon gap counts [n_pad] (recounted over the replicate's sites).
+hipError_t launch_protein_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
+                                unsigned L, unsigned B, uint8_t* codes, hipStream_t stream);
+hipError_t launch_protein_gather(const uint8_t* codes, unsigned n_pad, unsigned B,
+                                 const unsigned* col_index, unsigned idx_stride,
+                                 const unsigned* lengths, unsigned R, unsigned W_pad,
+                                 uint32_t* planes, size_t rep_words, hipStream_t stream);
 
 }  // namespace rnj

@@ -522,10 +522,11 @@ __global__ __launch_bounds__(256) void patch_kernel(const JobDesc* __restrict__ 
   }
 }
 
-// codes[s][t] = V << 2 | H << 1 | L for site s < L of taxon t.
 // Per-site codes of every taxon for the column gathers: nibbles, two taxa per byte, site-major
-// (codes[s][t / 2], taxon t in the low nibble when even), nibble = V << 2 | H << 1 | L. One
-// thread per (site, 8 taxa): one u32 of output.
+// (codes[s][t / 2], taxon t in the low nibble when even), nibble = U << 3 | V << 2 | H << 1 | L
+// (U, the naive rules' fourth plane, only when naive). One thread per (site, 8 taxa): one u32 of
+// output.
+template <bool NAIVE>
 __global__ void planes_to_codes_kernel(const uint32_t* __restrict__ planes, unsigned n_pad,
                                        unsigned W_pad, unsigned L, uint32_t* __restrict__ codes) {
   const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;  // group of 8 taxa
@@ -541,7 +542,8 @@ __global__ void planes_to_codes_kernel(const uint32_t* __restrict__ planes, unsi
       const uint32_t h = (planes[o + j] >> sh) & 1u;
       const uint32_t l = (planes[ps + o + j] >> sh) & 1u;
       const uint32_t v = (planes[2 * ps + o + j] >> sh) & 1u;
-      x |= (v << 2 | h << 1 | l) << (4 * j);
+      const uint32_t u = NAIVE ? (planes[3 * ps + o + j] >> sh) & 1u : 0u;
+      x |= (u << 3 | v << 2 | h << 1 | l) << (4 * j);
     }
     codes[(size_t)s * groups + g] = x;
   }
@@ -586,25 +588,13 @@ __global__ __launch_bounds__(1024) void bootstrap_index_kernel(
   if (tid == 0) lengths[r] = carry_s;
 }
 
-// In-register transpose of a 32 x 32 bit matrix, bit c of a[r] <-> bit r of a[c] (recursive
-// swap of the off-diagonal blocks: 16 x 16, then 8 x 8 inside each, ... down to 1 x 1).
-__device__ __forceinline__ void transpose32(uint32_t (&a)[32]) {
-#pragma unroll
-  for (int j = 16, m = 0x0000FFFF; j != 0; j >>= 1, m ^= m << j) {
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-      if (k & j) continue;
-      const uint32_t t = ((a[k] >> j) ^ a[k + j]) & (uint32_t)m;
-      a[k] ^= t << j;
-      a[k + j] ^= t;
-    }
-  }
-}
 
 // Replicate r's packed planes from the expanded column list: target site s takes source column
 // col_index[r][s]. Sites at or beyond lengths[r] are zero (not counted). A thread takes 8 taxa
 // and one 32-site word: 32 u32 loads (one per site, the 8 taxa's nibbles), a bit transpose that
-// turns them into 32 words of site bits (word 4j + p: taxon j's plane p), and 6 16-byte stores.
+// turns them into 32 words of site bits (word 4j + p: taxon j's plane p), and 6 16-byte stores
+// (8 with the naive rules' U plane, word 4j + 3).
+template <bool NAIVE>
 __global__ __launch_bounds__(64) void bootstrap_gather_kernel(
     const uint32_t* __restrict__ codes, unsigned n_pad, const unsigned* __restrict__ col_index,
     unsigned idx_stride, const unsigned* __restrict__ lengths, unsigned W_pad,
@@ -616,7 +606,7 @@ __global__ __launch_bounds__(64) void bootstrap_gather_kernel(
   const unsigned len = min(lengths[r], idx_stride);
   const unsigned* idx = col_index + (size_t)r * idx_stride;
   const size_t ps = (size_t)W_pad * n_pad;
-  uint32_t* pr = planes + (size_t)r * 3 * ps;
+  uint32_t* pr = planes + (size_t)r * (NAIVE ? 4 : 3) * ps;
   for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
     uint32_t a[32];
 #pragma unroll
@@ -636,6 +626,11 @@ __global__ __launch_bounds__(64) void bootstrap_gather_kernel(
     l[1] = make_uint4(a[16], a[20], a[24], a[28]);
     v[0] = make_uint4(a[2], a[6], a[10], a[14]);
     v[1] = make_uint4(a[18], a[22], a[26], a[30]);
+    if constexpr (NAIVE) {
+      uint4* u = reinterpret_cast<uint4*>(pr + 3 * ps + o);
+      u[0] = make_uint4(a[3], a[7], a[11], a[15]);
+      u[1] = make_uint4(a[19], a[23], a[27], a[31]);
+    }
   }
 }
 
@@ -772,10 +767,11 @@ hipError_t launch_saturation(const JobDesc* jobs, unsigned njobs, unsigned long 
 }
 
 hipError_t launch_planes_to_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
-                                  unsigned L, uint32_t* codes, hipStream_t stream) {
+                                  unsigned L, uint32_t* codes, hipStream_t stream, bool naive) {
   const unsigned groups = n_pad / 8;
-  planes_to_codes_kernel<<<dim3((groups + 255) / 256, ygrid(L)), dim3(256), 0, stream>>>(
-      planes, n_pad, W_pad, L, codes);
+  const dim3 grid((groups + 255) / 256, ygrid(L)), block(256);
+  if (naive) planes_to_codes_kernel<true><<<grid, block, 0, stream>>>(planes, n_pad, W_pad, L, codes);
+  else planes_to_codes_kernel<false><<<grid, block, 0, stream>>>(planes, n_pad, W_pad, L, codes);
   return hipGetLastError();
 }
 
@@ -789,10 +785,16 @@ hipError_t launch_bootstrap_index(const uint16_t* weights, unsigned R, unsigned 
 
 hipError_t launch_bootstrap_gather(const uint32_t* codes, unsigned n_pad, const unsigned* col_index,
                                    unsigned idx_stride, const unsigned* lengths, unsigned R,
-                                   unsigned W_pad, uint32_t* planes, hipStream_t stream) {
+                                   unsigned W_pad, uint32_t* planes, hipStream_t stream,
+                                   bool naive) {
   const unsigned groups = n_pad / 8;
-  bootstrap_gather_kernel<<<dim3((groups + 63) / 64, ygrid(W_pad), R), dim3(64), 0, stream>>>(
-      codes, n_pad, col_index, idx_stride, lengths, W_pad, planes);
+  const dim3 grid((groups + 63) / 64, ygrid(W_pad), R), block(64);
+  if (naive)
+    bootstrap_gather_kernel<true><<<grid, block, 0, stream>>>(codes, n_pad, col_index, idx_stride,
+                                                              lengths, W_pad, planes);
+  else
+    bootstrap_gather_kernel<false><<<grid, block, 0, stream>>>(codes, n_pad, col_index,
+                                                               idx_stride, lengths, W_pad, planes);
   return hipGetLastError();
 }
 

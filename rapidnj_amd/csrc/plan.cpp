@@ -28,8 +28,9 @@ struct rnj_plan {
   uint16_t* d_weights = nullptr;
   unsigned* d_idx = nullptr;
   unsigned* d_len = nullptr;
-  uint32_t* d_codes = nullptr;
+  void* d_codes = nullptr;
   uint32_t* d_planes = nullptr;
+  size_t rep_words = 0;
   Workspace ws;
   std::vector<JobSpec> specs;
   LaunchShape shape;
@@ -71,8 +72,8 @@ int enqueue_plan(rnj_plan* p, int buf, bool upload_table) {
   RNJ_HIP(hipMemcpyAsync(p->d_weights, p->h_weights[buf], (size_t)p->R * p->L * sizeof(uint16_t),
                          hipMemcpyHostToDevice, p->stream));
   RNJ_HIP(launch_bootstrap_index(p->d_weights, p->R, p->L, p->d_idx, p->Lmax, p->d_len, p->stream));
-  RNJ_HIP(launch_bootstrap_gather(p->d_codes, a->n_pad, p->d_idx, p->Lmax, p->d_len, p->R, p->Wr,
-                                  p->d_planes, p->stream));
+  RNJ_HIP(launch_replicate_gather(a, p->d_codes, p->d_idx, p->Lmax, p->d_len, p->R, p->Wr,
+                                  p->d_planes, p->rep_words, p->stream));
   return enqueue_launch(p->ctx, p->specs, p->shape, p->config, &p->ws, p->raw, p->exact,
                         p->stream, upload_table);
 }
@@ -98,8 +99,6 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
                               size_t ld, int flags, rnj_plan** plan) {
   if (!ctx || !aln || !out || !plan) return fail(RNJ_E_INVALID, "null argument");
   if (aln->ctx != ctx) return fail(RNJ_E_INVALID, "alignment belongs to another context");
-  if (aln->kind != kDNA)
-    return fail(RNJ_E_NOT_DNA, "bootstrap replicates are built for DNA alignments only");
   if (model != RNJ_JC && model != RNJ_K2P) return fail(RNJ_E_INVALID, "model must be RNJ_JC or RNJ_K2P");
   if (layout != RNJ_TRIANGLE && layout != RNJ_SQUARE) return fail(RNJ_E_INVALID, "unknown layout");
   if (R == 0) return fail(RNJ_E_INVALID, "a plan needs at least one replicate");
@@ -118,7 +117,7 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
   p->Wr = words_for_sites(p->Lmax);
   p->raw = (flags & RNJ_FLAG_RAW) != 0;
   p->exact = (flags & RNJ_FLAG_EXACT) != 0;
-  const size_t plane_words = (size_t)3 * p->Wr * aln->n_pad;
+  p->rep_words = replicate_words(aln, p->Wr);
   hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
   for (int k = 0; k < 2; ++k) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done[k], hipEventDisableTiming);
@@ -129,28 +128,18 @@ int rnj_bootstrap_plan_create(rnj_context* ctx, const rnj_alignment* aln, int mo
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_weights, (size_t)R * L * sizeof(uint16_t));
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_idx, (size_t)R * p->Lmax * sizeof(unsigned));
   if (e == hipSuccess) e = hipMalloc((void**)&p->d_len, (size_t)R * sizeof(unsigned));
-  if (e == hipSuccess) e = hipMalloc((void**)&p->d_codes, codes_bytes(aln->n_pad, L));
-  if (e == hipSuccess) e = hipMalloc((void**)&p->d_planes, (size_t)R * plane_words * 4);
-  if (e == hipSuccess)
-    e = launch_planes_to_codes(aln->planes, aln->n_pad, aln->W, L, p->d_codes, p->stream);
+  if (e == hipSuccess) e = hipMalloc(&p->d_codes, alignment_codes_bytes(aln));
+  if (e == hipSuccess) e = hipMalloc((void**)&p->d_planes, (size_t)R * p->rep_words * 4);
+  if (e == hipSuccess) e = launch_alignment_codes(aln, p->d_codes, p->stream);
   if (e != hipSuccess) {
     destroy_plan(p);
     return hip_fail(e, "rnj_bootstrap_plan_create");
   }
-  for (unsigned r = 0; r < R; ++r) {
-    JobSpec j{};
-    j.planes = p->d_planes + (size_t)r * plane_words;
-    j.n = n;
-    j.n_pad = aln->n_pad;
-    j.W_pad = p->Wr;
-    j.W = p->Wr;
-    j.W_live = (p->Lmax + 31) / 32;  // every replicate holds at most Lmax sites
-    j.model = model;
-    j.layout = layout;
-    j.ld = ld;
-    j.out = out + (size_t)r * matrix_stride;
-    p->specs.push_back(j);
-  }
+  for (unsigned r = 0; r < R; ++r)  // every replicate holds at most Lmax sites; a protein
+    // replicate's own length is read on the device, as it changes with the weights of each run
+    p->specs.push_back(replicate_spec(aln, p->d_planes + (size_t)r * p->rep_words, p->Wr, p->Lmax,
+                                      p->d_len + r, model, layout, out + (size_t)r * matrix_stride,
+                                      ld));
   p->shape = shape_of(p->specs);
   int rc = prepare_workspace(&p->ws, R);
   // the original alignment as every replicate: a valid eager run that settles the configuration

@@ -118,9 +118,10 @@ __device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long l
 // Thread (ty, tx) of tile (bi, bj) holds x (differing sites) and g (gap-gap sites) of its
 // MI x MJ pairs. The reference walks 128 * (L/128 + 1) positions and counts the padding past L
 // as gaps, then subtracts from L rounded up to 128 (bitDistanceProtein.cpp:31-61): its valid
-// count is the residue pairs, less 128 when L is a multiple of 128 (modulo 2^64). J.valid_bias
-// holds that 128 (or 0; always 0 for the -f path's Hamming counter, which counts the residue
-// pairs, simpleDistanceCalculator.cpp:40-55).
+// count is the residue pairs, less 128 when L is a multiple of 128 (modulo 2^64). J.bias_rule
+// selects that shortfall (it is 0 for the -f path's Hamming counter, which counts the residue
+// pairs, simpleDistanceCalculator.cpp:40-55); a bootstrap replicate's length is read from
+// J.len_dev.
 template <int TILE, int MI, int MJ, bool kExact>
 __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile, unsigned bi,
                                                  unsigned bj, unsigned ty, unsigned tx,
@@ -128,7 +129,8 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile
                                                  const uint32_t (&acc_x)[MI][MJ],
                                                  const uint32_t (&acc_g)[MI][MJ]) {
   const unsigned n = J.n;
-  const unsigned long long L = J.L, bias = J.valid_bias;
+  const unsigned long long L = J.len_dev ? *J.len_dev : J.L;
+  const unsigned long long bias = (J.bias_rule && L % 128u == 0) ? 128ull : 0ull;
   uint32_t gi[MI], gj[MJ];
 #pragma unroll
   for (int a = 0; a < MI; ++a) gi[a] = J.gaps[bi * TILE + ty * MI + a];
@@ -447,6 +449,86 @@ __global__ __launch_bounds__(256) void pack_protein_kernel(
   if (t < n && gap_sites) atomicAdd(&gaps[t], gap_sites);
 }
 
+// Per-site code bytes of a protein alignment for the column gathers: codes[s][t], the residue
+// code of taxon t at site s (0 for a gap), one thread per (site, 4 taxa), one u32 of output.
+__global__ void protein_codes_kernel(const uint32_t* __restrict__ planes, unsigned n_pad,
+                                     unsigned W_pad, unsigned L, unsigned B,
+                                     uint32_t* __restrict__ codes) {
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;  // group of 4 taxa
+  const unsigned groups = n_pad / 4;
+  if (g >= groups) return;
+  const size_t ps = (size_t)W_pad * n_pad;
+  for (unsigned s = blockIdx.y; s < L; s += gridDim.y) {
+    const size_t o = (size_t)(s / 32) * n_pad + 4 * g;
+    const unsigned sh = s % 32;
+    uint32_t x = 0;
+    for (unsigned j = 0; j < 4; ++j) {
+      uint32_t c = 0;
+      for (unsigned p = 0; p < B; ++p) c |= ((planes[(1 + p) * ps + o + j] >> sh) & 1u) << p;
+      x |= c << (8 * j);
+    }
+    codes[(size_t)s * groups + g] = x;
+  }
+}
+
+// Gap counts of every replicate to zero (they are summed by atomics in the gather).
+__global__ void protein_zero_gaps_kernel(uint32_t* __restrict__ planes, size_t rep_words,
+                                         size_t gaps_off, unsigned n_pad) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_pad) planes[(size_t)blockIdx.y * rep_words + gaps_off + t] = 0u;
+}
+
+// Replicate r's planes from the expanded column list (target site s takes source column
+// col_index[r][s]; sites at or beyond lengths[r] are code 0 without the gap bit). A thread takes
+// 4 taxa and a range of 32-site words: per word, 32 u32 loads (one per site, the 4 taxa's code
+// bytes), a bit transpose (word 8j + p: bit p of taxon j's codes), the B code planes and the gap
+// plane (code 0 at a live site) stored 16 bytes at a time, and the taxa's gap sites summed; one
+// atomic add per taxon at the end.
+__global__ __launch_bounds__(64) void protein_gather_kernel(
+    const uint32_t* __restrict__ codes, unsigned n_pad, unsigned B,
+    const unsigned* __restrict__ col_index, unsigned idx_stride,
+    const unsigned* __restrict__ lengths, unsigned W_pad, uint32_t* __restrict__ planes,
+    size_t rep_words) {
+  const unsigned groups = n_pad / 4;
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned r = blockIdx.z;
+  if (g >= groups) return;
+  const unsigned len = min(lengths[r], idx_stride);
+  const unsigned* idx = col_index + (size_t)r * idx_stride;
+  const size_t ps = (size_t)W_pad * n_pad;
+  uint32_t* pr = planes + (size_t)r * rep_words;
+  uint32_t gap_sum[4] = {0, 0, 0, 0};
+  for (unsigned w = blockIdx.y; w < W_pad; w += gridDim.y) {
+    uint32_t a[32];
+#pragma unroll
+    for (unsigned k = 0; k < 32; ++k) {
+      const unsigned s = w * 32 + k;
+      a[k] = s < len ? codes[(size_t)idx[s] * groups + g] : 0u;
+    }
+    transpose32(a);
+    const unsigned live = w * 32 >= len ? 0u : (len - w * 32 >= 32 ? 32u : len - w * 32);
+    const uint32_t live_mask = live == 32 ? ~0u : ((1u << live) - 1u);
+    const size_t o = (size_t)w * n_pad + 4 * g;
+    uint32_t gap[4];
+#pragma unroll
+    for (unsigned j = 0; j < 4; ++j) {
+      uint32_t any = 0;
+#pragma unroll
+      for (unsigned p = 0; p < 8; ++p) any |= a[8 * j + p];
+      gap[j] = ~any & live_mask;
+      gap_sum[j] += __popc(gap[j]);
+    }
+    *reinterpret_cast<uint4*>(pr + o) = make_uint4(gap[0], gap[1], gap[2], gap[3]);
+    for (unsigned p = 0; p < B; ++p)
+      *reinterpret_cast<uint4*>(pr + (1 + p) * ps + o) =
+          make_uint4(a[p], a[8 + p], a[16 + p], a[24 + p]);
+  }
+  uint32_t* gaps = pr + (1 + B) * ps;
+#pragma unroll
+  for (unsigned j = 0; j < 4; ++j)
+    if (gap_sum[j]) atomicAdd(&gaps[4 * g + j], gap_sum[j]);
+}
+
 }  // namespace
 
 // ---- launchers ---------------------------------------------------------------------------------
@@ -480,6 +562,37 @@ hipError_t launch_protein_tiles(int config, unsigned B, bool exact, const JobDes
     case 8: return launch_protein_b<8>(config, exact, jobs, njobs, total_items, finish_items, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+}  // namespace rnj
+
+namespace rnj {
+
+hipError_t launch_protein_codes(const uint32_t* planes, unsigned n_pad, unsigned W_pad,
+                                unsigned L, unsigned B, uint8_t* codes, hipStream_t stream) {
+  const unsigned groups = n_pad / 4;
+  const unsigned ys = L < 1 ? 1 : (L > 65535 ? 65535 : L);
+  protein_codes_kernel<<<dim3((groups + 255) / 256, ys), dim3(256), 0, stream>>>(
+      planes, n_pad, W_pad, L, B, reinterpret_cast<uint32_t*>(codes));
+  return hipGetLastError();
+}
+
+hipError_t launch_protein_gather(const uint8_t* codes, unsigned n_pad, unsigned B,
+                                 const unsigned* col_index, unsigned idx_stride,
+                                 const unsigned* lengths, unsigned R, unsigned W_pad,
+                                 uint32_t* planes, size_t rep_words, hipStream_t stream) {
+  if (R == 0) return hipSuccess;
+  const size_t gaps_off = (size_t)(1 + B) * W_pad * n_pad;
+  protein_zero_gaps_kernel<<<dim3((n_pad + 255) / 256, R), dim3(256), 0, stream>>>(
+      planes, rep_words, gaps_off, n_pad);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const unsigned groups = n_pad / 4;
+  const unsigned ys = W_pad < 16 ? W_pad : 16;  // word ranges per taxon group: few atomics
+  protein_gather_kernel<<<dim3((groups + 63) / 64, ys, R), dim3(64), 0, stream>>>(
+      reinterpret_cast<const uint32_t*>(codes), n_pad, B, col_index, idx_stride, lengths, W_pad,
+      planes, rep_words);
+  return hipGetLastError();
 }
 
 }  // namespace rnj

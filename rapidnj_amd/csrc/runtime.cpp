@@ -227,7 +227,8 @@ void describe_job(JobDesc& d, const JobSpec& s, int config, const JobPlan& jp,
   d.out_valid = s.out_valid;
   d.gaps = s.gaps;
   d.L = s.L;
-  d.valid_bias = s.valid_bias;
+  d.bias_rule = s.bias_rule;
+  d.len_dev = s.len_dev;
   d.maxbits = maxbits;
   d.sat_word = sat + sat_off;
   d.sat_tile = sat + cap_units + sat_off;
@@ -634,7 +635,7 @@ JobSpec rnj::alignment_spec(const Alignment* a, int model, int layout, float* ou
   // bitDistanceProtein walks 128 * (L/128 + 1) positions and subtracts from L rounded up to 128,
   // so its valid count is 128 short when L is a multiple of 128 (bitDistanceProtein.cpp:31-61);
   // the Hamming counter of the -f path counts the residue pairs exactly
-  s.valid_bias = (a->kind == kProtein && !a->hamming && a->L % 128u == 0) ? 128u : 0u;
+  s.bias_rule = (a->kind == kProtein && !a->hamming) ? 1u : 0u;
   s.planes = a->planes;
   s.n = a->n;
   s.n_pad = a->n_pad;
@@ -653,6 +654,57 @@ JobSpec spec_for(const rnj_alignment* a, int model, int layout, float* out, size
   return alignment_spec(a, model, layout, out, ld);
 }
 }  // namespace
+
+size_t rnj::alignment_codes_bytes(const Alignment* a) {
+  return a->kind == kProtein ? protein_codes_bytes(a->n_pad, a->L) : codes_bytes(a->n_pad, a->L);
+}
+
+size_t rnj::replicate_words(const Alignment* a, unsigned Wr) {
+  return ((size_t)a->nplanes() * Wr + (a->kind == kProtein ? 1 : 0)) * a->n_pad;
+}
+
+hipError_t rnj::launch_alignment_codes(const Alignment* a, void* codes, hipStream_t s) {
+  if (a->kind == kProtein)
+    return launch_protein_codes(a->planes, a->n_pad, a->W, a->L, a->code_bits, (uint8_t*)codes, s);
+  return launch_planes_to_codes(a->planes, a->n_pad, a->W, a->L, (uint32_t*)codes, s,
+                                a->kind == kNaive);
+}
+
+hipError_t rnj::launch_replicate_gather(const Alignment* a, const void* codes,
+                                        const unsigned* col_index, unsigned idx_stride,
+                                        const unsigned* lengths, unsigned R, unsigned Wr,
+                                        uint32_t* planes, size_t rep_words, hipStream_t s) {
+  if (a->kind == kProtein)
+    return launch_protein_gather((const uint8_t*)codes, a->n_pad, a->code_bits, col_index,
+                                 idx_stride, lengths, R, Wr, planes, rep_words, s);
+  // the DNA gather lays replicates out back to back, rep_words apart
+  return launch_bootstrap_gather((const uint32_t*)codes, a->n_pad, col_index, idx_stride, lengths,
+                                 R, Wr, planes, s, a->kind == kNaive);
+}
+
+JobSpec rnj::replicate_spec(const Alignment* a, const uint32_t* planes, unsigned Wr, unsigned len,
+                            const unsigned* len_dev, int model, int layout, float* out, size_t ld) {
+  JobSpec j{};
+  j.kind = a->kind;
+  j.code_bits = a->code_bits;
+  j.planes = planes;
+  j.n = a->n;
+  j.n_pad = a->n_pad;
+  j.W_pad = Wr;
+  j.W = std::min(words_for_sites(len), Wr);
+  j.W_live = (len + 31) / 32;
+  j.model = model;
+  j.layout = layout;
+  j.ld = ld;
+  j.out = out;
+  if (a->kind == kProtein) {
+    j.gaps = planes + (size_t)a->nplanes() * Wr * a->n_pad;
+    j.L = len;
+    j.len_dev = len_dev;
+    j.bias_rule = a->hamming ? 0u : 1u;
+  }
+  return j;
+}
 
 extern "C" {
 
@@ -1305,8 +1357,6 @@ int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
   int rc = check_aln(ctx, aln);
   if (rc == RNJ_OK) rc = check_model(model);
   if (rc != RNJ_OK) return rc;
-  if (aln->kind != kDNA)
-    return fail(RNJ_E_NOT_DNA, "bootstrap replicates are built for DNA alignments only");
   if (!weights || !out) return fail(RNJ_E_INVALID, "null argument");
   if (R == 0) return RNJ_OK;
   if (R > kMaxLaunchJobs) return fail(RNJ_E_INVALID, "more than 65535 replicates in one call");
@@ -1329,38 +1379,27 @@ int rnj_bootstrap_device(rnj_context* ctx, const rnj_alignment* aln, int model,
     maxlen = std::max(maxlen, len[r]);
   }
   const unsigned Wr = words_for_sites(maxlen);
-  const size_t plane_words = (size_t)3 * Wr * aln->n_pad;
+  const size_t rep_words = replicate_words(aln, Wr);
   uint16_t* dw = nullptr;
   unsigned *didx = nullptr, *dlen = nullptr;
-  uint32_t* dcodes = nullptr;
+  void* dcodes = nullptr;
   uint32_t* dplanes = nullptr;
   hipError_t e = hipMallocAsync((void**)&dw, (size_t)R * L * sizeof(uint16_t), s);
   if (e == hipSuccess) e = hipMallocAsync((void**)&didx, (size_t)R * maxlen * sizeof(unsigned), s);
   if (e == hipSuccess) e = hipMallocAsync((void**)&dlen, (size_t)R * sizeof(unsigned), s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&dcodes, codes_bytes(aln->n_pad, L), s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&dplanes, (size_t)R * plane_words * 4, s);
+  if (e == hipSuccess) e = hipMallocAsync(&dcodes, alignment_codes_bytes(aln), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dplanes, (size_t)R * rep_words * 4, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(dw, weights, (size_t)R * L * sizeof(uint16_t), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = launch_planes_to_codes(aln->planes, aln->n_pad, aln->W, L, dcodes, s);
+  if (e == hipSuccess) e = launch_alignment_codes(aln, dcodes, s);
   if (e == hipSuccess) e = launch_bootstrap_index(dw, R, L, didx, maxlen, dlen, s);
   if (e == hipSuccess)
-    e = launch_bootstrap_gather(dcodes, aln->n_pad, didx, maxlen, dlen, R, Wr, dplanes, s);
+    e = launch_replicate_gather(aln, dcodes, didx, maxlen, dlen, R, Wr, dplanes, rep_words, s);
   if (e == hipSuccess) {
     std::vector<JobSpec> jobs(R);
-    for (unsigned r = 0; r < R; ++r) {
-      JobSpec j{};
-      j.planes = dplanes + (size_t)r * plane_words;
-      j.n = n;
-      j.n_pad = aln->n_pad;
-      j.W_pad = Wr;
-      j.W = words_for_sites(len[r]);
-      j.W_live = (len[r] + 31) / 32;
-      j.model = model;
-      j.layout = layout;
-      j.ld = ld;
-      j.out = out + (size_t)r * matrix_stride;
-      jobs[r] = j;
-    }
+    for (unsigned r = 0; r < R; ++r)
+      jobs[r] = replicate_spec(aln, dplanes + (size_t)r * rep_words, Wr, len[r], nullptr, model,
+                               layout, out + (size_t)r * matrix_stride, ld);
     rc = launch_jobs(ctx, jobs, (flags & RNJ_FLAG_RAW) != 0, (flags & RNJ_FLAG_EXACT) != 0, s);
   } else {
     rc = hip_fail(e, "rnj_bootstrap_device");

@@ -142,7 +142,8 @@ struct JobSpec {
   unsigned code_bits;     // protein
   const uint32_t* gaps;   // protein: gap sites per taxon
   unsigned L;             // protein: sites
-  unsigned valid_bias;    // protein: 128 when bitDistanceProtein's valid count is 128 short
+  unsigned bias_rule;     // protein: 1 for bitDistanceProtein's valid count (JobDesc::bias_rule)
+  const unsigned* len_dev;  // protein: the sites in device memory (bootstrap replicates), or null
   const uint32_t* planes;
   unsigned n, n_pad, W_pad, W;
   unsigned W_live;  // words holding sites (0: W); the DNA tile kernel counts no word beyond
@@ -156,6 +157,20 @@ struct JobSpec {
 
 // The launch description of one alignment's matrix.
 JobSpec alignment_spec(const Alignment* a, int model, int layout, float* out, size_t ld);
+
+// Bootstrap replicates of an alignment (DNA under either character rules, or protein): per-site
+// codes of the alignment for the column gathers, the replicates' planes (replicate r at
+// planes + r * replicate_words(a, Wr), Wr words each; protein ones followed by their gap counts)
+// and the launch description of replicate r, of `len` sites (a protein replicate's length is read
+// from len_dev when that is set, the case of captured plans whose lengths change per run).
+size_t alignment_codes_bytes(const Alignment* a);
+size_t replicate_words(const Alignment* a, unsigned Wr);
+hipError_t launch_alignment_codes(const Alignment* a, void* codes, hipStream_t s);
+hipError_t launch_replicate_gather(const Alignment* a, const void* codes, const unsigned* col_index,
+                                   unsigned idx_stride, const unsigned* lengths, unsigned R,
+                                   unsigned Wr, uint32_t* planes, size_t rep_words, hipStream_t s);
+JobSpec replicate_spec(const Alignment* a, const uint32_t* planes, unsigned Wr, unsigned len,
+                       const unsigned* len_dev, int model, int layout, float* out, size_t ld);
 
 unsigned round_up(unsigned x, unsigned m);
 unsigned words_for_sites(unsigned sites);  // ceil(sites/32) rounded up to kWordPad

@@ -304,9 +304,40 @@ def test_naive_rules_every_config(ctx, N):
     finally:
         ctx.set_tile_config(-1)
     with pytest.raises(N.RnjError):
-        ctx.bootstrap(A, np.ones((2, L), dtype=np.uint16), "k2p")
-    with pytest.raises(N.RnjError):
         A.update(aln)
+
+
+def test_naive_bootstrap_replicates(ctx):
+    """The -f path's resampling (dataloader::sample_char, dataloader.cpp:60-69) on a mixed-case
+    alignment: every replicate, eager and captured, equals the naive run on its column-expanded
+    alignment bit for bit, and its counts are the oracle's naive counter on that alignment."""
+    n, L = 120, 700
+    aln = _mixed_case_alignment(n, L, 43)
+    w = bootstrap_weights(L, 4, seed0=7)
+    w[3] = 0
+    w[3, 5:40] = 2                                  # a short replicate (70 sites)
+    ctx.set_char_rules(True)
+    try:
+        A = ctx.alignment(aln)
+        expanded = [ctx.alignment(expand_columns(aln, w[r])) for r in range(4)]
+    finally:
+        ctx.set_char_rules(False)
+    iu = np.triu_indices(n, 1)
+    for model in ("jc", "k2p"):
+        got = _np(ctx.bootstrap(A, w, model, raw=True))
+        with ctx.bootstrap_plan(A, 4, model, max_length=L, raw=True) as plan:
+            cap = _np(plan.run(w))
+        for r in range(4):
+            ref = _np(ctx.distances(expanded[r], model, raw=True))
+            np.testing.assert_array_equal(got[r].view(np.uint32), ref.view(np.uint32))
+            np.testing.assert_array_equal(cap[r].view(np.uint32), ref.view(np.uint32))
+    rep = expand_columns(aln, w[1])
+    cnt = np.array([O.count_naive(rep[i].tobytes(), rep[j].tobytes()) for i, j in zip(*iu)],
+                   dtype=np.uint32).reshape(-1, 3)
+    tri, _, _ = O.correct_triangle(cnt[:, 0], cnt[:, 1], cnt[:, 2], n, "k2p", with_full=False)
+    got = _np(ctx.bootstrap(A, w, "k2p", raw=True))[1]
+    np.testing.assert_array_equal(got == -1, tri == -1)
+    np.testing.assert_allclose(got, tri, rtol=DIST_RTOL, atol=0)
 
 
 def test_bootstrap_matches_reference_on_expanded_columns(ctx):

@@ -300,15 +300,45 @@ def test_input_type(ctx, N):
     assert ctx.alignment(protein_alignment(5, 64, seed=3)).is_protein
 
 
+@pytest.mark.parametrize("n,L", [(90, 640), (130, 1000)])
+def test_protein_bootstrap_replicates(ctx, n, L):
+    """Protein resampling (dataloader::sample_fast_dist at 8 bits per site, dataloader.cpp:16-58):
+    every replicate, eager and captured, equals the protein run on its column-expanded alignment
+    bit for bit, in both models, including a short replicate and lengths that are multiples of
+    128 (bitDistanceProtein's valid count 128 short) and ones that are not; the Hamming counter's
+    replicates (naive rules) likewise."""
+    from rapidnj_amd.workloads import bootstrap_weights, expand_columns
+    aln = protein_alignment(n, L, seed=n + 5, alphabet=AMINO + b"*acd", gap=0.1)
+    w = bootstrap_weights(L, 5, seed0=11)
+    w[4] = 0
+    w[4, 10:74] = 2                                  # 128 sites: a multiple of 128
+    for naive in (False, True):
+        ctx.set_char_rules(naive)
+        try:
+            A = ctx.alignment(aln)
+            expanded = [ctx.alignment(expand_columns(aln, w[r])) for r in range(5)]
+        finally:
+            ctx.set_char_rules(False)
+        assert A.is_protein and all(e.is_protein for e in expanded)
+        for model in ("k2p", "jc"):
+            got = ctx.bootstrap(A, w, model, raw=True).cpu().numpy()
+            with ctx.bootstrap_plan(A, 5, model, max_length=L, raw=True) as plan:
+                for _ in range(2):
+                    cap = plan.run(w).cpu().numpy()
+            for r in range(5):
+                ref = ctx.distances(expanded[r], model, raw=True).cpu().numpy()
+                np.testing.assert_array_equal(got[r].view(np.uint32), ref.view(np.uint32))
+                np.testing.assert_array_equal(cap[r].view(np.uint32), ref.view(np.uint32))
+    # one replicate against the oracle's counter on its expanded alignment
+    rep = expand_columns(aln, w[4])
+    want, _ = O.protein_matrix(rep, "k2p")
+    A = ctx.alignment(aln)
+    got = ctx.bootstrap(A, w, "k2p", layout="square", exact=True).cpu().numpy()[4]
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def test_dna_only_operations_refuse_protein(ctx, N):
     aln = protein_alignment(8, 50, seed=4)
-    A = ctx.alignment(aln)
-    with pytest.raises(N.RnjError) as e:
-        ctx.bootstrap(A, np.ones((2, 50), dtype=np.uint16))
-    assert e.value.status == N.RNJ_E_NOT_DNA
-    with pytest.raises(N.RnjError) as e:
-        ctx.bootstrap_plan(A, 2)
-    assert e.value.status == N.RNJ_E_NOT_DNA
     srv = ctx.server()
     try:
         with pytest.raises(N.RnjError) as e:
