@@ -394,6 +394,7 @@ def nj_side(ctx, A, cpu: bool, sample_n: int = 5000) -> dict:
     t0 = time.perf_counter()
     tree = ctx.nj(d)
     out["device_s"] = round(time.perf_counter() - t0, 3)
+    out["host_searches"] = tree.host_searches  # iterations whose search the host settled
     tree.close()
     sub = d[:sample_n, :sample_n].contiguous()
     torch.cuda.synchronize()

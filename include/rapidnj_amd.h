@@ -343,10 +343,15 @@ RNJ_EXPORT int rnj_plan_destroy(rnj_plan* plan);
 /* ---- neighbour joining on the device ----------------------------------------------------------
  * rapidNJ::run (src/rapidNJ.cpp:45-88) with the matrix kept in HBM: the same joins in the same
  * order (the criterion's float arithmetic, rapidNJ's visiting order and row caches, redundant
- * copies, negative_branches), giving polytree's Newick text (src/polytree.cpp:402-458). For
- * exact ties in both distance and criterion within one row the reference's order is std::sort's;
- * here the partner with the lower index comes first. Replaces `new rapidNJ(reader, n,
- * negative_branches, pb)->run()` (src/main.cpp) for in-memory matrices. */
+ * copies, negative_branches), giving polytree's Newick text (src/polytree.cpp:402-458). Every
+ * live pair is evaluated on the device; the iterations where rapidNJ's sorted row lists decide
+ * the result -- several pairs of a winning row with equal criterion and distance (taken in
+ * std::sort's order of the row list), or a row whose scan rapidNJ ends by its rounded bound
+ * (src/rapidNJ.cpp:340) before a pair below the running minimum -- are detected on the device
+ * and searched on the host over the rows' lists as rapidNJ built and sorted them. Replaces
+ * `new rapidNJ(reader, n, negative_branches, pb)->run()` (src/main.cpp) for in-memory matrices.
+ * Device memory: two slot matrices of (1.5 n)^2 floats and a log of every join's new row
+ * (about 4.5 n^2 bytes). */
 typedef struct rnj_tree rnj_tree;
 /* matrix: n rows of ld floats in device memory (post-processed, symmetric, diagonal 0), read
  * only; enqueued on `stream` after the work already there, synchronous. n >= 3. */
@@ -362,6 +367,9 @@ RNJ_EXPORT int rnj_nj_rows(rnj_context* ctx, const float* const* rows, unsigned 
 RNJ_EXPORT int rnj_nj_alignment(rnj_context* ctx, const rnj_alignment* aln, int model,
                                 int negative_branches, rnj_tree** out);
 RNJ_EXPORT unsigned rnj_tree_size(const rnj_tree* tree); /* leaves */
+/* Iterations whose search the host settled (ties or the scan bound, above). No counterpart in
+ * the reference. */
+RNJ_EXPORT unsigned rnj_tree_host_searches(const rnj_tree* tree);
 /* Join k (0 <= k < n-2) joins clusters left[k] and right[k] (leaves 0..n-1, join k creates id
  * n+k) with branch lengths left_len[k], right_len[k] (polytree::addInternalNode); last[0..1] and
  * last_dist are the final two clusters and their distance. Any pointer may be NULL. */

@@ -2,26 +2,24 @@
  * (polytree.cpp). TEST INFRASTRUCTURE ONLY, like rnj_oracle.c: only tests/ and bench.py's CPU leg
  * load it, as the checker.
  *
- * rapidNJ keeps every row's partners sorted by distance and stops a row's scan at a bound; the
- * global minimum it finds is the minimum of the criterion over all live pairs, and the pair it
- * returns is the first one reaching it in its visiting order. This file restates that result
- * directly (every live pair evaluated, no sorted rows), with the reference's arithmetic and
- * visiting order:
- *   - rows are visited by index, cyclically from the row the previous search cached
- *     (rapidNJ.cpp:290-319), and a row's pairs in increasing distance (its sorted list, built by
- *     std::sort on the distance only, cluster_pair.h:8); pairs of one row with equal distance
- *     AND equal criterion are taken in increasing partner index here (the reference's order for
- *     such exact ties is std::sort's);
- *   - a pair (a, b) lives in the row of the cluster with the larger id, against the partner with
- *     the smaller id (rows are built from lower ids, new clusters get the next id), and its
- *     criterion is the float (d - sep[partner]) - sep[row] (rapidNJ.cpp:336);
- *   - a row with redundant copies first offers the copy join at -2 * sep[row]
- *     (rapidNJ.cpp:418-430);
- *   - the caches of the rows where the minimum was last improved follow rapidNJ.cpp:337-345.
+ * The search is restated with rapidNJ's own data: every row keeps the list of its partners (the
+ * live clusters of smaller id when it was built), sorted by distance alone with std::sort
+ * (cluster_pair.h:8; rapidNJ.cpp:141-153 for initial rows, :219-234 for new ones; the sort is
+ * rnj_oracle_sort.cpp's, std::sort itself, whose order among equal distances is not the
+ * construction order). findMin (rapidNJ.cpp:290-361) visits rows by index, cyclically from the
+ * row its previous search cached, offers a row's redundant-copy join first (:413-430), and scans
+ * the row's live pairs in list order until the first one whose
+ * float(distance - (sep[row] + maxRowSeparation[row])) is not below the running minimum (:340).
+ * The criterion is the float (d - sep[partner]) - sep[row] (:345). Both are kept to the bit:
+ * the break's rounding differs from the criterion's, so it can end a row one pair before a
+ * criterion one unit in the last place below the running minimum, and the reference then joins
+ * another pair. maxRowSeparations follow rapidNJ.cpp:114-119 (every smaller index, copies
+ * included) and :177-214 (the live clusters of smaller id, after each update).
  * Merging (rapidNJ.cpp:240-281), the update of distances and separations (rapidNJ.cpp:166-238, in
  * id order, with float arithmetic and the sequential float sum of the new separation sum),
  * redundancy detection (rapidNJ.cpp:91-138,155-164) and the last join (rapidNJ.cpp:52-85) are
- * restated step for step. Pinned against the reference's own trees (tests/test_nj_oracle.py,
+ * restated step for step. Garbage collection of dead pairs (rapidNJ.cpp:363-410) keeps the list
+ * order and is left out. Pinned against the reference's own trees (tests/test_nj_oracle.py,
  * tests/golden/nj.npz written by the compiled reference).
  */
 #include <float.h>
@@ -36,13 +34,19 @@ typedef struct {
   float* m;  /* n x n by index, updated in place */
   float* sepsum;
   float* sep;
+  float* maxrow;  /* maxRowSeparations by index */
   int *index_to_id, *id_to_index, *red_count, *rtm, *red_head, *red_next;
   int cluster_count, current_id;
   int min1, min2, cache1, cache2;
   float global_min;
   int new_row, obs_row;
   int negative_branches;
+  int* list_id;    /* n x n: the sorted list of the row at each index (partner ids) */
+  float* list_d;   /* and their distances */
+  int* list_len;
 } nj_state;
+
+void rnj_oracle_sort_row(int* id, float* dist, int len);
 
 /* redundantMap[i] as a singly linked list: push_front / front / pop_front */
 static void red_push(nj_state* s, int i, int j) {
@@ -66,13 +70,14 @@ static void handle_redundancy(nj_state* s, int i, int j) {
   s->red_count[i]++;
 }
 
-/* rapidNJ.cpp:91-138 (the sorted rows are not built: the search below evaluates every pair) */
+/* rapidNJ.cpp:91-155 */
 static void initialize(nj_state* s) {
   const unsigned n = s->n;
   s->cluster_count = (int)n;
   s->current_id = (int)n;
   s->cache1 = s->cache2 = 0;
   s->min1 = s->min2 = 0;
+  float max_sep = FLT_MIN;
   for (unsigned i = 0; i < n; i++) {
     s->index_to_id[i] = (int)i;
     s->id_to_index[i] = (int)i;
@@ -83,11 +88,28 @@ static void initialize(nj_state* s) {
     for (unsigned j = 0; j < n; j++) sum += s->m[(size_t)i * n + j];
     s->sepsum[i] = sum;
     s->sep[i] = sum / (float)(s->cluster_count - 2);
+    s->maxrow[i] = max_sep;
+    if (s->sep[i] > max_sep) max_sep = s->sep[i];
   }
   for (int i = (int)n - 1; i >= 0; i--) {
     if (s->index_to_id[i] == -1) continue;
     for (int j = 0; j < i; j++)
       if (s->m[(size_t)i * n + j] == 0) handle_redundancy(s, i, j);
+  }
+  /* buildSortedRow: partners j < i still visible, in index order, then sorted */
+  for (unsigned i = 0; i < n; i++) {
+    s->list_len[i] = 0;
+    if (s->index_to_id[i] == -1) continue;
+    int* ids = s->list_id + (size_t)i * n;
+    float* ds = s->list_d + (size_t)i * n;
+    int len = 0;
+    for (unsigned j = 0; j < i; j++) {
+      if (s->index_to_id[j] == -1) continue;
+      ids[len] = (int)j;
+      ds[len++] = s->m[(size_t)i * n + j];
+    }
+    rnj_oracle_sort_row(ids, ds, len);
+    s->list_len[i] = len;
   }
 }
 
@@ -102,7 +124,7 @@ static void improve(nj_state* s, int i, int j, float value) {
   s->min2 = j;
 }
 
-/* rapidNJ.cpp:290-361 (findMin, findRowMin, searchRedundantEntries) as described above */
+/* rapidNJ.cpp:290-361, 413-430 */
 static void find_min(nj_state* s) {
   const unsigned n = s->n;
   s->global_min = FLT_MAX;
@@ -111,26 +133,23 @@ static void find_min(nj_state* s) {
   do {
     if (s->index_to_id[i] != -1) {
       const float row_sep = s->sep[i];
+      const float row_max_sep = row_sep + s->maxrow[i];
       if (s->red_count[i] > 1) {
         const float value = -2 * s->sep[i];
         if (value < s->global_min) improve(s, i, s->red_head[i], value);
       }
-      const int id_i = s->index_to_id[i];
-      int best = -1;
-      float best_v = 0, best_d = 0;
-      for (unsigned j = 0; j < n; j++) {
-        const int id_j = s->index_to_id[j];
-        if (id_j == -1 || id_j >= id_i) continue;
-        const float d = s->m[(size_t)i * n + j];
-        float v = d - s->sep[s->rtm[j]];
+      const int* ids = s->list_id + (size_t)i * n;
+      const float* ds = s->list_d + (size_t)i * n;
+      for (int k = 0; k < s->list_len[i]; k++) {
+        const int o = s->id_to_index[ids[k]];
+        if (o == -1) continue; /* a dead pair */
+        const float d = ds[k];
+        const float bound = d - row_max_sep;
+        if (bound >= s->global_min) break;
+        float v = d - s->sep[s->rtm[o]];
         v = v - row_sep;
-        if (best < 0 || v < best_v || (v == best_v && d < best_d)) {
-          best = (int)j;
-          best_v = v;
-          best_d = d;
-        }
+        if (v < s->global_min) improve(s, i, o, v);
       }
-      if (best >= 0 && best_v < s->global_min) improve(s, i, best, best_v);
     }
     i = (i + 1) % (int)n;
   } while (i != start);
@@ -176,7 +195,7 @@ static void merge_min_nodes(nj_state* s, int* left, int* right, double* llen, do
   rlen[step] = dist2;
 }
 
-/* rapidNJ.cpp:166-238 (the sorted row of the new cluster is not built) */
+/* rapidNJ.cpp:166-238 */
 static void update_data(nj_state* s) {
   const unsigned n = s->n;
   float new_sepsum = 0;
@@ -185,6 +204,7 @@ static void update_data(nj_state* s) {
   const float* row1 = s->m + (size_t)r1 * n;
   const float* row2 = s->m + (size_t)r2 * n;
   float* new_row = s->m + (size_t)s->new_row * n;
+  float max_sep = FLT_MIN;
   for (int cur = 0; cur < s->current_id; cur++) {
     const int i = s->id_to_index[cur];
     if (i == -1 || s->index_to_id[i] == -1) continue;
@@ -199,6 +219,8 @@ static void update_data(nj_state* s) {
       new_sepsum += dist * (float)s->red_count[i];
       s->sepsum[i] += (dist - val1 - val2);
       s->sep[i] = s->sepsum[i] / (float)(s->cluster_count - 2);
+      s->maxrow[i] = max_sep;
+      if (s->sep[i] > max_sep) max_sep = s->sep[i];
       new_row[i] = dist;
       s->m[(size_t)i * n + s->new_row] = dist;
     }
@@ -208,13 +230,26 @@ static void update_data(nj_state* s) {
   s->index_to_id[s->obs_row] = -1;
   s->sepsum[s->new_row] = new_sepsum;
   s->sepsum[s->obs_row] = 0;
+  s->list_len[s->obs_row] = 0;
   s->sep[s->new_row] = new_sepsum / (float)(s->cluster_count - 2);
+  s->maxrow[s->new_row] = max_sep;
   s->sep[s->obs_row] = FLT_MIN;
   s->rtm[s->new_row] = s->new_row;
   s->rtm[s->obs_row] = s->obs_row;
   const int next_id = s->current_id++;
   s->index_to_id[s->new_row] = next_id;
   s->id_to_index[next_id] = s->new_row;
+  /* the new cluster's list: every other live index, in index order, then sorted */
+  int* ids = s->list_id + (size_t)s->new_row * n;
+  float* ds = s->list_d + (size_t)s->new_row * n;
+  int len = 0;
+  for (unsigned i = 0; i < n; i++) {
+    if ((int)i == s->new_row || s->index_to_id[i] == -1) continue;
+    ids[len] = s->index_to_id[i];
+    ds[len++] = new_row[i];
+  }
+  rnj_oracle_sort_row(ids, ds, len);
+  s->list_len[s->new_row] = len;
 }
 
 int rnj_oracle_nj(const float* matrix, unsigned n, int negative_branches, int* left, int* right,
@@ -233,8 +268,12 @@ int rnj_oracle_nj(const float* matrix, unsigned n, int negative_branches, int* l
   s.rtm = (int*)malloc(n * sizeof(int));
   s.red_head = (int*)malloc(n * sizeof(int));
   s.red_next = (int*)malloc(n * sizeof(int));
+  s.maxrow = (float*)malloc(n * sizeof(float));
+  s.list_id = (int*)malloc((size_t)n * n * sizeof(int));
+  s.list_d = (float*)malloc((size_t)n * n * sizeof(float));
+  s.list_len = (int*)malloc(n * sizeof(int));
   if (!s.m || !s.sepsum || !s.sep || !s.index_to_id || !s.id_to_index || !s.red_count || !s.rtm ||
-      !s.red_head || !s.red_next)
+      !s.red_head || !s.red_next || !s.maxrow || !s.list_id || !s.list_d || !s.list_len)
     return -2;
   memcpy(s.m, matrix, (size_t)n * n * sizeof(float));
   for (size_t k = 0; k < 2 * (size_t)n; k++) s.id_to_index[k] = -1;
@@ -282,6 +321,10 @@ int rnj_oracle_nj(const float* matrix, unsigned n, int negative_branches, int* l
   free(s.rtm);
   free(s.red_head);
   free(s.red_next);
+  free(s.maxrow);
+  free(s.list_id);
+  free(s.list_d);
+  free(s.list_len);
   return rc;
 }
 

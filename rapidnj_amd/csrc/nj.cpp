@@ -1,9 +1,13 @@
 // Host driver of the device neighbour joining (nj.hip): set-up as rapidNJ::initialize
-// (rapidNJ.cpp:91-138), the iteration loop of rapidNJ::run (:45-50) enqueued without waits, the
-// last join (:52-85), and the tree with polytree's Newick text (polytree.cpp:402-458).
+// (rapidNJ.cpp:91-138), the iteration loop of rapidNJ::run (:45-50) enqueued in windows without
+// waits, the searches the device leaves to the host (exact ties and the row-scan break, settled
+// with rapidNJ's own sorted row lists), the last join (:52-85), and the tree with polytree's
+// Newick text (polytree.cpp:402-458).
 #include <algorithm>
+#include <cfloat>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -17,6 +21,7 @@ struct rnj_tree {
   int last[2] = {0, 0};          // polytree::set_serialization_indices
   float last_dist = 0.0f;
   std::vector<std::string> names;
+  unsigned host_searches = 0;    // iterations whose search the host settled
 };
 
 namespace rnj {
@@ -45,10 +50,17 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   // device state
   DevBuf<NjState> st;
   DevBuf<int> index_to_id, id_to_index, red_count, rtm, red_head, red_next, slot_of_index;
-  DevBuf<int> slot_index, alive, row_p, map, left, right;
-  DevBuf<float> sepsum, sep, newcol, term, sums, S0, S1;
+  DevBuf<int> slot_index, alive, row_p, map, left, right, row_tie, newidx, log_idx;
+  DevBuf<float> sepsum, sep, newcol, term, sums, S0, S1, row_maxsep, init_maxsep, log_val;
+  DevBuf<unsigned> row_sigma, key_by_index;
   DevBuf<unsigned long long> row_key;
   DevBuf<double> llen, rlen;
+  // the iteration plan: slot bound, compaction before it, slot-matrix buffer, log offset
+  const int iters = (int)n - 2;
+  std::vector<int> plan_ub(iters), plan_buf(iters);
+  std::vector<char> plan_compact(iters);
+  std::vector<size_t> plan_log(iters);
+  size_t log_total = 0;
   NJ_HIP(st.alloc(1));
   NJ_HIP(index_to_id.alloc(n));
   NJ_HIP(id_to_index.alloc(2 * (size_t)n));
@@ -73,6 +85,12 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   NJ_HIP(rlen.alloc(n));
   NJ_HIP(S0.alloc(cap * cap));
   NJ_HIP(S1.alloc(cap * cap));
+  NJ_HIP(row_sigma.alloc(cap));
+  NJ_HIP(row_tie.alloc(cap));
+  NJ_HIP(row_maxsep.alloc(cap));
+  NJ_HIP(init_maxsep.alloc(cap));
+  NJ_HIP(key_by_index.alloc(n));
+  NJ_HIP(newidx.alloc(n));
   // declared after the buffers, so destroyed first: no buffer is freed under a running kernel
   struct StreamWait {
     hipStream_t s;
@@ -157,14 +175,56 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
     h_alive[a] = 1;
     h_slot_index[a] = vis[a];
   }
+  // maxRowSeparations of the first search: the largest separation of every smaller index,
+  // copies included (rapidNJ.cpp:112-119)
+  std::vector<float> h_init_maxsep(cap, FLT_MIN);
+  {
+    std::vector<float> by_index(n);
+    float mx = FLT_MIN;
+    for (unsigned i = 0; i < n; ++i) {
+      by_index[i] = mx;
+      const float sp = h_sums[i] / (float)((int)n - 2);
+      if (sp > mx) mx = sp;
+    }
+    for (int a = 0; a < v; ++a) h_init_maxsep[a] = by_index[vis[a]];
+  }
+  const std::vector<int> init_visible = h_index_to_id;  // -1: a copy hidden at set-up
   NjState h_st{};
   h_st.n = (int)n;
   h_st.used = v;
   h_st.cluster_count = (int)n;
   h_st.current_id = (int)n;
   h_st.negative_branches = negative_branches ? 1 : 0;
+
+  // rapidNJ::run: n - 2 joins. The plan's slot bound bounds the slots in use (exact until the
+  // first compaction, then the live count's bound clusterCount plus the slots added since). The
+  // slot matrix is compacted when that bound passes 1.125 x clusterCount (or the capacity).
+  {
+    size_t b = (size_t)v;
+    int buf = 0;
+    for (int it = 0; it < iters; ++it) {
+      const size_t cc = n - (size_t)it;  // clusterCount before this join
+      if (b + 1 > cap || (b > 256 && b * 8 > cc * 9)) {
+        plan_compact[it] = 1;
+        b = std::min(cc, b);
+        buf ^= 1;
+      }
+      plan_ub[it] = (int)b;
+      plan_buf[it] = buf;
+      plan_log[it] = log_total;
+      log_total += b;
+      ++b;
+    }
+  }
+  NJ_HIP(log_val.alloc(log_total));
+  NJ_HIP(log_idx.alloc(log_total));
+  NJ_HIP(hipMemsetAsync(log_idx.p, 0xff, std::max<size_t>(log_total, 1) * sizeof(int), s));
+
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+  };
+  auto down = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
   };
   DevBuf<int> d_vis;
   NJ_HIP(d_vis.alloc(std::max(v, 1)));
@@ -181,6 +241,8 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   NJ_HIP(up(alive.p, h_alive.data(), cap * sizeof(int)));
   NJ_HIP(up(sepsum.p, h_sepsum.data(), cap * sizeof(float)));
   NJ_HIP(up(sep.p, h_sep.data(), cap * sizeof(float)));
+  NJ_HIP(up(init_maxsep.p, h_init_maxsep.data(), cap * sizeof(float)));
+  NJ_HIP(hipMemsetAsync(key_by_index.p, 0xff, n * sizeof(unsigned), s));  // no answers yet
   NJ_HIP(nj_launch_gather(M, ldm, d_vis.p, v, S0.p, cap, s));
 
   NjDevice D{};
@@ -199,34 +261,256 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   D.sep = sep.p;
   D.row_key = row_key.p;
   D.row_p = row_p.p;
+  D.row_sigma = row_sigma.p;
+  D.row_tie = row_tie.p;
+  D.row_maxsep = row_maxsep.p;
+  D.init_maxsep = init_maxsep.p;
   D.newcol = newcol.p;
   D.term = term.p;
   D.map = map.p;
+  D.key_by_index = key_by_index.p;
+  D.newidx = newidx.p;
+  D.log_val = log_val.p;
+  D.log_idx = log_idx.p;
   D.left = left.p;
   D.right = right.p;
   D.llen = llen.p;
   D.rlen = rlen.p;
+  float* const Sbuf[2] = {S0.p, S1.p};
 
-  // rapidNJ::run: n - 2 joins. used_bound bounds the slots in use (exact until the first
-  // compaction, then the live count's bound clusterCount plus the slots added since). The slot
-  // matrix is compacted when that bound passes 1.125 x clusterCount (or the capacity).
-  float* S = S0.p;
-  float* S2 = S1.p;
-  size_t used_bound = (size_t)v;
-  bool pending = false;  // the last join's new cluster still needs its separation and row
-  for (unsigned it = 0; it + 2 < n; ++it) {
-    const size_t cc = n - it;  // clusterCount before this join
-    if (used_bound + 1 > cap || (used_bound > 256 && used_bound * 8 > cc * 9)) {
-      if (pending) NJ_HIP(nj_launch_finish(D, S, s));
-      pending = false;
-      NJ_HIP(nj_launch_compact(D, S, S2, (int)std::min(cc, used_bound), s));
-      std::swap(S, S2);
-      used_bound = std::min(cc, used_bound);
+  // Iteration `it` as planned (or only its step, after the host settled its search). The
+  // previous join's new cluster is finished by this iteration's scan, or before a compaction.
+  auto enqueue = [&](int it, bool step_only) -> hipError_t {
+    float* S = Sbuf[plan_buf[it]];
+    if (step_only) return nj_launch_step(D, S, plan_log[it], s);
+    if (plan_compact[it]) {
+      float* Sp = Sbuf[plan_buf[it] ^ 1];
+      if (it > 0) {
+        const hipError_t e = nj_launch_finish(D, Sp, s);
+        if (e != hipSuccess) return e;
+      }
+      const hipError_t e = nj_launch_compact(D, Sp, S, plan_ub[it], s);
+      if (e != hipSuccess) return e;
     }
-    NJ_HIP(nj_launch_iteration(D, S, (int)used_bound, pending, s));
-    pending = true;
-    ++used_bound;
+    return nj_launch_iteration(D, S, plan_ub[it], it > 0 && !plan_compact[it], plan_log[it], s);
+  };
+
+  // The search of a halted iteration, as rapidNJ's findMin makes it (rapidNJ.cpp:290-361): rows
+  // in visiting order; a row that cannot improve the running minimum G (its answer is not below
+  // G) is passed over; a row whose answer stands (G above its break bound, no tie at its best) is
+  // a record with that answer; any other row is scanned as rapidNJ scans it, over its sorted
+  // list (the list as the row was built, std::sort on the distance alone: rapidNJ.cpp:141-153,
+  // 219-234), with the break (:340). The result goes back as NjState::forced.
+  int t_halt = 0;
+  auto settle = [&]() -> int {
+    NjState h{};
+    NJ_HIP(down(&h, st.p, sizeof h));
+    NJ_HIP(hipStreamSynchronize(s));
+    if (h.error != kNjHalted) return fail(RNJ_E_INVALID, "neighbour joining stopped early");
+    const int step = h.step, used = h.used;
+    if (step < 0 || step >= iters || used <= 0 || (size_t)used > cap)
+      return fail(RNJ_E_INVALID, "neighbour joining state out of range");
+    std::vector<int> i2id(n), id2i(2 * (size_t)n), rc(n), rhead(n), rt(n), soi(n), nidx(step + 1);
+    std::vector<unsigned long long> key(used);
+    std::vector<int> rp(used), tie(used);
+    std::vector<unsigned> sig(used);
+    std::vector<float> sp(used), mxs(used);
+    NJ_HIP(down(i2id.data(), index_to_id.p, n * sizeof(int)));
+    NJ_HIP(down(id2i.data(), id_to_index.p, 2 * (size_t)n * sizeof(int)));
+    NJ_HIP(down(rc.data(), red_count.p, n * sizeof(int)));
+    NJ_HIP(down(rhead.data(), red_head.p, n * sizeof(int)));
+    NJ_HIP(down(rt.data(), rtm.p, n * sizeof(int)));
+    NJ_HIP(down(soi.data(), slot_of_index.p, n * sizeof(int)));
+    if (step > 0) NJ_HIP(down(nidx.data(), newidx.p, step * sizeof(int)));
+    NJ_HIP(down(key.data(), row_key.p, used * sizeof(unsigned long long)));
+    NJ_HIP(down(rp.data(), row_p.p, used * sizeof(int)));
+    NJ_HIP(down(tie.data(), row_tie.p, used * sizeof(int)));
+    NJ_HIP(down(sig.data(), row_sigma.p, used * sizeof(unsigned)));
+    NJ_HIP(down(sp.data(), sep.p, used * sizeof(float)));
+    NJ_HIP(down(mxs.data(), row_maxsep.p, used * sizeof(float)));
+    NJ_HIP(hipStreamSynchronize(s));
+    auto slot_ok = [&](int slot) { return slot >= 0 && slot < used; };
+    auto sep_of = [&](int idx) -> float {  // separations[redundantToMasterMap[idx]]
+      const int slot = soi[rt[idx]];
+      return slot_ok(slot) ? sp[slot] : 0.0f;
+    };
+    auto fkey_host = [](float x) -> unsigned {
+      unsigned u;
+      const float y = x + 0.0f;
+      std::memcpy(&u, &y, 4);
+      return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    };
+    auto unkey = [](unsigned u) -> float {
+      const unsigned b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+      float x;
+      std::memcpy(&x, &b, 4);
+      return x;
+    };
+    struct Pair {  // cluster_pair: compared on the distance alone (cluster_pair.h:8-10)
+      int id;
+      float distance;
+      bool operator<(const Pair& o) const { return distance < o.distance; }
+    };
+    // the row list of cluster `cid` as it was built, then sorted
+    auto sorted_list = [&](int cid, std::vector<Pair>& out) -> int {
+      out.clear();
+      if (cid < (int)n) {  // an initial row: visible indices below it (rapidNJ.cpp:141-153)
+        std::vector<float> row(std::max(cid, 1));
+        if (cid > 0) {
+          NJ_HIP(down(row.data(), M + (size_t)cid * ldm, cid * sizeof(float)));
+          NJ_HIP(hipStreamSynchronize(s));
+        }
+        for (int j = 0; j < cid; ++j)
+          if (init_visible[j] != -1) out.push_back(Pair{j, row[j]});
+      } else {  // made at step k: every other live index, in index order (:219-234)
+        const int k = cid - (int)n;
+        if (k < 0 || k >= step) return fail(RNJ_E_INVALID, "row list of an unknown cluster");
+        const size_t len = (size_t)plan_ub[k];
+        std::vector<float> val(len);
+        std::vector<int> idx(len);
+        NJ_HIP(down(val.data(), log_val.p + plan_log[k], len * sizeof(float)));
+        NJ_HIP(down(idx.data(), log_idx.p + plan_log[k], len * sizeof(int)));
+        NJ_HIP(hipStreamSynchronize(s));
+        // the cluster at each index then: the last one made there before step k, else the
+        // index's own
+        std::vector<int> at(n);
+        for (unsigned i = 0; i < n; ++i) at[i] = (int)i;
+        for (int j = 0; j < k; ++j)
+          if (nidx[j] >= 0 && nidx[j] < (int)n) at[nidx[j]] = (int)n + j;
+        std::vector<std::pair<int, float>> by_index;
+        for (size_t b = 0; b < len; ++b)
+          if (idx[b] >= 0 && idx[b] < (int)n) by_index.emplace_back(idx[b], val[b]);
+        std::sort(by_index.begin(), by_index.end(),
+                  [](const std::pair<int, float>& x, const std::pair<int, float>& y) {
+                    return x.first < y.first;
+                  });
+        for (const auto& e : by_index) out.push_back(Pair{at[e.first], e.second});
+      }
+      std::sort(out.begin(), out.end());
+      return RNJ_OK;
+    };
+    float G = FLT_MAX;
+    int partner = -1;
+    std::vector<int> records;
+    std::vector<Pair> list;
+    const int start = i2id[h.cache1] != -1 ? h.cache1 : h.cache2;
+    for (unsigned c = 0; c < n; ++c) {
+      const int r = (int)((start + c) % n);
+      if (i2id[r] == -1) continue;
+      const int a = soi[r];
+      if (!slot_ok(a) || key[a] == ~0ull) continue;
+      const float vr = unkey((unsigned)(key[a] >> 32));
+      if (!(vr < G)) continue;
+      if (fkey_host(G) > sig[a] && !tie[a]) {  // the row's own answer stands
+        G = vr;
+        partner = rp[a];
+        records.push_back(r);
+        continue;
+      }
+      // findRowMin (rapidNJ.cpp:324-361) with searchRedundantEntries (:413-430)
+      bool improved = false;
+      const float row_sep = sp[a];
+      const float row_max_sep = row_sep + mxs[a];
+      if (rc[r] > 1) {
+        const float cv = -2.0f * row_sep;
+        if (cv < G) {
+          G = cv;
+          partner = rhead[r];
+          improved = true;
+        }
+      }
+      const int lrc = sorted_list(i2id[r], list);
+      if (lrc != RNJ_OK) return lrc;
+      for (const Pair& e : list) {
+        if (e.id < 0 || e.id >= 2 * (int)n) continue;
+        const int o = id2i[e.id];
+        if (o == -1) continue;  // a dead pair
+        const float bound = e.distance - row_max_sep;
+        if (bound >= G) break;
+        float val = e.distance - sep_of(o);
+        val = val - row_sep;
+        if (val < G) {
+          G = val;
+          partner = o;
+          improved = true;
+        }
+      }
+      if (improved) records.push_back(r);
+    }
+    if (records.empty()) return fail(RNJ_E_INVALID, "neighbour joining stopped early (no pair found)");
+    int m = h.min1, c1 = h.cache1, c2 = h.cache2;
+    for (int r : records) {
+      if (r != m) {
+        c2 = c1;
+        c1 = m;
+        m = r;
+      }
+    }
+    h.f_min1 = m;
+    h.f_min2 = partner;
+    h.f_cache1 = c1;
+    h.f_cache2 = c2;
+    h.forced = 1;
+    h.error = 0;
+    NJ_HIP(up(st.p, &h, sizeof h));
+    t_halt = step;
+    return RNJ_OK;
+  };
+
+  // Windows of iterations, each followed by a copy of the state into pinned memory; the next
+  // window is enqueued before the previous one is checked, so the device does not wait. A halt
+  // turns the kernels after it into no-ops (they test NjState::error first); the host then
+  // settles the search and resumes with that iteration's step.
+  struct Pinned {
+    NjState* p = nullptr;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+  } pinned;
+  NJ_HIP(hipHostMalloc((void**)&pinned.p, 2 * sizeof(NjState), hipHostMallocDefault));
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (auto x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  NJ_HIP(hipEventCreateWithFlags(&ev.e[0], hipEventDisableTiming));
+  NJ_HIP(hipEventCreateWithFlags(&ev.e[1], hipEventDisableTiming));
+  int it = 0, window = 32, slot = 0;
+  bool step_only = false;
+  std::deque<int> inflight;
+  while (true) {
+    if (it < iters && inflight.size() < 2) {
+      const int to = std::min(iters, it + window);
+      for (int k = it; k < to; ++k) NJ_HIP(enqueue(k, step_only && k == it));
+      step_only = false;
+      NJ_HIP(down(&pinned.p[slot], st.p, sizeof(NjState)));
+      NJ_HIP(hipEventRecord(ev.e[slot], s));
+      inflight.push_back(slot);
+      slot ^= 1;
+      it = to;
+      window = std::min(2 * window, 1024);
+      continue;
+    }
+    if (inflight.empty()) break;
+    const int w = inflight.front();
+    inflight.pop_front();
+    NJ_HIP(hipEventSynchronize(ev.e[w]));
+    const int err = pinned.p[w].error;
+    if (err == kNjFailed) break;  // reported below
+    if (err == kNjHalted) {
+      NJ_HIP(hipStreamSynchronize(s));
+      inflight.clear();
+      const int rc = settle();
+      if (rc != RNJ_OK) return rc;
+      ++t->host_searches;
+      it = t_halt;
+      step_only = true;
+      window = 32;
+    }
   }
+  float* S = Sbuf[plan_buf[iters - 1]];
   // results
   std::vector<int> hl(n), hr(n);
   std::vector<double> hll(n), hrl(n);
@@ -459,6 +743,8 @@ int rnj_nj_alignment(rnj_context* ctx, const rnj_alignment* aln, int model, int 
 }
 
 unsigned rnj_tree_size(const rnj_tree* t) { return t ? t->n : 0u; }
+
+unsigned rnj_tree_host_searches(const rnj_tree* t) { return t ? t->host_searches : 0u; }
 
 int rnj_tree_joins(const rnj_tree* t, int* left, int* right, double* left_len, double* right_len,
                    int* last, float* last_dist) {

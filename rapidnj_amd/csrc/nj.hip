@@ -1,24 +1,35 @@
 // Neighbour joining on the device: rapidNJ::run (rapidNJ.cpp:45-88) with the matrix in HBM.
 //
-// rapidNJ keeps each row's partners sorted by distance and stops scanning a row at a bound; the
-// pair it joins is the first one, in its visiting order, reaching the minimum of the criterion
-// (d - sep[partner]) - sep[row] over all live pairs. Here every live pair is evaluated each
-// iteration (the scan is a full pass over the live matrix, which HBM and the 256 MB last-level
-// cache serve at several TB/s), with the reference's float arithmetic and its visiting order:
+// rapidNJ keeps each row's partners sorted by distance (std::sort on the distance alone) and
+// scans a row's live pairs in that order until the first whose float
+// distance - (sep[row] + maxRowSeparation[row]) is not below the running minimum; it joins the
+// first pair, in its visiting order, that lowers the minimum of the criterion
+// (d - sep[partner]) - sep[row] last. Here every live pair is evaluated each iteration (the scan
+// is a full pass over the live matrix, which HBM and the 256 MB last-level cache serve at several
+// TB/s), with the reference's float arithmetic and visiting order:
 //   * a pair (a, b) lives in the row of the cluster with the larger id; the matrix is kept in
 //     "slots" ordered by cluster id (new clusters take the next slot), so row slot a's partners
 //     are exactly the live slots b < a;
 //   * rows are visited by matrix index, cyclically from the row rapidNJ cached in the previous
-//     search, and a row's pairs by increasing distance, then increasing partner index (for
-//     exact ties in both distance and criterion the reference's order is std::sort's);
+//     search, and a row's pairs by increasing distance;
 //   * a row with redundant copies offers the copy join at -2 * sep first.
+// Two things the full evaluation cannot see are left to the host, which the step halts for
+// (NjState::error = kNjHalted) and which then supplies the search result (NjState::forced):
+//   * exact ties: the winning row has several pairs of equal criterion and equal distance, and
+//     rapidNJ takes the one std::sort put first in the row's list;
+//   * the scan's break: its bound is rounded differently from the criterion, so a row can stop
+//     one pair before a criterion one unit in the last place below the running minimum. The scan
+//     gives each row a bound sigma (below) such that the row's own result stands whenever the
+//     running minimum that reaches it is above sigma; a record row reached at or below it
+//     halts the step.
 // Per iteration: nj_scan_kernel (16 row slots per work-group, one per wave: each row's best
 // pair; it also writes the previous join's column) and nj_step_kernel (one work-group: the
 // global winner and rapidNJ's row caches, the merge with its redundancy bookkeeping, the update
-// of distances and separations in id order, the new slot's row). The new cluster's separation
-// sum (a sequential float sum in id order) is finished by the next scan. Every few iterations
-// nj_compact_* rebuilds the slot matrix from the live slots. The host enqueues all iterations
-// without waiting: the iteration count (n - 2) and the slot bounds are known ahead.
+// of distances and separations in id order, the new slot's row, logged for the host). The new
+// cluster's separation sum (a sequential float sum in id order) is finished by the next scan.
+// Every few iterations nj_compact_* rebuilds the slot matrix from the live slots. The host
+// enqueues iterations in windows without waiting: the iteration count (n - 2) and the slot
+// bounds are known ahead.
 #include "nj.hpp"
 
 #include <float.h>
@@ -30,7 +41,9 @@ constexpr int kScanThreads = 1024;  // 16 waves, one row each
 constexpr int kScanRows = kScanThreads / 64;
 constexpr int kSepChunk = 4096;     // separations staged in LDS per step of a row scan
 constexpr int kStepThreads = 1024;
-constexpr int kKeyCache = 20000;    // row keys the step keeps in LDS between its record passes
+constexpr int kStepWaves = kStepThreads / 64;
+constexpr int kKeyCache = 32768;    // row keys the step's search keeps in LDS
+constexpr uint32_t kNone = 0xffffffffu;
 
 // Floats as order-preserving unsigned keys (after -0 -> +0, so that equal values compare equal
 // as the reference's float comparisons do).
@@ -39,63 +52,117 @@ __device__ __forceinline__ uint32_t fkey(float x) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// A row's candidate: criterion v, distance d, partner index p; ordered by (v, d, p) with NaN v
-// (gone partners) never better.
+constexpr float kInf = __builtin_inff();
+constexpr unsigned long long kNoKey = ~0ull;
+
+// A row's best pair: criterion v, distance d, partner index p, and the smallest criterion among
+// the row's other pairs (v2; v2 == v when another pair shares the best criterion). Among pairs
+// of equal criterion any may stand: such a row is settled by the host whenever it wins, and its
+// break bound (below) is conservative for either distance. Merged by (v, d, p), with NaN v (gone
+// partners) never better.
 struct Cand {
   float v, d;
   int p;
+  float v2;
 };
 
-__device__ __forceinline__ bool cand_less(const Cand& x, const Cand& y) {
-  if (x.v < y.v) return true;
-  if (!(x.v == y.v)) return false;
-  if (x.d < y.d) return true;
-  if (!(x.d == y.d)) return false;
-  return x.p < y.p;
+// Merge o into c (symmetric, so that both lanes of a shuffle agree).
+__device__ __forceinline__ void cand_merge(Cand& c, const Cand& o) {
+  if (o.v < c.v) {
+    const float v2 = fminf(o.v2, c.v);
+    c = o;
+    c.v2 = v2;
+  } else if (c.v < o.v) {
+    c.v2 = fminf(c.v2, o.v);
+  } else {
+    c.v2 = fminf(fminf(c.v2, o.v2), c.v);
+    if (o.d < c.d || (o.d == c.d && o.p < c.p)) {
+      c.d = o.d;
+      c.p = o.p;
+    }
+  }
 }
 
-__device__ __forceinline__ Cand wave_min(Cand c) {
+__device__ __forceinline__ Cand wave_merge(Cand c) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    Cand o{__shfl_xor(c.v, off), __shfl_xor(c.d, off), __shfl_xor(c.p, off)};
-    if (cand_less(o, c)) c = o;
+    const Cand o{__shfl_xor(c.v, off), __shfl_xor(c.d, off), __shfl_xor(c.p, off),
+                 __shfl_xor(c.v2, off)};
+    cand_merge(c, o);
   }
   return c;
 }
 
-constexpr float kInf = __builtin_inff();
-constexpr unsigned long long kNoKey = ~0ull;
+__device__ __forceinline__ float wave_fmax(float x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off));
+  return x;
+}
+
+// One lane's running state over its share of a row: best criterion, its distance and slot, the
+// smallest criterion of the lane's other pairs, and the largest separation seen
+// (maxRowSeparation's running max, from FLT_MIN).
+struct Lane {
+  float bv, bd, v2, mx;
+  int bb;
+};
+
+__device__ __forceinline__ Lane lane_start() { return Lane{kInf, kInf, kInf, FLT_MIN, -1}; }
+
+// Candidate (d, separation) of partner slot b into this lane's state: the criterion, then five
+// operations (NaN from gone slots leaves every field unchanged).
+__device__ __forceinline__ void consider(float d, float s_b, float s_a, int b, Lane& l) {
+  const float v = __fsub_rn(__fsub_rn(d, s_b), s_a);
+  l.mx = fmaxf(l.mx, s_b);
+  const bool lt = v < l.bv;
+  l.v2 = fminf(l.v2, lt ? l.bv : v);
+  l.bv = lt ? v : l.bv;
+  l.bd = lt ? d : l.bd;
+  l.bb = lt ? b : l.bb;
+}
+
+// A lane's state as a candidate for the wave merge.
+__device__ __forceinline__ Cand lane_cand(const NjDevice& D, const Lane& l) {
+  if (l.bb < 0) return Cand{kInf, kInf, 0x7fffffff, l.v2};
+  return Cand{l.bv, l.bd, D.slot_index[l.bb], l.v2};
+}
 
 // The row's answer as rapidNJ's findRowMin gives it: the redundant-copy join first when the row
 // has copies (searchRedundantEntries), then the best pair unless it is not strictly better.
 // Rows whose answer does not beat FLT_MAX (the search's start value) get no key.
-__device__ __forceinline__ void write_row(const NjDevice& D, int a, const Cand& best, float s_a) {
+// The break bound: rapidNJ reaches the best pair e (distance d, criterion v) unless the running
+// minimum G it meets there is at most t = float(d - float(sep[a] + maxsep)). When t <= v that
+// cannot happen (G > v at e, or e would not improve it). Otherwise G at e is the minimum that
+// reaches the row, lowered first by the copy join and by pairs before e, whose criteria are at
+// least v2: with v2 and the copy join above t, the row stands exactly when the minimum reaching
+// it is above sigma = t; else sigma = +inf (the host searches whenever the row is a record).
+__device__ __forceinline__ void write_row(const NjDevice& D, int a, Cand best, float s_a,
+                                          float maxsep) {
   const int idx = D.slot_index[a];
-  float v = best.p >= 0 ? best.v : kInf;
-  int p = best.p;
-  if (D.red_count[idx] > 1) {
-    const float rv = -2.0f * s_a;
-    if (!(best.p >= 0 && best.v < rv)) {
-      v = rv;
-      p = D.red_head[idx];
-    }
+  const bool has_pair = best.p != 0x7fffffff && best.v < kInf;
+  const bool copies = D.red_count[idx] > 1;
+  const float cv = copies ? -2.0f * s_a : kInf;
+  const bool pair = has_pair && best.v < cv;
+  float v = kInf;
+  int p = -1;
+  if (pair) {
+    v = best.v;
+    p = best.p;
+  } else if (copies) {
+    v = cv;
+    p = D.red_head[idx];
+  }
+  float sigma = -kInf;
+  if (pair) {
+    const float t = __fsub_rn(best.d, __fadd_rn(s_a, maxsep));
+    if (t > best.v) sigma = (best.v2 <= t || cv <= t) ? kInf : t;
   }
   D.row_key[a] = v < FLT_MAX ? ((unsigned long long)fkey(v) << 32) | (uint32_t)idx : kNoKey;
+  D.key_by_index[idx] = v < FLT_MAX ? fkey(v) : kNone;
   D.row_p[a] = p;
-}
-
-// Candidate (d, separation) of partner b into this lane's best.
-__device__ __forceinline__ void consider(const NjDevice& D, float d_raw, float s_b, float s_a,
-                                         int b, Cand& best, int& best_b) {
-  const float v = __fsub_rn(__fsub_rn(d_raw, s_b), s_a) + 0.0f;
-  const float d = d_raw + 0.0f;
-  if (v < best.v || (v == best.v && d < best.d)) {
-    best.v = v;
-    best.d = d;
-    best_b = b;
-  } else if (v == best.v && d == best.d && best_b >= 0 && D.slot_index[b] < D.slot_index[best_b]) {
-    best_b = b;  // exact tie: the lower partner index first
-  }
+  D.row_sigma[a] = fkey(sigma);
+  D.row_tie[a] = pair && best.v2 <= best.v ? 1 : 0;
+  D.row_maxsep[a] = maxsep;
 }
 
 // Finish the previous join's new cluster (slot used - 1), on one work-group: the reference's
@@ -108,6 +175,7 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
   constexpr int kChunk = 4096;
   __shared__ float terms[kChunk];
   __shared__ Cand wbest[kScanThreads / 64];
+  __shared__ float wmax[kScanThreads / 64];
   float acc = 0.0f;
   for (int base = 0; base < ns; base += kChunk) {
     const int cnt = min(kChunk, ns - base);
@@ -132,20 +200,23 @@ __device__ void finish_new_cluster(const NjDevice& D, const float* __restrict__ 
   __syncthreads();
   const float s_ns = D.sep[ns];
   // the new row: partners are every live slot below it
-  Cand best{kInf, kInf, -1};
-  int best_b = -1;
-  for (int b = tid; b < ns; b += kScanThreads) consider(D, D.newcol[b], D.sep[b], s_ns, b, best, best_b);
-  best.p = best_b >= 0 ? D.slot_index[best_b] : 0x7fffffff;
-  if (best_b < 0) best.v = kInf;
-  best = wave_min(best);
-  if (lane == 0) wbest[wave] = best;
+  Lane l = lane_start();
+  for (int b = tid; b < ns; b += kScanThreads) consider(D.newcol[b], D.sep[b], s_ns, b, l);
+  Cand best = wave_merge(lane_cand(D, l));
+  const float mx = wave_fmax(l.mx);
+  if (lane == 0) {
+    wbest[wave] = best;
+    wmax[wave] = mx;
+  }
   __syncthreads();
   if (tid == 0) {
     Cand b = wbest[0];
-    for (int w = 1; w < kScanThreads / 64; ++w)
-      if (cand_less(wbest[w], b)) b = wbest[w];
-    if (!(b.v < kInf)) b.p = -1;
-    write_row(D, ns, b, s_ns);
+    float m = wmax[0];
+    for (int w = 1; w < kScanThreads / 64; ++w) {
+      cand_merge(b, wbest[w]);
+      m = fmaxf(m, wmax[w]);
+    }
+    write_row(D, ns, b, s_ns, m);
   }
 }
 
@@ -182,16 +253,14 @@ __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, float
   const int reach = min(a0 + kScanRows, used) - 1;  // longest row prefix in this group
   const float s_a = mine ? D.sep[a] : 0.0f;
   const float* row = S + (size_t)a * D.ld;
-  Cand best{kInf, kInf, -1};
-  int best_b = -1;
+  Lane l = lane_start();
   for (int c0 = 0; c0 < reach; c0 += kSepChunk) {
     const int cnt = min(kSepChunk, reach - c0);
     for (int k = 4 * tid; k < cnt; k += 4 * kScanThreads)
       *reinterpret_cast<float4*>(ssep + k) = *reinterpret_cast<const float4*>(D.sep + c0 + k);
     __syncthreads();
     if (mine) {
-      // four 4-wide loads in flight per lane; a lane's candidates are the same as one at a time
-      // (ties are settled by partner index, not by visiting order)
+      // four 4-wide loads in flight per lane
       const int end = min(a, c0 + cnt);
       for (int b0 = c0 + 4 * lane; b0 < end; b0 += 1024) {
         float4 dv[4];
@@ -203,23 +272,21 @@ __global__ __launch_bounds__(kScanThreads) void nj_scan_kernel(NjDevice D, float
           const int bu = b0 + 256 * u;
           if (bu >= end) break;
           const float4 sv = *reinterpret_cast<const float4*>(ssep + (bu - c0));
-          consider(D, dv[u].x, sv.x, s_a, bu, best, best_b);
-          if (bu + 1 < end) consider(D, dv[u].y, sv.y, s_a, bu + 1, best, best_b);
-          if (bu + 2 < end) consider(D, dv[u].z, sv.z, s_a, bu + 2, best, best_b);
-          if (bu + 3 < end) consider(D, dv[u].w, sv.w, s_a, bu + 3, best, best_b);
+          consider(dv[u].x, sv.x, s_a, bu, l);
+          if (bu + 1 < end) consider(dv[u].y, sv.y, s_a, bu + 1, l);
+          if (bu + 2 < end) consider(dv[u].z, sv.z, s_a, bu + 2, l);
+          if (bu + 3 < end) consider(dv[u].w, sv.w, s_a, bu + 3, l);
         }
       }
     }
     __syncthreads();
   }
   if (!mine) return;
-  best.p = best_b >= 0 ? D.slot_index[best_b] : 0x7fffffff;
-  if (best_b < 0) best.v = kInf;
-  best = wave_min(best);
-  if (lane == 0) {
-    if (!(best.v < kInf)) best.p = -1;
-    write_row(D, a, best, s_a);
-  }
+  const Cand best = wave_merge(lane_cand(D, l));
+  // maxRowSeparation: the largest separation of the live clusters of smaller id (rapidNJ.cpp:
+  // 177-214), and for the first search that of every smaller index (:114-119)
+  const float mx = wave_fmax(l.mx);
+  if (lane == 0) write_row(D, a, best, s_a, st.step == 0 ? D.init_maxsep[a] : mx);
 }
 
 // Launch A's finishing work alone (before a compaction, which copies whole rows and columns):
@@ -232,76 +299,103 @@ __global__ __launch_bounds__(kScanThreads) void nj_finish_kernel(NjDevice D, flo
 }
 
 // ---- the one-work-group step ------------------------------------------------------------------
-struct RowKey {  // (v, cyc) lexicographic; slot -1 = none
-  uint32_t v;
-  int cyc, slot;
-};
-
-__device__ __forceinline__ bool rk_less(const RowKey& x, const RowKey& y) {
-  if (x.slot < 0) return false;
-  if (y.slot < 0) return true;
-  if (x.v != y.v) return x.v < y.v;
-  return x.cyc < y.cyc;
-}
-
-// Fold the key of slot b into k when it has one and its cyclic position is below `below`.
-__device__ __forceinline__ void fold_key(unsigned long long key, int b, int start, int n, int below,
-                                         RowKey& k) {
-  if (key == kNoKey) return;
-  int cyc = (int)(uint32_t)key - start;
-  if (cyc < 0) cyc += n;
-  if (cyc >= below) return;
-  const RowKey c{(uint32_t)(key >> 32), cyc, b};
-  if (rk_less(c, k)) k = c;
-}
-
-// Wave-wide minimum (every lane gets it).
-__device__ __forceinline__ RowKey wave_min_key(RowKey k) {
+// Exclusive prefix (in thread order) of x under min (identity kNone) or + (identity 0) over
+// kStepThreads threads; *total gets the whole block's (`sh` holds kStepWaves words).
+template <bool kMin>
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t x, uint32_t* sh, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = x;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    RowKey o{(uint32_t)__shfl_xor((int)k.v, off), __shfl_xor(k.cyc, off), __shfl_xor(k.slot, off)};
-    if (rk_less(o, k)) k = o;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)inc, off);
+    if (lane >= off) inc = kMin ? min(inc, o) : inc + o;
   }
-  return k;
+  if (lane == 63) sh[wave] = inc;
+  __syncthreads();
+  uint32_t before = kMin ? kNone : 0u, all = kMin ? kNone : 0u;
+#pragma unroll
+  for (int w = 0; w < kStepWaves; ++w) {
+    const uint32_t y = sh[w];
+    if (w < wave) before = kMin ? min(before, y) : before + y;
+    all = kMin ? min(all, y) : all + y;
+  }
+  __syncthreads();
+  const uint32_t excl_lane = (uint32_t)__shfl_up((int)inc, 1);
+  const uint32_t mine = lane == 0 ? (kMin ? kNone : 0u) : excl_lane;
+  *total = all;
+  return kMin ? min(before, mine) : before + mine;
 }
 
-// Block-wide minimum: shuffles within waves, then one pass over the 16 wave results.
-__device__ RowKey block_min(RowKey k, RowKey* sh) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  k = wave_min_key(k);
-  if (lane == 0) sh[wave] = k;
-  __syncthreads();
-  RowKey r = sh[0];
-  for (int w = 1; w < kStepThreads / 64; ++w)
-    if (rk_less(sh[w], r)) r = sh[w];
-  __syncthreads();
-  return r;
-}
-
-// The first row (in visiting order) reaching the minimum among rows with a key (live, answer
-// below FLT_MAX) whose cyclic position is below `below`. The first pass (fill) copies the keys of
-// the first kKeyCache slots into `cache`; later passes read them from there (block_min's
-// barriers order the copy before them).
-__device__ RowKey best_row_before(const NjDevice& D, int used, int start, int n, int below,
-                                  RowKey* sh, unsigned long long* cache, bool fill) {
-  RowKey k{0xffffffffu, 0x7fffffff, -1};
-  // four keys per thread and pass, loaded together
-  for (int b0 = threadIdx.x; b0 < used; b0 += 4 * kStepThreads) {
-    unsigned long long key[4];
+// findMin over the rows' answers (rapidNJ.cpp:290-321): rows are visited by matrix index,
+// cyclically from `start`; a record is a row whose key is below every earlier one (and below
+// FLT_MAX's). The keys come by matrix index (key_by_index: kNone for rows without an answer,
+// gone rows and hidden copies), read coalesced into LDS, then each thread takes a run of
+// consecutive visiting positions: a block-wide exclusive minimum gives each run the minimum
+// before it. Gives the last four records' slots in visiting order (s_rec[0..*s_nrec-1]) and sets
+// *s_event when a record meets a running minimum at or below its break bound.
+__device__ void search_records(const NjDevice& D, int n, int start, uint32_t* kc, int* s_rec,
+                               int* s_nrec, int* s_event, uint32_t* shu) {
+  const int tid = threadIdx.x;
+  auto key_at = [&](int c) {  // visiting position c: index start + c (mod n)
+    int idx = start + c;
+    if (idx >= n) idx -= n;
+    return D.key_by_index[idx];
+  };
+  const int cached = min(n, kKeyCache);
+  for (int c0 = 0; c0 < cached; c0 += 4 * kStepThreads) {  // four loads in flight per thread
+    uint32_t k[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int b = b0 + u * kStepThreads;
-      key[u] = kNoKey;
-      if (b < used) key[u] = (fill || b >= kKeyCache) ? D.row_key[b] : cache[b];
+      const int c = c0 + u * kStepThreads + tid;
+      k[u] = c < cached ? key_at(c) : kNone;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int b = b0 + u * kStepThreads;
-      if (fill && b < used && b < kKeyCache) cache[b] = key[u];
-      fold_key(key[u], b, start, n, below, k);
+      const int c = c0 + u * kStepThreads + tid;
+      if (c < cached) kc[c] = k[u];
     }
   }
-  return block_min(k, sh);
+  __syncthreads();
+  const int per = (n + kStepThreads - 1) / kStepThreads;
+  const int c0 = min(n, tid * per), c1 = min(n, c0 + per);
+  uint32_t seg = kNone;
+  for (int c = c0; c < c1; ++c) seg = min(seg, c < kKeyCache ? kc[c] : key_at(c));
+  uint32_t unused;
+  uint32_t G = block_exclusive<true>(seg, shu, &unused);
+  G = min(G, fkey(FLT_MAX));  // the search starts at FLT_MAX (rapidNJ.cpp:291)
+  int cnt = 0, r0 = -1, r1 = -1, r2 = -1, r3 = -1;  // this thread's last records, newest first
+  bool unsafe = false;
+  for (int c = c0; c < c1; ++c) {
+    const uint32_t key = c < kKeyCache ? kc[c] : key_at(c);
+    if (key < G) {
+      int idx = start + c;
+      if (idx >= n) idx -= n;
+      const int slot = D.slot_of_index[idx];
+      unsafe |= G <= D.row_sigma[slot];
+      G = key;
+      r3 = r2;
+      r2 = r1;
+      r1 = r0;
+      r0 = slot;
+      ++cnt;
+    }
+  }
+  uint32_t total;
+  const int rank0 = (int)block_exclusive<false>((uint32_t)cnt, shu, &total);
+  const int R = (int)total, first = R > 4 ? R - 4 : 0;
+  const int mine[4] = {r0, r1, r2, r3};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int rank = rank0 + cnt - 1 - j;  // the j-th newest of this thread's records
+    if (j < cnt && rank >= first) s_rec[rank - first] = mine[j];
+  }
+  if (tid == 0) {
+    *s_nrec = R < 4 ? R : 4;
+    *s_event = 0;
+  }
+  __syncthreads();
+  if (unsafe) *s_event = 1;
+  __syncthreads();
 }
 
 __device__ __forceinline__ int red_pop(const NjDevice& D, int i) {
@@ -310,41 +404,60 @@ __device__ __forceinline__ int red_pop(const NjDevice& D, int i) {
   return j;
 }
 
-// Launch B, one work-group: findMin's result and caches (rapidNJ.cpp:290-345), mergeMinNodes
-// (:240-281), clusterCount--, updateData (:166-238) except the new separation sum, which the
-// next launch A finishes.
-__global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float* __restrict__ S) {
+// Launch B, one work-group: findMin's result and caches (rapidNJ.cpp:290-345) -- searched here,
+// or the host's when it set NjState::forced --, mergeMinNodes (:240-281), clusterCount--,
+// updateData (:166-238) except the new separation sum, which the next launch A finishes. The new
+// row goes to the log at log_off (by slot: distance and partner index, -1 where none).
+__global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float* __restrict__ S,
+                                                               size_t log_off) {
   NjState& st = *D.st;
   const int tid = threadIdx.x;
-  __shared__ RowKey sh[kStepThreads / 64];
-  __shared__ unsigned long long key_cache[kKeyCache];
+  __shared__ uint32_t key_cache[kKeyCache];
+  __shared__ uint32_t shu[kStepWaves];
+  __shared__ int s_rec[4], s_nrec, s_event;
   __shared__ int s_rows[2], s_skip[2], s_fail;
   __shared__ float s_mutual;
   if (st.error) return;
   const int n = st.n, used = st.used;
-  const int start = D.index_to_id[st.cache1] != -1 ? st.cache1 : st.cache2;
-  // records: rows where the running minimum improved, last first
-  // (unrolled with constant indices, so that rec stays in registers; nrec is uniform)
-  RowKey rec[4];
-  int nrec = 0;
-  int below = n + 1;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (nrec == r) {
-      rec[r] = best_row_before(D, used, start, n, below, sh, key_cache, r == 0);
-      if (rec[r].slot >= 0) {
-        below = rec[r].cyc;
-        nrec = r + 1;
-      }
-    }
+  const bool forced = st.forced != 0;
+  if (!forced) {
+    const int start = D.index_to_id[st.cache1] != -1 ? st.cache1 : st.cache2;
+    search_records(D, n, start, key_cache, s_rec, &s_nrec, &s_event, shu);
   }
   if (tid == 0) {
     // the winner must name live rows with slots, and the new cluster needs a free slot
-    bool bad = nrec == 0 || used >= (int)D.ld;
     int min1 = 0, min2 = 0, r1 = 0, r2 = 0, s1 = 0, s2 = 0;
-    if (!bad) {
-      min1 = (int)(uint32_t)D.row_key[rec[0].slot];
-      min2 = D.row_p[rec[0].slot];
+    int c1 = st.cache1, c2 = st.cache2;
+    bool bad = used >= (int)D.ld;
+    bool halt = false;
+    if (forced) {
+      min1 = st.f_min1;
+      min2 = st.f_min2;
+      c1 = st.f_cache1;
+      c2 = st.f_cache2;
+      bad = bad || min1 < 0 || min1 >= n;
+    } else {
+      const int nrec = s_nrec;
+      bad = bad || nrec == 0;
+      if (!bad) {
+        const int win = s_rec[nrec - 1];
+        min1 = (int)(uint32_t)D.row_key[win];
+        min2 = D.row_p[win];
+        halt = s_event != 0 || D.row_tie[win] != 0;
+        // caches: replay the improvements in visiting order (with 4 or more records only the
+        // last three matter, and they are distinct rows)
+        int m = st.min1;
+        for (int t = 0; t < nrec; ++t) {
+          const int row = (int)(uint32_t)D.row_key[s_rec[t]];
+          if (row != m) {
+            c2 = c1;
+            c1 = m;
+            m = row;
+          }
+        }
+      }
+    }
+    if (!bad && !halt) {
       bad = min2 < 0 || min2 >= n;
       if (!bad) {
         r1 = D.rtm[min1];
@@ -357,23 +470,13 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
         }
       }
     }
-    s_fail = bad;
+    s_fail = bad || halt;
     if (bad) {
-      st.error = 1;
+      st.error = kNjFailed;
+    } else if (halt) {
+      st.error = kNjHalted;
     } else {
-      // caches: replay the improvements in visiting order (records are listed last first; with
-      // 4 or more records only the last three matter, and they are distinct rows)
-      int m = st.min1, c1 = st.cache1, c2 = st.cache2;
-#pragma unroll
-      for (int t = 3; t >= 0; --t) {
-        if (t >= nrec) continue;
-        const int row = (int)(uint32_t)D.row_key[rec[t].slot];
-        if (row != m) {
-          c2 = c1;
-          c1 = m;
-          m = row;
-        }
-      }
+      st.forced = 0;
       st.min1 = min1;
       st.min2 = min2;
       st.cache1 = c1;
@@ -412,6 +515,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
       D.right[k] = D.index_to_id[obs_row];
       D.llen[k] = dist1;
       D.rlen[k] = dist2;
+      D.newidx[k] = new_row;
       st.new_row = new_row;
       st.obs_row = obs_row;
       st.cluster_count--;
@@ -433,6 +537,8 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
   const float denom = (float)(st.cluster_count - 2);
   const int ns = used;
   float* new_row_p = S + (size_t)ns * D.ld;
+  float* log_v = D.log_val + log_off;
+  int* log_i = D.log_idx + log_off;
   // Four slots per thread and pass, their loads issued together (every index is in bounds for
   // dead slots too: slots below `used`, and slot_index always holds a matrix index).
   const int skip0 = s_skip[0], skip1 = s_skip[1];
@@ -461,6 +567,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
       const int b = b0 + u * kStepThreads;
       if (b >= used) break;
       float dist = 0.0f, term = 0.0f;
+      int partner = -1;
       if (al[u]) {
         const int i = si[u];
         if (i != skip0 && i != skip1) {
@@ -472,6 +579,7 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
           const float ss = __fadd_rn(ssum[u], __fsub_rn(__fsub_rn(dist, val1), val2));
           D.sepsum[b] = ss;
           D.sep[b] = ss / denom;
+          partner = i;
         } else {
           D.alive[b] = 0;
           D.sep[b] = __int_as_float(0x7fc00000);
@@ -481,12 +589,15 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
       D.newcol[b] = dist;
       D.term[b] = term;
       new_row_p[b] = dist;  // the column S[b][ns] is written by the next launch A
+      log_v[b] = dist;
+      log_i[b] = partner;
     }
   }
   if (tid == 0) {
     new_row_p[ns] = 0.0f;
     const int new_row = st.new_row, obs_row = st.obs_row;
     D.slot_of_index[obs_row] = -1;
+    D.key_by_index[obs_row] = kNone;
     D.id_to_index[D.index_to_id[new_row]] = -1;
     D.id_to_index[D.index_to_id[obs_row]] = -1;
     D.index_to_id[obs_row] = -1;
@@ -506,23 +617,30 @@ __global__ __launch_bounds__(kStepThreads) void nj_step_kernel(NjDevice D, float
 }
 
 // ---- compaction ------------------------------------------------------------------------------
-// Positions of the live slots (one work-group, in slot order) into D.map, count into st.alive.
-__global__ __launch_bounds__(1024) void nj_compact_map_kernel(NjDevice D) {
-  NjState& st = *D.st;
-  __shared__ int part[1024];
-  const int used = st.used, tid = threadIdx.x;
-  const int per = (used + 1023) / 1024;
-  const int b0 = tid * per, b1 = min(used, b0 + per);
-  int cnt = 0;
-  for (int b = b0; b < b1; ++b) cnt += D.alive[b] ? 1 : 0;
-  part[tid] = cnt;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
+// Inclusive scan of part[0..1023] in place (one work-group of 1024).
+__device__ __forceinline__ void scan1024(int* part) {
+  const int tid = threadIdx.x;
+  for (int off = 1; off < 1024; off <<= 1) {
     const int x = tid >= off ? part[tid - off] : 0;
     __syncthreads();
     part[tid] += x;
     __syncthreads();
   }
+}
+
+// Positions of the live slots (one work-group, in slot order) into D.map, count into st.alive.
+__global__ __launch_bounds__(1024) void nj_compact_map_kernel(NjDevice D) {
+  NjState& st = *D.st;
+  __shared__ int part[1024];
+  if (st.error) return;
+  const int used = st.used, tid = threadIdx.x;
+  const int per = (used + 1023) / 1024;
+  const int b0 = min(used, tid * per), b1 = min(used, b0 + per);
+  int cnt = 0;
+  for (int b = b0; b < b1; ++b) cnt += D.alive[b] ? 1 : 0;
+  part[tid] = cnt;
+  __syncthreads();
+  scan1024(part);
   int pos = part[tid] - cnt;
   for (int b = b0; b < b1; ++b)
     if (D.alive[b]) D.map[pos++] = b;
@@ -532,6 +650,7 @@ __global__ __launch_bounds__(1024) void nj_compact_map_kernel(NjDevice D) {
 // New slot matrix: S2[a][b] = S[map[a]][map[b]] for a, b < alive (grid: rows x column blocks).
 __global__ __launch_bounds__(256) void nj_compact_copy_kernel(NjDevice D, const float* __restrict__ S,
                                                               float* __restrict__ S2) {
+  if (D.st->error) return;
   const int alive = D.st->alive;
   for (int a = blockIdx.y; a < alive; a += gridDim.y) {
     const float* src = S + (size_t)D.map[a] * D.ld;
@@ -544,6 +663,7 @@ __global__ __launch_bounds__(256) void nj_compact_copy_kernel(NjDevice D, const 
 // Per-slot arrays follow the new numbering (one work-group; in place, ascending: map[a] >= a).
 __global__ __launch_bounds__(1024) void nj_compact_slots_kernel(NjDevice D) {
   NjState& st = *D.st;
+  if (st.error) return;
   const int alive = st.alive, used = st.used;
   for (int base = 0; base < alive; base += 1024) {
     const int a = base + threadIdx.x;
@@ -662,7 +782,7 @@ hipError_t nj_launch_gather(const float* M, size_t ldm, const int* vis, int v, f
 }
 
 hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool finish_previous,
-                               hipStream_t s) {
+                               size_t log_off, hipStream_t s) {
   const int rows_blocks = (used_bound + kScanRows - 1) / kScanRows;
   const int grid = rows_blocks + (finish_previous ? 1 : 0);
   if (grid > 0) {
@@ -670,7 +790,12 @@ hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  nj_step_kernel<<<dim3(1), dim3(kStepThreads), 0, s>>>(D, S);
+  nj_step_kernel<<<dim3(1), dim3(kStepThreads), 0, s>>>(D, S, log_off);
+  return hipGetLastError();
+}
+
+hipError_t nj_launch_step(const NjDevice& D, float* S, size_t log_off, hipStream_t s) {
+  nj_step_kernel<<<dim3(1), dim3(kStepThreads), 0, s>>>(D, S, log_off);
   return hipGetLastError();
 }
 

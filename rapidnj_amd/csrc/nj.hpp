@@ -19,9 +19,15 @@ struct NjState {
   int new_row, obs_row;
   int step;           // joins made
   int negative_branches;
-  int error;          // set when an iteration finds no pair or runs out of slots (then all stop)
+  int error;          // kNjFailed or kNjHalted; every kernel does nothing while it is set
+  int forced;         // the next step takes f_* as its search result instead of searching
+  int f_min1, f_min2, f_cache1, f_cache2;
 };
 // A slot's separation is NaN once its cluster is gone, so that scans skip it without a flag.
+
+// NjState::error values.
+constexpr int kNjFailed = 1;  // an iteration found no pair or ran out of slots
+constexpr int kNjHalted = 2;  // the step left its search to the host (see nj.hip)
 
 // Device arrays of one run, passed by value to the kernels.
 struct NjDevice {
@@ -42,9 +48,18 @@ struct NjDevice {
   float* sep;          // separations
   unsigned long long* row_key;  // each row's best: (criterion key << 32) | row index; ~0 = none
   int* row_p;          // its partner index
+  unsigned* row_sigma; // its break bound (key): the search is the host's if a record meets a
+                       // running minimum at or below it
+  int* row_tie;        // 1 when several pairs share its best criterion and distance
+  float* row_maxsep;   // its maxRowSeparation, as the scan found it
+  float* init_maxsep;  // maxRowSeparations of the first search (every smaller index)
   float* newcol;       // new cluster's distances (step)
   float* term;         // dist * redundantCount terms of the new separation sum
   int* map;            // compaction: live slots in order
+  unsigned* key_by_index;  // each row's criterion key by matrix index (~0: no answer)
+  int* newidx;         // matrix index of the cluster made at each step
+  float* log_val;      // each step's new row, by slot: distance
+  int* log_idx;        //   and partner matrix index (-1: no partner)
   // joins (polytree::addInternalNode arguments), n - 2 of them
   int* left;
   int* right;
@@ -62,9 +77,12 @@ hipError_t nj_launch_same_rows(const float* M, size_t ldm, int n, const int* pai
 hipError_t nj_launch_gather(const float* M, size_t ldm, const int* vis, int v, float* S, size_t ld,
                             hipStream_t s);
 // One iteration over slots < used_bound: the scan (after finishing the previous join's new
-// cluster when `finish_previous`), then the step.
+// cluster when `finish_previous`), then the step, which logs its new row at log_val/log_idx +
+// log_off (used_bound entries).
 hipError_t nj_launch_iteration(const NjDevice& D, float* S, int used_bound, bool finish_previous,
-                               hipStream_t s);
+                               size_t log_off, hipStream_t s);
+// The step alone (after the host settled its search).
+hipError_t nj_launch_step(const NjDevice& D, float* S, size_t log_off, hipStream_t s);
 // Finish the last join's new cluster alone (before a compaction).
 hipError_t nj_launch_finish(const NjDevice& D, float* S, hipStream_t s);
 // Rebuild the slot matrix into S2 from the live slots (at most alive_bound of them).

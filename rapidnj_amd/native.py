@@ -96,6 +96,7 @@ SIGNATURES = {
     "rnj_nj_rows": (ctypes.c_int, [_vp, _vp, _u, ctypes.c_int, ctypes.POINTER(_vp)]),
     "rnj_nj_alignment": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "rnj_tree_size": (_u, [_vp]),
+    "rnj_tree_host_searches": (_u, [_vp]),
     "rnj_tree_joins": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rnj_tree_newick": (ctypes.c_int, [_vp, _vp, _vp, _sz, ctypes.POINTER(_sz)]),
     "rnj_tree_free": (ctypes.c_int, [_vp]),
@@ -545,6 +546,7 @@ class Tree:
     def __init__(self, handle: int):
         self.handle = handle
         self.n = lib().rnj_tree_size(handle)
+        self.host_searches = lib().rnj_tree_host_searches(handle)
 
     def joins(self):
         """(left, right, left_len, right_len, last (2,), last_dist) as polytree receives them."""

@@ -16,7 +16,14 @@ negative_branches off and on. The cases:
 * ``sim300``: 300 taxa x 800 sites, K2P;
 * ``sim300jc``: the same alignment, JC;
 * ``sim800``: 800 taxa x 400 sites, K2P; stored as the alignment only (the matrix is the
-  reference's K2P matrix of it, which the oracle reproduces bit for bit).
+  reference's K2P matrix of it, which the oracle reproduces bit for bit);
+* tie-heavy matrices written here directly (not from alignments), where rapidNJ's choice among
+  pairs of equal criterion and equal distance is std::sort's order of the row list, and where
+  its row-scan break (rapidNJ.cpp:340), rounded differently from the criterion, ends a row one
+  pair before a criterion one unit in the last place below the running minimum:
+  ``eq30``, ``eq64``, ``eq200`` (every distance 1), ``eqc120`` (every distance 0.0371),
+  ``int30`` and ``int300`` (random integers 1..3 and 1..5), ``block60`` and ``block140`` (two
+  distances, 0.2 within and 0.7 between random groups).
 
 npz keys per case <c>: <c>_matrix (n x n float32; absent for sim800), <c>_aln (uint8, sim800
 only), <c>_newick and <c>_newick_neg (uint8 text, trailing newline kept).
@@ -62,6 +69,37 @@ def ref_newick(matrix: np.ndarray, tmp: str, neg: bool) -> bytes:
         return fh.read()
 
 
+def all_equal(n: int, value: float) -> np.ndarray:
+    m = np.full((n, n), np.float32(value), dtype=np.float32)
+    np.fill_diagonal(m, 0)
+    return m
+
+
+def integers(n: int, hi: int, seed: int) -> np.ndarray:
+    a = np.random.default_rng(seed).integers(1, hi + 1, size=(n, n)).astype(np.float32)
+    m = np.tril(a, -1)
+    return m + m.T
+
+
+def blocks(n: int, groups: int, seed: int) -> np.ndarray:
+    lab = np.random.default_rng(seed).integers(0, groups, n)
+    m = np.where(lab[:, None] == lab[None, :], np.float32(0.2), np.float32(0.7)).astype(np.float32)
+    np.fill_diagonal(m, 0)
+    return m
+
+
+TIE_CASES = {
+    "eq30": lambda: all_equal(30, 1.0),
+    "eq64": lambda: all_equal(64, 1.0),
+    "eq200": lambda: all_equal(200, 1.0),
+    "eqc120": lambda: all_equal(120, 0.0371),
+    "int30": lambda: integers(30, 3, 1),
+    "int300": lambda: integers(300, 5, 3),
+    "block60": lambda: blocks(60, 2, 7),
+    "block140": lambda: blocks(140, 4, 6),
+}
+
+
 def dup_alignment() -> np.ndarray:
     base = simulate_alignment(52, 300, 0.02, 41)
     rows = list(base)
@@ -85,6 +123,7 @@ def main() -> None:
             "sim300": ref_matrix(sim300, "kim", tmp),
             "sim300jc": ref_matrix(sim300, "jc", tmp),
         }
+        cases.update({name: make() for name, make in TIE_CASES.items()})
         for name, m in cases.items():
             res[f"{name}_matrix"] = m
             for neg in (False, True):

@@ -54,14 +54,63 @@ def same_joins(tree, ref):
     assert np.float32(ld).tobytes() == np.float32(rld).tobytes()
 
 
-@pytest.mark.parametrize("case", ["dup", "short", "sat", "sim300", "sim300jc"])
+CASES = ["dup", "short", "sat", "sim300", "sim300jc"]
+# tie-heavy matrices (make_nj_golden.py): equal criterion and distance within a row, where the
+# reference takes std::sort's order, and rows its rounded scan bound ends early
+TIE_CASES = ["eq30", "eq64", "eq200", "eqc120", "int30", "int300", "block60", "block140"]
+
+
+@pytest.mark.parametrize("case", CASES + TIE_CASES)
 @pytest.mark.parametrize("neg", [False, True])
 def test_reference_trees(ctx, nj_golden, case, neg):
     m = nj_golden[f"{case}_matrix"]
     want = bytes(nj_golden[f"{case}_newick_neg" if neg else f"{case}_newick"])
+    print(f"nj {case} n={m.shape[0]} neg={neg}", flush=True)
     tree = ctx.nj(device(m), negative_branches=neg)
     assert tree.newick() == want
     same_joins(tree, O.nj(m, neg))
+    if case in TIE_CASES:
+        assert tree.host_searches > 0  # the host settled the tied or cut-short searches
+    else:
+        assert tree.host_searches == 0
+
+
+def _blocks(n, groups, seed, a=0.2, b=0.7):
+    lab = np.random.default_rng(seed).integers(0, groups, n)
+    m = np.where(lab[:, None] == lab[None, :], np.float32(a), np.float32(b)).astype(np.float32)
+    np.fill_diagonal(m, 0)
+    return m
+
+
+def _integers(n, hi, seed):
+    a = np.random.default_rng(seed).integers(1, hi + 1, size=(n, n)).astype(np.float32)
+    m = np.tril(a, -1)
+    return m + m.T
+
+
+@pytest.mark.parametrize("case", ["eq1500", "blocks1200", "int1000", "int900dup"])
+def test_tie_heavy_larger(ctx, case):
+    """Join-by-join equality with the oracle (rapidNJ's sorted lists and scan bound, restated)
+    on tie-heavy matrices large enough for many slot-matrix compactions and host searches in
+    every enqueue window."""
+    if case == "eq1500":
+        m = np.ones((1500, 1500), dtype=np.float32)
+        np.fill_diagonal(m, 0)
+    elif case == "blocks1200":
+        m = _blocks(1200, 6, 11)
+    elif case == "int1000":
+        m = _integers(1000, 4, 12)
+    else:  # integers with identical rows (redundant copies)
+        m = _integers(900, 3, 13)
+        for i, j in ((5, 17), (40, 17), (899, 3), (300, 301)):
+            m[i] = m[j]
+            m[:, i] = m[:, j]
+        np.fill_diagonal(m, 0)
+    print(f"nj {case}", flush=True)
+    tree = ctx.nj(device(m))
+    same_joins(tree, O.nj(m))
+    if case != "int900dup":  # (its ties never reach a winning row)
+        assert tree.host_searches > 0
 
 
 @pytest.mark.parametrize("model", ["k2p", "jc"])

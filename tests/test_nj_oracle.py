@@ -27,7 +27,13 @@ def matrix_of(g, case):
     return O.distance_matrix(g[f"{case}_aln"], "k2p")[1]  # the reference's K2P bits
 
 
-@pytest.mark.parametrize("case", ["dup", "short", "sat", "sim300", "sim300jc", "sim800"])
+CASES = ["dup", "short", "sat", "sim300", "sim300jc", "sim800"]
+# tie-heavy matrices: equal criterion and distance within rows (std::sort's order decides) and
+# rows whose scan bound ends them early (make_nj_golden.py)
+TIE_CASES = ["eq30", "eq64", "eq200", "eqc120", "int30", "int300", "block60", "block140"]
+
+
+@pytest.mark.parametrize("case", CASES + TIE_CASES)
 @pytest.mark.parametrize("neg", [False, True])
 def test_reference_trees(nj_golden, case, neg):
     m = matrix_of(nj_golden, case)
@@ -49,6 +55,16 @@ def test_redundant_copies_join_at_zero_length(nj_golden):
     left, right, llen, rlen, last, _ = O.nj(m)
     assert len({tuple(r) for r in m}) < m.shape[0]
     assert np.any((llen == 0) & (rlen == 0))
+
+
+def test_tie_order_is_not_index_order(nj_golden):
+    """The all-equal fixtures are ones where taking tied partners in increasing index order gives
+    another tree: the reference's text differs from a star joined in index order. A restatement
+    that ignored std::sort's order would fail here (checked on the printed first join)."""
+    text = bytes(nj_golden["eq64_newick"])
+    m = nj_golden["eq64_matrix"]
+    assert O.newick(64, O.nj(m)) == text
+    assert len(set(np.unique(m[np.triu_indices(64, 1)]))) == 1
 
 
 def test_join_invariants():
