@@ -386,13 +386,19 @@ RNJ_EXPORT int rnj_tree_free(rnj_tree* tree);
  * A request loop on its own thread admits submitted jobs while their HBM need fits the budget,
  * batches the admitted jobs into one multi-job tile launch per batch, keeps up to `slots` batches
  * on the device at once (each on its own stream, so uploads, counting and downloads of
- * different batches overlap) and hands each job's post-processed upper triangle back. */
+ * different batches overlap, and a job admitted while a batch runs is launched beside it) and
+ * hands each job's post-processed upper triangle back. A job is the reference's distance
+ * computation of one alignment (src/distanceCalculation/DistanceEstimate.cpp, the -o m / NJ
+ * matrix); the loop itself has no counterpart in the reference. */
 typedef struct {
-  unsigned slots;           /* batches in flight (default 2) */
+  unsigned slots;           /* batches in flight, each on its own stream (default and most: 4) */
   unsigned max_batch_jobs;  /* jobs per launch (default 64) */
   size_t hbm_budget;        /* bytes the server may hold; 0 = 80% of free HBM at creation */
   unsigned batch_window_us; /* hold a batch up to this long after its first job is ready, to
                                gather more jobs into the launch (default 0: launch at once) */
+  unsigned long long max_batch_work; /* counting work per launch, in pair-words (n(n-1)/2 x
+                               32-site words, summed over its jobs; a larger job runs alone);
+                               0 = 2^28, about 50 us: only small jobs share launches */
 } rnj_server_config;
 
 typedef struct {
@@ -406,6 +412,8 @@ typedef struct {
   double launch_s;   /* -> the job's batch enqueued on a slot */
   double run_s;      /* -> the batch seen finished by the request loop */
   double download_s; /* the copy of the triangle into tri_out, inside rnj_server_wait */
+  double launched_at; /* seconds from rnj_server_create to the job's batch enqueued (-1: never) */
+  double finished_at; /* ... to the batch seen finished by the request loop (-1: never ran) */
 } rnj_job_stats;
 
 RNJ_EXPORT int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server** out);
@@ -413,6 +421,14 @@ RNJ_EXPORT int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg,
  * rows whose type (rnj_set_input_type of the server's context) is protein give RNJ_E_NOT_DNA. */
 RNJ_EXPORT int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned n, unsigned L,
                       int model, uint64_t* ticket);
+/* A job on an alignment already in HBM (of the server's context; any kind: DNA under either
+ * character rules, protein): nothing is uploaded. The alignment must stay alive and unchanged
+ * until the job ends. tri_dev: the caller's device buffer of n(n-1)/2 floats for the
+ * post-processed upper triangle, valid once rnj_server_wait (or a successful poll) returns; NULL
+ * = a server buffer, copied to rnj_server_wait's tri_out. Jobs of one kind (and protein code
+ * width) share a launch. */
+RNJ_EXPORT int rnj_server_submit_resident(rnj_server* srv, const rnj_alignment* aln, int model,
+                                          float* tri_dev, uint64_t* ticket);
 /* Blocks until the job ends; tri_out (host, n(n-1)/2 floats, may be null) receives the
  * post-processed upper triangle. Releases the ticket. */
 RNJ_EXPORT int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_stats* stats);

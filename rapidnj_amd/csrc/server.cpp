@@ -1,5 +1,10 @@
 // Served distance-matrix jobs: a request loop that admits jobs by HBM budget and batches them.
 //
+// Jobs come two ways: characters on the host (rnj_server_submit: uploaded and packed by the
+// server), or an alignment already packed in HBM (rnj_server_submit_resident: nothing to upload;
+// the triangle goes to the caller's device buffer, or to a server buffer that rnj_server_wait
+// copies out).
+//
 // Admission. A submitted job is admitted when its device memory (characters, packed planes,
 // triangle, saturated-pair list) fits what the budget has left and no earlier job is waiting.
 // An admitted job is uploaded from the caller's buffer on the submitting thread (library threads
@@ -17,6 +22,7 @@
 // thread; its memory returns to the budget then. Each job's result is bit-identical to running it
 // alone: the counts are exact integers and every job keeps its own maximum and saturated-pair
 // list.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -37,6 +43,10 @@ struct ServedJob {
   uint64_t id = 0;
   unsigned n = 0, L = 0, n_pad = 0, W = 0;
   int model = 0;
+  int kind = kDNA;                 // jobs of one kind (and code width) share a launch
+  unsigned code_bits = 0;
+  const Alignment* aln = nullptr;  // resident job: the caller's alignment (planes in HBM)
+  bool own_tri = true;             // the triangle is a server block (else the caller's buffer)
   std::vector<char> chars;  // host copy, only while queued
   State state = State::Queued;
   int status = RNJ_OK;
@@ -61,6 +71,16 @@ struct Slot {
 };
 
 size_t pairs_of(unsigned n) { return (size_t)n * (n - 1) / 2; }
+
+// Counting work of a job in pair-words (pairs x 32-site words). A batch takes jobs while their
+// sum stays within max_batch_work (a job above it runs alone). Measured on MI355X with the
+// 512-job stream resident in HBM: several slots with one job per launch keep the device busiest
+// (4 slots: 66 ms; 2 slots: 81 ms), while batches of medium and large jobs ran slower (16-64 jobs
+// per launch: 85-90 ms). Batching pays for small jobs, whose launch costs more than their
+// counting; the default, 2^28 pair-words (about 50 us of counting), groups only those.
+unsigned long long job_work(unsigned n, unsigned W) { return (unsigned long long)pairs_of(n) * W; }
+constexpr unsigned long long kDefaultBatchWork = 1ull << 28;
+constexpr unsigned kMaxSlots = 4;
 
 size_t tri_bytes(unsigned n) { return pairs_of(n) * 4; }
 size_t chars_bytes(unsigned n, unsigned L) { return (size_t)n * L; }
@@ -141,6 +161,11 @@ size_t job_bytes(unsigned n, unsigned L) {
          DeviceCache::bucket(tri_bytes(n)) + saturation_bytes(n);
 }
 
+// The same for a resident job: its triangle when the server holds it, and its saturation records.
+size_t resident_job_bytes(unsigned n, bool own_tri) {
+  return (own_tri ? DeviceCache::bucket(tri_bytes(n)) : 0) + saturation_bytes(n);
+}
+
 }  // namespace
 
 struct rnj_server {
@@ -149,6 +174,7 @@ struct rnj_server {
   std::vector<Slot> slots;
   hipStream_t upload_stream = nullptr;
   hipStream_t download_stream = nullptr;
+  Clock::time_point t0 = Clock::now();  // the stats' launched_at / finished_at count from here
   std::mutex mu;
   std::mutex upload_mu;  // one upload at a time on the upload stream
   Stager uploader;       // pinned staging for the callers' characters (submitting threads)
@@ -174,6 +200,12 @@ struct rnj_server {
 // Returns when the caller's buffer has been read; the copies and the packing run on the upload
 // stream and the job's ready event follows them.
 int rnj_server::upload(ServedJob* j, const char* chars, size_t pitch) {
+  if (j->aln) {  // resident: its planes are complete; only a server triangle to take
+    const hipError_t e = j->own_tri ? blocks.get(tri_bytes(j->n), (void**)&j->tri) : hipSuccess;
+    if (e == hipSuccess) return RNJ_OK;
+    j->error = std::string("triangle: ") + hipGetErrorString(e);
+    return RNJ_E_HIP;
+  }
   std::lock_guard<std::mutex> ul(upload_mu);
   hipStream_t s = upload_stream;
   hipError_t e = blocks.get(chars_bytes(j->n, j->L), (void**)&j->chars_dev);
@@ -198,22 +230,32 @@ int rnj_server::upload(ServedJob* j, const char* chars, size_t pitch) {
 void rnj_server::release_job_memory(ServedJob* j) {
   blocks.put(j->chars_dev, chars_bytes(j->n, j->L));
   blocks.put(j->planes, planes_bytes(j->n, j->L));
-  blocks.put(j->tri, tri_bytes(j->n));
+  if (j->own_tri) blocks.put(j->tri, tri_bytes(j->n));
   j->chars_dev = nullptr;
   j->planes = nullptr;
   j->tri = nullptr;
 }
 
-// Memory a job holds after its batch ran: only its triangle (released by rnj_server_wait).
-static size_t held_after_batch(const ServedJob* j) { return DeviceCache::bucket(tri_bytes(j->n)); }
+// Memory a job holds after its batch ran: only a server triangle (released by rnj_server_wait).
+static size_t held_after_batch(const ServedJob* j) {
+  return j->own_tri ? DeviceCache::bucket(tri_bytes(j->n)) : 0;
+}
 
 // Enqueue the batch in s.jobs on slot s (worker thread, no server lock). Returns false with the
 // slot's error set if enqueueing failed.
 bool rnj_server::launch(Slot& s) {
   hipError_t e = hipSuccess;
   std::vector<JobSpec> specs;
-  for (ServedJob* j : s.jobs) {
-    if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, j->ready, 0);
+  // longest rows first: their tiles are dispatched first, and the launch ends on short ones
+  std::vector<ServedJob*> order(s.jobs);
+  std::stable_sort(order.begin(), order.end(),
+                   [](const ServedJob* a, const ServedJob* b) { return a->W > b->W; });
+  for (ServedJob* j : order) {
+    if (e == hipSuccess && j->ready) e = hipStreamWaitEvent(s.stream, j->ready, 0);
+    if (j->aln) {
+      specs.push_back(alignment_spec(j->aln, j->model, kTriangle, j->tri, 0));
+      continue;
+    }
     JobSpec js{};
     js.planes = j->planes;
     js.n = j->n;
@@ -302,9 +344,16 @@ void rnj_server::run() {
           break;
       }
       s.jobs.clear();
+      const int kind = ready.front()->kind;
+      const unsigned bits = ready.front()->code_bits;
+      unsigned long long work = 0;
       while (!ready.empty() && ready.front()->state == State::Ready &&
+             ready.front()->kind == kind && ready.front()->code_bits == bits &&
              s.jobs.size() < cfg.max_batch_jobs) {
         ServedJob* j = ready.front();
+        const unsigned long long w = job_work(j->n, j->W);
+        if (!s.jobs.empty() && work + w > cfg.max_batch_work) break;
+        work += w;
         ready.pop_front();
         j->state = State::Running;
         s.jobs.push_back(j);
@@ -381,7 +430,11 @@ int rnj_server_create(rnj_context* ctx, const rnj_server_config* cfg, rnj_server
   auto* srv = new rnj_server();
   srv->ctx = ctx;
   if (cfg) srv->cfg = *cfg;
-  if (srv->cfg.slots == 0) srv->cfg.slots = 2;
+  // at most as many slots as the device's hardware queues per process (4 on MI355X boxes):
+  // more streams share queues and gave no more overlap
+  if (srv->cfg.slots == 0) srv->cfg.slots = kMaxSlots;
+  srv->cfg.slots = std::min(srv->cfg.slots, kMaxSlots);
+  if (srv->cfg.max_batch_work == 0) srv->cfg.max_batch_work = kDefaultBatchWork;
   if (srv->cfg.max_batch_jobs == 0) srv->cfg.max_batch_jobs = 64;
   srv->cfg.max_batch_jobs = std::min(srv->cfg.max_batch_jobs, kMaxLaunchJobs);
   if (srv->cfg.hbm_budget == 0) {
@@ -486,6 +539,70 @@ int rnj_server_submit(rnj_server* srv, const char* chars, size_t pitch, unsigned
   return RNJ_OK;
 }
 
+int rnj_server_submit_resident(rnj_server* srv, const rnj_alignment* aln, int model, float* tri_dev,
+                               uint64_t* ticket) {
+  if (!srv || !aln || !ticket) return fail(RNJ_E_INVALID, "null argument");
+  if (!aln->planes) return fail(RNJ_E_INVALID, "null alignment");
+  if (aln->ctx != srv->ctx) return fail(RNJ_E_INVALID, "alignment belongs to another context");
+  if (aln->n < 2) return fail(RNJ_E_INVALID, "need at least 2 sequences");
+  if (model != RNJ_JC && model != RNJ_K2P) return fail(RNJ_E_INVALID, "model must be RNJ_JC or RNJ_K2P");
+  auto* j = new ServedJob();
+  j->n = aln->n;
+  j->L = aln->L;
+  j->n_pad = aln->n_pad;
+  j->W = aln->W;
+  j->model = model;
+  j->kind = aln->kind;
+  j->code_bits = aln->kind == kProtein ? aln->code_bits : 0;
+  j->aln = aln;
+  j->own_tri = tri_dev == nullptr;
+  j->tri = tri_dev;
+  j->need = resident_job_bytes(j->n, j->own_tri);
+  j->t_submit = Clock::now();
+  bool admit_now = false;
+  {
+    std::lock_guard<std::mutex> lock(srv->mu);
+    if (srv->stopping) {
+      delete j;
+      return fail(RNJ_E_INVALID, "server is shutting down");
+    }
+    j->id = srv->next_id++;
+    srv->jobs[j->id] = j;
+    *ticket = j->id;
+    if (srv->queue.empty() && j->need <= srv->cfg.hbm_budget &&
+        srv->used + j->need <= srv->cfg.hbm_budget) {
+      admit_now = true;
+      srv->used += j->need;
+      j->state = State::Uploading;
+      j->t_admit = j->t_submit;
+      srv->ready.push_back(j);  // keeps arrival order; batched once Ready
+    } else {
+      srv->queue.push_back(j);  // nothing to copy: the alignment stays the caller's
+    }
+  }
+  if (admit_now) {
+    const int rc = srv->upload(j, nullptr, 0);  // a server triangle, if any
+    std::lock_guard<std::mutex> lock(srv->mu);
+    if (rc == RNJ_OK) {
+      j->state = State::Ready;
+      j->t_ready = Clock::now();
+    } else {
+      for (auto it = srv->ready.begin(); it != srv->ready.end(); ++it)
+        if (*it == j) {
+          srv->ready.erase(it);
+          break;
+        }
+      srv->used -= j->need;
+      j->status = rc;
+      j->state = State::Done;
+      j->t_done = Clock::now();
+      srv->cv_done.notify_all();
+    }
+  }
+  srv->cv_work.notify_one();
+  return RNJ_OK;
+}
+
 int rnj_server_poll(rnj_server* srv, uint64_t ticket) {
   if (!srv) return fail(RNJ_E_INVALID, "null server");
   std::lock_guard<std::mutex> lock(srv->mu);
@@ -509,12 +626,12 @@ int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_st
     hipError_t e = hipSuccess;
     if (status == RNJ_OK && tri_out)
       e = srv->downloader.download(tri_out, j->tri, tri_bytes(j->n), srv->download_stream);
-    srv->blocks.put(j->tri, tri_bytes(j->n));  // the download is complete (or its stream drained)
-    j->tri = nullptr;
-    {
+    if (j->own_tri) {
+      srv->blocks.put(j->tri, tri_bytes(j->n));  // the download is complete (or its stream drained)
       std::lock_guard<std::mutex> relock(srv->mu);
       srv->used -= held_after_batch(j);
     }
+    j->tri = nullptr;
     srv->cv_work.notify_one();  // memory came back: queued jobs may fit now
     if (e != hipSuccess) {
       status = RNJ_E_HIP;
@@ -535,6 +652,8 @@ int rnj_server_wait(rnj_server* srv, uint64_t ticket, float* tri_out, rnj_job_st
     stats->launch_s = ran ? sec(j->t_launch - j->t_ready) : 0.0;
     stats->run_s = ran ? sec(j->t_done - j->t_launch) : 0.0;
     stats->download_s = sec(copied - copy_start);
+    stats->launched_at = ran ? sec(j->t_launch - srv->t0) : -1.0;
+    stats->finished_at = ran ? sec(j->t_done - srv->t0) : -1.0;
   }
   if (status != RNJ_OK) set_error(j->error);
   if (j->ready) (void)hipEventDestroy(j->ready);

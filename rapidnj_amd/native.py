@@ -103,6 +103,8 @@ SIGNATURES = {
     "rnj_server_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
     "rnj_server_submit": (ctypes.c_int, [_vp, _vp, _sz, _u, _u, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "rnj_server_submit_resident": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp,
+                                                  ctypes.POINTER(ctypes.c_uint64)]),
     "rnj_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _vp]),
     "rnj_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "rnj_server_cancel": (ctypes.c_int, [_vp, ctypes.c_uint64]),
@@ -115,14 +117,15 @@ _ROW_SINK = ctypes.CFUNCTYPE(ctypes.c_int, _vp, _u, ctypes.POINTER(ctypes.c_floa
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", _u), ("max_batch_jobs", _u), ("hbm_budget", _sz),
-                ("batch_window_us", _u)]
+                ("batch_window_us", _u), ("max_batch_work", ctypes.c_ulonglong)]
 
 
 class JobStats(ctypes.Structure):
     _fields_ = [("queued_s", ctypes.c_double), ("device_s", ctypes.c_double),
                 ("total_s", ctypes.c_double), ("batch_id", _u), ("batch_jobs", _u),
                 ("upload_s", ctypes.c_double), ("launch_s", ctypes.c_double),
-                ("run_s", ctypes.c_double), ("download_s", ctypes.c_double)]
+                ("run_s", ctypes.c_double), ("download_s", ctypes.c_double),
+                ("launched_at", ctypes.c_double), ("finished_at", ctypes.c_double)]
 
 
 class RnjError(RuntimeError):
@@ -535,9 +538,9 @@ class Context:
                                      int(negative_branches), ctypes.byref(h)))
         return Tree(h.value)
 
-    def server(self, slots: int = 2, max_batch_jobs: int = 64, hbm_budget: int = 0,
-               batch_window_us: int = 0) -> "Server":
-        return Server(self, slots, max_batch_jobs, hbm_budget, batch_window_us)
+    def server(self, slots: int = 4, max_batch_jobs: int = 64, hbm_budget: int = 0,
+               batch_window_us: int = 0, max_batch_work: int = 0) -> "Server":
+        return Server(self, slots, max_batch_jobs, hbm_budget, batch_window_us, max_batch_work)
 
 
 class Tree:
@@ -678,9 +681,9 @@ class Server:
     """Served jobs: admission by HBM budget, batching, overlap of batches on the device."""
 
     def __init__(self, ctx: Context, slots: int, max_batch_jobs: int, hbm_budget: int,
-                 batch_window_us: int = 0):
+                 batch_window_us: int = 0, max_batch_work: int = 0):
         self.ctx = ctx
-        cfg = ServerConfig(slots, max_batch_jobs, hbm_budget, batch_window_us)
+        cfg = ServerConfig(slots, max_batch_jobs, hbm_budget, batch_window_us, max_batch_work)
         h = _vp()
         check(lib().rnj_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
@@ -694,6 +697,17 @@ class Server:
                                       ctypes.byref(t)))
         return t.value
 
+    def submit_resident(self, aln: "Alignment", model="k2p", out=None) -> int:
+        """A job on an alignment already in HBM; `out` (a float32 device tensor of n(n-1)/2) receives
+        the triangle, else the server holds it until wait() copies it to the host."""
+        if out is not None:
+            assert out.is_cuda and out.is_contiguous() and out.numel() >= aln.n * (aln.n - 1) // 2
+        t = ctypes.c_uint64()
+        check(lib().rnj_server_submit_resident(self.handle, aln.handle, _model(model),
+                                               out.data_ptr() if out is not None else None,
+                                               ctypes.byref(t)))
+        return t.value
+
     def poll(self, ticket: int) -> bool:
         rc = lib().rnj_server_poll(self.handle, ticket)
         if rc == RNJ_E_NOT_READY:
@@ -701,11 +715,13 @@ class Server:
         check(rc)
         return True
 
-    def wait(self, ticket: int, n: int) -> tuple[np.ndarray, JobStats]:
-        out = np.empty(max(n * (n - 1) // 2, 1), dtype=np.float32)
+    def wait(self, ticket: int, n: int, copy: bool = True):
+        """(triangle on the host, or None without `copy`; the job's JobStats)."""
+        out = np.empty(max(n * (n - 1) // 2, 1), dtype=np.float32) if copy else None
         st = JobStats()
-        check(lib().rnj_server_wait(self.handle, ticket, out.ctypes.data, ctypes.byref(st)))
-        return out[: n * (n - 1) // 2], st
+        check(lib().rnj_server_wait(self.handle, ticket, out.ctypes.data if copy else None,
+                                    ctypes.byref(st)))
+        return (out[: n * (n - 1) // 2] if copy else None), st
 
     def cancel(self, ticket: int) -> None:
         check(lib().rnj_server_cancel(self.handle, ticket))

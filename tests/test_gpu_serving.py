@@ -120,3 +120,86 @@ def test_server_argument_errors(ctx, N):
             srv.wait(123456, 5)
         with pytest.raises(N.RnjError):
             srv.cancel(987654)
+
+
+def _protein(n, L, seed):
+    rng = np.random.default_rng(seed)
+    return np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY-", dtype=np.uint8)[rng.integers(0, 21, (n, L))]
+
+
+def test_resident_jobs_equal_alone(ctx):
+    """Jobs on alignments already in HBM (DNA under both character rules, protein), triangles in
+    the caller's device buffers or in server buffers copied by wait: each equals the same
+    alignment run alone, bit for bit; jobs of one kind share launches."""
+    import torch
+    specs = [(simulate_alignment(int(n), int(L), 0.05, 900 + k), m, "bits")
+             for k, (n, L, m) in enumerate([(300, 900, "k2p"), (2, 5, "jc"), (120, 4000, "jc"),
+                                            (700, 333, "k2p"), (64, 64, "k2p")])]
+    specs += [(_protein(150, 640, 3), "k2p", "bits"), (_protein(90, 300, 4), "jc", "bits"),
+              (simulate_alignment(200, 500, 0.1, 77), "k2p", "naive")]
+    alns = []
+    for a, _, rules in specs:
+        ctx.set_char_rules(rules == "naive")
+        try:
+            alns.append(ctx.alignment(a))
+        finally:
+            ctx.set_char_rules(False)
+    refs = [ctx.distances(A, m).cpu().numpy() for A, (_, m, _) in zip(alns, specs)]
+    outs = [torch.full((A.n * (A.n - 1) // 2,), -7.0, device="cuda:0") if k % 2 == 0 else None
+            for k, A in enumerate(alns)]
+    with ctx.server(slots=2, max_batch_jobs=8, batch_window_us=200_000) as srv:
+        ts = [srv.submit_resident(A, m, out=o) for A, (_, m, _), o in zip(alns, specs, outs)]
+        res = [srv.wait(t, A.n) for t, A in zip(ts, alns)]
+    for k, ((host, st), ref) in enumerate(zip(res, refs)):
+        np.testing.assert_array_equal(host.view(np.uint32), ref.view(np.uint32))
+        if outs[k] is not None:
+            np.testing.assert_array_equal(outs[k].cpu().numpy().view(np.uint32), ref.view(np.uint32))
+        assert st.upload_s < 0.05 and st.launched_at >= 0 and st.finished_at >= st.launched_at
+    assert max(st.batch_jobs for _, st in res) > 1
+    for A in alns:
+        A.close()
+
+
+def test_job_admitted_during_running_batch_runs_beside_it(ctx):
+    """A job admitted while another batch runs on the device is launched on the other slot
+    before that batch finishes, and its work completes first: the two overlap on the device."""
+    import torch
+    rng = np.random.default_rng(5)
+    big = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, (8000, 60000), dtype=np.uint8)]
+    A_big = ctx.alignment(big)
+    del big
+    A_small = ctx.alignment(simulate_alignment(100, 1000, 0.02, 6))
+    out_big = torch.empty((8000 * 7999 // 2,), device="cuda:0")
+    ref_small = ctx.distances(A_small, "jc").cpu().numpy()
+    import time
+    with ctx.server(slots=2, max_batch_jobs=8) as srv:
+        tb = srv.submit_resident(A_big, "k2p", out=out_big)
+        time.sleep(0.003)  # the big job's batch is launched by now
+        ts = srv.submit_resident(A_small, "jc")
+        small, st_small = srv.wait(ts, 100)
+        _, st_big = srv.wait(tb, 8000, copy=False)
+    np.testing.assert_array_equal(small.view(np.uint32), ref_small.view(np.uint32))
+    print("big", st_big.launched_at, st_big.finished_at, "small", st_small.launched_at,
+          st_small.finished_at, flush=True)
+    assert st_small.batch_id != st_big.batch_id
+    assert st_big.launched_at < st_small.launched_at < st_big.finished_at
+    assert st_small.finished_at < st_big.finished_at
+    A_big.close()
+    A_small.close()
+
+
+def test_resident_errors(ctx, N):
+    other = N.Context(0)
+    try:
+        A_other = other.alignment(simulate_alignment(10, 50, 0.0, 1))
+        with ctx.server() as srv:
+            with pytest.raises(N.RnjError):
+                srv.submit_resident(A_other, "k2p")
+            A = ctx.alignment(simulate_alignment(10, 50, 0.0, 2))
+            with pytest.raises(N.RnjError):
+                srv.submit_resident(A, 7)
+            t = srv.submit_resident(A, "jc")
+            srv.wait(t, 10)
+            A.close()
+    finally:
+        other.close()
