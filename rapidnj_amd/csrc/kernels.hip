@@ -168,10 +168,15 @@ __device__ __forceinline__ void counts_from_same(uint32_t (&acc_a)[MI][MJ],
 // One work-group computes a TILE x TILE block of taxon pairs (bi <= bj) over one range of words
 // (the whole alignment, or split `split` of J.splits); thread (ty, tx) owns the MI x MJ
 // micro-tile i = bi*TILE + ty*MI + a, j = bj*TILE + tx*MJ + b. Sites are walked in chunks of CW
-// words: the chunk's H/L/V words of both taxon blocks are staged in LDS (layout
-// [plane][word][taxon], written linearly), the next chunk is prefetched into registers while
-// the current one is counted. Unsplit jobs finish in place; split jobs leave exact partial
-// counts for finish_kernel.
+// words; the chunk's H/L/V words of both taxon blocks are staged in LDS (layout
+// [plane][word][taxon], written linearly) while the previous chunk is counted:
+//  * work-groups of several waves load them straight from global memory into LDS
+//    (global_load_lds_dwordx4, each wave filling 64 consecutive uint4) into two buffers in
+//    turn, so no registers hold a chunk on the way (t64 K2P: 103 VGPRs instead of 110, JC 79
+//    instead of 94, 6 waves per SIMD);
+//  * one-wave work-groups prefetch the next chunk into registers and write it to one buffer:
+//    a second buffer would cost them occupancy, which LDS bounds for them.
+// Unsplit jobs finish in place; split jobs leave exact partial counts for finish_kernel.
 template <int TILE, int MI, int MJ, int CW, int MODE, bool kExact>
 __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
     const JobDesc* __restrict__ jobs, unsigned njobs) {
@@ -181,10 +186,13 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
   constexpr int P = MODE == kModeNaive ? 4 : 3;  // planes
   constexpr int E = P * CW * Q;               // uint4 per block and chunk
   static_assert(MI % 4 == 0 && MJ % 4 == 0, "micro-tiles are read as uint4");
+  static_assert(NT % 64 == 0, "whole waves");
   constexpr int EP = (E + NT - 1) / NT;       // uint4 per thread and block (last may be partial)
+  constexpr bool kDirect = NT >= 128;         // LDS loads, two buffers
+  constexpr int NB = kDirect ? 2 : 1;
 
-  __shared__ v4u sI[E];
-  __shared__ v4u sJ[E];
+  __shared__ v4u sI[NB][E];
+  __shared__ v4u sJ[NB][E];
 
   const unsigned item = xcd_item(blockIdx.x, gridDim.x);
   const unsigned k = find_job(jobs, njobs, item);
@@ -196,6 +204,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
 
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
+  const unsigned wave0 = tid & ~63u;  // the wave's first thread
   const gptr<v4u> gI = (gptr<v4u>)(J.planes + (size_t)bi * TILE);
   const gptr<v4u> gJ = (gptr<v4u>)(J.planes + (size_t)bj * TILE);
   const size_t row4 = J.n_pad / 4;                       // uint4 per word row
@@ -213,24 +222,55 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
   const unsigned w1 = min(J.W_live, w0 + J.words_per_split);
   const unsigned c0 = w0 / CW, c1 = (w1 + CW - 1) / CW;  // ranges start on chunk boundaries
 
-  v4u pI[EP], pJ[EP];
-  if (c0 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c0, row4, plane4);
-  for (unsigned c = c0; c < c1; ++c) {
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < EP; ++e) {
-      if (E % NT != 0 && tid + e * NT >= (unsigned)E) break;
-      sI[tid + e * NT] = pI[e];
-      sJ[tid + e * NT] = pJ[e];
-    }
-    __syncthreads();
-    if (c + 1 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
-
+  auto count_chunk = [&](int buf, unsigned c) {
     const int live = (int)min((unsigned)CW, w1 - c * CW);  // uniform over the work-group
 #pragma unroll 4
     for (int w = 0; w < CW; ++w) {
       if (w >= live) break;
-      count_word<MI, MJ, CW, Q, MODE>(sI, sJ, w, ty, tx, acc_a, acc_b, acc_v);
+      count_word<MI, MJ, CW, Q, MODE>(sI[buf], sJ[buf], w, ty, tx, acc_a, acc_b, acc_v);
+    }
+  };
+
+  if constexpr (kDirect) {
+    // Chunk `chunk` of both blocks into buffer `buf`: element lin = q + Q*(w + CW*p) is word
+    // row (chunk*CW + w) of plane p, uint4 column q; lane l of a wave writes element (base + l).
+    auto stage = [&](int buf, unsigned chunk) {
+#pragma unroll
+      for (int e = 0; e < EP; ++e) {
+        const unsigned lin = tid + e * NT;
+        if (E % NT != 0 && lin >= (unsigned)E) break;
+        const unsigned q = lin % Q, w = (lin / Q) % CW, p = lin / (Q * CW);
+        const size_t off = p * plane4 + (size_t)(chunk * CW + w) * row4 + q;
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(gI + off), (__attribute__((address_space(3))) void*)&sI[buf][wave0 + e * NT],
+            16, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(gJ + off), (__attribute__((address_space(3))) void*)&sJ[buf][wave0 + e * NT],
+            16, 0, 0);
+      }
+    };
+    if (c0 < c1) stage(0, c0);
+    for (unsigned c = c0; c < c1; ++c) {
+      const int buf = (int)((c - c0) & 1u);
+      __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's part of chunk c is in LDS
+      __syncthreads();  // every wave's part too, and every wave is done with the other buffer
+      if (c + 1 < c1) stage(buf ^ 1, c + 1);
+      count_chunk(buf, c);
+    }
+  } else {
+    v4u pI[EP], pJ[EP];
+    if (c0 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c0, row4, plane4);
+    for (unsigned c = c0; c < c1; ++c) {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < EP; ++e) {
+        if (E % NT != 0 && tid + e * NT >= (unsigned)E) break;
+        sI[0][tid + e * NT] = pI[e];
+        sJ[0][tid + e * NT] = pJ[e];
+      }
+      __syncthreads();
+      if (c + 1 < c1) fetch_chunk<EP, NT, Q, CW, E>(pI, pJ, gI, gJ, tid, c + 1, row4, plane4);
+      count_chunk(0, c);
     }
   }
   counts_from_same<MI, MJ, MODE>(acc_a, acc_v);
