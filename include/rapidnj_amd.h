@@ -155,6 +155,18 @@ RNJ_EXPORT int rnj_get_tile_config_for(rnj_context* ctx, const rnj_alignment* al
  * path, which no data sheet states. */
 RNJ_EXPORT int rnj_valu_issue_rates(rnj_context* ctx, double* bitwise, double* bcnt);
 
+/* Count-loop instruction mixes (rnj_valu_mix_rate). */
+enum { RNJ_MIX_K2P = 0, RNJ_MIX_JC = 1, RNJ_MIX_NAIVE = 2, RNJ_MIX_PROTEIN = 3 };
+/* Measured rate (lane-operations per second) at which this context's device executes one count
+ * loop's own per-pair instruction sequence: K2P (v_and, 2 v_bitop3, 3 v_bcnt), JC (v_and,
+ * 2 v_bitop3, 2 v_bcnt), the naive rules (v_and, v_xor, 4 v_bitop3, 3 v_bcnt) or protein with
+ * `planes` = 5, 6 or 8 code planes (v_xor, planes - 1 v_bitop3, v_and, 2 v_bcnt), from registers
+ * at 8 waves per SIMD, best of three timed launches. Mixed, the two classes do not reach the
+ * rates rnj_valu_issue_rates measures for each alone, so this is the VALU ceiling of a count
+ * loop: no schedule of that instruction sequence runs faster. Takes a few milliseconds. No
+ * counterpart in the reference. */
+RNJ_EXPORT int rnj_valu_mix_rate(rnj_context* ctx, int mix, int planes, double* rate);
+
 /* Type of the alignments this context reads from characters, rows and FASTA from now on:
  * RNJ_TYPE_AUTO (default) classifies the first sequence as the reference's readers do
  * (dataloaderFasta.hpp:54-76: DNA unless a byte >= 'A' is not one of ACGTUN of either case, else

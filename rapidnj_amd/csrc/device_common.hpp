@@ -33,6 +33,23 @@ __device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {
   return r;
 }
 
+// The same operations as volatile asm, which the compiler keeps in program order: the count
+// loop issues each pair's and / bitop3 / bcnt instructions back to back.
+__device__ __forceinline__ uint32_t ordered_and(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm volatile("v_and_b32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+template <unsigned kTable>
+__device__ __forceinline__ uint32_t ordered_bitop3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm volatile("v_bitop3_b32 %0, %1, %2, %3 bitop3:%4" : "=v"(r) : "v"(a), "v"(b), "v"(c), "i"(kTable));
+  return r;
+}
+__device__ __forceinline__ void ordered_bcnt_add(uint32_t x, uint32_t& acc) {
+  asm volatile("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(x));
+}
+
 // Pointer in the global address space, so loads compile to global_load rather than flat_load
 // (the job table hands over generic pointers).
 template <class T>

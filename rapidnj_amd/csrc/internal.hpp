@@ -130,6 +130,11 @@ unsigned tile_threads(int config);  // work-group size of a configuration
 // Issue-rate probe (op 0: v_bitop3_b32, 1: v_bcnt_u32_b32): blocks x 256 threads, each running
 // 16 dependency chains for iters x 32 instructions; out holds blocks x 256 u32.
 hipError_t launch_issue_probe(int op, uint32_t* out, unsigned blocks, int iters, hipStream_t s);
+// Count-loop instruction mixes timed by the mix probe (values of rnj_valu_mix_rate's `mix`).
+enum CountMix : int { kMixK2P = 0, kMixJC = 1, kMixNaive = 2, kMixProtein = 3 };
+int mix_ops_per_pair(int mix, int planes);
+hipError_t launch_mix_probe(int mix, int planes, uint32_t* out, unsigned blocks, int iters,
+                            hipStream_t s);
 
 // After the tile (and finish) kernels of a launch: job_max_kernel folds each job's wave words
 // into its maximum, then, unless raw, patch_kernel writes 2 * maximum over the saturated pairs

@@ -839,6 +839,49 @@ int rnj_valu_issue_rates(rnj_context* ctx, double* bitwise, double* bcnt) {
   return RNJ_OK;
 }
 
+static_assert(RNJ_MIX_K2P == kMixK2P && RNJ_MIX_JC == kMixJC && RNJ_MIX_NAIVE == kMixNaive &&
+                  RNJ_MIX_PROTEIN == kMixProtein,
+              "the ABI's mix values are the probe's");
+
+int rnj_valu_mix_rate(rnj_context* ctx, int mix, int planes, double* rate) {
+  if (!ctx || !rate) return fail(RNJ_E_INVALID, "null argument");
+  if (mix < RNJ_MIX_K2P || mix > RNJ_MIX_PROTEIN)
+    return fail(RNJ_E_INVALID, "mix must be RNJ_MIX_K2P, _JC, _NAIVE or _PROTEIN");
+  if (mix == RNJ_MIX_PROTEIN && planes != 5 && planes != 6 && planes != 8)
+    return fail(RNJ_E_INVALID, "protein code planes must be 5, 6 or 8");
+  const int rc = set_device(ctx);
+  if (rc != RNJ_OK) return rc;
+  // two rounds of 8 waves per SIMD (work-groups of 4 waves, 8 per CU at a time)
+  const unsigned blocks = (unsigned)std::max(1, ctx->prop.multiProcessorCount) * 16;
+  const int iters = 1000;
+  hipStream_t s = ctx->stream;
+  uint32_t* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&out, (size_t)blocks * 256 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_mix_probe(mix, planes, out, blocks, iters / 4, s);  // warm
+  float best = 0.0f;
+  for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+    e = hipEventRecord(e0, s);
+    if (e == hipSuccess) e = launch_mix_probe(mix, planes, out, blocks, iters, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && ms > 0.0f && (best == 0.0f || ms < best)) best = ms;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (out) (void)hipFree(out);
+  RNJ_HIP(e);
+  // lane-ops: threads x iterations x 16 pairs x instructions per pair
+  *rate = best > 0.0f ? (double)blocks * 256 * iters * 16 * mix_ops_per_pair(mix, planes) /
+                            ((double)best * 1e-3)
+                      : 0.0;
+  return RNJ_OK;
+}
+
 int rnj_set_char_rules(rnj_context* ctx, int rules) {
   if (!ctx) return fail(RNJ_E_INVALID, "null context");
   if (rules != RNJ_CHARS_BITS && rules != RNJ_CHARS_NAIVE)

@@ -24,6 +24,7 @@ TYPE_AUTO, TYPE_DNA, TYPE_PROTEIN = 0, 1, 2
 TRIANGLE, SQUARE = 0, 1
 FLAG_RAW = 1
 FLAG_EXACT = 2
+MIX_K2P, MIX_JC, MIX_NAIVE, MIX_PROTEIN = 0, 1, 2, 3
 
 _vp = ctypes.c_void_p
 _u = ctypes.c_uint
@@ -75,6 +76,8 @@ SIGNATURES = {
     "rnj_copy_to_host": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "rnj_valu_issue_rates": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)]),
+    "rnj_valu_mix_rate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double)]),
     "rnj_distances_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "rnj_distances_rows_lower": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "rnj_distances_stream_rows": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp]),
@@ -418,6 +421,14 @@ class Context:
         a, b = ctypes.c_double(), ctypes.c_double()
         check(lib().rnj_valu_issue_rates(self.handle, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def valu_mix_rate(self, mix: str, planes: int = 0) -> float:
+        """Measured lane-ops/s of one count loop's instruction sequence ("k2p", "jc", "naive" or
+        "protein" with its code planes) on this device: that loop's VALU ceiling."""
+        m = {"k2p": MIX_K2P, "jc": MIX_JC, "naive": MIX_NAIVE, "protein": MIX_PROTEIN}[mix]
+        r = ctypes.c_double()
+        check(lib().rnj_valu_mix_rate(self.handle, m, planes, ctypes.byref(r)))
+        return r.value
 
     def to_host(self, t, stream=None) -> np.ndarray:
         """A contiguous device tensor as a numpy array, copied through the library's pinned

@@ -122,3 +122,19 @@ def test_valu_issue_rates(ctx):
     """The issue-rate probe behind the roofline peak: both classes measured, bitwise faster."""
     bitwise, bcnt = ctx.valu_issue_rates()
     assert 1e12 < bcnt < bitwise < 1e15
+
+
+def test_valu_mix_rates(ctx, N):
+    """The count loops' instruction-mix probe behind the roofline peak: every mix is measured,
+    sits between the two classes' separate rates (mixed, neither class is faster than alone),
+    and invalid mixes or plane counts are refused."""
+    bitwise, bcnt = ctx.valu_issue_rates()
+    for mix, planes in (("k2p", 0), ("jc", 0), ("naive", 0), ("protein", 5), ("protein", 6),
+                        ("protein", 8)):
+        rate = ctx.valu_mix_rate(mix, planes)
+        assert 0.9 * bcnt < rate < 1.05 * bitwise, (mix, planes, rate, bcnt, bitwise)
+    with pytest.raises(N.RnjError):
+        ctx.valu_mix_rate("protein", 7)
+    import ctypes
+    r = ctypes.c_double()
+    assert N.lib().rnj_valu_mix_rate(ctx.handle, 9, 0, ctypes.byref(r)) == N.RNJ_E_INVALID
