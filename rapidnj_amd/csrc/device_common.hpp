@@ -9,22 +9,6 @@
 namespace rnj {
 namespace {
 
-// (a ^ b) & c in one v_bitop3_b32 (truth table over a=0xF0, b=0xCC, c=0xAA).
-__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x28u);
-}
-
-// ~(a ^ b) & c, (a ^ b) | ~c and a & b & c, each one v_bitop3_b32 (same truth-table inputs).
-__device__ __forceinline__ uint32_t xnor_and(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x82u);
-}
-__device__ __forceinline__ uint32_t xor_or_not(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x7Du);
-}
-__device__ __forceinline__ uint32_t and3(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x80u);
-}
-
 // popcount(x) + acc as one v_bcnt_u32_b32. Written out because LLVM otherwise re-associates
 // the per-word accumulations into v_bcnt + v_add3 trees (1.5 extra VALU ops per pair and word).
 __device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {
@@ -33,8 +17,9 @@ __device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {
   return r;
 }
 
-// The same operations as volatile asm, which the compiler keeps in program order: the count
-// loop issues each pair's and / bitop3 / bcnt instructions back to back.
+// The count loops' instructions as volatile asm, which the compiler keeps in program order.
+// v_bitop3_b32's table is over a = 0xF0, b = 0xCC, c = 0xAA: 0x28 = (a ^ b) & c,
+// 0x82 = ~(a ^ b) & c, 0x7D = (a ^ b) | ~c, 0x80 = a & b & c, 0xF6 = a | (b ^ c).
 __device__ __forceinline__ uint32_t ordered_and(uint32_t a, uint32_t b) {
   uint32_t r;
   asm volatile("v_and_b32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
@@ -94,7 +79,7 @@ __device__ __forceinline__ uint32_t* partial_slot(const JobDesc& J, unsigned til
 //    from the float proportions in the reference's order (exact for K2P), and the saturation
 //    tests on them, so the saturated pairs (-1) are the same;
 //  * zero distances: ts = tv = 0 gives -0.0f in both models, as the reference's -c*log(1.0) does.
-// The logarithms of those double arguments use log_unit (below) unless kExact, which calls the
+// The logarithms of those double arguments use log_table (above) unless kExact, which calls the
 // library's compensated FP64 log on the reference's expression; k2p_value / jc_value state how
 // the default path regroups them and what that costs in accuracy.
 #pragma clang fp contract(off)
@@ -103,56 +88,72 @@ __device__ __forceinline__ float proportion(uint32_t a, uint32_t valid) {
   return (float)((double)a / (double)valid);
 }
 
-// Natural log for 0 < x <= 1 (a normal double), in plain FP64 arithmetic: x = 2^k * m with
-// m in [sqrt(1/2), sqrt(2)), s = (m - 1) / (m + 1), log(m) = 2 atanh(s) = 2s (1 + s^2/3 + ... +
-// s^20/21), which leaves a truncation below 2^-56 for |s| <= 0.1716, and k ln 2 with ln 2 split
-// into a 32-bit head (k * head is exact) and a tail. The quotient uses the hardware reciprocal
-// refined by two Newton steps. The result is within about 2 ulp of double, so after rounding to
-// float it equals the reference's float value except in rare ties to a float rounding boundary;
-// it costs about a third of the instructions of the compensated library log.
-// log_unit's FP64 constants as values held in SGPRs for a whole epilogue. A VOP3 instruction
-// takes no 64-bit literal, so constants written as literals were rebuilt with two v_mov for
-// every pair an epilogue corrects; an empty asm with an SGPR operand makes each one an opaque
-// uniform value that the FP64 FMAs read straight from its SGPR pair.
-struct LogUnitConsts {
-  double inv_odd[10];  // 1/21, 1/19, ..., 1/3
-  double ln2_hi, ln2_lo, sqrt_half;
+// Natural logarithm of a normal double 0 < z <= 1 in FP64 arithmetic, through a 128-entry table
+// (Tang's table-driven method): z = 2^k m with m in [0.5, 1); i = the mantissa's 7 bits below the
+// leading one picks m's interval [0.5 + i/256, 0.5 + (i+1)/256) and its right end
+// c = 0.5 + (i+1)/256 (the last is 1, so near z = 1 the reduced argument is z - 1 itself); m - c
+// is exact and r = (m - c) / c = (m - c) * (1/c) has -2^-7 < r <= 0; then
+//   ln z = k ln 2 + ln c + ln(1 + r),  ln(1 + r) = r - r^2/2 + r^3/3 - r^4/4 + r^5/5,
+// whose truncation is below |r|^6/6 < 4e-14 absolute (and 5e-12 relative on the last interval,
+// where ln c = 0). Where ln c != 0 and k = 0, |ln z| >= ln(256/255), so the relative error is
+// below 2e-11 everywhere: rounded to float, the result equals the reference's libm value except
+// in ties within ~1e-9 of a float rounding boundary. 1/c and ln c come rounded to double from the
+// table (log_table.inc, written by gen_log_table.py); k ln 2 uses ln 2 split into a 32-bit head
+// (k * head is exact) and a tail. z = 1 (= 2^1 * 0.5, which the reduction would only bring to
+// within 1e-13 of 0) returns 0 exactly, as libm does. 20 VALU instructions and one LDS read: the
+// kernel stages the table in LDS (stage_log_table), so that every lane reads its own entry
+// without a divergent global load.
+struct LogEntry {
+  double inv_c, ln_c;
 };
 
-__device__ __forceinline__ LogUnitConsts log_unit_consts() {
-  LogUnitConsts K = {{1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0,
-                      1.0 / 9.0, 1.0 / 7.0, 1.0 / 5.0, 1.0 / 3.0},
-                     0x1.62e42ff000000p-1,    // ln 2, 32-bit head: k * head is exact
-                     -0x1.718432a1b0e26p-35,  // ln 2 - head
-                     0.70710678118654757};
-#pragma unroll
-  for (int q = 0; q < 10; ++q) asm volatile("" : "+s"(K.inv_odd[q]));
+__device__ const LogEntry kLogTable[128] = {
+#include "log_table.inc"
+};
+
+// The work-group's `nthreads` threads copy the table into `dst` (LDS); the caller synchronises
+// before the first log_table call.
+__device__ __forceinline__ void stage_log_table(LogEntry* dst, unsigned tid, unsigned nthreads) {
+  for (unsigned t = tid; t < 128; t += nthreads) dst[t] = kLogTable[t];
+}
+
+// The FP64 constants of log_table as values held in SGPRs for a whole epilogue: a VOP3
+// instruction takes no 64-bit literal, so constants written as literals were rebuilt with two
+// v_mov for every pair an epilogue corrects; an empty asm with an SGPR operand makes each one an
+// opaque uniform value that the FP64 instructions read straight from its SGPR pair.
+struct LogConsts {
+  double ln2_hi, ln2_lo, c0, step, p5, p4, p3;
+  const LogEntry* tab;  // the table in LDS
+};
+
+__device__ __forceinline__ LogConsts log_consts(const LogEntry* tab) {
+  LogConsts K = {0x1.62e42ff000000p-1,    // ln 2, 32-bit head: k * head is exact
+                 -0x1.718432a1b0e26p-35,  // ln 2 - head
+                 0.5 + 1.0 / 256.0, 1.0 / 256.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0, tab};
   asm volatile("" : "+s"(K.ln2_hi));
   asm volatile("" : "+s"(K.ln2_lo));
-  asm volatile("" : "+s"(K.sqrt_half));
+  asm volatile("" : "+s"(K.c0));
+  asm volatile("" : "+s"(K.step));
+  asm volatile("" : "+s"(K.p5));
+  asm volatile("" : "+s"(K.p4));
+  asm volatile("" : "+s"(K.p3));
   return K;
 }
 
-__device__ __forceinline__ double log_unit(double x, const LogUnitConsts& K) {
+__device__ __forceinline__ double log_table(double z, const LogConsts& K) {
   int k;
-  double m = frexp(x, &k);            // m in [0.5, 1)
-  const bool low = m < K.sqrt_half;
-  m = low ? m * 2.0 : m;              // m in [sqrt(1/2), sqrt(2))
-  k = low ? k - 1 : k;
-  const double f = m - 1.0;           // exact
-  const double d = 2.0 + f;
-  double r = __builtin_amdgcn_rcp(d);
-  r = fma(r, fma(-d, r, 1.0), r);
-  r = fma(r, fma(-d, r, 1.0), r);
-  const double sd = f * r;
-  const double t = sd * sd;
-  double p = K.inv_odd[0];
-#pragma unroll
-  for (int q = 1; q < 10; ++q) p = fma(p, t, K.inv_odd[q]);
-  const double two_s = 2.0 * sd;
-  const double lm = fma(two_s * t, p, two_s);
+  const double m = frexp(z, &k);  // m in [0.5, 1)
+  const unsigned i = ((unsigned)__double2hiint(m) >> 13) & 127u;
+  const double c = fma((double)i, K.step, K.c0);  // 0.5 + (i+1)/256, exact
+  const LogEntry e = K.tab[i];
+  const double r = (m - c) * e.inv_c;             // m - c exact (Sterbenz)
+  double p = fma(r, K.p5, K.p4);
+  p = fma(p, r, K.p3);
+  p = fma(p, r, -0.5);
+  const double l1 = fma(r * r, p, r);
   const double kd = (double)k;
-  return fma(kd, K.ln2_hi, fma(kd, K.ln2_lo, lm));
+  const double lz = fma(kd, K.ln2_hi, fma(kd, K.ln2_lo, e.ln_c + l1));
+  return z == 1.0 ? 0.0 : lz;
 }
 
 // Two proportions a/v and b/v as correctly rounded float quotients, from one FP64 reciprocal
@@ -189,7 +190,7 @@ __device__ __forceinline__ void proportions2(uint32_t a, uint32_t b, uint32_t v,
 // 1e-9 relative of a float rounding boundary; it is always well inside 1e-6.
 template <bool kExact, bool kSmall>
 __device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t valid,
-                                           const LogUnitConsts& K) {
+                                           const LogConsts& K) {
   const uint32_t v = valid ? valid : 1u;
   float P, Q;
   if (kExact) {
@@ -203,14 +204,14 @@ __device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t va
   const bool sat = valid == 0 || (float)x <= 0.0f || (float)y <= 0.0f;
   const double xs = sat ? 1.0 : x, ys = sat ? 1.0 : y;
   const float d = kExact ? (float)(-0.5 * log(xs) - 0.25 * log(ys))
-                         : (float)(-0.25 * log_unit(xs * xs * ys, K));
+                         : (float)(-0.25 * log_table(xs * xs * ys, K));
   // valid == 0 implies ts = tv = 0
   return (ts | tv) == 0 ? (valid == 0 ? -1.0f : -0.0f) : (sat ? -1.0f : d);
 }
 
 template <bool kExact, bool kSmall>
 __device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid,
-                                          const LogUnitConsts& K) {
+                                          const LogConsts& K) {
   const double b = 0.75;  // (alpha - 1) / alpha
   const uint32_t v = valid ? valid : 1u;
   const float p = kSmall ? (float)mismatches / (float)v : proportion(mismatches, v);
@@ -222,7 +223,7 @@ __device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid,
     d = (float)(-b * log(sat ? 1.0 : 1.0 - q));
   } else {
     sat = p >= 0.75f;  // the same test: a float p < 0.75 gives p / 0.75 < 1 - 2^-25 in double
-    d = (float)(-b * log_unit(sat ? 1.0 : fma(-(double)p, 4.0 / 3.0, 1.0), K));
+    d = (float)(-b * log_table(sat ? 1.0 : fma(-(double)p, 4.0 / 3.0, 1.0), K));
   }
   return (valid == 0 || mismatches == 0) ? -0.0f : (sat ? -1.0f : d);
 }

@@ -29,7 +29,8 @@ namespace {
 // folds the wave's maximum into the job's running maximum. A tile's pairs are those with i < j,
 // or for a transposed tile of a kRows job (lower) those with i > j.
 template <int TILE, int MI, int MJ, bool JC6, bool kExact>
-__device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, unsigned bi,
+__device__ __forceinline__ void tile_epilogue(const JobDesc& J, const LogEntry* log_tab,
+                                              unsigned tile, unsigned bi,
                                               unsigned bj, bool lower, unsigned ty, unsigned tx,
                                               unsigned tid,
                                               const uint32_t (&acc_a)[MI][MJ],
@@ -54,7 +55,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, unsigned tile, u
     return;
   }
   // One row of MJ corrections as straight-line code (interleaved by the compiler), then stores.
-  const LogUnitConsts K = log_unit_consts();
+  const LogConsts K = log_consts(log_tab);
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
@@ -91,52 +92,51 @@ __device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u
                                            int w, unsigned ty, unsigned tx,
                                            uint32_t (&acc_a)[MI][MJ], uint32_t (&acc_b)[MI][MJ],
                                            uint32_t (&acc_v)[MI][MJ]) {
-  uint32_t hi[MI], li[MI], vi[MI], hj[MJ], lj[MJ], vj[MJ];
+  // Operands of plane p for the i side (ty's MI taxa) and the j side (tx's MJ taxa), read in the
+  // order the counts use them: V first, then (naive rules) U, then H and L.
+  uint32_t vi[MI], vj[MJ], hi[MI], hj[MJ], li[MI], lj[MJ];
   uint32_t ui[MODE == kModeNaive ? MI : 1], uj[MODE == kModeNaive ? MJ : 1];
-  if constexpr (MODE == kModeNaive) {
+  auto read = [&](int p, uint32_t (&oi)[MI], uint32_t (&oj)[MJ]) {
 #pragma unroll
     for (int r = 0; r < MI / 4; ++r) {
-      const v4u u = sI[(3 * CW + w) * Q + ty * (MI / 4) + r];
-      ui[4 * r] = u.x; ui[4 * r + 1] = u.y; ui[4 * r + 2] = u.z; ui[4 * r + 3] = u.w;
+      const v4u q = sI[(p * CW + w) * Q + ty * (MI / 4) + r];
+      oi[4 * r] = q.x; oi[4 * r + 1] = q.y; oi[4 * r + 2] = q.z; oi[4 * r + 3] = q.w;
     }
 #pragma unroll
     for (int r = 0; r < MJ / 4; ++r) {
-      const v4u u = sJ[(3 * CW + w) * Q + tx * (MJ / 4) + r];
-      uj[4 * r] = u.x; uj[4 * r + 1] = u.y; uj[4 * r + 2] = u.z; uj[4 * r + 3] = u.w;
+      const v4u q = sJ[(p * CW + w) * Q + tx * (MJ / 4) + r];
+      oj[4 * r] = q.x; oj[4 * r + 1] = q.y; oj[4 * r + 2] = q.z; oj[4 * r + 3] = q.w;
     }
-  }
+  };
+  read(2, vi, vj);
+  if constexpr (MODE == kModeNaive) read(3, ui, uj);
+  read(0, hi, hj);
+  read(1, li, lj);
+  if constexpr (MODE == kModeNaive) {
+    // Naive rules (simpleDistanceCalculator.cpp:58-98): valid = both counted; ts = both
+    // uppercase ACGT, same H, different L; tv = i uppercase ACGT, j counted, and j not uppercase
+    // ACGT or a different H. Ordered asm in two passes over the micro-tile: the valid counts need
+    // only the V planes, so they run while the other planes' reads are still in flight.
 #pragma unroll
-  for (int r = 0; r < MI / 4; ++r) {
-    const v4u h = sI[(0 * CW + w) * Q + ty * (MI / 4) + r];
-    const v4u l = sI[(1 * CW + w) * Q + ty * (MI / 4) + r];
-    const v4u v = sI[(2 * CW + w) * Q + ty * (MI / 4) + r];
-    hi[4 * r] = h.x; hi[4 * r + 1] = h.y; hi[4 * r + 2] = h.z; hi[4 * r + 3] = h.w;
-    li[4 * r] = l.x; li[4 * r + 1] = l.y; li[4 * r + 2] = l.z; li[4 * r + 3] = l.w;
-    vi[4 * r] = v.x; vi[4 * r + 1] = v.y; vi[4 * r + 2] = v.z; vi[4 * r + 3] = v.w;
-  }
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
-  for (int r = 0; r < MJ / 4; ++r) {
-    const v4u h = sJ[(0 * CW + w) * Q + tx * (MJ / 4) + r];
-    const v4u l = sJ[(1 * CW + w) * Q + tx * (MJ / 4) + r];
-    const v4u v = sJ[(2 * CW + w) * Q + tx * (MJ / 4) + r];
-    hj[4 * r] = h.x; hj[4 * r + 1] = h.y; hj[4 * r + 2] = h.z; hj[4 * r + 3] = h.w;
-    lj[4 * r] = l.x; lj[4 * r + 1] = l.y; lj[4 * r + 2] = l.z; lj[4 * r + 3] = l.w;
-    vj[4 * r] = v.x; vj[4 * r + 1] = v.y; vj[4 * r + 2] = v.z; vj[4 * r + 3] = v.w;
-  }
+      for (int b = 0; b < MJ; ++b) ordered_bcnt_add(ordered_and(vi[a], vj[b]), acc_v[a][b]);
 #pragma unroll
-  for (int a = 0; a < MI; ++a)
+    for (int a = 0; a < MI; ++a)
 #pragma unroll
-    for (int b = 0; b < MJ; ++b) {
-      if constexpr (MODE == kModeNaive) {
-        // ts: both uppercase ACGT, same H, different L; tv: i uppercase ACGT, j counted, and j
-        // not uppercase ACGT or a different H (simpleDistanceCalculator.cpp:58-98)
-        const uint32_t g = vi[a] & vj[b];
-        acc_v[a][b] = bcnt_add(g, acc_v[a][b]);
-        const uint32_t x = and3(ui[a], uj[b], li[a] ^ lj[b]);
-        const uint32_t y = xor_or_not(hi[a], hj[b], uj[b]);
-        acc_b[a][b] = bcnt_add(xnor_and(hi[a], hj[b], x), acc_b[a][b]);
-        acc_a[a][b] = bcnt_add(and3(ui[a], vj[b], y), acc_a[a][b]);
-      } else {
+      for (int b = 0; b < MJ; ++b) {
+        uint32_t t;
+        asm volatile("v_xor_b32 %0, %1, %2" : "=v"(t) : "v"(li[a]), "v"(lj[b]));
+        const uint32_t x = ordered_bitop3<0x80>(ui[a], uj[b], t);         // ui & uj & (li ^ lj)
+        const uint32_t y = ordered_bitop3<0x7D>(hi[a], hj[b], uj[b]);     // (hi ^ hj) | ~uj
+        ordered_bcnt_add(ordered_bitop3<0x82>(hi[a], hj[b], x), acc_b[a][b]);  // ~(hi ^ hj) & x
+        ordered_bcnt_add(ordered_bitop3<0x80>(ui[a], vj[b], y), acc_a[a][b]);  // ui & vj & y
+      }
+  } else {
+#pragma unroll
+    for (int a = 0; a < MI; ++a)
+#pragma unroll
+      for (int b = 0; b < MJ; ++b) {
         // One pair's instructions back to back, in this order (ordered asm): the VALU runs the
         // mix of bitwise and v_bcnt instructions about 4 % faster this way than grouped by class
         // over the micro-tile, which is how the compiler schedules the same operations.
@@ -155,7 +155,7 @@ __device__ __forceinline__ void count_word(const v4u* __restrict__ sI, const v4u
           ordered_bcnt_add(x, acc_b[a][b]);
         }
       }
-    }
+  }
 }
 
 // After the word loop of the K2P and JC modes acc_a holds #u (K2P: same-class sites) or #same
@@ -179,8 +179,8 @@ __device__ __forceinline__ void counts_from_same(uint32_t (&acc_a)[MI][MJ],
 // [plane][word][taxon], written linearly) while the previous chunk is counted:
 //  * work-groups of several waves load them straight from global memory into LDS
 //    (global_load_lds_dwordx4, each wave filling 64 consecutive uint4) into two buffers in
-//    turn, so no registers hold a chunk on the way (t64 K2P: 103 VGPRs instead of 110, JC 79
-//    instead of 94, 6 waves per SIMD);
+//    turn, so no registers hold a chunk on the way (t64_m4x4_c8: K2P 96 VGPRs, 5 waves per
+//    SIMD; JC 79, 6 waves);
 //  * one-wave work-groups prefetch the next chunk into registers and write it to one buffer:
 //    a second buffer would cost them occupancy, which LDS bounds for them.
 // Unsplit jobs finish in place; split jobs leave exact partial counts for finish_kernel.
@@ -202,6 +202,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
   // loads before counting the current one.
   __shared__ v4u sI0[E], sJ0[E];
   __shared__ v4u sI1[kDirect ? E : 1], sJ1[kDirect ? E : 1];
+  static_assert(E * sizeof(v4u) >= 128 * sizeof(LogEntry), "the log table fits a staging buffer");
 
   const unsigned item = xcd_item(blockIdx.x, gridDim.x);
   const unsigned k = find_job(jobs, njobs, item);
@@ -233,7 +234,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
 
   auto count_chunk = [&](const v4u* bI, const v4u* bJ, unsigned c) {
     const int live = (int)min((unsigned)CW, w1 - c * CW);  // uniform over the work-group
-#pragma unroll 4
+#pragma unroll 1
     for (int w = 0; w < CW; ++w) {
       if (w >= live) break;
       count_word<MI, MJ, CW, Q, MODE>(bI, bJ, w, ty, tx, acc_a, acc_b, acc_v);
@@ -318,7 +319,14 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
       }
     return;
   }
-  tile_epilogue<TILE, MI, MJ, MODE == kModeJC6, kExact>(J, tile, bi, bj, lower, ty, tx, tid, acc_a,
+  // The log table goes into the first staging buffer once every wave is done counting, so that
+  // it costs the work-group no LDS of its own (at 2 KB it would cost the naive rules' t64_m4x4_c8
+  // work-groups their fifth wave per SIMD).
+  LogEntry* const sLog = reinterpret_cast<LogEntry*>(sI0);
+  __syncthreads();
+  stage_log_table(sLog, tid, NT);
+  __syncthreads();
+  tile_epilogue<TILE, MI, MJ, MODE == kModeJC6, kExact>(J, sLog, tile, bi, bj, lower, ty, tx, tid, acc_a,
                                                         acc_b, acc_v);
 }
 
@@ -337,7 +345,10 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
   }
   const JobDesc& J = jobs[lo];
   const unsigned tile = item - J.fin_base;
-  if (J.splits <= 1 || tile >= J.tiles) return;
+  if (J.splits <= 1 || tile >= J.tiles) return;  // uniform over the work-group
+  __shared__ LogEntry sLog[128];
+  stage_log_table(sLog, threadIdx.x, NT);
+  __syncthreads();
   unsigned bi, bj;
   const bool lower = job_tile(J, tile, bi, bj);
   const unsigned tid = threadIdx.x;
@@ -359,7 +370,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void finish_kernel(
         acc_v[a][b] += slot[2 * plane + a * MJ + b];
       }
   }
-  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, tile, bi, bj, lower, ty, tx, tid, acc_a, acc_b,
+  tile_epilogue<TILE, MI, MJ, JC6, kExact>(J, sLog, tile, bi, bj, lower, ty, tx, tid, acc_a, acc_b,
                                            acc_v);
 }
 

@@ -65,7 +65,7 @@ __constant__ float kDayhoff[181] = {
 // mism and valid are the reference's data[0] (data[1] is 0 for protein) and data[2], valid being
 // its 64-bit count. Proportions are float quotients of the float-converted counts, as the
 // reference's `total / (distType) data[2]` computes them. kExact takes the library log; the
-// default log_unit rounds to the same float except in rare ties (see log_unit).
+// default log_table rounds to the same float except in rare ties (see log_table).
 #pragma clang fp contract(off)
 // (float)mism / (float)valid. Below 2^24 both convert exactly and the float division is the
 // correctly rounded quotient of the integers, which the default path forms from the FP64
@@ -87,12 +87,12 @@ __device__ __forceinline__ float protein_proportion(uint32_t mism, unsigned long
 // case computed, the result selected) so that a caller can interleave several pairs.
 template <bool kExact>
 __device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid,
-                                                     const LogUnitConsts& K) {
+                                                     const LogConsts& K) {
   const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : -1.0f;
   const bool formula = p < 0.75f && p > 0.0f;
   const float pp = p * p;
   const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
-  const double lg = kExact ? log(arg) : log_unit(arg, K);
+  const double lg = kExact ? log(arg) : log_table(arg, K);
   int idx = (int)((double)p * 1000.0 - 750.0);
   idx = idx < 0 ? 0 : (idx > 180 ? 180 : idx);
   const float table = (double)p > 0.930 ? 10.0f : kDayhoff[idx];
@@ -103,13 +103,13 @@ __device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned lo
 // quotient in double, as the reference forms it, so that the saturated pairs are the same).
 template <bool kExact>
 __device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long long valid,
-                                                 const LogUnitConsts& K) {
+                                                 const LogConsts& K) {
   const double b = 19.0 / 20.0;  // (alpha - 1) / alpha
   const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : 0.0f;
   const double q = (double)p / b;
   const bool sat = q >= 1.0;
   const double arg = sat ? 1.0 : 1.0 - q;
-  const double lg = kExact ? log(arg) : log_unit(arg, K);
+  const double lg = kExact ? log(arg) : log_table(arg, K);
   return sat ? -1.0f : (float)(-b * lg);
 }
 #pragma clang fp contract(on)
@@ -123,7 +123,8 @@ __device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long l
 // pairs, simpleDistanceCalculator.cpp:40-55); a bootstrap replicate's length is read from
 // J.len_dev.
 template <int TILE, int MI, int MJ, bool kExact>
-__device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile, unsigned bi,
+__device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntry* log_tab,
+                                                 unsigned tile, unsigned bi,
                                                  unsigned bj, unsigned ty, unsigned tx,
                                                  unsigned tid,
                                                  const uint32_t (&acc_x)[MI][MJ],
@@ -162,7 +163,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, unsigned tile
   }
   float lmax = 0.0f;
   uint32_t sat = 0;  // saturated pairs of the micro-tile, bit a * MJ + b
-  const LogUnitConsts K = log_unit_consts();
+  const LogConsts K = log_consts(log_tab);
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
@@ -208,6 +209,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel
 
   __shared__ v4u sI[E];
   __shared__ v4u sJ[E];
+  __shared__ LogEntry sLog[128];
 
   const unsigned item = xcd_item(blockIdx.x, gridDim.x);
   const unsigned k = find_job(jobs, njobs, item);
@@ -219,6 +221,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel
 
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
+  stage_log_table(sLog, tid, NT);  // visible after the chunk loop's first barrier
   const gptr<v4u> gI = (gptr<v4u>)(J.planes + (size_t)bi * TILE);
   const gptr<v4u> gJ = (gptr<v4u>)(J.planes + (size_t)bj * TILE);
   const size_t row4 = J.n_pad / 4;
@@ -293,7 +296,8 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel
       }
     return;
   }
-  protein_epilogue<TILE, MI, MJ, kExact>(J, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
+  if (c0 >= c1) __syncthreads();  // no chunk, so no barrier yet since the table was staged
+  protein_epilogue<TILE, MI, MJ, kExact>(J, sLog, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
 }
 
 // Sums the partial counts of the splits of one tile and finishes it (kernels.hip numbering).
@@ -310,7 +314,10 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_finish_kern
   }
   const JobDesc& J = jobs[lo];
   const unsigned tile = item - J.fin_base;
-  if (J.splits <= 1 || tile >= J.tiles) return;
+  if (J.splits <= 1 || tile >= J.tiles) return;  // uniform over the work-group
+  __shared__ LogEntry sLog[128];
+  stage_log_table(sLog, threadIdx.x, NT);
+  __syncthreads();
   unsigned bi, bj;
   patch_coords(tile, J.nblk, bi, bj);
   const unsigned tid = threadIdx.x;
@@ -331,7 +338,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_finish_kern
         acc_g[a][b] += slot[plane + a * MJ + b];
       }
   }
-  protein_epilogue<TILE, MI, MJ, kExact>(J, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
+  protein_epilogue<TILE, MI, MJ, kExact>(J, sLog, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
 }
 
 template <int TILE, int MI, int MJ, int CW, int B>

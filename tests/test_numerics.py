@@ -78,3 +78,67 @@ def test_jc_fma_argument_within_tolerance():
     got = (-0.75 * np.log(arg)).astype(np.float32)
     rel = np.abs(got.astype(np.float64) - ref) / np.abs(ref)
     assert rel.max() < 1e-7
+
+
+# ---- the table-driven FP64 logarithm of the device epilogues (device_common.hpp log_table) ------
+def _log_table_entries():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "..", "rapidnj_amd", "csrc", "log_table.inc")
+    inv, ln = [], []
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("{"):
+            a, b = line.strip("{},").split(",")
+            inv.append(float.fromhex(a.strip()))
+            ln.append(float.fromhex(b.strip()))
+    return np.array(inv), np.array(ln)
+
+
+def test_log_table_matches_its_generator():
+    """log_table.inc holds 1/c_i and ln(c_i) for c_i = 0.5 + (i+1)/256, each the double nearest
+    the exact value (checked here with an independent 60-digit evaluation)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    inv, ln = _log_table_entries()
+    assert len(inv) == 128
+    for i in range(128):
+        c = Decimal(1) / 2 + Decimal(i + 1) / 256
+        assert inv[i] == float(Decimal(1) / c)
+        assert ln[i] == float(c.ln())
+
+
+def _log_table_np(z):
+    """The device's log_table restated in numpy FP64 (fma taken as multiply-add: the bounds below
+    hold with room for that one extra rounding per step)."""
+    inv, ln = _log_table_entries()
+    m, k = np.frexp(z)                                   # m in [0.5, 1)
+    i = ((m.view(np.uint64) >> np.uint64(45)) & np.uint64(127)).astype(np.int64)
+    c = i * (1.0 / 256.0) + (0.5 + 1.0 / 256.0)
+    r = (m - c) * inv[i]
+    p = r * 0.2 - 0.25
+    p = p * r + 1.0 / 3.0
+    p = p * r - 0.5
+    l1 = (r * r) * p + r
+    kd = k.astype(np.float64)
+    ln2_hi, ln2_lo = float.fromhex("0x1.62e42ff000000p-1"), float.fromhex("-0x1.718432a1b0e26p-35")
+    return kd * ln2_hi + (kd * ln2_lo + (ln[i] + l1))
+
+
+def test_log_table_accuracy():
+    """Against libm's log (within an ulp): relative error below 2e-11 over z in [2^-80, 1),
+    densely near 1 (the small distances, where the last interval reduces to z - 1 exactly) and at
+    every table interval's ends -- so the float distances it yields are the reference's except in
+    ties within ~1e-9 of a float rounding boundary. z = 1 is returned as 0 by a select; without it
+    the reduction (1 = 2 * 0.5) would leave an error below 1e-13."""
+    z = np.concatenate([
+        np.exp2(-RNG.random(200_000) * 80.0),
+        1.0 - np.exp2(-RNG.random(200_000) * 52.0),
+        0.5 + np.arange(1, 256) / 512.0,                 # interval ends and centres
+        np.nextafter(0.5 + np.arange(1, 257) / 256.0, 0.0),
+        np.array([0.5, 0.75, np.nextafter(1.0, 0.0)]),
+    ])
+    got = _log_table_np(z)
+    want = np.log(z)
+    rel = np.abs(got - want) / np.abs(want)
+    assert rel.max() < 2e-11, (rel.max(), z[np.argmax(rel)])
+    assert abs(_log_table_np(np.array([1.0]))[0]) < 1e-13
