@@ -6,6 +6,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace rnj {
 namespace {
 
@@ -39,6 +41,9 @@ __device__ __forceinline__ void ordered_bcnt_add(uint32_t x, uint32_t& acc) {
 // (the job table hands over generic pointers).
 template <class T>
 using gptr = const __attribute__((address_space(1))) T*;
+
+template <class T>
+using gptr_mut = __attribute__((address_space(1))) T*;
 
 // Native 4 x u32 vector (ext_vector_type) for 128-bit global and LDS accesses.
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
@@ -162,20 +167,32 @@ __device__ __forceinline__ double log_table(double z, const LogConsts& K) {
 // never a float rounding midpoint (that would need a 25-bit odd numerator a/v' with v' | a), and
 // its distance to the nearest midpoint is at least 1 / (v * 2^25) >= 2^-49 relative, so the
 // rounded product lands on the same float as the exact quotient. Above 2^24 the double
-// division of proportion() is used.
+// division of proportion() is used. kSmall: v < 2^24 is known, and no branch is taken.
+__device__ __forceinline__ double reciprocal(uint32_t v) {
+  const double dv = (double)v;
+  double r = __builtin_amdgcn_rcp(dv);
+  r = fma(r, fma(-dv, r, 1.0), r);
+  return fma(r, fma(-dv, r, 1.0), r);
+}
+
+template <bool kSmall>
 __device__ __forceinline__ void proportions2(uint32_t a, uint32_t b, uint32_t v, float& pa,
                                              float& pb) {
-  if (v < (1u << 24)) {
-    const double dv = (double)v;
-    double r = __builtin_amdgcn_rcp(dv);
-    r = fma(r, fma(-dv, r, 1.0), r);
-    r = fma(r, fma(-dv, r, 1.0), r);
+  if (kSmall || v < (1u << 24)) {
+    const double r = reciprocal(v);
     pa = (float)((double)a * r);
     pb = (float)((double)b * r);
   } else {
     pa = (float)((double)a / (double)v);
     pb = (float)((double)b / (double)v);
   }
+}
+
+// One proportion a/v the same way (JC).
+template <bool kSmall>
+__device__ __forceinline__ float proportion1(uint32_t a, uint32_t v) {
+  if (kSmall || v < (1u << 24)) return (float)((double)a * reciprocal(v));
+  return (float)((double)a / (double)v);
 }
 
 // The corrections as straight-line code (every case computed, the result selected), so that a
@@ -197,7 +214,7 @@ __device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t va
     P = kSmall ? (float)ts / (float)v : proportion(ts, v);
     Q = kSmall ? (float)tv / (float)v : proportion(tv, v);
   } else {
-    proportions2(ts, tv, v, P, Q);
+    proportions2<kSmall>(ts, tv, v, P, Q);
   }
   const double x = 1.0 - 2.0 * (double)P - (double)Q;
   const double y = 1.0 - 2.0 * (double)Q;
@@ -214,7 +231,8 @@ __device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid,
                                           const LogConsts& K) {
   const double b = 0.75;  // (alpha - 1) / alpha
   const uint32_t v = valid ? valid : 1u;
-  const float p = kSmall ? (float)mismatches / (float)v : proportion(mismatches, v);
+  const float p = kExact ? (kSmall ? (float)mismatches / (float)v : proportion(mismatches, v))
+                         : proportion1<kSmall>(mismatches, v);
   float d;
   bool sat;
   if (kExact) {

@@ -54,8 +54,44 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, const LogEntry* 
     }
     return;
   }
-  // One row of MJ corrections as straight-line code (interleaved by the compiler), then stores.
   const LogConsts K = log_consts(log_tab);
+  // Inner tiles of an upper-triangle output (bi < bj, the whole tile below n, as almost every
+  // tile of a large job is): every pair has i < j < n, so each row's MJ corrections run as
+  // straight-line code (no valid >= 2^24 branch: fewer than 2^24 sites were counted) that the
+  // compiler interleaves, and the row's MJ entries, consecutive in the triangle, are stored from
+  // one index. The rest take the general path below.
+  const bool inner = !kExact && J.layout == kTriangle && !lower && bi < bj &&
+                     (bj + 1) * TILE <= n && J.W_live < (1u << 19);
+  if (inner) {
+    const unsigned j0 = bj * TILE + tx * MJ;
+    auto rows = [&](auto model) {
+#pragma unroll
+      for (int a = 0; a < MI; ++a) {
+        const unsigned i = bi * TILE + ty * MI + a;
+        float d[MJ];
+#pragma unroll
+        for (int b = 0; b < MJ; ++b) {
+          if constexpr (decltype(model)::value == kJC)
+            d[b] = jc_value<false, true>(JC6 ? acc_a[a][b] : acc_a[a][b] + acc_b[a][b],
+                                         acc_v[a][b], K);
+          else
+            d[b] = k2p_value<false, true>(acc_b[a][b], acc_a[a][b], acc_v[a][b], K);
+        }
+        const gptr_mut<float> row = (gptr_mut<float>)J.out + tri_index(i, j0, n);
+#pragma unroll
+        for (int b = 0; b < MJ; ++b) {
+          row[b] = d[b];
+          if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
+          else lmax = fmaxf(lmax, d[b]);
+        }
+      }
+    };
+    if (JC6 || J.model == kJC) rows(std::integral_constant<int, kJC>());
+    else rows(std::integral_constant<int, kK2P>());
+    wave_record(J, tile * (NT / 64) + tid / 64, bi << 16 | bj, sat, lmax);
+    return;
+  }
+  // General path: one row of MJ corrections, then the stores each pair's position allows.
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
@@ -172,6 +208,13 @@ __device__ __forceinline__ void counts_from_same(uint32_t (&acc_a)[MI][MJ],
   }
 }
 
+// Waves per SIMD the register allocation aims for: 6 for the JC count loop of 64-pair tiles with
+// 8-word chunks (its count loop needs fewer than 84 VGPRs; left alone, the compiler interleaves
+// the epilogue's corrections so far that the kernel needs 107 and runs 2 % slower at 4 waves),
+// otherwise no constraint.
+template <int TILE, int CW, int MODE, bool kExact>
+constexpr int kTileWaves = (TILE == 64 && CW == 8 && MODE == kModeJC6 && !kExact) ? 6 : 1;
+
 // One work-group computes a TILE x TILE block of taxon pairs (bi <= bj) over one range of words
 // (the whole alignment, or split `split` of J.splits); thread (ty, tx) owns the MI x MJ
 // micro-tile i = bi*TILE + ty*MI + a, j = bj*TILE + tx*MJ + b. Sites are walked in chunks of CW
@@ -185,7 +228,8 @@ __device__ __forceinline__ void counts_from_same(uint32_t (&acc_a)[MI][MJ],
 //    a second buffer would cost them occupancy, which LDS bounds for them.
 // Unsplit jobs finish in place; split jobs leave exact partial counts for finish_kernel.
 template <int TILE, int MI, int MJ, int CW, int MODE, bool kExact>
-__global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void tile_kernel(
+__global__ __launch_bounds__((TILE / MI) * (TILE / MJ))
+__attribute__((amdgpu_waves_per_eu(kTileWaves<TILE, CW, MODE, kExact>))) void tile_kernel(
     const JobDesc* __restrict__ jobs, unsigned njobs) {
   constexpr int NTX = TILE / MJ;
   constexpr int NT = (TILE / MI) * NTX;
