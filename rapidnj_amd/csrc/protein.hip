@@ -86,9 +86,7 @@ __device__ __forceinline__ float protein_proportion(uint32_t mism, unsigned long
 // float, the rest in double), the Dayhoff table up to 0.930 and 10 above. Straight-line (every
 // case computed, the result selected) so that a caller can interleave several pairs.
 template <bool kExact>
-__device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid,
-                                                     const LogConsts& K) {
-  const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : -1.0f;
+__device__ __forceinline__ float kimura_protein_of_p(float p, const LogConsts& K) {
   const bool formula = p < 0.75f && p > 0.0f;
   const float pp = p * p;
   const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
@@ -99,18 +97,29 @@ __device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned lo
   return p < 0.75f ? (formula ? (float)(-lg) : p) : table;
 }
 
+template <bool kExact>
+__device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid,
+                                                     const LogConsts& K) {
+  return kimura_protein_of_p<kExact>(valid != 0 ? protein_proportion<kExact>(mism, valid) : -1.0f,
+                                     K);
+}
+
 // JCdistance.cpp:170-183 with alpha = 20: p = 0 when valid is 0, -1 when p / 0.95 >= 1 (the
 // quotient in double, as the reference forms it, so that the saturated pairs are the same).
 template <bool kExact>
-__device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long long valid,
-                                                 const LogConsts& K) {
+__device__ __forceinline__ float jc_protein_of_p(float p, const LogConsts& K) {
   const double b = 19.0 / 20.0;  // (alpha - 1) / alpha
-  const float p = valid != 0 ? protein_proportion<kExact>(mism, valid) : 0.0f;
   const double q = (double)p / b;
   const bool sat = q >= 1.0;
   const double arg = sat ? 1.0 : 1.0 - q;
   const double lg = kExact ? log(arg) : log_table(arg, K);
   return sat ? -1.0f : (float)(-b * lg);
+}
+
+template <bool kExact>
+__device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long long valid,
+                                                 const LogConsts& K) {
+  return jc_protein_of_p<kExact>(valid != 0 ? protein_proportion<kExact>(mism, valid) : 0.0f, K);
 }
 #pragma clang fp contract(on)
 
@@ -164,6 +173,45 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntr
   float lmax = 0.0f;
   uint32_t sat = 0;  // saturated pairs of the micro-tile, bit a * MJ + b
   const LogConsts K = log_consts(log_tab);
+  // Inner tiles of an upper-triangle output with fewer than 2^24 sites: the counts in 32-bit
+  // arithmetic and each row's corrections as straight-line code, stored from one triangle index.
+  // The valid count is then s = L - bias - (gaps_i + gaps_j - #g) in int32; the reference's
+  // 64-bit count wraps when s < 0 (fewer residue pairs than the 128 of the bias), and its float
+  // is 2^64, so the proportion is mism * 2^-64 exactly (as the float division gives).
+  if (!kExact && J.layout == kTriangle && bi < bj && (bj + 1) * TILE <= n && L < (1u << 24)) {
+    const int lb = (int)(L - bias);
+    const unsigned j0 = bj * TILE + tx * MJ;
+    auto rows = [&](auto model) {
+#pragma unroll
+      for (int a = 0; a < MI; ++a) {
+        const unsigned i = bi * TILE + ty * MI + a;
+        float d[MJ];
+#pragma unroll
+        for (int b = 0; b < MJ; ++b) {
+          const uint32_t gsum = gi[a] + gj[b];
+          const uint32_t mism = acc_x[a][b] + 2u * acc_g[a][b] - gsum;
+          const int s = lb - (int)gsum + (int)acc_g[a][b];
+          const float pq = (float)((double)mism * reciprocal(s > 0 ? (uint32_t)s : 1u));
+          const float pw = __builtin_ldexpf((float)mism, -64);
+          if constexpr (decltype(model)::value == kJC)
+            d[b] = jc_protein_of_p<false>(s == 0 ? 0.0f : (s < 0 ? pw : pq), K);
+          else
+            d[b] = kimura_protein_of_p<false>(s == 0 ? -1.0f : (s < 0 ? pw : pq), K);
+        }
+        const gptr_mut<float> row = (gptr_mut<float>)J.out + tri_index(i, j0, n);
+#pragma unroll
+        for (int b = 0; b < MJ; ++b) {
+          row[b] = d[b];
+          if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
+          else lmax = fmaxf(lmax, d[b]);
+        }
+      }
+    };
+    if (J.model == kJC) rows(std::integral_constant<int, kJC>());
+    else rows(std::integral_constant<int, kK2P>());
+    wave_record(J, tile * ((TILE / MI) * (TILE / MJ) / 64) + tid / 64, bi << 16 | bj, sat, lmax);
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < MI; ++a) {
     const unsigned i = bi * TILE + ty * MI + a;
@@ -193,11 +241,18 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntr
 }
 
 // ---- the pair-tile kernel ------------------------------------------------------------------
+// Waves per SIMD the register allocation aims for: 4 for 64-pair tiles with 8-word chunks (the
+// epilogue's interleaved corrections would otherwise take it to 130 VGPRs and 3 waves: 1.95 ms
+// against 1.82 ms at 20000 x 1250, for a 12-byte spill), otherwise no constraint.
+template <int TILE, int CW, bool kExact>
+constexpr int kProteinWaves = (TILE == 64 && CW == 8 && !kExact) ? 4 : 1;
+
 // The DNA tile kernel's schedule (kernels.hip) over 1 + B planes: one work-group per TILE x TILE
 // block of pairs and range of words, chunks of CW words of both blocks staged in LDS as
 // [plane][word][taxon], the next chunk prefetched into registers while this one is counted.
 template <int TILE, int MI, int MJ, int CW, int B, bool kExact>
-__global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_tile_kernel(
+__global__ __launch_bounds__((TILE / MI) * (TILE / MJ))
+__attribute__((amdgpu_waves_per_eu(kProteinWaves<TILE, CW, kExact>))) void protein_tile_kernel(
     const JobDesc* __restrict__ jobs, unsigned njobs) {
   constexpr int P = 1 + B;
   constexpr int NTX = TILE / MJ;

@@ -346,3 +346,35 @@ def test_dna_only_operations_refuse_protein(ctx, N):
         assert e.value.status == N.RNJ_E_NOT_DNA
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("L", [256, 300])
+def test_triangle_every_config_with_wrapped_and_saturated_pairs(ctx, N, L):
+    """The upper-triangle output (where tiles strictly above the diagonal take the 32-bit
+    straight-line epilogue) in every tile configuration, both models, against the oracle: rows of
+    gaps and of X scattered over the blocks give pairs without common residues (Kimura -1, patched
+    to 2 * max) and, at a multiple-of-128 length, the reference's wrapped 64-bit valid count
+    (distance -0.0 or +0.0)."""
+    n = 200
+    aln = protein_alignment(n, L, seed=21, rates=(0.0, 1.0), gap=0.05)
+    for r in (4, 70, 150, 199):
+        aln[r] = ord('-')
+    for r in (7, 100, 181):
+        aln[r, : L - 6] = ord('X')
+    A = ctx.alignment(aln)
+    iu = np.triu_indices(n, 1)
+    try:
+        for model in ("k2p", "jc"):
+            want = O.protein_matrix(aln, model)[0][iu]
+            for c in range(N.lib().rnj_tile_config_count()):
+                ctx.set_tile_config(c)
+                got = ctx.distances(A, model).cpu().numpy()
+                msg = f"{model} config {c}"
+                np.testing.assert_allclose(got, want, rtol=1e-6, atol=0, err_msg=msg)
+                assert (np.signbit(got) == np.signbit(want)).all(), msg
+                exact = ctx.distances(A, model, exact=True).cpu().numpy()
+                np.testing.assert_array_equal(exact.view(np.uint32), want.view(np.uint32), err_msg=msg)
+    finally:
+        ctx.set_tile_config(-1)
+    if L % 128 == 0:  # the wrapped pairs are there: zero distances between gap rows and others
+        assert (want == 0).sum() > 0
