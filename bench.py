@@ -440,6 +440,23 @@ def side_main(args) -> int:
     """--side: one side measurement alone (for profiling it)."""
     from rapidnj_amd import native as N
     from rapidnj_amd.workloads import WORKLOADS, make_alignment
+    if args.side == "valu":
+        # the VALU ceilings alone: both instruction classes' issue rates and every count loop's
+        # instruction-mix rate (lane-ops/s), three measurements each
+        ctx = N.Context(0)
+        res = {"issue_rates_tops": [], "mix_rates_tops": {}}
+        for _ in range(3):
+            bw, bc = ctx.valu_issue_rates()
+            res["issue_rates_tops"].append({"bitwise": round(bw / 1e12, 2),
+                                            "bcnt": round(bc / 1e12, 2)})
+        for mix, planes in (("k2p", 0), ("jc", 0), ("naive", 0), ("protein", 5), ("protein", 6),
+                            ("protein", 8)):
+            key = mix if not planes else f"{mix}_b{planes}"
+            res["mix_rates_tops"][key] = [round(ctx.valu_mix_rate(mix, planes) / 1e12, 2)
+                                          for _ in range(3)]
+        print(json.dumps({"side": "valu", **res}), flush=True)
+        ctx.close()
+        return 0
     wl = WORKLOADS[args.workload]
     aln = make_alignment(wl)
     ctx = N.Context(0)
@@ -688,7 +705,7 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM bytes per launch")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--side", choices=["ingest", "handoff", "nj"],
+    ap.add_argument("--side", choices=["ingest", "handoff", "nj", "valu"],
                     help="run only this side measurement on the workload and print its JSON")
     args = ap.parse_args()
     if args.probe_traffic:
