@@ -188,10 +188,12 @@ def probe_traffic(args) -> int:
 
 
 def ingest_compare(ctx, aln) -> dict:
-    """FASTA -> packed alignment in HBM for the benchmark alignment (60-column lines): from host
-    bytes (upload + device parse + pack), from a device-resident file image (parse + pack only),
-    and from a file (pinned read + upload + parse + pack), next to the reference's reader
-    (dataloaderFasta, readData with fastdist) on the same file."""
+    """FASTA and Stockholm -> packed alignment in HBM for the benchmark alignment: from host bytes
+    (upload + device parse + pack), from a device-resident file image (parse + pack only), and
+    from a file (pinned read + upload + parse + pack), next to the reference's readers
+    (dataloaderFasta / dataloaderStockholm, readData with fastdist) on the same file. The FASTA
+    text has 60-column lines; the Stockholm text has one line per sequence (the reference reads
+    single-line Stockholm), a header and the closing "//"."""
     import torch
     from oracle import oracle as O
     lines = []
@@ -199,8 +201,9 @@ def ingest_compare(ctx, aln) -> dict:
         lines.append(b">t%d" % i)
         b = row.tobytes()
         lines += [b[k:k + 60] for k in range(0, len(b), 60)]
-    data = b"\n".join(lines) + b"\n"
-    out = {"bytes": len(data), "taxa": int(aln.shape[0]), "sites": int(aln.shape[1])}
+    fasta = b"\n".join(lines) + b"\n"
+    sth = (b"# STOCKHOLM 1.0\n" + b"".join(b"t%d %s\n" % (i, row.tobytes())
+                                          for i, row in enumerate(aln)) + b"//\n")
 
     def best(fn, reps=3):
         t = []
@@ -212,25 +215,32 @@ def ingest_compare(ctx, aln) -> dict:
             t.append(time.perf_counter() - t0)
             a.close()
         return min(t)
-    host = best(lambda: ctx.alignment_from_fasta(data))
-    dev = torch.frombuffer(bytearray(data), dtype=torch.uint8).to("cuda:0")
-    device = best(lambda: ctx.alignment_from_fasta(dev))
-    del dev
-    with tempfile.TemporaryDirectory() as tmp:
-        path = os.path.join(tmp, "bench.fa")
-        with open(path, "wb") as fh:
-            fh.write(data)
-        from_file = best(lambda: ctx.alignment_from_fasta_file(path))
-        ref = None
-        if O.ref_harness_available():
-            r = subprocess.run([O.REF_HARNESS, "load", path, "1"], check=True, capture_output=True,
-                               text=True)
-            ref = json.loads(r.stdout.strip().splitlines()[-1])["best_s"]
-    out.update({"host_bytes_ms": round(host * 1e3, 2), "device_image_ms": round(device * 1e3, 2),
-                "file_ms": round(from_file * 1e3, 2),
+
+    def leg(data, parse, parse_file, suffix, ref_mode):
+        host = best(lambda: parse(data))
+        dev = torch.frombuffer(bytearray(data), dtype=torch.uint8).to("cuda:0")
+        device = best(lambda: parse(dev))
+        del dev
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "bench" + suffix)
+            with open(path, "wb") as fh:
+                fh.write(data)
+            from_file = best(lambda: parse_file(path))
+            ref = None
+            if O.ref_harness_available():
+                r = subprocess.run([O.REF_HARNESS, ref_mode, path, "1"], check=True,
+                                   capture_output=True, text=True)
+                ref = json.loads(r.stdout.strip().splitlines()[-1])["best_s"]
+        return {"bytes": len(data), "host_bytes_ms": round(host * 1e3, 2),
+                "device_image_ms": round(device * 1e3, 2), "file_ms": round(from_file * 1e3, 2),
                 "device_image_GBps": round(len(data) / device / 1e9, 1),
                 "cpu_reference_reader_ms": None if ref is None else round(ref * 1e3, 1),
-                "speedup_file_vs_reference": None if ref is None else round(ref / from_file, 1)})
+                "speedup_file_vs_reference": None if ref is None else round(ref / from_file, 1)}
+
+    out = {"taxa": int(aln.shape[0]), "sites": int(aln.shape[1])}
+    out.update(leg(fasta, ctx.alignment_from_fasta, ctx.alignment_from_fasta_file, ".fa", "load"))
+    out["stockholm"] = leg(sth, ctx.alignment_from_stockholm, ctx.alignment_from_stockholm_file,
+                           ".sth", "sload")
     return out
 
 
@@ -843,7 +853,7 @@ def main() -> int:
             extra["naive_rules"] = naive_side(ctx, aln, wl, out, stream)
         log("side run: host hand-off of the 20000 x 5000 matrix (full rows, lower triangle)")
         extra["handoff"] = handoff_compare(ctx, A)
-        log("side run: FASTA ingest (device reader) vs the reference reader")
+        log("side run: FASTA and Stockholm ingest (device readers) vs the reference readers")
         extra["ingest"] = ingest_compare(ctx, aln)
         log("side run: batched bootstrap vs one launch per replicate")
         extra["bootstrap"] = bootstrap_compare(ctx, dev)

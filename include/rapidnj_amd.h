@@ -233,6 +233,27 @@ RNJ_EXPORT int rnj_alignment_from_fasta(rnj_context* ctx, const char* data, size
 /* The same from a file: read through pinned buffers and uploaded while reading
  * (dataloaderFasta::load, dataloaderFasta.hpp:24-52). */
 RNJ_EXPORT int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignment** out);
+/* Single-line Stockholm parsed on the device, replacing dataloaderStockholm::load
+ * (dataloaderStockholm.hpp:22-111, 183-236, 294-314, the reader behind `-i sth`, src/main.cpp
+ * :362-363,400-401): the reference's byte-at-a-time reader restated as a scan of a two-state
+ * line automaton over the file image in HBM, then the packer. Lines starting (after bytes below
+ * 32) with '#' are markup; "//" ends the alignment and nothing after it is read; any other line
+ * is a sequence: its name is the first run of bytes above 32 (a name ending its line takes its
+ * data from the next line), its data every byte above 32 after that up to the '
+'. Interleaved
+ * blocks are read as the reference reads them, every line a sequence. Errors as the reference's
+ * exits (RNJ_E_INVALID, first in file order): a sequence whose length differs from the first
+ * (message names it), a name followed by an empty line ("has zero length"), the input ending
+ * before "//", fewer than 2 sequences; and, where the reference would go on with empty
+ * sequences, zero-length sequences. The type as for from_chars, except that under the naive
+ * character rules it is discovered after the reference's resolveChar (- . X x z Z b B J j ? as
+ * gaps), as the -f path does. Synchronous. */
+RNJ_EXPORT int rnj_alignment_from_stockholm(rnj_context* ctx, const char* data, size_t len,
+                                            int data_on_device, rnj_alignment** out);
+/* The same from a file (dataloaderStockholm::parseData's input, read through pinned buffers and
+ * uploaded while reading). */
+RNJ_EXPORT int rnj_alignment_from_stockholm_file(rnj_context* ctx, const char* path,
+                                                 rnj_alignment** out);
 /* Names of an alignment read from FASTA, one per '>' record in file order
  * (dataloaderFasta::getSequenceNames, dataloaderFasta.hpp:298-300); 0 / NULL otherwise. */
 RNJ_EXPORT unsigned rnj_alignment_name_count(const rnj_alignment* aln);

@@ -76,6 +76,9 @@ def lib() -> ctypes.CDLL:
         L.rnj_oracle_parse_fasta.restype = ctypes.c_int
         L.rnj_oracle_parse_fasta.argtypes = [ctypes.c_char_p, ctypes.c_size_t] + \
             [ctypes.c_void_p] * 9
+        L.rnj_oracle_parse_stockholm.restype = ctypes.c_int
+        L.rnj_oracle_parse_stockholm.argtypes = [ctypes.c_char_p, ctypes.c_size_t] + \
+            [ctypes.c_void_p] * 8
         L.rnj_oracle_correct_triangle.restype = ctypes.c_int
         L.rnj_oracle_correct_triangle.argtypes = [_u32p, _u32p, _u32p, ctypes.c_uint, ctypes.c_int,
                                                   ctypes.c_void_p, ctypes.c_void_p,
@@ -350,6 +353,38 @@ def parse_fasta(data: bytes) -> tuple[list[bytes], np.ndarray]:
                                  rows.ctypes.data_as(ctypes.c_void_p), names, lens)
     out, o = [], 0
     for k in range(H.value):
+        out.append(names.raw[o:o + lens[k]])
+        o += lens[k]
+    return out, rows
+
+
+class StockholmError(ValueError):
+    """The reference's exits while reading Stockholm: kind 1 = a sequence's length differs from
+    the first one's (seq, length), 2 = fewer than 2 sequences, 3 = a sequence of zero length
+    (seq), 4 = the input ends before "//"."""
+
+    def __init__(self, kind: int, seq: int = 0, length: int = 0):
+        super().__init__(f"stockholm error {kind}: sequence {seq} length {length}")
+        self.kind, self.seq, self.length = kind, seq, length
+
+
+def parse_stockholm(data: bytes) -> tuple[list[bytes], np.ndarray]:
+    """(the name of every sequence, (n, L) uint8 kept bytes of every sequence)."""
+    raw = bytes(data)
+    u = ctypes.c_uint
+    n, L, bad, bad_len = u(), u(), u(), u()
+    nb = ctypes.c_size_t()
+    args = (raw, len(raw), ctypes.byref(n), ctypes.byref(L), ctypes.byref(nb), ctypes.byref(bad),
+            ctypes.byref(bad_len))
+    rc = lib().rnj_oracle_parse_stockholm(*args, None, None, None)
+    if rc:
+        raise StockholmError(rc, bad.value, bad_len.value)
+    rows = np.zeros((n.value, L.value), dtype=np.uint8)
+    names = ctypes.create_string_buffer(max(nb.value, 1))
+    lens = (ctypes.c_uint * max(n.value, 1))()
+    lib().rnj_oracle_parse_stockholm(*args, rows.ctypes.data_as(ctypes.c_void_p), names, lens)
+    out, o = [], 0
+    for k in range(n.value):
         out.append(names.raw[o:o + lens[k]])
         o += lens[k]
     return out, rows

@@ -5,6 +5,7 @@
 //
 // Entry points it drives, all in the reference tree:
 //   dataloaderFasta::readData / getBitStrings / getGapFilters   dataloaderFasta.hpp:24-99,219-251
+//   dataloaderStockholm::load (fastdist)                        dataloaderStockholm.hpp:22-111,183-236
 //   bitDistanceGap::computeDistance                             bitDistanceGap.cpp:60-98
 //   bitDistanceProtein::computeDistance                         bitDistanceProtein.cpp:31-61
 //   simpleDistanceCalculator::computeDistance (the -f path)     simpleDistanceCalculator.cpp:8-16
@@ -18,7 +19,10 @@
 //   fasta  FASTA OUT                  u32 n, L, bitStringsCount, H; H names (u32 length + bytes,
 //                                     every '>' record in file order); then bit strings and gap
 //                                     filters as in `pack` (the reference exits 1 on bad input)
+//   sth    STOCKHOLM OUT              the same for a (single-line) Stockholm alignment, with the
+//                                     name of every sequence line
 //   load   FASTA REPS                 prints {"best_s":..} of parse + pack (readData, fastdist)
+//   sload  STOCKHOLM REPS             the same for the Stockholm reader
 //   pcounts FASTA OUT [naive]         protein (bitDistanceProtein, or with naive the -f path's
 //                                     simpleDistanceCalculator): u32 n, L; then for i<j
 //                                     u64 data[0], data[1], data[2] as the estimator returns them
@@ -38,6 +42,7 @@
 
 #include "stdinclude.h"
 #include "dataloaderFasta.hpp"
+#include "dataloaderStockholm.hpp"
 #include "bitDistanceGap.hpp"
 #include "bitDistanceProtein.hpp"
 #include "simpleDistanceCalculator.hpp"
@@ -134,14 +139,34 @@ int mode_fasta(int argc, char** argv) {
   return 0;
 }
 
+int mode_sth(int argc, char** argv) {
+  if (argc < 4) die("sth STOCKHOLM OUT");
+  dataloaderStockholm* dl = new dataloaderStockholm();
+  dl->readData(argv[2], true, false, UNKNOWN, false);  // exits 1 itself on malformed input
+  if (dl->type != DNA) die("input is not detected as DNA");
+  uint32_t n = dl->getSequenceCount(), L = dl->getSequenceLength(), bsc = dl->getBitStringsCount();
+  vector<string>* names = dl->getSequenceNames();
+  std::ofstream o = open_out(argv[3]);
+  put_u32(o, n); put_u32(o, L); put_u32(o, bsc); put_u32(o, (uint32_t)names->size());
+  for (const string& s : *names) {
+    put_u32(o, (uint32_t)s.size());
+    o.write(s.data(), s.size());
+  }
+  for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getBitStrings()[i]), bsc * 16);
+  for (uint32_t i = 0; i < n; i++) o.write(reinterpret_cast<const char*>(dl->getGapFilters()[i]), bsc * 16);
+  return 0;
+}
+
+// load / sload: best wall time of the reference reader (parse + pack, fastdist) over REPS runs.
+template <class Loader>
 int mode_load(int argc, char** argv) {
-  if (argc < 4) die("load FASTA REPS");
+  if (argc < 4) die("load|sload FILE REPS");
   int reps = atoi(argv[3]);
   double best = 1e30;
   uint32_t n = 0, L = 0;
   for (int r = 0; r < reps; r++) {
     auto t0 = std::chrono::steady_clock::now();
-    dataloaderFasta* dl = new dataloaderFasta();
+    Loader* dl = new Loader();
     dl->readData(argv[2], true, false, UNKNOWN, false);
     auto t1 = std::chrono::steady_clock::now();
     best = std::min(best, std::chrono::duration<double>(t1 - t0).count());
@@ -327,11 +352,13 @@ int mode_tree(int argc, char** argv) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2) die("usage: ref_harness pack|fasta|load|pcounts|pmatrix|counts|matrix|dmatrix|time|ptime|nj|tree ...");
+  if (argc < 2) die("usage: ref_harness pack|fasta|sth|load|sload|pcounts|pmatrix|counts|matrix|dmatrix|time|ptime|nj|tree ...");
   std::string mode = argv[1];
   if (mode == "pack") return mode_pack(argc, argv);
   if (mode == "fasta") return mode_fasta(argc, argv);
-  if (mode == "load") return mode_load(argc, argv);
+  if (mode == "sth") return mode_sth(argc, argv);
+  if (mode == "load") return mode_load<dataloaderFasta>(argc, argv);
+  if (mode == "sload") return mode_load<dataloaderStockholm>(argc, argv);
   if (mode == "pcounts") return mode_pcounts(argc, argv);
   if (mode == "pmatrix") return mode_pmatrix(argc, argv);
   if (mode == "counts") return mode_counts(argc, argv);

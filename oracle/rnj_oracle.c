@@ -419,6 +419,81 @@ int rnj_oracle_parse_fasta(const unsigned char* data, size_t len, unsigned* n, u
   return nseq < 2 ? 2 : 0;
 }
 
+/* ---- Stockholm reader (dataloaderStockholm.hpp:83-111, 118-132, 183-236, 294-314) --------- */
+typedef struct {
+  const unsigned char* d;
+  size_t len, pos;
+} sth_reader;
+
+/* getNextChar (dataloaderStockholm.hpp:118-123): the next byte as a signed char, or -1000 past
+ * the end, where the reference's fillBuffer exits ("Stream ended unexpectedly", :109-117). */
+static int sth_get(sth_reader* r) {
+  if (r->pos < r->len) return (int)(signed char)r->d[r->pos++];
+  return -1000;
+}
+
+int rnj_oracle_parse_stockholm(const unsigned char* data, size_t len, unsigned* n, unsigned* L,
+                               size_t* name_bytes, unsigned* bad, unsigned* bad_len,
+                               unsigned char* rows, unsigned char* names, unsigned* name_len) {
+  sth_reader r = {data, len, 0};
+  unsigned nseq = 0, first_len = 0;
+  size_t nb = 0;
+  const unsigned row_len = rows ? *L : 0;
+#define STH_GET()                        do {                                     c = sth_get(&r);                       if (c == -1000) return 4;            } while (0)
+  int c;
+  STH_GET();
+  for (;;) {                                          /* parseData */
+    while (c < 32) STH_GET();
+    if (c == '#') {                                   /* parseMarkupLine */
+      while (c != 10) STH_GET();
+    } else {
+      size_t s = 0;
+      if (c == '/') {
+        STH_GET();
+        if (c == '/') break;                          /* "//": the end of the alignment */
+        if (names) names[nb] = '/';                   /* tempBuffer = "/" */
+        s = 1;
+      }
+      while (c > 32) {                                /* parseSequenceName */
+        if (names) names[nb + s] = (unsigned char)c;
+        s++;
+        STH_GET();
+      }
+      if (name_len) name_len[nseq] = (unsigned)s;
+      nb += s;
+      while (c < 32) {
+        STH_GET();
+        if (c == 10) {                                /* "has zero length" */
+          *bad = nseq;
+          return 3;
+        }
+      }
+      size_t cur = 0;                                 /* parseSequenceLineFastDist */
+      while (c != 10) {
+        if (c > 32) {
+          if (rows && cur < row_len) rows[(size_t)nseq * row_len + cur] = (unsigned char)c;
+          cur++;
+        }
+        STH_GET();
+      }
+      if (nseq == 0) {
+        first_len = (unsigned)cur;
+      } else if (cur != first_len) {
+        *bad = nseq;
+        *bad_len = (unsigned)cur;
+        return 1;
+      }
+      nseq++;
+    }
+    STH_GET();
+  }
+#undef STH_GET
+  *n = nseq;
+  if (!rows) *L = first_len;
+  *name_bytes = nb;
+  return nseq < 2 ? 2 : 0;
+}
+
 /* ---- protein (dataloaderFasta.hpp:184-199, bitDistanceProtein.cpp:31-119, KimuraDistance.cpp
  * :30-49,322-338, JCdistance.cpp:155-183) --------------------------------------------------- */
 char rnj_oracle_protein_resolve(char c) {

@@ -40,6 +40,20 @@ int rnj_oracle_parse_fasta(const unsigned char* data, size_t len, unsigned* n, u
                            unsigned* H, size_t* name_bytes, unsigned* bad, unsigned* bad_len,
                            unsigned char* rows, unsigned char* names, unsigned* name_len);
 
+/* Stockholm reader, dataloaderStockholm::parseData / parseSequenceName /
+ * parseSequenceLineFastDist restated (dataloaderStockholm.hpp:83-111, 118-132, 183-236, 294-314),
+ * bytes as signed chars: lines starting (after bytes below 32) with '#' are markup, "//" ends the
+ * alignment, any other line is a sequence whose name is its first run of bytes above 32 (a
+ * leading '/' included) and whose data is every byte above 32 after it up to the '\n' -- on
+ * the next line when the name ends the line. Two calls as for rnj_oracle_parse_fasta (one name
+ * per sequence). Returns 0; 1 when a sequence's length differs from the first one's (*bad =
+ * its index, *bad_len = its length); 2 with fewer than 2 sequences; 3 when a name is followed
+ * by an empty line (*bad = its index: "has zero length"); 4 when the input ends before "//"
+ * ("Stream ended unexpectedly"). */
+int rnj_oracle_parse_stockholm(const unsigned char* data, size_t len, unsigned* n, unsigned* L,
+                               size_t* name_bytes, unsigned* bad, unsigned* bad_len,
+                               unsigned char* rows, unsigned char* names, unsigned* name_len);
+
 /* Protein (PROTEIN type): resolveChar (dataloaderFasta.hpp:184-199) maps - . X x z Z b B J j ?
  * to '-', and the byte compare of bitDistanceProtein::computeDistance (bitDistanceProtein.cpp
  * :31-119) over the resolved residues: out[0] = positions where both are residues and differ,

@@ -349,6 +349,201 @@ __global__ void name_len_kernel(unsigned H, const unsigned long long* __restrict
   if (h < H) len[h] = spans[2 * h + 1] - spans[2 * h];
 }
 
+// ---- Stockholm ------------------------------------------------------------------------------
+// The reference reads (single-line) Stockholm with another byte-at-a-time state machine
+// (dataloaderStockholm.hpp:83-111, 118-132, 183-236, 294-314), bytes as signed chars:
+//   * the main loop skips bytes below 32 (blank lines included); then '#' is a markup line
+//     (skipped to its '\n'), "//" ends the alignment, anything else is a sequence line;
+//   * a sequence's name is its first run of bytes above 32; then bytes below 32 are skipped,
+//     each read after the first ending the input with "has zero length" if it is a '\n': a name
+//     that ends its line takes its data from the next line, one followed only by tabs and a
+//     '\n' is an error; the data is every byte above 32 up to the next '\n';
+//   * reading past the end of the input before "//" is an error.
+// Per line this is a map of two states, S (the main loop at the line's start) and P (a name's
+// data pending from the previous line), onto S, P, E (ended) and X (error). The maps are
+// composed by a scan over the lines; then every line knows its state and what it holds.
+enum SthState : uint8_t { kSthS = 0, kSthP = 1, kSthE = 2, kSthX = 3 };
+enum SthError : uint8_t { kSthNoError = 0, kSthZeroLength = 1, kSthEof = 2 };
+
+__device__ __forceinline__ unsigned sth_apply(unsigned map, unsigned st) {
+  return st < 2 ? (map >> (2 * st)) & 3u : st;
+}
+
+struct SthLine {
+  uint8_t out, err;                  // the state after the line; why it is X
+  bool named;                        // a sequence's name starts on this line
+  unsigned long long ns, ne, ds, de;  // name and data spans (empty when absent)
+};
+
+// What line L does when the main loop reaches it in state `st` (kSthS or kSthP); `ended`: the
+// line has its '\n' (the last line of an input without a final '\n' has not).
+__device__ SthLine sth_walk(const uint8_t* b, Line L, bool ended, unsigned st) {
+  SthLine r = {kSthX, kSthEof, false, L.start, L.start, L.start, L.start};
+  unsigned long long p = L.start;
+  auto data_from = [&](unsigned long long q) {  // data from q to the line's end
+    r.ds = q;
+    r.de = L.end;
+    r.out = ended ? kSthS : kSthX;
+    r.err = ended ? kSthNoError : kSthEof;
+  };
+  if (st == kSthP) {  // the skip after a name that ended the previous line
+    while (p < L.end && sbyte(b, p) < 32) ++p;
+    if (p == L.end) r.err = ended ? kSthZeroLength : kSthEof;
+    else data_from(p);
+    return r;
+  }
+  while (p < L.end && sbyte(b, p) < 32) ++p;
+  if (p == L.end) {  // blank: the main loop goes on to the next line
+    if (ended) r.out = kSthS, r.err = kSthNoError;
+    return r;
+  }
+  const int c = sbyte(b, p);
+  if (c == '#') {
+    if (ended) r.out = kSthS, r.err = kSthNoError;
+    return r;
+  }
+  if (c == '/' && p + 1 < L.end && sbyte(b, p + 1) == '/') {
+    r.out = kSthE;
+    r.err = kSthNoError;
+    return r;
+  }
+  unsigned long long q = p;  // the name: bytes > 32 (a leading '/' is one of them)
+  while (q < L.end && sbyte(b, q) > 32) ++q;
+  r.named = true;
+  r.ns = p;
+  r.ne = q;
+  if (q == L.end) {  // the name ends the line: its data is on the next one
+    if (ended) r.out = kSthP, r.err = kSthNoError;
+    return r;
+  }
+  if (sbyte(b, q) == 32) {  // data from the space on
+    data_from(q);
+    return r;
+  }
+  unsigned long long t = q + 1;  // tabs, '\r', other controls, bytes >= 128
+  while (t < L.end && sbyte(b, t) < 32) ++t;
+  if (t == L.end) r.err = ended ? kSthZeroLength : kSthEof;
+  else data_from(t);
+  return r;
+}
+
+// Map of each line: bits 0-1 the state after it from S, bits 2-3 from P.
+__global__ void sth_map_kernel(const uint8_t* __restrict__ b, size_t N,
+                               const unsigned long long* __restrict__ nl, unsigned long long n_nl,
+                               unsigned long long M, uint8_t* __restrict__ map) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M) return;
+  const Line L = line_at(nl, n_nl, N, l);
+  const bool ended = l < n_nl;
+  map[l] = (uint8_t)(sth_walk(b, L, ended, kSthS).out | (sth_walk(b, L, ended, kSthP).out << 2));
+}
+
+// The state at every line's start (one work-group: each thread composes a run of lines, thread 0
+// chains the runs, then each thread walks its run again); *final = the state after the last line.
+__global__ __launch_bounds__(256) void sth_state_kernel(const uint8_t* __restrict__ map,
+                                                        unsigned long long M,
+                                                        uint8_t* __restrict__ state,
+                                                        uint8_t* __restrict__ final_state) {
+  __shared__ uint8_t agg[256], entry[257];
+  const unsigned t = threadIdx.x;
+  const unsigned long long run = (M + 255) / 256;
+  const unsigned long long lo = t * run, hi = lo + run < M ? lo + run : M;
+  unsigned m = kSthS | (kSthP << 2);  // identity
+  for (unsigned long long l = lo; l < hi; ++l)
+    m = sth_apply(map[l], sth_apply(m, kSthS)) | (sth_apply(map[l], sth_apply(m, kSthP)) << 2);
+  agg[t] = (uint8_t)m;
+  __syncthreads();
+  if (t == 0) {
+    unsigned st = kSthS;
+    for (unsigned k = 0; k < 256; ++k) {
+      entry[k] = (uint8_t)st;
+      st = sth_apply(agg[k], st);
+    }
+    entry[256] = (uint8_t)st;
+    *final_state = (uint8_t)st;
+  }
+  __syncthreads();
+  unsigned st = entry[t];
+  for (unsigned long long l = lo; l < hi; ++l) {
+    state[l] = (uint8_t)st;
+    st = sth_apply(map[l], st);
+  }
+}
+
+// Each line in its state: whether a name starts on it, the name and data spans, the data's kept
+// bytes and the kept count before the span; the first line that ends in an error.
+__global__ void sth_line_kernel(const uint8_t* __restrict__ b, size_t N,
+                                const unsigned long long* __restrict__ nl, unsigned long long n_nl,
+                                unsigned long long M, const unsigned long long* __restrict__ Kc,
+                                const uint8_t* __restrict__ state, uint32_t* __restrict__ named,
+                                unsigned long long* __restrict__ nspan,
+                                unsigned long long* __restrict__ dspan,
+                                uint32_t* __restrict__ dcount, unsigned long long* __restrict__ kds,
+                                unsigned long long* __restrict__ err_at) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M) return;
+  const Line L = line_at(nl, n_nl, N, l);
+  SthLine r = {kSthE, kSthNoError, false, L.start, L.start, L.start, L.start};
+  if (state[l] < 2) r = sth_walk(b, L, l < n_nl, state[l]);
+  named[l] = r.named;
+  nspan[2 * l] = r.ns;
+  nspan[2 * l + 1] = r.ne;
+  dspan[2 * l] = r.ds;
+  dspan[2 * l + 1] = r.de;
+  const unsigned long long k0 = kept_before(b, Kc, r.ds);
+  const unsigned long long k1 = r.ds == r.de ? k0 : kept_before(b, Kc, r.de);
+  dcount[l] = (uint32_t)(k1 - k0);
+  kds[l] = k0;
+  if (r.out == kSthX && r.err != kSthNoError) atomicMin(err_at, (l << 2) | r.err);
+}
+
+// Every data line's sequence (the last name before or on it), that sequence's length and data
+// line, and the first sequence whose length differs from sequence 0's.
+__global__ void sth_record_kernel(unsigned long long M, const uint32_t* __restrict__ named,
+                                  const uint32_t* __restrict__ nbase,
+                                  const unsigned long long* __restrict__ dspan,
+                                  const uint32_t* __restrict__ dcount,
+                                  unsigned long long* __restrict__ rec_len,
+                                  unsigned long long* __restrict__ rec_line) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M || dspan[2 * l] == dspan[2 * l + 1]) return;
+  const unsigned r = nbase[l] + named[l] - 1;
+  rec_len[r] = dcount[l];
+  rec_line[r] = l;
+}
+
+__global__ void sth_length_check_kernel(unsigned R, const unsigned long long* __restrict__ rec_len,
+                                        const unsigned long long* __restrict__ rec_line,
+                                        unsigned long long* __restrict__ bad) {
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R && rec_line[r] != ~0ull && rec_len[r] != rec_len[0])
+    atomicMin(bad, (rec_line[r] << 32) | r);
+}
+
+// Destination of each data line's first kept byte (less the kept count before its span).
+__global__ void sth_dest_kernel(unsigned long long M, const uint32_t* __restrict__ named,
+                                const uint32_t* __restrict__ nbase,
+                                const unsigned long long* __restrict__ dspan,
+                                const unsigned long long* __restrict__ kds, unsigned long long L,
+                                unsigned long long* __restrict__ dbase) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M) return;
+  dbase[l] = dspan[2 * l] < dspan[2 * l + 1]
+                 ? (unsigned long long)(nbase[l] + named[l] - 1) * L - kds[l]
+                 : 0ull;
+}
+
+// Name span of every sequence, at its index.
+__global__ void sth_name_kernel(unsigned long long M, const uint32_t* __restrict__ named,
+                                const uint32_t* __restrict__ nbase,
+                                const unsigned long long* __restrict__ nspan,
+                                unsigned long long* __restrict__ spans) {
+  const unsigned long long l = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= M || !named[l]) return;
+  spans[2 * nbase[l]] = nspan[2 * l];
+  spans[2 * nbase[l] + 1] = nspan[2 * l + 1];
+}
+
 // ---- exclusive scan (three phases, recursive over block totals) ------------------------------
 template <class Tin, class Tout>
 __global__ __launch_bounds__(kThreads) void scan_blocks_kernel(const Tin* __restrict__ in,
@@ -484,11 +679,18 @@ T read_scalar(const T* d, hipStream_t s, hipError_t* e) {
   return v;
 }
 
-}  // namespace
+// The file image in device memory, its '\n' positions and its kept-byte chunk prefix.
+struct Image {
+  const uint8_t* b = nullptr;
+  size_t N = 0;
+  unsigned long long* nl = nullptr;  // '\n' positions
+  unsigned long long n_nl = 0;
+  unsigned long long* Kc = nullptr;  // kept bytes (33..127) before every 16-byte chunk
+  unsigned long long M = 0;          // lines (a last line without '\n' counts)
+};
 
-int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind stage,
-                       hipStream_t s, Alignment** out) {
-  Temps t{ctx, s, {}};
+int image_lines(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind stage, hipStream_t s,
+                Temps& t, Image* im) {
   hipError_t e;
   const uint8_t* b = src;
   if (stage == hipMemcpyHostToDevice || stage == hipMemcpyDeviceToDevice) {
@@ -500,8 +702,6 @@ int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind
       FA_HIP(hipMemcpyAsync(copy, src, N, stage, s));
     b = copy;
   }
-  // 1. '\n' positions
-  if (N == 0) return fail(RNJ_E_INVALID, "empty FASTA input: no sequences");
   const unsigned long long nblk = (N + kChunk - 1) / kChunk;
   const unsigned long long nchunks = (N + 15) / 16;
   uint32_t* blk = nullptr;
@@ -528,9 +728,64 @@ int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind
   uint8_t last = '\n';
   FA_HIP(hipMemcpyAsync(&last, b + N - 1, 1, hipMemcpyDeviceToHost, s));
   FA_HIP(hipStreamSynchronize(s));
-  const unsigned long long M = n_nl + (last != '\n' ? 1 : 0);  // lines
-  if (M == 0) return fail(RNJ_E_INVALID, "empty FASTA input: no sequences");
+  im->b = b;
+  im->N = N;
+  im->nl = nl;
+  im->n_nl = n_nl;
+  im->Kc = Kc;
+  im->M = n_nl + (last != '\n' ? 1 : 0);
+  return RNJ_OK;
+}
 
+// The H names whose spans (device, [2H]) are given, copied to the host.
+hipError_t gather_names(Temps& t, hipStream_t s, const uint8_t* b, unsigned H,
+                        const unsigned long long* spans, std::vector<std::string>* names) {
+  unsigned long long *nlen = nullptr, *noff = nullptr, *ntot = nullptr;
+  uint8_t* blob = nullptr;
+  hipError_t e = t.get(&nlen, H);
+  if (e == hipSuccess) e = t.get(&noff, H);
+  if (e == hipSuccess) e = t.get(&ntot, 1);
+  if (e == hipSuccess) {
+    name_len_kernel<<<dim3(grid_for(H, 256)), dim3(256), 0, s>>>(H, spans, nlen);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = exclusive_scan<unsigned long long, unsigned long long>(nlen, H, noff, ntot, s, t);
+  unsigned long long total = 0;
+  if (e == hipSuccess) total = read_scalar(ntot, s, &e);
+  if (e == hipSuccess) e = t.get(&blob, total);
+  if (e == hipSuccess && total > 0) {
+    name_gather_kernel<<<dim3(H), dim3(64), 0, s>>>(b, H, spans, noff, blob);
+    e = hipGetLastError();
+  }
+  std::vector<unsigned long long> hoff(H), hlen(H);
+  std::vector<char> hblob(total);
+  if (e == hipSuccess) e = hipMemcpyAsync(hoff.data(), noff, H * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(hlen.data(), nlen, H * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && total > 0)
+    e = hipMemcpyAsync(hblob.data(), blob, total, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) {
+    names->reserve(H);
+    for (unsigned h = 0; h < H; ++h) names->emplace_back(hblob.data() + hoff[h], hlen[h]);
+  }
+  return e;
+}
+
+}  // namespace
+
+int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind stage,
+                       hipStream_t s, Alignment** out) {
+  Temps t{ctx, s, {}};
+  hipError_t e;
+  // 1. '\n' positions
+  if (N == 0) return fail(RNJ_E_INVALID, "empty FASTA input: no sequences");
+  Image im;
+  int rc0 = image_lines(ctx, src, N, stage, s, t, &im);
+  if (rc0 != RNJ_OK) return rc0;
+  const uint8_t* b = im.b;
+  unsigned long long *nl = im.nl, *Kc = im.Kc;
+  const unsigned long long n_nl = im.n_nl, M = im.M;
+  if (M == 0) return fail(RNJ_E_INVALID, "empty FASTA input: no sequences");
   // 2. lines
   uint32_t *hcount = nullptr, *dcount = nullptr, *hbase = nullptr, *H_d = nullptr;
   unsigned long long *dspan = nullptr, *kds = nullptr, *off = nullptr, *total_d = nullptr;
@@ -629,45 +884,165 @@ int fasta_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind
   if (rc != RNJ_OK) return rc;
   e = hipSuccess;
   if (H > 0) {
-    unsigned long long *spans = nullptr, *nlen = nullptr, *noff = nullptr, *ntot = nullptr;
-    uint8_t* blob = nullptr;
+    unsigned long long* spans = nullptr;
     e = t.get(&spans, 2ull * H);
-    if (e == hipSuccess) e = t.get(&nlen, H);
-    if (e == hipSuccess) e = t.get(&noff, H);
-    if (e == hipSuccess) e = t.get(&ntot, 1);
     if (e == hipSuccess) {
       header_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(b, N, nl, n_nl, M, hcount,
                                                                                hbase, spans);
       e = hipGetLastError();
     }
-    if (e == hipSuccess) {
-      name_len_kernel<<<dim3(grid_for(H, 256)), dim3(256), 0, s>>>(H, spans, nlen);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = exclusive_scan<unsigned long long, unsigned long long>(nlen, H, noff, ntot, s, t);
-    unsigned long long total = 0;
-    if (e == hipSuccess) total = read_scalar(ntot, s, &e);
-    if (e == hipSuccess) e = t.get(&blob, total);
-    if (e == hipSuccess && total > 0) {
-      name_gather_kernel<<<dim3(H), dim3(64), 0, s>>>(b, H, spans, noff, blob);
-      e = hipGetLastError();
-    }
-    std::vector<unsigned long long> hoff(H), hlen(H);
-    std::vector<char> hblob(total);
-    if (e == hipSuccess) e = hipMemcpyAsync(hoff.data(), noff, H * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hlen.data(), nlen, H * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && total > 0)
-      e = hipMemcpyAsync(hblob.data(), blob, total, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e == hipSuccess) {
-      a->names.reserve(H);
-      for (unsigned h = 0; h < H; ++h) a->names.emplace_back(hblob.data() + hoff[h], hlen[h]);
-    }
+    if (e == hipSuccess) e = gather_names(t, s, b, H, spans, &a->names);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     rnj_alignment_free(static_cast<rnj_alignment*>(a));
     return hip_fail(e, "FASTA parse");
+  }
+  *out = a;
+  return RNJ_OK;
+}
+
+int stockholm_to_alignment(Context* ctx, const uint8_t* src, size_t N, hipMemcpyKind stage,
+                           hipStream_t s, Alignment** out) {
+  Temps t{ctx, s, {}};
+  hipError_t e;
+  const char* eof_msg =
+      "the Stockholm input ends before the \"//\" line that closes the alignment (the reference "
+      "exits: stream ended unexpectedly)";
+  if (N == 0) return fail(RNJ_E_INVALID, eof_msg);
+  Image im;
+  int rc = image_lines(ctx, src, N, stage, s, t, &im);
+  if (rc != RNJ_OK) return rc;
+  const unsigned long long M = im.M;
+  // 1. the state at every line's start
+  uint8_t *map = nullptr, *state = nullptr, *final_d = nullptr;
+  FA_HIP(t.get(&map, M));
+  FA_HIP(t.get(&state, M));
+  FA_HIP(t.get(&final_d, 1));
+  sth_map_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(im.b, N, im.nl, im.n_nl, M, map);
+  FA_HIP(hipGetLastError());
+  sth_state_kernel<<<dim3(1), dim3(256), 0, s>>>(map, M, state, final_d);
+  FA_HIP(hipGetLastError());
+  // 2. what every line holds
+  uint32_t *named = nullptr, *dcount = nullptr, *nbase = nullptr, *R_d = nullptr;
+  unsigned long long *nspan = nullptr, *dspan = nullptr, *kds = nullptr, *err_d = nullptr;
+  FA_HIP(t.get(&named, M));
+  FA_HIP(t.get(&dcount, M));
+  FA_HIP(t.get(&nspan, 2 * M));
+  FA_HIP(t.get(&dspan, 2 * M));
+  FA_HIP(t.get(&kds, M));
+  FA_HIP(t.get(&err_d, 1));
+  FA_HIP(hipMemsetAsync(err_d, 0xff, sizeof(unsigned long long), s));
+  sth_line_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(im.b, N, im.nl, im.n_nl, M, im.Kc,
+                                                               state, named, nspan, dspan, dcount,
+                                                               kds, err_d);
+  FA_HIP(hipGetLastError());
+  FA_HIP(t.get(&nbase, M));
+  FA_HIP(t.get(&R_d, 1));
+  FA_HIP((exclusive_scan<uint32_t, uint32_t>(named, M, nbase, R_d, s, t)));
+  const unsigned R = read_scalar(R_d, s, &e);
+  FA_HIP(e);
+  const uint8_t final_state = read_scalar(final_d, s, &e);
+  FA_HIP(e);
+  const unsigned long long err = read_scalar(err_d, s, &e);
+  FA_HIP(e);
+  // 3. sequences: length and data line of each, the first whose length differs from the first's
+  unsigned long long *rec_len = nullptr, *rec_line = nullptr, *bad_d = nullptr;
+  FA_HIP(t.get(&rec_len, R));
+  FA_HIP(t.get(&rec_line, R));
+  FA_HIP(t.get(&bad_d, 1));
+  FA_HIP(hipMemsetAsync(bad_d, 0xff, sizeof(unsigned long long), s));
+  if (R > 0) {
+    FA_HIP(hipMemsetAsync(rec_line, 0xff, R * sizeof(unsigned long long), s));
+    sth_record_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(M, named, nbase, dspan, dcount,
+                                                                   rec_len, rec_line);
+    FA_HIP(hipGetLastError());
+    sth_length_check_kernel<<<dim3(grid_for(R, 256)), dim3(256), 0, s>>>(R, rec_len, rec_line,
+                                                                         bad_d);
+    FA_HIP(hipGetLastError());
+  }
+  const unsigned long long bad = read_scalar(bad_d, s, &e);
+  FA_HIP(e);
+  // The reference stops at the first error in file order: a line that errs (a name with an
+  // empty line, the input ending), a sequence of another length (detected at its '\n'), or the
+  // end of the input without "//".
+  const unsigned long long err_line = err != ~0ull ? err >> 2 : ~0ull;
+  const unsigned long long bad_line = bad != ~0ull ? bad >> 32 : ~0ull;
+  const unsigned long long end_line = (final_state == kSthS || final_state == kSthP) ? M : ~0ull;
+  if (bad_line < err_line && bad_line < end_line) {
+    const unsigned r = (unsigned)(bad & 0xffffffffu);
+    unsigned long long blen = 0, len0 = 0;
+    FA_HIP(hipMemcpyAsync(&blen, rec_len + r, sizeof(blen), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(&len0, rec_len, sizeof(len0), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    return fail(RNJ_E_INVALID, "sequence " + std::to_string(r) + " has length " +
+                                   std::to_string(blen) + " while the preceding sequences have length " +
+                                   std::to_string(len0));
+  }
+  if (err_line != ~0ull && err_line <= end_line) {
+    if ((err & 3u) == kSthEof) return fail(RNJ_E_INVALID, eof_msg);
+    // the name of the sequence whose data is missing: the last name on or before the line
+    uint32_t nm = 0, nb = 0;
+    FA_HIP(hipMemcpyAsync(&nm, named + err_line, sizeof(nm), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(&nb, nbase + err_line, sizeof(nb), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    const unsigned long long src_line = nm ? err_line : err_line - 1;  // a P line's name is above
+    unsigned long long sp[2] = {0, 0};
+    FA_HIP(hipMemcpyAsync(sp, nspan + 2 * src_line, sizeof(sp), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    std::string name(sp[1] - sp[0], '\0');
+    if (!name.empty())
+      FA_HIP(hipMemcpyAsync(&name[0], im.b + sp[0], name.size(), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    return fail(RNJ_E_INVALID, "sequence " + std::to_string(nb + nm - 1) + " (\"" + name +
+                                   "\") has zero length");
+  }
+  if (end_line != ~0ull) return fail(RNJ_E_INVALID, eof_msg);
+  if (R < 2)
+    return fail(RNJ_E_INVALID, "found " + std::to_string(R) +
+                                   " sequences in the Stockholm input; at least 2 are needed");
+  unsigned long long L = 0;
+  FA_HIP(hipMemcpyAsync(&L, rec_len, sizeof(L), hipMemcpyDeviceToHost, s));
+  FA_HIP(hipStreamSynchronize(s));
+  if (L == 0) return fail(RNJ_E_INVALID, "the Stockholm sequences are empty");
+  if (L >= (1ull << 32) - 64) return fail(RNJ_E_INVALID, "sequences longer than 2^32 sites");
+  // 4. character matrix
+  uint8_t* chars = nullptr;
+  unsigned long long* dbase = nullptr;
+  FA_HIP(t.get(&chars, (unsigned long long)R * L));
+  FA_HIP(t.get(&dbase, M));
+  sth_dest_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(M, named, nbase, dspan, kds, L,
+                                                               dbase);
+  FA_HIP(hipGetLastError());
+  const unsigned long long scatter_waves = (N + kScatterBytes - 1) / kScatterBytes;
+  scatter_kernel<<<dim3(grid_for(scatter_waves, 4)), dim3(256), 0, s>>>(im.b, N, im.nl, im.n_nl,
+                                                                        im.Kc, dspan, dbase, chars);
+  FA_HIP(hipGetLastError());
+  // 5. type: the reference discovers it on the first sequence (dataloaderStockholm.hpp:53-76); on
+  // the -f path it does so after resolveChar, while the type is still unknown, i.e. with the
+  // protein rule that turns - . X x z Z b B J j ? into '-' (:238-249, :261-274)
+  std::vector<char> row0(L);
+  FA_HIP(hipMemcpyAsync(row0.data(), chars, L, hipMemcpyDeviceToHost, s));
+  FA_HIP(hipStreamSynchronize(s));
+  if (ctx->char_rules == RNJ_CHARS_NAIVE)
+    for (char& c : row0)
+      if (std::strchr("-.XxzZbBJj?", c) != nullptr && c != '\0') c = '-';
+  const int kind = alignment_kind(ctx, row0.data(), (unsigned)L);
+  // 6. pack + names
+  Alignment* a = nullptr;
+  rc = pack_alignment(ctx, chars, L, R, (unsigned)L, kind, s, &a);
+  if (rc != RNJ_OK) return rc;
+  unsigned long long* spans = nullptr;
+  e = t.get(&spans, 2ull * R);
+  if (e == hipSuccess) {
+    sth_name_kernel<<<dim3(grid_for(M, 256)), dim3(256), 0, s>>>(M, named, nbase, nspan, spans);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = gather_names(t, s, im.b, R, spans, &a->names);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    rnj_alignment_free(static_cast<rnj_alignment*>(a));
+    return hip_fail(e, "Stockholm parse");
   }
   *out = a;
   return RNJ_OK;

@@ -1053,8 +1053,13 @@ int rnj_alignment_free(rnj_alignment* aln) {
   return RNJ_OK;
 }
 
-int rnj_alignment_from_fasta(rnj_context* ctx, const char* data, size_t len, int data_on_device,
-                             rnj_alignment** out) {
+using TextParser = int (*)(Context*, const uint8_t*, size_t, hipMemcpyKind, hipStream_t,
+                           Alignment**);
+
+// A text alignment in host or device memory through `parse` (fasta_to_alignment or
+// stockholm_to_alignment).
+static int alignment_from_text(rnj_context* ctx, const char* data, size_t len, int data_on_device,
+                               rnj_alignment** out, TextParser parse) {
   if (!ctx || (!data && len) || !out) return fail(RNJ_E_INVALID, "null argument");
   int rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
@@ -1066,13 +1071,26 @@ int rnj_alignment_from_fasta(rnj_context* ctx, const char* data, size_t len, int
                               : aligned       ? hipMemcpyDefault
                                               : hipMemcpyDeviceToDevice;
   Alignment* a = nullptr;
-  rc = fasta_to_alignment(ctx, reinterpret_cast<const uint8_t*>(data), len, stage, ctx->stream, &a);
+  rc = parse(ctx, reinterpret_cast<const uint8_t*>(data), len, stage, ctx->stream, &a);
   if (rc != RNJ_OK) return rc;
   *out = static_cast<rnj_alignment*>(a);
   return RNJ_OK;
 }
 
-int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignment** out) {
+int rnj_alignment_from_fasta(rnj_context* ctx, const char* data, size_t len, int data_on_device,
+                             rnj_alignment** out) {
+  return alignment_from_text(ctx, data, len, data_on_device, out, fasta_to_alignment);
+}
+
+int rnj_alignment_from_stockholm(rnj_context* ctx, const char* data, size_t len,
+                                 int data_on_device, rnj_alignment** out) {
+  return alignment_from_text(ctx, data, len, data_on_device, out, stockholm_to_alignment);
+}
+
+// A text alignment file: read through the context's two pinned chunks and uploaded while
+// reading, then parsed on the device by `parse`.
+static int alignment_from_text_file(rnj_context* ctx, const char* path, rnj_alignment** out,
+                                    TextParser parse) {
   if (!ctx || !path || !out) return fail(RNJ_E_INVALID, "null argument");
   int rc = set_device(ctx);
   if (rc != RNJ_OK) return rc;
@@ -1131,15 +1149,23 @@ int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignm
   if (e != hipSuccess || short_read) {
     if (dev) (void)hipFree(dev);
     return short_read ? fail(RNJ_E_INVALID, std::string("short read: ") + path)
-                      : hip_fail(e, "rnj_alignment_from_fasta_file upload");
+                      : hip_fail(e, "alignment file upload");
   }
   Alignment* a = nullptr;
-  rc = fasta_to_alignment(ctx, dev, len, hipMemcpyDefault, ctx->stream, &a);
+  rc = parse(ctx, dev, len, hipMemcpyDefault, ctx->stream, &a);
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(dev);
   if (rc != RNJ_OK) return rc;
   *out = static_cast<rnj_alignment*>(a);
   return RNJ_OK;
+}
+
+int rnj_alignment_from_fasta_file(rnj_context* ctx, const char* path, rnj_alignment** out) {
+  return alignment_from_text_file(ctx, path, out, fasta_to_alignment);
+}
+
+int rnj_alignment_from_stockholm_file(rnj_context* ctx, const char* path, rnj_alignment** out) {
+  return alignment_from_text_file(ctx, path, out, stockholm_to_alignment);
 }
 
 unsigned rnj_alignment_name_count(const rnj_alignment* aln) {

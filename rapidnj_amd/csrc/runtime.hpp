@@ -210,6 +210,9 @@ int pack_alignment(Context* ctx, const uint8_t* chars, size_t pitch, unsigned n,
 // into a (cached) device buffer of the parse.
 int fasta_to_alignment(Context* ctx, const uint8_t* bytes, size_t len, hipMemcpyKind stage,
                        hipStream_t stream, Alignment** out);
+// The same for a (single-line) Stockholm image (fasta.hip).
+int stockholm_to_alignment(Context* ctx, const uint8_t* bytes, size_t len, hipMemcpyKind stage,
+                           hipStream_t stream, Alignment** out);
 
 // Enqueue a multi-job launch on `stream`: reset statistics, tile kernel, saturation patch
 // (unless raw), max copies. All jobs must share one tile configuration (chosen here). exact:

@@ -61,6 +61,10 @@ SIGNATURES = {
     "rnj_alignment_from_bitstrings": (ctypes.c_int, [_vp, _vp, _vp, _u, _u, ctypes.POINTER(_vp)]),
     "rnj_alignment_from_fasta": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_int, ctypes.POINTER(_vp)]),
     "rnj_alignment_from_fasta_file": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "rnj_alignment_from_stockholm": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_int,
+                                                    ctypes.POINTER(_vp)]),
+    "rnj_alignment_from_stockholm_file": (ctypes.c_int, [_vp, ctypes.c_char_p,
+                                                         ctypes.POINTER(_vp)]),
     "rnj_alignment_name_count": (_u, [_vp]),
     "rnj_alignment_name": (ctypes.c_char_p, [_vp, _u]),
     "rnj_alignment_free": (ctypes.c_int, [_vp]),
@@ -396,6 +400,26 @@ class Context:
     def alignment_from_fasta_file(self, path: str) -> Alignment:
         h = _vp()
         check(lib().rnj_alignment_from_fasta_file(self.handle, os.fsencode(path), ctypes.byref(h)))
+        return Alignment(self, h.value)
+
+    def alignment_from_stockholm(self, data) -> Alignment:
+        """Parse a single-line Stockholm image on the device (bytes / uint8 numpy on the host, or
+        a uint8 torch tensor on the device) and pack it; sequence names in Alignment.names."""
+        h = _vp()
+        if hasattr(data, "data_ptr"):
+            check(lib().rnj_alignment_from_stockholm(self.handle, data.data_ptr(), data.numel(), 1,
+                                                     ctypes.byref(h)))
+        else:
+            buf = np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray) \
+                else np.ascontiguousarray(data, dtype=np.uint8)
+            check(lib().rnj_alignment_from_stockholm(self.handle, buf.ctypes.data, buf.size, 0,
+                                                     ctypes.byref(h)))
+        return Alignment(self, h.value)
+
+    def alignment_from_stockholm_file(self, path: str) -> Alignment:
+        h = _vp()
+        check(lib().rnj_alignment_from_stockholm_file(self.handle, os.fsencode(path),
+                                                      ctypes.byref(h)))
         return Alignment(self, h.value)
 
     # ---- distances ----------------------------------------------------------------------------
