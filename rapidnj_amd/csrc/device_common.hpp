@@ -93,21 +93,25 @@ __device__ __forceinline__ float proportion(uint32_t a, uint32_t valid) {
   return (float)((double)a / (double)valid);
 }
 
-// Natural logarithm of a normal double 0 < z <= 1 in FP64 arithmetic, through a 128-entry table
+// Natural logarithm of a normal double 0 < z < 1 in FP64 arithmetic, through a 128-entry table
 // (Tang's table-driven method): z = 2^k m with m in [0.5, 1); i = the mantissa's 7 bits below the
 // leading one picks m's interval [0.5 + i/256, 0.5 + (i+1)/256) and its right end
-// c = 0.5 + (i+1)/256 (the last is 1, so near z = 1 the reduced argument is z - 1 itself); m - c
-// is exact and r = (m - c) / c = (m - c) * (1/c) has -2^-7 < r <= 0; then
+// c = 0.5 + (i+1)/256 (the last is 1, so near z = 1 the reduced argument is z - 1 itself), and
+// r = m/c - 1, with -2^-7 < r <= 0, is formed as fma(m, 1/c, -1): one rounding plus m times the
+// rounding of the table's 1/c, below 2^-52 absolute (and exact on the last interval, where
+// 1/c = 1). Then
 //   ln z = k ln 2 + ln c + ln(1 + r),  ln(1 + r) = r - r^2/2 + r^3/3 - r^4/4 + r^5/5,
 // whose truncation is below |r|^6/6 < 4e-14 absolute (and 5e-12 relative on the last interval,
 // where ln c = 0). Where ln c != 0 and k = 0, |ln z| >= ln(256/255), so the relative error is
 // below 2e-11 everywhere: rounded to float, the result equals the reference's libm value except
 // in ties within ~1e-9 of a float rounding boundary. 1/c and ln c come rounded to double from the
 // table (log_table.inc, written by gen_log_table.py); k ln 2 uses ln 2 split into a 32-bit head
-// (k * head is exact) and a tail. z = 1 (= 2^1 * 0.5, which the reduction would only bring to
-// within 1e-13 of 0) returns 0 exactly, as libm does. 20 VALU instructions and one LDS read: the
-// kernel stages the table in LDS (stage_log_table), so that every lane reads its own entry
-// without a divergent global load.
+// (k * head is exact) and a tail. z = 1 is not special-cased: reduced as 2^1 * 0.5 it gives a
+// value within 1e-13 of 0: the DNA corrections select their exact result wherever z can be 1,
+// and the protein ones use log_table_one.
+// Any other argument (0, negative) reads an in-range table entry and returns a value that the
+// callers discard. 15 VALU instructions and one LDS read: the kernel stages the table in LDS
+// (stage_log_table), so that every lane reads its own entry without a divergent global load.
 struct LogEntry {
   double inv_c, ln_c;
 };
@@ -127,18 +131,16 @@ __device__ __forceinline__ void stage_log_table(LogEntry* dst, unsigned tid, uns
 // v_mov for every pair an epilogue corrects; an empty asm with an SGPR operand makes each one an
 // opaque uniform value that the FP64 instructions read straight from its SGPR pair.
 struct LogConsts {
-  double ln2_hi, ln2_lo, c0, step, p5, p4, p3;
+  double ln2_hi, ln2_lo, p5, p4, p3;
   const LogEntry* tab;  // the table in LDS
 };
 
 __device__ __forceinline__ LogConsts log_consts(const LogEntry* tab) {
   LogConsts K = {0x1.62e42ff000000p-1,    // ln 2, 32-bit head: k * head is exact
                  -0x1.718432a1b0e26p-35,  // ln 2 - head
-                 0.5 + 1.0 / 256.0, 1.0 / 256.0, 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0, tab};
+                 1.0 / 5.0, -1.0 / 4.0, 1.0 / 3.0, tab};
   asm volatile("" : "+s"(K.ln2_hi));
   asm volatile("" : "+s"(K.ln2_lo));
-  asm volatile("" : "+s"(K.c0));
-  asm volatile("" : "+s"(K.step));
   asm volatile("" : "+s"(K.p5));
   asm volatile("" : "+s"(K.p4));
   asm volatile("" : "+s"(K.p3));
@@ -149,15 +151,20 @@ __device__ __forceinline__ double log_table(double z, const LogConsts& K) {
   int k;
   const double m = frexp(z, &k);  // m in [0.5, 1)
   const unsigned i = ((unsigned)__double2hiint(m) >> 13) & 127u;
-  const double c = fma((double)i, K.step, K.c0);  // 0.5 + (i+1)/256, exact
   const LogEntry e = K.tab[i];
-  const double r = (m - c) * e.inv_c;             // m - c exact (Sterbenz)
+  const double r = fma(m, e.inv_c, -1.0);  // m/c - 1
   double p = fma(r, K.p5, K.p4);
   p = fma(p, r, K.p3);
   p = fma(p, r, -0.5);
   const double l1 = fma(r * r, p, r);
   const double kd = (double)k;
-  const double lz = fma(kd, K.ln2_hi, fma(kd, K.ln2_lo, e.ln_c + l1));
+  return fma(kd, K.ln2_hi, fma(kd, K.ln2_lo, e.ln_c + l1));
+}
+
+// log_table with ln 1 = 0 exactly, for the protein corrections: their 64-bit valid counts (which
+// the reference lets wrap) can make a proportion so small that the log argument rounds to 1.
+__device__ __forceinline__ double log_table_one(double z, const LogConsts& K) {
+  const double lz = log_table(z, K);
   return z == 1.0 ? 0.0 : lz;
 }
 
@@ -205,6 +212,15 @@ __device__ __forceinline__ float proportion1(uint32_t a, uint32_t v) {
 //  * JC forms 1 - p/0.75 as fma(-p, 4/3, 1) (2^-52 absolute).
 // Either way the float result equals the reference's unless the exact value lies within about
 // 1e-9 relative of a float rounding boundary; it is always well inside 1e-6.
+// With kSmall the reference's x = 1 - 2P - Q and y = 1 - 2Q are formed in one fma each, to the
+// same doubles: P and Q are floats in [0, 1] quotients of integers below 2^24, so each is zero
+// or at least 2^-24 with an ulp of at least 2^-47, and 1 - Q, x and y are multiples of 2^-47 in
+// [-1, 1], exact in double whatever the order of the operations. For the same reason a nonzero
+// x or y has magnitude at least 2^-47, which float holds, so the reference's saturation test on
+// (float)x and (float)y is the test on x and y themselves.
+// The corrections of the default path take no select on their log argument: where the pair is
+// saturated or has no differences the argument is whatever x^2 y is, and the select at the end
+// replaces the log's value; every other argument is below 1.
 template <bool kExact, bool kSmall>
 __device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t valid,
                                            const LogConsts& K) {
@@ -216,14 +232,26 @@ __device__ __forceinline__ float k2p_value(uint32_t ts, uint32_t tv, uint32_t va
   } else {
     proportions2<kSmall>(ts, tv, v, P, Q);
   }
-  const double x = 1.0 - 2.0 * (double)P - (double)Q;
-  const double y = 1.0 - 2.0 * (double)Q;
-  const bool sat = valid == 0 || (float)x <= 0.0f || (float)y <= 0.0f;
-  const double xs = sat ? 1.0 : x, ys = sat ? 1.0 : y;
-  const float d = kExact ? (float)(-0.5 * log(xs) - 0.25 * log(ys))
-                         : (float)(-0.25 * log_table(xs * xs * ys, K));
-  // valid == 0 implies ts = tv = 0
-  return (ts | tv) == 0 ? (valid == 0 ? -1.0f : -0.0f) : (sat ? -1.0f : d);
+  double x, y;
+  bool sat;
+  if (kSmall && !kExact) {
+    x = fma(-2.0, (double)P, 1.0 - (double)Q);
+    y = fma(-2.0, (double)Q, 1.0);
+    sat = valid == 0 || x <= 0.0 || y <= 0.0;
+  } else {
+    x = 1.0 - 2.0 * (double)P - (double)Q;
+    y = 1.0 - 2.0 * (double)Q;
+    sat = valid == 0 || (float)x <= 0.0f || (float)y <= 0.0f;
+  }
+  float d;
+  if (kExact) {
+    const double xs = sat ? 1.0 : x, ys = sat ? 1.0 : y;
+    d = (float)(-0.5 * log(xs) - 0.25 * log(ys));
+  } else {
+    d = (float)(-0.25 * log_table(x * x * y, K));
+  }
+  // valid == 0 implies ts = tv = 0 (and sets sat); otherwise ts = tv = 0 gives x = y = 1
+  return sat ? -1.0f : ((ts | tv) == 0 ? -0.0f : d);
 }
 
 template <bool kExact, bool kSmall>
@@ -241,7 +269,7 @@ __device__ __forceinline__ float jc_value(uint32_t mismatches, uint32_t valid,
     d = (float)(-b * log(sat ? 1.0 : 1.0 - q));
   } else {
     sat = p >= 0.75f;  // the same test: a float p < 0.75 gives p / 0.75 < 1 - 2^-25 in double
-    d = (float)(-b * log_table(sat ? 1.0 : fma(-(double)p, 4.0 / 3.0, 1.0), K));
+    d = (float)(-b * log_table(fma(-(double)p, 4.0 / 3.0, 1.0), K));  // < 1 unless discarded
   }
   return (valid == 0 || mismatches == 0) ? -0.0f : (sat ? -1.0f : d);
 }

@@ -82,7 +82,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, const LogEntry* 
         for (int b = 0; b < MJ; ++b) {
           row[b] = d[b];
           if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
-          else lmax = fmaxf(lmax, d[b]);
+          lmax = fmaxf(lmax, d[b]);  // a saturated pair's -1 leaves it as it is
         }
       }
     };
@@ -110,7 +110,7 @@ __device__ __forceinline__ void tile_epilogue(const JobDesc& J, const LogEntry* 
       if (j >= n || i >= n) continue;
       if (lower ? i > j : i < j) {
         if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
-        else lmax = fmaxf(lmax, d[b]);
+        lmax = fmaxf(lmax, d[b]);  // a saturated pair's -1 leaves it as it is
         store_pair(J, i, j, d[b]);
       } else if (i == j) {
         finish_diagonal(J, i);

@@ -90,7 +90,7 @@ __device__ __forceinline__ float kimura_protein_of_p(float p, const LogConsts& K
   const bool formula = p < 0.75f && p > 0.0f;
   const float pp = p * p;
   const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
-  const double lg = kExact ? log(arg) : log_table(arg, K);
+  const double lg = kExact ? log(arg) : log_table_one(arg, K);
   int idx = (int)((double)p * 1000.0 - 750.0);
   idx = idx < 0 ? 0 : (idx > 180 ? 180 : idx);
   const float table = (double)p > 0.930 ? 10.0f : kDayhoff[idx];
@@ -112,7 +112,7 @@ __device__ __forceinline__ float jc_protein_of_p(float p, const LogConsts& K) {
   const double q = (double)p / b;
   const bool sat = q >= 1.0;
   const double arg = sat ? 1.0 : 1.0 - q;
-  const double lg = kExact ? log(arg) : log_table(arg, K);
+  const double lg = kExact ? log(arg) : log_table_one(arg, K);
   return sat ? -1.0f : (float)(-b * lg);
 }
 
@@ -203,7 +203,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntr
         for (int b = 0; b < MJ; ++b) {
           row[b] = d[b];
           if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
-          else lmax = fmaxf(lmax, d[b]);
+          lmax = fmaxf(lmax, d[b]);  // a saturated pair's -1 leaves it as it is
         }
       }
     };
@@ -230,7 +230,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntr
       if (j >= n) continue;
       if (i < j) {
         if (d[b] == -1.0f) sat |= 1u << (a * MJ + b);
-        else lmax = fmaxf(lmax, d[b]);
+        lmax = fmaxf(lmax, d[b]);  // a saturated pair's -1 leaves it as it is
         store_pair(J, i, j, d[b]);  // protein jobs are never kRows (their counts are symmetric)
       } else if (i == j) {
         finish_diagonal(J, i);

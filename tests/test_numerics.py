@@ -41,6 +41,33 @@ def _k2p_reference(ts, tv, valid):
     return x, y, ok
 
 
+def test_k2p_x_and_y_exact_and_saturation_on_the_doubles():
+    """Below 2^24 valid sites the reference's x = 1 - 2P - Q and y = 1 - 2Q are exact in double
+    (so the device's fma(-2, P, 1 - Q) and fma(-2, Q, 1) give the same doubles), and their float
+    roundings are <= 0 exactly when the doubles are: the device tests x <= 0 and y <= 0 directly.
+    Cases crowd the saturation boundary 2 ts + tv = valid and 2 tv = valid."""
+    from fractions import Fraction
+    k = 200_000
+    valid = RNG.integers(1, 1 << 24, size=k, dtype=np.int64)
+    tv = (RNG.random(k) * valid).astype(np.int64)
+    ts = np.maximum((valid - tv) // 2 + RNG.integers(-2, 3, size=k), 0)  # near 2 ts + tv = valid
+    ts = np.minimum(ts, valid - tv)
+    half = valid[:20_000] // 2 + RNG.integers(-1, 2, size=20_000)        # near 2 tv = valid
+    tv[:20_000] = np.clip(half, 0, valid[:20_000])
+    ts[:20_000] = 0
+    x, y, ok = _k2p_reference(ts, tv, valid)
+    assert np.array_equal(ok, (x > 0) & (y > 0))
+    assert ((x > 0) != (x.astype(np.float32) > 0)).sum() == 0
+    P = (ts.astype(np.float32) / valid.astype(np.float32)).astype(np.float64)
+    Q = (tv.astype(np.float32) / valid.astype(np.float32)).astype(np.float64)
+    for idx in RNG.choice(k, 3000, replace=False):
+        fp, fq = Fraction(P[idx]), Fraction(Q[idx])
+        assert Fraction(x[idx]) == 1 - 2 * fp - fq
+        assert Fraction(y[idx]) == 1 - 2 * fq
+        assert Fraction(1.0 - Q[idx]) == 1 - fq
+    assert (np.abs(x[x != 0]) >= 2.0 ** -47).all()
+
+
 def test_single_log_k2p_within_tolerance():
     k = 400_000
     valid = RNG.integers(1, 30001, size=k)
@@ -113,8 +140,7 @@ def _log_table_np(z):
     inv, ln = _log_table_entries()
     m, k = np.frexp(z)                                   # m in [0.5, 1)
     i = ((m.view(np.uint64) >> np.uint64(45)) & np.uint64(127)).astype(np.int64)
-    c = i * (1.0 / 256.0) + (0.5 + 1.0 / 256.0)
-    r = (m - c) * inv[i]
+    r = m * inv[i] - 1.0                                 # fma(m, 1/c, -1) on the device
     p = r * 0.2 - 0.25
     p = p * r + 1.0 / 3.0
     p = p * r - 0.5
@@ -128,8 +154,10 @@ def test_log_table_accuracy():
     """Against libm's log (within an ulp): relative error below 2e-11 over z in [2^-80, 1),
     densely near 1 (the small distances, where the last interval reduces to z - 1 exactly) and at
     every table interval's ends -- so the float distances it yields are the reference's except in
-    ties within ~1e-9 of a float rounding boundary. z = 1 is returned as 0 by a select; without it
-    the reduction (1 = 2 * 0.5) would leave an error below 1e-13."""
+    ties within ~1e-9 of a float rounding boundary. z = 1 is not special-cased: the reduction
+    (1 = 2 * 0.5) leaves a value below 1e-13; the DNA corrections select their exact result for
+    every pair whose argument can be 1 (no differences, saturation), and the protein corrections
+    call log_table_one, which returns 0 for 1."""
     z = np.concatenate([
         np.exp2(-RNG.random(200_000) * 80.0),
         1.0 - np.exp2(-RNG.random(200_000) * 52.0),
