@@ -413,6 +413,7 @@ def nj_side(ctx, A, cpu: bool, sample_n: int = 5000) -> dict:
     import torch
     from oracle import oracle as O
     n = A.n
+    sample_n = min(sample_n, n)
     d = torch.empty((n, n), dtype=torch.float32, device="cuda:0")
     ctx.distances(A, "k2p", layout="square", out=d, exact=True)
     torch.cuda.synchronize()
@@ -474,7 +475,7 @@ def side_main(args) -> int:
         res = ingest_compare(ctx, aln)
     elif args.side == "nj":
         A = ctx.alignment(aln)
-        res = nj_side(ctx, A, not args.no_cpu_baseline)
+        res = nj_side(ctx, A, not args.no_cpu_baseline, args.nj_reference_taxa)
         A.close()
     else:
         A = ctx.alignment(aln)
@@ -717,6 +718,10 @@ def main() -> int:
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--side", choices=["ingest", "handoff", "nj", "valu"],
                     help="run only this side measurement on the workload and print its JSON")
+    ap.add_argument("--nj-reference-taxa", type=int, default=5000,
+                    help="with --side nj: the leading block (taxa) on which the reference's "
+                         "rapidNJ is timed and its tree compared (default 5000; the whole "
+                         "matrix takes the reference about a minute)")
     args = ap.parse_args()
     if args.probe_traffic:
         return probe_traffic(args)

@@ -61,6 +61,20 @@ __constant__ float kDayhoff[181] = {
     RNJ_PAM(988)};
 #undef RNJ_PAM
 
+// The tables the corrections read, staged in LDS by each work-group: the log table and the
+// Dayhoff distances. Every lane reads its own entry, so an LDS copy serves the divergent
+// lookups that a read of the constant table would turn into a branch around a global load.
+struct ProteinTables {
+  LogEntry log[128];
+  float day[184];
+};
+
+__device__ __forceinline__ void stage_protein_tables(ProteinTables& t, unsigned tid,
+                                                     unsigned nthreads) {
+  stage_log_table(t.log, tid, nthreads);
+  for (unsigned k = tid; k < 181; k += nthreads) t.day[k] = kDayhoff[k];
+}
+
 // ---- corrections ---------------------------------------------------------------------------
 // mism and valid are the reference's data[0] (data[1] is 0 for protein) and data[2], valid being
 // its 64-bit count. Proportions are float quotients of the float-converted counts, as the
@@ -86,33 +100,42 @@ __device__ __forceinline__ float protein_proportion(uint32_t mism, unsigned long
 // float, the rest in double), the Dayhoff table up to 0.930 and 10 above. Straight-line (every
 // case computed, the result selected) so that a caller can interleave several pairs.
 template <bool kExact>
-__device__ __forceinline__ float kimura_protein_of_p(float p, const LogConsts& K) {
+__device__ __forceinline__ float kimura_protein_of_p(float p, const LogConsts& K,
+                                                    const float* day) {
   const bool formula = p < 0.75f && p > 0.0f;
   const float pp = p * p;
   const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
   const double lg = kExact ? log(arg) : log_table_one(arg, K);
   int idx = (int)((double)p * 1000.0 - 750.0);
   idx = idx < 0 ? 0 : (idx > 180 ? 180 : idx);
-  const float table = (double)p > 0.930 ? 10.0f : kDayhoff[idx];
+  float tab = day[idx];
+  asm volatile("" : "+v"(tab));  // read for every pair: kept out of a branch on p
+  const float table = (double)p > 0.930 ? 10.0f : tab;
   return p < 0.75f ? (formula ? (float)(-lg) : p) : table;
 }
 
 template <bool kExact>
 __device__ __forceinline__ float kimura_protein_value(uint32_t mism, unsigned long long valid,
-                                                     const LogConsts& K) {
+                                                     const LogConsts& K, const float* day) {
   return kimura_protein_of_p<kExact>(valid != 0 ? protein_proportion<kExact>(mism, valid) : -1.0f,
-                                     K);
+                                     K, day);
 }
 
 // JCdistance.cpp:170-183 with alpha = 20: p = 0 when valid is 0, -1 when p / 0.95 >= 1 (the
 // quotient in double, as the reference forms it, so that the saturated pairs are the same).
+// The default path tests p >= b instead (the same test: b = 0.95 rounded to double is no float,
+// so a float p above it gives a quotient of at least 1 and one below it a quotient below
+// 1 - 2^-25) and forms 1 - p/b as fma(-p, 1/b, 1), within 2^-52 absolute, as the DNA JC does.
 template <bool kExact>
 __device__ __forceinline__ float jc_protein_of_p(float p, const LogConsts& K) {
   const double b = 19.0 / 20.0;  // (alpha - 1) / alpha
-  const double q = (double)p / b;
-  const bool sat = q >= 1.0;
-  const double arg = sat ? 1.0 : 1.0 - q;
-  const double lg = kExact ? log(arg) : log_table_one(arg, K);
+  if (kExact) {
+    const double q = (double)p / b;
+    const bool sat = q >= 1.0;
+    return sat ? -1.0f : (float)(-b * log(sat ? 1.0 : 1.0 - q));
+  }
+  const bool sat = (double)p >= b;
+  const double lg = log_table_one(fma(-(double)p, 20.0 / 19.0, 1.0), K);
   return sat ? -1.0f : (float)(-b * lg);
 }
 
@@ -132,7 +155,7 @@ __device__ __forceinline__ float jc_protein_value(uint32_t mism, unsigned long l
 // pairs, simpleDistanceCalculator.cpp:40-55); a bootstrap replicate's length is read from
 // J.len_dev.
 template <int TILE, int MI, int MJ, bool kExact>
-__device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntry* log_tab,
+__device__ __forceinline__ void protein_epilogue(const JobDesc& J, const ProteinTables& T,
                                                  unsigned tile, unsigned bi,
                                                  unsigned bj, unsigned ty, unsigned tx,
                                                  unsigned tid,
@@ -172,7 +195,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntr
   }
   float lmax = 0.0f;
   uint32_t sat = 0;  // saturated pairs of the micro-tile, bit a * MJ + b
-  const LogConsts K = log_consts(log_tab);
+  const LogConsts K = log_consts(T.log);
   // Inner tiles of an upper-triangle output with fewer than 2^24 sites: the counts in 32-bit
   // arithmetic and each row's corrections as straight-line code, stored from one triangle index.
   // The valid count is then s = L - bias - (gaps_i + gaps_j - #g) in int32; the reference's
@@ -191,12 +214,14 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntr
           const uint32_t gsum = gi[a] + gj[b];
           const uint32_t mism = acc_x[a][b] + 2u * acc_g[a][b] - gsum;
           const int s = lb - (int)gsum + (int)acc_g[a][b];
-          const float pq = (float)((double)mism * reciprocal(s > 0 ? (uint32_t)s : 1u));
+          float pq = (float)((double)mism * reciprocal(s > 0 ? (uint32_t)s : 1u));
+          // formed for every pair: kept out of a branch on s, so the row's pairs interleave
+          asm volatile("" : "+v"(pq));
           const float pw = __builtin_ldexpf((float)mism, -64);
           if constexpr (decltype(model)::value == kJC)
             d[b] = jc_protein_of_p<false>(s == 0 ? 0.0f : (s < 0 ? pw : pq), K);
           else
-            d[b] = kimura_protein_of_p<false>(s == 0 ? -1.0f : (s < 0 ? pw : pq), K);
+            d[b] = kimura_protein_of_p<false>(s == 0 ? -1.0f : (s < 0 ? pw : pq), K, T.day);
         }
         const gptr_mut<float> row = (gptr_mut<float>)J.out + tri_index(i, j0, n);
 #pragma unroll
@@ -222,7 +247,7 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const LogEntr
       unsigned long long valid;
       counts(a, b, mism, valid);
       d[b] = J.model == kJC ? jc_protein_value<kExact>(mism, valid, K)
-                            : kimura_protein_value<kExact>(mism, valid, K);
+                            : kimura_protein_value<kExact>(mism, valid, K, T.day);
     }
 #pragma unroll
     for (int b = 0; b < MJ; ++b) {
@@ -264,7 +289,7 @@ __attribute__((amdgpu_waves_per_eu(kProteinWaves<TILE, CW, kExact>))) void prote
 
   __shared__ v4u sI[E];
   __shared__ v4u sJ[E];
-  __shared__ LogEntry sLog[128];
+  __shared__ ProteinTables sTab;
 
   const unsigned item = xcd_item(blockIdx.x, gridDim.x);
   const unsigned k = find_job(jobs, njobs, item);
@@ -276,7 +301,7 @@ __attribute__((amdgpu_waves_per_eu(kProteinWaves<TILE, CW, kExact>))) void prote
 
   const unsigned tid = threadIdx.x;
   const unsigned ty = tid / NTX, tx = tid % NTX;
-  stage_log_table(sLog, tid, NT);  // visible after the chunk loop's first barrier
+  stage_protein_tables(sTab, tid, NT);  // visible after the chunk loop's first barrier
   const gptr<v4u> gI = (gptr<v4u>)(J.planes + (size_t)bi * TILE);
   const gptr<v4u> gJ = (gptr<v4u>)(J.planes + (size_t)bj * TILE);
   const size_t row4 = J.n_pad / 4;
@@ -352,7 +377,7 @@ __attribute__((amdgpu_waves_per_eu(kProteinWaves<TILE, CW, kExact>))) void prote
     return;
   }
   if (c0 >= c1) __syncthreads();  // no chunk, so no barrier yet since the table was staged
-  protein_epilogue<TILE, MI, MJ, kExact>(J, sLog, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
+  protein_epilogue<TILE, MI, MJ, kExact>(J, sTab, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
 }
 
 // Sums the partial counts of the splits of one tile and finishes it (kernels.hip numbering).
@@ -370,8 +395,8 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_finish_kern
   const JobDesc& J = jobs[lo];
   const unsigned tile = item - J.fin_base;
   if (J.splits <= 1 || tile >= J.tiles) return;  // uniform over the work-group
-  __shared__ LogEntry sLog[128];
-  stage_log_table(sLog, threadIdx.x, NT);
+  __shared__ ProteinTables sTab;
+  stage_protein_tables(sTab, threadIdx.x, NT);
   __syncthreads();
   unsigned bi, bj;
   patch_coords(tile, J.nblk, bi, bj);
@@ -393,7 +418,7 @@ __global__ __launch_bounds__((TILE / MI) * (TILE / MJ)) void protein_finish_kern
         acc_g[a][b] += slot[plane + a * MJ + b];
       }
   }
-  protein_epilogue<TILE, MI, MJ, kExact>(J, sLog, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
+  protein_epilogue<TILE, MI, MJ, kExact>(J, sTab, tile, bi, bj, ty, tx, tid, acc_x, acc_g);
 }
 
 template <int TILE, int MI, int MJ, int CW, int B>

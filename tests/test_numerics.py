@@ -107,6 +107,21 @@ def test_jc_fma_argument_within_tolerance():
     assert rel.max() < 1e-7
 
 
+def test_jc_protein_saturation_and_fma_argument():
+    """JC with alpha = 20 (protein): for every float p in [0.9, 1.05] the reference's test
+    (double)p / 0.95 >= 1 equals the device's (double)p >= 0.95, and over [0, 0.95) the device's
+    fma(-p, 20/19, 1) stays within 2^-52 of the reference's 1 - p / 0.95."""
+    b = 19.0 / 20.0
+    lo, hi = np.float32(0.9).view(np.uint32), np.float32(1.05).view(np.uint32)
+    p = np.arange(lo, hi + 1, dtype=np.uint32).view(np.float32).astype(np.float64)
+    assert np.array_equal(p / b >= 1.0, p >= b)
+    q = np.concatenate([RNG.random(300_000).astype(np.float32), p[p < b].astype(np.float32)])
+    q = q[q.astype(np.float64) < b].astype(np.float64)
+    ref = 1.0 - q / b
+    got = 1.0 - q * (20.0 / 19.0)                 # the fma's argument up to one more rounding
+    assert np.abs(got - ref).max() <= 2.0 ** -52
+
+
 # ---- the table-driven FP64 logarithm of the device epilogues (device_common.hpp log_table) ------
 def _log_table_entries():
     import os
