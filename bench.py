@@ -316,42 +316,52 @@ def peak_fields(pk: dict, achieved: float) -> dict:
             "ceiling": pk["ceiling"], "class_model": cm}
 
 
-def naive_side(ctx, aln, wl, out, stream) -> dict:
-    """The workload's alignment under the naive character rules (rnj_set_char_rules: a fourth
-    plane, 6 bitwise + 3 v_bcnt per 32 site-pairs): step time, and the raw launch (reset + tile
-    kernel + job maximum) against the ceiling of that instruction mix."""
+def model_side(ctx, A, model: str, naive: bool, wl, out, stream) -> dict:
+    """Alignment A (the workload's taxa and sites) under `model`: step time, and the raw launch
+    (reset + tile kernel + job maximum) against the ceiling of its count loop's instruction
+    mix."""
     import torch
-    ctx.set_char_rules(True)
-    try:
-        An = ctx.alignment(aln)
-    finally:
-        ctx.set_char_rules(False)
     pairs = wl.n * (wl.n - 1) // 2
     for _ in range(3):
-        ctx.distances(An, wl.model, out=out, stream=stream)
+        ctx.distances(A, model, out=out, stream=stream)
     torch.cuda.synchronize()
     k = 10
     t0 = time.perf_counter()
     for _ in range(k):
-        ctx.distances(An, wl.model, out=out, stream=stream)
+        ctx.distances(A, model, out=out, stream=stream)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / k * 1e3
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(k):
-        ctx.distances(An, wl.model, out=out, raw=True, stream=stream)
+        ctx.distances(A, model, out=out, raw=True, stream=stream)
     e1.record(stream)
     torch.cuda.synchronize()
     kern_ms = e0.elapsed_time(e1) / k
-    pk = valu_peak(ctx, An, wl.model, naive=True)
-    achieved = 9 * pairs * (wl.L / 32.0) / (kern_ms / 1e3) / 1e12
-    cfg = ctx.tile_config_for(An, wl.model)
-    An.close()
-    return {"ms_per_step": round(ms, 4), "value": round(pairs * wl.L / (ms / 1e3) / 1e9, 2),
+    pk = valu_peak(ctx, A, model, naive=naive)
+    ops = 9 if naive else ops_per_word_pair(A, model)
+    achieved = ops * pairs * (wl.L / 32.0) / (kern_ms / 1e3) / 1e12
+    cfg = ctx.tile_config_for(A, model)
+    return {"model": model, "ms_per_step": round(ms, 4),
+            "value": round(pairs * wl.L / (ms / 1e3) / 1e9, 2),
             "unit": "G site-pairs/s", "kernel_ms": round(kern_ms, 4),
             "valu": {"achieved": round(achieved, 3), "unit": "TOP/s",
-                     **peak_fields(pk, achieved), "ops_per_32_site_pairs": 9},
+                     **peak_fields(pk, achieved), "ops_per_32_site_pairs": ops},
             "tile_config": tile_name(cfg)}
+
+
+def naive_side(ctx, aln, wl, out, stream) -> dict:
+    """The workload's alignment under the naive character rules (rnj_set_char_rules: a fourth
+    plane, 6 bitwise + 3 v_bcnt per 32 site-pairs)."""
+    ctx.set_char_rules(True)
+    try:
+        An = ctx.alignment(aln)
+    finally:
+        ctx.set_char_rules(False)
+    try:
+        return model_side(ctx, An, wl.model, True, wl, out, stream)
+    finally:
+        An.close()
 
 
 def tile_name(cfg: int) -> str:
@@ -856,6 +866,9 @@ def main() -> int:
         if not wl.protein and not wl.replicates:
             log(f"side run: naive character rules (the reference's -f counter), {wl.name}")
             extra["naive_rules"] = naive_side(ctx, aln, wl, out, stream)
+            if wl.model != "jc":
+                log(f"side run: JC on the same alignment (5 ops per 32 site-pairs), {wl.name}")
+                extra["jc"] = model_side(ctx, A, "jc", False, wl, out, stream)
         log("side run: host hand-off of the 20000 x 5000 matrix (full rows, lower triangle)")
         extra["handoff"] = handoff_compare(ctx, A)
         log("side run: FASTA and Stockholm ingest (device readers) vs the reference readers")
