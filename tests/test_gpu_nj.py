@@ -141,6 +141,25 @@ def test_oracle_larger(ctx, n, L, seed):
     same_joins(ctx.nj(device(m)), O.nj(m))
 
 
+@pytest.mark.skipif(not O.ref_harness_available(), reason="reference harness not built")
+def test_5000_taxa_same_newick_as_reference_rapidnj(ctx, tmp_path):
+    """A 5000-taxa K2P matrix (simulated 5000 x 1500 alignment, the oracle's post-processed
+    distances): the device tree's Newick text equals, byte for byte, the one the reference's own
+    rapidNJ (compiled from its sources) writes for the same matrix."""
+    import subprocess
+    n = 5000
+    aln = simulate_alignment(n, 1500, 0.05, 21)
+    _, m, _ = O.distance_matrix(aln, "k2p")
+    mf, nwk = tmp_path / "m.bin", tmp_path / "t.nwk"
+    with open(mf, "wb") as fh:
+        np.array([n], dtype=np.uint32).tofile(fh)
+        np.ascontiguousarray(m, dtype=np.float32).tofile(fh)
+    subprocess.run([O.REF_HARNESS, "nj", str(mf), str(nwk)], check=True, capture_output=True,
+                   timeout=120)
+    tree = ctx.nj(device(m))
+    assert tree.newick() == nwk.read_bytes()
+
+
 def test_strided_matrix_and_names(ctx):
     import torch
     d = np.load(os.path.join(GOLDEN, "lgt70.npz"), allow_pickle=False)
