@@ -50,11 +50,11 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   // device state
   DevBuf<NjState> st;
   DevBuf<int> index_to_id, id_to_index, red_count, rtm, red_head, red_next, slot_of_index;
-  DevBuf<int> slot_index, alive, row_p, map, left, right, row_tie, newidx, log_idx;
-  DevBuf<float> sepsum, sep, newcol, term, sums, S0, S1, row_maxsep, init_maxsep, log_val;
+  DevBuf<int> slot_index, alive, row_p, map, left, right, row_tie, newidx, log_idx, shave;
+  DevBuf<float> sepsum, sep, newcol, term, sums, S0, S1, row_maxsep, init_maxsep, log_val, smin;
   DevBuf<unsigned> row_sigma, key_by_index;
   DevBuf<unsigned long long> row_key;
-  DevBuf<double> llen, rlen;
+  DevBuf<double> llen, rlen, sref;
   // the iteration plan: slot bound, compaction before it, slot-matrix buffer, log offset
   const int iters = (int)n - 2;
   std::vector<int> plan_ub(iters), plan_buf(iters);
@@ -88,6 +88,9 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   NJ_HIP(row_sigma.alloc(cap));
   NJ_HIP(row_tie.alloc(cap));
   NJ_HIP(row_maxsep.alloc(cap));
+  NJ_HIP(smin.alloc(cap));
+  NJ_HIP(sref.alloc(cap));
+  NJ_HIP(shave.alloc(cap));
   NJ_HIP(init_maxsep.alloc(cap));
   NJ_HIP(key_by_index.alloc(n));
   NJ_HIP(newidx.alloc(n));
@@ -195,6 +198,8 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   h_st.cluster_count = (int)n;
   h_st.current_id = (int)n;
   h_st.negative_branches = negative_branches ? 1 : 0;
+  h_st.start_bound = FLT_MAX;  // no bound before the first step
+  h_st.start_row = -1;
 
   // rapidNJ::run: n - 2 joins. The plan's slot bound bounds the slots in use (exact until the
   // first compaction, then the live count's bound clusterCount plus the slots added since). The
@@ -243,6 +248,8 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   NJ_HIP(up(sep.p, h_sep.data(), cap * sizeof(float)));
   NJ_HIP(up(init_maxsep.p, h_init_maxsep.data(), cap * sizeof(float)));
   NJ_HIP(hipMemsetAsync(key_by_index.p, 0xff, n * sizeof(unsigned), s));  // no answers yet
+  NJ_HIP(hipMemsetAsync(row_p.p, 0xff, cap * sizeof(int), s));
+  NJ_HIP(hipMemsetAsync(shave.p, 0, cap * sizeof(int), s));
   NJ_HIP(nj_launch_gather(M, ldm, d_vis.p, v, S0.p, cap, s));
 
   NjDevice D{};
@@ -264,6 +271,9 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
   D.row_sigma = row_sigma.p;
   D.row_tie = row_tie.p;
   D.row_maxsep = row_maxsep.p;
+  D.smin = smin.p;
+  D.sref = sref.p;
+  D.shave = shave.p;
   D.init_maxsep = init_maxsep.p;
   D.newcol = newcol.p;
   D.term = term.p;
@@ -400,7 +410,7 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
       const int a = soi[r];
       if (!slot_ok(a) || key[a] == ~0ull) continue;
       const float vr = unkey((unsigned)(key[a] >> 32));
-      if (!(vr < G)) continue;
+      if (!(vr < G)) continue;  // for a row the scan skipped, vr is a lower bound of its answer
       if (fkey_host(G) > sig[a] && !tie[a]) {  // the row's own answer stands
         G = vr;
         partner = rp[a];
@@ -410,7 +420,13 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
       // findRowMin (rapidNJ.cpp:324-361) with searchRedundantEntries (:413-430)
       bool improved = false;
       const float row_sep = sp[a];
-      const float row_max_sep = row_sep + mxs[a];
+      float mx = mxs[a];
+      if (tie[a] == 2) {  // skipped by the scan: maxRowSeparation over the live slots below a
+        mx = FLT_MIN;
+        for (int b = 0; b < a; ++b)
+          if (sp[b] > mx) mx = sp[b];  // gone slots hold NaN
+      }
+      const float row_max_sep = row_sep + mx;
       if (rc[r] > 1) {
         const float cv = -2.0f * row_sep;
         if (cv < G) {

@@ -22,6 +22,12 @@ struct NjState {
   int error;          // kNjFailed or kNjHalted; every kernel does nothing while it is set
   int forced;         // the next step takes f_* as its search result instead of searching
   int f_min1, f_min2, f_cache1, f_cache2;
+  // Row skipping (nj.hip): the sum of each step's largest separation rise, and the criterion of
+  // one live pair of the next search's start row (an upper bound of that row's answer; +inf when
+  // none) with the start row.
+  double drift;
+  float start_bound;
+  int start_row;  // matrix index; -1 when there is no bound
 };
 // A slot's separation is NaN once its cluster is gone, so that scans skip it without a flag.
 
@@ -52,6 +58,9 @@ struct NjDevice {
                        // running minimum at or below it
   int* row_tie;        // 1 when several pairs share its best criterion and distance
   float* row_maxsep;   // its maxRowSeparation, as the scan found it
+  float* smin;         // min over the row's live pairs of float(d - sep[partner]) at its last scan
+  double* sref;        // NjState::drift at that scan
+  int* shave;          // 1 once smin / sref hold a scan's values
   float* init_maxsep;  // maxRowSeparations of the first search (every smaller index)
   float* newcol;       // new cluster's distances (step)
   float* term;         // dist * redundantCount terms of the new separation sum
