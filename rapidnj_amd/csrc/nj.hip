@@ -4,9 +4,10 @@
 // scans a row's live pairs in that order until the first whose float
 // distance - (sep[row] + maxRowSeparation[row]) is not below the running minimum; it joins the
 // first pair, in its visiting order, that lowers the minimum of the criterion
-// (d - sep[partner]) - sep[row] last. Here every live pair is evaluated each iteration (the scan
-// is a full pass over the live matrix, which HBM and the 256 MB last-level cache serve at several
-// TB/s), with the reference's float arithmetic and visiting order:
+// (d - sep[partner]) - sep[row] last. Here every live pair of every row that can hold a record
+// is evaluated each iteration (a pass over those rows of the live matrix, which HBM and the
+// 256 MB last-level cache serve at several TB/s; see "Row skipping" below), with the
+// reference's float arithmetic and visiting order:
 //   * a pair (a, b) lives in the row of the cluster with the larger id; the matrix is kept in
 //     "slots" ordered by cluster id (new clusters take the next slot), so row slot a's partners
 //     are exactly the live slots b < a;
