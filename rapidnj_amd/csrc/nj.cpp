@@ -399,6 +399,12 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
       std::sort(out.begin(), out.end());
       return RNJ_OK;
     };
+    // maxRowSeparation of a row the scan skipped: the largest separation of the live slots
+    // below it, from FLT_MIN as the scan takes it (gone slots hold NaN, which no comparison
+    // takes), as a prefix maximum over the slots
+    std::vector<float> below(used + 1);
+    below[0] = FLT_MIN;
+    for (int b = 0; b < used; ++b) below[b + 1] = sp[b] > below[b] ? sp[b] : below[b];
     float G = FLT_MAX;
     int partner = -1;
     std::vector<int> records;
@@ -420,13 +426,7 @@ int run_nj(Context* ctx, const float* M, size_t ldm, unsigned n, int negative_br
       // findRowMin (rapidNJ.cpp:324-361) with searchRedundantEntries (:413-430)
       bool improved = false;
       const float row_sep = sp[a];
-      float mx = mxs[a];
-      if (tie[a] == 2) {  // skipped by the scan: maxRowSeparation over the live slots below a
-        mx = FLT_MIN;
-        for (int b = 0; b < a; ++b)
-          if (sp[b] > mx) mx = sp[b];  // gone slots hold NaN
-      }
-      const float row_max_sep = row_sep + mx;
+      const float row_max_sep = row_sep + (tie[a] == 2 ? below[a] : mxs[a]);
       if (rc[r] > 1) {
         const float cv = -2.0f * row_sep;
         if (cv < G) {
