@@ -83,6 +83,9 @@ def lib() -> ctypes.CDLL:
         L.rnj_oracle_correct_triangle.argtypes = [_u32p, _u32p, _u32p, ctypes.c_uint, ctypes.c_int,
                                                   ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.POINTER(ctypes.c_float)]
+        L.rnj_oracle_nj_skip_audit.restype = ctypes.c_int
+        L.rnj_oracle_nj_skip_audit.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_void_p]
         L.rnj_oracle_nj.restype = ctypes.c_int
         L.rnj_oracle_nj.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int] + \
             [ctypes.c_void_p] * 5 + [ctypes.POINTER(ctypes.c_float)]
@@ -305,6 +308,20 @@ def nj(matrix: np.ndarray, negative_branches: bool = False):
     if rc != 0:
         raise RuntimeError(f"oracle nj failed ({rc})")
     return left[: n - 2], right[: n - 2], llen[: n - 2], rlen[: n - 2], last, ld.value
+
+
+def nj_skip_audit(matrix: np.ndarray, negative_branches: bool = False,
+                  drop_drift: bool = False) -> dict:
+    """The device NJ's row-skipping rule checked along rapidNJ's own run (O(n^3));
+    drop_drift plants a mistake (the bound without the separations' drift)."""
+    m = np.ascontiguousarray(matrix, dtype=np.float32)
+    out = np.zeros(4, dtype=np.int64)
+    rc = lib().rnj_oracle_nj_skip_audit(m.ctypes.data, m.shape[0], int(negative_branches),
+                                        1 if drop_drift else 0, out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle nj audit failed ({rc})")
+    return {"bound_violations": int(out[0]), "record_violations": int(out[1]),
+            "skipped": int(out[2]), "scanned": int(out[3])}
 
 
 def newick(n: int, tree, names=None) -> bytes:

@@ -87,6 +87,10 @@ int rnj_oracle_protein_matrix(const char* aln, unsigned n, unsigned L, int model
  * (polytree::set_serialization_indices). Returns 0. */
 int rnj_oracle_nj(const float* matrix, unsigned n, int negative_branches, int* left, int* right,
                   double* llen, double* rlen, int* last, float* last_dist);
+/* The device's row-skipping rule audited along rapidNJ's run (rnj_oracle_nj.c); out[4] = bound
+ * violations, record violations, rows skipped, rows evaluated; flags bit 0 plants a mistake. */
+int rnj_oracle_nj_skip_audit(const float* matrix, unsigned n, int negative_branches, int flags,
+                             long long* out);
 /* polytree::serialize_tree (polytree.cpp:402-458, no bootstrap counts) of such a tree into out
  * (NUL-terminated when it fits); returns the length the text needs. */
 size_t rnj_oracle_newick(unsigned n, const int* left, const int* right, const double* llen,

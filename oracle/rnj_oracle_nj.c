@@ -23,6 +23,7 @@
  * tests/golden/nj.npz written by the compiled reference).
  */
 #include <float.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -47,6 +48,104 @@ typedef struct {
 } nj_state;
 
 void rnj_oracle_sort_row(int* id, float* dist, int len);
+
+/* ---- audit of the device's row-skipping rule (rapidnj_amd/csrc/nj.hip) ---------------------
+ * Not part of rapidNJ: with an audit attached, each search first applies the device's rule to
+ * every live row (a row's partners are the live clusters of smaller id) and checks its two
+ * claims along rapidNJ's own trajectory: (1) a row's bound (the smallest float(d - sep) of its
+ * last evaluation, less the drift of separations since, less its own separation and a 1e-5
+ * relative margin) is at most every criterion the row has now; (2) a row the rule skips (bound
+ * at least U, the criterion of the start row's pair that was best at its last evaluation, or
+ * its copy join) never lowers the running minimum in the search (is never a record). */
+typedef struct {
+  long long bound_violations, record_violations, skipped, scanned;
+  int no_drift; /* a planted mistake for the audit's own test: the drift left out */
+  double drift;
+  float* smin;
+  double* sref;
+  int *have, *rowid, *bestid, *skip, *prev_id;
+  float* prev_sep;
+} nj_audit;
+
+static nj_audit* g_audit = NULL;
+
+static float crit(const nj_state* s, int i, int j) {  /* the search's criterion of pair (i, j) */
+  const float d = s->m[(size_t)i * s->n + j];
+  float v = d - s->sep[s->rtm[j]];
+  return v - s->sep[i];
+}
+
+static void audit_before_search(nj_state* s, int start) {
+  nj_audit* A = g_audit;
+  const unsigned n = s->n;
+  /* the drift: the largest rise of a live cluster's separation since the previous search */
+  float rise = 0;
+  for (unsigned i = 0; i < n; i++) {
+    const int id = s->index_to_id[i];
+    if (id == -1) continue;
+    if (A->prev_id[i] == id) {
+      const float r = s->sep[i] - A->prev_sep[i];
+      if (r > rise) rise = r;
+    }
+  }
+  A->drift += rise;
+  for (unsigned i = 0; i < n; i++) {
+    A->prev_id[i] = s->index_to_id[i];
+    A->prev_sep[i] = s->sep[i];
+  }
+  /* U: the start row's pair that was best at its last evaluation, if still a live partner */
+  float U = FLT_MAX;
+  if (start >= 0 && s->index_to_id[start] != -1 && A->have[start] &&
+      A->rowid[start] == s->index_to_id[start]) {
+    const int b = A->bestid[start] >= 0 ? s->id_to_index[A->bestid[start]] : -1;
+    if (b >= 0 && s->index_to_id[b] != -1 && s->index_to_id[b] < s->index_to_id[start])
+      U = crit(s, start, b);
+    if (s->red_count[start] > 1 && -2 * s->sep[start] < U) U = -2 * s->sep[start];
+  }
+  for (unsigned i = 0; i < n; i++) {
+    A->skip[i] = 0;
+    const int id = s->index_to_id[i];
+    if (id == -1) continue;
+    const int known = A->have[i] && A->rowid[i] == id;
+    float lb = -FLT_MAX;
+    if (known) {
+      const float m = A->smin[i];
+      const float dr = A->no_drift ? 0.0f : (float)(A->drift - A->sref[i]);
+      lb = (m - dr) - s->sep[i];
+      lb = lb - 1e-5f * (1.0f + fabsf(m) + dr + fabsf(s->sep[i]));
+      for (unsigned j = 0; j < n; j++) {
+        const int jd = s->index_to_id[j];
+        if (j == i || jd == -1 || jd > id) continue;
+        if (crit(s, (int)i, (int)j) < lb) A->bound_violations++;
+      }
+    }
+    if (known && (int)i != start && s->red_count[i] <= 1 && U < FLT_MAX && lb >= U) {
+      A->skip[i] = 1;
+      A->skipped++;
+      continue;
+    }
+    /* evaluated: a fresh bound and best pair */
+    A->scanned++;
+    float m = INFINITY, bv = INFINITY;
+    int bid = -1;
+    for (unsigned j = 0; j < n; j++) {
+      const int jd = s->index_to_id[j];
+      if (j == i || jd == -1 || jd > id) continue;
+      const float t = s->m[(size_t)i * n + j] - s->sep[s->rtm[j]];
+      if (t < m) m = t;
+      const float v = t - s->sep[i];
+      if (v < bv) {
+        bv = v;
+        bid = jd;
+      }
+    }
+    A->smin[i] = m;
+    A->sref[i] = A->drift;
+    A->have[i] = 1;
+    A->rowid[i] = id;
+    A->bestid[i] = bid;
+  }
+}
 
 /* redundantMap[i] as a singly linked list: push_front / front / pop_front */
 static void red_push(nj_state* s, int i, int j) {
@@ -115,6 +214,7 @@ static void initialize(nj_state* s) {
 
 /* a new minimum: rapidNJ.cpp:337-345 */
 static void improve(nj_state* s, int i, int j, float value) {
+  if (g_audit && g_audit->skip[i]) g_audit->record_violations++;
   if (i != s->min1) {
     s->cache2 = s->cache1;
     s->cache1 = s->min1;
@@ -129,6 +229,7 @@ static void find_min(nj_state* s) {
   const unsigned n = s->n;
   s->global_min = FLT_MAX;
   const int start = s->index_to_id[s->cache1] != -1 ? s->cache1 : s->cache2;
+  if (g_audit) audit_before_search(s, start);
   int i = start;
   do {
     if (s->index_to_id[i] != -1) {
@@ -325,6 +426,43 @@ int rnj_oracle_nj(const float* matrix, unsigned n, int negative_branches, int* l
   free(s.list_id);
   free(s.list_d);
   free(s.list_len);
+  return rc;
+}
+
+/* rapidNJ's run with the row-skipping audit attached; out[4] = bound violations, record
+ * violations, rows skipped, rows evaluated (O(n^3): small matrices only). flags bit 0 leaves
+ * the drift out of the bound (a mistake the audit must catch). */
+int rnj_oracle_nj_skip_audit(const float* matrix, unsigned n, int negative_branches, int flags,
+                             long long* out) {
+  if (n < 3) return -1;
+  nj_audit A;
+  memset(&A, 0, sizeof A);
+  A.no_drift = flags & 1;
+  A.smin = (float*)calloc(n, sizeof(float));
+  A.sref = (double*)calloc(n, sizeof(double));
+  A.have = (int*)calloc(n, sizeof(int));
+  A.rowid = (int*)calloc(n, sizeof(int));
+  A.bestid = (int*)calloc(n, sizeof(int));
+  A.skip = (int*)calloc(n, sizeof(int));
+  A.prev_id = (int*)malloc(n * sizeof(int));
+  A.prev_sep = (float*)calloc(n, sizeof(float));
+  int *left = (int*)malloc(n * sizeof(int)), *right = (int*)malloc(n * sizeof(int)), last[2];
+  double *llen = (double*)malloc(n * sizeof(double)), *rlen = (double*)malloc(n * sizeof(double));
+  float ld;
+  int rc = -2;
+  if (A.smin && A.sref && A.have && A.rowid && A.bestid && A.skip && A.prev_id && A.prev_sep &&
+      left && right && llen && rlen) {
+    for (unsigned i = 0; i < n; i++) A.prev_id[i] = -1;
+    g_audit = &A;
+    rc = rnj_oracle_nj(matrix, n, negative_branches, left, right, llen, rlen, last, &ld);
+    g_audit = NULL;
+    out[0] = A.bound_violations;
+    out[1] = A.record_violations;
+    out[2] = A.skipped;
+    out[3] = A.scanned;
+  }
+  free(A.smin); free(A.sref); free(A.have); free(A.rowid); free(A.bestid); free(A.skip);
+  free(A.prev_id); free(A.prev_sep); free(left); free(right); free(llen); free(rlen);
   return rc;
 }
 

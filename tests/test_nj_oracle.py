@@ -78,3 +78,40 @@ def test_join_invariants():
     assert sorted(ids) == list(range(2 * n - 2))
     for k in range(n - 2):
         assert left[k] < n + k and right[k] < n + k
+
+
+# ---- the device NJ's row skipping (rapidnj_amd/csrc/nj.hip), audited along rapidNJ's runs ----
+SKIP_CASES = ["dup", "short", "sat", "sim300", "sim300jc", "eq200", "eqc120", "int30", "int300",
+              "block60", "block140"]
+
+
+@pytest.mark.parametrize("case", SKIP_CASES)
+@pytest.mark.parametrize("neg", [False, True])
+def test_row_skipping_rule_holds(nj_golden, case, neg):
+    """At every search of rapidNJ's own run, the device's bound of every row (its last
+    evaluation's smallest float(d - sep), less the separations' drift since, less its own
+    separation and the margin) is at most each of its current criteria, and no row the rule
+    skips is ever a record."""
+    r = O.nj_skip_audit(nj_golden[f"{case}_matrix"], neg)
+    assert r["bound_violations"] == 0 and r["record_violations"] == 0, r
+
+
+def test_row_skipping_skips_and_the_audit_catches_a_missing_drift(nj_golden):
+    """The rule skips most rows of a simulated matrix, and a bound without the drift term (a
+    plausible mistake, planted in the oracle's copy of the rule) breaks both claims."""
+    m = nj_golden["sim300_matrix"]
+    r = O.nj_skip_audit(m)
+    assert r["skipped"] > 3 * r["scanned"]
+    bad = O.nj_skip_audit(m, drop_drift=True)
+    assert bad["bound_violations"] > 0 and bad["record_violations"] > 0
+
+
+def test_row_skipping_rule_on_simulated_alignments():
+    """Larger and lower-divergence matrices from the simulator, K2P and JC."""
+    from rapidnj_amd.workloads import simulate_alignment
+    for n, L, seed, model in [(400, 800, 31, "k2p"), (350, 3000, 32, "jc"), (300, 120, 33, "k2p")]:
+        aln = simulate_alignment(n, L, 0.04, seed)
+        _, m, _ = O.distance_matrix(aln, model)
+        r = O.nj_skip_audit(m)
+        assert r["bound_violations"] == 0 and r["record_violations"] == 0, (n, L, r)
+        assert r["skipped"] > 0
