@@ -141,6 +141,19 @@ def test_oracle_larger(ctx, n, L, seed):
     same_joins(ctx.nj(device(m)), O.nj(m))
 
 
+@pytest.mark.parametrize("n,L,gap,seed,model,dups", [
+    (600, 80, 0.00, 41, "k2p", 0), (900, 400, 0.05, 42, "jc", 3), (1200, 2000, 0.08, 43, "k2p", 5),
+    (1000, 60, 0.02, 44, "jc", 0), (1300, 700, 0.03, 45, "k2p", 8)])
+def test_row_skipping_join_by_join(ctx, n, L, gap, seed, model, dups):
+    """Trees whose scans skip most rows (simulated alignments of several lengths, with
+    redundant copies), with many compactions: joins equal to the oracle's, one by one."""
+    aln = simulate_alignment(n, L, gap, seed)
+    for k in range(dups):
+        aln[(7 * k + 3) % n] = aln[(11 * k + 5) % n]
+    _, m, _ = O.distance_matrix(aln, model)
+    same_joins(ctx.nj(device(m)), O.nj(m))
+
+
 @pytest.mark.skipif(not O.ref_harness_available(), reason="reference harness not built")
 def test_5000_taxa_same_newick_as_reference_rapidnj(ctx, tmp_path):
     """A 5000-taxa K2P matrix (simulated 5000 x 1500 alignment, the oracle's post-processed
