@@ -99,19 +99,22 @@ __device__ __forceinline__ float protein_proportion(uint32_t mism, unsigned long
 // KimuraDistance.cpp:322-338: -1 when valid is 0, -log(1 - p - 0.2 p^2) below 0.75 (p * p in
 // float, the rest in double), the Dayhoff table up to 0.930 and 10 above. Straight-line (every
 // case computed, the result selected) so that a caller can interleave several pairs.
-template <bool kExact>
+// kTable = false leaves the Dayhoff part out: for callers that know p < 0.75.
+template <bool kExact, bool kTable = true>
 __device__ __forceinline__ float kimura_protein_of_p(float p, const LogConsts& K,
                                                     const float* day) {
   const bool formula = p < 0.75f && p > 0.0f;
   const float pp = p * p;
   const double arg = formula ? (1.0 - (double)p) - (double)pp * 0.20 : 1.0;
   const double lg = kExact ? log(arg) : log_table_one(arg, K);
+  const float low = formula ? (float)(-lg) : p;
+  if constexpr (!kTable) return low;
   int idx = (int)((double)p * 1000.0 - 750.0);
   idx = idx < 0 ? 0 : (idx > 180 ? 180 : idx);
   float tab = day[idx];
   asm volatile("" : "+v"(tab));  // read for every pair: kept out of a branch on p
   const float table = (double)p > 0.930 ? 10.0f : tab;
-  return p < 0.75f ? (formula ? (float)(-lg) : p) : table;
+  return p < 0.75f ? low : table;
 }
 
 template <bool kExact>
@@ -221,7 +224,20 @@ __device__ __forceinline__ void protein_epilogue(const JobDesc& J, const Protein
           if constexpr (decltype(model)::value == kJC)
             d[b] = jc_protein_of_p<false>(s == 0 ? 0.0f : (s < 0 ? pw : pq), K);
           else
-            d[b] = kimura_protein_of_p<false>(s == 0 ? -1.0f : (s < 0 ? pw : pq), K, T.day);
+            d[b] = s == 0 ? -1.0f : (s < 0 ? pw : pq);  // p, corrected below
+        }
+        if constexpr (decltype(model)::value != kJC) {
+          // the Dayhoff part only in waves where some pair of the row has p >= 0.75
+          bool far = false;
+#pragma unroll
+          for (int b = 0; b < MJ; ++b) far |= d[b] >= 0.75f;
+          if (__ballot(far) != 0ull) {
+#pragma unroll
+            for (int b = 0; b < MJ; ++b) d[b] = kimura_protein_of_p<false>(d[b], K, T.day);
+          } else {
+#pragma unroll
+            for (int b = 0; b < MJ; ++b) d[b] = kimura_protein_of_p<false, false>(d[b], K, T.day);
+          }
         }
         const gptr_mut<float> row = (gptr_mut<float>)J.out + tri_index(i, j0, n);
 #pragma unroll
