@@ -277,17 +277,20 @@ __attribute__((amdgpu_waves_per_eu(kTileWaves<TILE, CW, MODE, kExact>))) void ti
   const unsigned c0 = w0 / CW, c1 = (w1 + CW - 1) / CW;  // ranges start on chunk boundaries
 
   auto count_chunk = [&](const v4u* bI, const v4u* bJ, unsigned c) {
-    // words in pairs (the second word's reads take immediate offsets from the first's address);
+    // words in fours (the later words' reads take immediate offsets from the first's address);
     // `live` is uniform over the work-group, and readfirstlane lets the compiler keep the loop
     // counter in an SGPR
     const int live = __builtin_amdgcn_readfirstlane((int)min((unsigned)CW, w1 - c * CW));
     int w = 0;
 #pragma unroll 1
-    for (; w + 1 < live; w += 2) {
+    for (; w + 3 < live; w += 4) {
       count_word<MI, MJ, CW, Q, MODE>(bI, bJ, w, ty, tx, acc_a, acc_b, acc_v);
       count_word<MI, MJ, CW, Q, MODE>(bI, bJ, w + 1, ty, tx, acc_a, acc_b, acc_v);
+      count_word<MI, MJ, CW, Q, MODE>(bI, bJ, w + 2, ty, tx, acc_a, acc_b, acc_v);
+      count_word<MI, MJ, CW, Q, MODE>(bI, bJ, w + 3, ty, tx, acc_a, acc_b, acc_v);
     }
-    if (w < live) count_word<MI, MJ, CW, Q, MODE>(bI, bJ, w, ty, tx, acc_a, acc_b, acc_v);
+#pragma unroll 1
+    for (; w < live; ++w) count_word<MI, MJ, CW, Q, MODE>(bI, bJ, w, ty, tx, acc_a, acc_b, acc_v);
   };
 
   if constexpr (kDirect) {
