@@ -487,6 +487,17 @@ def side_main(args) -> int:
         A = ctx.alignment(aln)
         res = nj_side(ctx, A, not args.no_cpu_baseline, args.nj_reference_taxa)
         A.close()
+    elif args.side in ("jc", "naive"):
+        import torch
+        out = torch.empty(wl.n * (wl.n - 1) // 2, dtype=torch.float32, device="cuda:0")
+        stream = torch.cuda.current_stream()
+        if args.side == "naive":
+            res = naive_side(ctx, aln, wl, out, stream)
+        else:
+            A = ctx.alignment(aln)
+            res = model_side(ctx, A, "jc", False, wl, out, stream)
+            A.close()
+        del out
     else:
         A = ctx.alignment(aln)
         res = handoff_compare(ctx, A)
@@ -726,7 +737,7 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure HBM bytes per launch")
     ap.add_argument("--probe-traffic", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--side", choices=["ingest", "handoff", "nj", "valu"],
+    ap.add_argument("--side", choices=["ingest", "handoff", "nj", "valu", "jc", "naive"],
                     help="run only this side measurement on the workload and print its JSON")
     ap.add_argument("--nj-reference-taxa", type=int, default=5000,
                     help="with --side nj: the leading block (taxa) on which the reference's "
